@@ -1,0 +1,187 @@
+"""ctypes binding of libampsgpu.so (the C ABI declared in include/ampsgpu.h).
+
+The library is loaded from this package directory.  If it is missing, or the
+machine has no GPU, every call fails loudly: there is no CPU fallback.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libampsgpu.so")
+
+AMPS_OK = 0
+CODE_REFLECTED = -1
+CODE_LEFT_DOMAIN = 2
+CODE_FINISHED = 3
+CODE_SPHERE = 5
+MOVER_LAPENTA = 0
+MOVER_BORIS = 1
+BC_DELETE = 0
+BC_REFLECT = 1
+FP64 = 0
+FP32 = 1
+K_PUSH, K_COMPACT, K_DEPOSIT_CELL, K_DEPOSIT_NODE, K_TRACE, K_MLP = range(6)
+K_COUNT = 6
+
+
+class AmpsGpuError(RuntimeError):
+    pass
+
+
+class MeshDesc(C.Structure):
+    _fields_ = [("xmin", C.c_double * 3), ("xmax", C.c_double * 3), ("lmax", C.c_int32),
+                ("nblocks", C.c_int32)]
+
+
+class PushOpts(C.Structure):
+    _fields_ = [("mover", C.c_int32), ("bc_mode", C.c_int32), ("compact", C.c_int32),
+                ("tile", C.c_int32), ("r_sphere", C.c_double)]
+
+
+class TraceParams(C.Structure):
+    _fields_ = [
+        ("q_C", C.c_double), ("qabs_C", C.c_double), ("m0_kg", C.c_double),
+        ("dt_trace_s", C.c_double), ("max_time_s", C.c_double), ("max_steps", C.c_int64),
+        ("max_distance_m", C.c_double), ("dphi_max", C.c_double), ("adaptive", C.c_int32),
+        ("dist_limiter", C.c_int32), ("dt_floor", C.c_int32), ("outer_sphere", C.c_int32),
+        ("retry", C.c_int32), ("pad_", C.c_int32), ("box_min_Re", C.c_double * 3),
+        ("box_max_Re", C.c_double * 3), ("r_outer_Re", C.c_double), ("r_inner_Re", C.c_double),
+        ("earth_radius_m", C.c_double), ("moment_Am2", C.c_double), ("m_hat", C.c_double * 3),
+        ("B_ext_T", C.c_double * 3),
+    ]
+
+
+class TraceResult(C.Structure):
+    _fields_ = [("termination", C.c_int32), ("retry_count", C.c_int32), ("steps", C.c_int64),
+                ("time_s", C.c_double), ("distance_m", C.c_double), ("x", C.c_double * 3),
+                ("p", C.c_double * 3)]
+
+
+TRACE_RESULT_DTYPE = np.dtype([("termination", np.int32), ("retry_count", np.int32),
+                               ("steps", np.int64), ("time_s", np.float64),
+                               ("distance_m", np.float64), ("x", np.float64, 3),
+                               ("p", np.float64, 3)])
+
+_lib = None
+
+
+def _ptr(a, ctype):
+    if a is None:
+        return None
+    return a.ctypes.data_as(C.POINTER(ctype))
+
+
+def load():
+    """Load the HIP library (raises AmpsGpuError when it is not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise AmpsGpuError(f"{LIB_PATH} is not built; run __graft_entry__.build()")
+    lib = C.CDLL(LIB_PATH)
+    vp = C.c_void_p
+    i32, i64, dbl = C.c_int32, C.c_int64, C.c_double
+    P = C.POINTER
+    sig = {
+        "amps_gpu_create": (C.c_int, [C.c_int, P(vp)]),
+        "amps_gpu_destroy": (C.c_int, [vp]),
+        "amps_gpu_last_error": (C.c_char_p, []),
+        "amps_gpu_synchronize": (C.c_int, [vp]),
+        "amps_gpu_device_info": (C.c_int, [vp, C.c_char_p, P(C.c_int), P(i64)]),
+        "amps_gpu_set_param": (C.c_int, [vp, C.c_char_p, i64]),
+        "amps_gpu_set_mesh": (C.c_int, [vp, P(MeshDesc), P(i32), P(i64)]),
+        "amps_gpu_node_count": (i64, [vp]),
+        "amps_gpu_get_node_map": (C.c_int, [vp, P(i32), P(i64)]),
+        "amps_gpu_set_fields": (C.c_int, [vp, P(dbl), P(dbl)]),
+        "amps_gpu_set_species": (C.c_int, [vp, i32, P(dbl), P(dbl), P(dbl), P(dbl)]),
+        "amps_gpu_set_particles": (C.c_int, [vp, i64, P(dbl), P(dbl), P(i32), P(i32), P(i32),
+                                             P(i64), i32]),
+        "amps_gpu_particle_count": (i64, [vp]),
+        "amps_gpu_get_particles": (C.c_int, [vp, P(dbl), P(dbl), P(i32), P(i32), P(i32),
+                                             P(i64), P(i32)]),
+        "amps_gpu_move_particles": (C.c_int, [vp, P(PushOpts)]),
+        "amps_gpu_compact": (C.c_int, [vp]),
+        "amps_gpu_update_j_mass_matrix": (C.c_int, [vp, dbl, P(dbl), P(dbl)]),
+        "amps_gpu_set_timing": (C.c_int, [vp, i32]),
+        "amps_gpu_get_timing": (C.c_int, [vp, P(dbl), P(i64)]),
+        "amps_gpu_reset_timing": (C.c_int, [vp]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = lib
+    return lib
+
+
+def exported_symbols():
+    """Names every entry point of the C ABI must export."""
+    return sorted(_declared_names())
+
+
+def _declared_names():
+    import re
+    hdr = os.path.join(os.path.dirname(HERE), "include", "ampsgpu.h")
+    txt = open(hdr).read()
+    return set(re.findall(r"^\s*(?:int|int64_t|const char\*)\s+(amps_gpu_\w+)\s*\(", txt, re.M))
+
+
+def check(rc, what=""):
+    if rc != AMPS_OK:
+        msg = load().amps_gpu_last_error().decode(errors="replace")
+        raise AmpsGpuError(f"{what}: status {rc}: {msg}")
+
+
+class Context:
+    """One library context on one device (owns all device buffers)."""
+
+    def __init__(self, device=0):
+        lib = load()
+        h = C.c_void_p()
+        check(lib.amps_gpu_create(device, C.byref(h)), "amps_gpu_create")
+        self.h = h
+        self.lib = lib
+
+    def close(self):
+        if self.h:
+            self.lib.amps_gpu_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def sync(self):
+        check(self.lib.amps_gpu_synchronize(self.h), "synchronize")
+
+    def device_info(self):
+        name = C.create_string_buffer(64)
+        ncu = C.c_int()
+        hbm = C.c_int64()
+        check(self.lib.amps_gpu_device_info(self.h, name, C.byref(ncu), C.byref(hbm)), "device_info")
+        return name.value.decode(), ncu.value, hbm.value
+
+    def set_param(self, name, value):
+        check(self.lib.amps_gpu_set_param(self.h, name.encode(), int(value)), f"set_param {name}")
+
+    def set_timing(self, on):
+        check(self.lib.amps_gpu_set_timing(self.h, 1 if on else 0), "set_timing")
+
+    def reset_timing(self):
+        check(self.lib.amps_gpu_reset_timing(self.h), "reset_timing")
+
+    def timing(self):
+        ms = np.zeros(K_COUNT, np.float64)
+        n = np.zeros(K_COUNT, np.int64)
+        check(self.lib.amps_gpu_get_timing(self.h, _ptr(ms, C.c_double), _ptr(n, C.c_int64)), "get_timing")
+        return ms, n

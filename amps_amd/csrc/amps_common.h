@@ -1,0 +1,134 @@
+// amps_common.h — shared definitions of the gfx950 particle hot path library.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/ampsgpu.h"
+
+namespace amps {
+
+constexpr int NC = 8;                  // cells per block edge (_BLOCK_CELLS_X_)
+constexpr int NN = 9;                  // corner nodes per block edge
+constexpr int NCELL = NC * NC * NC;    // 512
+constexpr int NCORNER = NN * NN * NN;  // 729
+constexpr int CELL_VALS = 24 + 36 * 9; // per-cell J (8x3) + lower-triangular 8x8 pairs x 9
+constexpr int NODE_VALS = 3 + 243;     // per-node J + mass-matrix row block
+constexpr int MAX_SPEC = 8;
+
+// Device view of the block mesh (leaf blocks of the AMR octree).
+struct MeshView {
+  double gxmin[3], gxmax[3], dx_root[3], dx_maxref[3], eps;
+  int lmax, nb, lf, nf;      // finest block level present and its lattice size
+  const double* blk_xmin;    // [nb*3]
+  const double* blk_xmax;    // [nb*3]
+  const double* blk_dxc;     // FindCellIndex cell size dxRootBlock/2^L/8, [nb*3]
+  const double* blk_dxs;     // InitStencil cell size (xmax-xmin)/8, [nb*3]
+  const int* leaf_of;        // [nf^3] finest-level lattice -> block id
+};
+
+struct Species {
+  int n;
+  double charge[MAX_SPEC], mass[MAX_SPEC], dt[MAX_SPEC], weight[MAX_SPEC];
+};
+
+struct Tile {
+  int block;
+  int start;
+  int count;
+  int pad;
+};
+
+// Boundary face of the root box: ExternalBoundaryFaceTable (pic_mover.cpp:24-28).
+struct Face {
+  double norm[3], e0[3], e1[3], x0[3], lE0, lE1;
+};
+struct Faces {
+  Face f[6];
+};
+
+}  // namespace amps
+
+struct amps_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  // mesh
+  bool has_mesh = false;
+  amps::MeshView mv{};
+  std::vector<int> h_level;
+  std::vector<int64_t> h_ijk;
+  std::vector<double> h_xmin, h_xmax;
+  double *d_xmin = nullptr, *d_xmax = nullptr, *d_dxc = nullptr, *d_dxs = nullptr;
+  int* d_leaf = nullptr;
+  amps::Faces faces{};
+  int64_t nnodes = 0;
+  std::vector<int32_t> h_node_of_corner;
+  std::vector<int64_t> h_node_keys;
+  int* d_inc_start = nullptr;  // [nnodes+1]
+  int* d_inc = nullptr;        // cell_gid*8 + corner, ascending cell order per node
+  // fields and species
+  bool has_fields = false;
+  double *d_E = nullptr, *d_B = nullptr;
+  amps::Species sp{};
+  bool has_species = false;
+  // particles: ping-pong SoA buffers
+  int64_t n = 0, cap = 0;
+  int precision = AMPS_FP64;
+  int cur = 0;
+  double* d_x[2] = {nullptr, nullptr};
+  double* d_v[2] = {nullptr, nullptr};
+  uint8_t* d_spec[2] = {nullptr, nullptr};
+  int* d_key[2] = {nullptr, nullptr};
+  int64_t* d_id[2] = {nullptr, nullptr};
+  int8_t* d_code = nullptr;    // per-particle mover code of the current order
+  int8_t* d_code2 = nullptr;   // spare code buffer (swapped at compaction)
+  int* d_n = nullptr;          // live particle count on the device
+  // migration and tiling
+  int* d_count = nullptr;      // [nkeys+1]
+  int* d_offset = nullptr;     // [nkeys+1]
+  int* d_cursor = nullptr;     // [nkeys]
+  int* d_scan_tmp = nullptr;
+  int64_t scan_tmp_len = 0;
+  amps::Tile* d_tiles = nullptr;
+  int* d_ntiles = nullptr;
+  int* d_tcount = nullptr;     // per-block tile count / offsets [nb+1]
+  int64_t max_tiles = 0;
+  int tile = 2048;
+  bool offsets_valid = false;  // d_offset describes the current order
+  bool tiles_valid = false;    // d_tiles describes the current order
+  int dep_nch = 4;             // deposit output chunks per cell (waves per workgroup)
+  int dep_exact = 0;           // 1: unfused mul+add in the deposit (bit-exact order)
+  // deposit
+  double* d_cellpart = nullptr;
+  double* d_nodeout = nullptr;
+  // timing
+  bool timing = false;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  double ms[AMPS_K_COUNT] = {0};
+  int64_t launches[AMPS_K_COUNT] = {0};
+  // tracer / surrogate state lives in their own files
+  void* trace_state = nullptr;
+  void* mlp_state = nullptr;
+};
+
+namespace amps {
+void set_error(const std::string& msg);
+int fail(int code, const std::string& msg);
+int hip_fail(hipError_t e, const char* what);
+// timing helpers (no-ops when timing is off)
+void tbegin(amps_ctx* c);
+void tend(amps_ctx* c, int k);
+// per-feature state release (trace.hip, mlp.hip)
+void trace_free(amps_ctx* c);
+void mlp_free(amps_ctx* c);
+// device-wide exclusive scan of n ints in place; total written to total_out
+int scan_exclusive(amps_ctx* c, int* data, int64_t n, int* total_out);
+}  // namespace amps
+
+#define HIP_TRY(expr)                                    \
+  do {                                                   \
+    hipError_t e_ = (expr);                              \
+    if (e_ != hipSuccess) return amps::hip_fail(e_, #expr); \
+  } while (0)

@@ -1,0 +1,686 @@
+// capi.hip — C ABI of the particle hot path: context, mesh, fields,
+// particles, mover, migration and ECSIM deposit orchestration.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "amps_common.h"
+
+using namespace amps;
+
+// kernels (pic_kernels.hip)
+template <int MOVER>
+__global__ void k_push_f64(MeshView, Faces, Species, const double*, const double*, const Tile*,
+                           const int*, int, double, int64_t, double*, double*, const uint8_t*,
+                           int*, int8_t*);
+__global__ void k_fill_int(int*, int64_t, int);
+__global__ void k_count_keys(const int*, const int*, const int8_t*, int*);
+__global__ void k_scatter(const int*, const int*, const int8_t*, const int*, int*, int64_t,
+                          const double*, const double*, const uint8_t*, const int64_t*, double*,
+                          double*, uint8_t*, int*, int64_t*);
+__global__ void k_copy_total(const int*, int*);
+__global__ void k_tile_count(int, int, const int*, int*);
+__global__ void k_tile_emit(int, int, const int*, const int*, Tile*, int*);
+template <int NCH, bool EXACT>
+__global__ void k_deposit_cell(MeshView, Species, double, const double*, const int*, int64_t,
+                               const double*, const double*, const uint8_t*, double*);
+__global__ void k_deposit_node(int64_t, const int*, const int*, const double*, double*);
+namespace amps {
+int upload_inv_index_matrix(hipStream_t s);
+}
+
+static thread_local std::string g_err;
+
+namespace amps {
+void set_error(const std::string& msg) { g_err = msg; }
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+int hip_fail(hipError_t e, const char* what) {
+  g_err = std::string("HIP error ") + hipGetErrorString(e) + " in " + what;
+  return AMPS_E_HIP;
+}
+void tbegin(amps_ctx* c) {
+  if (c->timing) (void)hipEventRecord(c->ev0, c->stream);
+}
+void tend(amps_ctx* c, int k) {
+  if (!c->timing) return;
+  (void)hipEventRecord(c->ev1, c->stream);
+  (void)hipEventSynchronize(c->ev1);
+  float ms = 0.f;
+  (void)hipEventElapsedTime(&ms, c->ev0, c->ev1);
+  c->ms[k] += ms;
+  c->launches[k] += 1;
+}
+}  // namespace amps
+
+extern "C" const char* amps_gpu_last_error(void) { return g_err.c_str(); }
+
+static void free_particles(amps_ctx* c) {
+  for (int i = 0; i < 2; i++) {
+    (void)hipFree(c->d_x[i]);
+    (void)hipFree(c->d_v[i]);
+    (void)hipFree(c->d_spec[i]);
+    (void)hipFree(c->d_key[i]);
+    (void)hipFree(c->d_id[i]);
+    c->d_x[i] = c->d_v[i] = nullptr;
+    c->d_spec[i] = nullptr;
+    c->d_key[i] = nullptr;
+    c->d_id[i] = nullptr;
+  }
+  (void)hipFree(c->d_code);
+  c->d_code = nullptr;
+  (void)hipFree(c->d_code2);
+  c->d_code2 = nullptr;
+  c->cap = 0;
+  c->n = 0;
+}
+
+static void free_mesh(amps_ctx* c) {
+  (void)hipFree(c->d_xmin);
+  (void)hipFree(c->d_xmax);
+  (void)hipFree(c->d_dxc);
+  (void)hipFree(c->d_dxs);
+  (void)hipFree(c->d_leaf);
+  (void)hipFree(c->d_inc_start);
+  (void)hipFree(c->d_inc);
+  (void)hipFree(c->d_offset);
+  (void)hipFree(c->d_cursor);
+  (void)hipFree(c->d_tcount);
+  (void)hipFree(c->d_cellpart);
+  (void)hipFree(c->d_nodeout);
+  (void)hipFree(c->d_E);
+  (void)hipFree(c->d_B);
+  c->d_xmin = c->d_xmax = c->d_dxc = c->d_dxs = nullptr;
+  c->d_leaf = c->d_inc_start = c->d_inc = nullptr;
+  c->d_offset = c->d_cursor = c->d_tcount = nullptr;
+  c->d_cellpart = c->d_nodeout = nullptr;
+  c->d_E = c->d_B = nullptr;
+  c->has_mesh = c->has_fields = false;
+}
+
+extern "C" int amps_gpu_create(int device, amps_ctx** out) {
+  if (!out) return fail(AMPS_E_ARG, "amps_gpu_create: out is NULL");
+  int ndev = 0;
+  HIP_TRY(hipGetDeviceCount(&ndev));
+  if (device < 0 || device >= ndev) return fail(AMPS_E_ARG, "amps_gpu_create: bad device index");
+  HIP_TRY(hipSetDevice(device));
+  amps_ctx* c = new amps_ctx();
+  c->device = device;
+  HIP_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  HIP_TRY(hipEventCreate(&c->ev0));
+  HIP_TRY(hipEventCreate(&c->ev1));
+  HIP_TRY(hipMalloc(&c->d_n, sizeof(int) * 4));
+  HIP_TRY(hipMalloc(&c->d_ntiles, sizeof(int) * 4));
+  if (int r = upload_inv_index_matrix(c->stream)) return r;
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  *out = c;
+  return AMPS_OK;
+}
+
+extern "C" int amps_gpu_destroy(amps_ctx* c) {
+  if (!c) return AMPS_OK;
+  (void)hipSetDevice(c->device);
+  (void)hipStreamSynchronize(c->stream);
+  free_particles(c);
+  free_mesh(c);
+  trace_free(c);
+  mlp_free(c);
+  (void)hipFree(c->d_n);
+  (void)hipFree(c->d_ntiles);
+  (void)hipFree(c->d_tiles);
+  (void)hipFree(c->d_scan_tmp);
+  (void)hipEventDestroy(c->ev0);
+  (void)hipEventDestroy(c->ev1);
+  (void)hipStreamDestroy(c->stream);
+  delete c;
+  return AMPS_OK;
+}
+
+extern "C" int amps_gpu_synchronize(amps_ctx* c) {
+  if (!c) return fail(AMPS_E_ARG, "null context");
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return AMPS_OK;
+}
+
+extern "C" int amps_gpu_device_info(amps_ctx* c, char* name, int* n_cu, int64_t* hbm) {
+  if (!c) return fail(AMPS_E_ARG, "null context");
+  hipDeviceProp_t p;
+  HIP_TRY(hipGetDeviceProperties(&p, c->device));
+  if (name) {
+    std::snprintf(name, 64, "%s", p.gcnArchName);
+  }
+  if (n_cu) *n_cu = p.multiProcessorCount;
+  if (hbm) *hbm = (int64_t)p.totalGlobalMem;
+  return AMPS_OK;
+}
+
+extern "C" int amps_gpu_set_param(amps_ctx* c, const char* name, int64_t value) {
+  if (!c || !name) return fail(AMPS_E_ARG, "amps_gpu_set_param: null argument");
+  std::string n(name);
+  if (n == "tile") {
+    if (value < 64 || value > (1 << 20)) return fail(AMPS_E_ARG, "tile out of range [64, 2^20]");
+    c->tile = (int)value;
+    c->tiles_valid = false;
+  } else if (n == "deposit_chunks") {
+    if (value != 2 && value != 3 && value != 4 && value != 6)
+      return fail(AMPS_E_ARG, "deposit_chunks must be 2, 3, 4 or 6");
+    c->dep_nch = (int)value;
+  } else if (n == "deposit_exact") {
+    c->dep_exact = value ? 1 : 0;
+  } else {
+    return fail(AMPS_E_ARG, "unknown parameter " + n);
+  }
+  return AMPS_OK;
+}
+
+// ---------------------------------------------------------------------
+// mesh
+// ---------------------------------------------------------------------
+static void build_faces(const double gxmin[3], const double gxmax[3], Faces& F) {
+  // ExternalBoundaryFaceTable and its Init, pic_mover.cpp:24-28, 60-74
+  static const double nrm[6][3] = {{-1, 0, 0}, {1, 0, 0}, {0, -1, 0}, {0, 1, 0}, {0, 0, -1}, {0, 0, 1}};
+  static const int nX0[6][3] = {{0, 0, 0}, {1, 0, 0}, {0, 0, 0}, {0, 1, 0}, {0, 0, 0}, {0, 0, 1}};
+  static const double e0[6][3] = {{0, 1, 0}, {0, 1, 0}, {1, 0, 0}, {1, 0, 0}, {1, 0, 0}, {1, 0, 0}};
+  static const double e1[6][3] = {{0, 0, 1}, {0, 0, 1}, {0, 0, 1}, {0, 0, 1}, {0, 1, 0}, {0, 1, 0}};
+  for (int nf = 0; nf < 6; nf++) {
+    double cE0 = 0.0, cE1 = 0.0;
+    for (int d = 0; d < 3; d++) {
+      F.f[nf].norm[d] = nrm[nf][d];
+      F.f[nf].e0[d] = e0[nf][d];
+      F.f[nf].e1[d] = e1[nf][d];
+      F.f[nf].x0[d] = (nX0[nf][d] == 0) ? gxmin[d] : gxmax[d];
+      double a = ((e0[nf][d] + nX0[nf][d] < 0.5) ? gxmin[d] : gxmax[d]) - F.f[nf].x0[d];
+      double b = ((e1[nf][d] + nX0[nf][d] < 0.5) ? gxmin[d] : gxmax[d]) - F.f[nf].x0[d];
+      cE0 += std::pow(a, 2);
+      cE1 += std::pow(b, 2);
+    }
+    F.f[nf].lE0 = std::sqrt(cE0);
+    F.f[nf].lE1 = std::sqrt(cE1);
+  }
+}
+
+extern "C" int amps_gpu_set_mesh(amps_ctx* c, const amps_mesh_desc* desc, const int32_t* level,
+                                 const int64_t* ijk) {
+  if (!c || !desc || !level || !ijk) return fail(AMPS_E_ARG, "amps_gpu_set_mesh: null argument");
+  const int nb = desc->nblocks, lmax = desc->lmax;
+  if (nb <= 0) return fail(AMPS_E_ARG, "amps_gpu_set_mesh: nblocks must be > 0");
+  if (lmax < 3 || lmax > 20) return fail(AMPS_E_ARG, "amps_gpu_set_mesh: lmax out of [3,20]");
+  if ((int64_t)nb * NCELL >= ((int64_t)1 << 30))
+    return fail(AMPS_E_RANGE, "amps_gpu_set_mesh: too many blocks for 32-bit cell keys");
+  for (int d = 0; d < 3; d++)
+    if (!(desc->xmax[d] > desc->xmin[d])) return fail(AMPS_E_ARG, "amps_gpu_set_mesh: empty domain");
+  (void)hipSetDevice(c->device);
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  free_mesh(c);
+  free_particles(c);
+  int lf = 0;
+  for (int b = 0; b < nb; b++) {
+    if (level[b] < 0 || level[b] > lmax - 3)
+      return fail(AMPS_E_ARG, "amps_gpu_set_mesh: block level out of [0, lmax-3]");
+    lf = std::max(lf, (int)level[b]);
+  }
+  if (lf > 9) return fail(AMPS_E_RANGE, "amps_gpu_set_mesh: finest block level above 9");
+  const int nf = 1 << lf;
+  MeshView& m = c->mv;
+  m.lmax = lmax;
+  m.nb = nb;
+  m.lf = lf;
+  m.nf = nf;
+  for (int d = 0; d < 3; d++) {
+    m.gxmin[d] = desc->xmin[d];
+    m.gxmax[d] = desc->xmax[d];
+    m.dx_root[d] = desc->xmax[d] - desc->xmin[d];
+    m.dx_maxref[d] = (desc->xmax[d] - desc->xmin[d]) / (1 << lmax);
+  }
+  // EPS, meshAMRgeneric.h:2340-2352
+  m.eps = 0.0001 * m.dx_root[0] / (double)NC / (1 << lmax);
+  for (int d = 1; d < 3; d++) {
+    double e = 0.0001 * m.dx_root[d] / (double)NC / (1 << lmax);
+    if (m.eps > e) m.eps = e;
+  }
+  build_faces(m.gxmin, m.gxmax, c->faces);
+  // leaves must tile the root exactly
+  std::vector<int> leaf((size_t)nf * nf * nf, -1);
+  std::vector<double> xmin(3 * (size_t)nb), xmax(3 * (size_t)nb), dxc(3 * (size_t)nb),
+      dxs(3 * (size_t)nb);
+  for (int b = 0; b < nb; b++) {
+    const int L = level[b];
+    const int64_t span = (int64_t)1 << (lf - L);
+    int64_t i0[3];
+    for (int d = 0; d < 3; d++) {
+      const int64_t q = ijk[3 * (size_t)b + d];
+      if (q < 0 || q >= ((int64_t)1 << L)) return fail(AMPS_E_ARG, "amps_gpu_set_mesh: block index out of range");
+      i0[d] = q * span;
+      const int64_t xg = q << (lmax - L), size = (int64_t)1 << (lmax - L);
+      xmin[3 * b + d] = m.gxmin[d] + (double)xg * m.dx_maxref[d];
+      xmax[3 * b + d] = m.gxmin[d] + (double)(xg + size) * m.dx_maxref[d];
+      dxs[3 * b + d] = (xmax[3 * b + d] - xmin[3 * b + d]) / NC;
+      dxc[3 * b + d] = m.dx_root[d] / (1 << L) / (double)NC;
+    }
+    for (int64_t k = 0; k < span; k++)
+      for (int64_t j = 0; j < span; j++)
+        for (int64_t i = 0; i < span; i++) {
+          int& slot = leaf[((size_t)(i0[2] + k) * nf + (i0[1] + j)) * nf + (i0[0] + i)];
+          if (slot >= 0) return fail(AMPS_E_ARG, "amps_gpu_set_mesh: overlapping blocks");
+          slot = b;
+        }
+  }
+  for (size_t i = 0; i < leaf.size(); i++)
+    if (leaf[i] < 0) return fail(AMPS_E_ARG, "amps_gpu_set_mesh: blocks do not cover the domain");
+  // unique corner nodes by lattice key
+  const int64_t S = ((int64_t)1 << lmax) + 1;
+  const int64_t total = (int64_t)nb * NCORNER;
+  std::vector<int64_t> keys((size_t)total);
+  for (int b = 0; b < nb; b++) {
+    const int L = level[b];
+    const int64_t cs = ((int64_t)1 << (lmax - L)) / NC;
+    const int64_t g0 = ijk[3 * (size_t)b] << (lmax - L), g1 = ijk[3 * (size_t)b + 1] << (lmax - L),
+                  g2 = ijk[3 * (size_t)b + 2] << (lmax - L);
+    for (int k = 0; k < NN; k++)
+      for (int j = 0; j < NN; j++)
+        for (int i = 0; i < NN; i++)
+          keys[(size_t)b * NCORNER + i + NN * (j + NN * k)] =
+              ((g2 + k * cs) * S + (g1 + j * cs)) * S + (g0 + i * cs);
+  }
+  std::vector<int64_t> uk(keys);
+  std::sort(uk.begin(), uk.end());
+  uk.erase(std::unique(uk.begin(), uk.end()), uk.end());
+  c->nnodes = (int64_t)uk.size();
+  if (c->nnodes >= ((int64_t)1 << 31)) return fail(AMPS_E_RANGE, "too many nodes");
+  c->h_node_of_corner.resize((size_t)total);
+  for (int64_t i = 0; i < total; i++)
+    c->h_node_of_corner[(size_t)i] =
+        (int32_t)(std::lower_bound(uk.begin(), uk.end(), keys[(size_t)i]) - uk.begin());
+  c->h_node_keys = uk;
+  // node -> (cell, corner) incidences in ascending global cell order
+  static const int CornerOff[8][3] = {{0, 0, 0}, {1, 0, 0}, {1, 1, 0}, {0, 1, 0},
+                                      {0, 0, 1}, {1, 0, 1}, {1, 1, 1}, {0, 1, 1}};
+  std::vector<int> inc_start((size_t)c->nnodes + 1, 0);
+  for (int b = 0; b < nb; b++)
+    for (int lc = 0; lc < NCELL; lc++) {
+      const int ci = lc % NC, cj = (lc / NC) % NC, ck = lc / (NC * NC);
+      for (int q = 0; q < 8; q++) {
+        const int corner = (ci + CornerOff[q][0]) + NN * ((cj + CornerOff[q][1]) + NN * (ck + CornerOff[q][2]));
+        inc_start[(size_t)c->h_node_of_corner[(size_t)b * NCORNER + corner] + 1]++;
+      }
+    }
+  for (int64_t i = 0; i < c->nnodes; i++) inc_start[(size_t)i + 1] += inc_start[(size_t)i];
+  std::vector<int> inc((size_t)inc_start.back());
+  std::vector<int> fillp(inc_start.begin(), inc_start.end() - 1);
+  for (int b = 0; b < nb; b++)
+    for (int lc = 0; lc < NCELL; lc++) {
+      const int ci = lc % NC, cj = (lc / NC) % NC, ck = lc / (NC * NC);
+      const int gid = b * NCELL + lc;
+      for (int q = 0; q < 8; q++) {
+        const int corner = (ci + CornerOff[q][0]) + NN * ((cj + CornerOff[q][1]) + NN * (ck + CornerOff[q][2]));
+        const int node = c->h_node_of_corner[(size_t)b * NCORNER + corner];
+        inc[(size_t)fillp[(size_t)node]++] = gid * 8 + q;
+      }
+    }
+  // device upload
+  HIP_TRY(hipMalloc(&c->d_xmin, sizeof(double) * 3 * nb));
+  HIP_TRY(hipMalloc(&c->d_xmax, sizeof(double) * 3 * nb));
+  HIP_TRY(hipMalloc(&c->d_dxc, sizeof(double) * 3 * nb));
+  HIP_TRY(hipMalloc(&c->d_dxs, sizeof(double) * 3 * nb));
+  HIP_TRY(hipMalloc(&c->d_leaf, sizeof(int) * leaf.size()));
+  HIP_TRY(hipMalloc(&c->d_inc_start, sizeof(int) * inc_start.size()));
+  HIP_TRY(hipMalloc(&c->d_inc, sizeof(int) * std::max<size_t>(1, inc.size())));
+  HIP_TRY(hipMemcpy(c->d_xmin, xmin.data(), sizeof(double) * 3 * nb, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(c->d_xmax, xmax.data(), sizeof(double) * 3 * nb, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(c->d_dxc, dxc.data(), sizeof(double) * 3 * nb, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(c->d_dxs, dxs.data(), sizeof(double) * 3 * nb, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(c->d_leaf, leaf.data(), sizeof(int) * leaf.size(), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(c->d_inc_start, inc_start.data(), sizeof(int) * inc_start.size(), hipMemcpyHostToDevice));
+  if (!inc.empty()) HIP_TRY(hipMemcpy(c->d_inc, inc.data(), sizeof(int) * inc.size(), hipMemcpyHostToDevice));
+  const int64_t nkeys = (int64_t)nb * NCELL;
+  HIP_TRY(hipMalloc(&c->d_offset, sizeof(int) * (nkeys + 1)));
+  HIP_TRY(hipMalloc(&c->d_cursor, sizeof(int) * nkeys));
+  HIP_TRY(hipMalloc(&c->d_tcount, sizeof(int) * (nb + 1)));
+  m.blk_xmin = c->d_xmin;
+  m.blk_xmax = c->d_xmax;
+  m.blk_dxc = c->d_dxc;
+  m.blk_dxs = c->d_dxs;
+  m.leaf_of = c->d_leaf;
+  c->h_level.assign(level, level + nb);
+  c->h_ijk.assign(ijk, ijk + 3 * (size_t)nb);
+  c->h_xmin = xmin;
+  c->h_xmax = xmax;
+  c->has_mesh = true;
+  c->offsets_valid = false;
+  c->tiles_valid = false;
+  return AMPS_OK;
+}
+
+extern "C" int64_t amps_gpu_node_count(amps_ctx* c) { return (c && c->has_mesh) ? c->nnodes : -1; }
+
+extern "C" int amps_gpu_get_node_map(amps_ctx* c, int32_t* node_of_corner, int64_t* keys) {
+  if (!c || !c->has_mesh) return fail(AMPS_E_STATE, "amps_gpu_get_node_map: no mesh");
+  if (node_of_corner)
+    std::memcpy(node_of_corner, c->h_node_of_corner.data(), sizeof(int32_t) * c->h_node_of_corner.size());
+  if (keys) std::memcpy(keys, c->h_node_keys.data(), sizeof(int64_t) * c->h_node_keys.size());
+  return AMPS_OK;
+}
+
+extern "C" int amps_gpu_set_fields(amps_ctx* c, const double* E, const double* B) {
+  if (!c || !E || !B) return fail(AMPS_E_ARG, "amps_gpu_set_fields: null argument");
+  if (!c->has_mesh) return fail(AMPS_E_STATE, "amps_gpu_set_fields: set the mesh first");
+  const size_t nbytes = sizeof(double) * (size_t)c->mv.nb * NCORNER * 3;
+  if (!c->d_E) HIP_TRY(hipMalloc(&c->d_E, nbytes));
+  if (!c->d_B) HIP_TRY(hipMalloc(&c->d_B, nbytes));
+  HIP_TRY(hipMemcpyAsync(c->d_E, E, nbytes, hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(hipMemcpyAsync(c->d_B, B, nbytes, hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  c->has_fields = true;
+  return AMPS_OK;
+}
+
+extern "C" int amps_gpu_set_species(amps_ctx* c, int32_t nspec, const double* charge,
+                                    const double* mass, const double* dt, const double* weight) {
+  if (!c || !charge || !mass || !dt || !weight) return fail(AMPS_E_ARG, "amps_gpu_set_species: null argument");
+  if (nspec <= 0 || nspec > MAX_SPEC) return fail(AMPS_E_ARG, "amps_gpu_set_species: nspec out of [1,8]");
+  c->sp.n = nspec;
+  for (int s = 0; s < nspec; s++) {
+    if (!(mass[s] > 0.0)) return fail(AMPS_E_ARG, "amps_gpu_set_species: mass must be > 0");
+    c->sp.charge[s] = charge[s];
+    c->sp.mass[s] = mass[s];
+    c->sp.dt[s] = dt[s];
+    c->sp.weight[s] = weight[s];
+  }
+  c->has_species = true;
+  return AMPS_OK;
+}
+
+// ---------------------------------------------------------------------
+// particles
+// ---------------------------------------------------------------------
+static int build_tiles(amps_ctx* c) {
+  const int nb = c->mv.nb;
+  const int64_t need = nb + c->cap / c->tile + 2;
+  if (need > c->max_tiles) {
+    (void)hipFree(c->d_tiles);
+    c->d_tiles = nullptr;
+    HIP_TRY(hipMalloc(&c->d_tiles, sizeof(Tile) * need));
+    c->max_tiles = need;
+  }
+  hipLaunchKernelGGL(k_tile_count, dim3((nb + 256) / 256), dim3(256), 0, c->stream, nb, c->tile,
+                     c->d_offset, c->d_tcount);
+  if (int r = scan_exclusive(c, c->d_tcount, nb + 1, nullptr)) return r;
+  hipLaunchKernelGGL(k_tile_emit, dim3((nb + 256) / 256), dim3(256), 0, c->stream, nb, c->tile,
+                     c->d_offset, c->d_tcount, c->d_tiles, c->d_ntiles);
+  c->tiles_valid = true;
+  return AMPS_OK;
+}
+
+extern "C" int amps_gpu_set_particles(amps_ctx* c, int64_t n, const double* x, const double* v,
+                                      const int32_t* spec, const int32_t* blk,
+                                      const int32_t* cell, const int64_t* id, int32_t precision) {
+  if (!c) return fail(AMPS_E_ARG, "null context");
+  if (!c->has_mesh) return fail(AMPS_E_STATE, "amps_gpu_set_particles: set the mesh first");
+  if (n < 0 || n >= ((int64_t)1 << 31) - 1) return fail(AMPS_E_RANGE, "particle count out of [0, 2^31-1)");
+  if (n > 0 && (!x || !v || !spec || !blk || !cell)) return fail(AMPS_E_ARG, "amps_gpu_set_particles: null array");
+  if (precision != AMPS_FP64) return fail(AMPS_E_ARG, "amps_gpu_set_particles: precision must be AMPS_FP64 here");
+  (void)hipSetDevice(c->device);
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  const int nb = c->mv.nb;
+  const int64_t nkeys = (int64_t)nb * NCELL;
+  std::vector<int> cnt((size_t)nkeys + 1, 0);
+  for (int64_t p = 0; p < n; p++) {
+    if (blk[p] < 0 || blk[p] >= nb || cell[p] < 0 || cell[p] >= NCELL)
+      return fail(AMPS_E_ARG, "amps_gpu_set_particles: block or cell id out of range");
+    if (spec[p] < 0 || spec[p] >= std::max(1, c->sp.n) || spec[p] > 255)
+      return fail(AMPS_E_ARG, "amps_gpu_set_particles: species id out of range");
+    cnt[(size_t)blk[p] * NCELL + cell[p] + 1]++;
+  }
+  for (int64_t k = 0; k < nkeys; k++) cnt[(size_t)k + 1] += cnt[(size_t)k];
+  std::vector<int> fill(cnt.begin(), cnt.end() - 1);
+  std::vector<int64_t> perm((size_t)n);
+  for (int64_t p = 0; p < n; p++) perm[(size_t)fill[(size_t)blk[p] * NCELL + cell[p]]++] = p;
+  if (n > c->cap) {
+    free_particles(c);
+    const int64_t cap = std::max<int64_t>(n, 64);
+    for (int i = 0; i < 2; i++) {
+      HIP_TRY(hipMalloc(&c->d_x[i], sizeof(double) * 3 * cap));
+      HIP_TRY(hipMalloc(&c->d_v[i], sizeof(double) * 3 * cap));
+      HIP_TRY(hipMalloc(&c->d_spec[i], cap));
+      HIP_TRY(hipMalloc(&c->d_key[i], sizeof(int) * cap));
+      HIP_TRY(hipMalloc(&c->d_id[i], sizeof(int64_t) * cap));
+    }
+    HIP_TRY(hipMalloc(&c->d_code, (cap + 4) & ~(int64_t)3));
+    HIP_TRY(hipMalloc(&c->d_code2, (cap + 4) & ~(int64_t)3));
+    c->cap = cap;
+  }
+  const int64_t cap = c->cap;
+  std::vector<double> hx(3 * (size_t)cap, 0.0), hv(3 * (size_t)cap, 0.0);
+  std::vector<uint8_t> hs((size_t)cap, 0);
+  std::vector<int> hk((size_t)cap, 0);
+  std::vector<int64_t> hid((size_t)cap, 0);
+  for (int64_t q = 0; q < n; q++) {
+    const int64_t p = perm[(size_t)q];
+    for (int d = 0; d < 3; d++) {
+      hx[(size_t)(d * cap + q)] = x[d * n + p];
+      hv[(size_t)(d * cap + q)] = v[d * n + p];
+    }
+    hs[(size_t)q] = (uint8_t)spec[p];
+    hk[(size_t)q] = blk[p] * NCELL + cell[p];
+    hid[(size_t)q] = id ? id[p] : p;
+  }
+  c->cur = 0;
+  HIP_TRY(hipMemcpy(c->d_x[0], hx.data(), sizeof(double) * 3 * cap, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(c->d_v[0], hv.data(), sizeof(double) * 3 * cap, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(c->d_spec[0], hs.data(), cap, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(c->d_key[0], hk.data(), sizeof(int) * cap, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(c->d_id[0], hid.data(), sizeof(int64_t) * cap, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemset(c->d_code, AMPS_CODE_FINISHED, cap));
+  HIP_TRY(hipMemcpy(c->d_offset, cnt.data(), sizeof(int) * (nkeys + 1), hipMemcpyHostToDevice));
+  const int nn = (int)n;
+  HIP_TRY(hipMemcpy(c->d_n, &nn, sizeof(int), hipMemcpyHostToDevice));
+  c->n = n;
+  c->precision = precision;
+  c->offsets_valid = true;
+  if (int r = build_tiles(c)) return r;
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return AMPS_OK;
+}
+
+extern "C" int64_t amps_gpu_particle_count(amps_ctx* c) {
+  if (!c) return -1;
+  int nn = 0;
+  if (hipStreamSynchronize(c->stream) != hipSuccess) return -1;
+  if (hipMemcpy(&nn, c->d_n, sizeof(int), hipMemcpyDeviceToHost) != hipSuccess) return -1;
+  c->n = nn;
+  return nn;
+}
+
+extern "C" int amps_gpu_get_particles(amps_ctx* c, double* x, double* v, int32_t* spec,
+                                      int32_t* blk, int32_t* cell, int64_t* id, int32_t* code) {
+  if (!c) return fail(AMPS_E_ARG, "null context");
+  const int64_t n = amps_gpu_particle_count(c);
+  if (n < 0) return fail(AMPS_E_HIP, "amps_gpu_get_particles: cannot read the particle count");
+  if (n == 0) return AMPS_OK;
+  const int64_t cap = c->cap;
+  const int k = c->cur;
+  if (x)
+    for (int d = 0; d < 3; d++)
+      HIP_TRY(hipMemcpy(x + d * n, c->d_x[k] + d * cap, sizeof(double) * n, hipMemcpyDeviceToHost));
+  if (v)
+    for (int d = 0; d < 3; d++)
+      HIP_TRY(hipMemcpy(v + d * n, c->d_v[k] + d * cap, sizeof(double) * n, hipMemcpyDeviceToHost));
+  if (spec) {
+    std::vector<uint8_t> s((size_t)n);
+    HIP_TRY(hipMemcpy(s.data(), c->d_spec[k], n, hipMemcpyDeviceToHost));
+    for (int64_t i = 0; i < n; i++) spec[i] = s[(size_t)i];
+  }
+  if (blk || cell) {
+    std::vector<int> kk((size_t)n);
+    HIP_TRY(hipMemcpy(kk.data(), c->d_key[k], sizeof(int) * n, hipMemcpyDeviceToHost));
+    for (int64_t i = 0; i < n; i++) {
+      if (blk) blk[i] = kk[(size_t)i] / NCELL;
+      if (cell) cell[i] = kk[(size_t)i] % NCELL;
+    }
+  }
+  if (id) HIP_TRY(hipMemcpy(id, c->d_id[k], sizeof(int64_t) * n, hipMemcpyDeviceToHost));
+  if (code) {
+    std::vector<int8_t> s((size_t)n);
+    HIP_TRY(hipMemcpy(s.data(), c->d_code, n, hipMemcpyDeviceToHost));
+    for (int64_t i = 0; i < n; i++) code[i] = s[(size_t)i];
+  }
+  return AMPS_OK;
+}
+
+// ---------------------------------------------------------------------
+// migration
+// ---------------------------------------------------------------------
+static int compact_impl(amps_ctx* c) {
+  const int64_t nkeys = (int64_t)c->mv.nb * NCELL;
+  const int64_t cap = c->cap;
+  const unsigned gridp = (unsigned)((cap + 255) / 256);
+  const int a = c->cur, b = 1 - c->cur;
+  hipLaunchKernelGGL(k_fill_int, dim3(1024), dim3(256), 0, c->stream, c->d_offset, nkeys + 1, 0);
+  hipLaunchKernelGGL(k_fill_int, dim3(1024), dim3(256), 0, c->stream, c->d_cursor, nkeys, 0);
+  hipLaunchKernelGGL(k_count_keys, dim3(gridp), dim3(256), 0, c->stream, c->d_n, c->d_key[a],
+                     c->d_code, c->d_offset);
+  if (int r = scan_exclusive(c, c->d_offset, nkeys + 1, nullptr)) return r;
+  hipLaunchKernelGGL(k_scatter, dim3(gridp), dim3(256), 0, c->stream, c->d_n, c->d_key[a],
+                     c->d_code, c->d_offset, c->d_cursor, cap, c->d_x[a], c->d_v[a],
+                     c->d_spec[a], c->d_id[a], c->d_x[b], c->d_v[b], c->d_spec[b], c->d_key[b],
+                     c->d_id[b]);
+  hipLaunchKernelGGL(k_copy_total, dim3(1), dim3(1), 0, c->stream, c->d_offset + nkeys, c->d_n);
+  c->cur = b;
+  c->offsets_valid = true;
+  // all survivors carry a committed code
+  std::swap(c->d_code, c->d_code2);
+  hipLaunchKernelGGL(k_fill_int, dim3(256), dim3(256), 0, c->stream, (int*)c->d_code, (cap + 3) / 4,
+                     0x03030303);
+  return build_tiles(c);
+}
+
+extern "C" int amps_gpu_compact(amps_ctx* c) {
+  if (!c || !c->has_mesh || c->cap == 0) return fail(AMPS_E_STATE, "amps_gpu_compact: no particles");
+  (void)hipSetDevice(c->device);
+  tbegin(c);
+  int r = compact_impl(c);
+  tend(c, AMPS_K_COMPACT);
+  return r;
+}
+
+// ---------------------------------------------------------------------
+// mover
+// ---------------------------------------------------------------------
+extern "C" int amps_gpu_move_particles(amps_ctx* c, const amps_push_opts* o) {
+  if (!c || !o) return fail(AMPS_E_ARG, "amps_gpu_move_particles: null argument");
+  if (!c->has_mesh || !c->has_fields || !c->has_species)
+    return fail(AMPS_E_STATE, "amps_gpu_move_particles: mesh, fields and species are required");
+  if (o->mover != AMPS_MOVER_LAPENTA && o->mover != AMPS_MOVER_BORIS)
+    return fail(AMPS_E_ARG, "amps_gpu_move_particles: unknown mover");
+  if (o->bc_mode != AMPS_BC_DELETE && o->bc_mode != AMPS_BC_REFLECT)
+    return fail(AMPS_E_ARG, "amps_gpu_move_particles: unknown boundary mode");
+  if (c->cap == 0) return AMPS_OK;
+  (void)hipSetDevice(c->device);
+  if (o->tile > 0 && o->tile != c->tile) {
+    if (int r = amps_gpu_set_param(c, "tile", o->tile)) return r;
+  }
+  if (!c->offsets_valid) {
+    if (int r = amps_gpu_compact(c)) return r;
+  } else if (!c->tiles_valid) {
+    if (int r = build_tiles(c)) return r;
+  }
+  const int k = c->cur;
+  tbegin(c);
+  if (o->mover == AMPS_MOVER_LAPENTA)
+    hipLaunchKernelGGL(k_push_f64<AMPS_MOVER_LAPENTA>, dim3((unsigned)c->max_tiles), dim3(256), 0,
+                       c->stream, c->mv, c->faces, c->sp, c->d_E, c->d_B, c->d_tiles, c->d_ntiles,
+                       o->bc_mode, o->r_sphere, c->cap, c->d_x[k], c->d_v[k], c->d_spec[k],
+                       c->d_key[k], c->d_code);
+  else
+    hipLaunchKernelGGL(k_push_f64<AMPS_MOVER_BORIS>, dim3((unsigned)c->max_tiles), dim3(256), 0,
+                       c->stream, c->mv, c->faces, c->sp, c->d_E, c->d_B, c->d_tiles, c->d_ntiles,
+                       o->bc_mode, o->r_sphere, c->cap, c->d_x[k], c->d_v[k], c->d_spec[k],
+                       c->d_key[k], c->d_code);
+  HIP_TRY(hipGetLastError());
+  tend(c, AMPS_K_PUSH);
+  c->offsets_valid = false;
+  c->tiles_valid = false;
+  if (o->compact) return amps_gpu_compact(c);
+  return AMPS_OK;
+}
+
+// ---------------------------------------------------------------------
+// ECSIM deposit
+// ---------------------------------------------------------------------
+template <int NCH, bool EX>
+static void launch_dep(amps_ctx* c, double dt) {
+  const int k = c->cur;
+  hipLaunchKernelGGL((k_deposit_cell<NCH, EX>), dim3((unsigned)c->mv.nb * 8), dim3(64 * NCH), 0,
+                     c->stream, c->mv, c->sp, dt, c->d_B, c->d_offset, c->cap, c->d_x[k],
+                     c->d_v[k], c->d_spec[k], c->d_cellpart);
+}
+
+extern "C" int amps_gpu_update_j_mass_matrix(amps_ctx* c, double dt, double* J_out, double* M_out) {
+  if (!c) return fail(AMPS_E_ARG, "null context");
+  if (!c->has_mesh || !c->has_fields || !c->has_species)
+    return fail(AMPS_E_STATE, "amps_gpu_update_j_mass_matrix: mesh, fields and species are required");
+  (void)hipSetDevice(c->device);
+  if (!c->d_cellpart) HIP_TRY(hipMalloc(&c->d_cellpart, sizeof(double) * (size_t)c->mv.nb * NCELL * CELL_VALS));
+  if (!c->d_nodeout) HIP_TRY(hipMalloc(&c->d_nodeout, sizeof(double) * (size_t)c->nnodes * NODE_VALS));
+  if (c->cap == 0) {
+    HIP_TRY(hipMemsetAsync(c->d_nodeout, 0, sizeof(double) * (size_t)c->nnodes * NODE_VALS, c->stream));
+  } else {
+    if (!c->offsets_valid)
+      if (int r = amps_gpu_compact(c)) return r;
+    tbegin(c);
+    const bool ex = c->dep_exact != 0;
+    switch (c->dep_nch) {
+      case 2: ex ? launch_dep<2, true>(c, dt) : launch_dep<2, false>(c, dt); break;
+      case 3: ex ? launch_dep<3, true>(c, dt) : launch_dep<3, false>(c, dt); break;
+      case 6: ex ? launch_dep<6, true>(c, dt) : launch_dep<6, false>(c, dt); break;
+      default: ex ? launch_dep<4, true>(c, dt) : launch_dep<4, false>(c, dt); break;
+    }
+    HIP_TRY(hipGetLastError());
+    tend(c, AMPS_K_DEPOSIT_CELL);
+    tbegin(c);
+    hipLaunchKernelGGL(k_deposit_node, dim3((unsigned)((c->nnodes + 3) / 4)), dim3(256), 0,
+                       c->stream, c->nnodes, c->d_inc_start, c->d_inc, c->d_cellpart,
+                       c->d_nodeout);
+    HIP_TRY(hipGetLastError());
+    tend(c, AMPS_K_DEPOSIT_NODE);
+  }
+  if (J_out || M_out) {
+    std::vector<double> h((size_t)c->nnodes * NODE_VALS);
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    HIP_TRY(hipMemcpy(h.data(), c->d_nodeout, sizeof(double) * h.size(), hipMemcpyDeviceToHost));
+    for (int64_t i = 0; i < c->nnodes; i++) {
+      if (J_out) std::memcpy(J_out + 3 * i, &h[(size_t)i * NODE_VALS], 3 * sizeof(double));
+      if (M_out) std::memcpy(M_out + 243 * i, &h[(size_t)i * NODE_VALS + 3], 243 * sizeof(double));
+    }
+  }
+  return AMPS_OK;
+}
+
+// ---------------------------------------------------------------------
+// timing
+// ---------------------------------------------------------------------
+extern "C" int amps_gpu_set_timing(amps_ctx* c, int32_t enable) {
+  if (!c) return fail(AMPS_E_ARG, "null context");
+  c->timing = enable != 0;
+  return AMPS_OK;
+}
+extern "C" int amps_gpu_get_timing(amps_ctx* c, double* ms, int64_t* launches) {
+  if (!c) return fail(AMPS_E_ARG, "null context");
+  for (int k = 0; k < AMPS_K_COUNT; k++) {
+    if (ms) ms[k] = c->ms[k];
+    if (launches) launches[k] = c->launches[k];
+  }
+  return AMPS_OK;
+}
+extern "C" int amps_gpu_reset_timing(amps_ctx* c) {
+  if (!c) return fail(AMPS_E_ARG, "null context");
+  for (int k = 0; k < AMPS_K_COUNT; k++) c->ms[k] = 0, c->launches[k] = 0;
+  return AMPS_OK;
+}

@@ -1,0 +1,401 @@
+// pic_device.h — per-particle device routines of the FP64 push and deposit.
+// Each routine cites the reference lines it follows; the expression order
+// is the reference's (the library is compiled with -ffp-contract=off).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "amps_common.h"
+
+namespace amps {
+
+struct BlockGeom {
+  double xmin[3], xmax[3], dxs[3], dxc[3];
+};
+
+__device__ __forceinline__ void load_block_geom(const MeshView& m, int b, BlockGeom& g) {
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    g.xmin[d] = m.blk_xmin[3 * b + d];
+    g.xmax[d] = m.blk_xmax[3 * b + d];
+    g.dxs[d] = m.blk_dxs[3 * b + d];
+    g.dxc[d] = m.blk_dxc[3 * b + d];
+  }
+}
+
+__device__ __forceinline__ int clampi(int a, int lo, int hi) { return a < lo ? lo : (a > hi ? hi : a); }
+
+// CornerBased::InitStencil — pic_interpolation_routines.cpp:1074-1194.
+// wn[l], l = 4i+2j+k in the reference's stencil loop order, normalized as
+// cStencil::Normalize (pic.h:7255-7262); table[] in the reference's
+// InterpolationCoefficientTable order 0,1,3,2,4,5,7,6.  x may be nudged off
+// the block's upper face exactly as the reference does.
+__device__ __forceinline__ void corner_stencil(const BlockGeom& g, double x[3], double wn[8],
+                                               double table[8], int iX[3]) {
+  double xLoc[3];
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    if (fabs(x[d] - g.xmax[d]) < 1e-10 * g.dxs[d]) x[d] = g.xmax[d] - 1e-10 * g.dxs[d];
+    xLoc[d] = (x[d] - g.xmin[d]) / g.dxs[d];
+    iX[d] = (int)(xLoc[d]);
+    xLoc[d] -= iX[d];
+  }
+  constexpr int tslot[8] = {0, 4, 3, 7, 1, 5, 2, 6};
+  double norm = 0.0;
+#pragma unroll
+  for (int l = 0; l < 8; l++) {
+    const int i = l >> 2, j = (l >> 1) & 1, k = l & 1;
+    const double a = i ? xLoc[0] : (1.0 - xLoc[0]);
+    const double b = j ? xLoc[1] : (1.0 - xLoc[1]);
+    const double c = k ? xLoc[2] : (1.0 - xLoc[2]);
+    const double w = a * b * c;
+    table[tslot[l]] = w;
+    wn[l] = w;
+    norm += w;
+  }
+  if (norm > 0.0) {
+#pragma unroll
+    for (int l = 0; l < 8; l++) wn[l] /= norm;
+  }
+}
+
+// findTreeNode(int*) as a lookup: the descent of meshAMRgeneric.h:2793-2848
+// returns the leaf containing the lattice point, or NULL outside the root.
+__device__ __forceinline__ int lookup_leaf(const MeshView& m, const int ix[3]) {
+  const int lim = 1 << m.lmax;
+#pragma unroll
+  for (int d = 0; d < 3; d++)
+    if (ix[d] < 0 || ix[d] >= lim) return -1;
+  const int sh = m.lmax - m.lf;
+  return m.leaf_of[((ix[2] >> sh) * m.nf + (ix[1] >> sh)) * m.nf + (ix[0] >> sh)];
+}
+
+// findTreeNode(double*) — meshAMRgeneric.h:2852-2882, including the
+// half-open correction against the found block's bounds.
+__device__ __forceinline__ int find_block(const MeshView& m, const double x[3]) {
+  int ix[3];
+#pragma unroll
+  for (int d = 0; d < 3; d++) ix[d] = (int)floor((x[d] - m.gxmin[d]) / m.dx_maxref[d]);
+  int res = lookup_leaf(m, ix);
+  if (res >= 0) {
+    bool flag = false;
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+      if (x[d] < m.blk_xmin[3 * res + d]) ix[d]--, flag = true;
+      if (x[d] >= m.blk_xmax[3 * res + d]) ix[d]++, flag = true;
+    }
+    if (flag) res = lookup_leaf(m, ix);
+  }
+  return res;
+}
+
+// FindCellIndex — meshAMRgeneric.h:2256-2323 (ExitFlag=false).
+__device__ __forceinline__ int find_cell(const MeshView& m, const double x[3], int b) {
+  int ijk[3];
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    const double xmn = m.blk_xmin[3 * b + d], xmx = m.blk_xmax[3 * b + d];
+    if ((x[d] < xmn) || (xmx < x[d])) return -1;
+    int i = (int)((x[d] - xmn) / m.blk_dxc[3 * b + d]);
+    if (i == NC) i = NC - 1;
+    ijk[d] = clampi(i, 0, NC - 1);
+  }
+  return ijk[0] + NC * (ijk[1] + NC * ijk[2]);
+}
+
+// One particle of PIC::Mover::Lapenta2017 (pic_mover_boris.cpp:876-1380) or
+// PIC::Mover::Boris (:126-553, rotation :253-277) with corner E and B from
+// LDS.  Returns the AMPS_CODE_*; xp/vp hold the committed state.
+template <int MOVER>
+__device__ __forceinline__ int push_particle(const MeshView& m, const Faces& faces,
+                                             const BlockGeom& g, const double* sE,
+                                             const double* sB, double q, double mass,
+                                             double dtTotal, int bc_mode, double r_sphere,
+                                             double xp[3], double vp[3], int* blk, int* cell) {
+  double vInit[3] = {vp[0], vp[1], vp[2]}, xInit[3] = {xp[0], xp[1], xp[2]};
+  double vFinal[3], xFinal[3];
+  double wn[8], table[8];
+  int iX[3];
+  corner_stencil(g, xInit, wn, table, iX);
+#pragma unroll
+  for (int d = 0; d < 3; d++) iX[d] = clampi(iX[d], 0, NC - 1);
+  double E[3] = {0.0, 0.0, 0.0}, B[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+  for (int l = 0; l < 8; l++) {
+    const int c = (iX[0] + (l >> 2)) + NN * ((iX[1] + ((l >> 1) & 1)) + NN * (iX[2] + (l & 1)));
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+      E[d] += wn[l] * sE[3 * c + d];
+      B[d] += wn[l] * sB[3 * c + d];
+    }
+  }
+  if (MOVER == AMPS_MOVER_LAPENTA) {
+    // :1033-1081
+    const double QdT_over_m = q * dtTotal / mass;
+    const double QdT_over_2m = 0.5 * QdT_over_m;
+    const double QdT_over_2m_squared = QdT_over_2m * QdT_over_2m;
+    double BB[3][3], P[3], alpha[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+      P[i] = -QdT_over_2m * B[i];
+#pragma unroll
+      for (int j = 0; j <= i; j++) {
+        BB[i][j] = QdT_over_2m_squared * B[i] * B[j];
+        BB[j][i] = BB[i][j];
+      }
+    }
+    const double c0 =
+        1.0 / (1.0 + QdT_over_2m_squared * (B[0] * B[0] + B[1] * B[1] + B[2] * B[2]));
+    alpha[0][0] = c0 * (1.0 + BB[0][0]);
+    alpha[0][1] = c0 * (-P[2] + BB[0][1]);
+    alpha[0][2] = c0 * (P[1] + BB[0][2]);
+    alpha[1][0] = c0 * (P[2] + BB[1][0]);
+    alpha[1][1] = c0 * (1.0 + BB[1][1]);
+    alpha[1][2] = c0 * (-P[0] + BB[1][2]);
+    alpha[2][0] = c0 * (-P[1] + BB[2][0]);
+    alpha[2][1] = c0 * (P[0] + BB[2][1]);
+    alpha[2][2] = c0 * (1.0 + BB[2][2]);
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+      double s = 0.0;
+#pragma unroll
+      for (int j = 0; j < 3; j++) s += alpha[i][j] * (vInit[j] + QdT_over_2m * E[j]);
+      vFinal[i] = 2.0 * s - vInit[i];
+    }
+#pragma unroll
+    for (int i = 0; i < 3; i++) xFinal[i] = xInit[i] + dtTotal * vFinal[i];
+  } else {
+    // BorisSplitAcceleration_default :87-98, rotation :253-277
+    const double Charge2Mass = q / mass;
+    double accl[3], rot[3], u[3], h[3], U[3];
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+      accl[d] = 0.0 + Charge2Mass * E[d];
+      rot[d] = 0.0 - Charge2Mass * B[d];
+    }
+    const double dtTemp = dtTotal, dtTempOverTwo = dtTotal / 2.0;
+#pragma unroll
+    for (int d = 0; d < 3; d++) u[d] = vInit[d] + dtTempOverTwo * accl[d];
+#pragma unroll
+    for (int d = 0; d < 3; d++) h[d] = -dtTempOverTwo * rot[d];
+    const double h2 = h[0] * h[0] + h[1] * h[1] + h[2] * h[2];
+    const double uh = u[0] * h[0] + u[1] * h[1] + u[2] * h[2];
+    U[0] = ((1 - h2) * u[0] + 2 * (u[1] * h[2] - h[1] * u[2] + uh * h[0])) / (1 + h2);
+    U[1] = ((1 - h2) * u[1] + 2 * (u[2] * h[0] - h[2] * u[0] + uh * h[1])) / (1 + h2);
+    U[2] = ((1 - h2) * u[2] + 2 * (u[0] * h[1] - h[0] * u[1] + uh * h[2])) / (1 + h2);
+#pragma unroll
+    for (int d = 0; d < 3; d++) vFinal[d] = U[d] + dtTempOverTwo * accl[d];
+#pragma unroll
+    for (int d = 0; d < 3; d++) xFinal[d] = xInit[d] + dtTemp * vFinal[d];
+  }
+
+  // internal sphere (:1088-1117): absorbed
+  double rFinal2;
+  if (r_sphere > 0.0 &&
+      (rFinal2 = xFinal[0] * xFinal[0] + xFinal[1] * xFinal[1] + xFinal[2] * xFinal[2]) <
+          r_sphere * r_sphere) {
+    const double r = sqrt(rFinal2);
+#pragma unroll
+    for (int d = 0; d < 3; d++) xFinal[d] *= r_sphere / r;
+#pragma unroll
+    for (int d = 0; d < 3; d++) xp[d] = xFinal[d], vp[d] = vFinal[d];
+    return AMPS_CODE_SPHERE;
+  }
+  int newNode = find_block(m, xFinal);
+  int code = AMPS_CODE_FINISHED;
+  if (newNode < 0) {
+    if (bc_mode == AMPS_BC_DELETE) {
+#pragma unroll
+      for (int d = 0; d < 3; d++) xp[d] = xFinal[d], vp[d] = vFinal[d];
+      return AMPS_CODE_LEFT_DOMAIN;
+    }
+    // exit face (:1156-1229); the face index starts at -1 here
+    int nIntersectionFace = -1;
+    const double vMiddle[3] = {0.5 * (vInit[0] + vFinal[0]), 0.5 * (vInit[1] + vFinal[1]),
+                               0.5 * (vInit[2] + vFinal[2])};
+    double dtIntersection = -1.0, r0[3];
+    for (int nface = 0; nface < 6; nface++) {
+      const Face& F = faces.f[nface];
+      double cx = 0.0, cv = 0.0;
+#pragma unroll
+      for (int d = 0; d < 3; d++) {
+        r0[d] = xInit[d] - F.x0[d];
+        cx += r0[d] * F.norm[d];
+        cv += vMiddle[d] * F.norm[d];
+      }
+      if (cv > 0.0) {
+        const double dt = -cx / cv;
+        if ((dtIntersection < 0.0) || ((dt < dtIntersection) && (dt > 0.0))) {
+          double cE0 = 0.0, cE1 = 0.0;
+#pragma unroll
+          for (int d = 0; d < 3; d++) {
+            const double c = r0[d] + dt * vMiddle[d];
+            cE0 += c * F.e0[d], cE1 += c * F.e1[d];
+          }
+          if ((cE0 < -m.eps) || (cE0 > F.lE0 + m.eps) || (cE1 < -m.eps) || (cE1 > F.lE1 + m.eps))
+            continue;
+          nIntersectionFace = nface, dtIntersection = dt;
+        }
+      }
+    }
+    if (nIntersectionFace == -1) {
+#pragma unroll
+      for (int d = 0; d < 3; d++) xp[d] = xFinal[d], vp[d] = vFinal[d];
+      return AMPS_CODE_LEFT_DOMAIN;
+    }
+    const Face& F = faces.f[nIntersectionFace];
+    const double tVelocityIncrement =
+        ((dtIntersection / dtTotal < 1) ? dtIntersection / dtTotal : 1);
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+      xInit[d] += dtIntersection * vMiddle[d] - F.norm[d] * m.eps;
+      vInit[d] += tVelocityIncrement * (vFinal[d] - vInit[d]);
+    }
+    newNode = find_block(m, xInit);
+    if (newNode < 0) {
+#pragma unroll
+      for (int d = 0; d < 3; d++) {
+        if (m.gxmin[d] >= xInit[d]) xInit[d] = m.gxmin[d] + m.eps;
+        if (m.gxmax[d] <= xInit[d]) xInit[d] = m.gxmax[d] - m.eps;
+      }
+      newNode = find_block(m, xInit);
+      if (newNode < 0) {
+#pragma unroll
+        for (int d = 0; d < 3; d++) xp[d] = xInit[d], vp[d] = vInit[d];
+        return AMPS_CODE_LEFT_DOMAIN;
+      }
+    }
+    // specular reflection (:1236-1253), committed
+    double cc = 0.0;
+#pragma unroll
+    for (int d = 0; d < 3; d++) cc += F.norm[d] * vInit[d];
+#pragma unroll
+    for (int d = 0; d < 3; d++) vInit[d] -= 2.0 * cc * F.norm[d];
+#pragma unroll
+    for (int d = 0; d < 3; d++) vFinal[d] = vInit[d], xFinal[d] = xInit[d];
+    code = AMPS_CODE_REFLECTED;
+  }
+  const int c_id = find_cell(m, xFinal, newNode);
+  if (c_id < 0) return AMPS_CODE_LEFT_DOMAIN;
+  *blk = newNode;
+  *cell = c_id;
+#pragma unroll
+  for (int d = 0; d < 3; d++) xp[d] = xFinal[d], vp[d] = vFinal[d];
+  return code;
+}
+
+// ---------------------------------------------------------------------
+// ProcessCell per-particle body restricted to one chunk [O0,O1) of the
+// 348 per-cell outputs: 0..23 J (corner c = o/3, dim o%3), then 36 lower
+// triangular corner pairs (i >= j, i-major) x 9 alpha entries.
+// ---------------------------------------------------------------------
+constexpr int pair_i(int pr) {
+  int i = 0;
+  while ((i + 1) * (i + 2) / 2 <= pr) i++;
+  return i;
+}
+constexpr int pair_j(int pr) { return pr - pair_i(pr) * (pair_i(pr) + 1) / 2; }
+
+template <int O0, int O1, bool EXACT>
+__device__ __forceinline__ void cell_chunk(const BlockGeom& g, const Species& sp, double dtTotal,
+                                           const double* sB, int sx, int sy, int sz, int p0,
+                                           int p1, int64_t cap, const double* __restrict__ x,
+                                           const double* __restrict__ v,
+                                           const uint8_t* __restrict__ spec, double CellVolume,
+                                           double* out) {
+  constexpr int NO = O1 - O0;
+  double acc[NO];
+#pragma unroll
+  for (int o = 0; o < NO; o++) acc[o] = 0.0;
+  for (int p = p0; p < p1; ++p) {
+    double xI[3] = {x[p], x[cap + p], x[2 * cap + p]};
+    const double vI[3] = {v[p], v[cap + p], v[2 * cap + p]};
+    const int s = spec[p];
+    double wn[8], table[8];
+    int iX[3];
+    corner_stencil(g, xI, wn, table, iX);
+    iX[0] = clampi(iX[0], sx, sx + 3);
+    iX[1] = clampi(iX[1], sy, sy + 3);
+    iX[2] = clampi(iX[2], sz, sz + 3);
+    double Bv[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int l = 0; l < 8; l++) {
+      const int c = (iX[0] - sx + (l >> 2)) + 5 * ((iX[1] - sy + ((l >> 1) & 1)) + 5 * (iX[2] - sz + (l & 1)));
+#pragma unroll
+      for (int d = 0; d < 3; d++) Bv[d] += wn[l] * sB[3 * c + d];
+    }
+    double chargeQ = sp.charge[s], mass = sp.mass[s];
+    chargeQ *= sp.weight[s];
+    mass *= sp.weight[s];
+    const double QdT_over_m = chargeQ * dtTotal / mass;
+    const double QdT_over_2m = 0.5 * QdT_over_m;
+    const double QdT_over_2m_squared = QdT_over_2m * QdT_over_2m;
+    double BB[3][3], P[3], alpha[9];
+#pragma unroll
+    for (int ii = 0; ii < 3; ii++) {
+      P[ii] = -QdT_over_2m * Bv[ii];
+#pragma unroll
+      for (int jj = 0; jj <= ii; jj++) {
+        BB[ii][jj] = QdT_over_2m_squared * Bv[ii] * Bv[jj];
+        BB[jj][ii] = BB[ii][jj];
+      }
+    }
+    const double c0 =
+        1.0 / (1.0 + QdT_over_2m_squared * (Bv[0] * Bv[0] + Bv[1] * Bv[1] + Bv[2] * Bv[2]));
+    alpha[0] = c0 * (1.0 + BB[0][0]);
+    alpha[1] = c0 * (-P[2] + BB[0][1]);
+    alpha[2] = c0 * (P[1] + BB[0][2]);
+    alpha[3] = c0 * (P[2] + BB[1][0]);
+    alpha[4] = c0 * (1.0 + BB[1][1]);
+    alpha[5] = c0 * (-P[0] + BB[1][2]);
+    alpha[6] = c0 * (-P[1] + BB[2][0]);
+    alpha[7] = c0 * (P[0] + BB[2][1]);
+    alpha[8] = c0 * (1.0 + BB[2][2]);
+    double vRot[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+      for (int jj = 0; jj < 3; jj++) vRot[i] += alpha[3 * i + jj] * vI[jj];
+    const double matrixConst = chargeQ * QdT_over_2m / CellVolume;
+    double twc[8];
+#pragma unroll
+    for (int i = 0; i < 8; i++) twc[i] = matrixConst * table[i];
+#pragma unroll
+    for (int o = O0; o < O1; o++) {
+      if (o < 24) {
+        const double t = chargeQ * table[o / 3];
+        if (EXACT) acc[o - O0] += t * vRot[o % 3];
+        else acc[o - O0] = fma(t, vRot[o % 3], acc[o - O0]);
+      } else {
+        const int pr = (o - 24) / 9, k = (o - 24) % 9;
+        const double twp = table[pair_j(pr)] * twc[pair_i(pr)];
+        if (EXACT) acc[o - O0] += alpha[k] * twp;
+        else acc[o - O0] = fma(alpha[k], twp, acc[o - O0]);
+      }
+    }
+  }
+#pragma unroll
+  for (int o = O0; o < O1; o++) out[o] = (o < 24) ? acc[o - O0] / CellVolume : acc[o - O0];
+}
+
+template <int NCH, bool EXACT, int W = 0>
+__device__ __forceinline__ void dispatch_chunk(int w, const BlockGeom& g, const Species& sp,
+                                               double dtTotal, const double* sB, int sx, int sy,
+                                               int sz, int p0, int p1, int64_t cap,
+                                               const double* __restrict__ x,
+                                               const double* __restrict__ v,
+                                               const uint8_t* __restrict__ spec,
+                                               double CellVolume, double* out) {
+  if constexpr (W < NCH) {
+    constexpr int CH = (CELL_VALS + NCH - 1) / NCH;
+    constexpr int O0 = W * CH;
+    constexpr int O1 = (O0 + CH < CELL_VALS) ? O0 + CH : CELL_VALS;
+    if (w == W)
+      cell_chunk<O0, O1, EXACT>(g, sp, dtTotal, sB, sx, sy, sz, p0, p1, cap, x, v, spec,
+                                CellVolume, out);
+    else
+      dispatch_chunk<NCH, EXACT, W + 1>(w, g, sp, dtTotal, sB, sx, sy, sz, p0, p1, cap, x, v,
+                                        spec, CellVolume, out);
+  }
+}
+
+}  // namespace amps

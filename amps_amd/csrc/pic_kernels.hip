@@ -1,0 +1,379 @@
+// pic_kernels.hip — push, migration and ECSIM deposit kernels for gfx950.
+//
+// Built with -ffp-contract=off: every FP64 expression of the FP64 path rounds
+// as written, in the reference's order, so positions, velocities, block and
+// cell ids reproduce the CPU restatement bit for bit.  Deliberate fused
+// multiply-adds are written as fma().
+#include <hip/hip_runtime.h>
+
+#include "amps_common.h"
+#include "pic_device.h"
+
+using namespace amps;
+
+// ======================================================================
+// Push: one workgroup per tile of particles of a single block; the block's
+// corner E and B (729 x 3 doubles each) are staged in LDS (SetBlock_E/B,
+// pic_mover.cpp:84-186), particles are read and written coalesced (SoA).
+// ======================================================================
+template <int MOVER>
+__global__ __launch_bounds__(256) void k_push_f64(MeshView m, Faces faces, Species sp,
+                                                   const double* __restrict__ E,
+                                                   const double* __restrict__ B,
+                                                   const Tile* __restrict__ tiles,
+                                                   const int* __restrict__ ntiles, int bc_mode,
+                                                   double r_sphere, int64_t cap, double* x,
+                                                   double* v, const uint8_t* __restrict__ spec,
+                                                   int* key, int8_t* code) {
+  __shared__ double sE[NCORNER * 3];
+  __shared__ double sB[NCORNER * 3];
+  const int t = blockIdx.x;
+  if (t >= *ntiles) return;
+  const Tile tl = tiles[t];
+  const int b = tl.block;
+  const double* Eb = E + (size_t)b * NCORNER * 3;
+  const double* Bb = B + (size_t)b * NCORNER * 3;
+  for (int i = threadIdx.x; i < NCORNER * 3; i += blockDim.x) {
+    sE[i] = Eb[i];
+    sB[i] = Bb[i];
+  }
+  __syncthreads();
+  BlockGeom g;
+  load_block_geom(m, b, g);
+  const int p1 = tl.start + tl.count;
+  for (int p = tl.start + threadIdx.x; p < p1; p += blockDim.x) {
+    double xp[3] = {x[p], x[cap + p], x[2 * cap + p]};
+    double vp[3] = {v[p], v[cap + p], v[2 * cap + p]};
+    const int s = spec[p];
+    int nb_out = b, cell_out = 0;
+    const int c = push_particle<MOVER>(m, faces, g, sE, sB, sp.charge[s], sp.mass[s], sp.dt[s],
+                                       bc_mode, r_sphere, xp, vp, &nb_out, &cell_out);
+    x[p] = xp[0];
+    x[cap + p] = xp[1];
+    x[2 * cap + p] = xp[2];
+    v[p] = vp[0];
+    v[cap + p] = vp[1];
+    v[2 * cap + p] = vp[2];
+    if (c == AMPS_CODE_FINISHED || c == AMPS_CODE_REFLECTED) key[p] = nb_out * NCELL + cell_out;
+    code[p] = (int8_t)c;
+  }
+}
+
+template __global__ void k_push_f64<AMPS_MOVER_LAPENTA>(MeshView, Faces, Species, const double*,
+                                                         const double*, const Tile*, const int*,
+                                                         int, double, int64_t, double*, double*,
+                                                         const uint8_t*, int*, int8_t*);
+template __global__ void k_push_f64<AMPS_MOVER_BORIS>(MeshView, Faces, Species, const double*,
+                                                       const double*, const Tile*, const int*, int,
+                                                       double, int64_t, double*, double*,
+                                                       const uint8_t*, int*, int8_t*);
+
+// ======================================================================
+// Migration: counting sort by key = block*512 + cell with wavefront
+// run-length aggregation (ballot + lane masks) so that one atomic serves a
+// whole run of equal keys; deleted particles are dropped.
+// ======================================================================
+__device__ __forceinline__ bool alive_code(int c) {
+  return c == AMPS_CODE_FINISHED || c == AMPS_CODE_REFLECTED;
+}
+
+struct RunInfo {
+  bool head;
+  int head_lane;
+  int len;  // valid on head lanes
+};
+
+// Runs of equal valid keys inside the wavefront.
+__device__ __forceinline__ RunInfo wave_runs(bool valid, int k) {
+  const int lane = __lane_id();
+  int prev = __shfl_up(k, 1);
+  bool pvalid = __shfl_up((int)valid, 1) != 0;
+  if (lane == 0) pvalid = false;
+  const bool head = valid && (!pvalid || prev != k);
+  const uint64_t heads = __ballot(head);
+  const uint64_t breaks = __ballot(head || !valid);
+  RunInfo r;
+  r.head = head;
+  const uint64_t upto = (lane == 63) ? ~0ull : ((2ull << lane) - 1ull);
+  const uint64_t hm = heads & upto;
+  r.head_lane = hm ? (63 - __clzll((long long)hm)) : lane;
+  const uint64_t after = (lane == 63) ? 0ull : (breaks & ~upto);
+  const int nxt = after ? (__ffsll((long long)after) - 1) : 64;
+  r.len = nxt - lane;
+  return r;
+}
+
+__global__ __launch_bounds__(256) void k_fill_int(int* a, int64_t n, int val) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    a[i] = val;
+}
+
+__global__ __launch_bounds__(256) void k_count_keys(const int* __restrict__ n_dev,
+                                                     const int* __restrict__ key,
+                                                     const int8_t* __restrict__ code,
+                                                     int* count) {
+  const int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int n = *n_dev;
+  const bool valid = (p < n) && alive_code(code[p]);
+  const int k = valid ? key[p] : -1;
+  const RunInfo r = wave_runs(valid, k);
+  if (r.head) atomicAdd(&count[k], r.len);
+}
+
+__global__ __launch_bounds__(256) void k_scatter(const int* __restrict__ n_dev,
+                                                  const int* __restrict__ key,
+                                                  const int8_t* __restrict__ code,
+                                                  const int* __restrict__ offset, int* cursor,
+                                                  int64_t cap, const double* __restrict__ xa,
+                                                  const double* __restrict__ va,
+                                                  const uint8_t* __restrict__ sa,
+                                                  const int64_t* __restrict__ ia, double* xb,
+                                                  double* vb, uint8_t* sb, int* kb,
+                                                  int64_t* ib) {
+  const int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int n = *n_dev;
+  const bool valid = (p < n) && alive_code(code[p]);
+  const int k = valid ? key[p] : -1;
+  const RunInfo r = wave_runs(valid, k);
+  int base = 0;
+  if (r.head) base = atomicAdd(&cursor[k], r.len);
+  base = __shfl(base, r.head_lane);
+  if (!valid) return;
+  const int64_t d = (int64_t)offset[k] + base + (__lane_id() - r.head_lane);
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    xb[c * cap + d] = xa[c * cap + p];
+    vb[c * cap + d] = va[c * cap + p];
+  }
+  sb[d] = sa[p];
+  kb[d] = k;
+  ib[d] = ia[p];
+}
+
+__global__ void k_copy_total(const int* __restrict__ src, int* dst) { *dst = *src; }
+
+// tiles of at most T particles, never straddling a block
+__global__ __launch_bounds__(256) void k_tile_count(int nb, int T, const int* __restrict__ offset,
+                                                     int* tcount) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b < nb) {
+    const int cnt = offset[(b + 1) * NCELL] - offset[b * NCELL];
+    tcount[b] = (cnt + T - 1) / T;
+  } else if (b == nb) {
+    tcount[nb] = 0;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_tile_emit(int nb, int T, const int* __restrict__ offset,
+                                                    const int* __restrict__ toff, Tile* tiles,
+                                                    int* ntiles) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b == 0) *ntiles = toff[nb];
+  if (b >= nb) return;
+  const int start = offset[b * NCELL];
+  const int cnt = offset[(b + 1) * NCELL] - start;
+  const int t0 = toff[b];
+  for (int t = 0; t * T < cnt; t++) {
+    Tile tl;
+    tl.block = b;
+    tl.start = start + t * T;
+    tl.count = min(T, cnt - t * T);
+    tl.pad = 0;
+    tiles[t0 + t] = tl;
+  }
+}
+
+// ======================================================================
+// Device-wide exclusive scan (in place), three phases.
+// ======================================================================
+constexpr int SCAN_BLOCK = 256;
+constexpr int SCAN_ITEMS = 8;
+constexpr int SCAN_TILE = SCAN_BLOCK * SCAN_ITEMS;
+
+__global__ __launch_bounds__(SCAN_BLOCK) void k_scan_tile(int* data, int64_t n, int* sums) {
+  __shared__ int s[SCAN_BLOCK];
+  const int64_t base = (int64_t)blockIdx.x * SCAN_TILE + threadIdx.x * SCAN_ITEMS;
+  int vals[SCAN_ITEMS];
+  int tot = 0;
+#pragma unroll
+  for (int i = 0; i < SCAN_ITEMS; i++) {
+    const int64_t j = base + i;
+    vals[i] = (j < n) ? data[j] : 0;
+    tot += vals[i];
+  }
+  s[threadIdx.x] = tot;
+  __syncthreads();
+  for (int off = 1; off < SCAN_BLOCK; off <<= 1) {
+    int t = (threadIdx.x >= off) ? s[threadIdx.x - off] : 0;
+    __syncthreads();
+    s[threadIdx.x] += t;
+    __syncthreads();
+  }
+  int run = s[threadIdx.x] - tot;  // exclusive prefix of this thread
+  if (threadIdx.x == SCAN_BLOCK - 1) sums[blockIdx.x] = s[threadIdx.x];
+#pragma unroll
+  for (int i = 0; i < SCAN_ITEMS; i++) {
+    const int64_t j = base + i;
+    if (j < n) data[j] = run;
+    run += vals[i];
+  }
+}
+
+__global__ __launch_bounds__(SCAN_BLOCK) void k_scan_add(int* data, int64_t n,
+                                                          const int* __restrict__ sums) {
+  const int add = sums[blockIdx.x];
+  const int64_t base = (int64_t)blockIdx.x * SCAN_TILE;
+  for (int i = threadIdx.x; i < SCAN_TILE; i += SCAN_BLOCK) {
+    const int64_t j = base + i;
+    if (j < n) data[j] += add;
+  }
+}
+
+namespace amps {
+static int scan_rec(amps_ctx* c, int* data, int64_t n, int* tmp) {
+  const int64_t nt = (n + SCAN_TILE - 1) / SCAN_TILE;
+  hipLaunchKernelGGL(k_scan_tile, dim3((unsigned)nt), dim3(SCAN_BLOCK), 0, c->stream, data, n, tmp);
+  if (nt > 1) {
+    int r = scan_rec(c, tmp, nt, tmp + nt);
+    if (r) return r;
+    hipLaunchKernelGGL(k_scan_add, dim3((unsigned)nt), dim3(SCAN_BLOCK), 0, c->stream, data, n,
+                       tmp);
+  }
+  return 0;
+}
+
+int scan_exclusive(amps_ctx* c, int* data, int64_t n, int* /*total_out*/) {
+  // temporary space: sum over levels of ceil(n / TILE^k) < n/(TILE-1) + levels
+  int64_t need = 0, m = n;
+  while (m > 1) {
+    m = (m + SCAN_TILE - 1) / SCAN_TILE;
+    need += m;
+  }
+  need += 8;
+  if (need > c->scan_tmp_len) {
+    if (c->d_scan_tmp) (void)hipFree(c->d_scan_tmp);
+    c->d_scan_tmp = nullptr;
+    HIP_TRY(hipMalloc(&c->d_scan_tmp, sizeof(int) * need));
+    c->scan_tmp_len = need;
+  }
+  return scan_rec(c, data, n, c->d_scan_tmp);
+}
+}  // namespace amps
+
+// ======================================================================
+// ECSIM deposit, phase 1: per-cell accumulation (ProcessCell,
+// pic_field_solver_ecsim.cpp:1882-2438).  One workgroup per 4x4x4 cell
+// sub-tile of a block; wave w owns output chunk w of the 348 per-cell
+// values and lane l owns cell l of the sub-tile, walking that cell's
+// particles in order with the chunk's accumulators held in registers.
+// ======================================================================
+template <int NCH, bool EXACT>
+__global__ __launch_bounds__(64 * NCH) void k_deposit_cell(MeshView m, Species sp, double dtTotal,
+                                                            const double* __restrict__ B,
+                                                            const int* __restrict__ offset,
+                                                            int64_t cap,
+                                                            const double* __restrict__ x,
+                                                            const double* __restrict__ v,
+                                                            const uint8_t* __restrict__ spec,
+                                                            double* __restrict__ cellpart) {
+  __shared__ double sB[125 * 3];
+  const int b = blockIdx.x >> 3;
+  const int st = blockIdx.x & 7;
+  const int sx = (st & 1) * 4, sy = ((st >> 1) & 1) * 4, sz = (st >> 2) * 4;
+  const double* Bb = B + (size_t)b * NCORNER * 3;
+  for (int i = threadIdx.x; i < 125 * 3; i += blockDim.x) {
+    const int cidx = i / 3, d = i % 3;
+    const int a0 = cidx % 5, a1 = (cidx / 5) % 5, a2 = cidx / 25;
+    sB[i] = Bb[3 * ((sx + a0) + NN * ((sy + a1) + NN * (sz + a2))) + d];
+  }
+  __syncthreads();
+  BlockGeom g;
+  load_block_geom(m, b, g);
+  const int lane = threadIdx.x & 63;
+  const int w = threadIdx.x >> 6;
+  const int ci = sx + (lane & 3), cj = sy + ((lane >> 2) & 3), ck = sz + (lane >> 4);
+  const int key = b * NCELL + ci + NC * (cj + NC * ck);
+  const int p0 = offset[key], p1 = offset[key + 1];
+  double dxc[3], CellVolume = 1;
+  for (int d = 0; d < 3; d++) dxc[d] = (g.xmax[d] - g.xmin[d]) / NC * 1.0;
+  for (int d = 0; d < 3; d++) CellVolume *= dxc[d];
+  double* out = cellpart + (size_t)key * CELL_VALS;
+  dispatch_chunk<NCH, EXACT>(w, g, sp, dtTotal, sB, sx, sy, sz, p0, p1, cap, x, v, spec,
+                             CellVolume, out);
+}
+
+#define INST_DEP(NCH, EX)                                                                  \
+  template __global__ void k_deposit_cell<NCH, EX>(MeshView, Species, double, const double*, \
+                                                    const int*, int64_t, const double*,     \
+                                                    const double*, const uint8_t*, double*);
+INST_DEP(2, true)
+INST_DEP(2, false)
+INST_DEP(3, true)
+INST_DEP(3, false)
+INST_DEP(4, true)
+INST_DEP(4, false)
+INST_DEP(6, true)
+INST_DEP(6, false)
+
+// ======================================================================
+// ECSIM deposit, phase 2: corner flush.  One wavefront per unique node sums
+// the contributions of its incident cells in ascending cell order — the
+// order of the reference's serial cell loop — into J (3) and the 27x9 mass
+// matrix row block (IndexMatrix, pic_field_solver_ecsim.cpp:1377-1379).
+// ======================================================================
+__constant__ signed char c_inv_im[8][27];  // slot -> partner corner j, or -1
+
+__global__ __launch_bounds__(256) void k_deposit_node(int64_t nnodes,
+                                                       const int* __restrict__ inc_start,
+                                                       const int* __restrict__ inc,
+                                                       const double* __restrict__ cellpart,
+                                                       double* __restrict__ nodeout) {
+  const int64_t node = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (node >= nnodes) return;
+  const int lane = threadIdx.x & 63;
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  const int e0 = inc_start[node], e1 = inc_start[node + 1];
+  for (int e = e0; e < e1; e++) {
+    const int ic = inc[e];
+    const int q = ic & 7;
+    const double* P = cellpart + (size_t)(ic >> 3) * CELL_VALS;
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      const int val = lane + 64 * r;
+      if (val < 3) {
+        acc[r] += P[3 * q + val];
+      } else if (val < NODE_VALS) {
+        const int s = (val - 3) / 9, k = (val - 3) % 9;
+        const int j = c_inv_im[q][s];
+        if (j >= 0) {
+          const int hi = q > j ? q : j, lo = q > j ? j : q;
+          acc[r] += P[24 + 9 * (hi * (hi + 1) / 2 + lo) + k];
+        }
+      }
+    }
+  }
+  double* o = nodeout + (size_t)node * NODE_VALS;
+#pragma unroll
+  for (int r = 0; r < 4; r++) {
+    const int val = lane + 64 * r;
+    if (val < NODE_VALS) o[val] = acc[r];
+  }
+}
+
+namespace amps {
+int upload_inv_index_matrix(hipStream_t s) {
+  static const int IM[8][8] = {{0, 2, 8, 6, 18, 20, 26, 24}, {1, 0, 6, 7, 19, 18, 24, 25},
+                               {4, 3, 0, 1, 22, 21, 18, 19}, {3, 5, 2, 0, 21, 23, 20, 18},
+                               {9, 11, 17, 15, 0, 2, 8, 6},  {10, 9, 15, 16, 1, 0, 6, 7},
+                               {13, 12, 9, 10, 4, 3, 0, 1},  {12, 14, 11, 9, 3, 5, 2, 0}};
+  signed char inv[8][27];
+  for (int q = 0; q < 8; q++) {
+    for (int s2 = 0; s2 < 27; s2++) inv[q][s2] = -1;
+    for (int j = 0; j < 8; j++) inv[q][IM[q][j]] = (signed char)j;
+  }
+  HIP_TRY(hipMemcpyToSymbolAsync(HIP_SYMBOL(c_inv_im), inv, sizeof(inv), 0,
+                                 hipMemcpyHostToDevice, s));
+  return 0;
+}
+}  // namespace amps

@@ -1,0 +1,115 @@
+"""Host mirror of the reference's PIC mover / ECSIM deposit interface.
+
+``PicState`` owns one library context holding the mesh, corner fields,
+species table and the resident SoA particle population.  Its methods carry
+the reference's names:
+
+* ``MoveParticles``  — PIC::Mover::MoveParticles (pic_mover.cpp:580-1088)
+  through the per-particle Lapenta2017 (pic_mover_boris.cpp:876-1380) or
+  Boris (:126-553) mover.
+* ``UpdateJMassMatrix`` — ECSIM::UpdateJMassMatrix
+  (pic_field_solver_ecsim.cpp:3244-3995) through ProcessCell (:1882-2438).
+
+All work runs in the HIP library; there is no CPU path here.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _lib as L
+from .mesh import Particles
+
+
+class Species:
+    def __init__(self, charge, mass, dt, weight=None):
+        self.charge = np.asarray(charge, np.float64)
+        self.mass = np.asarray(mass, np.float64)
+        self.dt = np.broadcast_to(np.asarray(dt, np.float64), self.charge.shape).copy()
+        self.weight = (np.ones_like(self.charge) if weight is None
+                       else np.broadcast_to(np.asarray(weight, np.float64), self.charge.shape).copy())
+
+    @property
+    def n(self):
+        return len(self.charge)
+
+
+class PicState:
+    def __init__(self, mesh, E_corner, B_corner, species, ctx=None):
+        self.ctx = ctx if ctx is not None else L.Context(0)
+        self.lib = self.ctx.lib
+        self.mesh = mesh
+        self.species = species
+        d = L.MeshDesc()
+        for i in range(3):
+            d.xmin[i] = mesh.xmin[i]
+            d.xmax[i] = mesh.xmax[i]
+        d.lmax = mesh.lmax
+        d.nblocks = mesh.nblocks
+        lv, ijk = mesh.desc_arrays()
+        L.check(self.lib.amps_gpu_set_mesh(self.ctx.h, C.byref(d), L._ptr(lv, C.c_int32),
+                                           L._ptr(ijk, C.c_int64)), "set_mesh")
+        self.set_fields(E_corner, B_corner)
+        s = species
+        L.check(self.lib.amps_gpu_set_species(self.ctx.h, s.n, L._ptr(s.charge, C.c_double),
+                                              L._ptr(s.mass, C.c_double), L._ptr(s.dt, C.c_double),
+                                              L._ptr(s.weight, C.c_double)), "set_species")
+        self.nnodes = int(self.lib.amps_gpu_node_count(self.ctx.h))
+
+    def set_fields(self, E_corner, B_corner):
+        E = np.ascontiguousarray(E_corner, np.float64)
+        B = np.ascontiguousarray(B_corner, np.float64)
+        assert E.shape == (self.mesh.nblocks, 729, 3) and B.shape == E.shape
+        L.check(self.lib.amps_gpu_set_fields(self.ctx.h, L._ptr(E, C.c_double),
+                                             L._ptr(B, C.c_double)), "set_fields")
+
+    def node_map(self):
+        noc = np.zeros(self.mesh.nblocks * 729, np.int32)
+        keys = np.zeros(self.nnodes, np.int64)
+        L.check(self.lib.amps_gpu_get_node_map(self.ctx.h, L._ptr(noc, C.c_int32),
+                                               L._ptr(keys, C.c_int64)), "get_node_map")
+        return noc, keys
+
+    def set_particles(self, p: Particles, precision=L.FP64):
+        L.check(self.lib.amps_gpu_set_particles(
+            self.ctx.h, p.n, L._ptr(p.x, C.c_double), L._ptr(p.v, C.c_double),
+            L._ptr(p.spec, C.c_int32), L._ptr(p.blk, C.c_int32), L._ptr(p.cell, C.c_int32),
+            L._ptr(p.id, C.c_int64), precision), "set_particles")
+
+    def particle_count(self):
+        n = int(self.lib.amps_gpu_particle_count(self.ctx.h))
+        if n < 0:
+            L.check(L.AMPS_OK + 3, "particle_count")
+        return n
+
+    def get_particles(self):
+        n = self.particle_count()
+        x = np.zeros((3, n))
+        v = np.zeros((3, n))
+        spec = np.zeros(n, np.int32)
+        blk = np.zeros(n, np.int32)
+        cell = np.zeros(n, np.int32)
+        pid = np.zeros(n, np.int64)
+        code = np.zeros(n, np.int32)
+        L.check(self.lib.amps_gpu_get_particles(
+            self.ctx.h, L._ptr(x, C.c_double), L._ptr(v, C.c_double), L._ptr(spec, C.c_int32),
+            L._ptr(blk, C.c_int32), L._ptr(cell, C.c_int32), L._ptr(pid, C.c_int64),
+            L._ptr(code, C.c_int32)), "get_particles")
+        return Particles(x, v, spec, blk, cell, pid), code
+
+    def MoveParticles(self, mover=L.MOVER_LAPENTA, bc_mode=L.BC_REFLECT, r_sphere=0.0,
+                      compact=True, tile=0):
+        o = L.PushOpts(mover, bc_mode, 1 if compact else 0, tile, r_sphere)
+        L.check(self.lib.amps_gpu_move_particles(self.ctx.h, C.byref(o)), "move_particles")
+
+    def compact(self):
+        L.check(self.lib.amps_gpu_compact(self.ctx.h), "compact")
+
+    def UpdateJMassMatrix(self, dt, copy_out=True):
+        if not copy_out:
+            L.check(self.lib.amps_gpu_update_j_mass_matrix(self.ctx.h, dt, None, None), "deposit")
+            return None, None
+        J = np.zeros((self.nnodes, 3))
+        M = np.zeros((self.nnodes, 243))
+        L.check(self.lib.amps_gpu_update_j_mass_matrix(self.ctx.h, dt, L._ptr(J, C.c_double),
+                                                       L._ptr(M, C.c_double)), "deposit")
+        return J, M

@@ -1,0 +1,936 @@
+/*
+ * amps_oracle.c — CPU FP64 restatement of the AMPS particle hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY (see amps_oracle.h).  Built with
+ * -ffp-contract=off so that every expression rounds exactly as written,
+ * in the order the reference writes it.
+ */
+#include "amps_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+/* ===================================================================== */
+/* AMR octree: restates cTreeNodeAMR's search fields                      */
+/* (src/meshAMR/meshAMRgeneric.h:2793-2882).                              */
+/* ===================================================================== */
+typedef struct {
+  int64_t xmin_g[3]; /* xMinGlobalIndex */
+  int64_t size_g;    /* NodeGeometricSizeIndex */
+  int level;         /* RefinmentLevel */
+  double xmin[3], xmax[3];
+  int child[8];      /* downNode[i+2(j+2k)] */
+  int up;            /* upNode */
+  int block;         /* leaf block id, -1 for internal nodes */
+} orc_tnode;
+
+struct orc_mesh {
+  double gxmin[3], gxmax[3], dx_root[3], dx_maxref[3], eps;
+  int lmax, nblocks, nnodes, cap;
+  orc_tnode* t;
+  int* block_node;
+};
+
+static int new_tnode(orc_mesh* m, int up, const int64_t xg[3], int64_t size, int level) {
+  if (m->nnodes == m->cap) {
+    m->cap = m->cap ? 2 * m->cap : 1024;
+    m->t = (orc_tnode*)realloc(m->t, sizeof(orc_tnode) * (size_t)m->cap);
+  }
+  orc_tnode* n = &m->t[m->nnodes];
+  for (int d = 0; d < 3; d++) {
+    n->xmin_g[d] = xg[d];
+    /* Block bounds on the max-refinement lattice; exact for the dyadic domains
+     * used here, as any splitting rule of the reference is. */
+    n->xmin[d] = m->gxmin[d] + (double)xg[d] * m->dx_maxref[d];
+    n->xmax[d] = m->gxmin[d] + (double)(xg[d] + size) * m->dx_maxref[d];
+  }
+  n->size_g = size;
+  n->level = level;
+  for (int c = 0; c < 8; c++) n->child[c] = -1;
+  n->up = up;
+  n->block = -1;
+  return m->nnodes++;
+}
+
+orc_mesh* orc_mesh_create(const double gxmin[3], const double gxmax[3], int lmax, int nblocks,
+                          const int* level, const int64_t* ijk) {
+  orc_mesh* m = (orc_mesh*)calloc(1, sizeof(orc_mesh));
+  m->lmax = lmax;
+  m->nblocks = nblocks;
+  /* meshAMRgeneric.h:2338-2366 (init): dxRootBlock, EPS, dx_max_refinment */
+  for (int d = 0; d < 3; d++) {
+    m->gxmin[d] = gxmin[d];
+    m->gxmax[d] = gxmax[d];
+    m->dx_root[d] = gxmax[d] - gxmin[d];
+    m->dx_maxref[d] = (gxmax[d] - gxmin[d]) / (1 << lmax);
+  }
+  m->eps = 0.0001 * m->dx_root[0] / (double)ORC_NC / (1 << lmax);
+  for (int d = 1; d < 3; d++) {
+    double e = 0.0001 * m->dx_root[d] / (double)ORC_NC / (1 << lmax);
+    if (m->eps > e) m->eps = e;
+  }
+  int64_t zero[3] = {0, 0, 0};
+  new_tnode(m, -1, zero, (int64_t)1 << lmax, 0);
+  m->block_node = (int*)malloc(sizeof(int) * (size_t)nblocks);
+  for (int b = 0; b < nblocks; b++) {
+    int L = level[b];
+    if (L < 0 || L > lmax - 3) goto fail;
+    int64_t tg[3];
+    for (int d = 0; d < 3; d++) tg[d] = ijk[3 * b + d] << (lmax - L);
+    int cur = 0;
+    while (m->t[cur].level < L) {
+      if (m->t[cur].block >= 0) goto fail; /* leaf cannot be refined */
+      int64_t half = m->t[cur].size_g / 2;
+      int c = 0;
+      int64_t cg[3];
+      for (int d = 0; d < 3; d++) {
+        int bit = (tg[d] - m->t[cur].xmin_g[d]) >= half;
+        c |= bit << d;
+        cg[d] = m->t[cur].xmin_g[d] + (bit ? half : 0);
+      }
+      if (m->t[cur].child[c] < 0) {
+        int nn = new_tnode(m, cur, cg, half, m->t[cur].level + 1);
+        m->t[cur].child[c] = nn;
+      }
+      cur = m->t[cur].child[c];
+    }
+    if (m->t[cur].block >= 0) goto fail;
+    for (int c = 0; c < 8; c++)
+      if (m->t[cur].child[c] >= 0) goto fail;
+    m->t[cur].block = b;
+    m->block_node[b] = cur;
+  }
+  /* every internal node must have all eight children (the leaves tile the root) */
+  for (int i = 0; i < m->nnodes; i++) {
+    if (m->t[i].block >= 0) continue;
+    for (int c = 0; c < 8; c++)
+      if (m->t[i].child[c] < 0) goto fail;
+  }
+  return m;
+fail:
+  orc_mesh_destroy(m);
+  return NULL;
+}
+
+void orc_mesh_destroy(orc_mesh* m) {
+  if (!m) return;
+  free(m->t);
+  free(m->block_node);
+  free(m);
+}
+
+double orc_mesh_eps(const orc_mesh* m) { return m->eps; }
+
+void orc_block_bounds(const orc_mesh* m, int b, double xmin[3], double xmax[3]) {
+  const orc_tnode* n = &m->t[m->block_node[b]];
+  for (int d = 0; d < 3; d++) {
+    xmin[d] = n->xmin[d];
+    xmax[d] = n->xmax[d];
+  }
+}
+
+/* findTreeNode(int*) — meshAMRgeneric.h:2793-2848 */
+static int find_tnode_int(const orc_mesh* m, const int ix[3], int start) {
+  int cur = start;
+  for (;;) {
+    const orc_tnode* n = &m->t[cur];
+    int inblock = 1;
+    for (int d = 0; d < 3; d++) {
+      if ((ix[d] < n->xmin_g[d]) || (ix[d] >= n->xmin_g[d] + n->size_g)) {
+        inblock = 0;
+        break;
+      }
+    }
+    if (inblock) {
+      int i = (ix[0] - n->xmin_g[0] < n->size_g / 2) ? 0 : 1;
+      int j = (ix[1] - n->xmin_g[1] < n->size_g / 2) ? 0 : 1;
+      int k = (ix[2] - n->xmin_g[2] < n->size_g / 2) ? 0 : 1;
+      int t = n->child[i + 2 * (j + 2 * k)];
+      if (t >= 0) {
+        cur = t;
+        continue;
+      }
+      return cur;
+    }
+    if (n->up >= 0) {
+      cur = n->up;
+      continue;
+    }
+    return -1;
+  }
+}
+
+/* findTreeNode(double*) — meshAMRgeneric.h:2852-2882 */
+static int find_tnode(const orc_mesh* m, const double x[3], int start) {
+  int ix[3];
+  for (int d = 0; d < 3; d++) ix[d] = (int)floor((x[d] - m->gxmin[d]) / m->dx_maxref[d]);
+  int res = find_tnode_int(m, ix, start);
+  int flag = 0;
+  if (res >= 0) {
+    for (int d = 0; d < 3; d++) {
+      if (x[d] < m->t[res].xmin[d]) ix[d]--, flag = 1;
+      if (x[d] >= m->t[res].xmax[d]) ix[d]++, flag = 1;
+    }
+  }
+  if (flag) res = find_tnode_int(m, ix, res);
+  return res;
+}
+
+int orc_find_block(const orc_mesh* m, const double x[3], int start_block) {
+  int start = (start_block >= 0) ? m->block_node[start_block] : 0;
+  int r = find_tnode(m, x, start);
+  return (r >= 0) ? m->t[r].block : -1;
+}
+
+/* FindCellIndex — meshAMRgeneric.h:2256-2323 (ExitFlag=false) */
+static int find_cell_node(const orc_mesh* m, const double x[3], const orc_tnode* n, int ijk[3]) {
+  for (int d = 0; d < 3; d++) {
+    if ((x[d] < n->xmin[d]) || (n->xmax[d] < x[d])) return -1;
+    double dx = m->dx_root[d] / (1 << n->level) / (double)ORC_NC;
+    int i = (int)((x[d] - n->xmin[d]) / dx);
+    if (i == ORC_NC) i = ORC_NC - 1;
+    ijk[d] = i;
+  }
+  return ijk[0] + ORC_NC * (ijk[1] + ORC_NC * ijk[2]);
+}
+
+int orc_find_cell(const orc_mesh* m, const double x[3], int block) {
+  int ijk[3];
+  return find_cell_node(m, x, &m->t[m->block_node[block]], ijk);
+}
+
+/* CornerBased::InitStencil — pic_interpolation_routines.cpp:1074-1194.
+ * Stencil loop order: i outer, k inner (l = 4i+2j+k).  table[] follows the
+ * reference's InterpolationCoefficientTable ordering 0,1,3,2,4,5,7,6.
+ * Normalize(): pic.h:7255-7262. */
+static void corner_stencil_node(const orc_tnode* n, double x[3], double w_norm[8],
+                                double table[8], int corner[8]) {
+  double dx[3], xLoc[3];
+  int iX[3];
+  for (int d = 0; d < 3; d++) dx[d] = (n->xmax[d] - n->xmin[d]) / ORC_NC;
+  for (int d = 0; d < 3; d++) {
+    if (fabs(x[d] - n->xmax[d]) < 1e-10 * dx[d]) x[d] = n->xmax[d] - 1e-10 * dx[d];
+    xLoc[d] = (x[d] - n->xmin[d]) / dx[d];
+    iX[d] = (int)(xLoc[d]);
+    xLoc[d] -= iX[d];
+  }
+  static const int tslot[8] = {0, 4, 3, 7, 1, 5, 2, 6}; /* l = 4i+2j+k -> table index */
+  double norm = 0.0;
+  for (int i = 0; i < 2; i++)
+    for (int j = 0; j < 2; j++)
+      for (int k = 0; k < 2; k++) {
+        int l = 4 * i + 2 * j + k;
+        double a = i ? xLoc[0] : (1.0 - xLoc[0]);
+        double b = j ? xLoc[1] : (1.0 - xLoc[1]);
+        double c = k ? xLoc[2] : (1.0 - xLoc[2]);
+        double w = a * b * c;
+        table[tslot[l]] = w;
+        w_norm[l] = w;
+        corner[l] = (iX[0] + i) + ORC_NN * ((iX[1] + j) + ORC_NN * (iX[2] + k));
+        norm += w;
+      }
+  if (norm > 0.0)
+    for (int l = 0; l < 8; l++) w_norm[l] /= norm;
+}
+
+void orc_corner_stencil(const orc_mesh* m, double x[3], int block, double w_norm[8],
+                        double table[8], int corner[8]) {
+  corner_stencil_node(&m->t[m->block_node[block]], x, w_norm, table, corner);
+}
+
+/* ===================================================================== */
+/* Mover                                                                  */
+/* ===================================================================== */
+typedef struct {
+  double norm[3], e0[3], e1[3], x0[3], lE0, lE1;
+} orc_face;
+
+/* ExternalBoundaryFaceTable + Init — pic_mover.cpp:24-28, 60-74 */
+static void build_faces(const orc_mesh* m, orc_face f[6]) {
+  static const double nrm[6][3] = {{-1, 0, 0}, {1, 0, 0}, {0, -1, 0}, {0, 1, 0}, {0, 0, -1}, {0, 0, 1}};
+  static const int nX0[6][3] = {{0, 0, 0}, {1, 0, 0}, {0, 0, 0}, {0, 1, 0}, {0, 0, 0}, {0, 0, 1}};
+  static const double e0[6][3] = {{0, 1, 0}, {0, 1, 0}, {1, 0, 0}, {1, 0, 0}, {1, 0, 0}, {1, 0, 0}};
+  static const double e1[6][3] = {{0, 0, 1}, {0, 0, 1}, {0, 0, 1}, {0, 0, 1}, {0, 1, 0}, {0, 1, 0}};
+  for (int nf = 0; nf < 6; nf++) {
+    double cE0 = 0.0, cE1 = 0.0;
+    for (int d = 0; d < 3; d++) {
+      f[nf].norm[d] = nrm[nf][d];
+      f[nf].e0[d] = e0[nf][d];
+      f[nf].e1[d] = e1[nf][d];
+      f[nf].x0[d] = (nX0[nf][d] == 0) ? m->gxmin[d] : m->gxmax[d];
+      double a = ((e0[nf][d] + nX0[nf][d] < 0.5) ? m->gxmin[d] : m->gxmax[d]) - f[nf].x0[d];
+      double b = ((e1[nf][d] + nX0[nf][d] < 0.5) ? m->gxmin[d] : m->gxmax[d]) - f[nf].x0[d];
+      cE0 += pow(a, 2);
+      cE1 += pow(b, 2);
+    }
+    f[nf].lE0 = sqrt(cE0);
+    f[nf].lE1 = sqrt(cE1);
+  }
+}
+
+/* One particle: PIC::Mover::Lapenta2017 (pic_mover_boris.cpp:876-1380) or
+ * PIC::Mover::Boris (pic_mover_boris.cpp:126-553, rotation at :253-277), with
+ * corner E and B.  Returns the code; x/v/blk/cell updated when committed. */
+static int push_one(const orc_mesh* m, const orc_face* faces, const double* E_corner,
+                    const double* B_corner, double q, double mass, double dtTotal, int mover,
+                    int bc_mode, double r_sphere, double xp[3], double vp[3], int* blk, int* cell) {
+  double vInit[3] = {vp[0], vp[1], vp[2]}, xInit[3] = {xp[0], xp[1], xp[2]};
+  double vFinal[3], xFinal[3];
+  int startNode = m->block_node[*blk];
+  const orc_tnode* sn = &m->t[startNode];
+
+  /* stencil + field accumulation (:945-977) */
+  double w[8], table[8];
+  int cr[8];
+  corner_stencil_node(sn, xInit, w, table, cr);
+  double E[3] = {0.0, 0.0, 0.0}, B[3] = {0.0, 0.0, 0.0};
+  const double* Eb = E_corner + (size_t)(*blk) * ORC_NCORNER * 3;
+  const double* Bb = B_corner + (size_t)(*blk) * ORC_NCORNER * 3;
+  for (int l = 0; l < 8; l++) {
+    const double* e = Eb + 3 * cr[l];
+    const double* b = Bb + 3 * cr[l];
+    for (int d = 0; d < 3; d++) {
+      E[d] += w[l] * e[d];
+      B[d] += w[l] * b[d];
+    }
+  }
+
+  if (mover == ORC_MOVER_LAPENTA) {
+    /* :1033-1081 */
+    double QdT_over_m = q * dtTotal / mass;
+    double QdT_over_2m = 0.5 * QdT_over_m;
+    double QdT_over_2m_squared = QdT_over_2m * QdT_over_2m;
+    double BB[3][3], P[3], alpha[3][3];
+    for (int i = 0; i < 3; i++) {
+      P[i] = -QdT_over_2m * B[i];
+      for (int j = 0; j <= i; j++) {
+        BB[i][j] = QdT_over_2m_squared * B[i] * B[j];
+        BB[j][i] = BB[i][j];
+      }
+    }
+    double c0 = 1.0 / (1.0 + QdT_over_2m_squared * (B[0] * B[0] + B[1] * B[1] + B[2] * B[2]));
+    alpha[0][0] = c0 * (1.0 + BB[0][0]);
+    alpha[0][1] = c0 * (-P[2] + BB[0][1]);
+    alpha[0][2] = c0 * (P[1] + BB[0][2]);
+    alpha[1][0] = c0 * (P[2] + BB[1][0]);
+    alpha[1][1] = c0 * (1.0 + BB[1][1]);
+    alpha[1][2] = c0 * (-P[0] + BB[1][2]);
+    alpha[2][0] = c0 * (-P[1] + BB[2][0]);
+    alpha[2][1] = c0 * (P[0] + BB[2][1]);
+    alpha[2][2] = c0 * (1.0 + BB[2][2]);
+    for (int i = 0; i < 3; i++) {
+      double s = 0.0;
+      for (int j = 0; j < 3; j++) s += alpha[i][j] * (vInit[j] + QdT_over_2m * E[j]);
+      vFinal[i] = 2.0 * s - vInit[i];
+    }
+    for (int i = 0; i < 3; i++) xFinal[i] = xInit[i] + dtTotal * vFinal[i];
+  } else {
+    /* BorisSplitAcceleration_default :87-98 and the rotation :253-277 */
+    double Charge2Mass = q / mass, accl[3], rot[3], u[3], h[3], U[3];
+    for (int d = 0; d < 3; d++) {
+      accl[d] = 0.0 + Charge2Mass * E[d];
+      rot[d] = 0.0 - Charge2Mass * B[d];
+    }
+    double dtTemp = dtTotal, dtTempOverTwo = dtTotal / 2.0;
+    for (int d = 0; d < 3; d++) u[d] = vInit[d] + dtTempOverTwo * accl[d];
+    for (int d = 0; d < 3; d++) h[d] = -dtTempOverTwo * rot[d];
+    double h2 = h[0] * h[0] + h[1] * h[1] + h[2] * h[2];
+    double uh = u[0] * h[0] + u[1] * h[1] + u[2] * h[2];
+    U[0] = ((1 - h2) * u[0] + 2 * (u[1] * h[2] - h[1] * u[2] + uh * h[0])) / (1 + h2);
+    U[1] = ((1 - h2) * u[1] + 2 * (u[2] * h[0] - h[2] * u[0] + uh * h[1])) / (1 + h2);
+    U[2] = ((1 - h2) * u[2] + 2 * (u[0] * h[1] - h[0] * u[1] + uh * h[2])) / (1 + h2);
+    for (int d = 0; d < 3; d++) vFinal[d] = U[d] + dtTempOverTwo * accl[d];
+    for (int d = 0; d < 3; d++) xFinal[d] = xInit[d] + dtTemp * vFinal[d];
+  }
+
+  /* internal sphere (:1088-1117): absorb */
+  int newNode;
+  double rFinal2;
+  if (r_sphere > 0.0 &&
+      (rFinal2 = xFinal[0] * xFinal[0] + xFinal[1] * xFinal[1] + xFinal[2] * xFinal[2]) <
+          r_sphere * r_sphere) {
+    double r = sqrt(rFinal2);
+    for (int d = 0; d < 3; d++) xFinal[d] *= r_sphere / r;
+    for (int d = 0; d < 3; d++) xp[d] = xFinal[d], vp[d] = vFinal[d];
+    return ORC_CODE_SPHERE;
+  }
+  newNode = find_tnode(m, xFinal, startNode);
+
+  int code = ORC_CODE_FINISHED;
+  if (newNode < 0) {
+    if (bc_mode == ORC_BC_DELETE) {
+      for (int d = 0; d < 3; d++) xp[d] = xFinal[d], vp[d] = vFinal[d];
+      return ORC_CODE_LEFT_DOMAIN;
+    }
+    /* exit-face search and move to the face (:1156-1229); the reference reads
+     * nIntersectionFace uninitialised, here it starts at -1 */
+    int nIntersectionFace = -1;
+    double vMiddle[3] = {0.5 * (vInit[0] + vFinal[0]), 0.5 * (vInit[1] + vFinal[1]),
+                         0.5 * (vInit[2] + vFinal[2])};
+    double dtIntersection = -1.0, r0[3], c;
+    for (int nface = 0; nface < 6; nface++) {
+      double cx = 0.0, cv = 0.0;
+      for (int d = 0; d < 3; d++) {
+        r0[d] = xInit[d] - faces[nface].x0[d];
+        cx += r0[d] * faces[nface].norm[d];
+        cv += vMiddle[d] * faces[nface].norm[d];
+      }
+      if (cv > 0.0) {
+        double dt = -cx / cv;
+        if ((dtIntersection < 0.0) || ((dt < dtIntersection) && (dt > 0.0))) {
+          double cE0 = 0.0, cE1 = 0.0;
+          for (int d = 0; d < 3; d++) {
+            c = r0[d] + dt * vMiddle[d];
+            cE0 += c * faces[nface].e0[d], cE1 += c * faces[nface].e1[d];
+          }
+          if ((cE0 < -m->eps) || (cE0 > faces[nface].lE0 + m->eps) || (cE1 < -m->eps) ||
+              (cE1 > faces[nface].lE1 + m->eps))
+            continue;
+          nIntersectionFace = nface, dtIntersection = dt;
+        }
+      }
+    }
+    if (nIntersectionFace == -1) {
+      for (int d = 0; d < 3; d++) xp[d] = xFinal[d], vp[d] = vFinal[d];
+      return ORC_CODE_LEFT_DOMAIN; /* no face found: deleted */
+    }
+    double tVelocityIncrement = ((dtIntersection / dtTotal < 1) ? dtIntersection / dtTotal : 1);
+    for (int d = 0; d < 3; d++) {
+      xInit[d] += dtIntersection * vMiddle[d] - faces[nIntersectionFace].norm[d] * m->eps;
+      vInit[d] += tVelocityIncrement * (vFinal[d] - vInit[d]);
+    }
+    newNode = find_tnode(m, xInit, startNode);
+    if (newNode < 0) {
+      for (int d = 0; d < 3; d++) {
+        if (m->gxmin[d] >= xInit[d]) xInit[d] = m->gxmin[d] + m->eps;
+        if (m->gxmax[d] <= xInit[d]) xInit[d] = m->gxmax[d] - m->eps;
+      }
+      newNode = find_tnode(m, xInit, startNode);
+      if (newNode < 0) {
+        for (int d = 0; d < 3; d++) xp[d] = xInit[d], vp[d] = vInit[d];
+        return ORC_CODE_LEFT_DOMAIN;
+      }
+    }
+    /* specular reflection (:1236-1253); the reference then aborts in its
+     * switch(code) — here the reflected particle is committed. */
+    {
+      double cc = 0.0;
+      for (int d = 0; d < 3; d++) cc += faces[nIntersectionFace].norm[d] * vInit[d];
+      for (int d = 0; d < 3; d++) vInit[d] -= 2.0 * cc * faces[nIntersectionFace].norm[d];
+    }
+    for (int d = 0; d < 3; d++) vFinal[d] = vInit[d], xFinal[d] = xInit[d];
+    code = ORC_CODE_REFLECTED;
+  }
+
+  /* FindCellIndex (:1286) and commit (:1374-1375) */
+  int ijk[3];
+  int c_id = find_cell_node(m, xFinal, &m->t[newNode], ijk);
+  if (c_id < 0) return ORC_CODE_LEFT_DOMAIN;
+  *blk = m->t[newNode].block;
+  *cell = c_id;
+  for (int d = 0; d < 3; d++) xp[d] = xFinal[d], vp[d] = vFinal[d];
+  return code;
+}
+
+int orc_push(const orc_mesh* m, const double* E_corner, const double* B_corner,
+             const double* charge, const double* mass, const double* dt_spec, int mover,
+             int bc_mode, double r_sphere, int64_t n, double* x, double* v, const int* spec,
+             int* blk, int* cell, int* code, int nthreads) {
+  orc_face faces[6];
+  build_faces(m, faces);
+#ifdef _OPENMP
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#pragma omp parallel for schedule(static)
+#endif
+  for (int64_t p = 0; p < n; p++) {
+    double xp[3] = {x[p], x[n + p], x[2 * n + p]};
+    double vp[3] = {v[p], v[n + p], v[2 * n + p]};
+    int s = spec[p];
+    code[p] = push_one(m, faces, E_corner, B_corner, charge[s], mass[s], dt_spec[s], mover,
+                       bc_mode, r_sphere, xp, vp, &blk[p], &cell[p]);
+    for (int d = 0; d < 3; d++) {
+      x[d * n + p] = xp[d];
+      v[d * n + p] = vp[d];
+    }
+  }
+  return 0;
+}
+
+/* ===================================================================== */
+/* ECSIM deposition                                                       */
+/* ===================================================================== */
+static int cmp_i64(const void* a, const void* b) {
+  int64_t x = *(const int64_t*)a, y = *(const int64_t*)b;
+  return (x > y) - (x < y);
+}
+
+/* node lattice key of local corner (i,j,k) of block node n */
+static int64_t corner_key(const orc_mesh* m, const orc_tnode* n, int i, int j, int k) {
+  int64_t S = ((int64_t)1 << m->lmax) + 1;
+  int64_t cs = n->size_g / ORC_NC;
+  int64_t gx = n->xmin_g[0] + i * cs, gy = n->xmin_g[1] + j * cs, gz = n->xmin_g[2] + k * cs;
+  return (gz * S + gy) * S + gx;
+}
+
+int64_t orc_build_nodes(const orc_mesh* m, int* node_of_corner, int64_t* keys_out) {
+  int64_t total = (int64_t)m->nblocks * ORC_NCORNER;
+  int64_t* keys = (int64_t*)malloc(sizeof(int64_t) * (size_t)total);
+  for (int b = 0; b < m->nblocks; b++) {
+    const orc_tnode* n = &m->t[m->block_node[b]];
+    for (int k = 0; k < ORC_NN; k++)
+      for (int j = 0; j < ORC_NN; j++)
+        for (int i = 0; i < ORC_NN; i++)
+          keys[(int64_t)b * ORC_NCORNER + i + ORC_NN * (j + ORC_NN * k)] = corner_key(m, n, i, j, k);
+  }
+  int64_t* sorted = (int64_t*)malloc(sizeof(int64_t) * (size_t)total);
+  memcpy(sorted, keys, sizeof(int64_t) * (size_t)total);
+  qsort(sorted, (size_t)total, sizeof(int64_t), cmp_i64);
+  int64_t nu = 0;
+  for (int64_t i = 0; i < total; i++)
+    if (i == 0 || sorted[i] != sorted[i - 1]) sorted[nu++] = sorted[i];
+  for (int64_t i = 0; i < total; i++) {
+    int64_t lo = 0, hi = nu - 1;
+    while (lo < hi) {
+      int64_t mid = (lo + hi) / 2;
+      if (sorted[mid] < keys[i]) lo = mid + 1;
+      else hi = mid;
+    }
+    node_of_corner[i] = (int)lo;
+  }
+  if (keys_out) memcpy(keys_out, sorted, sizeof(int64_t) * (size_t)nu);
+  free(keys);
+  free(sorted);
+  return nu;
+}
+
+/* IndexMatrix — pic_field_solver_ecsim.cpp:1377-1379 */
+static const int IndexMatrix[8][8] = {{0, 2, 8, 6, 18, 20, 26, 24}, {1, 0, 6, 7, 19, 18, 24, 25},
+                                      {4, 3, 0, 1, 22, 21, 18, 19}, {3, 5, 2, 0, 21, 23, 20, 18},
+                                      {9, 11, 17, 15, 0, 2, 8, 6},  {10, 9, 15, 16, 1, 0, 6, 7},
+                                      {13, 12, 9, 10, 4, 3, 0, 1},  {12, 14, 11, 9, 3, 5, 2, 0}};
+/* cell corner c (ProcessCell offset[] order, :2033-2040) -> (di,dj,dk) */
+static const int CornerOff[8][3] = {{0, 0, 0}, {1, 0, 0}, {1, 1, 0}, {0, 1, 0},
+                                    {0, 0, 1}, {1, 0, 1}, {1, 1, 1}, {0, 1, 1}};
+
+#define CELL_VALS (24 + 36 * 9)
+
+/* Per-cell accumulation: ProcessCell scalar path, pic_field_solver_ecsim.cpp:1882-2438
+ * (normalized units: B_conv = length_conv = LightSpeed = 1, :193-206). */
+static void process_cell(const orc_tnode* n, const double* Bb, const double* charge,
+                         const double* mass, const double* weight, double dtTotal,
+                         int64_t np, const int64_t* idx, int64_t ntot, const double* x,
+                         const double* v, const int* spec, double* out) {
+  double Jg[8][3], MM[8][8][9];
+  memset(Jg, 0, sizeof(Jg));
+  memset(MM, 0, sizeof(MM));
+  double dxc[3], CellVolume = 1;
+  for (int d = 0; d < 3; d++) dxc[d] = (n->xmax[d] - n->xmin[d]) / ORC_NC * 1.0;
+  for (int d = 0; d < 3; d++) CellVolume *= dxc[d];
+  for (int64_t q = 0; q < np; q++) {
+    int64_t p = idx[q];
+    double xInit[3] = {x[p], x[ntot + p], x[2 * ntot + p]};
+    double vInit[3] = {v[p], v[ntot + p], v[2 * ntot + p]};
+    int s = spec[p];
+    double w[8], table[8], B[3] = {0.0, 0.0, 0.0};
+    int cr[8];
+    corner_stencil_node(n, xInit, w, table, cr);
+    for (int l = 0; l < 8; l++)
+      for (int d = 0; d < 3; d++) B[d] += w[l] * Bb[3 * cr[l] + d];
+    double chargeQ = charge[s], m_ = mass[s];
+    chargeQ *= weight[s];
+    m_ *= weight[s];
+    double QdT_over_m = chargeQ * dtTotal / m_;
+    double QdT_over_2m = 0.5 * QdT_over_m;
+    double QdT_over_2m_squared = QdT_over_2m * QdT_over_2m;
+    double BB[3][3], P[3], alpha[9];
+    for (int ii = 0; ii < 3; ii++) {
+      P[ii] = -QdT_over_2m * B[ii];
+      for (int jj = 0; jj <= ii; jj++) {
+        BB[ii][jj] = QdT_over_2m_squared * B[ii] * B[jj];
+        BB[jj][ii] = BB[ii][jj];
+      }
+    }
+    double c0 = 1.0 / (1.0 + QdT_over_2m_squared * (B[0] * B[0] + B[1] * B[1] + B[2] * B[2]));
+    alpha[0] = c0 * (1.0 + BB[0][0]);
+    alpha[1] = c0 * (-P[2] + BB[0][1]);
+    alpha[2] = c0 * (P[1] + BB[0][2]);
+    alpha[3] = c0 * (P[2] + BB[1][0]);
+    alpha[4] = c0 * (1.0 + BB[1][1]);
+    alpha[5] = c0 * (-P[0] + BB[1][2]);
+    alpha[6] = c0 * (-P[1] + BB[2][0]);
+    alpha[7] = c0 * (P[0] + BB[2][1]);
+    alpha[8] = c0 * (1.0 + BB[2][2]);
+    /* the deposit weights are the raw table from a second InitStencil call */
+    corner_stencil_node(n, xInit, w, table, cr);
+    double vRot[3] = {0.0, 0.0, 0.0};
+    for (int i = 0; i < 3; i++)
+      for (int jj = 0; jj < 3; jj++) vRot[i] += alpha[3 * i + jj] * vInit[jj];
+    for (int c = 0; c < 8; c++) {
+      double t = chargeQ * table[c];
+      for (int d = 0; d < 3; d++) Jg[c][d] += t * vRot[d];
+    }
+    double matrixConst = chargeQ * QdT_over_2m / CellVolume;
+    for (int i = 0; i < 8; i++) {
+      double tempWeightConst = matrixConst * table[i];
+      for (int j = 0; j <= i; j++) {
+        double tempWeightProduct = table[j] * tempWeightConst;
+        for (int k = 0; k < 9; k++) MM[i][j][k] += alpha[k] * tempWeightProduct;
+      }
+    }
+  }
+  /* out: 8x3 J already divided by the volume, then 36 pairs (i>=j) x 9 */
+  for (int c = 0; c < 8; c++)
+    for (int d = 0; d < 3; d++) out[3 * c + d] = Jg[c][d] / CellVolume;
+  int pr = 0;
+  for (int i = 0; i < 8; i++)
+    for (int j = 0; j <= i; j++, pr++)
+      for (int k = 0; k < 9; k++) out[24 + 9 * pr + k] = MM[i][j][k];
+}
+
+int orc_deposit(const orc_mesh* m, const double* B_corner, const double* charge,
+                const double* mass, const double* weight, double dt, int64_t n, const double* x,
+                const double* v, const int* spec, const int* blk, const int* cell,
+                const int* node_of_corner, int64_t nnodes, double* J, double* M, int nthreads) {
+  int64_t ncells = (int64_t)m->nblocks * ORC_NCELL;
+  /* stable counting sort of particles by (block, cell) */
+  int64_t* start = (int64_t*)calloc((size_t)ncells + 1, sizeof(int64_t));
+  int64_t* idx = (int64_t*)malloc(sizeof(int64_t) * (size_t)(n > 0 ? n : 1));
+  for (int64_t p = 0; p < n; p++) start[(int64_t)blk[p] * ORC_NCELL + cell[p] + 1]++;
+  for (int64_t c = 0; c < ncells; c++) start[c + 1] += start[c];
+  int64_t* fill = (int64_t*)malloc(sizeof(int64_t) * (size_t)ncells);
+  memcpy(fill, start, sizeof(int64_t) * (size_t)ncells);
+  for (int64_t p = 0; p < n; p++) idx[fill[(int64_t)blk[p] * ORC_NCELL + cell[p]]++] = p;
+  free(fill);
+  memset(J, 0, sizeof(double) * (size_t)nnodes * 3);
+  memset(M, 0, sizeof(double) * (size_t)nnodes * 243);
+#ifdef _OPENMP
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#endif
+  const int64_t CH = 16384;
+  double* buf = (double*)malloc(sizeof(double) * (size_t)CH * CELL_VALS);
+  for (int64_t c0 = 0; c0 < ncells; c0 += CH) {
+    int64_t c1 = (c0 + CH < ncells) ? c0 + CH : ncells;
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic, 64)
+#endif
+    for (int64_t c = c0; c < c1; c++) {
+      int64_t np = start[c + 1] - start[c];
+      if (np == 0) continue;
+      int b = (int)(c / ORC_NCELL);
+      const orc_tnode* nd = &m->t[m->block_node[b]];
+      process_cell(nd, B_corner + (size_t)b * ORC_NCORNER * 3, charge, mass, weight, dt, np,
+                   idx + start[c], n, x, v, spec, buf + (c - c0) * CELL_VALS);
+    }
+    /* corner flush in cell order (:2362-2369, :2398-2426) */
+    for (int64_t c = c0; c < c1; c++) {
+      if (start[c + 1] == start[c]) continue;
+      const double* o = buf + (c - c0) * CELL_VALS;
+      int b = (int)(c / ORC_NCELL), lc = (int)(c % ORC_NCELL);
+      int ci = lc % ORC_NC, cj = (lc / ORC_NC) % ORC_NC, ck = lc / (ORC_NC * ORC_NC);
+      int nodeid[8];
+      for (int q = 0; q < 8; q++) {
+        int li = ci + CornerOff[q][0], lj = cj + CornerOff[q][1], lk = ck + CornerOff[q][2];
+        nodeid[q] = node_of_corner[(int64_t)b * ORC_NCORNER + li + ORC_NN * (lj + ORC_NN * lk)];
+      }
+      for (int q = 0; q < 8; q++)
+        for (int d = 0; d < 3; d++) J[(int64_t)nodeid[q] * 3 + d] += o[3 * q + d];
+      int pr = 0;
+      for (int i = 0; i < 8; i++)
+        for (int j = 0; j <= i; j++, pr++) {
+          const double* mm = o + 24 + 9 * pr;
+          if (i == j) {
+            double* cm = M + (int64_t)nodeid[i] * 243;
+            for (int k = 0; k < 9; k++) cm[k] += mm[k];
+          } else {
+            double* mi = M + (int64_t)nodeid[i] * 243 + 9 * IndexMatrix[i][j];
+            double* mj = M + (int64_t)nodeid[j] * 243 + 9 * IndexMatrix[j][i];
+            for (int k = 0; k < 9; k++) {
+              mi[k] += mm[k];
+              mj[k] += mm[k];
+            }
+          }
+        }
+    }
+  }
+  free(buf);
+  free(start);
+  free(idx);
+  return 0;
+}
+
+/* ===================================================================== */
+/* Gridless cutoff tracer (SI)                                            */
+/* ===================================================================== */
+#define ORC_C 299792458.0 /* SpeedOfLight, constants.h:40 */
+
+static inline double dmin(double a, double b) { return (b < a) ? b : a; } /* std::min */
+static inline double dmax(double a, double b) { return (a < b) ? b : a; } /* std::max */
+static inline double dot3(const double* a, const double* b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+
+/* Dipole::GetB_Tesla — DipoleInterface.h:99-125, plus a uniform field */
+void orc_dipole_B(const orc_trace_params* P, const double xm[3], double B[3]) {
+  const double mu0_over_4pi = 1.0e-7;
+  double x = xm[0], y = xm[1], z = xm[2];
+  double r2 = x * x + y * y + z * z;
+  if (r2 <= 0.0) {
+    B[0] = B[1] = B[2] = 0.0;
+  } else {
+    double r = sqrt(r2), r3 = r2 * r, r5 = r3 * r2;
+    double M = P->moment_Am2;
+    double mx = M * P->m_hat[0], my = M * P->m_hat[1], mz = M * P->m_hat[2];
+    double mdotr = mx * x + my * y + mz * z;
+    B[0] = mu0_over_4pi * (3.0 * x * mdotr / r5 - mx / r3);
+    B[1] = mu0_over_4pi * (3.0 * y * mdotr / r5 - my / r3);
+    B[2] = mu0_over_4pi * (3.0 * z * mdotr / r5 - mz / r3);
+  }
+  B[0] = B[0] + P->B_ext_T[0];
+  B[1] = B[1] + P->B_ext_T[1];
+  B[2] = B[2] + P->B_ext_T[2];
+}
+
+/* GammaFromMomentum — GridlessParticleMovers.cpp:111-115 */
+static inline double gamma_p(const double p[3], double m0) {
+  double p2 = dot3(p, p);
+  double mc = m0 * ORC_C;
+  return sqrt(1.0 + p2 / (mc * mc));
+}
+
+/* BorisStep — GridlessParticleMovers.cpp:781-801 */
+void orc_boris_step(const orc_trace_params* P, double x[3], double p[3], double dt) {
+  double B[3];
+  orc_dipole_B(P, x, B);
+  double m0 = P->m0_kg, q = P->q_C;
+  double gam = gamma_p(p, m0);
+  double iv = 1.0 / (gam * m0);
+  double vv[3] = {iv * p[0], iv * p[1], iv * p[2]};
+  double h = 0.5 * dt;
+  for (int d = 0; d < 3; d++) x[d] = x[d] + h * vv[d];
+  double tc = (q * dt) / (2.0 * gam * m0);
+  double t[3] = {tc * B[0], tc * B[1], tc * B[2]};
+  double t2 = dot3(t, t);
+  double sc = 2.0 / (1.0 + t2);
+  double s[3] = {sc * t[0], sc * t[1], sc * t[2]};
+  double pm[3] = {p[0], p[1], p[2]};
+  double pp[3] = {pm[0] + (pm[1] * t[2] - pm[2] * t[1]), pm[1] + (pm[2] * t[0] - pm[0] * t[2]),
+                  pm[2] + (pm[0] * t[1] - pm[1] * t[0])};
+  p[0] = pm[0] + (pp[1] * s[2] - pp[2] * s[1]);
+  p[1] = pm[1] + (pp[2] * s[0] - pp[0] * s[2]);
+  p[2] = pm[2] + (pp[0] * s[1] - pp[1] * s[0]);
+  double gam2 = gamma_p(p, m0);
+  double iv2 = 1.0 / (gam2 * m0);
+  double v2[3] = {iv2 * p[0], iv2 * p[1], iv2 * p[2]};
+  for (int d = 0; d < 3; d++) x[d] = x[d] + h * v2[d];
+}
+
+/* InsideInnerSphereM / SegmentHitsInnerSphere — GridlessParticleMovers.cpp:132-149 */
+static inline int inside_inner_m(const double x[3], double r) { return (r > 0.0) && (dot3(x, x) <= r * r); }
+static int segment_hits_inner(const double a[3], const double b[3], double r) {
+  if (r <= 0.0) return 0;
+  if (inside_inner_m(a, r) || inside_inner_m(b, r)) return 1;
+  double d[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]};
+  double dd = dot3(d, d);
+  if (dd <= 0.0) return 0;
+  double t = -dot3(a, d) / dd;
+  if (t < 0.0) t = 0.0;
+  if (t > 1.0) t = 1.0;
+  double q[3] = {a[0] + t * d[0], a[1] + t * d[1], a[2] + t * d[2]};
+  return dot3(q, q) <= r * r;
+}
+
+/* SelectLegacyCutoffTraceDt — CutoffRigidityGridless.cpp:1086-1152.  The
+ * gyro budget, limiter and floor are switchable; with outer_sphere the
+ * boundary distance is the distance to the outer sphere. */
+static double select_dt(const orc_trace_params* P, const double x[3], const double p[3],
+                        double dtTrace, double timeRemaining) {
+  double dt = dtTrace;
+  if (timeRemaining < dt) dt = timeRemaining;
+  if (!P->adaptive) return dt;
+  double m0 = P->m0_kg;
+  double p2 = dot3(p, p);
+  double mc = m0 * ORC_C;
+  double gamma = sqrt(1.0 + p2 / (mc * mc));
+  double pMag = sqrt(dmax(0.0, p2));
+  double vMag = (gamma > 0.0 && m0 > 0.0) ? pMag / (gamma * m0) : 0.0;
+  double B[3];
+  orc_dipole_B(P, x, B);
+  double Bmag = sqrt(dot3(B, B));
+  if (Bmag > 0.0) {
+    double omega = fabs(P->q_C) * Bmag / (gamma * m0);
+    if (omega > 0.0) dt = dmin(dt, P->dphi_max / omega);
+  }
+  double vFor = vMag;
+  if (P->dist_limiter) {
+    double Re = P->earth_radius_m;
+    double xRe[3] = {x[0] / Re, x[1] / Re, x[2] / Re};
+    double rRe = sqrt(xRe[0] * xRe[0] + xRe[1] * xRe[1] + xRe[2] * xRe[2]);
+    double dInner = 1.0e300;
+    if (rRe > P->r_inner_Re) dInner = (rRe - P->r_inner_Re) * Re;
+    double dBox = 1.0e300;
+    if (P->outer_sphere) {
+      if (rRe < P->r_outer_Re) dBox = (P->r_outer_Re - rRe) * Re;
+    } else if (xRe[0] >= P->box_min_Re[0] && xRe[0] <= P->box_max_Re[0] &&
+               xRe[1] >= P->box_min_Re[1] && xRe[1] <= P->box_max_Re[1] &&
+               xRe[2] >= P->box_min_Re[2] && xRe[2] <= P->box_max_Re[2]) {
+      double dxp = (P->box_max_Re[0] - xRe[0]) * Re, dxm = (xRe[0] - P->box_min_Re[0]) * Re;
+      double dyp = (P->box_max_Re[1] - xRe[1]) * Re, dym = (xRe[1] - P->box_min_Re[1]) * Re;
+      double dzp = (P->box_max_Re[2] - xRe[2]) * Re, dzm = (xRe[2] - P->box_min_Re[2]) * Re;
+      dBox = dmin(dmin(dmin(dxp, dxm), dmin(dyp, dym)), dmin(dzp, dzm));
+    }
+    double dNear = dmin(dInner, dBox);
+    if (vFor > 0.0 && dNear < 1.0e299) dt = dmin(dt, 0.20 * dNear / vFor);
+  }
+  if (P->dt_floor) {
+    double safeSpeed = dmax(vFor, 1.0e-12);
+    double dtFloor = dmax(dmax(1.0e-12, 1.0e-9 * dmax(dtTrace, 1.0)), 100.0e3 / safeSpeed);
+    dt = dmax(dtFloor, dt);
+  }
+  if (timeRemaining > 0.0) dt = dmin(dt, timeRemaining);
+  return dt;
+}
+
+static int isfin3(const double a[3]) { return isfinite(a[0]) && isfinite(a[1]) && isfinite(a[2]); }
+
+/* TraceTrajectoryImpl — CutoffRigidityGridless.cpp:1246-1457 (BORIS mover,
+ * legacy policy, trap detector off).  Termination values follow
+ * util/TrajectoryTermination.h:10-21. */
+static void trace_impl(const orc_trace_params* P, const double x0[3], const double dir[3],
+                       double R_GV, double dtTrace, int64_t maxSteps, double maxTime,
+                       orc_trace_result* res) {
+  double Re = P->earth_radius_m;
+  double boxmin[3], boxmax[3];
+  for (int d = 0; d < 3; d++) boxmin[d] = P->box_min_Re[d] * Re, boxmax[d] = P->box_max_Re[d] * Re;
+  double rInner = P->r_inner_Re * Re, rOuter = P->r_outer_Re * Re;
+  double pMag = (R_GV * 1.0e9 * P->qabs_C) / ORC_C;
+  double v0[3] = {-1.0 * dir[0], -1.0 * dir[1], -1.0 * dir[2]};
+  double p[3] = {pMag * v0[0], pMag * v0[1], pMag * v0[2]};
+  double x[3] = {x0[0], x0[1], x0[2]};
+  double t = 0.0, s = 0.0;
+  int64_t nSteps = 0;
+  int term = -1;
+  while (nSteps < maxSteps && t < maxTime && (P->max_distance_m <= 0.0 || s < P->max_distance_m)) {
+    if (!isfin3(x) || !isfin3(p)) { term = 8; break; }
+    if (rInner > 0.0 && x[0] * x[0] + x[1] * x[1] + x[2] * x[2] <= dmax(0.0, rInner + 0.0) * dmax(0.0, rInner + 0.0)) {
+      term = 1;
+      break;
+    }
+    int inside;
+    if (P->outer_sphere) inside = (x[0] * x[0] + x[1] * x[1] + x[2] * x[2] <= rOuter * rOuter);
+    else
+      inside = x[0] >= boxmin[0] - 0.0 && x[0] <= boxmax[0] + 0.0 && x[1] >= boxmin[1] - 0.0 &&
+               x[1] <= boxmax[1] + 0.0 && x[2] >= boxmin[2] - 0.0 && x[2] <= boxmax[2] + 0.0;
+    if (!inside) { term = 0; break; }
+    double dt = select_dt(P, x, p, dtTrace, maxTime - t);
+    if (!(dt > 0.0) || !isfinite(dt)) { term = 6; break; }
+    double xb[3] = {x[0], x[1], x[2]};
+    orc_boris_step(P, x, p, dt);
+    if (segment_hits_inner(xb, x, rInner) || inside_inner_m(x, rInner)) { term = 1; break; }
+    if (!isfin3(x) || !isfin3(p)) { term = 8; break; }
+    double dd[3] = {x[0] - xb[0], x[1] - xb[1], x[2] - xb[2]};
+    double seg = sqrt(dot3(dd, dd));
+    s += isfinite(seg) ? seg : 0.0;
+    t += dt;
+    ++nSteps;
+  }
+  if (term < 0) {
+    if (nSteps >= maxSteps) term = 4;
+    else if (P->max_distance_m > 0.0 && s >= P->max_distance_m) term = 5;
+    else if (t >= maxTime) term = 3;
+    else term = 8;
+  }
+  res->termination = term;
+  res->retry_count = 0;
+  res->steps = nSteps;
+  res->time_s = t;
+  res->distance_m = s;
+  for (int d = 0; d < 3; d++) res->x[d] = x[d], res->p[d] = p[d];
+}
+
+/* TraceTrajectoryWithSingleRetry — CutoffRigidityGridless.cpp:1459-1496 */
+void orc_trace(const orc_trace_params* P, const double x0[3], const double dir[3], double R_GV,
+               orc_trace_result* res) {
+  trace_impl(P, x0, dir, R_GV, P->dt_trace_s, P->max_steps, P->max_time_s, res);
+  int t = res->termination;
+  if (!P->retry || t <= 5) return;
+  double dt2 = dmax(1.0e-12, 0.5 * P->dt_trace_s);
+  int64_t ms2 = (P->max_steps <= (int64_t)2147483647 / 2) ? 2 * P->max_steps : (int64_t)2147483647;
+  trace_impl(P, x0, dir, R_GV, dt2, ms2, 2.0 * P->max_time_s, res);
+  res->retry_count = 1;
+}
+
+void orc_trace_batch(const orc_trace_params* P, int64_t n, const double* x0, const double* dir,
+                     const double* R_GV, orc_trace_result* res, int nthreads) {
+#ifdef _OPENMP
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#pragma omp parallel for schedule(dynamic, 1)
+#endif
+  for (int64_t i = 0; i < n; i++) orc_trace(P, x0 + 3 * i, dir + 3 * i, R_GV[i], &res[i]);
+}
+
+/* BuildCutoffSearchGrid_GV (LOG spacing) — CutoffRigidityGridless.cpp:2578-2617 */
+void orc_rigidity_grid(double Rmin, double Rmax, int nScan, double* grid) {
+  if (nScan < 2) nScan = 2;
+  double lmin = log(Rmin), lmax = log(Rmax);
+  for (int i = 0; i < nScan; i++) {
+    double a = (double)i / (double)(nScan - 1);
+    grid[i] = exp((1.0 - a) * lmin + a * lmax);
+  }
+  grid[0] = Rmin;
+  grid[nScan - 1] = Rmax;
+}
+
+/* CutoffForDirectionUpperScan_GV bin semantics — CutoffRigidityGridless.cpp:2805-2834 */
+int orc_upper_cutoff_bin(const int* allowed, int n) {
+  if (!allowed[n - 1]) return -1;
+  for (int i = n - 2; i >= 0; --i)
+    if (!allowed[i]) return i + 1;
+  return 0;
+}
+
+/* AnalyzeCutoffBandSamples — util/CutoffBandSearch.h:96-194 */
+void orc_band_topology(const int* st, int n, int* out) {
+  int lF = -1, lA = -1, uF = -1, uA = -1, firstA = -1, lastF = -1, nT = 0, nAI = 0, nU = 0;
+  int allA = 0, allF = 0, lBelow = 0, lAbove = 0, uBelow = 0, uAbove = 0, lUn = 0, uUn = 0;
+  if (n > 0) {
+    int inA = 0;
+    for (int i = 0; i < n; i++) {
+      int s = st[i];
+      if (s == 1) {
+        if (firstA < 0) firstA = i;
+        if (!inA) { ++nAI; inA = 1; }
+      } else {
+        inA = 0;
+        if (s == 0) lastF = i;
+        else ++nU;
+      }
+      if (i > 0 && st[i - 1] != 2 && s != 2 && st[i - 1] != s) ++nT;
+    }
+    allA = (firstA == 0 && lastF < 0 && nU == 0);
+    allF = (firstA < 0 && lastF == n - 1 && nU == 0);
+    if (firstA == 0) lBelow = 1;
+    else if (firstA > 0) {
+      int clean = 1;
+      for (int i = 0; i < firstA; i++)
+        if (st[i] != 0) { clean = 0; break; }
+      if (clean) { lF = firstA - 1; lA = firstA; }
+      else lUn = 1;
+    } else if (nU > 0) lUn = 1;
+    else lAbove = 1;
+    if (lastF < 0) {
+      if (nU > 0) uUn = 1;
+      else uBelow = 1;
+    } else if (lastF == n - 1) uAbove = 1;
+    else {
+      int clean = 1;
+      for (int i = lastF + 1; i < n; i++)
+        if (st[i] != 1) { clean = 0; break; }
+      if (clean) { uF = lastF; uA = lastF + 1; }
+      else uUn = 1;
+    }
+  }
+  int v[15] = {lF, lA, uF, uA, nT, nAI, nU, allA, allF, lBelow, lAbove, uBelow, uAbove, lUn, uUn};
+  for (int i = 0; i < 15; i++) out[i] = v[i];
+}

@@ -1,0 +1,130 @@
+/*
+ * amps_oracle.h — CPU FP64 oracle for the AMPS particle hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library restates, in plain C, the algorithms
+ * of the reference's particle mover, ECSIM deposition, AMR point location and
+ * gridless cutoff tracer so that the GPU library can be checked against it.
+ * Only the test suite, the smoke entry point and the CPU-baseline leg of the
+ * benchmark may load it; the product path never does.
+ *
+ * Every function cites the reference file:line it follows.  Units are the
+ * reference's: normalized PIC units (c = 1) for the mover/deposit, SI for the
+ * gridless tracer.
+ */
+#ifndef AMPS_ORACLE_H
+#define AMPS_ORACLE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ORC_NC 8      /* cells per block edge (_BLOCK_CELLS_X_) */
+#define ORC_NN 9      /* corner nodes per block edge */
+#define ORC_NCORNER (ORC_NN * ORC_NN * ORC_NN)
+#define ORC_NCELL (ORC_NC * ORC_NC * ORC_NC)
+
+/* Mover return codes (pic.h:5955-5960) plus the two classifications the
+ * reference folds into _PARTICLE_LEFT_THE_DOMAIN_. */
+#define ORC_CODE_REFLECTED (-1)   /* _PARTICLE_REJECTED_ON_THE_FACE_, committed */
+#define ORC_CODE_LEFT_DOMAIN 2    /* deleted on an outer face */
+#define ORC_CODE_FINISHED 3       /* _PARTICLE_MOTION_FINISHED_ */
+#define ORC_CODE_SPHERE 5         /* absorbed by the internal sphere */
+
+#define ORC_BC_DELETE 0
+#define ORC_BC_REFLECT 1
+
+#define ORC_MOVER_LAPENTA 0   /* PIC::Mover::Lapenta2017 (alpha-matrix form) */
+#define ORC_MOVER_BORIS 1     /* PIC::Mover::Boris (h/s rotation form) */
+
+typedef struct orc_mesh orc_mesh;
+
+orc_mesh* orc_mesh_create(const double gxmin[3], const double gxmax[3], int lmax,
+                          int nblocks, const int* level, const int64_t* ijk);
+void orc_mesh_destroy(orc_mesh* m);
+double orc_mesh_eps(const orc_mesh* m);
+/* block bounds (for tests) */
+void orc_block_bounds(const orc_mesh* m, int b, double xmin[3], double xmax[3]);
+/* point location: returns block id or -1 (findTreeNode(double*)) */
+int orc_find_block(const orc_mesh* m, const double x[3], int start_block);
+/* returns local cell id i+8(j+8k) or -1 (FindCellIndex) */
+int orc_find_cell(const orc_mesh* m, const double x[3], int block);
+/* trilinear corner stencil; x may be nudged (InitStencil). w_norm[8] in
+ * stencil loop order, table[8] in the reference's table order, corner[8]
+ * local corner index in stencil loop order. */
+void orc_corner_stencil(const orc_mesh* m, double x[3], int block, double w_norm[8],
+                        double table[8], int corner[8]);
+
+/* One push of all particles; x,v,blk,cell updated in place, code written. */
+int orc_push(const orc_mesh* m, const double* E_corner, const double* B_corner,
+             const double* charge, const double* mass, const double* dt_spec,
+             int mover, int bc_mode, double r_sphere, int64_t n, double* x, double* v,
+             const int* spec, int* blk, int* cell, int* code, int nthreads);
+
+/* Unique corner nodes: fills node_of_corner[nb*729], returns node count and,
+ * if keys!=NULL, the lattice key of each node (sorted ascending). */
+int64_t orc_build_nodes(const orc_mesh* m, int* node_of_corner, int64_t* keys);
+
+/* ECSIM J + mass matrix.  Particles may be in any order; within a cell they
+ * are processed in array order.  J[nnodes*3], M[nnodes*243] are overwritten. */
+int orc_deposit(const orc_mesh* m, const double* B_corner, const double* charge,
+                const double* mass, const double* weight, double dt, int64_t n,
+                const double* x, const double* v, const int* spec, const int* blk,
+                const int* cell, const int* node_of_corner, int64_t nnodes, double* J,
+                double* M, int nthreads);
+
+/* ---------------- gridless cutoff tracer (SI units) ---------------- */
+typedef struct {
+  double q_C;            /* signed backtrace charge [C] */
+  double qabs_C;         /* |physical charge| for rigidity conversion [C] */
+  double m0_kg;
+  double dt_trace_s;     /* DT_TRACE */
+  double max_time_s;
+  int64_t max_steps;
+  double max_distance_m; /* <= 0: off */
+  double dphi_max;       /* gyro-phase budget per step [rad] */
+  int32_t adaptive;      /* adaptiveDt */
+  int32_t dist_limiter;  /* 0.20 * d_near / v */
+  int32_t dt_floor;      /* legacy max(1e-12, 1e-9*DT, 100 km / v) floor */
+  int32_t outer_sphere;  /* 0: box, 1: sphere of radius r_outer_Re */
+  int32_t retry;         /* single retry on numerical failure */
+  int32_t pad_;
+  double box_min_Re[3], box_max_Re[3];
+  double r_outer_Re, r_inner_Re;
+  double earth_radius_m; /* length of one Re for all geometry conversions */
+  double moment_Am2;     /* dipole |m| */
+  double m_hat[3];
+  double B_ext_T[3];     /* uniform field added to the dipole */
+} orc_trace_params;
+
+typedef struct {
+  int32_t termination;   /* TrajectoryTermination value */
+  int32_t retry_count;
+  int64_t steps;
+  double time_s, distance_m;
+  double x[3], p[3];
+} orc_trace_result;
+
+void orc_dipole_B(const orc_trace_params* P, const double x[3], double B[3]);
+void orc_boris_step(const orc_trace_params* P, double x[3], double p[3], double dt);
+void orc_trace(const orc_trace_params* P, const double x0[3], const double dir_unit[3],
+               double R_GV, orc_trace_result* res);
+void orc_trace_batch(const orc_trace_params* P, int64_t n, const double* x0,
+                     const double* dir, const double* R_GV, orc_trace_result* res,
+                     int nthreads);
+/* Rigidity search grid (BuildCutoffSearchGrid_GV, log spacing). */
+void orc_rigidity_grid(double rmin, double rmax, int n, double* grid);
+/* Upper-cutoff bin from allowed flags (1 = allowed), UPPER_SCAN semantics:
+ * -1 when the highest rigidity is forbidden, else the index of the allowed
+ * side of the highest forbidden->allowed transition (0 when all allowed). */
+int orc_upper_cutoff_bin(const int* allowed, int n);
+/* AnalyzeCutoffBandSamples: states 0 forbidden, 1 allowed, 2 unresolved;
+ * out[15] = lowerForbidden, lowerAllowed, upperForbidden, upperAllowed,
+ * nTransitions, nAllowedIntervals, nUnresolved, allAllowed, allForbidden,
+ * lowerBelow, lowerAbove, upperBelow, upperAbove, lowerUnres, upperUnres. */
+void orc_band_topology(const int* states, int n, int* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

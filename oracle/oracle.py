@@ -1,0 +1,207 @@
+"""ctypes front end of the CPU oracle (liboracle.so) and of the reference's
+own compiled gridless sources (_ref/libamps_ref.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+CPU-baseline leg of bench.py, never by the product package.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liboracle.so")
+REF = os.path.join(HERE, "_ref", "libamps_ref.so")
+
+CODE_REFLECTED, CODE_LEFT_DOMAIN, CODE_FINISHED, CODE_SPHERE = -1, 2, 3, 5
+
+# TrajectoryTermination (srcEarth/util/TrajectoryTermination.h:10-21)
+TERM_NAMES = ["OUTER_BOUNDARY_ALLOWED", "INNER_BOUNDARY_FORBIDDEN",
+              "MAGNETICALLY_TRAPPED_FORBIDDEN", "TIME_LIMIT", "STEP_LIMIT", "DISTANCE_LIMIT",
+              "INVALID_TIME_STEP", "INVALID_FIELD", "NUMERICAL_FAILURE"]
+
+
+class TraceParams(C.Structure):
+    _fields_ = [
+        ("q_C", C.c_double), ("qabs_C", C.c_double), ("m0_kg", C.c_double),
+        ("dt_trace_s", C.c_double), ("max_time_s", C.c_double), ("max_steps", C.c_int64),
+        ("max_distance_m", C.c_double), ("dphi_max", C.c_double), ("adaptive", C.c_int32),
+        ("dist_limiter", C.c_int32), ("dt_floor", C.c_int32), ("outer_sphere", C.c_int32),
+        ("retry", C.c_int32), ("pad_", C.c_int32), ("box_min_Re", C.c_double * 3),
+        ("box_max_Re", C.c_double * 3), ("r_outer_Re", C.c_double), ("r_inner_Re", C.c_double),
+        ("earth_radius_m", C.c_double), ("moment_Am2", C.c_double), ("m_hat", C.c_double * 3),
+        ("B_ext_T", C.c_double * 3),
+    ]
+
+
+RESULT_DTYPE = np.dtype([("termination", np.int32), ("retry_count", np.int32),
+                         ("steps", np.int64), ("time_s", np.float64), ("distance_m", np.float64),
+                         ("x", np.float64, 3), ("p", np.float64, 3)])
+
+_lib = None
+_ref = None
+
+
+def build():
+    """Compile the oracle (and the reference pieces when /root/reference exists)."""
+    subprocess.run(["make", "-s", "-C", HERE, "all"], check=True)
+
+
+def _p(a, t):
+    return None if a is None else a.ctypes.data_as(C.POINTER(t))
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        l = C.CDLL(LIB)
+        P, vp, i32, i64, dbl = C.POINTER, C.c_void_p, C.c_int, C.c_int64, C.c_double
+        l.orc_mesh_create.restype = vp
+        l.orc_mesh_create.argtypes = [P(dbl), P(dbl), i32, i32, P(C.c_int), P(i64)]
+        l.orc_mesh_destroy.argtypes = [vp]
+        l.orc_mesh_eps.restype = dbl
+        l.orc_mesh_eps.argtypes = [vp]
+        l.orc_find_block.argtypes = [vp, P(dbl), i32]
+        l.orc_find_cell.argtypes = [vp, P(dbl), i32]
+        l.orc_corner_stencil.argtypes = [vp, P(dbl), i32, P(dbl), P(dbl), P(C.c_int)]
+        l.orc_push.argtypes = [vp, P(dbl), P(dbl), P(dbl), P(dbl), P(dbl), i32, i32, dbl, i64,
+                               P(dbl), P(dbl), P(C.c_int), P(C.c_int), P(C.c_int), P(C.c_int), i32]
+        l.orc_build_nodes.restype = i64
+        l.orc_build_nodes.argtypes = [vp, P(C.c_int), P(i64)]
+        l.orc_deposit.argtypes = [vp, P(dbl), P(dbl), P(dbl), P(dbl), dbl, i64, P(dbl), P(dbl),
+                                  P(C.c_int), P(C.c_int), P(C.c_int), P(C.c_int), i64, P(dbl),
+                                  P(dbl), i32]
+        l.orc_dipole_B.argtypes = [P(TraceParams), P(dbl), P(dbl)]
+        l.orc_boris_step.argtypes = [P(TraceParams), P(dbl), P(dbl), dbl]
+        l.orc_trace_batch.argtypes = [P(TraceParams), i64, P(dbl), P(dbl), P(dbl), vp, i32]
+        l.orc_rigidity_grid.argtypes = [dbl, dbl, i32, P(dbl)]
+        l.orc_upper_cutoff_bin.argtypes = [P(C.c_int), i32]
+        l.orc_band_topology.argtypes = [P(C.c_int), i32, P(C.c_int)]
+        _lib = l
+    return _lib
+
+
+def ref_available():
+    return os.path.exists(REF)
+
+
+def ref():
+    """The reference's compiled gridless mover/dipole/band sources."""
+    global _ref
+    if _ref is None:
+        r = C.CDLL(REF)
+        P, dbl = C.POINTER, C.c_double
+        r.ref_set_dipole.argtypes = [dbl, dbl]
+        r.ref_get_dipole.argtypes = [P(dbl), P(dbl)]
+        r.ref_field.argtypes = [P(dbl), P(dbl), P(dbl)]
+        r.ref_boris_steps.argtypes = [C.c_int, P(dbl), P(dbl), dbl, dbl, dbl, P(dbl), P(dbl)]
+        r.ref_step_checked.argtypes = [P(dbl), P(dbl), dbl, dbl, dbl, P(dbl), dbl]
+        r.ref_step_checked.restype = C.c_int
+        r.ref_band_topology.argtypes = [C.c_int, P(C.c_int), P(C.c_int)]
+        _ref = r
+    return _ref
+
+
+class Mesh:
+    """Oracle octree built from the same leaf list as the GPU library."""
+
+    def __init__(self, m):
+        l = lib()
+        lv, ijk = m.desc_arrays()
+        self._lv, self._ijk = lv.astype(np.int32), ijk.astype(np.int64)
+        xmin = np.ascontiguousarray(m.xmin, np.float64)
+        xmax = np.ascontiguousarray(m.xmax, np.float64)
+        self.h = l.orc_mesh_create(_p(xmin, C.c_double), _p(xmax, C.c_double), m.lmax,
+                                   m.nblocks, _p(self._lv, C.c_int), _p(self._ijk, C.c_int64))
+        if not self.h:
+            raise ValueError("oracle rejected the mesh")
+        self.nblocks = m.nblocks
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_mesh_destroy(self.h)
+
+    @property
+    def eps(self):
+        return lib().orc_mesh_eps(self.h)
+
+    def find_block(self, x, start=-1):
+        x = np.ascontiguousarray(x, np.float64)
+        return lib().orc_find_block(self.h, _p(x, C.c_double), start)
+
+    def find_cell(self, x, b):
+        x = np.ascontiguousarray(x, np.float64)
+        return lib().orc_find_cell(self.h, _p(x, C.c_double), b)
+
+    def stencil(self, x, b):
+        x = np.array(x, np.float64)
+        w = np.zeros(8)
+        t = np.zeros(8)
+        c = np.zeros(8, np.int32)
+        lib().orc_corner_stencil(self.h, _p(x, C.c_double), b, _p(w, C.c_double),
+                                 _p(t, C.c_double), _p(c, C.c_int))
+        return x, w, t, c
+
+    def build_nodes(self):
+        noc = np.zeros(self.nblocks * 729, np.int32)
+        keys = np.zeros(self.nblocks * 729, np.int64)
+        nn = lib().orc_build_nodes(self.h, _p(noc, C.c_int), _p(keys, C.c_int64))
+        return noc, keys[:nn]
+
+
+def push(omesh, E, B, species, parts, mover=0, bc_mode=1, r_sphere=0.0, nthreads=0):
+    """One oracle push of a Particles object; returns (new Particles, code)."""
+    p = parts.copy()
+    code = np.zeros(p.n, np.int32)
+    E = np.ascontiguousarray(E, np.float64)
+    B = np.ascontiguousarray(B, np.float64)
+    lib().orc_push(omesh.h, _p(E, C.c_double), _p(B, C.c_double),
+                   _p(species.charge, C.c_double), _p(species.mass, C.c_double),
+                   _p(species.dt, C.c_double), mover, bc_mode, r_sphere, p.n,
+                   _p(p.x, C.c_double), _p(p.v, C.c_double), _p(p.spec, C.c_int),
+                   _p(p.blk, C.c_int), _p(p.cell, C.c_int), _p(code, C.c_int), nthreads)
+    return p, code
+
+
+def deposit(omesh, B, species, dt, parts, node_of_corner, nnodes, nthreads=0):
+    J = np.zeros((nnodes, 3))
+    M = np.zeros((nnodes, 243))
+    B = np.ascontiguousarray(B, np.float64)
+    noc = np.ascontiguousarray(node_of_corner, np.int32)
+    lib().orc_deposit(omesh.h, _p(B, C.c_double), _p(species.charge, C.c_double),
+                      _p(species.mass, C.c_double), _p(species.weight, C.c_double), dt, parts.n,
+                      _p(parts.x, C.c_double), _p(parts.v, C.c_double), _p(parts.spec, C.c_int),
+                      _p(parts.blk, C.c_int), _p(parts.cell, C.c_int), _p(noc, C.c_int), nnodes,
+                      _p(J, C.c_double), _p(M, C.c_double), nthreads)
+    return J, M
+
+
+def trace_batch(params, x0, dirs, R, nthreads=0):
+    x0 = np.ascontiguousarray(x0, np.float64).reshape(-1, 3)
+    dirs = np.ascontiguousarray(dirs, np.float64).reshape(-1, 3)
+    R = np.ascontiguousarray(R, np.float64)
+    out = np.zeros(len(R), RESULT_DTYPE)
+    lib().orc_trace_batch(C.byref(params), len(R), _p(x0, C.c_double), _p(dirs, C.c_double),
+                          _p(R, C.c_double), out.ctypes.data_as(C.c_void_p), nthreads)
+    return out
+
+
+def rigidity_grid(rmin, rmax, n):
+    g = np.zeros(n)
+    lib().orc_rigidity_grid(rmin, rmax, n, _p(g, C.c_double))
+    return g
+
+
+def upper_cutoff_bin(allowed):
+    a = np.ascontiguousarray(allowed, np.int32)
+    return lib().orc_upper_cutoff_bin(_p(a, C.c_int), len(a))
+
+
+def band_topology(states):
+    s = np.ascontiguousarray(states, np.int32)
+    out = np.zeros(15, np.int32)
+    lib().orc_band_topology(_p(s, C.c_int), len(s), _p(out, C.c_int))
+    return out

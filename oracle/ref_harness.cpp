@@ -1,0 +1,96 @@
+// ref_harness.cpp — C entry points over the reference's own gridless mover,
+// dipole and cutoff-band sources, compiled from /root/reference by
+// oracle/Makefile into oracle/_ref/.  Test infrastructure only: used to pin
+// the oracle's restatement, never by the product path.
+#include <vector>
+
+#include "CutoffBandSearch.h"
+#include "DipoleInterface.h"
+#include "GridlessParticleMovers.h"
+
+namespace {
+struct DipolePlusUniform : public IGridlessFieldEvaluator {
+  double bext[3] = {0.0, 0.0, 0.0};
+  void GetB_T(const V3& x, V3& B) const override {
+    const double xm[3] = {x.x, x.y, x.z};
+    double b[3];
+    Earth::GridlessMode::Dipole::GetB_Tesla(xm, b);
+    B = V3{b[0] + bext[0], b[1] + bext[1], b[2] + bext[2]};
+  }
+};
+}  // namespace
+
+extern "C" {
+
+void ref_set_dipole(double moment_scale, double tilt_deg) {
+  Earth::GridlessMode::Dipole::SetMomentScale(moment_scale);
+  Earth::GridlessMode::Dipole::SetTiltDeg(tilt_deg);
+}
+
+void ref_get_dipole(double* moment_Am2, double m_hat[3]) {
+  using namespace Earth::GridlessMode::Dipole;
+  *moment_Am2 = gParams.momentScale_Me * M_E_Am2;
+  for (int d = 0; d < 3; d++) m_hat[d] = gParams.m_hat[d];
+}
+
+void ref_field(const double x[3], const double bext[3], double B[3]) {
+  DipolePlusUniform f;
+  for (int d = 0; d < 3; d++) f.bext[d] = bext[d];
+  V3 b;
+  f.GetB_T(V3{x[0], x[1], x[2]}, b);
+  B[0] = b.x, B[1] = b.y, B[2] = b.z;
+}
+
+// n consecutive BorisStep calls; x,p updated in place (3*n outputs per call
+// are written to traj when traj != nullptr).
+void ref_boris_steps(int n, double x[3], double p[3], double q, double m0, double dt,
+                     const double bext[3], double* traj) {
+  DipolePlusUniform f;
+  for (int d = 0; d < 3; d++) f.bext[d] = bext[d];
+  V3 xv{x[0], x[1], x[2]}, pv{p[0], p[1], p[2]};
+  for (int i = 0; i < n; i++) {
+    BorisStep(xv, pv, q, m0, dt, f);
+    if (traj) {
+      traj[6 * i + 0] = xv.x, traj[6 * i + 1] = xv.y, traj[6 * i + 2] = xv.z;
+      traj[6 * i + 3] = pv.x, traj[6 * i + 4] = pv.y, traj[6 * i + 5] = pv.z;
+    }
+  }
+  x[0] = xv.x, x[1] = xv.y, x[2] = xv.z;
+  p[0] = pv.x, p[1] = pv.y, p[2] = pv.z;
+}
+
+int ref_step_checked(double x[3], double p[3], double q, double m0, double dt,
+                     const double bext[3], double r_inner) {
+  DipolePlusUniform f;
+  for (int d = 0; d < 3; d++) f.bext[d] = bext[d];
+  V3 xv{x[0], x[1], x[2]}, pv{p[0], p[1], p[2]};
+  bool ok = StepParticleChecked(MoverType::BORIS, xv, pv, q, m0, dt, f, r_inner);
+  x[0] = xv.x, x[1] = xv.y, x[2] = xv.z;
+  p[0] = pv.x, p[1] = pv.y, p[2] = pv.z;
+  return ok ? 1 : 0;
+}
+
+// AnalyzeCutoffBandSamples over states numbered as CutoffSampleState
+// (0 physical forbidden, 1 allowed, 2 unresolved).
+void ref_band_topology(int n, const int* states, int* out) {
+  std::vector<EarthUtil::CutoffSampleState> s((size_t)n);
+  for (int i = 0; i < n; i++)
+    s[(size_t)i] = static_cast<EarthUtil::CutoffSampleState>(states[i]);
+  EarthUtil::CutoffBandTopology t = EarthUtil::AnalyzeCutoffBandSamples(s);
+  out[0] = t.lowerForbiddenIndex;
+  out[1] = t.lowerAllowedIndex;
+  out[2] = t.upperForbiddenIndex;
+  out[3] = t.upperAllowedIndex;
+  out[4] = t.nTransitions;
+  out[5] = t.nAllowedIntervals;
+  out[6] = t.nUnresolved;
+  out[7] = t.allAllowed ? 1 : 0;
+  out[8] = t.allForbidden ? 1 : 0;
+  out[9] = t.lowerBelowRange ? 1 : 0;
+  out[10] = t.lowerAboveRange ? 1 : 0;
+  out[11] = t.upperBelowRange ? 1 : 0;
+  out[12] = t.upperAboveRange ? 1 : 0;
+  out[13] = t.lowerBracketUnresolved ? 1 : 0;
+  out[14] = t.upperBracketUnresolved ? 1 : 0;
+}
+}

@@ -1,0 +1,385 @@
+"""Pins of the CPU oracle (no GPU).
+
+* against the reference's own compiled gridless sources (oracle/_ref): the
+  relativistic Boris step, the inner-sphere checked step, the dipole and the
+  cutoff-band topology, bit for bit;
+* against the reference's golden vectors: Stormer vertical cutoffs
+  (srcEarth/test/C1/reference_C1_stormer.csv, C12 BORIS rows) and the C4
+  exit classification / rigidity-conservation table;
+* against what the mathematics fixes: trilinear weights, half-open block
+  search, alpha-form == Boris rotation, deposit sums and symmetries.
+"""
+import math
+import os
+
+import numpy as np
+import pytest
+
+from amps_amd import mesh as M
+from amps_amd.pic import Species
+from oracle import oracle as O
+from tests import tracer_setup as TS
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+needs_ref = pytest.mark.skipif(not O.ref_available(), reason="oracle/_ref not built (no reference tree)")
+
+
+# ---------------------------------------------------------------------------
+# tracer: bit-exact against the reference's compiled BorisStep / dipole
+# ---------------------------------------------------------------------------
+@needs_ref
+@pytest.mark.parametrize("tilt,bz", [(0.0, 0.0), (10.0, -5e-9), (23.5, 2e-9)])
+def test_boris_step_matches_reference_bitwise(tilt, bz):
+    r = O.ref()
+    r.ref_set_dipole(3.1 / 3.12, tilt)
+    P = TS.params(tilt_deg=tilt, moment_scale=3.1 / 3.12, bz_T=bz)
+    mom = np.zeros(1)
+    mhat = np.zeros(3)
+    r.ref_get_dipole(O._p(mom, O.C.c_double), O._p(mhat, O.C.c_double))
+    assert mom[0] == P.moment_Am2 and list(mhat) == list(P.m_hat)
+    rng = np.random.default_rng(11)
+    bext = np.array([0.0, 0.0, bz])
+    for trial in range(20):
+        x = rng.normal(size=3)
+        x *= rng.uniform(1.05, 8.0) * TS.RE_DIPOLE / np.linalg.norm(x)
+        d = rng.normal(size=3)
+        d /= np.linalg.norm(d)
+        R = 10 ** rng.uniform(-1, 1.3)
+        p = d * (R * 1e9 * TS.QE) / TS.C
+        dt = rng.uniform(1e-4, 5e-3)
+        xr, pr = x.copy(), p.copy()
+        traj = np.zeros(6 * 50)
+        r.ref_boris_steps(50, O._p(xr, O.C.c_double), O._p(pr, O.C.c_double), P.q_C, P.m0_kg, dt,
+                          O._p(bext, O.C.c_double), O._p(traj, O.C.c_double))
+        xo, po = x.copy(), p.copy()
+        for i in range(50):
+            O.lib().orc_boris_step(O.C.byref(P), O._p(xo, O.C.c_double), O._p(po, O.C.c_double), dt)
+            assert np.array_equal(xo, traj[6 * i:6 * i + 3]) and np.array_equal(po, traj[6 * i + 3:6 * i + 6])
+        B1 = np.zeros(3)
+        B2 = np.zeros(3)
+        r.ref_field(O._p(x, O.C.c_double), O._p(bext, O.C.c_double), O._p(B1, O.C.c_double))
+        O.lib().orc_dipole_B(O.C.byref(P), O._p(x, O.C.c_double), O._p(B2, O.C.c_double))
+        assert np.array_equal(B1, B2)
+
+
+@needs_ref
+def test_band_topology_matches_reference():
+    r = O.ref()
+    rng = np.random.default_rng(5)
+    for n in [1, 2, 3, 7, 48]:
+        for _ in range(300):
+            st = rng.choice([0, 1, 2], size=n, p=[0.45, 0.45, 0.1]).astype(np.int32)
+            if rng.random() < 0.5:
+                st[rng.integers(0, n):] = 1
+            out = np.zeros(15, np.int32)
+            r.ref_band_topology(n, O._p(st, O.C.c_int), O._p(out, O.C.c_int))
+            assert list(O.band_topology(st)) == list(out), st
+
+
+def test_upper_cutoff_bin_semantics():
+    # CutoffForDirectionUpperScan_GV: -1 if Rmax forbidden, Rmin index when all allowed
+    assert O.upper_cutoff_bin([0, 0, 1, 1]) == 2
+    assert O.upper_cutoff_bin([1, 1, 1, 1]) == 0
+    assert O.upper_cutoff_bin([1, 1, 1, 0]) == -1
+    assert O.upper_cutoff_bin([1, 0, 1, 0, 1, 1]) == 4
+    g = O.rigidity_grid(0.1, 20.0, 48)
+    assert g[0] == 0.1 and g[-1] == 20.0 and np.all(np.diff(g) > 0)
+    assert np.allclose(np.diff(np.log(g)), math.log(200.0) / 47, rtol=1e-12)
+
+
+def test_boris_pure_B_conserves_momentum():
+    P = TS.params(tilt_deg=10.0, moment_scale=1.0)
+    x = np.array([2.0, 0.3, -0.4]) * TS.RE_DIPOLE
+    p = np.array([0.3, -0.2, 0.9]) * 1e-18
+    p0 = np.linalg.norm(p)
+    for _ in range(2000):
+        O.lib().orc_boris_step(O.C.byref(P), O._p(x, O.C.c_double), O._p(p, O.C.c_double), 1e-3)
+    assert abs(np.linalg.norm(p) / p0 - 1) < 1e-13
+
+
+def test_boris_uniform_field_rotation_angle():
+    # uniform B only: the Boris rotation turns p by exactly 2*atan(|t|) per step
+    P = TS.params(moment_scale=0.0, bz_T=-5e-9)
+    P.moment_Am2 = 0.0
+    x = np.array([3.0, 0.0, 0.0]) * TS.RE_DIPOLE
+    p = np.array([1.0, 0.0, 0.0]) * 1e-19
+    dt = 0.01
+    gam = math.sqrt(1 + np.dot(p, p) / (P.m0_kg * TS.C) ** 2)
+    t = abs(P.q_C * dt / (2 * gam * P.m0_kg) * 5e-9)
+    O.lib().orc_boris_step(O.C.byref(P), O._p(x, O.C.c_double), O._p(p, O.C.c_double), dt)
+    ang = math.atan2(p[1], p[0])
+    assert abs(abs(ang) - 2 * math.atan(t)) < 1e-12
+
+
+# ---------------------------------------------------------------------------
+# tracer: reference golden vectors (srcEarth/test)
+# ---------------------------------------------------------------------------
+def _c1_rows():
+    import csv
+    with open(os.path.join(GOLD, "C1_stormer.csv")) as f:
+        return [(float(r["alt_km"]), float(r["lat_deg"]), float(r["Rc_stormer_GV"])) for r in csv.DictReader(f)]
+
+
+def test_c1_vertical_stormer_cutoffs():
+    """C1: vertical cutoffs (UPPER_SCAN over 50 log bins + bisection) vs the
+    analytic Stormer table with run_C1.py's direct-field tolerances (max 5e-2
+    for |lat|<=30 deg, 2.5e-1 above).  The C1 deck runs the default RK4 mover;
+    the BORIS mover meets the tolerances with a 0.02 rad gyro-phase budget (at
+    the legacy 0.15 rad it is 10% low at +-30 deg, 9000 km).  At 500 km the
+    solid Earth shadows part of the cone: the access function has a forbidden
+    band above the Stormer value (converged in the step size), so there the
+    lower band edge (first forbidden->allowed bracket) is compared with the
+    Stormer value and the upper cutoff must lie at or above it."""
+    rows = _c1_rows()
+    P = TS.c1_params()
+    P.dphi_max = 0.02
+    rmin, rmax = TS.c1_rigidity_range()
+    grid = O.rigidity_grid(rmin, rmax, 50)
+    for alt, lat, rc_ref in rows:
+        x0 = TS.shell_point(0.0, lat, alt, TS.RE_AMPS)
+        dirv = -x0 / np.linalg.norm(x0)
+        rc = TS.oracle_cutoff(P, x0, dirv, rmin, rmax, 50)
+        tol = 5e-2 if abs(lat) <= 30 else 2.5e-1
+        if alt > 1000.0:
+            assert abs(rc / rc_ref - 1) <= tol, (alt, lat, rc, rc_ref)
+        else:
+            res = O.trace_batch(P, np.tile(x0, (50, 1)), np.tile(dirv, (50, 1)), grid)
+            st = np.where(res["termination"] == 0, 1, 0)
+            topo = O.band_topology(st)
+            lo_f, lo_a = topo[0], topo[1]
+            assert lo_a > 0 and grid[lo_f] * (1 - tol) <= rc_ref <= grid[lo_a] * (1 + tol), (lat, topo)
+            assert rc >= rc_ref * (1 - tol)
+
+
+def test_c12_boris_stormer_rows():
+    """C12 BORIS rows (both shells) with the C1 tolerances: the table's 1e-3
+    needs a smaller fixed step than this CPU test affords.  Rows whose Stormer
+    value lies below the scan's Rmin are skipped (the scan returns Rmin)."""
+    import csv
+    P = TS.c1_params()
+    P.dphi_max = 0.02
+    rmin, rmax = TS.c1_rigidity_range()
+    with open(os.path.join(GOLD, "C12_stormer_movers.csv")) as f:
+        rows = [r for r in csv.DictReader(f) if r["mover"] == "BORIS"]
+    assert len(rows) == 14
+    n = 0
+    for r in rows:
+        lat = float(r["lat_deg"])
+        rc_ref = float(r["Rc_stormer_GV"])
+        if rc_ref < rmin:
+            continue
+        x0 = TS.shell_point(0.0, lat, float(r["alt_km"]), TS.RE_AMPS)
+        rc = TS.oracle_cutoff(P, x0, -x0 / np.linalg.norm(x0), rmin, rmax, 80)
+        tol = 5e-2 if abs(lat) <= 30 else 2.5e-1
+        assert abs(rc / rc_ref - 1) <= tol, (lat, rc, rc_ref)
+        n += 1
+    assert n >= 12
+
+
+def test_c4_exit_classification_and_invariants():
+    """C4: allowed (outer box) at 2 x Rc, not at 0.5 x Rc, |p| kept to 1e-6."""
+    import csv
+    P = TS.c1_params()
+    with open(os.path.join(GOLD, "C4_invariants.csv")) as f:
+        rows = list(csv.DictReader(f))
+    x0s, dirs, Rs, exp = [], [], [], []
+    for r in rows:
+        x0 = TS.shell_point(float(r["lon_deg"]), float(r["lat_deg"]), float(r["alt_km"]), TS.RE_AMPS)
+        x0s.append(x0)
+        dirs.append(-x0 / np.linalg.norm(x0))
+        Rs.append(float(r["R_GV"]))
+        exp.append(int(r["expected_allowed"]))
+    res = O.trace_batch(P, np.array(x0s), np.array(dirs), np.array(Rs))
+    for i, r in enumerate(rows):
+        allowed = int(res["termination"][i] == 0)
+        assert allowed == exp[i], (r, O.TERM_NAMES[res["termination"][i]])
+        p0 = Rs[i] * 1e9 * TS.QE / TS.C
+        assert abs(np.linalg.norm(res["p"][i]) / p0 - 1) <= float(r["rel_dR_limit"])
+
+
+# ---------------------------------------------------------------------------
+# PIC: mesh search, stencil, mover, deposit
+# ---------------------------------------------------------------------------
+@pytest.fixture(scope="module")
+def amr():
+    m = M.central_octant_mesh()
+    return m, O.Mesh(m)
+
+
+def test_find_block_matches_brute_force(amr):
+    m, om = amr
+    rng = np.random.default_rng(1)
+    pts = rng.uniform(-1, 1, size=(4000, 3))
+    # points exactly on block faces, edges and corners at every level
+    g = rng.integers(0, 9, size=(2000, 3)) * 0.25 - 1.0
+    pts = np.concatenate([pts, g, np.array([[1.0, 0.0, 0.0], [-1.0, -1.0, -1.0], [1.0, 1.0, 1.0]])])
+    for x in pts:
+        b = om.find_block(x, 0)
+        inside = np.all((x >= m.blk_xmin) & (x < m.blk_xmax), axis=1)
+        want = int(np.flatnonzero(inside)[0]) if inside.any() else -1
+        assert b == want, x
+        if b >= 0:
+            c = om.find_cell(x, b)
+            dxc = 2.0 / (1 << m.level[b]) / 8.0
+            ijk = np.minimum(((x - m.blk_xmin[b]) / dxc).astype(int), 7)
+            assert c == ijk[0] + 8 * (ijk[1] + 8 * ijk[2])
+
+
+def test_find_block_start_independent(amr):
+    m, om = amr
+    rng = np.random.default_rng(2)
+    for x in rng.uniform(-1, 1, size=(500, 3)):
+        want = om.find_block(x, 0)
+        for s in rng.integers(0, m.nblocks, 4):
+            assert om.find_block(x, int(s)) == want
+
+
+def test_stencil_weights_and_affine_reproduction(amr):
+    m, om = amr
+    rng = np.random.default_rng(3)
+    for _ in range(300):
+        b = int(rng.integers(0, m.nblocks))
+        x = m.blk_xmin[b] + rng.random(3) * (m.blk_xmax[b] - m.blk_xmin[b])
+        xs, w, t, c = om.stencil(x, b)
+        assert abs(w.sum() - 1) < 1e-15 and np.all(w >= 0)
+        # table order 0,1,3,2,4,5,7,6 of the loop order (i outer, k inner)
+        assert np.array_equal(t, w * 0 + t) and np.allclose(t[[0, 4, 3, 7, 1, 5, 2, 6]], w, rtol=1e-15)
+        dx = (m.blk_xmax[b] - m.blk_xmin[b]) / 8
+        ci = c % 9, (c // 9) % 9, c // 81
+        node = m.blk_xmin[b] + np.stack(ci, 1) * dx
+        a = np.array([0.3, -1.2, 2.5])
+        f = node @ a + 0.7
+        assert abs(w @ f - (xs @ a + 0.7)) < 1e-13
+    # the upper-face nudge (InitStencil :1094)
+    b = 0
+    x = m.blk_xmax[b].copy()
+    xs, w, t, c = om.stencil(x, b)
+    dx = (m.blk_xmax[b] - m.blk_xmin[b]) / 8
+    assert np.all(xs == m.blk_xmax[b] - 1e-10 * dx)
+
+
+def test_lapenta_alpha_form_equals_boris_rotation():
+    """2 alpha (v + qdt/2m E) - v of Lapenta2017 is the Boris rotation of A2."""
+    m = M.uniform_mesh(2)
+    om = O.Mesh(m)
+    E, B = M.corner_fields(m)
+    parts = M.uniform_particles(m, 3000, 0.05)
+    sp = Species([1.0], [1.0], [0.02])
+    a, ca = O.push(om, E, B, sp, parts, mover=0, bc_mode=0, r_sphere=0.2)
+    b, cb = O.push(om, E, B, sp, parts, mover=1, bc_mode=0, r_sphere=0.2)
+    assert np.array_equal(ca, cb)
+    ok = ca == 3
+    assert np.allclose(a.v[:, ok], b.v[:, ok], rtol=1e-12, atol=1e-15)
+    assert np.allclose(a.x[:, ok], b.x[:, ok], rtol=1e-13, atol=1e-15)
+
+
+def test_push_uniform_fields_exb_drift():
+    """Uniform E x B: the time-averaged velocity is E x B / B^2."""
+    m = M.uniform_mesh(1)
+    om = O.Mesh(m)
+    Ev = np.array([0.0, 0.02, 0.0])
+    Bv = np.array([0.0, 0.0, 1.0])
+    E = np.broadcast_to(Ev, (m.nblocks, 729, 3)).copy()
+    B = np.broadcast_to(Bv, (m.nblocks, 729, 3)).copy()
+    x = np.array([[0.1], [0.1], [0.1]])
+    parts = M.Particles(x, np.zeros((3, 1)), [0], m.locate(x.T), M.cell_of(m, x, m.locate(x.T)))
+    sp = Species([1.0], [1.0], [0.05])
+    xs = []
+    for _ in range(500):  # 25 time units: 4 gyro periods, 0.5 of drift
+        parts, code = O.push(om, E, B, sp, parts, mover=0, bc_mode=1)
+        assert code[0] == 3
+        xs.append(parts.x[:, 0].copy())
+    xs = np.array(xs)
+    drift = (xs[-1] - xs[0]) / (0.05 * (len(xs) - 1))
+    assert abs(drift[0] - 0.02) < 2e-3 and abs(drift[1]) < 2e-3
+
+
+def test_push_reflect_and_absorb_classification():
+    m = M.uniform_mesh(2)
+    om = O.Mesh(m)
+    E, B = M.corner_fields(m)
+    parts = M.uniform_particles(m, 20000, 0.5, r_excl=0.2)
+    sp = Species([1.0], [1.0], [0.05])
+    p2, code = O.push(om, E, B, sp, parts, mover=0, bc_mode=1, r_sphere=0.2)
+    assert set(np.unique(code)) <= {-1, 3, 5}
+    refl = code == -1
+    assert refl.any() and (code == 5).any()
+    # reflected particles are inside the domain by EPS and move inward
+    xin = p2.x[:, refl]
+    assert np.all(np.abs(xin) < 1.0)
+    p3, code3 = O.push(om, E, B, sp, parts, mover=0, bc_mode=0, r_sphere=0.2)
+    assert np.array_equal(code3 == 2, refl)
+
+
+def test_deposit_single_particle_closed_form():
+    """One particle: J_c = q W_c (alpha v) / V, and the mass-matrix blocks sum
+    over all (i, slot) pairs to (q^2 dt / 2 m V) alpha (sum W)^2."""
+    m = M.uniform_mesh(1)
+    om = O.Mesh(m)
+    E, B = M.corner_fields(m)
+    noc, keys = om.build_nodes()
+    x = np.array([[0.123], [-0.456], [0.789]])
+    blk = m.locate(x.T)
+    parts = M.Particles(x, np.array([[0.01], [0.02], [-0.03]]), [0], blk, M.cell_of(m, x, blk))
+    sp = Species([-1.0], [0.01], [1.0], [1.0])
+    dt = 1e-3
+    J, Mm = O.deposit(om, B, sp, dt, parts, noc, len(keys))
+    xs, w, t, c = om.stencil(x[:, 0], int(blk[0]))
+    Bi = (w[:, None] * B[blk[0]][c]).sum(0)
+    q, mass = -1.0, 0.01
+    h = q * dt / (2 * mass)
+    c0 = 1 / (1 + h * h * Bi @ Bi)
+    alpha = c0 * (np.eye(3) + h * h * np.outer(Bi, Bi) - h * np.array([[0, -Bi[2], Bi[1]], [Bi[2], 0, -Bi[0]], [-Bi[1], Bi[0], 0]]))
+    dx = (m.blk_xmax[blk[0]] - m.blk_xmin[blk[0]]) / 8
+    V = np.prod(dx)
+    want_Jsum = q * (alpha @ parts.v[:, 0]) * t.sum() / V
+    assert np.allclose(J.sum(0), want_Jsum, rtol=1e-12)
+    Ms = Mm.reshape(-1, 27, 9).sum(axis=(0, 1)).reshape(3, 3)
+    assert np.allclose(Ms, q * h / V * alpha * t.sum() ** 2, rtol=1e-12)
+
+
+def test_deposit_symmetry_and_linearity(amr):
+    m, om = amr
+    E, B = M.corner_fields(m)
+    noc, keys = om.build_nodes()
+    pa = M.per_cell_particles(m, [1, 1], [0.05, 0.005], drift=(0.01, 0, 0))
+    sp = Species([-1.0, 1.0], [0.01, 1.0], [1e-3, 1e-3])
+    J, Mm = O.deposit(om, B, sp, 1e-3, pa, noc, len(keys))
+    # linearity in the particle set: deposit(A) + deposit(B) == deposit(A u B)
+    half = np.arange(pa.n) % 2 == 0
+    J1, M1 = O.deposit(om, B, sp, 1e-3, pa.take(np.flatnonzero(half)), noc, len(keys))
+    J2, M2 = O.deposit(om, B, sp, 1e-3, pa.take(np.flatnonzero(~half)), noc, len(keys))
+    assert np.allclose(J1 + J2, J, rtol=1e-12, atol=1e-12 * np.abs(J).max())
+    assert np.allclose(M1 + M2, Mm, rtol=1e-12, atol=1e-12 * np.abs(Mm).max())
+
+
+def test_deposit_mass_matrix_symmetry():
+    """On a one-level mesh, node a's block for neighbour b equals node b's block
+    for neighbour a (same 3x3 block scattered to both, :2398-2426)."""
+    m = M.uniform_mesh(2)
+    om = O.Mesh(m)
+    E, B = M.corner_fields(m)
+    noc, keys = om.build_nodes()
+    pa = M.per_cell_particles(m, [2], [0.05])
+    sp = Species([-1.0], [0.01], [1e-3])
+    J, Mm = O.deposit(om, B, sp, 1e-3, pa, noc, len(keys))
+    S = (1 << m.lmax) + 1
+    cs = (1 << (m.lmax - 2)) // 8
+    pos = {int(k): i for i, k in enumerate(keys)}
+    amap = [0, -1, 1]
+    inv = {0: 0, -1: 1, 1: 2}
+    n_checked = 0
+    for a, ka in enumerate(keys.tolist()):
+        ga = np.array([ka % S, (ka // S) % S, ka // (S * S)])
+        for s in range(27):
+            off = np.array([amap[s % 3], amap[(s // 3) % 3], amap[s // 9]])
+            blk = Mm[a, 9 * s:9 * s + 9]
+            gb = ga + off * cs
+            kb = int((gb[2] * S + gb[1]) * S + gb[0])
+            if kb not in pos:
+                assert not np.any(blk)
+                continue
+            s2 = inv[-off[0]] + 3 * inv[-off[1]] + 9 * inv[-off[2]]
+            assert np.array_equal(blk, Mm[pos[kb], 9 * s2:9 * s2 + 9])
+            n_checked += 1
+    assert n_checked > 10000

@@ -1,0 +1,144 @@
+"""GPU parity of the FP64 push, migration and ECSIM deposit against the CPU
+oracle on the same seeded inputs (bit-exact ids, codes, x and v)."""
+import numpy as np
+import pytest
+
+from amps_amd import _lib as L
+from amps_amd import mesh as M
+from amps_amd.pic import PicState, Species
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _by_id(p, code=None):
+    o = np.argsort(p.id)
+    q = p.take(o)
+    return q, (None if code is None else code[o])
+
+
+def _case(mesh, n, vth, dt, seed=1):
+    E, B = M.corner_fields(mesh)
+    parts = M.uniform_particles(mesh, n, vth, seed_x=seed, seed_v=seed + 1)
+    return E, B, parts, Species([1.0], [1.0], [dt])
+
+
+@pytest.mark.parametrize("mover", [L.MOVER_LAPENTA, L.MOVER_BORIS])
+@pytest.mark.parametrize("bc", [L.BC_REFLECT, L.BC_DELETE])
+@pytest.mark.parametrize("amr", [False, True])
+def test_push_single_step_bitexact(gpu_ctx, mover, bc, amr):
+    mesh = M.central_octant_mesh() if amr else M.uniform_mesh(2)
+    E, B, parts, sp = _case(mesh, 40000, 0.5, 0.05)
+    print(f"push case mover={mover} bc={bc} amr={amr} n={parts.n}", flush=True)
+    st = PicState(mesh, E, B, sp, ctx=gpu_ctx)
+    st.set_particles(parts)
+    st.MoveParticles(mover=mover, bc_mode=bc, r_sphere=0.2, compact=False)
+    g, gcode = st.get_particles()
+    om = O.Mesh(mesh)
+    o, ocode = O.push(om, E, B, sp, parts, mover=mover, bc_mode=bc, r_sphere=0.2)
+    g, gcode = _by_id(g, gcode)
+    o, ocode = _by_id(o, ocode)
+    assert np.array_equal(gcode, ocode)
+    assert set(np.unique(ocode)) >= ({-1, 3, 5} if bc == L.BC_REFLECT else {2, 3, 5})
+    assert np.array_equal(g.x, o.x) and np.array_equal(g.v, o.v)
+    alive = (ocode == 3) | (ocode == -1)
+    assert np.array_equal(g.blk[alive], o.blk[alive]) and np.array_equal(g.cell[alive], o.cell[alive])
+
+
+def test_push_multistep_with_migration(gpu_ctx):
+    mesh = M.central_octant_mesh()
+    E, B, parts, sp = _case(mesh, 60000, 0.2, 0.05, seed=7)
+    st = PicState(mesh, E, B, sp, ctx=gpu_ctx)
+    st.set_particles(parts)
+    om = O.Mesh(mesh)
+    o = parts
+    for step in range(12):
+        st.MoveParticles(mover=L.MOVER_LAPENTA, bc_mode=L.BC_REFLECT, r_sphere=0.2, compact=True)
+        o, code = O.push(om, E, B, sp, o, mover=0, bc_mode=1, r_sphere=0.2)
+        o = o.take(np.flatnonzero((code == 3) | (code == -1)))
+        g, _ = st.get_particles()
+        key = g.blk.astype(np.int64) * 512 + g.cell
+        assert np.all(np.diff(key) >= 0), "not sorted by (block, cell)"
+        gs, _ = _by_id(g)
+        os_, _ = _by_id(o)
+        assert g.n == o.n, step
+        assert np.array_equal(gs.id, os_.id)
+        assert np.array_equal(gs.x, os_.x) and np.array_equal(gs.v, os_.v)
+        assert np.array_equal(gs.blk, os_.blk) and np.array_equal(gs.cell, os_.cell)
+
+
+def test_config0_workload_steps(gpu_ctx):
+    """The plain FP64 workload (2^18 of the 2^20 protons, dt from max
+    omega_c dt = 0.1) for 20 steps with reflection and absorption."""
+    mesh = M.uniform_mesh(2)
+    E, B = M.corner_fields(mesh)
+    parts = M.uniform_particles(mesh, 1 << 18, 0.01)
+    dt = 0.1 / np.max(np.linalg.norm(B, axis=-1))
+    sp = Species([1.0], [1.0], [dt])
+    st = PicState(mesh, E, B, sp, ctx=gpu_ctx)
+    st.set_particles(parts)
+    om = O.Mesh(mesh)
+    o = parts
+    for _ in range(20):
+        st.MoveParticles(mover=L.MOVER_LAPENTA, bc_mode=L.BC_REFLECT, r_sphere=0.2, compact=True)
+        o, code = O.push(om, E, B, sp, o, mover=0, bc_mode=1, r_sphere=0.2)
+        o = o.take(np.flatnonzero((code == 3) | (code == -1)))
+    g, _ = st.get_particles()
+    gs, _ = _by_id(g)
+    os_, _ = _by_id(o)
+    assert np.array_equal(gs.id, os_.id)
+    assert np.array_equal(gs.x, os_.x) and np.array_equal(gs.v, os_.v)
+    assert np.array_equal(gs.blk, os_.blk) and np.array_equal(gs.cell, os_.cell)
+
+
+@pytest.fixture(scope="module")
+def dep_case():
+    mesh = M.central_octant_mesh()
+    E, B = M.corner_fields(mesh)
+    parts = M.per_cell_particles(mesh, [4, 4], [0.05, 0.005], drift=(0.01, 0.0, 0.0))
+    dt = 0.1 / (np.max(np.linalg.norm(B, axis=-1)) / 0.01)
+    sp = Species([-1.0, 1.0], [0.01, 1.0], [dt, dt])
+    return mesh, E, B, parts, sp, dt
+
+
+@pytest.mark.parametrize("exact", [1, 0])
+@pytest.mark.parametrize("nch", [4, 2, 3, 6])
+def test_deposit_parity(gpu_ctx, dep_case, exact, nch):
+    mesh, E, B, parts, sp, dt = dep_case
+    print(f"deposit case exact={exact} nch={nch} n={parts.n}", flush=True)
+    st = PicState(mesh, E, B, sp, ctx=gpu_ctx)
+    st.set_particles(parts)
+    gpu_ctx.set_param("deposit_chunks", nch)
+    gpu_ctx.set_param("deposit_exact", exact)
+    J, Mm = st.UpdateJMassMatrix(dt)
+    g, _ = st.get_particles()  # the device order (the oracle sums in it)
+    om = O.Mesh(mesh)
+    noc, keys = om.build_nodes()
+    gnoc, gkeys = st.node_map()
+    assert np.array_equal(noc, gnoc) and np.array_equal(keys, gkeys)
+    Jo, Mo = O.deposit(om, B, sp, dt, g, noc, len(keys))
+    if exact:
+        assert np.array_equal(J, Jo) and np.array_equal(Mm, Mo)
+    else:
+        nj = np.linalg.norm(J - Jo, axis=1) / np.maximum(np.linalg.norm(Jo, axis=1), 1e-300)
+        nm = np.linalg.norm(Mm - Mo, axis=1) / np.maximum(np.linalg.norm(Mo, axis=1), 1e-300)
+        assert nj.max() <= 1e-12 and nm.max() <= 1e-12, (nj.max(), nm.max())
+    gpu_ctx.set_param("deposit_exact", 0)
+    gpu_ctx.set_param("deposit_chunks", 4)
+
+
+def test_deposit_after_push_and_migration(gpu_ctx, dep_case):
+    mesh, E, B, parts, sp, dt = dep_case
+    st = PicState(mesh, E, B, sp, ctx=gpu_ctx)
+    st.set_particles(parts)
+    for _ in range(3):
+        st.MoveParticles(mover=L.MOVER_LAPENTA, bc_mode=L.BC_REFLECT, r_sphere=0.0, compact=True)
+    J, Mm = st.UpdateJMassMatrix(dt)
+    g, _ = st.get_particles()
+    om = O.Mesh(mesh)
+    noc, keys = om.build_nodes()
+    Jo, Mo = O.deposit(om, B, sp, dt, g, noc, len(keys))
+    nm = np.linalg.norm(Mm - Mo, axis=1) / np.maximum(np.linalg.norm(Mo, axis=1), 1e-300)
+    assert nm.max() <= 1e-12
+    # J = sum over particles of q W (alpha v) / V: the total is order-independent
+    assert np.allclose(J.sum(0), Jo.sum(0), rtol=1e-12, atol=1e-12 * np.abs(Jo).sum())
