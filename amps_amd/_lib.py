@@ -104,6 +104,17 @@ def load():
         "amps_gpu_move_particles": (C.c_int, [vp, P(PushOpts)]),
         "amps_gpu_compact": (C.c_int, [vp]),
         "amps_gpu_update_j_mass_matrix": (C.c_int, [vp, dbl, P(dbl), P(dbl)]),
+        "amps_gpu_trace_batch": (C.c_int, [vp, P(TraceParams), i32, i64, P(dbl), P(dbl), P(dbl),
+                                           vp, P(i64)]),
+        "amps_gpu_cutoff_scan": (C.c_int, [vp, P(TraceParams), i32, i32, i64, P(dbl), P(dbl), i32,
+                                           P(dbl), P(i32), P(i32), P(C.c_int8), P(i32), P(i64)]),
+        "amps_gpu_trace_set_param": (C.c_int, [vp, C.c_char_p, i64]),
+        "amps_gpu_mlp_init": (C.c_int, [vp, P(C.c_float)]),
+        "amps_gpu_mlp_get_params": (C.c_int, [vp, P(C.c_float), P(i64)]),
+        "amps_gpu_mlp_set_data": (C.c_int, [vp, i64, P(C.c_float), P(C.c_float)]),
+        "amps_gpu_mlp_grad": (C.c_int, [vp, i64, P(C.c_float), P(C.c_float)]),
+        "amps_gpu_mlp_train": (C.c_int, [vp, i64, i32, dbl, P(C.c_float)]),
+        "amps_gpu_mlp_predict": (C.c_int, [vp, i64, P(C.c_float), P(C.c_float)]),
         "amps_gpu_set_timing": (C.c_int, [vp, i32]),
         "amps_gpu_get_timing": (C.c_int, [vp, P(dbl), P(i64)]),
         "amps_gpu_reset_timing": (C.c_int, [vp]),

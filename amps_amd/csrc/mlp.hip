@@ -1,8 +1,554 @@
-// mlp.hip — BF16-MFMA surrogate network (filled in below).
+// mlp.hip — the cutoff surrogate network on BF16 MFMA (gfx950).
+//
+// Network 6 -> 128 -> 128 -> 1 with GELU, BF16 weights and activations, FP32
+// accumulation (v_mfma_f32_16x16x16_bf16).  One workgroup (4 wavefronts)
+// processes 64 samples: wave w owns rows 16w..16w+15 of every activation tile.
+// Training fuses forward, loss and backward in one kernel that writes a
+// per-workgroup gradient partial; a second kernel reduces the partials in a
+// fixed order and applies Adam to the FP32 master weights, then refreshes the
+// BF16 copies (W2 is kept in both orientations so every MFMA operand is a
+// contiguous 64-bit load).  The training step loop is captured in a hipGraph;
+// batch offsets and Adam step counts live in device memory.
+//
+// Parameter layout: W1[128][6] | b1[128] | W2[128][128] | b2[128] | w3[128] | b3
 #include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <vector>
 
 #include "amps_common.h"
 
+using namespace amps;
+
+namespace {
+
+constexpr int DIN = 6, HID = 128, TB = 64;
+constexpr int OFF_W1 = 0, OFF_B1 = OFF_W1 + HID * DIN, OFF_W2 = OFF_B1 + HID, OFF_B2 = OFF_W2 + HID * HID,
+              OFF_W3 = OFF_B2 + HID, OFF_B3 = OFF_W3 + HID, NPARAM = OFF_B3 + 1;
+
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ unsigned short f2bf(float f) {
+  unsigned u = __float_as_uint(f);
+  if ((u & 0x7f800000u) == 0x7f800000u) return (unsigned short)((u >> 16) | ((u & 0xffffu) ? 0x40u : 0u));
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (unsigned short)(u >> 16);
+}
+__device__ __forceinline__ float bf2f(unsigned short h) { return __uint_as_float((unsigned)h << 16); }
+__device__ __forceinline__ float gelu(float z) { return 0.5f * z * (1.0f + erff(z * 0.70710678118654752f)); }
+__device__ __forceinline__ float gelu_grad(float z) {
+  return 0.5f * (1.0f + erff(z * 0.70710678118654752f)) + z * __expf(-0.5f * z * z) * 0.39894228040143268f;
+}
+__device__ __forceinline__ bf16x4 ld4(const unsigned short* p) { return *(const bf16x4*)p; }
+__device__ __forceinline__ f32x4 mfma(bf16x4 a, bf16x4 b, f32x4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ float sum16(float x) {  // over lanes sharing lane/16
+  x += __shfl_xor(x, 1);
+  x += __shfl_xor(x, 2);
+  x += __shfl_xor(x, 4);
+  x += __shfl_xor(x, 8);
+  return x;
+}
+__device__ __forceinline__ float sum_rows(float x) {  // over the 4 row groups (lane/16)
+  x += __shfl_xor(x, 16);
+  x += __shfl_xor(x, 32);
+  return x;
+}
+
+struct MlpView {
+  const float* theta;             // FP32 master parameters
+  const unsigned short* W1b;      // [128][16] bf16, k >= 6 zero
+  const unsigned short* W2b;      // [128][128] bf16 (n, k)
+  const unsigned short* W2Tb;     // [128][128] bf16 (k, n)
+};
+
+// Layers 1-2 for the wave's 16 rows.  z1/z2 hold pre-activations in the MFMA
+// D layout (row 16w + 4(l/16) + v, column 16 nt + l%16).
+__device__ __forceinline__ void forward_rows(const MlpView& mv, const unsigned short* Xb,
+                                             unsigned short* H1b, unsigned short* H1Tb, int w,
+                                             int l, float z1[8][4], float z2[8][4],
+                                             float h2[8][4]) {
+  const int r0 = 16 * w, rl = l & 15, kq = 4 * (l >> 4);
+  const bf16x4 ax = ld4(Xb + (r0 + rl) * 16 + kq);
+#pragma unroll
+  for (int nt = 0; nt < 8; nt++) {
+    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+    acc = mfma(ax, ld4(mv.W1b + (16 * nt + rl) * 16 + kq), acc);
+    const float b1 = mv.theta[OFF_B1 + 16 * nt + rl];
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+      z1[nt][v] = acc[v] + b1;
+      const unsigned short h = f2bf(gelu(z1[nt][v]));
+      const int m = r0 + kq + v, n = 16 * nt + rl;
+      H1b[m * HID + n] = h;
+      H1Tb[n * TB + m] = h;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int nt = 0; nt < 8; nt++) {
+    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int kt = 0; kt < 8; kt++)
+      acc = mfma(ld4(H1b + (r0 + rl) * HID + 16 * kt + kq), ld4(mv.W2b + (16 * nt + rl) * HID + 16 * kt + kq), acc);
+    const float b2 = mv.theta[OFF_B2 + 16 * nt + rl];
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+      z2[nt][v] = acc[v] + b2;
+      h2[nt][v] = bf2f(f2bf(gelu(z2[nt][v])));
+    }
+  }
+}
+
+__device__ __forceinline__ void load_x(const float* X, int64_t row0, int64_t n, unsigned short* Xb,
+                                       unsigned short* XTb) {
+  for (int i = threadIdx.x; i < TB * 16; i += blockDim.x) {
+    const int m = i >> 4, k = i & 15;
+    const int64_t r = row0 + m;
+    const float val = (k < DIN && r < n) ? X[r * DIN + k] : 0.0f;
+    const unsigned short h = f2bf(val);
+    Xb[m * 16 + k] = h;
+    XTb[k * TB + m] = h;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_mlp_forward(MlpView mv, const float* __restrict__ X, int64_t n,
+                                                      float* __restrict__ y) {
+  __shared__ unsigned short Xb[TB * 16], XTb[16 * TB], H1b[TB * HID], H1Tb[HID * TB];
+  const int64_t row0 = (int64_t)blockIdx.x * TB;
+  load_x(X, row0, n, Xb, XTb);
+  __syncthreads();
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  float z1[8][4], z2[8][4], h2[8][4];
+  forward_rows(mv, Xb, H1b, H1Tb, w, l, z1, z2, h2);
+  const float b3 = mv.theta[OFF_B3];
+#pragma unroll
+  for (int v = 0; v < 4; v++) {
+    float s = 0.f;
+#pragma unroll
+    for (int nt = 0; nt < 8; nt++) s += h2[nt][v] * mv.theta[OFF_W3 + 16 * nt + (l & 15)];
+    s = sum16(s);
+    const int64_t r = row0 + 16 * w + 4 * (l >> 4) + v;
+    if ((l & 15) == 0 && r < n) y[r] = s + b3;
+  }
+}
+
+// Fused forward + MSE loss + backward for 64 rows of the batch starting at
+// rows [*batch_off, ...) of the data (wrapping at n).  Writes this
+// workgroup's gradient partial (NPARAM floats) and loss partial.
+__global__ __launch_bounds__(256) void k_mlp_train(MlpView mv, const float* __restrict__ X,
+                                                    const float* __restrict__ T, int64_t n, int batch,
+                                                    const int64_t* __restrict__ batch_off,
+                                                    float* __restrict__ gpart,
+                                                    float* __restrict__ lpart) {
+  __shared__ unsigned short Xb[TB * 16], XTb[16 * TB], H1b[TB * HID], H1Tb[HID * TB];
+  __shared__ unsigned short D2b[TB * HID], D2Tb[HID * TB], D1Tb[HID * TB];
+  __shared__ float red[4][3][HID];
+  __shared__ float lred[4];
+  const int64_t base = (batch_off ? *batch_off : 0) + (int64_t)blockIdx.x * TB;
+  // gather rows (with wrap-around) into contiguous LDS
+  for (int i = threadIdx.x; i < TB * 16; i += blockDim.x) {
+    const int m = i >> 4, k = i & 15;
+    const int64_t r = (base + m) % n;
+    const float val = (k < DIN) ? X[r * DIN + k] : 0.0f;
+    const unsigned short h = f2bf(val);
+    Xb[m * 16 + k] = h;
+    XTb[k * TB + m] = h;
+  }
+  __syncthreads();
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63, rl = l & 15, kq = 4 * (l >> 4), r0 = 16 * w;
+  float z1[8][4], z2[8][4], h2[8][4];
+  forward_rows(mv, Xb, H1b, H1Tb, w, l, z1, z2, h2);
+  const float b3 = mv.theta[OFF_B3];
+  float dy[4];
+  float lsum = 0.f;
+#pragma unroll
+  for (int v = 0; v < 4; v++) {
+    float s = 0.f;
+#pragma unroll
+    for (int nt = 0; nt < 8; nt++) s += h2[nt][v] * mv.theta[OFF_W3 + 16 * nt + rl];
+    s = sum16(s) + b3;
+    const int64_t r = (base + r0 + kq + v) % n;
+    const float res = s - T[r];
+    dy[v] = 2.0f * res / (float)batch;
+    lsum += res * res;
+  }
+  // dZ2 = dy w3 gelu'(z2); bias and w3 gradients
+  float db2[8], dw3[8];
+#pragma unroll
+  for (int nt = 0; nt < 8; nt++) {
+    const int nn = 16 * nt + rl;
+    const float w3 = mv.theta[OFF_W3 + nn];
+    float sb = 0.f, sw = 0.f;
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+      const float d = dy[v] * w3 * gelu_grad(z2[nt][v]);
+      const unsigned short h = f2bf(d);
+      const int m = r0 + kq + v;
+      D2b[m * HID + nn] = h;
+      D2Tb[nn * TB + m] = h;
+      sb += d;
+      sw += dy[v] * h2[nt][v];
+    }
+    db2[nt] = sum_rows(sb);
+    dw3[nt] = sum_rows(sw);
+  }
+  float dysum = dy[0] + dy[1] + dy[2] + dy[3];
+  dysum = sum_rows(dysum);
+  __syncthreads();
+  // dW2[n][k] = sum_m dZ2[m][n] H1[m][k]: wave w owns n-tiles 2w, 2w+1
+#pragma unroll
+  for (int ii = 0; ii < 2; ii++) {
+    const int it = 2 * w + ii;
+#pragma unroll
+    for (int jt = 0; jt < 8; jt++) {
+      f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int ks = 0; ks < 4; ks++)
+        acc = mfma(ld4(D2Tb + (16 * it + rl) * TB + 16 * ks + kq), ld4(H1Tb + (16 * jt + rl) * TB + 16 * ks + kq), acc);
+#pragma unroll
+      for (int v = 0; v < 4; v++)
+        gpart[(int64_t)blockIdx.x * NPARAM + OFF_W2 + (16 * it + kq + v) * HID + 16 * jt + rl] = acc[v];
+    }
+  }
+  // dH1 = dZ2 W2 ; dZ1 = dH1 gelu'(z1)
+  float db1[8];
+#pragma unroll
+  for (int jt = 0; jt < 8; jt++) {
+    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int ks = 0; ks < 8; ks++)
+      acc = mfma(ld4(D2b + (r0 + rl) * HID + 16 * ks + kq), ld4(mv.W2Tb + (16 * jt + rl) * HID + 16 * ks + kq), acc);
+    float sb = 0.f;
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+      const float d = acc[v] * gelu_grad(z1[jt][v]);
+      D1Tb[(16 * jt + rl) * TB + r0 + kq + v] = f2bf(d);
+      sb += d;
+    }
+    db1[jt] = sum_rows(sb);
+  }
+  // per-wave vector partials, reduced in a fixed order below
+  if (l < 16) {
+#pragma unroll
+    for (int nt = 0; nt < 8; nt++) {
+      red[w][0][16 * nt + l] = db1[nt];
+      red[w][1][16 * nt + l] = db2[nt];
+      red[w][2][16 * nt + l] = dw3[nt];
+    }
+  }
+  const float ls = sum_rows(lsum);
+  if (l == 0) lred[w] = ls;
+  __syncthreads();
+  // dW1[n][j] = sum_m dZ1[m][n] X[m][j]: wave w owns n-tiles 2w, 2w+1
+#pragma unroll
+  for (int ii = 0; ii < 2; ii++) {
+    const int it = 2 * w + ii;
+    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int ks = 0; ks < 4; ks++)
+      acc = mfma(ld4(D1Tb + (16 * it + rl) * TB + 16 * ks + kq), ld4(XTb + rl * TB + 16 * ks + kq), acc);
+    if (rl < DIN) {
+#pragma unroll
+      for (int v = 0; v < 4; v++)
+        gpart[(int64_t)blockIdx.x * NPARAM + OFF_W1 + (16 * it + kq + v) * DIN + rl] = acc[v];
+    }
+  }
+  for (int i = threadIdx.x; i < HID; i += blockDim.x) {
+    float* g = gpart + (int64_t)blockIdx.x * NPARAM;
+    g[OFF_B1 + i] = ((red[0][0][i] + red[1][0][i]) + red[2][0][i]) + red[3][0][i];
+    g[OFF_B2 + i] = ((red[0][1][i] + red[1][1][i]) + red[2][1][i]) + red[3][1][i];
+    g[OFF_W3 + i] = ((red[0][2][i] + red[1][2][i]) + red[2][2][i]) + red[3][2][i];
+  }
+  if (threadIdx.x == 0) {
+    // b3: every wave's lanes hold the wave sum of dy; take it from red via lred path
+    lpart[blockIdx.x] = ((lred[0] + lred[1]) + lred[2]) + lred[3];
+  }
+  __shared__ float dys[4];
+  if (l == 0) dys[w] = dysum;
+  __syncthreads();
+  if (threadIdx.x == 0)
+    gpart[(int64_t)blockIdx.x * NPARAM + OFF_B3] = ((dys[0] + dys[1]) + dys[2]) + dys[3];
+}
+
+// Sum the workgroup partials in order, Adam on the FP32 master copy, BF16
+// refresh.  step_dev counts Adam steps (incremented by thread 0 after use via
+// a separate tiny kernel so every thread reads the same value).
+__global__ __launch_bounds__(256) void k_mlp_adam(const float* __restrict__ gpart, int nwg,
+                                                   const float* __restrict__ lpart, float* theta,
+                                                   float* m, float* v, float* grad_out,
+                                                   const int64_t* __restrict__ step_dev, float lr,
+                                                   int apply, unsigned short* W1b,
+                                                   unsigned short* W2b, unsigned short* W2Tb) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= NPARAM) return;
+  float g = 0.f;
+  for (int k = 0; k < nwg; k++) g += gpart[(int64_t)k * NPARAM + j];
+  if (grad_out) grad_out[j] = g;
+  if (!apply) return;
+  const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+  const int64_t step = *step_dev + 1;
+  const float mj = b1 * m[j] + (1.f - b1) * g;
+  const float vj = b2 * v[j] + (1.f - b2) * g * g;
+  m[j] = mj;
+  v[j] = vj;
+  const float mh = mj / (1.f - powf(b1, (float)step));
+  const float vh = vj / (1.f - powf(b2, (float)step));
+  const float th = theta[j] - lr * mh / (sqrtf(vh) + eps);
+  theta[j] = th;
+  const unsigned short hb = f2bf(th);
+  if (j < OFF_B1) {
+    W1b[(j / DIN) * 16 + (j % DIN)] = hb;
+  } else if (j >= OFF_W2 && j < OFF_B2) {
+    const int q = j - OFF_W2, nn = q / HID, kk = q % HID;
+    W2b[nn * HID + kk] = hb;
+    W2Tb[kk * HID + nn] = hb;
+  }
+}
+
+__global__ void k_mlp_advance(int64_t* step_dev, int64_t* batch_off, int64_t batch, int64_t n) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    *step_dev += 1;
+    *batch_off = (*batch_off + batch) % n;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_mlp_refresh_bf16(const float* theta, unsigned short* W1b,
+                                                           unsigned short* W2b, unsigned short* W2Tb) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < HID * 16) {
+    const int nn = j / 16, kk = j % 16;
+    W1b[j] = (kk < DIN) ? f2bf(theta[OFF_W1 + nn * DIN + kk]) : (unsigned short)0;
+  }
+  if (j < HID * HID) {
+    const int nn = j / HID, kk = j % HID;
+    const unsigned short h = f2bf(theta[OFF_W2 + j]);
+    W2b[j] = h;
+    W2Tb[kk * HID + nn] = h;
+  }
+}
+
+}  // namespace
+
+// =====================================================================
+// host side
+// =====================================================================
+struct MlpState {
+  float* theta = nullptr;
+  float* m = nullptr;
+  float* v = nullptr;
+  unsigned short *W1b = nullptr, *W2b = nullptr, *W2Tb = nullptr;
+  float *X = nullptr, *T = nullptr;
+  int64_t n = 0, cap = 0;
+  float* gpart = nullptr;
+  float* lpart = nullptr;
+  int64_t gpart_wg = 0;
+  int64_t* step_dev = nullptr;   // [0] Adam step count, [1] batch offset
+  float* grad = nullptr;
+  float* qX = nullptr;
+  float* qy = nullptr;
+  int64_t qcap = 0;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  int graph_batch = 0;
+  float graph_lr = 0.f;
+};
+
 namespace amps {
-void mlp_free(amps_ctx*) {}
+void mlp_free(amps_ctx* c) {
+  MlpState* s = (MlpState*)c->mlp_state;
+  if (!s) return;
+  if (s->gexec) (void)hipGraphExecDestroy(s->gexec);
+  if (s->graph) (void)hipGraphDestroy(s->graph);
+  for (void* p : {(void*)s->theta, (void*)s->m, (void*)s->v, (void*)s->W1b, (void*)s->W2b,
+                  (void*)s->W2Tb, (void*)s->X, (void*)s->T, (void*)s->gpart, (void*)s->lpart,
+                  (void*)s->step_dev, (void*)s->grad, (void*)s->qX, (void*)s->qy})
+    (void)hipFree(p);
+  delete s;
+  c->mlp_state = nullptr;
+}
 }  // namespace amps
+
+static MlpView view(MlpState* s) { return MlpView{s->theta, s->W1b, s->W2b, s->W2Tb}; }
+
+static int ensure_gpart(amps_ctx* c, MlpState* s, int64_t nwg) {
+  if (nwg <= s->gpart_wg) return 0;
+  (void)hipFree(s->gpart);
+  (void)hipFree(s->lpart);
+  s->gpart = nullptr;
+  s->lpart = nullptr;
+  HIP_TRY(hipMalloc(&s->gpart, sizeof(float) * NPARAM * nwg));
+  HIP_TRY(hipMalloc(&s->lpart, sizeof(float) * nwg));
+  s->gpart_wg = nwg;
+  (void)c;
+  return 0;
+}
+
+extern "C" int amps_gpu_mlp_init(amps_ctx* c, const float* theta) {
+  if (!c || !theta) return fail(AMPS_E_ARG, "mlp_init: null argument");
+  (void)hipSetDevice(c->device);
+  MlpState* s = (MlpState*)c->mlp_state;
+  if (!s) {
+    s = new MlpState();
+    c->mlp_state = s;
+    HIP_TRY(hipMalloc(&s->theta, sizeof(float) * NPARAM));
+    HIP_TRY(hipMalloc(&s->m, sizeof(float) * NPARAM));
+    HIP_TRY(hipMalloc(&s->v, sizeof(float) * NPARAM));
+    HIP_TRY(hipMalloc(&s->grad, sizeof(float) * NPARAM));
+    HIP_TRY(hipMalloc(&s->W1b, 2 * HID * 16));
+    HIP_TRY(hipMalloc(&s->W2b, 2 * HID * HID));
+    HIP_TRY(hipMalloc(&s->W2Tb, 2 * HID * HID));
+    HIP_TRY(hipMalloc(&s->step_dev, sizeof(int64_t) * 2));
+  }
+  HIP_TRY(hipMemcpyAsync(s->theta, theta, sizeof(float) * NPARAM, hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(hipMemsetAsync(s->m, 0, sizeof(float) * NPARAM, c->stream));
+  HIP_TRY(hipMemsetAsync(s->v, 0, sizeof(float) * NPARAM, c->stream));
+  HIP_TRY(hipMemsetAsync(s->step_dev, 0, sizeof(int64_t) * 2, c->stream));
+  hipLaunchKernelGGL(k_mlp_refresh_bf16, dim3((HID * HID + 255) / 256), dim3(256), 0, c->stream, s->theta,
+                     s->W1b, s->W2b, s->W2Tb);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return AMPS_OK;
+}
+
+static MlpState* mstate(amps_ctx* c) { return c ? (MlpState*)c->mlp_state : nullptr; }
+
+extern "C" int amps_gpu_mlp_get_params(amps_ctx* c, float* theta, int64_t* step) {
+  MlpState* s = mstate(c);
+  if (!s) return fail(AMPS_E_STATE, "mlp: call amps_gpu_mlp_init first");
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  if (theta) HIP_TRY(hipMemcpy(theta, s->theta, sizeof(float) * NPARAM, hipMemcpyDeviceToHost));
+  if (step) HIP_TRY(hipMemcpy(step, s->step_dev, sizeof(int64_t), hipMemcpyDeviceToHost));
+  return AMPS_OK;
+}
+
+extern "C" int amps_gpu_mlp_set_data(amps_ctx* c, int64_t n, const float* X, const float* T) {
+  MlpState* s = mstate(c);
+  if (!s) return fail(AMPS_E_STATE, "mlp: call amps_gpu_mlp_init first");
+  if (n <= 0 || !X || !T) return fail(AMPS_E_ARG, "mlp_set_data: empty data");
+  if (n > s->cap) {
+    (void)hipFree(s->X);
+    (void)hipFree(s->T);
+    s->X = nullptr;
+    s->T = nullptr;
+    HIP_TRY(hipMalloc(&s->X, sizeof(float) * DIN * n));
+    HIP_TRY(hipMalloc(&s->T, sizeof(float) * n));
+    s->cap = n;
+  }
+  HIP_TRY(hipMemcpyAsync(s->X, X, sizeof(float) * DIN * n, hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(hipMemcpyAsync(s->T, T, sizeof(float) * n, hipMemcpyHostToDevice, c->stream));
+  const int64_t zero = 0;
+  HIP_TRY(hipMemcpyAsync(s->step_dev + 1, &zero, sizeof(int64_t), hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  s->n = n;
+  return AMPS_OK;
+}
+
+static void enqueue_step(amps_ctx* c, MlpState* s, int batch, float lr) {
+  const int nwg = batch / TB;
+  hipLaunchKernelGGL(k_mlp_train, dim3(nwg), dim3(256), 0, c->stream, view(s), s->X, s->T, s->n,
+                     batch, s->step_dev + 1, s->gpart, s->lpart);
+  hipLaunchKernelGGL(k_mlp_adam, dim3((NPARAM + 255) / 256), dim3(256), 0, c->stream, s->gpart, nwg,
+                     s->lpart, s->theta, s->m, s->v, (float*)nullptr, s->step_dev, lr, 1, s->W1b,
+                     s->W2b, s->W2Tb);
+  hipLaunchKernelGGL(k_mlp_advance, dim3(1), dim3(64), 0, c->stream, s->step_dev, s->step_dev + 1,
+                     (int64_t)batch, s->n);
+}
+
+// Gradient of the mean-squared error over the first n rows of the resident
+// data (n a multiple of 64), optionally followed by one Adam step.
+extern "C" int amps_gpu_mlp_grad(amps_ctx* c, int64_t n, float* grad_out, float* loss_out) {
+  MlpState* s = mstate(c);
+  if (!s || s->n == 0) return fail(AMPS_E_STATE, "mlp_grad: init and set_data first");
+  if (n <= 0 || n % TB || n > s->n || n > (1 << 24)) return fail(AMPS_E_ARG, "mlp_grad: n must be a positive multiple of 64 <= data size");
+  const int nwg = (int)(n / TB);
+  if (int r = ensure_gpart(c, s, nwg)) return r;
+  int64_t zero = 0;
+  HIP_TRY(hipMemcpyAsync(s->step_dev + 1, &zero, sizeof(int64_t), hipMemcpyHostToDevice, c->stream));
+  hipLaunchKernelGGL(k_mlp_train, dim3(nwg), dim3(256), 0, c->stream, view(s), s->X, s->T, s->n, (int)n,
+                     s->step_dev + 1, s->gpart, s->lpart);
+  hipLaunchKernelGGL(k_mlp_adam, dim3((NPARAM + 255) / 256), dim3(256), 0, c->stream, s->gpart, nwg,
+                     s->lpart, s->theta, s->m, s->v, s->grad, s->step_dev, 0.f, 0, s->W1b, s->W2b,
+                     s->W2Tb);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  if (grad_out) HIP_TRY(hipMemcpy(grad_out, s->grad, sizeof(float) * NPARAM, hipMemcpyDeviceToHost));
+  if (loss_out) {
+    std::vector<float> lp((size_t)nwg);
+    HIP_TRY(hipMemcpy(lp.data(), s->lpart, sizeof(float) * nwg, hipMemcpyDeviceToHost));
+    double L = 0;
+    for (float x : lp) L += x;
+    *loss_out = (float)(L / (double)n);
+  }
+  return AMPS_OK;
+}
+
+// nsteps Adam steps on consecutive batches of the resident data (wrapping),
+// replayed from one captured hipGraph.  loss_out[nsteps] (may be NULL)
+// receives each step's batch loss.
+extern "C" int amps_gpu_mlp_train(amps_ctx* c, int64_t nsteps, int32_t batch, double lr, float* loss_out) {
+  MlpState* s = mstate(c);
+  if (!s || s->n == 0) return fail(AMPS_E_STATE, "mlp_train: init and set_data first");
+  if (batch <= 0 || batch % TB || batch > (1 << 20)) return fail(AMPS_E_ARG, "mlp_train: batch must be a positive multiple of 64");
+  if (nsteps < 0) return fail(AMPS_E_ARG, "mlp_train: nsteps < 0");
+  (void)hipSetDevice(c->device);
+  const int nwg = batch / TB;
+  if (int r = ensure_gpart(c, s, nwg)) return r;
+  if (!s->gexec || s->graph_batch != batch || s->graph_lr != (float)lr) {
+    if (s->gexec) (void)hipGraphExecDestroy(s->gexec);
+    if (s->graph) (void)hipGraphDestroy(s->graph);
+    s->gexec = nullptr;
+    s->graph = nullptr;
+    HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    enqueue_step(c, s, batch, (float)lr);
+    HIP_TRY(hipStreamEndCapture(c->stream, &s->graph));
+    HIP_TRY(hipGraphInstantiate(&s->gexec, s->graph, nullptr, nullptr, 0));
+    s->graph_batch = batch;
+    s->graph_lr = (float)lr;
+  }
+  std::vector<float> lp;
+  tbegin(c);
+  for (int64_t i = 0; i < nsteps; i++) {
+    HIP_TRY(hipGraphLaunch(s->gexec, c->stream));
+    if (loss_out) {
+      lp.resize((size_t)nwg);
+      HIP_TRY(hipMemcpyAsync(lp.data(), s->lpart, sizeof(float) * nwg, hipMemcpyDeviceToHost, c->stream));
+      HIP_TRY(hipStreamSynchronize(c->stream));
+      double L = 0;
+      for (float x : lp) L += x;
+      loss_out[i] = (float)(L / batch);
+    }
+  }
+  tend(c, AMPS_K_MLP);
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return AMPS_OK;
+}
+
+extern "C" int amps_gpu_mlp_predict(amps_ctx* c, int64_t n, const float* X, float* y) {
+  MlpState* s = mstate(c);
+  if (!s) return fail(AMPS_E_STATE, "mlp: call amps_gpu_mlp_init first");
+  if (n < 0 || (n > 0 && (!X || !y))) return fail(AMPS_E_ARG, "mlp_predict: bad arguments");
+  if (n == 0) return AMPS_OK;
+  (void)hipSetDevice(c->device);
+  if (n > s->qcap) {
+    (void)hipFree(s->qX);
+    (void)hipFree(s->qy);
+    s->qX = nullptr;
+    s->qy = nullptr;
+    HIP_TRY(hipMalloc(&s->qX, sizeof(float) * DIN * n));
+    HIP_TRY(hipMalloc(&s->qy, sizeof(float) * n));
+    s->qcap = n;
+  }
+  HIP_TRY(hipMemcpyAsync(s->qX, X, sizeof(float) * DIN * n, hipMemcpyHostToDevice, c->stream));
+  const int64_t nwg = (n + TB - 1) / TB;
+  tbegin(c);
+  hipLaunchKernelGGL(k_mlp_forward, dim3((unsigned)nwg), dim3(256), 0, c->stream, view(s), s->qX, n, s->qy);
+  HIP_TRY(hipGetLastError());
+  tend(c, AMPS_K_MLP);
+  HIP_TRY(hipMemcpyAsync(y, s->qy, sizeof(float) * n, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return AMPS_OK;
+}

@@ -1,8 +1,773 @@
-// trace.hip — persistent work-queue cutoff tracer (filled in below).
+// trace.hip — batched backward cutoff tracing in one persistent kernel.
+//
+// Each lane owns one trajectory at a time.  When a lane's trajectory ends it
+// writes its result and takes the next work item from a device queue: the
+// wavefront ballots the idle lanes, one lane reserves that many items with a
+// single atomicAdd, and mbcnt ranks hand them out.  Trajectories of very
+// different length (tens to 1e5 steps) therefore never leave lanes idle
+// waiting for the longest trajectory of a static batch.
+//
+// FP64 path: TraceTrajectoryImpl + TraceTrajectoryWithSingleRetry
+// (CutoffRigidityGridless.cpp:1246-1496) with BorisStep/StepParticleChecked
+// (GridlessParticleMovers.cpp:781-801, 1035-1046) in SI units, expression for
+// expression, so it reproduces the CPU restatement bit for bit.
+// FP32 path: the same loop in units of (Re, Re/c, m0 c) with FP32 field,
+// momentum and rotation arithmetic; positions, time and path length are
+// accumulated in FP64.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
 #include "amps_common.h"
+#include "trace_common.h"
+
+using namespace amps;
+
+namespace {
+
+constexpr double C_LIGHT = 299792458.0;  // SpeedOfLight, constants.h:40
+
+__device__ __forceinline__ double dmin(double a, double b) { return (b < a) ? b : a; }
+__device__ __forceinline__ double dmax(double a, double b) { return (a < b) ? b : a; }
+__device__ __forceinline__ double dot3(const double* a, const double* b) {
+  return a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
+}
+__device__ __forceinline__ bool fin3(const double* a) {
+  return isfinite(a[0]) && isfinite(a[1]) && isfinite(a[2]);
+}
+
+// ---------------------------------------------------------------------
+// FP64 stepper (SI units)
+// ---------------------------------------------------------------------
+struct Fp64Lane {
+  double x[3], p[3], t, s, dtTrace, maxTime;
+  int64_t n, maxSteps;
+  int retry;
+};
+
+// Dipole::GetB_Tesla (DipoleInterface.h:99-125) plus a uniform field
+__device__ __forceinline__ void dipole64(const amps_trace_params& P, const double* xm, double* B) {
+  const double mu0_over_4pi = 1.0e-7;
+  const double x = xm[0], y = xm[1], z = xm[2];
+  const double r2 = x * x + y * y + z * z;
+  if (r2 <= 0.0) {
+    B[0] = B[1] = B[2] = 0.0;
+  } else {
+    const double r = sqrt(r2), r3 = r2 * r, r5 = r3 * r2;
+    const double M = P.moment_Am2;
+    const double mx = M * P.m_hat[0], my = M * P.m_hat[1], mz = M * P.m_hat[2];
+    const double mdotr = mx * x + my * y + mz * z;
+    B[0] = mu0_over_4pi * (3.0 * x * mdotr / r5 - mx / r3);
+    B[1] = mu0_over_4pi * (3.0 * y * mdotr / r5 - my / r3);
+    B[2] = mu0_over_4pi * (3.0 * z * mdotr / r5 - mz / r3);
+  }
+  B[0] = B[0] + P.B_ext_T[0];
+  B[1] = B[1] + P.B_ext_T[1];
+  B[2] = B[2] + P.B_ext_T[2];
+}
+
+__device__ __forceinline__ double gamma64(const double* p, double m0) {
+  const double p2 = dot3(p, p);
+  const double mc = m0 * C_LIGHT;
+  return sqrt(1.0 + p2 / (mc * mc));
+}
+
+// BorisStep, GridlessParticleMovers.cpp:781-801
+__device__ __forceinline__ void boris64(const amps_trace_params& P, double* x, double* p, double dt) {
+  double B[3];
+  dipole64(P, x, B);
+  const double m0 = P.m0_kg, q = P.q_C;
+  const double gam = gamma64(p, m0);
+  const double iv = 1.0 / (gam * m0);
+  const double vv[3] = {iv * p[0], iv * p[1], iv * p[2]};
+  const double h = 0.5 * dt;
+#pragma unroll
+  for (int d = 0; d < 3; d++) x[d] = x[d] + h * vv[d];
+  const double tc = (q * dt) / (2.0 * gam * m0);
+  const double t[3] = {tc * B[0], tc * B[1], tc * B[2]};
+  const double t2 = dot3(t, t);
+  const double sc = 2.0 / (1.0 + t2);
+  const double s[3] = {sc * t[0], sc * t[1], sc * t[2]};
+  const double pm[3] = {p[0], p[1], p[2]};
+  const double pp[3] = {pm[0] + (pm[1] * t[2] - pm[2] * t[1]), pm[1] + (pm[2] * t[0] - pm[0] * t[2]),
+                        pm[2] + (pm[0] * t[1] - pm[1] * t[0])};
+  p[0] = pm[0] + (pp[1] * s[2] - pp[2] * s[1]);
+  p[1] = pm[1] + (pp[2] * s[0] - pp[0] * s[2]);
+  p[2] = pm[2] + (pp[0] * s[1] - pp[1] * s[0]);
+  const double gam2 = gamma64(p, m0);
+  const double iv2 = 1.0 / (gam2 * m0);
+  const double v2[3] = {iv2 * p[0], iv2 * p[1], iv2 * p[2]};
+#pragma unroll
+  for (int d = 0; d < 3; d++) x[d] = x[d] + h * v2[d];
+}
+
+__device__ __forceinline__ bool inside_inner_m(const double* x, double r) {
+  return (r > 0.0) && (dot3(x, x) <= r * r);
+}
+// SegmentHitsInnerSphere, GridlessParticleMovers.cpp:136-149
+__device__ __forceinline__ bool seg_hits(const double* a, const double* b, double r) {
+  if (r <= 0.0) return false;
+  if (inside_inner_m(a, r) || inside_inner_m(b, r)) return true;
+  const double d[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]};
+  const double dd = dot3(d, d);
+  if (dd <= 0.0) return false;
+  double t = -dot3(a, d) / dd;
+  if (t < 0.0) t = 0.0;
+  if (t > 1.0) t = 1.0;
+  const double q[3] = {a[0] + t * d[0], a[1] + t * d[1], a[2] + t * d[2]};
+  return dot3(q, q) <= r * r;
+}
+
+// SelectLegacyCutoffTraceDt, CutoffRigidityGridless.cpp:1086-1152 (switchable parts)
+__device__ __forceinline__ double select_dt64(const amps_trace_params& P, const double* x,
+                                              const double* p, double dtTrace, double timeRemaining) {
+  double dt = dtTrace;
+  if (timeRemaining < dt) dt = timeRemaining;
+  if (!P.adaptive) return dt;
+  const double m0 = P.m0_kg;
+  const double p2 = dot3(p, p);
+  const double mc = m0 * C_LIGHT;
+  const double gamma = sqrt(1.0 + p2 / (mc * mc));
+  const double pMag = sqrt(dmax(0.0, p2));
+  const double vMag = (gamma > 0.0 && m0 > 0.0) ? pMag / (gamma * m0) : 0.0;
+  double B[3];
+  dipole64(P, x, B);
+  const double Bmag = sqrt(dot3(B, B));
+  if (Bmag > 0.0) {
+    const double omega = fabs(P.q_C) * Bmag / (gamma * m0);
+    if (omega > 0.0) dt = dmin(dt, P.dphi_max / omega);
+  }
+  const double vFor = vMag;
+  if (P.dist_limiter) {
+    const double Re = P.earth_radius_m;
+    const double xRe[3] = {x[0] / Re, x[1] / Re, x[2] / Re};
+    const double rRe = sqrt(xRe[0] * xRe[0] + xRe[1] * xRe[1] + xRe[2] * xRe[2]);
+    double dInner = 1.0e300;
+    if (rRe > P.r_inner_Re) dInner = (rRe - P.r_inner_Re) * Re;
+    double dBox = 1.0e300;
+    if (P.outer_sphere) {
+      if (rRe < P.r_outer_Re) dBox = (P.r_outer_Re - rRe) * Re;
+    } else if (xRe[0] >= P.box_min_Re[0] && xRe[0] <= P.box_max_Re[0] && xRe[1] >= P.box_min_Re[1] &&
+               xRe[1] <= P.box_max_Re[1] && xRe[2] >= P.box_min_Re[2] && xRe[2] <= P.box_max_Re[2]) {
+      const double dxp = (P.box_max_Re[0] - xRe[0]) * Re, dxm = (xRe[0] - P.box_min_Re[0]) * Re;
+      const double dyp = (P.box_max_Re[1] - xRe[1]) * Re, dym = (xRe[1] - P.box_min_Re[1]) * Re;
+      const double dzp = (P.box_max_Re[2] - xRe[2]) * Re, dzm = (xRe[2] - P.box_min_Re[2]) * Re;
+      dBox = dmin(dmin(dmin(dxp, dxm), dmin(dyp, dym)), dmin(dzp, dzm));
+    }
+    const double dNear = dmin(dInner, dBox);
+    if (vFor > 0.0 && dNear < 1.0e299) dt = dmin(dt, 0.20 * dNear / vFor);
+  }
+  if (P.dt_floor) {
+    const double safeSpeed = dmax(vFor, 1.0e-12);
+    const double dtFloor = dmax(dmax(1.0e-12, 1.0e-9 * dmax(dtTrace, 1.0)), 100.0e3 / safeSpeed);
+    dt = dmax(dtFloor, dt);
+  }
+  if (timeRemaining > 0.0) dt = dmin(dt, timeRemaining);
+  return dt;
+}
+
+struct Fp64Stepper {
+  using Lane = Fp64Lane;
+  static __device__ __forceinline__ void init(const TraceView& a, Lane& L, const double* x0,
+                                              const double* dir, double R, int attempt) {
+    const amps_trace_params& P = a.P;
+    const double pMag = (R * 1.0e9 * P.qabs_C) / C_LIGHT;
+    const double v0[3] = {-1.0 * dir[0], -1.0 * dir[1], -1.0 * dir[2]};
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+      L.p[d] = pMag * v0[d];
+      L.x[d] = x0[d];
+    }
+    L.t = 0.0;
+    L.s = 0.0;
+    L.n = 0;
+    L.retry = attempt;
+    if (attempt == 0) {
+      L.dtTrace = P.dt_trace_s;
+      L.maxSteps = P.max_steps;
+      L.maxTime = P.max_time_s;
+    } else {
+      L.dtTrace = dmax(1.0e-12, 0.5 * P.dt_trace_s);
+      L.maxSteps = (P.max_steps <= (int64_t)2147483647 / 2) ? 2 * P.max_steps : (int64_t)2147483647;
+      L.maxTime = 2.0 * P.max_time_s;
+    }
+  }
+  // one iteration of the trace loop; returns a termination code or -1
+  static __device__ __forceinline__ int step(const TraceView& a, Lane& L) {
+    const amps_trace_params& P = a.P;
+    if (!(L.n < L.maxSteps && L.t < L.maxTime && (P.max_distance_m <= 0.0 || L.s < P.max_distance_m))) {
+      if (L.n >= L.maxSteps) return 4;
+      if (P.max_distance_m > 0.0 && L.s >= P.max_distance_m) return 5;
+      if (L.t >= L.maxTime) return 3;
+      return 8;
+    }
+    double* x = L.x;
+    double* p = L.p;
+    if (!fin3(x) || !fin3(p)) return 8;
+    const double rIn = a.r_inner_m;
+    if (rIn > 0.0 && x[0] * x[0] + x[1] * x[1] + x[2] * x[2] <= dmax(0.0, rIn + 0.0) * dmax(0.0, rIn + 0.0))
+      return 1;
+    bool inside;
+    if (P.outer_sphere)
+      inside = (x[0] * x[0] + x[1] * x[1] + x[2] * x[2] <= a.r_outer_m * a.r_outer_m);
+    else
+      inside = x[0] >= a.box_min_m[0] - 0.0 && x[0] <= a.box_max_m[0] + 0.0 && x[1] >= a.box_min_m[1] - 0.0 &&
+               x[1] <= a.box_max_m[1] + 0.0 && x[2] >= a.box_min_m[2] - 0.0 && x[2] <= a.box_max_m[2] + 0.0;
+    if (!inside) return 0;
+    const double dt = select_dt64(P, x, p, L.dtTrace, L.maxTime - L.t);
+    if (!(dt > 0.0) || !isfinite(dt)) return 6;
+    const double xb[3] = {x[0], x[1], x[2]};
+    boris64(P, x, p, dt);
+    if (seg_hits(xb, x, rIn) || inside_inner_m(x, rIn)) return 1;
+    if (!fin3(x) || !fin3(p)) return 8;
+    const double dd[3] = {x[0] - xb[0], x[1] - xb[1], x[2] - xb[2]};
+    const double seg = sqrt(dot3(dd, dd));
+    L.s += isfinite(seg) ? seg : 0.0;
+    L.t += dt;
+    ++L.n;
+    return -1;
+  }
+  static __device__ __forceinline__ void result(const TraceView& a, const Lane& L, int term,
+                                                amps_trace_result& r) {
+    r.termination = term;
+    r.retry_count = L.retry;
+    r.steps = L.n;
+    r.time_s = L.t;
+    r.distance_m = L.s;
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+      r.x[d] = L.x[d];
+      r.p[d] = L.p[d];
+    }
+  }
+};
+
+// ---------------------------------------------------------------------
+// FP32 stepper: units of length Re, time Re/c, momentum m0 c.  FP32 field,
+// momentum and rotation; FP64 position/time/path accumulation.
+// ---------------------------------------------------------------------
+struct Fp32Lane {
+  double x[3];
+  float u[3];
+  double t, s, dtTrace, maxTime;
+  int64_t n, maxSteps;
+  int retry;
+};
+
+__device__ __forceinline__ void bfield32(const TraceView& a, const double* xd, float* b) {
+#pragma clang fp contract(fast)
+  const float x = (float)xd[0], y = (float)xd[1], z = (float)xd[2];
+  const float r2 = x * x + y * y + z * z;
+  const float ir = rsqrtf(r2);
+  const float ir2 = ir * ir;
+  const float ir3 = ir2 * ir;
+  const float mdr = a.f_mhat[0] * x + a.f_mhat[1] * y + a.f_mhat[2] * z;
+  const float k5 = 3.0f * mdr * ir3 * ir2;
+  b[0] = a.f_cb * (k5 * x - a.f_mhat[0] * ir3) + a.f_bext[0];
+  b[1] = a.f_cb * (k5 * y - a.f_mhat[1] * ir3) + a.f_bext[1];
+  b[2] = a.f_cb * (k5 * z - a.f_mhat[2] * ir3) + a.f_bext[2];
+}
+
+struct Fp32Stepper {
+  using Lane = Fp32Lane;
+  static __device__ __forceinline__ void init(const TraceView& a, Lane& L, const double* x0,
+                                              const double* dir, double R, int attempt) {
+    const amps_trace_params& P = a.P;
+    // |u| = p/(m0 c) with p = R*1e9*|q|/c
+    const double umag = (R * 1.0e9 * P.qabs_C) / C_LIGHT / (P.m0_kg * C_LIGHT);
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+      L.u[d] = (float)(umag * (-1.0 * dir[d]));
+      L.x[d] = x0[d] / a.len;
+    }
+    L.t = 0.0;
+    L.s = 0.0;
+    L.n = 0;
+    L.retry = attempt;
+    const double T = a.len / C_LIGHT;
+    if (attempt == 0) {
+      L.dtTrace = P.dt_trace_s / T;
+      L.maxSteps = P.max_steps;
+      L.maxTime = P.max_time_s / T;
+    } else {
+      L.dtTrace = dmax(1.0e-12, 0.5 * P.dt_trace_s) / T;
+      L.maxSteps = (P.max_steps <= (int64_t)2147483647 / 2) ? 2 * P.max_steps : (int64_t)2147483647;
+      L.maxTime = 2.0 * P.max_time_s / T;
+    }
+  }
+  static __device__ __forceinline__ int step(const TraceView& a, Lane& L) {
+#pragma clang fp contract(fast)
+    const amps_trace_params& P = a.P;
+    if (!(L.n < L.maxSteps && L.t < L.maxTime && (a.maxdist_n <= 0.0 || L.s < a.maxdist_n))) {
+      if (L.n >= L.maxSteps) return 4;
+      if (a.maxdist_n > 0.0 && L.s >= a.maxdist_n) return 5;
+      if (L.t >= L.maxTime) return 3;
+      return 8;
+    }
+    double* x = L.x;
+    float* u = L.u;
+    if (!fin3(x) || !isfinite(u[0]) || !isfinite(u[1]) || !isfinite(u[2])) return 8;
+    const double r2 = x[0] * x[0] + x[1] * x[1] + x[2] * x[2];
+    if (a.rin_n > 0.0 && r2 <= a.rin_n * a.rin_n) return 1;
+    bool inside;
+    if (P.outer_sphere)
+      inside = r2 <= a.rout_n * a.rout_n;
+    else
+      inside = x[0] >= P.box_min_Re[0] && x[0] <= P.box_max_Re[0] && x[1] >= P.box_min_Re[1] &&
+               x[1] <= P.box_max_Re[1] && x[2] >= P.box_min_Re[2] && x[2] <= P.box_max_Re[2];
+    if (!inside) return 0;
+    // time step (normalized)
+    const float u2 = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+    const float gam = sqrtf(1.0f + u2);
+    const float igam = 1.0f / gam;
+    const float vmag = sqrtf(u2) * igam;
+    float b[3];
+    bfield32(a, x, b);
+    double dt = L.dtTrace;
+    const double trem = L.maxTime - L.t;
+    if (trem < dt) dt = trem;
+    if (P.adaptive) {
+      const float bm = sqrtf(b[0] * b[0] + b[1] * b[1] + b[2] * b[2]);
+      if (bm > 0.0f) {
+        const float omega = bm * igam;
+        dt = dmin(dt, (double)(a.f_dphi / omega));
+      }
+      if (P.dist_limiter) {
+        const double r = sqrt(r2);
+        double dIn = 1.0e300, dOut = 1.0e300;
+        if (r > a.rin_n) dIn = r - a.rin_n;
+        if (P.outer_sphere) {
+          if (r < a.rout_n) dOut = a.rout_n - r;
+        } else if (inside) {
+          dOut = dmin(dmin(dmin(P.box_max_Re[0] - x[0], x[0] - P.box_min_Re[0]),
+                           dmin(P.box_max_Re[1] - x[1], x[1] - P.box_min_Re[1])),
+                      dmin(P.box_max_Re[2] - x[2], x[2] - P.box_min_Re[2]));
+        }
+        const double dn = dmin(dIn, dOut);
+        if (vmag > 0.0f && dn < 1.0e299) dt = dmin(dt, 0.20 * dn / (double)vmag);
+      }
+      if (P.dt_floor) {
+        const double sv = dmax((double)vmag, 1.0e-12 / C_LIGHT);
+        const double fl = dmax(dmax(a.floor_abs_n, a.floor_rel_n), a.floor_dist_n / sv);
+        dt = dmax(fl, dt);
+      }
+      if (trem > 0.0) dt = dmin(dt, trem);
+    }
+    if (!(dt > 0.0) || !isfinite(dt)) return 6;
+    const float dtf = (float)dt;
+    const float hf = 0.5f * dtf;
+    const double xb[3] = {x[0], x[1], x[2]};
+    // drift - kick - drift (BorisStep with B sampled at the start point)
+#pragma unroll
+    for (int d = 0; d < 3; d++) x[d] += (double)(hf * (u[d] * igam));
+    const float tc = dtf * 0.5f * igam;
+    const float t[3] = {tc * b[0], tc * b[1], tc * b[2]};
+    const float t2 = t[0] * t[0] + t[1] * t[1] + t[2] * t[2];
+    const float sc = 2.0f / (1.0f + t2);
+    const float s[3] = {sc * t[0], sc * t[1], sc * t[2]};
+    const float pp[3] = {u[0] + (u[1] * t[2] - u[2] * t[1]), u[1] + (u[2] * t[0] - u[0] * t[2]),
+                         u[2] + (u[0] * t[1] - u[1] * t[0])};
+    const float un[3] = {u[0] + (pp[1] * s[2] - pp[2] * s[1]), u[1] + (pp[2] * s[0] - pp[0] * s[2]),
+                         u[2] + (pp[0] * s[1] - pp[1] * s[0])};
+    u[0] = un[0];
+    u[1] = un[1];
+    u[2] = un[2];
+    const float igam2 = 1.0f / sqrtf(1.0f + u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+#pragma unroll
+    for (int d = 0; d < 3; d++) x[d] += (double)(hf * (u[d] * igam2));
+    if (seg_hits(xb, x, a.rin_n) || inside_inner_m(x, a.rin_n)) return 1;
+    if (!fin3(x) || !isfinite(u[0]) || !isfinite(u[1]) || !isfinite(u[2])) return 8;
+    const double dd[3] = {x[0] - xb[0], x[1] - xb[1], x[2] - xb[2]};
+    const double seg = sqrt(dot3(dd, dd));
+    L.s += isfinite(seg) ? seg : 0.0;
+    L.t += dt;
+    ++L.n;
+    return -1;
+  }
+  static __device__ __forceinline__ void result(const TraceView& a, const Lane& L, int term,
+                                                amps_trace_result& r) {
+    const double T = a.len / C_LIGHT;
+    const double pu = a.P.m0_kg * C_LIGHT;
+    r.termination = term;
+    r.retry_count = L.retry;
+    r.steps = L.n;
+    r.time_s = L.t * T;
+    r.distance_m = L.s * a.len;
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+      r.x[d] = L.x[d] * a.len;
+      r.p[d] = (double)L.u[d] * pu;
+    }
+  }
+};
+
+__device__ __forceinline__ bool retryable(int t) { return t >= 6 && t <= 8; }
+__device__ __forceinline__ int8_t state_of(int term) {
+  // 1 allowed; 0 cutoff-forbidden (physical or trace limit); 2 numerical failure
+  return term == 0 ? 1 : (term <= 5 ? 0 : 2);
+}
+
+// ---------------------------------------------------------------------
+// Persistent kernel.  Work item semantics by mode:
+//   TRACE: item i = trajectory i of an explicit (x0, dir, R) list
+//   FULL : item i = (loc i / nbin, bin lo(loc) + i % nbin)   [states only]
+//   UPPER: item i = loc i, bins walked downward from hi(loc) until the first
+//          forbidden sample (CutoffForDirectionUpperScan_GV semantics)
+// ---------------------------------------------------------------------
+template <class S>
+__global__ __launch_bounds__(256) void k_trace_persistent(TraceView a) {
+  typename S::Lane L;
+  int64_t item = -1;
+  int loc = 0, bin = 0, attempt = 0;
+  bool has = false, done = false;
+  int64_t my_steps = 0;
+  const int lane = __lane_id();
+  for (;;) {
+    const bool need = !has && !done;
+    const uint64_t mask = __ballot(need);
+    if (mask) {
+      const int leader = __ffsll((long long)mask) - 1;
+      unsigned long long base = 0;
+      if (lane == leader) base = atomicAdd(a.queue, (unsigned long long)__popcll(mask));
+      base = __shfl(base, leader);
+      if (need) {
+        const uint64_t below = (lane == 0) ? 0ull : (mask & ((1ull << lane) - 1ull));
+        const unsigned long long my = base + (unsigned long long)__popcll(below);
+        if ((int64_t)my < a.nitems) {
+          item = (int64_t)my;
+          attempt = 0;
+          if (a.mode == TRACE_MODE_LIST) {
+            loc = (int)item;
+            S::init(a, L, a.x0 + 3 * item, a.dir + 3 * item, a.R[item], 0);
+          } else if (a.mode == TRACE_MODE_FULL) {
+            loc = (int)(item / a.nbin_item);
+            bin = (a.lo ? a.lo[loc] : 0) + (int)(item % a.nbin_item);
+            S::init(a, L, a.x0 + 3 * (int64_t)loc, a.dir + 3 * (int64_t)loc, a.grid[bin], 0);
+          } else {
+            loc = (int)item;
+            bin = a.hi ? a.hi[loc] : a.nR - 1;
+            S::init(a, L, a.x0 + 3 * (int64_t)loc, a.dir + 3 * (int64_t)loc, a.grid[bin], 0);
+          }
+          has = true;
+        } else {
+          done = true;
+        }
+      }
+    }
+    if (__ballot(has) == 0ull) break;
+    for (int k = 0; k < a.chunk && has; k++) {
+      const int term = S::step(a, L);
+      if (term < 0) continue;
+      if (a.P.retry && attempt == 0 && retryable(term)) {
+        attempt = 1;
+        my_steps += L.n;
+        if (a.mode == TRACE_MODE_LIST)
+          S::init(a, L, a.x0 + 3 * item, a.dir + 3 * item, a.R[item], 1);
+        else
+          S::init(a, L, a.x0 + 3 * (int64_t)loc, a.dir + 3 * (int64_t)loc, a.grid[bin], 1);
+        continue;
+      }
+      my_steps += L.n;
+      if (a.mode == TRACE_MODE_LIST) {
+        amps_trace_result r;
+        S::result(a, L, term, r);
+        a.out[item] = r;
+        has = false;
+      } else if (a.mode == TRACE_MODE_FULL) {
+        a.states[(int64_t)loc * a.nR + bin] = state_of(term);
+        has = false;
+      } else {
+        const int8_t st = state_of(term);
+        a.states[(int64_t)loc * a.nR + bin] = st;
+        const int lo = a.lo ? a.lo[loc] : 0;
+        if (st == 1 && bin > lo) {
+          bin--;
+          attempt = 0;
+          S::init(a, L, a.x0 + 3 * (int64_t)loc, a.dir + 3 * (int64_t)loc, a.grid[bin], 0);
+        } else {
+          // first non-allowed sample from the top, or the bracket's bottom
+          a.cutoff[loc] = (st == 1) ? bin : bin + 1;
+          has = false;
+        }
+      }
+    }
+  }
+  // total steps of this wavefront
+  unsigned long long tot = (unsigned long long)my_steps;
+  for (int off = 32; off > 0; off >>= 1) tot += __shfl_down(tot, off);
+  if (lane == 0 && a.steps_total) atomicAdd(a.steps_total, tot);
+}
+
+}  // namespace
+
+// =====================================================================
+// host side
+// =====================================================================
+struct TraceState {
+  unsigned long long* d_queue = nullptr;
+  unsigned long long* d_steps = nullptr;
+  double* d_x0 = nullptr;
+  double* d_dir = nullptr;
+  double* d_R = nullptr;
+  double* d_grid = nullptr;
+  int* d_lo = nullptr;
+  int* d_hi = nullptr;
+  int* d_cut = nullptr;
+  int8_t* d_states = nullptr;
+  amps_trace_result* d_out = nullptr;
+  size_t b_x0 = 0, b_dir = 0, b_R = 0, b_lo = 0, b_hi = 0, b_cut = 0, b_states = 0, b_out = 0;
+  int grid_waves = 0;  // persistent grid size (workgroups); 0 = 8 per CU
+  int chunk = 32;      // steps between queue refills
+};
 
 namespace amps {
-void trace_free(amps_ctx*) {}
+void trace_free(amps_ctx* c) {
+  TraceState* t = (TraceState*)c->trace_state;
+  if (!t) return;
+  (void)hipFree(t->d_queue);
+  (void)hipFree(t->d_steps);
+  (void)hipFree(t->d_x0);
+  (void)hipFree(t->d_dir);
+  (void)hipFree(t->d_R);
+  (void)hipFree(t->d_grid);
+  (void)hipFree(t->d_lo);
+  (void)hipFree(t->d_hi);
+  (void)hipFree(t->d_cut);
+  (void)hipFree(t->d_states);
+  (void)hipFree(t->d_out);
+  delete t;
+  c->trace_state = nullptr;
+}
+
+static TraceState* tstate(amps_ctx* c) {
+  if (!c->trace_state) {
+    TraceState* t = new TraceState();
+    if (hipMalloc(&t->d_queue, 64) != hipSuccess || hipMalloc(&t->d_steps, 64) != hipSuccess) {
+      delete t;
+      return nullptr;
+    }
+    c->trace_state = t;
+  }
+  return (TraceState*)c->trace_state;
+}
+
+static int ensure(void** p, size_t* have, size_t bytes) {
+  if (bytes <= *have && *p) return 0;
+  (void)hipFree(*p);
+  *p = nullptr;
+  *have = 0;
+  HIP_TRY(hipMalloc(p, std::max<size_t>(bytes, 64)));
+  *have = std::max<size_t>(bytes, 64);
+  return 0;
+}
+
+static int check_params(const amps_trace_params* P) {
+  if (!P) return fail(AMPS_E_ARG, "trace: null params");
+  if (!(P->m0_kg > 0.0) || !(P->qabs_C > 0.0)) return fail(AMPS_E_ARG, "trace: mass and |charge| must be > 0");
+  if (!(P->earth_radius_m > 0.0)) return fail(AMPS_E_ARG, "trace: earth_radius_m must be > 0");
+  if (P->max_steps <= 0) return fail(AMPS_E_ARG, "trace: max_steps must be > 0");
+  return 0;
+}
+
+static void fill_view(amps_ctx* c, TraceState* t, const amps_trace_params* P, TraceView& a) {
+  std::memset(&a, 0, sizeof(a));
+  a.P = *P;
+  const double Re = P->earth_radius_m;
+  for (int d = 0; d < 3; d++) {
+    a.box_min_m[d] = P->box_min_Re[d] * Re;
+    a.box_max_m[d] = P->box_max_Re[d] * Re;
+  }
+  a.r_inner_m = P->r_inner_Re * Re;
+  a.r_outer_m = P->r_outer_Re * Re;
+  // normalized (FP32) constants: length unit Re, time unit Re/c
+  a.len = Re;
+  const double T = Re / C_LIGHT;
+  const double kq = P->q_C * T / P->m0_kg;
+  a.f_cb = (float)(kq * 1.0e-7 * P->moment_Am2 / (Re * Re * Re));
+  for (int d = 0; d < 3; d++) {
+    a.f_mhat[d] = (float)P->m_hat[d];
+    a.f_bext[d] = (float)(kq * P->B_ext_T[d]);
+  }
+  a.f_dphi = (float)P->dphi_max;
+  a.rin_n = P->r_inner_Re;
+  a.rout_n = P->r_outer_Re;
+  a.maxdist_n = (P->max_distance_m > 0.0) ? P->max_distance_m / Re : -1.0;
+  a.floor_abs_n = 1.0e-12 / T;
+  a.floor_rel_n = 1.0e-9 * std::max(P->dt_trace_s, 1.0) / T;
+  a.floor_dist_n = 100.0e3 / Re;
+  a.queue = t->d_queue;
+  a.steps_total = t->d_steps;
+  a.chunk = t->chunk;
+}
+
+static int launch(amps_ctx* c, TraceState* t, TraceView& a, int precision) {
+  HIP_TRY(hipMemsetAsync(t->d_queue, 0, 8, c->stream));
+  int grid = t->grid_waves;
+  if (grid <= 0) {
+    hipDeviceProp_t p;
+    HIP_TRY(hipGetDeviceProperties(&p, c->device));
+    grid = p.multiProcessorCount * 8;
+  }
+  tbegin(c);
+  if (precision == AMPS_FP32)
+    hipLaunchKernelGGL(k_trace_persistent<Fp32Stepper>, dim3(grid), dim3(256), 0, c->stream, a);
+  else
+    hipLaunchKernelGGL(k_trace_persistent<Fp64Stepper>, dim3(grid), dim3(256), 0, c->stream, a);
+  HIP_TRY(hipGetLastError());
+  tend(c, AMPS_K_TRACE);
+  return 0;
+}
 }  // namespace amps
+
+extern "C" int amps_gpu_trace_set_param(amps_ctx* c, const char* name, int64_t v) {
+  if (!c || !name) return fail(AMPS_E_ARG, "trace_set_param: null argument");
+  TraceState* t = tstate(c);
+  if (!t) return fail(AMPS_E_HIP, "trace: allocation failed");
+  std::string n(name);
+  if (n == "workgroups") {
+    if (v < 0 || v > (1 << 20)) return fail(AMPS_E_ARG, "workgroups out of range");
+    t->grid_waves = (int)v;
+  } else if (n == "chunk") {
+    if (v < 1 || v > 100000) return fail(AMPS_E_ARG, "chunk out of range [1, 1e5]");
+    t->chunk = (int)v;
+  } else {
+    return fail(AMPS_E_ARG, "unknown trace parameter " + n);
+  }
+  return AMPS_OK;
+}
+
+extern "C" int amps_gpu_trace_batch(amps_ctx* c, const amps_trace_params* P, int32_t precision,
+                                    int64_t n, const double* x0, const double* dir,
+                                    const double* R_GV, amps_trace_result* out,
+                                    int64_t* steps_total) {
+  if (!c) return fail(AMPS_E_ARG, "null context");
+  if (int r = check_params(P)) return r;
+  if (precision != AMPS_FP64 && precision != AMPS_FP32) return fail(AMPS_E_ARG, "trace: bad precision");
+  if (n < 0 || n > ((int64_t)1 << 31)) return fail(AMPS_E_RANGE, "trace: n out of range");
+  if (n == 0) return AMPS_OK;
+  if (!x0 || !dir || !R_GV || !out) return fail(AMPS_E_ARG, "trace: null array");
+  (void)hipSetDevice(c->device);
+  TraceState* t = tstate(c);
+  if (!t) return fail(AMPS_E_HIP, "trace: allocation failed");
+  if (int r = ensure((void**)&t->d_x0, &t->b_x0, sizeof(double) * 3 * n)) return r;
+  if (int r = ensure((void**)&t->d_dir, &t->b_dir, sizeof(double) * 3 * n)) return r;
+  if (int r = ensure((void**)&t->d_R, &t->b_R, sizeof(double) * n)) return r;
+  if (int r = ensure((void**)&t->d_out, &t->b_out, sizeof(amps_trace_result) * n)) return r;
+  HIP_TRY(hipMemcpyAsync(t->d_x0, x0, sizeof(double) * 3 * n, hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(hipMemcpyAsync(t->d_dir, dir, sizeof(double) * 3 * n, hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(hipMemcpyAsync(t->d_R, R_GV, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(hipMemsetAsync(t->d_steps, 0, 8, c->stream));
+  TraceView a;
+  fill_view(c, t, P, a);
+  a.mode = TRACE_MODE_LIST;
+  a.nitems = n;
+  a.x0 = t->d_x0;
+  a.dir = t->d_dir;
+  a.R = t->d_R;
+  a.out = t->d_out;
+  if (int r = launch(c, t, a, precision)) return r;
+  HIP_TRY(hipMemcpyAsync(out, t->d_out, sizeof(amps_trace_result) * n, hipMemcpyDeviceToHost, c->stream));
+  unsigned long long st = 0;
+  HIP_TRY(hipMemcpyAsync(&st, t->d_steps, 8, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  if (steps_total) *steps_total = (int64_t)st;
+  return AMPS_OK;
+}
+
+extern "C" int amps_gpu_cutoff_scan(amps_ctx* c, const amps_trace_params* P, int32_t precision,
+                                    int32_t mode, int64_t nloc, const double* x0,
+                                    const double* dir, int32_t nR, const double* grid,
+                                    const int32_t* bin_lo, const int32_t* bin_hi,
+                                    int8_t* states, int32_t* cutoff_bin, int64_t* steps_total) {
+  if (!c) return fail(AMPS_E_ARG, "null context");
+  if (int r = check_params(P)) return r;
+  if (precision != AMPS_FP64 && precision != AMPS_FP32) return fail(AMPS_E_ARG, "cutoff_scan: bad precision");
+  if (mode != AMPS_SCAN_FULL && mode != AMPS_SCAN_UPPER) return fail(AMPS_E_ARG, "cutoff_scan: bad mode");
+  if (nloc < 0 || nloc > ((int64_t)1 << 28) || nR < 2 || nR > 4096)
+    return fail(AMPS_E_RANGE, "cutoff_scan: nloc or nR out of range");
+  if (nloc == 0) return AMPS_OK;
+  if (!x0 || !dir || !grid || !states || !cutoff_bin) return fail(AMPS_E_ARG, "cutoff_scan: null array");
+  if ((bin_lo == nullptr) != (bin_hi == nullptr)) return fail(AMPS_E_ARG, "cutoff_scan: give both bracket arrays or neither");
+  int nbin_item = nR;
+  if (bin_lo) {
+    for (int64_t i = 0; i < nloc; i++)
+      if (bin_lo[i] < 0 || bin_hi[i] >= nR || bin_lo[i] > bin_hi[i])
+        return fail(AMPS_E_ARG, "cutoff_scan: bracket outside [0, nR)");
+    if (mode == AMPS_SCAN_FULL) {
+      const int w = bin_hi[0] - bin_lo[0] + 1;
+      for (int64_t i = 0; i < nloc; i++)
+        if (bin_hi[i] - bin_lo[i] + 1 != w) return fail(AMPS_E_ARG, "cutoff_scan: FULL mode needs equal bracket widths");
+      nbin_item = w;
+    }
+  }
+  (void)hipSetDevice(c->device);
+  TraceState* t = tstate(c);
+  if (!t) return fail(AMPS_E_HIP, "cutoff_scan: allocation failed");
+  if (int r = ensure((void**)&t->d_x0, &t->b_x0, sizeof(double) * 3 * nloc)) return r;
+  if (int r = ensure((void**)&t->d_dir, &t->b_dir, sizeof(double) * 3 * nloc)) return r;
+  if (int r = ensure((void**)&t->d_lo, &t->b_lo, sizeof(int) * nloc)) return r;
+  if (int r = ensure((void**)&t->d_hi, &t->b_hi, sizeof(int) * nloc)) return r;
+  if (int r = ensure((void**)&t->d_cut, &t->b_cut, sizeof(int) * nloc)) return r;
+  if (int r = ensure((void**)&t->d_states, &t->b_states, (size_t)nloc * nR)) return r;
+  if (!t->d_grid) HIP_TRY(hipMalloc(&t->d_grid, sizeof(double) * 4096));
+  HIP_TRY(hipMemcpyAsync(t->d_x0, x0, sizeof(double) * 3 * nloc, hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(hipMemcpyAsync(t->d_dir, dir, sizeof(double) * 3 * nloc, hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(hipMemcpyAsync(t->d_grid, grid, sizeof(double) * nR, hipMemcpyHostToDevice, c->stream));
+  if (bin_lo) {
+    HIP_TRY(hipMemcpyAsync(t->d_lo, bin_lo, sizeof(int) * nloc, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipMemcpyAsync(t->d_hi, bin_hi, sizeof(int) * nloc, hipMemcpyHostToDevice, c->stream));
+  }
+  HIP_TRY(hipMemsetAsync(t->d_states, -1, nloc * nR, c->stream));
+  HIP_TRY(hipMemsetAsync(t->d_cut, 0, sizeof(int) * nloc, c->stream));
+  HIP_TRY(hipMemsetAsync(t->d_steps, 0, 8, c->stream));
+  TraceView a;
+  fill_view(c, t, P, a);
+  a.mode = (mode == AMPS_SCAN_FULL) ? TRACE_MODE_FULL : TRACE_MODE_UPPER;
+  a.nitems = (mode == AMPS_SCAN_FULL) ? nloc * nbin_item : nloc;
+  a.nbin_item = nbin_item;
+  a.nR = nR;
+  a.x0 = t->d_x0;
+  a.dir = t->d_dir;
+  a.grid = t->d_grid;
+  a.lo = bin_lo ? t->d_lo : nullptr;
+  a.hi = bin_lo ? t->d_hi : nullptr;
+  a.states = t->d_states;
+  a.cutoff = t->d_cut;
+  if (int r = launch(c, t, a, precision)) return r;
+  HIP_TRY(hipMemcpyAsync(states, t->d_states, nloc * nR, hipMemcpyDeviceToHost, c->stream));
+  std::vector<int> cut((size_t)nloc);
+  HIP_TRY(hipMemcpyAsync(cut.data(), t->d_cut, sizeof(int) * nloc, hipMemcpyDeviceToHost, c->stream));
+  unsigned long long st = 0;
+  HIP_TRY(hipMemcpyAsync(&st, t->d_steps, 8, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  if (steps_total) *steps_total = (int64_t)st;
+  // cutoff bins
+  for (int64_t i = 0; i < nloc; i++) {
+    const int8_t* s = states + i * nR;
+    const int lo = bin_lo ? bin_lo[i] : 0, hi = bin_hi ? bin_hi[i] : nR - 1;
+    if (mode == AMPS_SCAN_UPPER) {
+      int b = cut[(size_t)i];
+      // bracket diagnostics: top sample forbidden below Rmax, or allowed down to a bottom above Rmin
+      if (s[hi] != 1) b = (hi == nR - 1) ? -1 : AMPS_BIN_ABOVE;
+      else if (b == lo && lo > 0 && s[lo] == 1) b = AMPS_BIN_BELOW;
+      cutoff_bin[i] = b;
+    } else {
+      // UPPER_SCAN result from a full sample vector (CutoffForDirectionUpperScan_GV)
+      int b;
+      if (s[hi] != 1) b = (hi == nR - 1) ? -1 : AMPS_BIN_ABOVE;
+      else {
+        b = lo;
+        for (int k = hi - 1; k >= lo; --k)
+          if (s[k] != 1) {
+            b = k + 1;
+            break;
+          }
+        if (b == lo && lo > 0) b = AMPS_BIN_BELOW;
+      }
+      cutoff_bin[i] = b;
+    }
+  }
+  return AMPS_OK;
+}

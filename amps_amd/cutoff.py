@@ -1,0 +1,198 @@
+"""Host mirror of the gridless cutoff-rigidity interface over the GPU tracer.
+
+* ``trace_batch``   — batched TraceTrajectoryShared / TraceAllowedShared
+  (CutoffRigidityGridless.h:245-315, .cpp:1246-1520).
+* ``cutoff_scan``   — CutoffForDirectionUpperScan_GV on the
+  BuildCutoffSearchGrid_GV grid (.cpp:2578-2834), full or downward scan,
+  optionally restricted to a per-direction bracket of bins.
+* ``build_search_grid``, ``local_enu``, ``local_dir`` — the grid and
+  direction helpers (.cpp:2578-2617, 1757-1800).
+* ``config2_*``     — the batched-map workload: a 64x32 lat/lon grid at
+  r = 1.05 Re, 8 arrival directions per point, 48 log-spaced rigidities
+  0.1..20 GV, tilted dipole (10 deg, B0 = 3.1e-5 T) plus Bz = -5 nT, steps of
+  0.01 gyro-period, r < 1 forbidden, r > 25 allowed, 1e5 steps forbidden.
+
+All tracing runs in the HIP library.
+"""
+import ctypes as C
+import math
+
+import numpy as np
+
+from . import _lib as L
+
+SPEED_OF_LIGHT = 299792458.0      # constants.h:40
+ELECTRON_CHARGE = 1.602176565e-19  # constants.h:31
+AMU = 1.66053906660e-27            # constants.AtomicData.h:11
+RE_AMPS = 6371.0e3                 # _EARTH__RADIUS_, constants.PlanetaryData.h:54
+RE_DIPOLE = 6371.2e3               # Re_km of DipoleInterface.h:63-64
+B_EQ = 3.12e-5                     # DipoleInterface.h:71
+PROTON_AMU = 1.007276466621        # DipoleInterface.h:67
+
+TERM_NAMES = ["OUTER_BOUNDARY_ALLOWED", "INNER_BOUNDARY_FORBIDDEN",
+              "MAGNETICALLY_TRAPPED_FORBIDDEN", "TIME_LIMIT", "STEP_LIMIT", "DISTANCE_LIMIT",
+              "INVALID_TIME_STEP", "INVALID_FIELD", "NUMERICAL_FAILURE"]
+
+SCAN_FULL = 0
+SCAN_UPPER = 1
+BIN_ABOVE = -2
+BIN_BELOW = -3
+
+
+def dipole_moment(scale=1.0):
+    """M_E_Am2 of DipoleInterface.h:71-72 times the moment scale."""
+    Re_m = 6371.2 * 1000.0
+    return scale * (B_EQ * Re_m * Re_m * Re_m / 1e-7)
+
+
+def make_params(*, tilt_deg=0.0, moment_scale=1.0, bz_T=0.0, mass_amu=PROTON_AMU, charge_e=1,
+                reversed_charge=False, dt_trace=1.0, max_time=600.0, max_steps=300000,
+                max_distance_Re=400.0, dphi=0.15, adaptive=True, dist_limiter=True, dt_floor=True,
+                outer_sphere=False, box_Re=35.0, r_outer_Re=35.0, r_inner_Re=1.0,
+                earth_radius_m=RE_AMPS, retry=True):
+    P = L.TraceParams()
+    q = charge_e * ELECTRON_CHARGE
+    P.q_C = -q if reversed_charge else q
+    P.qabs_C = abs(q)
+    P.m0_kg = mass_amu * AMU
+    P.dt_trace_s = dt_trace
+    P.max_time_s = max_time
+    P.max_steps = int(max_steps)
+    P.max_distance_m = max_distance_Re * earth_radius_m if max_distance_Re > 0 else -1.0
+    P.dphi_max = dphi
+    P.adaptive = int(adaptive)
+    P.dist_limiter = int(dist_limiter)
+    P.dt_floor = int(dt_floor)
+    P.outer_sphere = int(outer_sphere)
+    P.retry = int(retry)
+    for d in range(3):
+        P.box_min_Re[d] = -box_Re
+        P.box_max_Re[d] = box_Re
+    P.r_outer_Re = r_outer_Re
+    P.r_inner_Re = r_inner_Re
+    P.earth_radius_m = earth_radius_m
+    P.moment_Am2 = dipole_moment(moment_scale)
+    th = tilt_deg * math.pi / 180.0   # SetTiltDeg, DipoleInterface.cpp:22-28
+    P.m_hat[0] = math.sin(th)
+    P.m_hat[1] = 0.0
+    P.m_hat[2] = math.cos(th)
+    P.B_ext_T[0] = 0.0
+    P.B_ext_T[1] = 0.0
+    P.B_ext_T[2] = bz_T
+    return P
+
+
+def legacy_params(**kw):
+    """The srcEarth cutoff decks' defaults (C1/C12): DT_TRACE 1 s, 600 s,
+    400 Re path cap, box +-35 Re, R_INNER 1, legacy 0.15 rad step control."""
+    return make_params(**kw)
+
+
+def config2_params():
+    return make_params(tilt_deg=10.0, moment_scale=3.1 / 3.12, bz_T=-5e-9, dt_trace=1e30,
+                       max_time=1e30, max_steps=100000, max_distance_Re=-1.0,
+                       dphi=2.0 * math.pi * 0.01, dist_limiter=False, dt_floor=False,
+                       outer_sphere=True, r_outer_Re=25.0, r_inner_Re=1.0,
+                       earth_radius_m=RE_DIPOLE)
+
+
+def build_search_grid(rmin, rmax, n):
+    """BuildCutoffSearchGrid_GV, LOG spacing with exact endpoints."""
+    lmin, lmax = math.log(rmin), math.log(rmax)
+    g = np.array([math.exp((1.0 - i / (n - 1)) * lmin + (i / (n - 1)) * lmax) for i in range(n)])
+    g[0], g[-1] = rmin, rmax
+    return g
+
+
+def _unit(v):
+    n = math.sqrt(float(v @ v))
+    return v / n if n > 0 else v * 0.0
+
+
+def local_enu(x0):
+    """BuildLocalENU_GSM (CutoffRigidityGridless.cpp:1757-1773)."""
+    U = _unit(np.asarray(x0, np.float64))
+    E = np.cross([0.0, 0.0, 1.0], U)
+    if math.sqrt(E @ E) < 1e-12:
+        E = np.cross([1.0, 0.0, 0.0], U)
+    E = _unit(E)
+    N = _unit(np.cross(U, E))
+    return E, N, U
+
+
+def local_dir(frame, lon_deg, lat_deg):
+    """LocalLonLatToDir_GSM (.cpp:1786-1800): azimuth from +N toward +E,
+    elevation toward +U."""
+    E, N, U = frame
+    lon, lat = math.radians(lon_deg), math.radians(lat_deg)
+    cl = math.cos(lat)
+    return _unit(cl * math.sin(lon) * E + cl * math.cos(lon) * N + math.sin(lat) * U)
+
+
+def config2_points(nlon=64, nlat=32, r_Re=1.05, re_m=RE_DIPOLE):
+    lon = (np.arange(nlon) + 0.5) * 360.0 / nlon
+    lat = -90.0 + (np.arange(nlat) + 0.5) * 180.0 / nlat
+    LAT, LON = np.meshgrid(lat, lon, indexing="ij")
+    la, lo = np.radians(LAT.ravel()), np.radians(LON.ravel())
+    r = r_Re * re_m
+    x = np.stack([r * np.cos(la) * np.cos(lo), r * np.cos(la) * np.sin(lo), r * np.sin(la)], 1)
+    return x, LAT.ravel(), LON.ravel()
+
+
+def config2_directions(ndir=8, seed=5, max_zenith_deg=60.0):
+    """Fixed (zenith, azimuth) set: the first is vertical, the rest uniform in
+    cos(zenith) over [0, max_zenith] and in azimuth, from a seeded RNG."""
+    rng = np.random.default_rng(seed)
+    zen = np.zeros(ndir)
+    az = np.zeros(ndir)
+    cz = rng.uniform(math.cos(math.radians(max_zenith_deg)), 1.0, ndir - 1)
+    zen[1:] = np.degrees(np.arccos(cz))
+    az[1:] = rng.uniform(0.0, 360.0, ndir - 1)
+    return zen, az
+
+
+def arrival_dirs(x0, zen, az):
+    """Arrival direction (unit, pointing down along the particle's motion) for
+    every point and every (zenith, azimuth): the backtrace starts along
+    LocalLonLatToDir(azimuth, 90 - zenith)."""
+    out = np.zeros((len(x0), len(zen), 3))
+    for i, x in enumerate(x0):
+        fr = local_enu(x)
+        for k in range(len(zen)):
+            out[i, k] = -local_dir(fr, az[k], 90.0 - zen[k])
+    return out
+
+
+def trace_batch(ctx, params, x0, dirs, R, precision=L.FP64):
+    x0 = np.ascontiguousarray(x0, np.float64).reshape(-1, 3)
+    dirs = np.ascontiguousarray(dirs, np.float64).reshape(-1, 3)
+    R = np.ascontiguousarray(R, np.float64)
+    out = np.zeros(len(R), L.TRACE_RESULT_DTYPE)
+    steps = C.c_int64()
+    L.check(ctx.lib.amps_gpu_trace_batch(ctx.h, C.byref(params), precision, len(R),
+                                         L._ptr(x0, C.c_double), L._ptr(dirs, C.c_double),
+                                         L._ptr(R, C.c_double), out.ctypes.data_as(C.c_void_p),
+                                         C.byref(steps)), "trace_batch")
+    return out, steps.value
+
+
+def cutoff_scan(ctx, params, x0, dirs, grid, mode=SCAN_UPPER, lo=None, hi=None, precision=L.FP64):
+    x0 = np.ascontiguousarray(x0, np.float64).reshape(-1, 3)
+    dirs = np.ascontiguousarray(dirs, np.float64).reshape(-1, 3)
+    grid = np.ascontiguousarray(grid, np.float64)
+    n, nR = len(x0), len(grid)
+    states = np.zeros((n, nR), np.int8)
+    bins = np.zeros(n, np.int32)
+    steps = C.c_int64()
+    lo_a = None if lo is None else np.ascontiguousarray(lo, np.int32)
+    hi_a = None if hi is None else np.ascontiguousarray(hi, np.int32)
+    L.check(ctx.lib.amps_gpu_cutoff_scan(ctx.h, C.byref(params), precision, mode, n,
+                                         L._ptr(x0, C.c_double), L._ptr(dirs, C.c_double), nR,
+                                         L._ptr(grid, C.c_double), L._ptr(lo_a, C.c_int32),
+                                         L._ptr(hi_a, C.c_int32), L._ptr(states, C.c_int8),
+                                         L._ptr(bins, C.c_int32), C.byref(steps)), "cutoff_scan")
+    return states, bins, steps.value
+
+
+def set_trace_param(ctx, name, value):
+    L.check(ctx.lib.amps_gpu_trace_set_param(ctx.h, name.encode(), int(value)), f"trace param {name}")

@@ -1,0 +1,108 @@
+"""Learned cutoff surrogate: an MLP (6 -> 128 -> 128 -> 1, GELU) trained on
+the GPU tracer's own cutoff maps, and the bracketed scan that uses it.
+
+Features per (location, arrival direction): sin/cos latitude, sin/cos
+longitude, zenith / 90 deg and azimuth / 180 deg - 1.  Target: log10 of the
+upper cutoff rigidity (the allowed side of the highest forbidden -> allowed
+transition on the scan grid).  At inference the predicted bin +-3 is scanned
+downward from its top (CutoffForDirectionUpperScan_GV restricted to the
+bracket); the full scan is the fallback wherever the bracket does not
+contain the transition.
+"""
+import ctypes as C
+import math
+
+import numpy as np
+
+from . import _lib as L
+from . import cutoff as CU
+
+NPARAM = 17537
+D_IN = 6
+
+
+def features(lat_deg, lon_deg, zen_deg, az_deg):
+    la = np.radians(np.asarray(lat_deg, np.float64))
+    lo = np.radians(np.asarray(lon_deg, np.float64))
+    z = np.asarray(zen_deg, np.float64) / 90.0
+    a = np.asarray(az_deg, np.float64) / 180.0 - 1.0
+    return np.stack([np.sin(la), np.cos(la), np.sin(lo), np.cos(lo), z, a], axis=-1).astype(np.float32)
+
+
+def init_params(seed=0):
+    rng = np.random.default_rng(seed)
+    W1 = rng.uniform(-1, 1, (128, D_IN)) * math.sqrt(6.0 / D_IN)
+    W2 = rng.uniform(-1, 1, (128, 128)) * math.sqrt(6.0 / 128)
+    w3 = rng.uniform(-1, 1, 128) * math.sqrt(3.0 / 128)
+    return np.concatenate([W1.ravel(), np.zeros(128), W2.ravel(), np.zeros(128), w3, np.zeros(1)]).astype(np.float32)
+
+
+def bin_target(bins, grid):
+    """log10 of the cutoff rigidity for a scan result (-1: above Rmax)."""
+    b = np.where(np.asarray(bins) < 0, len(grid) - 1, bins)
+    return np.log10(np.asarray(grid)[b]).astype(np.float32)
+
+
+def predicted_bin(y, grid):
+    lg = np.log10(np.asarray(grid))
+    return np.clip(np.searchsorted((lg[1:] + lg[:-1]) / 2, y), 0, len(grid) - 1).astype(np.int32)
+
+
+class Surrogate:
+    """The network's parameters and training data live on the device."""
+
+    def __init__(self, ctx, theta=None, seed=0):
+        self.ctx = ctx
+        th = init_params(seed) if theta is None else np.ascontiguousarray(theta, np.float32)
+        L.check(ctx.lib.amps_gpu_mlp_init(ctx.h, L._ptr(th, C.c_float)), "mlp_init")
+
+    def params(self):
+        th = np.zeros(NPARAM, np.float32)
+        st = C.c_int64()
+        L.check(self.ctx.lib.amps_gpu_mlp_get_params(self.ctx.h, L._ptr(th, C.c_float), C.byref(st)), "mlp_get_params")
+        return th, st.value
+
+    def set_data(self, X, T):
+        X = np.ascontiguousarray(X, np.float32)
+        T = np.ascontiguousarray(T, np.float32)
+        L.check(self.ctx.lib.amps_gpu_mlp_set_data(self.ctx.h, len(T), L._ptr(X, C.c_float),
+                                                   L._ptr(T, C.c_float)), "mlp_set_data")
+
+    def grad(self, n):
+        g = np.zeros(NPARAM, np.float32)
+        loss = C.c_float()
+        L.check(self.ctx.lib.amps_gpu_mlp_grad(self.ctx.h, n, L._ptr(g, C.c_float), C.byref(loss)), "mlp_grad")
+        return g, loss.value
+
+    def train(self, nsteps, batch=4096, lr=1e-3, record_loss=False):
+        lo = np.zeros(nsteps, np.float32) if record_loss else None
+        L.check(self.ctx.lib.amps_gpu_mlp_train(self.ctx.h, nsteps, batch, lr, L._ptr(lo, C.c_float)), "mlp_train")
+        return lo
+
+    def predict(self, X):
+        X = np.ascontiguousarray(X, np.float32)
+        y = np.zeros(len(X), np.float32)
+        L.check(self.ctx.lib.amps_gpu_mlp_predict(self.ctx.h, len(X), L._ptr(X, C.c_float),
+                                                  L._ptr(y, C.c_float)), "mlp_predict")
+        return y
+
+
+def bracketed_map(ctx, sur, params, x0, dirs, feats, grid, half=3, precision=L.FP32):
+    """Cutoff bins with the surrogate bracket and the full-scan fallback.
+    Returns (bins, fallback mask, predicted bins, total steps)."""
+    y = sur.predict(feats)
+    pb = predicted_bin(y, grid)
+    nR = len(grid)
+    lo = np.clip(pb - half, 0, nR - 1).astype(np.int32)
+    hi = np.clip(pb + half, 0, nR - 1).astype(np.int32)
+    st, bins, steps = CU.cutoff_scan(ctx, params, x0, dirs, grid, mode=CU.SCAN_UPPER, lo=lo, hi=hi,
+                                     precision=precision)
+    fb = bins < -1
+    if fb.any():
+        idx = np.flatnonzero(fb)
+        _, b2, s2 = CU.cutoff_scan(ctx, params, x0[idx], dirs[idx], grid, mode=CU.SCAN_UPPER,
+                                   precision=precision)
+        bins = bins.copy()
+        bins[idx] = b2
+        steps += s2
+    return bins, fb, pb, steps

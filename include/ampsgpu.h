@@ -124,6 +124,94 @@ int amps_gpu_compact(amps_ctx* ctx);
  * the dense [node][246] result stays resident either way. */
 int amps_gpu_update_j_mass_matrix(amps_ctx* ctx, double dt, double* J_out, double* M_out);
 
+/* ---- gridless cutoff tracer ------------------------------------------ */
+/* Parameters of one trace (SI).  The FP64 path is TraceTrajectoryImpl +
+ * TraceTrajectoryWithSingleRetry (CutoffRigidityGridless.cpp:1246-1496) with
+ * the BORIS mover (GridlessParticleMovers.cpp:781-801, 1035-1046), the
+ * legacy step selector (:1086-1152) whose gyro budget, distance limiter and
+ * floor are switchable, the analytic dipole (DipoleInterface.h:99-125) plus a
+ * uniform field, and a box or spherical outer boundary. */
+typedef struct {
+  double q_C;            /* signed backtrace charge [C] */
+  double qabs_C;         /* |physical charge| for rigidity conversion [C] */
+  double m0_kg;
+  double dt_trace_s;     /* DT_TRACE */
+  double max_time_s;
+  int64_t max_steps;
+  double max_distance_m; /* <= 0: off */
+  double dphi_max;       /* gyro-phase budget per step [rad] (legacy 0.15) */
+  int32_t adaptive;      /* adaptiveDt */
+  int32_t dist_limiter;  /* 0.20 * d_near / v */
+  int32_t dt_floor;      /* legacy max(1e-12, 1e-9*DT, 100 km / v) floor */
+  int32_t outer_sphere;  /* 0: box, 1: sphere of radius r_outer_Re */
+  int32_t retry;         /* single retry on numerical failure */
+  int32_t pad_;
+  double box_min_Re[3], box_max_Re[3];
+  double r_outer_Re, r_inner_Re;
+  double earth_radius_m; /* length of one Re for all geometry conversions */
+  double moment_Am2;     /* dipole |m| */
+  double m_hat[3];
+  double B_ext_T[3];     /* uniform field added to the dipole */
+} amps_trace_params;
+
+/* TrajectoryResult (termination values of util/TrajectoryTermination.h:10-21) */
+typedef struct {
+  int32_t termination;
+  int32_t retry_count;
+  int64_t steps;
+  double time_s, distance_m;
+  double x[3], p[3];
+} amps_trace_result;
+
+/* Batched TraceTrajectoryShared (CutoffRigidityGridless.h:245-315): n
+ * trajectories from x0[3n] (m) with arrival directions dir[3n] (the backtrace
+ * starts along -dir) at rigidities R_GV[n].  One persistent kernel; lanes
+ * refill from a device work queue.  steps_total (may be NULL) receives the
+ * number of Boris steps taken. */
+int amps_gpu_trace_batch(amps_ctx* ctx, const amps_trace_params* params, int32_t precision,
+                         int64_t n, const double* x0, const double* dir, const double* R_GV,
+                         amps_trace_result* out, int64_t* steps_total);
+
+#define AMPS_SCAN_FULL 0   /* trace every bin of the bracket */
+#define AMPS_SCAN_UPPER 1  /* downward from the bracket top to the first forbidden bin */
+#define AMPS_BIN_ABOVE (-2) /* bracket top forbidden below Rmax: cutoff above the bracket */
+#define AMPS_BIN_BELOW (-3) /* bracket allowed down to a bottom above Rmin */
+/* Cutoff map (CutoffForDirectionUpperScan_GV, CutoffRigidityGridless.cpp:
+ * 2805-2834, on the grid of BuildCutoffSearchGrid_GV :2578-2617): for each of
+ * nloc (location, direction) pairs, classify the rigidities grid[nR] inside
+ * [bin_lo, bin_hi] (NULL: all bins).  states[nloc*nR]: -1 untraced,
+ * 0 forbidden (physical or trace limit), 1 allowed, 2 numerical failure.
+ * cutoff_bin[nloc]: index of the allowed side of the highest forbidden ->
+ * allowed transition, 0 when all allowed down to Rmin, -1 when Rmax is
+ * forbidden, or AMPS_BIN_ABOVE / AMPS_BIN_BELOW when the bracket does not
+ * contain the transition. */
+int amps_gpu_cutoff_scan(amps_ctx* ctx, const amps_trace_params* params, int32_t precision,
+                         int32_t mode, int64_t nloc, const double* x0, const double* dir,
+                         int32_t nR, const double* grid, const int32_t* bin_lo,
+                         const int32_t* bin_hi, int8_t* states, int32_t* cutoff_bin,
+                         int64_t* steps_total);
+/* Tracer tuning (autotuner): "workgroups" (persistent grid, 0 = 8 per CU),
+ * "chunk" (steps between work-queue refills). */
+int amps_gpu_trace_set_param(amps_ctx* ctx, const char* name, int64_t value);
+
+/* ---- learned cutoff surrogate (no reference counterpart) -------------- */
+/* MLP 6 -> 128 -> 128 -> 1 (GELU) on BF16 MFMA with FP32 accumulation,
+ * trained with Adam on the mean-squared error of log10 cutoff rigidity.
+ * Parameters (17,537 floats): W1[128][6] b1[128] W2[128][128] b2[128]
+ * w3[128] b3.  Inputs per sample: sin/cos latitude, sin/cos longitude,
+ * zenith and azimuth (scaled by the caller). */
+#define AMPS_MLP_NPARAM 17537
+int amps_gpu_mlp_init(amps_ctx* ctx, const float* theta);
+int amps_gpu_mlp_get_params(amps_ctx* ctx, float* theta, int64_t* adam_steps);
+/* Resident training set: X[n][6], targets T[n]. */
+int amps_gpu_mlp_set_data(amps_ctx* ctx, int64_t n, const float* X, const float* T);
+/* Gradient of the MSE over the first n (multiple of 64) resident rows. */
+int amps_gpu_mlp_grad(amps_ctx* ctx, int64_t n, float* grad, float* loss);
+/* nsteps Adam steps (batch a multiple of 64) over consecutive batches,
+ * replayed from one captured hipGraph. */
+int amps_gpu_mlp_train(amps_ctx* ctx, int64_t nsteps, int32_t batch, double lr, float* loss_out);
+int amps_gpu_mlp_predict(amps_ctx* ctx, int64_t n, const float* X, float* y);
+
 /* ---- profiling ------------------------------------------------------- */
 /* Per-kernel HIP-event timing on the library stream (ms, accumulated). */
 #define AMPS_K_PUSH 0

@@ -1,0 +1,131 @@
+"""GPU parity of the persistent cutoff tracer against the CPU oracle.
+
+FP64: termination, step count, time, path length, final x and p are
+bit-exact with the oracle (legacy deck and the batched-map parameterisation,
+plus the forced failure paths).  FP32: classification and cutoff-bin
+agreement with the FP64 oracle, stated as rates."""
+import numpy as np
+import pytest
+
+from amps_amd import _lib as L
+from amps_amd import cutoff as CU
+from oracle import oracle as O
+from tests import tracer_setup as TS
+
+pytestmark = pytest.mark.gpu
+
+
+def _oparams(P):
+    """The same parameter block as the oracle's struct."""
+    Q = O.TraceParams()
+    for name, _t in L.TraceParams._fields_:
+        v = getattr(P, name)
+        if hasattr(v, "__len__"):
+            for i in range(len(v)):
+                getattr(Q, name)[i] = v[i]
+        else:
+            setattr(Q, name, v)
+    return Q
+
+
+def _random_tasks(n, seed, rmin=0.1, rmax=20.0, rlo=1.05, rhi=1.3, re=CU.RE_DIPOLE):
+    rng = np.random.default_rng(seed)
+    x = rng.normal(size=(n, 3))
+    x *= (rng.uniform(rlo, rhi, n) * re / np.linalg.norm(x, axis=1))[:, None]
+    d = rng.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1)[:, None]
+    # arrival directions mostly downward
+    d -= 1.2 * x / np.linalg.norm(x, axis=1)[:, None]
+    d /= np.linalg.norm(d, axis=1)[:, None]
+    R = np.exp(rng.uniform(np.log(rmin), np.log(rmax), n))
+    return x, d, R
+
+
+def _same(g, o):
+    for f in ["termination", "retry_count", "steps"]:
+        assert np.array_equal(g[f], o[f]), f
+    for f in ["time_s", "distance_m", "x", "p"]:
+        assert np.array_equal(g[f], o[f]), f
+
+
+@pytest.mark.parametrize("which", ["config2", "legacy"])
+def test_trace_fp64_bitexact(gpu_ctx, which):
+    if which == "config2":
+        P = CU.config2_params()
+        P.max_steps = 20000
+    else:
+        P = CU.legacy_params(dphi=0.15)
+        P.max_time_s = 60.0
+    x, d, R = _random_tasks(3000, 3)
+    print(f"trace case {which} n={len(R)}", flush=True)
+    g, steps = CU.trace_batch(gpu_ctx, P, x, d, R, precision=L.FP64)
+    o = O.trace_batch(_oparams(P), x, d, R)
+    _same(g, o)
+    assert steps == int(o["steps"].sum())
+    assert len(np.unique(o["termination"])) >= 2
+
+
+def test_trace_forced_failures_match_oracle(gpu_ctx):
+    """Host-forced failure paths: zero step (InvalidTimeStep -> one retry with
+    dt = 1e-12 -> step limit), a tiny step cap, a start inside the inner sphere."""
+    x, d, R = _random_tasks(64, 9)
+    P = CU.legacy_params(adaptive=False, dt_trace=0.0, max_steps=50)
+    g, _ = CU.trace_batch(gpu_ctx, P, x, d, R)
+    o = O.trace_batch(_oparams(P), x, d, R)
+    _same(g, o)
+    assert np.all(o["retry_count"] == 1) and np.all(o["termination"] == 4)
+    P = CU.legacy_params(max_steps=3)
+    g, _ = CU.trace_batch(gpu_ctx, P, x, d, R)
+    o = O.trace_batch(_oparams(P), x, d, R)
+    _same(g, o)
+    assert np.all(o["termination"] == 4)
+    xi = x * 0.5
+    P = CU.legacy_params()
+    g, _ = CU.trace_batch(gpu_ctx, P, xi, d, R)
+    o = O.trace_batch(_oparams(P), xi, d, R)
+    _same(g, o)
+    assert np.all(o["termination"] == 1) and np.all(o["steps"] == 0)
+
+
+def test_cutoff_scan_full_vs_upper_and_oracle(gpu_ctx):
+    P = CU.config2_params()
+    P.max_steps = 20000
+    pts, lat, lon = CU.config2_points(nlon=8, nlat=6)
+    zen, az = CU.config2_directions(4)
+    dirs = CU.arrival_dirs(pts, zen, az).reshape(-1, 3)
+    x0 = np.repeat(pts, len(zen), axis=0)
+    grid = CU.build_search_grid(0.1, 20.0, 48)
+    og = O.rigidity_grid(0.1, 20.0, 48)
+    assert np.array_equal(grid, og)
+    sf, bf, _ = CU.cutoff_scan(gpu_ctx, P, x0, dirs, grid, mode=CU.SCAN_FULL)
+    su, bu, _ = CU.cutoff_scan(gpu_ctx, P, x0, dirs, grid, mode=CU.SCAN_UPPER)
+    assert np.array_equal(bf, bu)
+    traced = su >= 0
+    assert np.array_equal(su[traced], sf[traced])
+    # oracle on every (location, bin)
+    o = O.trace_batch(_oparams(P), np.repeat(x0, 48, axis=0), np.repeat(dirs, 48, axis=0),
+                      np.tile(grid, len(x0)))
+    ost = (o["termination"] == 0).astype(np.int8).reshape(-1, 48)
+    assert np.array_equal(sf, ost)
+    ob = np.array([O.upper_cutoff_bin(r) for r in ost])
+    assert np.array_equal(bf, ob)
+
+
+def test_fp32_tracer_agreement(gpu_ctx):
+    """FP32 arithmetic with FP64 position accumulation vs the FP64 oracle:
+    per-trajectory classification and per-direction cutoff bin."""
+    P = CU.config2_params()
+    P.max_steps = 20000
+    pts, lat, lon = CU.config2_points(nlon=12, nlat=8)
+    zen, az = CU.config2_directions(4)
+    dirs = CU.arrival_dirs(pts, zen, az).reshape(-1, 3)
+    x0 = np.repeat(pts, len(zen), axis=0)
+    grid = CU.build_search_grid(0.1, 20.0, 48)
+    s64, b64, _ = CU.cutoff_scan(gpu_ctx, P, x0, dirs, grid, mode=CU.SCAN_FULL, precision=L.FP64)
+    s32, b32, _ = CU.cutoff_scan(gpu_ctx, P, x0, dirs, grid, mode=CU.SCAN_FULL, precision=L.FP32)
+    agree = np.mean(s64 == s32)
+    off = np.abs(b64.astype(int) - b32.astype(int))
+    frac_gt1 = np.mean(off > 1)
+    print(f"FP32 vs FP64: state agreement {agree:.5f}, bins off by >1: {frac_gt1:.5f}")
+    assert agree >= 0.97
+    assert frac_gt1 <= 0.03
