@@ -104,6 +104,9 @@ def load():
         "amps_gpu_move_particles": (C.c_int, [vp, P(PushOpts)]),
         "amps_gpu_compact": (C.c_int, [vp]),
         "amps_gpu_update_j_mass_matrix": (C.c_int, [vp, dbl, P(dbl), P(dbl)]),
+        "amps_gpu_generate_particles": (C.c_int, [vp, i64, i32, P(dbl), P(dbl), C.c_uint64]),
+        "amps_gpu_run_steps": (C.c_int, [vp, P(PushOpts), dbl, i64, i32]),
+        "amps_gpu_copy_probe": (C.c_int, [vp, i64, i32, P(dbl)]),
         "amps_gpu_trace_batch": (C.c_int, [vp, P(TraceParams), i32, i64, P(dbl), P(dbl), P(dbl),
                                            vp, P(i64)]),
         "amps_gpu_cutoff_scan": (C.c_int, [vp, P(TraceParams), i32, i32, i64, P(dbl), P(dbl), i32,
@@ -181,6 +184,11 @@ class Context:
         hbm = C.c_int64()
         check(self.lib.amps_gpu_device_info(self.h, name, C.byref(ncu), C.byref(hbm)), "device_info")
         return name.value.decode(), ncu.value, hbm.value
+
+    def copy_probe(self, nbytes, reps=10):
+        ms = C.c_double()
+        check(self.lib.amps_gpu_copy_probe(self.h, int(nbytes), int(reps), C.byref(ms)), "copy_probe")
+        return ms.value
 
     def set_param(self, name, value):
         check(self.lib.amps_gpu_set_param(self.h, name.encode(), int(value)), f"set_param {name}")

@@ -1,0 +1,103 @@
+"""Autotuner for the launch parameters of the particle step and the tracer.
+
+Each candidate is timed with HIP events on the library stream over a few
+repetitions on the caller's real workload; the fastest is applied and
+returned.  Results can be cached in a JSON file keyed by a workload
+signature so that later runs skip the sweep.
+"""
+import json
+import os
+import time
+
+import numpy as np
+
+from . import _lib as L
+
+PUSH_TILES = (1024, 2048, 4096, 8192)
+DEPOSIT_CHUNKS = (3, 4, 6)
+
+
+def _load(path):
+    if path and os.path.exists(path):
+        try:
+            with open(path) as f:
+                return json.load(f)
+        except (OSError, ValueError):
+            return {}
+    return {}
+
+
+def _save(path, d):
+    if not path:
+        return
+    try:
+        with open(path, "w") as f:
+            json.dump(d, f, indent=1, sort_keys=True)
+    except OSError:
+        pass
+
+
+def tune_pic(ctx, state, opts_kw, deposit_dt, signature, cache=None, reps=2):
+    """Pick the push tile and the deposit chunk count for a PicState."""
+    db = _load(cache)
+    if signature in db:
+        best = db[signature]
+        ctx.set_param("tile", best["tile"])
+        ctx.set_param("deposit_chunks", best["deposit_chunks"])
+        return best, True
+    ctx.set_timing(True)
+    res = {}
+    for tile in PUSH_TILES:
+        ctx.set_param("tile", tile)
+        ctx.reset_timing()
+        for _ in range(reps):
+            state.MoveParticles(compact=True, **opts_kw)
+        ms, n = ctx.timing()
+        res[("tile", tile)] = ms[L.K_PUSH] / max(n[L.K_PUSH], 1)
+    best_tile = min(PUSH_TILES, key=lambda t: res[("tile", t)])
+    ctx.set_param("tile", best_tile)
+    for nch in DEPOSIT_CHUNKS:
+        ctx.set_param("deposit_chunks", nch)
+        ctx.reset_timing()
+        for _ in range(reps):
+            state.UpdateJMassMatrix(deposit_dt, copy_out=False)
+        ms, n = ctx.timing()
+        res[("dep", nch)] = (ms[L.K_DEPOSIT_CELL] + ms[L.K_DEPOSIT_NODE]) / max(n[L.K_DEPOSIT_CELL], 1)
+    best_nch = min(DEPOSIT_CHUNKS, key=lambda c: res[("dep", c)])
+    ctx.set_param("deposit_chunks", best_nch)
+    ctx.set_timing(False)
+    best = {"tile": int(best_tile), "deposit_chunks": int(best_nch),
+            "push_ms": {str(t): round(res[("tile", t)], 4) for t in PUSH_TILES},
+            "deposit_ms": {str(c): round(res[("dep", c)], 4) for c in DEPOSIT_CHUNKS}}
+    db[signature] = best
+    _save(cache, db)
+    return best, False
+
+
+def tune_tracer(ctx, run, n_cu, signature, cache=None):
+    """Pick the persistent grid size and refill chunk for the tracer; run()
+    executes one representative tracing call."""
+    from . import cutoff as CU
+
+    db = _load(cache)
+    if signature in db:
+        best = db[signature]
+        CU.set_trace_param(ctx, "workgroups", best["workgroups"])
+        CU.set_trace_param(ctx, "chunk", best["chunk"])
+        return best, True
+    res = {}
+    for wg in (2 * n_cu, 3 * n_cu, 4 * n_cu, 8 * n_cu):
+        for ch in (16, 64):
+            CU.set_trace_param(ctx, "workgroups", wg)
+            CU.set_trace_param(ctx, "chunk", ch)
+            t0 = time.perf_counter()
+            run()
+            res[(wg, ch)] = time.perf_counter() - t0
+    (wg, ch) = min(res, key=res.get)
+    CU.set_trace_param(ctx, "workgroups", wg)
+    CU.set_trace_param(ctx, "chunk", ch)
+    best = {"workgroups": int(wg), "chunk": int(ch),
+            "seconds": {f"{k[0]}x{k[1]}": round(v, 4) for k, v in res.items()}}
+    db[signature] = best
+    _save(cache, db)
+    return best, False

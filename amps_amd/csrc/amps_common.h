@@ -49,6 +49,14 @@ struct Faces {
   Face f[6];
 };
 
+// Synthetic-population generator arguments (k_generate).
+struct GenArgs {
+  int64_t ntotal, base, rem;
+  int nspec;
+  double vth[MAX_SPEC], drift[3];
+  uint64_t seed;
+};
+
 }  // namespace amps
 
 struct amps_ctx {
@@ -108,6 +116,9 @@ struct amps_ctx {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   double ms[AMPS_K_COUNT] = {0};
   int64_t launches[AMPS_K_COUNT] = {0};
+  // captured full-step graph
+  hipGraph_t step_graph = nullptr;
+  hipGraphExec_t step_exec = nullptr;
   // tracer / surrogate state lives in their own files
   void* trace_state = nullptr;
   void* mlp_state = nullptr;

@@ -30,6 +30,8 @@ template <int NCH, bool EXACT>
 __global__ void k_deposit_cell(MeshView, Species, double, const double*, const int*, int64_t,
                                const double*, const double*, const uint8_t*, double*);
 __global__ void k_deposit_node(int64_t, const int*, const int*, const double*, double*);
+__global__ void k_generate(MeshView, GenArgs, int64_t, double*, double*, uint8_t*, int*, int64_t*, int8_t*);
+__global__ void k_copy_probe(const float4*, float4*, int64_t);
 namespace amps {
 int upload_inv_index_matrix(hipStream_t s);
 }
@@ -136,6 +138,8 @@ extern "C" int amps_gpu_destroy(amps_ctx* c) {
   (void)hipFree(c->d_ntiles);
   (void)hipFree(c->d_tiles);
   (void)hipFree(c->d_scan_tmp);
+  if (c->step_exec) (void)hipGraphExecDestroy(c->step_exec);
+  if (c->step_graph) (void)hipGraphDestroy(c->step_graph);
   (void)hipEventDestroy(c->ev0);
   (void)hipEventDestroy(c->ev1);
   (void)hipStreamDestroy(c->stream);
@@ -400,6 +404,23 @@ extern "C" int amps_gpu_set_species(amps_ctx* c, int32_t nspec, const double* ch
 // ---------------------------------------------------------------------
 // particles
 // ---------------------------------------------------------------------
+static int alloc_particles(amps_ctx* c, int64_t n) {
+  if (n <= c->cap) return 0;
+  free_particles(c);
+  const int64_t cap = std::max<int64_t>(n, 64);
+  for (int i = 0; i < 2; i++) {
+    HIP_TRY(hipMalloc(&c->d_x[i], sizeof(double) * 3 * cap));
+    HIP_TRY(hipMalloc(&c->d_v[i], sizeof(double) * 3 * cap));
+    HIP_TRY(hipMalloc(&c->d_spec[i], cap));
+    HIP_TRY(hipMalloc(&c->d_key[i], sizeof(int) * cap));
+    HIP_TRY(hipMalloc(&c->d_id[i], sizeof(int64_t) * cap));
+  }
+  HIP_TRY(hipMalloc(&c->d_code, (cap + 4) & ~(int64_t)3));
+  HIP_TRY(hipMalloc(&c->d_code2, (cap + 4) & ~(int64_t)3));
+  c->cap = cap;
+  return 0;
+}
+
 static int build_tiles(amps_ctx* c) {
   const int nb = c->mv.nb;
   const int64_t need = nb + c->cap / c->tile + 2;
@@ -442,20 +463,7 @@ extern "C" int amps_gpu_set_particles(amps_ctx* c, int64_t n, const double* x, c
   std::vector<int> fill(cnt.begin(), cnt.end() - 1);
   std::vector<int64_t> perm((size_t)n);
   for (int64_t p = 0; p < n; p++) perm[(size_t)fill[(size_t)blk[p] * NCELL + cell[p]]++] = p;
-  if (n > c->cap) {
-    free_particles(c);
-    const int64_t cap = std::max<int64_t>(n, 64);
-    for (int i = 0; i < 2; i++) {
-      HIP_TRY(hipMalloc(&c->d_x[i], sizeof(double) * 3 * cap));
-      HIP_TRY(hipMalloc(&c->d_v[i], sizeof(double) * 3 * cap));
-      HIP_TRY(hipMalloc(&c->d_spec[i], cap));
-      HIP_TRY(hipMalloc(&c->d_key[i], sizeof(int) * cap));
-      HIP_TRY(hipMalloc(&c->d_id[i], sizeof(int64_t) * cap));
-    }
-    HIP_TRY(hipMalloc(&c->d_code, (cap + 4) & ~(int64_t)3));
-    HIP_TRY(hipMalloc(&c->d_code2, (cap + 4) & ~(int64_t)3));
-    c->cap = cap;
-  }
+  if (int r = alloc_particles(c, n)) return r;
   const int64_t cap = c->cap;
   std::vector<double> hx(3 * (size_t)cap, 0.0), hv(3 * (size_t)cap, 0.0);
   std::vector<uint8_t> hs((size_t)cap, 0);
@@ -573,6 +581,22 @@ extern "C" int amps_gpu_compact(amps_ctx* c) {
 // ---------------------------------------------------------------------
 // mover
 // ---------------------------------------------------------------------
+static int launch_push(amps_ctx* c, const amps_push_opts* o) {
+  const int k = c->cur;
+  if (o->mover == AMPS_MOVER_LAPENTA)
+    hipLaunchKernelGGL(k_push_f64<AMPS_MOVER_LAPENTA>, dim3((unsigned)c->max_tiles), dim3(256), 0,
+                       c->stream, c->mv, c->faces, c->sp, c->d_E, c->d_B, c->d_tiles, c->d_ntiles,
+                       o->bc_mode, o->r_sphere, c->cap, c->d_x[k], c->d_v[k], c->d_spec[k],
+                       c->d_key[k], c->d_code);
+  else
+    hipLaunchKernelGGL(k_push_f64<AMPS_MOVER_BORIS>, dim3((unsigned)c->max_tiles), dim3(256), 0,
+                       c->stream, c->mv, c->faces, c->sp, c->d_E, c->d_B, c->d_tiles, c->d_ntiles,
+                       o->bc_mode, o->r_sphere, c->cap, c->d_x[k], c->d_v[k], c->d_spec[k],
+                       c->d_key[k], c->d_code);
+  HIP_TRY(hipGetLastError());
+  return 0;
+}
+
 extern "C" int amps_gpu_move_particles(amps_ctx* c, const amps_push_opts* o) {
   if (!c || !o) return fail(AMPS_E_ARG, "amps_gpu_move_particles: null argument");
   if (!c->has_mesh || !c->has_fields || !c->has_species)
@@ -591,19 +615,8 @@ extern "C" int amps_gpu_move_particles(amps_ctx* c, const amps_push_opts* o) {
   } else if (!c->tiles_valid) {
     if (int r = build_tiles(c)) return r;
   }
-  const int k = c->cur;
   tbegin(c);
-  if (o->mover == AMPS_MOVER_LAPENTA)
-    hipLaunchKernelGGL(k_push_f64<AMPS_MOVER_LAPENTA>, dim3((unsigned)c->max_tiles), dim3(256), 0,
-                       c->stream, c->mv, c->faces, c->sp, c->d_E, c->d_B, c->d_tiles, c->d_ntiles,
-                       o->bc_mode, o->r_sphere, c->cap, c->d_x[k], c->d_v[k], c->d_spec[k],
-                       c->d_key[k], c->d_code);
-  else
-    hipLaunchKernelGGL(k_push_f64<AMPS_MOVER_BORIS>, dim3((unsigned)c->max_tiles), dim3(256), 0,
-                       c->stream, c->mv, c->faces, c->sp, c->d_E, c->d_B, c->d_tiles, c->d_ntiles,
-                       o->bc_mode, o->r_sphere, c->cap, c->d_x[k], c->d_v[k], c->d_spec[k],
-                       c->d_key[k], c->d_code);
-  HIP_TRY(hipGetLastError());
+  if (int r = launch_push(c, o)) return r;
   tend(c, AMPS_K_PUSH);
   c->offsets_valid = false;
   c->tiles_valid = false;
@@ -622,34 +635,43 @@ static void launch_dep(amps_ctx* c, double dt) {
                      c->d_v[k], c->d_spec[k], c->d_cellpart);
 }
 
+static int launch_deposit(amps_ctx* c, double dt) {
+  tbegin(c);
+  const bool ex = c->dep_exact != 0;
+  switch (c->dep_nch) {
+    case 2: ex ? launch_dep<2, true>(c, dt) : launch_dep<2, false>(c, dt); break;
+    case 3: ex ? launch_dep<3, true>(c, dt) : launch_dep<3, false>(c, dt); break;
+    case 6: ex ? launch_dep<6, true>(c, dt) : launch_dep<6, false>(c, dt); break;
+    default: ex ? launch_dep<4, true>(c, dt) : launch_dep<4, false>(c, dt); break;
+  }
+  HIP_TRY(hipGetLastError());
+  tend(c, AMPS_K_DEPOSIT_CELL);
+  tbegin(c);
+  hipLaunchKernelGGL(k_deposit_node, dim3((unsigned)((c->nnodes + 3) / 4)), dim3(256), 0, c->stream,
+                     c->nnodes, c->d_inc_start, c->d_inc, c->d_cellpart, c->d_nodeout);
+  HIP_TRY(hipGetLastError());
+  tend(c, AMPS_K_DEPOSIT_NODE);
+  return 0;
+}
+
+static int ensure_deposit_buffers(amps_ctx* c) {
+  if (!c->d_cellpart) HIP_TRY(hipMalloc(&c->d_cellpart, sizeof(double) * (size_t)c->mv.nb * NCELL * CELL_VALS));
+  if (!c->d_nodeout) HIP_TRY(hipMalloc(&c->d_nodeout, sizeof(double) * (size_t)c->nnodes * NODE_VALS));
+  return 0;
+}
+
 extern "C" int amps_gpu_update_j_mass_matrix(amps_ctx* c, double dt, double* J_out, double* M_out) {
   if (!c) return fail(AMPS_E_ARG, "null context");
   if (!c->has_mesh || !c->has_fields || !c->has_species)
     return fail(AMPS_E_STATE, "amps_gpu_update_j_mass_matrix: mesh, fields and species are required");
   (void)hipSetDevice(c->device);
-  if (!c->d_cellpart) HIP_TRY(hipMalloc(&c->d_cellpart, sizeof(double) * (size_t)c->mv.nb * NCELL * CELL_VALS));
-  if (!c->d_nodeout) HIP_TRY(hipMalloc(&c->d_nodeout, sizeof(double) * (size_t)c->nnodes * NODE_VALS));
+  if (int r = ensure_deposit_buffers(c)) return r;
   if (c->cap == 0) {
     HIP_TRY(hipMemsetAsync(c->d_nodeout, 0, sizeof(double) * (size_t)c->nnodes * NODE_VALS, c->stream));
   } else {
     if (!c->offsets_valid)
       if (int r = amps_gpu_compact(c)) return r;
-    tbegin(c);
-    const bool ex = c->dep_exact != 0;
-    switch (c->dep_nch) {
-      case 2: ex ? launch_dep<2, true>(c, dt) : launch_dep<2, false>(c, dt); break;
-      case 3: ex ? launch_dep<3, true>(c, dt) : launch_dep<3, false>(c, dt); break;
-      case 6: ex ? launch_dep<6, true>(c, dt) : launch_dep<6, false>(c, dt); break;
-      default: ex ? launch_dep<4, true>(c, dt) : launch_dep<4, false>(c, dt); break;
-    }
-    HIP_TRY(hipGetLastError());
-    tend(c, AMPS_K_DEPOSIT_CELL);
-    tbegin(c);
-    hipLaunchKernelGGL(k_deposit_node, dim3((unsigned)((c->nnodes + 3) / 4)), dim3(256), 0,
-                       c->stream, c->nnodes, c->d_inc_start, c->d_inc, c->d_cellpart,
-                       c->d_nodeout);
-    HIP_TRY(hipGetLastError());
-    tend(c, AMPS_K_DEPOSIT_NODE);
+    if (int r = launch_deposit(c, dt)) return r;
   }
   if (J_out || M_out) {
     std::vector<double> h((size_t)c->nnodes * NODE_VALS);
@@ -682,5 +704,124 @@ extern "C" int amps_gpu_get_timing(amps_ctx* c, double* ms, int64_t* launches) {
 extern "C" int amps_gpu_reset_timing(amps_ctx* c) {
   if (!c) return fail(AMPS_E_ARG, "null context");
   for (int k = 0; k < AMPS_K_COUNT; k++) c->ms[k] = 0, c->launches[k] = 0;
+  return AMPS_OK;
+}
+
+// ---------------------------------------------------------------------
+// synthetic population, full steps, copy probe
+// ---------------------------------------------------------------------
+extern "C" int amps_gpu_generate_particles(amps_ctx* c, int64_t n_per_species, int32_t nspec,
+                                           const double* vth, const double* drift, uint64_t seed) {
+  if (!c || !vth || !drift) return fail(AMPS_E_ARG, "amps_gpu_generate_particles: null argument");
+  if (!c->has_mesh) return fail(AMPS_E_STATE, "amps_gpu_generate_particles: set the mesh first");
+  if (nspec <= 0 || nspec > MAX_SPEC) return fail(AMPS_E_ARG, "amps_gpu_generate_particles: nspec out of [1,8]");
+  if (c->has_species && nspec > c->sp.n) return fail(AMPS_E_ARG, "amps_gpu_generate_particles: more species than set");
+  if (n_per_species <= 0 || n_per_species * nspec >= ((int64_t)1 << 31) - 1)
+    return fail(AMPS_E_RANGE, "amps_gpu_generate_particles: total count out of [1, 2^31-1)");
+  (void)hipSetDevice(c->device);
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  const int64_t ntot = n_per_species * nspec;
+  if (int r = alloc_particles(c, ntot)) return r;
+  const int64_t ncells = (int64_t)c->mv.nb * NCELL;
+  GenArgs ga;
+  std::memset(&ga, 0, sizeof(ga));
+  ga.ntotal = ntot;
+  ga.base = n_per_species / ncells;
+  ga.rem = n_per_species % ncells;
+  ga.nspec = nspec;
+  for (int s2 = 0; s2 < nspec; s2++) ga.vth[s2] = vth[s2];
+  for (int d = 0; d < 3; d++) ga.drift[d] = drift[d];
+  ga.seed = seed;
+  c->cur = 0;
+  hipLaunchKernelGGL(k_generate, dim3((unsigned)((ntot + 255) / 256)), dim3(256), 0, c->stream, c->mv, ga,
+                     c->cap, c->d_x[0], c->d_v[0], c->d_spec[0], c->d_key[0], c->d_id[0], c->d_code);
+  HIP_TRY(hipGetLastError());
+  const int nn = (int)ntot;
+  HIP_TRY(hipMemcpyAsync(c->d_n, &nn, sizeof(int), hipMemcpyHostToDevice, c->stream));
+  c->n = ntot;
+  c->precision = AMPS_FP64;
+  if (int r = compact_impl(c)) return r;
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return AMPS_OK;
+}
+
+static int one_step(amps_ctx* c, const amps_push_opts* o, double dt) {
+  tbegin(c);
+  if (int r = launch_push(c, o)) return r;
+  tend(c, AMPS_K_PUSH);
+  tbegin(c);
+  if (int r = compact_impl(c)) return r;
+  tend(c, AMPS_K_COMPACT);
+  return launch_deposit(c, dt);
+}
+
+extern "C" int amps_gpu_run_steps(amps_ctx* c, const amps_push_opts* o, double dt, int64_t nsteps,
+                                  int32_t use_graph) {
+  if (!c || !o) return fail(AMPS_E_ARG, "amps_gpu_run_steps: null argument");
+  if (!c->has_mesh || !c->has_fields || !c->has_species || c->cap == 0)
+    return fail(AMPS_E_STATE, "amps_gpu_run_steps: mesh, fields, species and particles are required");
+  if (o->mover != AMPS_MOVER_LAPENTA && o->mover != AMPS_MOVER_BORIS)
+    return fail(AMPS_E_ARG, "amps_gpu_run_steps: unknown mover");
+  if (o->bc_mode != AMPS_BC_DELETE && o->bc_mode != AMPS_BC_REFLECT)
+    return fail(AMPS_E_ARG, "amps_gpu_run_steps: unknown boundary mode");
+  if (nsteps <= 0) return AMPS_OK;
+  (void)hipSetDevice(c->device);
+  if (o->tile > 0 && o->tile != c->tile)
+    if (int r = amps_gpu_set_param(c, "tile", o->tile)) return r;
+  if (int r = ensure_deposit_buffers(c)) return r;
+  if (!c->offsets_valid)
+    if (int r = amps_gpu_compact(c)) return r;
+  if (!c->tiles_valid)
+    if (int r = build_tiles(c)) return r;
+  // eager first step (allocates every lazily sized buffer)
+  if (int r = one_step(c, o, dt)) return r;
+  int64_t done = 1;
+  if (use_graph && nsteps > 1) {
+    if (c->step_exec) (void)hipGraphExecDestroy(c->step_exec);
+    if (c->step_graph) (void)hipGraphDestroy(c->step_graph);
+    c->step_exec = nullptr;
+    c->step_graph = nullptr;
+    const bool tim = c->timing;
+    c->timing = false;
+    // two steps per graph: the ping-pong particle buffers and code arrays
+    // return to their captured roles after every replay
+    HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    int r = one_step(c, o, dt);
+    if (!r) r = one_step(c, o, dt);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(c->stream, &g);
+    c->timing = tim;
+    if (r) return r;
+    if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
+    c->step_graph = g;
+    HIP_TRY(hipGraphInstantiate(&c->step_exec, g, nullptr, nullptr, 0));
+    for (; done + 2 <= nsteps; done += 2) HIP_TRY(hipGraphLaunch(c->step_exec, c->stream));
+  }
+  for (; done < nsteps; done++)
+    if (int r = one_step(c, o, dt)) return r;
+  c->offsets_valid = true;
+  c->tiles_valid = true;
+  return AMPS_OK;
+}
+
+extern "C" int amps_gpu_copy_probe(amps_ctx* c, int64_t nbytes, int32_t reps, double* ms) {
+  if (!c || nbytes < 16 || reps <= 0) return fail(AMPS_E_ARG, "amps_gpu_copy_probe: bad arguments");
+  (void)hipSetDevice(c->device);
+  float4 *a = nullptr, *b = nullptr;
+  const int64_t n4 = nbytes / 16;
+  HIP_TRY(hipMalloc(&a, n4 * 16));
+  HIP_TRY(hipMalloc(&b, n4 * 16));
+  hipLaunchKernelGGL(k_fill_int, dim3(4096), dim3(256), 0, c->stream, (int*)a, n4 * 4, 1);
+  hipLaunchKernelGGL(k_copy_probe, dim3(8192), dim3(256), 0, c->stream, a, b, n4);
+  HIP_TRY(hipEventRecord(c->ev0, c->stream));
+  for (int i = 0; i < reps; i++)
+    hipLaunchKernelGGL(k_copy_probe, dim3(8192), dim3(256), 0, c->stream, a, b, n4);
+  HIP_TRY(hipEventRecord(c->ev1, c->stream));
+  HIP_TRY(hipEventSynchronize(c->ev1));
+  float t = 0.f;
+  HIP_TRY(hipEventElapsedTime(&t, c->ev0, c->ev1));
+  if (ms) *ms = t / reps;
+  (void)hipFree(a);
+  (void)hipFree(b);
   return AMPS_OK;
 }

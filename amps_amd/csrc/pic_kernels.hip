@@ -377,3 +377,75 @@ int upload_inv_index_matrix(hipStream_t s) {
   return 0;
 }
 }  // namespace amps
+
+// ======================================================================
+// Synthetic population on the device: counter-based RNG (SplitMix64 of
+// seed and particle index), equal counts per cell with the first cells
+// taking the remainder, uniform positions, Maxwellian velocities.
+// ======================================================================
+__device__ __forceinline__ uint64_t splitmix64(uint64_t x) {
+  x += 0x9E3779B97F4A7C15ull;
+  x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+  x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+  return x ^ (x >> 31);
+}
+__device__ __forceinline__ double u01(uint64_t h) { return (double)(h >> 11) * (1.0 / 9007199254740992.0); }
+
+
+__global__ __launch_bounds__(256) void k_generate(MeshView m, GenArgs ga, int64_t cap, double* x,
+                                                   double* v, uint8_t* spec, int* key, int64_t* id,
+                                                   int8_t* code) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= ga.ntotal) return;
+  const int64_t ns = ga.nspec;
+  const int64_t big = ns * ga.rem * (ga.base + 1);
+  int64_t c, j;
+  if (i < big) {
+    c = i / (ns * (ga.base + 1));
+    j = i - c * ns * (ga.base + 1);
+  } else {
+    c = ga.rem + (i - big) / (ns * ga.base);
+    j = i - big - (c - ga.rem) * ns * ga.base;
+  }
+  const int64_t nc = ga.base + (c < ga.rem ? 1 : 0);
+  const int s = (int)(j / nc);
+  const int b = (int)(c / NCELL), lc = (int)(c % NCELL);
+  const int ci[3] = {lc % NC, (lc / NC) % NC, lc / (NC * NC)};
+  double xp[3], vp[3];
+  const uint64_t key0 = splitmix64(ga.seed ^ ((uint64_t)i * 0x632BE59BD9B4E019ull));
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    const double dx = m.blk_dxs[3 * b + d];
+    const double lo = m.blk_xmin[3 * b + d] + ci[d] * dx;
+    double xx = lo + u01(splitmix64(key0 + d)) * dx;
+    if (!(xx < lo + dx)) xx = lo;
+    xp[d] = xx;
+  }
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    const double u1 = 1.0 - u01(splitmix64(key0 + 3 + 2 * d));
+    const double u2 = u01(splitmix64(key0 + 4 + 2 * d));
+    vp[d] = ga.drift[d] + ga.vth[s] * sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2);
+  }
+  int nb = find_block(m, xp);
+  int cell = (nb >= 0) ? find_cell(m, xp, nb) : -1;
+  if (nb < 0 || cell < 0) {
+    nb = b;
+    cell = lc;
+  }
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    x[d * cap + i] = xp[d];
+    v[d * cap + i] = vp[d];
+  }
+  spec[i] = (uint8_t)s;
+  key[i] = nb * NCELL + cell;
+  id[i] = i;
+  code[i] = AMPS_CODE_FINISHED;
+}
+
+__global__ __launch_bounds__(256) void k_copy_probe(const float4* __restrict__ a, float4* __restrict__ b,
+                                                     int64_t n4) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n4; i += (int64_t)gridDim.x * blockDim.x)
+    b[i] = a[i];
+}

@@ -116,6 +116,26 @@ int amps_gpu_move_particles(amps_ctx* ctx, const amps_push_opts* opts);
  * particles dropped (replaces the temp-list swap, pic_mover.cpp:982-992). */
 int amps_gpu_compact(amps_ctx* ctx);
 
+/* Synthetic population generated on the device (no reference counterpart;
+ * the reference's test apps fill cells on the host, srcFastWave/main.cpp:
+ * 120-300): n_per_species particles of each species spread over all cells
+ * (equal counts, the first cells get one extra), uniform in each cell,
+ * Maxwellian with thermal speed vth[s] plus drift[3].  Ids are consecutive;
+ * block/cell ids come from the same point location as the mover. */
+int amps_gpu_generate_particles(amps_ctx* ctx, int64_t n_per_species, int32_t nspec,
+                                const double* vth, const double* drift, uint64_t seed);
+
+/* nsteps full particle steps: MoveParticles, migration (compaction) and
+ * UpdateJMassMatrix at deposit_dt.  use_graph=1 runs one eager step, then
+ * replays a hipGraph holding two captured steps (an odd remainder runs
+ * eagerly). */
+int amps_gpu_run_steps(amps_ctx* ctx, const amps_push_opts* opts, double deposit_dt,
+                       int64_t nsteps, int32_t use_graph);
+
+/* Device copy of nbytes (read + write) repeated reps times; ms receives the
+ * average duration.  Used to calibrate HBM counters and bandwidth. */
+int amps_gpu_copy_probe(amps_ctx* ctx, int64_t nbytes, int32_t reps, double* ms);
+
 /* ---- ECSIM deposition ------------------------------------------------ */
 /* ECSIM::UpdateJMassMatrix (pic_field_solver_ecsim.cpp:3244-3995) through
  * ProcessCell (:1882-2438).  J_out[nnodes*3] and M_out[nnodes*243] (slot
