@@ -104,7 +104,7 @@ def load():
         "amps_gpu_move_particles": (C.c_int, [vp, P(PushOpts)]),
         "amps_gpu_compact": (C.c_int, [vp]),
         "amps_gpu_update_j_mass_matrix": (C.c_int, [vp, dbl, P(dbl), P(dbl)]),
-        "amps_gpu_generate_particles": (C.c_int, [vp, i64, i32, P(dbl), P(dbl), C.c_uint64]),
+        "amps_gpu_generate_particles": (C.c_int, [vp, i64, i32, P(dbl), P(dbl), C.c_uint64, i32]),
         "amps_gpu_run_steps": (C.c_int, [vp, P(PushOpts), dbl, i64, i32]),
         "amps_gpu_copy_probe": (C.c_int, [vp, i64, i32, P(dbl)]),
         "amps_gpu_trace_batch": (C.c_int, [vp, P(TraceParams), i32, i64, P(dbl), P(dbl), P(dbl),

@@ -55,6 +55,7 @@ struct GenArgs {
   int nspec;
   double vth[MAX_SPEC], drift[3];
   uint64_t seed;
+  int vf32;  // velocities stored as float
 };
 
 }  // namespace amps

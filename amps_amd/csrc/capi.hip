@@ -18,17 +18,21 @@ template <int MOVER>
 __global__ void k_push_f64(MeshView, Faces, Species, const double*, const double*, const Tile*,
                            const int*, int, double, int64_t, double*, double*, const uint8_t*,
                            int*, int8_t*);
+template <int MOVER>
+__global__ void k_push_f32(MeshView, Faces, Species, const double*, const double*, const Tile*,
+                           const int*, int, double, int64_t, double*, float*, const uint8_t*, int*,
+                           int8_t*);
 __global__ void k_fill_int(int*, int64_t, int);
 __global__ void k_count_keys(const int*, const int*, const int8_t*, int*);
 __global__ void k_scatter(const int*, const int*, const int8_t*, const int*, int*, int64_t,
-                          const double*, const double*, const uint8_t*, const int64_t*, double*,
-                          double*, uint8_t*, int*, int64_t*);
+                          const double*, const double*, int, const uint8_t*, const int64_t*,
+                          double*, double*, uint8_t*, int*, int64_t*);
 __global__ void k_copy_total(const int*, int*);
 __global__ void k_tile_count(int, int, const int*, int*);
 __global__ void k_tile_emit(int, int, const int*, const int*, Tile*, int*);
 template <int NCH, bool EXACT>
 __global__ void k_deposit_cell(MeshView, Species, double, const double*, const int*, int64_t,
-                               const double*, const double*, const uint8_t*, double*);
+                               const double*, const double*, int, const uint8_t*, double*);
 __global__ void k_deposit_node(int64_t, const int*, const int*, const double*, double*);
 __global__ void k_generate(MeshView, GenArgs, int64_t, double*, double*, uint8_t*, int*, int64_t*, int8_t*);
 __global__ void k_copy_probe(const float4*, float4*, int64_t);
@@ -446,7 +450,8 @@ extern "C" int amps_gpu_set_particles(amps_ctx* c, int64_t n, const double* x, c
   if (!c->has_mesh) return fail(AMPS_E_STATE, "amps_gpu_set_particles: set the mesh first");
   if (n < 0 || n >= ((int64_t)1 << 31) - 1) return fail(AMPS_E_RANGE, "particle count out of [0, 2^31-1)");
   if (n > 0 && (!x || !v || !spec || !blk || !cell)) return fail(AMPS_E_ARG, "amps_gpu_set_particles: null array");
-  if (precision != AMPS_FP64) return fail(AMPS_E_ARG, "amps_gpu_set_particles: precision must be AMPS_FP64 here");
+  if (precision != AMPS_FP64 && precision != AMPS_FP32)
+    return fail(AMPS_E_ARG, "amps_gpu_set_particles: precision must be AMPS_FP64 or AMPS_FP32");
   (void)hipSetDevice(c->device);
   HIP_TRY(hipStreamSynchronize(c->stream));
   const int nb = c->mv.nb;
@@ -481,7 +486,13 @@ extern "C" int amps_gpu_set_particles(amps_ctx* c, int64_t n, const double* x, c
   }
   c->cur = 0;
   HIP_TRY(hipMemcpy(c->d_x[0], hx.data(), sizeof(double) * 3 * cap, hipMemcpyHostToDevice));
-  HIP_TRY(hipMemcpy(c->d_v[0], hv.data(), sizeof(double) * 3 * cap, hipMemcpyHostToDevice));
+  if (precision == AMPS_FP32) {
+    std::vector<float> hf(3 * (size_t)cap);
+    for (size_t i = 0; i < hf.size(); i++) hf[i] = (float)hv[i];
+    HIP_TRY(hipMemcpy(c->d_v[0], hf.data(), sizeof(float) * 3 * cap, hipMemcpyHostToDevice));
+  } else {
+    HIP_TRY(hipMemcpy(c->d_v[0], hv.data(), sizeof(double) * 3 * cap, hipMemcpyHostToDevice));
+  }
   HIP_TRY(hipMemcpy(c->d_spec[0], hs.data(), cap, hipMemcpyHostToDevice));
   HIP_TRY(hipMemcpy(c->d_key[0], hk.data(), sizeof(int) * cap, hipMemcpyHostToDevice));
   HIP_TRY(hipMemcpy(c->d_id[0], hid.data(), sizeof(int64_t) * cap, hipMemcpyHostToDevice));
@@ -517,9 +528,16 @@ extern "C" int amps_gpu_get_particles(amps_ctx* c, double* x, double* v, int32_t
   if (x)
     for (int d = 0; d < 3; d++)
       HIP_TRY(hipMemcpy(x + d * n, c->d_x[k] + d * cap, sizeof(double) * n, hipMemcpyDeviceToHost));
-  if (v)
+  if (v && c->precision == AMPS_FP32) {
+    std::vector<float> f((size_t)n);
+    for (int d = 0; d < 3; d++) {
+      HIP_TRY(hipMemcpy(f.data(), (const float*)c->d_v[k] + d * cap, sizeof(float) * n, hipMemcpyDeviceToHost));
+      for (int64_t i = 0; i < n; i++) v[d * n + i] = f[(size_t)i];
+    }
+  } else if (v) {
     for (int d = 0; d < 3; d++)
       HIP_TRY(hipMemcpy(v + d * n, c->d_v[k] + d * cap, sizeof(double) * n, hipMemcpyDeviceToHost));
+  }
   if (spec) {
     std::vector<uint8_t> s((size_t)n);
     HIP_TRY(hipMemcpy(s.data(), c->d_spec[k], n, hipMemcpyDeviceToHost));
@@ -557,7 +575,7 @@ static int compact_impl(amps_ctx* c) {
   if (int r = scan_exclusive(c, c->d_offset, nkeys + 1, nullptr)) return r;
   hipLaunchKernelGGL(k_scatter, dim3(gridp), dim3(256), 0, c->stream, c->d_n, c->d_key[a],
                      c->d_code, c->d_offset, c->d_cursor, cap, c->d_x[a], c->d_v[a],
-                     c->d_spec[a], c->d_id[a], c->d_x[b], c->d_v[b], c->d_spec[b], c->d_key[b],
+                     c->precision == AMPS_FP32 ? 1 : 0, c->d_spec[a], c->d_id[a], c->d_x[b], c->d_v[b], c->d_spec[b], c->d_key[b],
                      c->d_id[b]);
   hipLaunchKernelGGL(k_copy_total, dim3(1), dim3(1), 0, c->stream, c->d_offset + nkeys, c->d_n);
   c->cur = b;
@@ -583,6 +601,20 @@ extern "C" int amps_gpu_compact(amps_ctx* c) {
 // ---------------------------------------------------------------------
 static int launch_push(amps_ctx* c, const amps_push_opts* o) {
   const int k = c->cur;
+  if (c->precision == AMPS_FP32) {
+    if (o->mover == AMPS_MOVER_LAPENTA)
+      hipLaunchKernelGGL(k_push_f32<AMPS_MOVER_LAPENTA>, dim3((unsigned)c->max_tiles), dim3(256), 0,
+                         c->stream, c->mv, c->faces, c->sp, c->d_E, c->d_B, c->d_tiles, c->d_ntiles,
+                         o->bc_mode, o->r_sphere, c->cap, c->d_x[k], (float*)c->d_v[k], c->d_spec[k],
+                         c->d_key[k], c->d_code);
+    else
+      hipLaunchKernelGGL(k_push_f32<AMPS_MOVER_BORIS>, dim3((unsigned)c->max_tiles), dim3(256), 0,
+                         c->stream, c->mv, c->faces, c->sp, c->d_E, c->d_B, c->d_tiles, c->d_ntiles,
+                         o->bc_mode, o->r_sphere, c->cap, c->d_x[k], (float*)c->d_v[k], c->d_spec[k],
+                         c->d_key[k], c->d_code);
+    HIP_TRY(hipGetLastError());
+    return 0;
+  }
   if (o->mover == AMPS_MOVER_LAPENTA)
     hipLaunchKernelGGL(k_push_f64<AMPS_MOVER_LAPENTA>, dim3((unsigned)c->max_tiles), dim3(256), 0,
                        c->stream, c->mv, c->faces, c->sp, c->d_E, c->d_B, c->d_tiles, c->d_ntiles,
@@ -632,7 +664,7 @@ static void launch_dep(amps_ctx* c, double dt) {
   const int k = c->cur;
   hipLaunchKernelGGL((k_deposit_cell<NCH, EX>), dim3((unsigned)c->mv.nb * 8), dim3(64 * NCH), 0,
                      c->stream, c->mv, c->sp, dt, c->d_B, c->d_offset, c->cap, c->d_x[k],
-                     c->d_v[k], c->d_spec[k], c->d_cellpart);
+                     c->d_v[k], c->precision == AMPS_FP32 ? 1 : 0, c->d_spec[k], c->d_cellpart);
 }
 
 static int launch_deposit(amps_ctx* c, double dt) {
@@ -711,7 +743,10 @@ extern "C" int amps_gpu_reset_timing(amps_ctx* c) {
 // synthetic population, full steps, copy probe
 // ---------------------------------------------------------------------
 extern "C" int amps_gpu_generate_particles(amps_ctx* c, int64_t n_per_species, int32_t nspec,
-                                           const double* vth, const double* drift, uint64_t seed) {
+                                           const double* vth, const double* drift, uint64_t seed,
+                                           int32_t precision) {
+  if (precision != AMPS_FP64 && precision != AMPS_FP32)
+    return fail(AMPS_E_ARG, "amps_gpu_generate_particles: bad precision");
   if (!c || !vth || !drift) return fail(AMPS_E_ARG, "amps_gpu_generate_particles: null argument");
   if (!c->has_mesh) return fail(AMPS_E_STATE, "amps_gpu_generate_particles: set the mesh first");
   if (nspec <= 0 || nspec > MAX_SPEC) return fail(AMPS_E_ARG, "amps_gpu_generate_particles: nspec out of [1,8]");
@@ -732,6 +767,7 @@ extern "C" int amps_gpu_generate_particles(amps_ctx* c, int64_t n_per_species, i
   for (int s2 = 0; s2 < nspec; s2++) ga.vth[s2] = vth[s2];
   for (int d = 0; d < 3; d++) ga.drift[d] = drift[d];
   ga.seed = seed;
+  ga.vf32 = precision == AMPS_FP32 ? 1 : 0;
   c->cur = 0;
   hipLaunchKernelGGL(k_generate, dim3((unsigned)((ntot + 255) / 256)), dim3(256), 0, c->stream, c->mv, ga,
                      c->cap, c->d_x[0], c->d_v[0], c->d_spec[0], c->d_key[0], c->d_id[0], c->d_code);
@@ -739,7 +775,7 @@ extern "C" int amps_gpu_generate_particles(amps_ctx* c, int64_t n_per_species, i
   const int nn = (int)ntot;
   HIP_TRY(hipMemcpyAsync(c->d_n, &nn, sizeof(int), hipMemcpyHostToDevice, c->stream));
   c->n = ntot;
-  c->precision = AMPS_FP64;
+  c->precision = precision;
   if (int r = compact_impl(c)) return r;
   HIP_TRY(hipStreamSynchronize(c->stream));
   return AMPS_OK;

@@ -102,92 +102,14 @@ __device__ __forceinline__ int find_cell(const MeshView& m, const double x[3], i
   return ijk[0] + NC * (ijk[1] + NC * ijk[2]);
 }
 
-// One particle of PIC::Mover::Lapenta2017 (pic_mover_boris.cpp:876-1380) or
-// PIC::Mover::Boris (:126-553, rotation :253-277) with corner E and B from
-// LDS.  Returns the AMPS_CODE_*; xp/vp hold the committed state.
-template <int MOVER>
-__device__ __forceinline__ int push_particle(const MeshView& m, const Faces& faces,
-                                             const BlockGeom& g, const double* sE,
-                                             const double* sB, double q, double mass,
-                                             double dtTotal, int bc_mode, double r_sphere,
-                                             double xp[3], double vp[3], int* blk, int* cell) {
-  double vInit[3] = {vp[0], vp[1], vp[2]}, xInit[3] = {xp[0], xp[1], xp[2]};
-  double vFinal[3], xFinal[3];
-  double wn[8], table[8];
-  int iX[3];
-  corner_stencil(g, xInit, wn, table, iX);
-#pragma unroll
-  for (int d = 0; d < 3; d++) iX[d] = clampi(iX[d], 0, NC - 1);
-  double E[3] = {0.0, 0.0, 0.0}, B[3] = {0.0, 0.0, 0.0};
-#pragma unroll
-  for (int l = 0; l < 8; l++) {
-    const int c = (iX[0] + (l >> 2)) + NN * ((iX[1] + ((l >> 1) & 1)) + NN * (iX[2] + (l & 1)));
-#pragma unroll
-    for (int d = 0; d < 3; d++) {
-      E[d] += wn[l] * sE[3 * c + d];
-      B[d] += wn[l] * sB[3 * c + d];
-    }
-  }
-  if (MOVER == AMPS_MOVER_LAPENTA) {
-    // :1033-1081
-    const double QdT_over_m = q * dtTotal / mass;
-    const double QdT_over_2m = 0.5 * QdT_over_m;
-    const double QdT_over_2m_squared = QdT_over_2m * QdT_over_2m;
-    double BB[3][3], P[3], alpha[3][3];
-#pragma unroll
-    for (int i = 0; i < 3; i++) {
-      P[i] = -QdT_over_2m * B[i];
-#pragma unroll
-      for (int j = 0; j <= i; j++) {
-        BB[i][j] = QdT_over_2m_squared * B[i] * B[j];
-        BB[j][i] = BB[i][j];
-      }
-    }
-    const double c0 =
-        1.0 / (1.0 + QdT_over_2m_squared * (B[0] * B[0] + B[1] * B[1] + B[2] * B[2]));
-    alpha[0][0] = c0 * (1.0 + BB[0][0]);
-    alpha[0][1] = c0 * (-P[2] + BB[0][1]);
-    alpha[0][2] = c0 * (P[1] + BB[0][2]);
-    alpha[1][0] = c0 * (P[2] + BB[1][0]);
-    alpha[1][1] = c0 * (1.0 + BB[1][1]);
-    alpha[1][2] = c0 * (-P[0] + BB[1][2]);
-    alpha[2][0] = c0 * (-P[1] + BB[2][0]);
-    alpha[2][1] = c0 * (P[0] + BB[2][1]);
-    alpha[2][2] = c0 * (1.0 + BB[2][2]);
-#pragma unroll
-    for (int i = 0; i < 3; i++) {
-      double s = 0.0;
-#pragma unroll
-      for (int j = 0; j < 3; j++) s += alpha[i][j] * (vInit[j] + QdT_over_2m * E[j]);
-      vFinal[i] = 2.0 * s - vInit[i];
-    }
-#pragma unroll
-    for (int i = 0; i < 3; i++) xFinal[i] = xInit[i] + dtTotal * vFinal[i];
-  } else {
-    // BorisSplitAcceleration_default :87-98, rotation :253-277
-    const double Charge2Mass = q / mass;
-    double accl[3], rot[3], u[3], h[3], U[3];
-#pragma unroll
-    for (int d = 0; d < 3; d++) {
-      accl[d] = 0.0 + Charge2Mass * E[d];
-      rot[d] = 0.0 - Charge2Mass * B[d];
-    }
-    const double dtTemp = dtTotal, dtTempOverTwo = dtTotal / 2.0;
-#pragma unroll
-    for (int d = 0; d < 3; d++) u[d] = vInit[d] + dtTempOverTwo * accl[d];
-#pragma unroll
-    for (int d = 0; d < 3; d++) h[d] = -dtTempOverTwo * rot[d];
-    const double h2 = h[0] * h[0] + h[1] * h[1] + h[2] * h[2];
-    const double uh = u[0] * h[0] + u[1] * h[1] + u[2] * h[2];
-    U[0] = ((1 - h2) * u[0] + 2 * (u[1] * h[2] - h[1] * u[2] + uh * h[0])) / (1 + h2);
-    U[1] = ((1 - h2) * u[1] + 2 * (u[2] * h[0] - h[2] * u[0] + uh * h[1])) / (1 + h2);
-    U[2] = ((1 - h2) * u[2] + 2 * (u[0] * h[1] - h[0] * u[1] + uh * h[2])) / (1 + h2);
-#pragma unroll
-    for (int d = 0; d < 3; d++) vFinal[d] = U[d] + dtTempOverTwo * accl[d];
-#pragma unroll
-    for (int d = 0; d < 3; d++) xFinal[d] = xInit[d] + dtTemp * vFinal[d];
-  }
-
+// Sphere, domain-boundary and block/cell relocation of a pushed particle
+// (pic_mover_boris.cpp:1088-1375), shared by the FP64 and FP32 pushes.  On
+// entry xInit/vInit hold the start state, xFinal/vFinal the pushed state;
+// xp/vp receive the committed state.
+__device__ __forceinline__ int relocate(const MeshView& m, const Faces& faces, double dtTotal,
+                                        int bc_mode, double r_sphere, double xInit[3],
+                                        double vInit[3], double xFinal[3], double vFinal[3],
+                                        double xp[3], double vp[3], int* blk, int* cell) {
   // internal sphere (:1088-1117): absorbed
   double rFinal2;
   if (r_sphere > 0.0 &&
@@ -283,6 +205,179 @@ __device__ __forceinline__ int push_particle(const MeshView& m, const Faces& fac
   return code;
 }
 
+// One particle of PIC::Mover::Lapenta2017 (pic_mover_boris.cpp:876-1380) or
+// PIC::Mover::Boris (:126-553, rotation :253-277) with corner E and B from
+// LDS.  Returns the AMPS_CODE_*; xp/vp hold the committed state.
+template <int MOVER>
+__device__ __forceinline__ int push_particle(const MeshView& m, const Faces& faces,
+                                             const BlockGeom& g, const double* sE,
+                                             const double* sB, double q, double mass,
+                                             double dtTotal, int bc_mode, double r_sphere,
+                                             double xp[3], double vp[3], int* blk, int* cell) {
+  double vInit[3] = {vp[0], vp[1], vp[2]}, xInit[3] = {xp[0], xp[1], xp[2]};
+  double vFinal[3], xFinal[3];
+  double wn[8], table[8];
+  int iX[3];
+  corner_stencil(g, xInit, wn, table, iX);
+#pragma unroll
+  for (int d = 0; d < 3; d++) iX[d] = clampi(iX[d], 0, NC - 1);
+  double E[3] = {0.0, 0.0, 0.0}, B[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+  for (int l = 0; l < 8; l++) {
+    const int c = (iX[0] + (l >> 2)) + NN * ((iX[1] + ((l >> 1) & 1)) + NN * (iX[2] + (l & 1)));
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+      E[d] += wn[l] * sE[3 * c + d];
+      B[d] += wn[l] * sB[3 * c + d];
+    }
+  }
+  if (MOVER == AMPS_MOVER_LAPENTA) {
+    // :1033-1081
+    const double QdT_over_m = q * dtTotal / mass;
+    const double QdT_over_2m = 0.5 * QdT_over_m;
+    const double QdT_over_2m_squared = QdT_over_2m * QdT_over_2m;
+    double BB[3][3], P[3], alpha[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+      P[i] = -QdT_over_2m * B[i];
+#pragma unroll
+      for (int j = 0; j <= i; j++) {
+        BB[i][j] = QdT_over_2m_squared * B[i] * B[j];
+        BB[j][i] = BB[i][j];
+      }
+    }
+    const double c0 =
+        1.0 / (1.0 + QdT_over_2m_squared * (B[0] * B[0] + B[1] * B[1] + B[2] * B[2]));
+    alpha[0][0] = c0 * (1.0 + BB[0][0]);
+    alpha[0][1] = c0 * (-P[2] + BB[0][1]);
+    alpha[0][2] = c0 * (P[1] + BB[0][2]);
+    alpha[1][0] = c0 * (P[2] + BB[1][0]);
+    alpha[1][1] = c0 * (1.0 + BB[1][1]);
+    alpha[1][2] = c0 * (-P[0] + BB[1][2]);
+    alpha[2][0] = c0 * (-P[1] + BB[2][0]);
+    alpha[2][1] = c0 * (P[0] + BB[2][1]);
+    alpha[2][2] = c0 * (1.0 + BB[2][2]);
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+      double s = 0.0;
+#pragma unroll
+      for (int j = 0; j < 3; j++) s += alpha[i][j] * (vInit[j] + QdT_over_2m * E[j]);
+      vFinal[i] = 2.0 * s - vInit[i];
+    }
+#pragma unroll
+    for (int i = 0; i < 3; i++) xFinal[i] = xInit[i] + dtTotal * vFinal[i];
+  } else {
+    // BorisSplitAcceleration_default :87-98, rotation :253-277
+    const double Charge2Mass = q / mass;
+    double accl[3], rot[3], u[3], h[3], U[3];
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+      accl[d] = 0.0 + Charge2Mass * E[d];
+      rot[d] = 0.0 - Charge2Mass * B[d];
+    }
+    const double dtTemp = dtTotal, dtTempOverTwo = dtTotal / 2.0;
+#pragma unroll
+    for (int d = 0; d < 3; d++) u[d] = vInit[d] + dtTempOverTwo * accl[d];
+#pragma unroll
+    for (int d = 0; d < 3; d++) h[d] = -dtTempOverTwo * rot[d];
+    const double h2 = h[0] * h[0] + h[1] * h[1] + h[2] * h[2];
+    const double uh = u[0] * h[0] + u[1] * h[1] + u[2] * h[2];
+    U[0] = ((1 - h2) * u[0] + 2 * (u[1] * h[2] - h[1] * u[2] + uh * h[0])) / (1 + h2);
+    U[1] = ((1 - h2) * u[1] + 2 * (u[2] * h[0] - h[2] * u[0] + uh * h[1])) / (1 + h2);
+    U[2] = ((1 - h2) * u[2] + 2 * (u[0] * h[1] - h[0] * u[1] + uh * h[2])) / (1 + h2);
+#pragma unroll
+    for (int d = 0; d < 3; d++) vFinal[d] = U[d] + dtTempOverTwo * accl[d];
+#pragma unroll
+    for (int d = 0; d < 3; d++) xFinal[d] = xInit[d] + dtTemp * vFinal[d];
+  }
+
+  return relocate(m, faces, dtTotal, bc_mode, r_sphere, xInit, vInit, xFinal, vFinal, xp, vp, blk,
+                  cell);
+}
+
+// FP32 push: the same mover with FP32 stencil weights, field interpolation
+// and velocity update; the position increment dt*v is formed in FP32 and
+// accumulated into the FP64 position, and block/cell location, sphere and
+// boundary handling run on the FP64 position (relocate()).
+template <int MOVER>
+__device__ __forceinline__ int push_particle_f32(const MeshView& m, const Faces& faces,
+                                                 const BlockGeom& g, const float* sE,
+                                                 const float* sB, float q, float mass,
+                                                 double dtTotal, int bc_mode, double r_sphere,
+                                                 double xp[3], float vf[3], int* blk, int* cell) {
+#pragma clang fp contract(fast)
+  double xInit[3] = {xp[0], xp[1], xp[2]};
+  float xl[3];
+  int iX[3];
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    if (fabs(xInit[d] - g.xmax[d]) < 1e-10 * g.dxs[d]) xInit[d] = g.xmax[d] - 1e-10 * g.dxs[d];
+    const double loc = (xInit[d] - g.xmin[d]) / g.dxs[d];
+    iX[d] = (int)loc;
+    xl[d] = (float)(loc - iX[d]);
+    iX[d] = clampi(iX[d], 0, NC - 1);
+  }
+  float w[8], norm = 0.0f;
+#pragma unroll
+  for (int l = 0; l < 8; l++) {
+    const float a = (l >> 2) ? xl[0] : (1.0f - xl[0]);
+    const float b = ((l >> 1) & 1) ? xl[1] : (1.0f - xl[1]);
+    const float c = (l & 1) ? xl[2] : (1.0f - xl[2]);
+    w[l] = a * b * c;
+    norm += w[l];
+  }
+  const float inorm = 1.0f / norm;
+  float E[3] = {0.f, 0.f, 0.f}, B[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+  for (int l = 0; l < 8; l++) {
+    const int c = (iX[0] + (l >> 2)) + NN * ((iX[1] + ((l >> 1) & 1)) + NN * (iX[2] + (l & 1)));
+    const float wl = w[l] * inorm;
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+      E[d] += wl * sE[3 * c + d];
+      B[d] += wl * sB[3 * c + d];
+    }
+  }
+  const float dt = (float)dtTotal;
+  float vF[3];
+  if (MOVER == AMPS_MOVER_LAPENTA) {
+    const float h = 0.5f * q * dt / mass;
+    const float hB[3] = {h * B[0], h * B[1], h * B[2]};
+    const float c0 = 1.0f / (1.0f + hB[0] * hB[0] + hB[1] * hB[1] + hB[2] * hB[2]);
+    const float u[3] = {vf[0] + h * E[0], vf[1] + h * E[1], vf[2] + h * E[2]};
+    // alpha u = c0 (u + (hB.u) hB + u x hB)
+    const float hu = hB[0] * u[0] + hB[1] * u[1] + hB[2] * u[2];
+    const float au[3] = {c0 * (u[0] + hu * hB[0] + (u[1] * hB[2] - u[2] * hB[1])),
+                         c0 * (u[1] + hu * hB[1] + (u[2] * hB[0] - u[0] * hB[2])),
+                         c0 * (u[2] + hu * hB[2] + (u[0] * hB[1] - u[1] * hB[0]))};
+#pragma unroll
+    for (int d = 0; d < 3; d++) vF[d] = 2.0f * au[d] - vf[d];
+  } else {
+    const float qm = q / mass, ht = 0.5f * dt;
+    const float u[3] = {vf[0] + ht * qm * E[0], vf[1] + ht * qm * E[1], vf[2] + ht * qm * E[2]};
+    const float hh[3] = {ht * qm * B[0], ht * qm * B[1], ht * qm * B[2]};
+    const float h2 = hh[0] * hh[0] + hh[1] * hh[1] + hh[2] * hh[2];
+    const float uh = u[0] * hh[0] + u[1] * hh[1] + u[2] * hh[2];
+    const float i1 = 1.0f / (1.0f + h2);
+    const float U[3] = {((1 - h2) * u[0] + 2 * (u[1] * hh[2] - hh[1] * u[2] + uh * hh[0])) * i1,
+                        ((1 - h2) * u[1] + 2 * (u[2] * hh[0] - hh[2] * u[0] + uh * hh[1])) * i1,
+                        ((1 - h2) * u[2] + 2 * (u[0] * hh[1] - hh[0] * u[1] + uh * hh[2])) * i1};
+#pragma unroll
+    for (int d = 0; d < 3; d++) vF[d] = U[d] + ht * qm * E[d];
+  }
+  double vInit[3] = {(double)vf[0], (double)vf[1], (double)vf[2]};
+  double vFinal[3] = {(double)vF[0], (double)vF[1], (double)vF[2]};
+  double xFinal[3];
+#pragma unroll
+  for (int d = 0; d < 3; d++) xFinal[d] = xInit[d] + (double)(dt * vF[d]);
+  double vo[3];
+  const int code =
+      relocate(m, faces, dtTotal, bc_mode, r_sphere, xInit, vInit, xFinal, vFinal, xp, vo, blk, cell);
+#pragma unroll
+  for (int d = 0; d < 3; d++) vf[d] = (float)vo[d];
+  return code;
+}
+
 // ---------------------------------------------------------------------
 // ProcessCell per-particle body restricted to one chunk [O0,O1) of the
 // 348 per-cell outputs: 0..23 J (corner c = o/3, dim o%3), then 36 lower
@@ -299,7 +394,7 @@ template <int O0, int O1, bool EXACT>
 __device__ __forceinline__ void cell_chunk(const BlockGeom& g, const Species& sp, double dtTotal,
                                            const double* sB, int sx, int sy, int sz, int p0,
                                            int p1, int64_t cap, const double* __restrict__ x,
-                                           const double* __restrict__ v,
+                                           const double* __restrict__ v, int vf32,
                                            const uint8_t* __restrict__ spec, double CellVolume,
                                            double* out) {
   constexpr int NO = O1 - O0;
@@ -308,7 +403,13 @@ __device__ __forceinline__ void cell_chunk(const BlockGeom& g, const Species& sp
   for (int o = 0; o < NO; o++) acc[o] = 0.0;
   for (int p = p0; p < p1; ++p) {
     double xI[3] = {x[p], x[cap + p], x[2 * cap + p]};
-    const double vI[3] = {v[p], v[cap + p], v[2 * cap + p]};
+    double vI[3];
+    if (vf32) {
+      const float* vf = (const float*)v;
+      vI[0] = vf[p], vI[1] = vf[cap + p], vI[2] = vf[2 * cap + p];
+    } else {
+      vI[0] = v[p], vI[1] = v[cap + p], vI[2] = v[2 * cap + p];
+    }
     const int s = spec[p];
     double wn[8], table[8];
     int iX[3];
@@ -382,7 +483,7 @@ __device__ __forceinline__ void dispatch_chunk(int w, const BlockGeom& g, const 
                                                double dtTotal, const double* sB, int sx, int sy,
                                                int sz, int p0, int p1, int64_t cap,
                                                const double* __restrict__ x,
-                                               const double* __restrict__ v,
+                                               const double* __restrict__ v, int vf32,
                                                const uint8_t* __restrict__ spec,
                                                double CellVolume, double* out) {
   if constexpr (W < NCH) {
@@ -390,11 +491,11 @@ __device__ __forceinline__ void dispatch_chunk(int w, const BlockGeom& g, const 
     constexpr int O0 = W * CH;
     constexpr int O1 = (O0 + CH < CELL_VALS) ? O0 + CH : CELL_VALS;
     if (w == W)
-      cell_chunk<O0, O1, EXACT>(g, sp, dtTotal, sB, sx, sy, sz, p0, p1, cap, x, v, spec,
+      cell_chunk<O0, O1, EXACT>(g, sp, dtTotal, sB, sx, sy, sz, p0, p1, cap, x, v, vf32, spec,
                                 CellVolume, out);
     else
       dispatch_chunk<NCH, EXACT, W + 1>(w, g, sp, dtTotal, sB, sx, sy, sz, p0, p1, cap, x, v,
-                                        spec, CellVolume, out);
+                                        vf32, spec, CellVolume, out);
   }
 }
 

@@ -68,6 +68,60 @@ template __global__ void k_push_f64<AMPS_MOVER_BORIS>(MeshView, Faces, Species, 
                                                        double, int64_t, double*, double*,
                                                        const uint8_t*, int*, int8_t*);
 
+// FP32 push: float E/B stencil in LDS (17.5 KB), float velocities, FP64
+// positions.
+template <int MOVER>
+__global__ __launch_bounds__(256) void k_push_f32(MeshView m, Faces faces, Species sp,
+                                                   const double* __restrict__ E,
+                                                   const double* __restrict__ B,
+                                                   const Tile* __restrict__ tiles,
+                                                   const int* __restrict__ ntiles, int bc_mode,
+                                                   double r_sphere, int64_t cap, double* x,
+                                                   float* v, const uint8_t* __restrict__ spec,
+                                                   int* key, int8_t* code) {
+  __shared__ float sE[NCORNER * 3];
+  __shared__ float sB[NCORNER * 3];
+  const int t = blockIdx.x;
+  if (t >= *ntiles) return;
+  const Tile tl = tiles[t];
+  const int b = tl.block;
+  const double* Eb = E + (size_t)b * NCORNER * 3;
+  const double* Bb = B + (size_t)b * NCORNER * 3;
+  for (int i = threadIdx.x; i < NCORNER * 3; i += blockDim.x) {
+    sE[i] = (float)Eb[i];
+    sB[i] = (float)Bb[i];
+  }
+  __syncthreads();
+  BlockGeom g;
+  load_block_geom(m, b, g);
+  const int p1 = tl.start + tl.count;
+  for (int p = tl.start + threadIdx.x; p < p1; p += blockDim.x) {
+    double xp[3] = {x[p], x[cap + p], x[2 * cap + p]};
+    float vp[3] = {v[p], v[cap + p], v[2 * cap + p]};
+    const int s = spec[p];
+    int nb_out = b, cell_out = 0;
+    const int c = push_particle_f32<MOVER>(m, faces, g, sE, sB, (float)sp.charge[s], (float)sp.mass[s],
+                                           sp.dt[s], bc_mode, r_sphere, xp, vp, &nb_out, &cell_out);
+    x[p] = xp[0];
+    x[cap + p] = xp[1];
+    x[2 * cap + p] = xp[2];
+    v[p] = vp[0];
+    v[cap + p] = vp[1];
+    v[2 * cap + p] = vp[2];
+    if (c == AMPS_CODE_FINISHED || c == AMPS_CODE_REFLECTED) key[p] = nb_out * NCELL + cell_out;
+    code[p] = (int8_t)c;
+  }
+}
+
+template __global__ void k_push_f32<AMPS_MOVER_LAPENTA>(MeshView, Faces, Species, const double*,
+                                                         const double*, const Tile*, const int*,
+                                                         int, double, int64_t, double*, float*,
+                                                         const uint8_t*, int*, int8_t*);
+template __global__ void k_push_f32<AMPS_MOVER_BORIS>(MeshView, Faces, Species, const double*,
+                                                       const double*, const Tile*, const int*, int,
+                                                       double, int64_t, double*, float*,
+                                                       const uint8_t*, int*, int8_t*);
+
 // ======================================================================
 // Migration: counting sort by key = block*512 + cell with wavefront
 // run-length aggregation (ballot + lane masks) so that one atomic serves a
@@ -126,7 +180,7 @@ __global__ __launch_bounds__(256) void k_scatter(const int* __restrict__ n_dev,
                                                   const int8_t* __restrict__ code,
                                                   const int* __restrict__ offset, int* cursor,
                                                   int64_t cap, const double* __restrict__ xa,
-                                                  const double* __restrict__ va,
+                                                  const double* __restrict__ va, int vf32,
                                                   const uint8_t* __restrict__ sa,
                                                   const int64_t* __restrict__ ia, double* xb,
                                                   double* vb, uint8_t* sb, int* kb,
@@ -144,7 +198,10 @@ __global__ __launch_bounds__(256) void k_scatter(const int* __restrict__ n_dev,
 #pragma unroll
   for (int c = 0; c < 3; c++) {
     xb[c * cap + d] = xa[c * cap + p];
-    vb[c * cap + d] = va[c * cap + p];
+    if (vf32)
+      ((float*)vb)[c * cap + d] = ((const float*)va)[c * cap + p];
+    else
+      vb[c * cap + d] = va[c * cap + p];
   }
   sb[d] = sa[p];
   kb[d] = k;
@@ -274,7 +331,7 @@ __global__ __launch_bounds__(64 * NCH) void k_deposit_cell(MeshView m, Species s
                                                             const int* __restrict__ offset,
                                                             int64_t cap,
                                                             const double* __restrict__ x,
-                                                            const double* __restrict__ v,
+                                                            const double* __restrict__ v, int vf32,
                                                             const uint8_t* __restrict__ spec,
                                                             double* __restrict__ cellpart) {
   __shared__ double sB[125 * 3];
@@ -299,14 +356,14 @@ __global__ __launch_bounds__(64 * NCH) void k_deposit_cell(MeshView m, Species s
   for (int d = 0; d < 3; d++) dxc[d] = (g.xmax[d] - g.xmin[d]) / NC * 1.0;
   for (int d = 0; d < 3; d++) CellVolume *= dxc[d];
   double* out = cellpart + (size_t)key * CELL_VALS;
-  dispatch_chunk<NCH, EXACT>(w, g, sp, dtTotal, sB, sx, sy, sz, p0, p1, cap, x, v, spec,
+  dispatch_chunk<NCH, EXACT>(w, g, sp, dtTotal, sB, sx, sy, sz, p0, p1, cap, x, v, vf32, spec,
                              CellVolume, out);
 }
 
 #define INST_DEP(NCH, EX)                                                                  \
   template __global__ void k_deposit_cell<NCH, EX>(MeshView, Species, double, const double*, \
                                                     const int*, int64_t, const double*,     \
-                                                    const double*, const uint8_t*, double*);
+                                                    const double*, int, const uint8_t*, double*);
 INST_DEP(2, true)
 INST_DEP(2, false)
 INST_DEP(3, true)
@@ -436,7 +493,10 @@ __global__ __launch_bounds__(256) void k_generate(MeshView m, GenArgs ga, int64_
 #pragma unroll
   for (int d = 0; d < 3; d++) {
     x[d * cap + i] = xp[d];
-    v[d * cap + i] = vp[d];
+    if (ga.vf32)
+      ((float*)v)[d * cap + i] = (float)vp[d];
+    else
+      v[d * cap + i] = vp[d];
   }
   spec[i] = (uint8_t)s;
   key[i] = nb * NCELL + cell;

@@ -75,13 +75,13 @@ class PicState:
             L._ptr(p.spec, C.c_int32), L._ptr(p.blk, C.c_int32), L._ptr(p.cell, C.c_int32),
             L._ptr(p.id, C.c_int64), precision), "set_particles")
 
-    def generate_particles(self, n_per_species, vth, drift=(0.0, 0.0, 0.0), seed=1234):
+    def generate_particles(self, n_per_species, vth, drift=(0.0, 0.0, 0.0), seed=1234, precision=L.FP64):
         """Device-side synthetic population (uniform in every cell, Maxwellian)."""
         vth = np.ascontiguousarray(vth, np.float64)
         drift = np.ascontiguousarray(drift, np.float64)
         L.check(self.lib.amps_gpu_generate_particles(self.ctx.h, int(n_per_species), len(vth),
                                                      L._ptr(vth, C.c_double), L._ptr(drift, C.c_double),
-                                                     seed), "generate_particles")
+                                                     seed, precision), "generate_particles")
 
     def run_steps(self, nsteps, deposit_dt, mover=L.MOVER_LAPENTA, bc_mode=L.BC_REFLECT, r_sphere=0.0,
                   use_graph=True, tile=0):

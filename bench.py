@@ -61,6 +61,112 @@ def cpu_baseline(target_s=12.0):
                       f"steps, {t_push + t_dep:.1f} s (push {t_push:.1f} s, deposit {t_dep:.1f} s)"}
 
 
+def cutoff_map_bench(ctx, n_cu, n_train=1 << 20, epochs=20, cpu_check_points=16, tune_cache=None):
+    """Batched backward cutoff map (64x32 points x 8 directions x 48 bins)
+    and the surrogate-bracketed map.  Returns a dict of timings and errors."""
+    import numpy as np
+
+    from amps_amd import _lib as L
+    from amps_amd import autotune
+    from amps_amd import cutoff as CU
+    from amps_amd import surrogate as SG
+
+    out = {}
+    P = CU.config2_params()
+    pts, lat, lon = CU.config2_points()
+    zen, az = CU.config2_directions(8)
+    dirs = CU.arrival_dirs(pts, zen, az).reshape(-1, 3)
+    x0 = np.repeat(pts, len(zen), axis=0)
+    flat_lat, flat_lon = np.repeat(lat, len(zen)), np.repeat(lon, len(zen))
+    flat_zen, flat_az = np.tile(zen, len(pts)), np.tile(az, len(pts))
+    grid = CU.build_search_grid(0.1, 20.0, 48)
+    nloc = len(x0)
+    sub = slice(0, nloc, 64)
+
+    def run_sub():
+        CU.cutoff_scan(ctx, P, x0[sub], dirs[sub], grid, mode=CU.SCAN_FULL, precision=L.FP32)
+
+    tune, cached = autotune.tune_tracer(ctx, run_sub, n_cu, f"trace|{n_cu}", cache=tune_cache)
+    out["tracer_tune"] = {"workgroups": tune["workgroups"], "chunk": tune["chunk"], "cached": cached}
+    # FP64 map (bit-exact with the CPU oracle): the reference map of this run
+    t0 = time.perf_counter()
+    s64, b64, st64 = CU.cutoff_scan(ctx, P, x0, dirs, grid, mode=CU.SCAN_FULL, precision=L.FP64)
+    out["fp64_full_s"] = time.perf_counter() - t0
+    out["fp64_steps"] = int(st64)
+    # plain FP32 maps: every bin, and the downward UPPER_SCAN
+    t0 = time.perf_counter()
+    s32, b32, st32 = CU.cutoff_scan(ctx, P, x0, dirs, grid, mode=CU.SCAN_FULL, precision=L.FP32)
+    out["plain_full_s"] = time.perf_counter() - t0
+    out["plain_full_steps"] = int(st32)
+    out["plain_full_steps_per_s"] = st32 / out["plain_full_s"]
+    t0 = time.perf_counter()
+    _, bu32, stu = CU.cutoff_scan(ctx, P, x0, dirs, grid, mode=CU.SCAN_UPPER, precision=L.FP32)
+    out["plain_upper_s"] = time.perf_counter() - t0
+    out["trajectories_full"] = int(nloc * len(grid))
+    out["fp32_states_equal_fp64"] = float(np.mean(s32 == s64))
+    out["fp32_bins_off_gt1"] = float(np.mean(np.abs(b32.astype(int) - b64.astype(int)) > 1))
+    # CPU oracle on a subsample of map points (every bin)
+    if cpu_check_points > 0:
+        from oracle import oracle as O
+        Q = O.TraceParams()
+        for name, _t in L.TraceParams._fields_:
+            v = getattr(P, name)
+            if hasattr(v, "__len__"):
+                for i in range(len(v)):
+                    getattr(Q, name)[i] = v[i]
+            else:
+                setattr(Q, name, v)
+        idx = np.linspace(0, nloc - 1, cpu_check_points).astype(int)
+        r = O.trace_batch(Q, np.repeat(x0[idx], 48, axis=0), np.repeat(dirs[idx], 48, axis=0),
+                          np.tile(grid, len(idx)),
+                          nthreads=int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)))
+        ost = (r["termination"] == 0).astype(np.int8).reshape(-1, 48)
+        out["cpu_oracle_check"] = {"points": int(len(idx)), "states_bitexact": bool(np.array_equal(ost, s64[idx]))}
+    # surrogate: training set from the tracer itself (UPPER_SCAN, FP32)
+    rng = np.random.default_rng(6)
+    tl = np.degrees(np.arcsin(rng.uniform(-1, 1, n_train)))
+    tg = rng.uniform(0, 360, n_train)
+    tz = np.degrees(np.arccos(rng.uniform(np.cos(np.radians(60.0)), 1.0, n_train)))
+    ta = rng.uniform(0, 360, n_train)
+    la, lo = np.radians(tl), np.radians(tg)
+    r0 = 1.05 * CU.RE_DIPOLE
+    tx = np.stack([r0 * np.cos(la) * np.cos(lo), r0 * np.cos(la) * np.sin(lo), r0 * np.sin(la)], 1)
+    U = tx / np.linalg.norm(tx, axis=1)[:, None]
+    E = np.cross(np.array([0.0, 0.0, 1.0]), U)
+    E /= np.maximum(np.linalg.norm(E, axis=1), 1e-300)[:, None]
+    N = np.cross(U, E)
+    el, azr = np.radians(90.0 - tz), np.radians(ta)
+    d = (np.cos(el) * np.sin(azr))[:, None] * E + (np.cos(el) * np.cos(azr))[:, None] * N + np.sin(el)[:, None] * U
+    tdir = -d / np.linalg.norm(d, axis=1)[:, None]
+    t0 = time.perf_counter()
+    _, tb, tsteps = CU.cutoff_scan(ctx, P, tx, tdir, grid, mode=CU.SCAN_UPPER, precision=L.FP32)
+    out["train_data_s"] = time.perf_counter() - t0
+    out["train_samples"] = int(n_train)
+    Xtr = SG.features(tl, tg, tz, ta)
+    Ttr = SG.bin_target(tb, grid)
+    sur = SG.Surrogate(ctx, seed=0)
+    sur.set_data(Xtr, Ttr)
+    nsteps = max(1, epochs * n_train // 4096)
+    t0 = time.perf_counter()
+    losses = sur.train(nsteps, batch=4096, lr=1e-3, record_loss=False)
+    ctx.sync()
+    out["train_s"] = time.perf_counter() - t0
+    out["train_steps"] = int(nsteps)
+    g, tl0 = sur.grad(4096)
+    out["train_loss_end"] = float(tl0)
+    # surrogate-bracketed map (predict + bracket scan + full-scan fallback)
+    feats = SG.features(flat_lat, flat_lon, flat_zen, flat_az)
+    t0 = time.perf_counter()
+    bs, fb, pb, sst = SG.bracketed_map(ctx, sur, P, x0, dirs, feats, grid, half=3, precision=L.FP32)
+    out["surrogate_s"] = time.perf_counter() - t0
+    out["surrogate_fallback_frac"] = float(np.mean(fb))
+    out["surrogate_bins_off_gt1"] = float(np.mean(np.abs(bs.astype(int) - b64.astype(int)) > 1))
+    out["surrogate_pred_bin_mae"] = float(np.mean(np.abs(pb.astype(int) - np.where(b64 < 0, 47, b64))))
+    out["speedup_vs_plain_full"] = out["plain_full_s"] / out["surrogate_s"]
+    out["speedup_vs_plain_upper"] = out["plain_upper_s"] / out["surrogate_s"]
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -70,6 +176,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--tune-cache", default=os.path.join(ROOT, "gpurun_out", "tuning.json"))
+    ap.add_argument("--workload", choices=["all", "pic", "map"], default="all")
+    ap.add_argument("--train-samples-log2", type=int, default=20)
+    ap.add_argument("--epochs", type=int, default=20)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); "
@@ -85,6 +194,22 @@ def main():
 
     ctx = L.Context(0)
     arch, ncu, hbm = ctx.device_info()
+    os.makedirs(os.path.dirname(args.tune_cache), exist_ok=True)
+    if args.workload == "map":
+        t0 = time.perf_counter()
+        mp = cutoff_map_bench(ctx, ncu, n_train=1 << args.train_samples_log2, epochs=args.epochs,
+                              tune_cache=args.tune_cache)
+        line = {"metric": "cutoff-rigidity map time, surrogate-bracketed vs plain full GPU scan",
+                "value": mp["surrogate_s"], "unit": "s", "n_gpus": 1, "steps": 1, "warmup": 0,
+                "ms_per_step": 1e3 * mp["surrogate_s"], "higher_is_better": False, "scaling": "strong",
+                "vs_baseline": None, "dtype": "f32 (FP64 positions)", "data": "synthetic",
+                "config": {"workload": "64x32 lat/lon x 8 directions x 48 rigidities 0.1-20 GV, tilted "
+                                       "dipole + Bz -5 nT, 0.01 gyro-period Boris steps, 1e5-step cap",
+                           "device": arch},
+                "cutoff_map": mp, "wall_s": time.perf_counter() - t0}
+        print(json.dumps(line), flush=True)
+        ctx.close()
+        return
     mesh = M.three_level_mesh()
     E, B = M.corner_fields(mesh)
     bmax = float(np.max(np.linalg.norm(B, axis=-1)))
@@ -97,7 +222,6 @@ def main():
     opts = dict(mover=L.MOVER_LAPENTA, bc_mode=L.BC_REFLECT, r_sphere=0.0)
     tune = None
     if not args.no_tune:
-        os.makedirs(os.path.dirname(args.tune_cache), exist_ok=True)
         sig = f"{arch}|pic|nb{mesh.nblocks}|n{n}"
         tune, cached = autotune.tune_pic(ctx, st, opts, dt, sig, cache=args.tune_cache)
         tune = {"tile": tune["tile"], "deposit_chunks": tune["deposit_chunks"], "cached": cached}
@@ -166,6 +290,12 @@ def main():
         "measured_copy_GBps": round(copy_gbps, 1),
         "autotune": tune,
     }
+    if args.workload == "all":
+        try:
+            line["cutoff_map"] = cutoff_map_bench(ctx, ncu, n_train=1 << args.train_samples_log2,
+                                                  epochs=args.epochs, tune_cache=args.tune_cache)
+        except Exception as e:  # the PIC line stays valid if the map part fails
+            line["cutoff_map"] = {"error": repr(e)}
     print(json.dumps(line), flush=True)
     ctx.close()
 

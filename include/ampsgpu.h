@@ -38,7 +38,8 @@ extern "C" {
 #define AMPS_BC_REFLECT 1         /* ..._SPECULAR_REFLECTION_ (pic_mover_boris.cpp:1236-1253) */
 
 #define AMPS_FP64 0
-#define AMPS_FP32 1               /* FP32 arithmetic, compensated FP64 positions */
+#define AMPS_FP32 1               /* FP32 push arithmetic and velocities; FP64 positions,
+                                     FP32 increments accumulated in FP64; FP64 deposit */
 
 typedef struct amps_ctx amps_ctx;
 
@@ -123,7 +124,8 @@ int amps_gpu_compact(amps_ctx* ctx);
  * Maxwellian with thermal speed vth[s] plus drift[3].  Ids are consecutive;
  * block/cell ids come from the same point location as the mover. */
 int amps_gpu_generate_particles(amps_ctx* ctx, int64_t n_per_species, int32_t nspec,
-                                const double* vth, const double* drift, uint64_t seed);
+                                const double* vth, const double* drift, uint64_t seed,
+                                int32_t precision);
 
 /* nsteps full particle steps: MoveParticles, migration (compaction) and
  * UpdateJMassMatrix at deposit_dt.  use_graph=1 runs one eager step, then

@@ -177,3 +177,35 @@ def test_generator_and_graph_steps_match_eager(gpu_ctx):
     assert np.array_equal(a.blk[o1], b.blk[o2]) and np.array_equal(a.cell[o1], b.cell[o2])
     nm = np.linalg.norm(Ma - Mb, axis=1) / np.maximum(np.linalg.norm(Ma, axis=1), 1e-300)
     assert nm.max() <= 1e-12
+
+
+@pytest.mark.parametrize("mover", [L.MOVER_LAPENTA, L.MOVER_BORIS])
+def test_push_fp32_error_against_fp64_oracle(gpu_ctx, mover):
+    """FP32 push (FP32 fields and velocity, FP64 positions): state the error of
+    every output against the FP64 oracle after one step and after 20 steps."""
+    mesh = M.central_octant_mesh()
+    E, B, parts, sp = _case(mesh, 40000, 0.2, 0.05, seed=21)
+    st = PicState(mesh, E, B, sp, ctx=gpu_ctx)
+    st.set_particles(parts, precision=L.FP32)
+    om = O.Mesh(mesh)
+    o = parts
+    for step in range(20):
+        st.MoveParticles(mover=mover, bc_mode=L.BC_REFLECT, r_sphere=0.2, compact=(step < 19))
+        o, ocode = O.push(om, E, B, sp, o, mover=mover, bc_mode=1, r_sphere=0.2)
+        if step < 19:
+            o = o.take(np.flatnonzero((ocode == 3) | (ocode == -1)))
+    g, gcode = st.get_particles()
+    g, gcode = _by_id(g, gcode)
+    o, ocode = _by_id(o, ocode)
+    common = np.intersect1d(g.id, o.id)
+    gi = np.searchsorted(g.id, common)
+    oi = np.searchsorted(o.id, common)
+    v_rel = np.max(np.abs(g.v[:, gi] - o.v[:, oi])) / np.max(np.abs(o.v))
+    x_abs = np.max(np.abs(g.x[:, gi] - o.x[:, oi]))
+    same_code = np.mean(gcode[gi] == ocode[oi])
+    same_cell = np.mean((g.blk[gi] == o.blk[oi]) & (g.cell[gi] == o.cell[oi]))
+    print(f"FP32 mover {mover}: 20 steps max|dv|/max|v| {v_rel:.2e}, max|dx| {x_abs:.2e}, "
+          f"codes equal {same_code:.6f}, cells equal {same_cell:.6f}, survivors {g.n} vs {o.n}")
+    assert v_rel < 1e-5 and x_abs < 1e-6
+    assert same_code > 0.999 and same_cell > 0.999
+    assert abs(g.n - o.n) <= max(2, o.n // 5000)
