@@ -14,7 +14,16 @@ import numpy as np
 from . import _lib as L
 
 PUSH_TILES = (1024, 2048, 4096, 8192)
-DEPOSIT_CHUNKS = (3, 4, 6)
+# (cell-pass kernel, chunks per cell): 0 = per-lane particle loop,
+# 1 = shared-memory record batches
+DEPOSIT_VARIANTS = ((1, 6), (1, 12), (1, 4), (0, 6))
+
+
+def set_deposit_variant(ctx, kernel, nch):
+    """Select a deposit cell-pass variant (4 and 6 chunks suit both kernels)."""
+    ctx.set_param("deposit_chunks", 6)
+    ctx.set_param("deposit_kernel", kernel)
+    ctx.set_param("deposit_chunks", nch)
 
 
 def _load(path):
@@ -38,12 +47,12 @@ def _save(path, d):
 
 
 def tune_pic(ctx, state, opts_kw, deposit_dt, signature, cache=None, reps=2):
-    """Pick the push tile and the deposit chunk count for a PicState."""
+    """Pick the push tile and the deposit cell-pass variant for a PicState."""
     db = _load(cache)
-    if signature in db:
+    if signature in db and "deposit_kernel" in db[signature]:
         best = db[signature]
         ctx.set_param("tile", best["tile"])
-        ctx.set_param("deposit_chunks", best["deposit_chunks"])
+        set_deposit_variant(ctx, best["deposit_kernel"], best["deposit_chunks"])
         return best, True
     ctx.set_timing(True)
     res = {}
@@ -56,19 +65,19 @@ def tune_pic(ctx, state, opts_kw, deposit_dt, signature, cache=None, reps=2):
         res[("tile", tile)] = ms[L.K_PUSH] / max(n[L.K_PUSH], 1)
     best_tile = min(PUSH_TILES, key=lambda t: res[("tile", t)])
     ctx.set_param("tile", best_tile)
-    for nch in DEPOSIT_CHUNKS:
-        ctx.set_param("deposit_chunks", nch)
+    for var in DEPOSIT_VARIANTS:
+        set_deposit_variant(ctx, *var)
         ctx.reset_timing()
         for _ in range(reps):
             state.UpdateJMassMatrix(deposit_dt, copy_out=False)
         ms, n = ctx.timing()
-        res[("dep", nch)] = (ms[L.K_DEPOSIT_CELL] + ms[L.K_DEPOSIT_NODE]) / max(n[L.K_DEPOSIT_CELL], 1)
-    best_nch = min(DEPOSIT_CHUNKS, key=lambda c: res[("dep", c)])
-    ctx.set_param("deposit_chunks", best_nch)
+        res[("dep", var)] = (ms[L.K_DEPOSIT_CELL] + ms[L.K_DEPOSIT_NODE]) / max(n[L.K_DEPOSIT_CELL], 1)
+    best_var = min(DEPOSIT_VARIANTS, key=lambda v: res[("dep", v)])
+    set_deposit_variant(ctx, *best_var)
     ctx.set_timing(False)
-    best = {"tile": int(best_tile), "deposit_chunks": int(best_nch),
+    best = {"tile": int(best_tile), "deposit_kernel": int(best_var[0]), "deposit_chunks": int(best_var[1]),
             "push_ms": {str(t): round(res[("tile", t)], 4) for t in PUSH_TILES},
-            "deposit_ms": {str(c): round(res[("dep", c)], 4) for c in DEPOSIT_CHUNKS}}
+            "deposit_ms": {f"k{k}c{c}": round(res[("dep", (k, c))], 4) for k, c in DEPOSIT_VARIANTS}}
     db[signature] = best
     _save(cache, db)
     return best, False

@@ -77,6 +77,7 @@ struct amps_ctx {
   std::vector<int64_t> h_node_keys;
   int* d_inc_start = nullptr;  // [nnodes+1]
   int* d_inc = nullptr;        // cell_gid*8 + corner, ascending cell order per node
+  int* d_node_order = nullptr; // node processing order of the corner flush
   // fields and species
   bool has_fields = false;
   double *d_E = nullptr, *d_B = nullptr;
@@ -107,8 +108,9 @@ struct amps_ctx {
   int tile = 2048;
   bool offsets_valid = false;  // d_offset describes the current order
   bool tiles_valid = false;    // d_tiles describes the current order
-  int dep_nch = 4;             // deposit output chunks per cell (waves per workgroup)
+  int dep_nch = 6;             // deposit output chunks per cell (waves per workgroup)
   int dep_exact = 0;           // 1: unfused mul+add in the deposit (bit-exact order)
+  int dep_kernel = 1;          // cell pass: 0 per-lane particle loop, 1 LDS record batches
   // deposit
   double* d_cellpart = nullptr;
   double* d_nodeout = nullptr;
@@ -137,6 +139,8 @@ void trace_free(amps_ctx* c);
 void mlp_free(amps_ctx* c);
 // device-wide exclusive scan of n ints in place; total written to total_out
 int scan_exclusive(amps_ctx* c, int* data, int64_t n, int* total_out);
+// ECSIM cell pass with LDS record batches (deposit.hip)
+int launch_deposit_cell_rec(amps_ctx* c, double dt);
 }  // namespace amps
 
 #define HIP_TRY(expr)                                    \

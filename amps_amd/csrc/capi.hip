@@ -33,7 +33,7 @@ __global__ void k_tile_emit(int, int, const int*, const int*, Tile*, int*);
 template <int NCH, bool EXACT>
 __global__ void k_deposit_cell(MeshView, Species, double, const double*, const int*, int64_t,
                                const double*, const double*, int, const uint8_t*, double*);
-__global__ void k_deposit_node(int64_t, const int*, const int*, const double*, double*);
+__global__ void k_deposit_node(int64_t, const int*, const int*, const int*, const double*, double*);
 __global__ void k_generate(MeshView, GenArgs, int64_t, double*, double*, uint8_t*, int*, int64_t*, int8_t*);
 __global__ void k_copy_probe(const float4*, float4*, int64_t);
 namespace amps {
@@ -96,6 +96,8 @@ static void free_mesh(amps_ctx* c) {
   (void)hipFree(c->d_leaf);
   (void)hipFree(c->d_inc_start);
   (void)hipFree(c->d_inc);
+  (void)hipFree(c->d_node_order);
+  c->d_node_order = nullptr;
   (void)hipFree(c->d_offset);
   (void)hipFree(c->d_cursor);
   (void)hipFree(c->d_tcount);
@@ -169,6 +171,12 @@ extern "C" int amps_gpu_device_info(amps_ctx* c, char* name, int* n_cu, int64_t*
   return AMPS_OK;
 }
 
+// chunk counts each deposit cell-pass kernel is instantiated for
+static bool deposit_variant_ok(int kernel, int64_t nch) {
+  if (kernel == 1) return nch == 4 || nch == 6 || nch == 12;
+  return nch == 2 || nch == 3 || nch == 4 || nch == 6;
+}
+
 extern "C" int amps_gpu_set_param(amps_ctx* c, const char* name, int64_t value) {
   if (!c || !name) return fail(AMPS_E_ARG, "amps_gpu_set_param: null argument");
   std::string n(name);
@@ -177,11 +185,17 @@ extern "C" int amps_gpu_set_param(amps_ctx* c, const char* name, int64_t value) 
     c->tile = (int)value;
     c->tiles_valid = false;
   } else if (n == "deposit_chunks") {
-    if (value != 2 && value != 3 && value != 4 && value != 6)
-      return fail(AMPS_E_ARG, "deposit_chunks must be 2, 3, 4 or 6");
+    if (!deposit_variant_ok(c->dep_kernel, value))
+      return fail(AMPS_E_ARG, "deposit_chunks must be 2, 3, 4 or 6 (deposit_kernel 0) or 4, 6 or 12 "
+                              "(deposit_kernel 1)");
     c->dep_nch = (int)value;
   } else if (n == "deposit_exact") {
     c->dep_exact = value ? 1 : 0;
+  } else if (n == "deposit_kernel") {
+    if (value != 0 && value != 1) return fail(AMPS_E_ARG, "deposit_kernel must be 0 or 1");
+    if (!deposit_variant_ok((int)value, c->dep_nch))
+      return fail(AMPS_E_ARG, "deposit_kernel: set deposit_chunks to a count this kernel runs with first");
+    c->dep_kernel = (int)value;
   } else {
     return fail(AMPS_E_ARG, "unknown parameter " + n);
   }
@@ -332,6 +346,15 @@ extern "C" int amps_gpu_set_mesh(amps_ctx* c, const amps_mesh_desc* desc, const 
         inc[(size_t)fillp[(size_t)node]++] = gid * 8 + q;
       }
     }
+  // node-gather order: by first incident cell, so consecutive workgroups read
+  // the partials of the same and neighbouring blocks while they are cached
+  std::vector<int> node_order((size_t)c->nnodes);
+  for (int64_t i = 0; i < c->nnodes; i++) node_order[(size_t)i] = (int)i;
+  std::stable_sort(node_order.begin(), node_order.end(), [&](int a, int b2) {
+    return inc[(size_t)inc_start[(size_t)a]] < inc[(size_t)inc_start[(size_t)b2]];
+  });
+  HIP_TRY(hipMalloc(&c->d_node_order, sizeof(int) * std::max<size_t>(1, node_order.size())));
+  HIP_TRY(hipMemcpy(c->d_node_order, node_order.data(), sizeof(int) * node_order.size(), hipMemcpyHostToDevice));
   // device upload
   HIP_TRY(hipMalloc(&c->d_xmin, sizeof(double) * 3 * nb));
   HIP_TRY(hipMalloc(&c->d_xmax, sizeof(double) * 3 * nb));
@@ -670,17 +693,22 @@ static void launch_dep(amps_ctx* c, double dt) {
 static int launch_deposit(amps_ctx* c, double dt) {
   tbegin(c);
   const bool ex = c->dep_exact != 0;
-  switch (c->dep_nch) {
-    case 2: ex ? launch_dep<2, true>(c, dt) : launch_dep<2, false>(c, dt); break;
-    case 3: ex ? launch_dep<3, true>(c, dt) : launch_dep<3, false>(c, dt); break;
-    case 6: ex ? launch_dep<6, true>(c, dt) : launch_dep<6, false>(c, dt); break;
-    default: ex ? launch_dep<4, true>(c, dt) : launch_dep<4, false>(c, dt); break;
+  if (c->dep_kernel == 1) {
+    if (int r = launch_deposit_cell_rec(c, dt)) return r;
+  } else {
+    switch (c->dep_nch) {
+      case 2: ex ? launch_dep<2, true>(c, dt) : launch_dep<2, false>(c, dt); break;
+      case 3: ex ? launch_dep<3, true>(c, dt) : launch_dep<3, false>(c, dt); break;
+      case 4: ex ? launch_dep<4, true>(c, dt) : launch_dep<4, false>(c, dt); break;
+      case 6: ex ? launch_dep<6, true>(c, dt) : launch_dep<6, false>(c, dt); break;
+      default: return fail(AMPS_E_ARG, "deposit_kernel 0 runs with deposit_chunks 2, 3, 4 or 6");
+    }
   }
   HIP_TRY(hipGetLastError());
   tend(c, AMPS_K_DEPOSIT_CELL);
   tbegin(c);
   hipLaunchKernelGGL(k_deposit_node, dim3((unsigned)((c->nnodes + 3) / 4)), dim3(256), 0, c->stream,
-                     c->nnodes, c->d_inc_start, c->d_inc, c->d_cellpart, c->d_nodeout);
+                     c->nnodes, c->d_node_order, c->d_inc_start, c->d_inc, c->d_cellpart, c->d_nodeout);
   HIP_TRY(hipGetLastError());
   tend(c, AMPS_K_DEPOSIT_NODE);
   return 0;

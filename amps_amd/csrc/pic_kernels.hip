@@ -382,12 +382,14 @@ INST_DEP(6, false)
 __constant__ signed char c_inv_im[8][27];  // slot -> partner corner j, or -1
 
 __global__ __launch_bounds__(256) void k_deposit_node(int64_t nnodes,
+                                                       const int* __restrict__ node_order,
                                                        const int* __restrict__ inc_start,
                                                        const int* __restrict__ inc,
                                                        const double* __restrict__ cellpart,
                                                        double* __restrict__ nodeout) {
-  const int64_t node = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (node >= nnodes) return;
+  const int64_t slot = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (slot >= nnodes) return;
+  const int64_t node = node_order[slot];
   const int lane = threadIdx.x & 63;
   double acc[4] = {0.0, 0.0, 0.0, 0.0};
   const int e0 = inc_start[node], e1 = inc_start[node + 1];

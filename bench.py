@@ -224,7 +224,8 @@ def main():
     if not args.no_tune:
         sig = f"{arch}|pic|nb{mesh.nblocks}|n{n}"
         tune, cached = autotune.tune_pic(ctx, st, opts, dt, sig, cache=args.tune_cache)
-        tune = {"tile": tune["tile"], "deposit_chunks": tune["deposit_chunks"], "cached": cached}
+        tune = {"tile": tune["tile"], "deposit_kernel": tune["deposit_kernel"],
+                "deposit_chunks": tune["deposit_chunks"], "deposit_ms": tune["deposit_ms"], "cached": cached}
     # warm-up, then the timed region (graph-captured steps)
     if args.warmup > 0:
         st.run_steps(args.warmup, dt, use_graph=True, **opts)

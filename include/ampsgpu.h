@@ -57,7 +57,9 @@ int amps_gpu_device_info(amps_ctx* ctx, char* name, int* n_cu, int64_t* hbm_byte
 /* Tuning parameters (no reference counterpart; set by the autotuner):
  * "tile" (particles per push workgroup, 64..2^20), "deposit_chunks"
  * (2,3,4,6 waves per deposit workgroup), "deposit_exact" (0/1: unfused
- * mul+add in the deposit accumulation for a bit-exact summation). */
+ * mul+add in the deposit accumulation for a bit-exact summation),
+ * "deposit_kernel" (0: each lane evaluates its cell's particles itself,
+ * 1 (default): particles evaluated once into shared-memory record batches). */
 int amps_gpu_set_param(amps_ctx* ctx, const char* name, int64_t value);
 
 /* ---- mesh ------------------------------------------------------------ */

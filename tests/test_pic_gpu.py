@@ -101,14 +101,24 @@ def dep_case():
     return mesh, E, B, parts, sp, dt
 
 
+DEPOSIT_VARIANTS = [(1, 6), (1, 4), (1, 12), (0, 4), (0, 2), (0, 3), (0, 6)]
+
+
+def set_deposit_variant(ctx, kernel, nch):
+    # chunk counts 4 and 6 are valid for both kernels, so pass through 6
+    ctx.set_param("deposit_chunks", 6)
+    ctx.set_param("deposit_kernel", kernel)
+    ctx.set_param("deposit_chunks", nch)
+
+
 @pytest.mark.parametrize("exact", [1, 0])
-@pytest.mark.parametrize("nch", [4, 2, 3, 6])
-def test_deposit_parity(gpu_ctx, dep_case, exact, nch):
+@pytest.mark.parametrize("kernel,nch", DEPOSIT_VARIANTS)
+def test_deposit_parity(gpu_ctx, dep_case, exact, kernel, nch):
     mesh, E, B, parts, sp, dt = dep_case
-    print(f"deposit case exact={exact} nch={nch} n={parts.n}", flush=True)
+    print(f"deposit case kernel={kernel} exact={exact} nch={nch} n={parts.n}", flush=True)
     st = PicState(mesh, E, B, sp, ctx=gpu_ctx)
     st.set_particles(parts)
-    gpu_ctx.set_param("deposit_chunks", nch)
+    set_deposit_variant(gpu_ctx, kernel, nch)
     gpu_ctx.set_param("deposit_exact", exact)
     J, Mm = st.UpdateJMassMatrix(dt)
     g, _ = st.get_particles()  # the device order (the oracle sums in it)
@@ -124,7 +134,7 @@ def test_deposit_parity(gpu_ctx, dep_case, exact, nch):
         nm = np.linalg.norm(Mm - Mo, axis=1) / np.maximum(np.linalg.norm(Mo, axis=1), 1e-300)
         assert nj.max() <= 1e-12 and nm.max() <= 1e-12, (nj.max(), nm.max())
     gpu_ctx.set_param("deposit_exact", 0)
-    gpu_ctx.set_param("deposit_chunks", 4)
+    set_deposit_variant(gpu_ctx, 1, 6)
 
 
 def test_deposit_after_push_and_migration(gpu_ctx, dep_case):
