@@ -18,6 +18,7 @@ CODE_FINISHED = 3
 CODE_SPHERE = 5
 MOVER_LAPENTA = 0
 MOVER_BORIS = 1
+MOVER_RELATIVISTIC = 2
 BC_DELETE = 0
 BC_REFLECT = 1
 FP64 = 0

@@ -622,6 +622,14 @@ extern "C" int amps_gpu_compact(amps_ctx* c) {
 // ---------------------------------------------------------------------
 // mover
 // ---------------------------------------------------------------------
+static int check_mover(const amps_ctx* c, const amps_push_opts* o, const char* who) {
+  if (o->mover != AMPS_MOVER_LAPENTA && o->mover != AMPS_MOVER_BORIS && o->mover != AMPS_MOVER_RELATIVISTIC)
+    return fail(AMPS_E_ARG, std::string(who) + ": unknown mover");
+  if (o->mover == AMPS_MOVER_RELATIVISTIC && c->precision == AMPS_FP32)
+    return fail(AMPS_E_ARG, std::string(who) + ": the relativistic mover runs in FP64 only");
+  return 0;
+}
+
 static int launch_push(amps_ctx* c, const amps_push_opts* o) {
   const int k = c->cur;
   if (c->precision == AMPS_FP32) {
@@ -643,6 +651,11 @@ static int launch_push(amps_ctx* c, const amps_push_opts* o) {
                        c->stream, c->mv, c->faces, c->sp, c->d_E, c->d_B, c->d_tiles, c->d_ntiles,
                        o->bc_mode, o->r_sphere, c->cap, c->d_x[k], c->d_v[k], c->d_spec[k],
                        c->d_key[k], c->d_code);
+  else if (o->mover == AMPS_MOVER_RELATIVISTIC)
+    hipLaunchKernelGGL(k_push_f64<AMPS_MOVER_RELATIVISTIC>, dim3((unsigned)c->max_tiles), dim3(256), 0,
+                       c->stream, c->mv, c->faces, c->sp, c->d_E, c->d_B, c->d_tiles, c->d_ntiles,
+                       o->bc_mode, o->r_sphere, c->cap, c->d_x[k], c->d_v[k], c->d_spec[k],
+                       c->d_key[k], c->d_code);
   else
     hipLaunchKernelGGL(k_push_f64<AMPS_MOVER_BORIS>, dim3((unsigned)c->max_tiles), dim3(256), 0,
                        c->stream, c->mv, c->faces, c->sp, c->d_E, c->d_B, c->d_tiles, c->d_ntiles,
@@ -656,8 +669,7 @@ extern "C" int amps_gpu_move_particles(amps_ctx* c, const amps_push_opts* o) {
   if (!c || !o) return fail(AMPS_E_ARG, "amps_gpu_move_particles: null argument");
   if (!c->has_mesh || !c->has_fields || !c->has_species)
     return fail(AMPS_E_STATE, "amps_gpu_move_particles: mesh, fields and species are required");
-  if (o->mover != AMPS_MOVER_LAPENTA && o->mover != AMPS_MOVER_BORIS)
-    return fail(AMPS_E_ARG, "amps_gpu_move_particles: unknown mover");
+  if (int r = check_mover(c, o, "amps_gpu_move_particles")) return r;
   if (o->bc_mode != AMPS_BC_DELETE && o->bc_mode != AMPS_BC_REFLECT)
     return fail(AMPS_E_ARG, "amps_gpu_move_particles: unknown boundary mode");
   if (c->cap == 0) return AMPS_OK;
@@ -824,8 +836,7 @@ extern "C" int amps_gpu_run_steps(amps_ctx* c, const amps_push_opts* o, double d
   if (!c || !o) return fail(AMPS_E_ARG, "amps_gpu_run_steps: null argument");
   if (!c->has_mesh || !c->has_fields || !c->has_species || c->cap == 0)
     return fail(AMPS_E_STATE, "amps_gpu_run_steps: mesh, fields, species and particles are required");
-  if (o->mover != AMPS_MOVER_LAPENTA && o->mover != AMPS_MOVER_BORIS)
-    return fail(AMPS_E_ARG, "amps_gpu_run_steps: unknown mover");
+  if (int r = check_mover(c, o, "amps_gpu_run_steps")) return r;
   if (o->bc_mode != AMPS_BC_DELETE && o->bc_mode != AMPS_BC_REFLECT)
     return fail(AMPS_E_ARG, "amps_gpu_run_steps: unknown boundary mode");
   if (nsteps <= 0) return AMPS_OK;

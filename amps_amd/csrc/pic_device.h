@@ -295,6 +295,190 @@ __device__ __forceinline__ int push_particle(const MeshView& m, const Faces& fac
                   cell);
 }
 
+constexpr double SPEED_OF_LIGHT = 299792458.0;  // SpeedOfLight, constants.h:40
+constexpr double PI_TIMES_2 = 6.28318530717958647692528676655900576839433879875021;  // constants.h:17
+
+// One particle of PIC::Mover::Relativistic::Boris
+// (pic_mover_relativistic_boris.cpp:16-583, scalar branch, forward time):
+// sub-steps of at most one gyro period (:113-124) with the fields of each
+// sub-step's block — the tile's block from LDS, any other block from HBM —
+// momentum-form Boris rotation (:134-167), sphere absorption, and the exit
+// face found along the chord xFinal - xInit with a fractional intersection
+// time (:317-399; the found face's normal is used where the reference indexes
+// its face table with the spent loop counter).  A reflected particle is
+// committed at the face and the rest of its step dropped (the reference
+// aborts there, :426-432).
+__device__ __forceinline__ int push_particle_rel(const MeshView& m, const Faces& faces, int b0,
+                                                 const BlockGeom& g0, const double* sE,
+                                                 const double* sB, const double* __restrict__ Eg,
+                                                 const double* __restrict__ Bg, double ElectricCharge,
+                                                 double mass, double dtTotalIn, int bc_mode,
+                                                 double r_sphere, double xp[3], double vp[3],
+                                                 int* blk, int* cell) {
+  const double c2 = SPEED_OF_LIGHT * SPEED_OF_LIGHT;
+  double vInit[3] = {vp[0], vp[1], vp[2]}, xInit[3] = {xp[0], xp[1], xp[2]};
+  double vFinal[3] = {vp[0], vp[1], vp[2]}, xFinal[3] = {xp[0], xp[1], xp[2]};
+  int curb = b0;
+  BlockGeom g = g0;
+  while (dtTotalIn > 0.0) {
+    double gamma = 1.0 / sqrt(1.0 - (vInit[0] * vInit[0] + vInit[1] * vInit[1] + vInit[2] * vInit[2]) / c2);
+    double wn[8], table[8];
+    int iX[3];
+    corner_stencil(g, xInit, wn, table, iX);
+#pragma unroll
+    for (int d = 0; d < 3; d++) iX[d] = clampi(iX[d], 0, NC - 1);
+    const double* fE = (curb == b0) ? sE : Eg + (size_t)curb * NCORNER * 3;
+    const double* fB = (curb == b0) ? sB : Bg + (size_t)curb * NCORNER * 3;
+    double E[3] = {0.0, 0.0, 0.0}, B[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int l = 0; l < 8; l++) {
+      const int c = (iX[0] + (l >> 2)) + NN * ((iX[1] + ((l >> 1) & 1)) + NN * (iX[2] + (l & 1)));
+#pragma unroll
+      for (int d = 0; d < 3; d++) {
+        E[d] += wn[l] * fE[3 * c + d];
+        B[d] += wn[l] * fB[3 * c + d];
+      }
+    }
+    double dt;
+    const double lB = sqrt(B[0] * B[0] + B[1] * B[1] + B[2] * B[2]);
+    if (lB > 1.0E-25) {
+      const double GyroFreq = fabs(ElectricCharge) * lB / (PI_TIMES_2 * mass * gamma);
+      const double dtMax = 1.0 / GyroFreq;
+      dt = (dtMax < dtTotalIn) ? dtMax : dtTotalIn;
+    } else {
+      dt = dtTotalIn;
+    }
+    dtTotalIn -= dt;
+    const double QdT_over_twoM = ElectricCharge * dt / (2.0 * mass);
+    double uMinus[3], t[3], s[3], uPrime[3], uPlus[3], uFinal[3], l2 = 0.0;
+#pragma unroll
+    for (int d = 0; d < 3; d++) uMinus[d] = gamma * vInit[d] + QdT_over_twoM * E[d];
+    gamma = sqrt(1.0 + (uMinus[0] * uMinus[0] + uMinus[1] * uMinus[1] + uMinus[2] * uMinus[2]) / c2);
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+      t[d] = QdT_over_twoM / gamma * B[d];
+      l2 += t[d] * t[d];
+    }
+    uPrime[0] = uMinus[1] * t[2] - uMinus[2] * t[1];
+    uPrime[1] = uMinus[2] * t[0] - uMinus[0] * t[2];
+    uPrime[2] = uMinus[0] * t[1] - uMinus[1] * t[0];
+#pragma unroll
+    for (int d = 0; d < 3; d++) uPrime[d] += uMinus[d];
+#pragma unroll
+    for (int d = 0; d < 3; d++) s[d] = 2.0 * t[d] / (1.0 + l2);
+    uPlus[0] = uPrime[1] * s[2] - uPrime[2] * s[1];
+    uPlus[1] = uPrime[2] * s[0] - uPrime[0] * s[2];
+    uPlus[2] = uPrime[0] * s[1] - uPrime[1] * s[0];
+#pragma unroll
+    for (int d = 0; d < 3; d++) uPlus[d] += uMinus[d];
+#pragma unroll
+    for (int d = 0; d < 3; d++) uFinal[d] = uPlus[d] + QdT_over_twoM * E[d];
+    gamma = sqrt(1.0 + (uFinal[0] * uFinal[0] + uFinal[1] * uFinal[1] + uFinal[2] * uFinal[2]) / c2);
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+      vFinal[d] = uFinal[d] / gamma;
+      xFinal[d] = xInit[d] + vFinal[d] * dt;
+    }
+    double rFinal2;
+    if (r_sphere > 0.0 &&
+        (rFinal2 = xFinal[0] * xFinal[0] + xFinal[1] * xFinal[1] + xFinal[2] * xFinal[2]) <
+            r_sphere * r_sphere) {
+      const double r = sqrt(rFinal2);
+#pragma unroll
+      for (int d = 0; d < 3; d++) xFinal[d] *= r_sphere / r;
+#pragma unroll
+      for (int d = 0; d < 3; d++) xp[d] = xFinal[d], vp[d] = vFinal[d];
+      return AMPS_CODE_SPHERE;
+    }
+    int newNode = find_block(m, xFinal);
+    if (newNode < 0) {
+      if (bc_mode == AMPS_BC_DELETE) {
+#pragma unroll
+        for (int d = 0; d < 3; d++) xp[d] = xFinal[d], vp[d] = vFinal[d];
+        return AMPS_CODE_LEFT_DOMAIN;
+      }
+      int nIntersectionFace = -1;
+      double vEffective[3], r0[3], dtIntersection = -1.0;
+#pragma unroll
+      for (int d = 0; d < 3; d++) vEffective[d] = xFinal[d] - xInit[d];
+      for (int nface = 0; nface < 6; nface++) {
+        const Face& F = faces.f[nface];
+        double cx = 0.0, cv = 0.0;
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+          r0[d] = xInit[d] - F.x0[d];
+          cx += r0[d] * F.norm[d];
+          cv += vEffective[d] * F.norm[d];
+        }
+        const double dtEffective = (cv > 0.0) ? -cx / cv : -1.0;
+        if (dtEffective > 0.0) {
+          if ((dtIntersection < 0.0) || ((dtEffective < dtIntersection) && (dtEffective > 0.0))) {
+            double cE0 = 0.0, cE1 = 0.0;
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+              const double c = r0[d] + dtEffective * vEffective[d];
+              cE0 += c * F.e0[d], cE1 += c * F.e1[d];
+            }
+            if ((cE0 < -m.eps) || (cE0 > F.lE0 + m.eps) || (cE1 < -m.eps) || (cE1 > F.lE1 + m.eps))
+              continue;
+            nIntersectionFace = nface, dtIntersection = dtEffective;
+          }
+        }
+      }
+      if (nIntersectionFace == -1) {
+#pragma unroll
+        for (int d = 0; d < 3; d++) xp[d] = xFinal[d], vp[d] = vFinal[d];
+        return AMPS_CODE_LEFT_DOMAIN;
+      }
+      const Face& F = faces.f[nIntersectionFace];
+#pragma unroll
+      for (int d = 0; d < 3; d++) {
+        xInit[d] += dtIntersection * (xFinal[d] - xInit[d]) - F.norm[d] * m.eps;
+        vInit[d] += dtIntersection * (vFinal[d] - vInit[d]);
+      }
+      newNode = find_block(m, xInit);
+      if (newNode < 0) {
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+          if (m.gxmin[d] >= xInit[d]) xInit[d] = m.gxmin[d] + m.eps;
+          if (m.gxmax[d] <= xInit[d]) xInit[d] = m.gxmax[d] - m.eps;
+        }
+        newNode = find_block(m, xInit);
+        if (newNode < 0) {
+#pragma unroll
+          for (int d = 0; d < 3; d++) xp[d] = xInit[d], vp[d] = vInit[d];
+          return AMPS_CODE_LEFT_DOMAIN;
+        }
+      }
+      double cc = 0.0;
+#pragma unroll
+      for (int d = 0; d < 3; d++) cc += F.norm[d] * vInit[d];
+#pragma unroll
+      for (int d = 0; d < 3; d++) vInit[d] -= 2.0 * cc * F.norm[d];
+      const int c_id = find_cell(m, xInit, newNode);
+      if (c_id < 0) return AMPS_CODE_LEFT_DOMAIN;
+      *blk = newNode;
+      *cell = c_id;
+#pragma unroll
+      for (int d = 0; d < 3; d++) xp[d] = xInit[d], vp[d] = vInit[d];
+      return AMPS_CODE_REFLECTED;
+    }
+    if (newNode != curb) {
+      curb = newNode;
+      load_block_geom(m, curb, g);
+    }
+#pragma unroll
+    for (int d = 0; d < 3; d++) xInit[d] = xFinal[d], vInit[d] = vFinal[d];
+  }
+  const int c_id = find_cell(m, xFinal, curb);
+  if (c_id < 0) return AMPS_CODE_LEFT_DOMAIN;
+  *blk = curb;
+  *cell = c_id;
+#pragma unroll
+  for (int d = 0; d < 3; d++) xp[d] = xFinal[d], vp[d] = vFinal[d];
+  return AMPS_CODE_FINISHED;
+}
+
 // FP32 push: the same mover with FP32 stencil weights, field interpolation
 // and velocity update; the position increment dt*v is formed in FP32 and
 // accumulated into the FP64 position, and block/cell location, sphere and

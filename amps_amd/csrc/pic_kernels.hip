@@ -46,8 +46,13 @@ __global__ __launch_bounds__(256) void k_push_f64(MeshView m, Faces faces, Speci
     double vp[3] = {v[p], v[cap + p], v[2 * cap + p]};
     const int s = spec[p];
     int nb_out = b, cell_out = 0;
-    const int c = push_particle<MOVER>(m, faces, g, sE, sB, sp.charge[s], sp.mass[s], sp.dt[s],
-                                       bc_mode, r_sphere, xp, vp, &nb_out, &cell_out);
+    int c;
+    if constexpr (MOVER == AMPS_MOVER_RELATIVISTIC)
+      c = push_particle_rel(m, faces, b, g, sE, sB, E, B, sp.charge[s], sp.mass[s], sp.dt[s], bc_mode,
+                            r_sphere, xp, vp, &nb_out, &cell_out);
+    else
+      c = push_particle<MOVER>(m, faces, g, sE, sB, sp.charge[s], sp.mass[s], sp.dt[s], bc_mode,
+                               r_sphere, xp, vp, &nb_out, &cell_out);
     x[p] = xp[0];
     x[cap + p] = xp[1];
     x[2 * cap + p] = xp[2];
@@ -67,6 +72,10 @@ template __global__ void k_push_f64<AMPS_MOVER_BORIS>(MeshView, Faces, Species, 
                                                        const double*, const Tile*, const int*, int,
                                                        double, int64_t, double*, double*,
                                                        const uint8_t*, int*, int8_t*);
+template __global__ void k_push_f64<AMPS_MOVER_RELATIVISTIC>(MeshView, Faces, Species, const double*,
+                                                              const double*, const Tile*, const int*,
+                                                              int, double, int64_t, double*, double*,
+                                                              const uint8_t*, int*, int8_t*);
 
 // FP32 push: float E/B stencil in LDS (17.5 KB), float velocities, FP64
 // positions.

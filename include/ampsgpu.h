@@ -33,6 +33,7 @@ extern "C" {
 
 #define AMPS_MOVER_LAPENTA 0      /* PIC::Mover::Lapenta2017, pic_mover_boris.cpp:876-1380 */
 #define AMPS_MOVER_BORIS 1        /* PIC::Mover::Boris, pic_mover_boris.cpp:126-553 */
+#define AMPS_MOVER_RELATIVISTIC 2 /* PIC::Mover::Relativistic::Boris, pic_mover_relativistic_boris.cpp:16-583 (FP64) */
 
 #define AMPS_BC_DELETE 0          /* _PIC_PARTICLE_DOMAIN_BOUNDARY_INTERSECTION_PROCESSING_MODE__DELETE_ */
 #define AMPS_BC_REFLECT 1         /* ..._SPECULAR_REFLECTION_ (pic_mover_boris.cpp:1236-1253) */
@@ -111,7 +112,9 @@ typedef struct {
   double r_sphere;    /* internal absorbing sphere radius, <= 0: none */
 } amps_push_opts;
 /* One PIC::Mover::MoveParticles step (pic_mover.cpp:580-1088) through the
- * per-particle Lapenta2017/Boris mover; codes stay readable until the next
+ * per-particle Lapenta2017, Boris or relativistic Boris mover (the last in
+ * FP64 only, with SpeedOfLight in the units of the mesh and fields); codes
+ * stay readable until the next
  * compaction.  Exits (AMPS_CODE_LEFT_DOMAIN) are what the reference hands
  * to ProcessOutsideDomainParticles (pic.h:6043-6044). */
 int amps_gpu_move_particles(amps_ctx* ctx, const amps_push_opts* opts);

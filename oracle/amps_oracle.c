@@ -436,6 +436,160 @@ static int push_one(const orc_mesh* m, const orc_face* faces, const double* E_co
   return code;
 }
 
+/* PIC::Mover::Relativistic::Boris, pic_mover_relativistic_boris.cpp:16-583,
+ * scalar (non-AVX) branch, forward time integration.  Sub-steps limited to one
+ * gyro period 1/f_gyro (:113-124, specfunc.h:1290-1292) with the fields
+ * re-interpolated in the block of each sub-step; momentum form
+ * u- = gamma v + (q dt/2m) E, rotation t = (q dt/2m/gamma) B,
+ * s = 2t/(1+|t|^2), u = u+ + (q dt/2m) E, v = u/gamma(u) (:134-167).
+ * Exit faces use the chord xFinal - xInit as the effective velocity and the
+ * fractional intersection time (:317-399).  Where the reference reads the face
+ * table at the loop counter after the loop (:391, :396 index nface == 6), the
+ * found face nIntersectionFace is used.  A reflected particle is committed at
+ * the face (the reference aborts in its final switch, :426-432) and the rest
+ * of the step is dropped; sphere contact absorbs as in the other movers. */
+static int push_rel_one(const orc_mesh* m, const orc_face* faces, const double* E_corner,
+                        const double* B_corner, double ElectricCharge, double mass, double dtTotalIn,
+                        int bc_mode, double r_sphere, double xp[3], double vp[3], int* blk,
+                        int* cell) {
+  const double c2 = ORC_SPEED_OF_LIGHT * ORC_SPEED_OF_LIGHT;
+  double vInit[3] = {vp[0], vp[1], vp[2]}, xInit[3] = {xp[0], xp[1], xp[2]};
+  double vFinal[3] = {vp[0], vp[1], vp[2]}, xFinal[3] = {xp[0], xp[1], xp[2]};
+  int startNode = m->block_node[*blk];
+  int newNode = startNode;
+  while (dtTotalIn > 0.0) {
+    double gamma = 1.0 / sqrt(1.0 - (vInit[0] * vInit[0] + vInit[1] * vInit[1] + vInit[2] * vInit[2]) / c2);
+    const orc_tnode* sn = &m->t[startNode];
+    double w[8], table[8];
+    int cr[8];
+    corner_stencil_node(sn, xInit, w, table, cr);
+    double E[3] = {0.0, 0.0, 0.0}, B[3] = {0.0, 0.0, 0.0};
+    const double* Eb = E_corner + (size_t)sn->block * ORC_NCORNER * 3;
+    const double* Bb = B_corner + (size_t)sn->block * ORC_NCORNER * 3;
+    for (int l = 0; l < 8; l++)
+      for (int d = 0; d < 3; d++) {
+        E[d] += w[l] * Eb[3 * cr[l] + d];
+        B[d] += w[l] * Bb[3 * cr[l] + d];
+      }
+    double dt;
+    const double lB = sqrt(B[0] * B[0] + B[1] * B[1] + B[2] * B[2]);
+    if (lB > 1.0E-25) {
+      /* GetGyroFrequency: |q| |B| / (2 pi m gamma(v)), gamma(v) as above */
+      const double GyroFreq = fabs(ElectricCharge) * lB / (ORC_PI_TIMES_2 * mass * gamma);
+      const double dtMax = 1.0 / GyroFreq;
+      dt = (dtMax < dtTotalIn) ? dtMax : dtTotalIn;
+    } else {
+      dt = dtTotalIn;
+    }
+    dtTotalIn -= dt;
+    const double QdT_over_twoM = ElectricCharge * dt / (2.0 * mass);
+    double uMinus[3], t[3], s[3], uPrime[3], uPlus[3], uFinal[3], l = 0.0;
+    for (int d = 0; d < 3; d++) uMinus[d] = gamma * vInit[d] + QdT_over_twoM * E[d];
+    gamma = sqrt(1.0 + (uMinus[0] * uMinus[0] + uMinus[1] * uMinus[1] + uMinus[2] * uMinus[2]) / c2);
+    for (int d = 0; d < 3; d++) {
+      t[d] = QdT_over_twoM / gamma * B[d];
+      l += t[d] * t[d];
+    }
+    uPrime[0] = uMinus[1] * t[2] - uMinus[2] * t[1];
+    uPrime[1] = uMinus[2] * t[0] - uMinus[0] * t[2];
+    uPrime[2] = uMinus[0] * t[1] - uMinus[1] * t[0];
+    for (int d = 0; d < 3; d++) uPrime[d] += uMinus[d];
+    for (int d = 0; d < 3; d++) s[d] = 2.0 * t[d] / (1.0 + l);
+    uPlus[0] = uPrime[1] * s[2] - uPrime[2] * s[1];
+    uPlus[1] = uPrime[2] * s[0] - uPrime[0] * s[2];
+    uPlus[2] = uPrime[0] * s[1] - uPrime[1] * s[0];
+    for (int d = 0; d < 3; d++) uPlus[d] += uMinus[d];
+    for (int d = 0; d < 3; d++) uFinal[d] = uPlus[d] + QdT_over_twoM * E[d];
+    gamma = sqrt(1.0 + (uFinal[0] * uFinal[0] + uFinal[1] * uFinal[1] + uFinal[2] * uFinal[2]) / c2);
+    for (int d = 0; d < 3; d++) {
+      vFinal[d] = uFinal[d] / gamma;
+      xFinal[d] = xInit[d] + vFinal[d] * dt;
+    }
+    /* internal sphere (:270-292): absorbed */
+    double rFinal2;
+    if (r_sphere > 0.0 &&
+        (rFinal2 = xFinal[0] * xFinal[0] + xFinal[1] * xFinal[1] + xFinal[2] * xFinal[2]) <
+            r_sphere * r_sphere) {
+      const double r = sqrt(rFinal2);
+      for (int d = 0; d < 3; d++) xFinal[d] *= r_sphere / r;
+      for (int d = 0; d < 3; d++) xp[d] = xFinal[d], vp[d] = vFinal[d];
+      return ORC_CODE_SPHERE;
+    }
+    newNode = find_tnode(m, xFinal, startNode);
+    if (newNode < 0) {
+      if (bc_mode == ORC_BC_DELETE) {
+        for (int d = 0; d < 3; d++) xp[d] = xFinal[d], vp[d] = vFinal[d];
+        return ORC_CODE_LEFT_DOMAIN;
+      }
+      int nIntersectionFace = -1;
+      double vEffective[3], r0[3], dtIntersection = -1.0;
+      for (int d = 0; d < 3; d++) vEffective[d] = xFinal[d] - xInit[d];
+      for (int nface = 0; nface < 6; nface++) {
+        double cx = 0.0, cv = 0.0;
+        for (int d = 0; d < 3; d++) {
+          r0[d] = xInit[d] - faces[nface].x0[d];
+          cx += r0[d] * faces[nface].norm[d];
+          cv += vEffective[d] * faces[nface].norm[d];
+        }
+        const double dtEffective = (cv > 0.0) ? -cx / cv : -1.0;
+        if (dtEffective > 0.0) {
+          if ((dtIntersection < 0.0) || ((dtEffective < dtIntersection) && (dtEffective > 0.0))) {
+            double cE0 = 0.0, cE1 = 0.0;
+            for (int d = 0; d < 3; d++) {
+              const double c = r0[d] + dtEffective * vEffective[d];
+              cE0 += c * faces[nface].e0[d], cE1 += c * faces[nface].e1[d];
+            }
+            if ((cE0 < -m->eps) || (cE0 > faces[nface].lE0 + m->eps) || (cE1 < -m->eps) ||
+                (cE1 > faces[nface].lE1 + m->eps))
+              continue;
+            nIntersectionFace = nface, dtIntersection = dtEffective;
+          }
+        }
+      }
+      if (nIntersectionFace == -1) {
+        for (int d = 0; d < 3; d++) xp[d] = xFinal[d], vp[d] = vFinal[d];
+        return ORC_CODE_LEFT_DOMAIN;
+      }
+      for (int d = 0; d < 3; d++) {
+        xInit[d] += dtIntersection * (xFinal[d] - xInit[d]) - faces[nIntersectionFace].norm[d] * m->eps;
+        vInit[d] += dtIntersection * (vFinal[d] - vInit[d]);
+      }
+      newNode = find_tnode(m, xInit, startNode);
+      if (newNode < 0) {
+        for (int d = 0; d < 3; d++) {
+          if (m->gxmin[d] >= xInit[d]) xInit[d] = m->gxmin[d] + m->eps;
+          if (m->gxmax[d] <= xInit[d]) xInit[d] = m->gxmax[d] - m->eps;
+        }
+        newNode = find_tnode(m, xInit, startNode);
+        if (newNode < 0) {
+          for (int d = 0; d < 3; d++) xp[d] = xInit[d], vp[d] = vInit[d];
+          return ORC_CODE_LEFT_DOMAIN;
+        }
+      }
+      double cc = 0.0;
+      for (int d = 0; d < 3; d++) cc += faces[nIntersectionFace].norm[d] * vInit[d];
+      for (int d = 0; d < 3; d++) vInit[d] -= 2.0 * cc * faces[nIntersectionFace].norm[d];
+      int ijk[3];
+      const int c_id = find_cell_node(m, xInit, &m->t[newNode], ijk);
+      if (c_id < 0) return ORC_CODE_LEFT_DOMAIN;
+      *blk = m->t[newNode].block;
+      *cell = c_id;
+      for (int d = 0; d < 3; d++) xp[d] = xInit[d], vp[d] = vInit[d];
+      return ORC_CODE_REFLECTED;
+    }
+    startNode = newNode;
+    for (int d = 0; d < 3; d++) xInit[d] = xFinal[d], vInit[d] = vFinal[d];
+  }
+  /* FindCellIndex (:553) and commit (:561-562) */
+  int ijk[3];
+  const int c_id = find_cell_node(m, xFinal, &m->t[newNode], ijk);
+  if (c_id < 0) return ORC_CODE_LEFT_DOMAIN;
+  *blk = m->t[newNode].block;
+  *cell = c_id;
+  for (int d = 0; d < 3; d++) xp[d] = xFinal[d], vp[d] = vFinal[d];
+  return ORC_CODE_FINISHED;
+}
+
 int orc_push(const orc_mesh* m, const double* E_corner, const double* B_corner,
              const double* charge, const double* mass, const double* dt_spec, int mover,
              int bc_mode, double r_sphere, int64_t n, double* x, double* v, const int* spec,
@@ -450,8 +604,12 @@ int orc_push(const orc_mesh* m, const double* E_corner, const double* B_corner,
     double xp[3] = {x[p], x[n + p], x[2 * n + p]};
     double vp[3] = {v[p], v[n + p], v[2 * n + p]};
     int s = spec[p];
-    code[p] = push_one(m, faces, E_corner, B_corner, charge[s], mass[s], dt_spec[s], mover,
-                       bc_mode, r_sphere, xp, vp, &blk[p], &cell[p]);
+    if (mover == ORC_MOVER_RELATIVISTIC)
+      code[p] = push_rel_one(m, faces, E_corner, B_corner, charge[s], mass[s], dt_spec[s], bc_mode,
+                             r_sphere, xp, vp, &blk[p], &cell[p]);
+    else
+      code[p] = push_one(m, faces, E_corner, B_corner, charge[s], mass[s], dt_spec[s], mover,
+                         bc_mode, r_sphere, xp, vp, &blk[p], &cell[p]);
     for (int d = 0; d < 3; d++) {
       x[d * n + p] = xp[d];
       v[d * n + p] = vp[d];

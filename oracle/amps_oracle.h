@@ -36,6 +36,10 @@ extern "C" {
 
 #define ORC_MOVER_LAPENTA 0   /* PIC::Mover::Lapenta2017 (alpha-matrix form) */
 #define ORC_MOVER_BORIS 1     /* PIC::Mover::Boris (h/s rotation form) */
+#define ORC_MOVER_RELATIVISTIC 2 /* PIC::Mover::Relativistic::Boris (gyro-limited sub-steps) */
+
+#define ORC_SPEED_OF_LIGHT 299792458.0                  /* SpeedOfLight, constants.h:40 */
+#define ORC_PI_TIMES_2 6.28318530717958647692528676655900576839433879875021 /* PiTimes2, constants.h:17 */
 
 typedef struct orc_mesh orc_mesh;
 

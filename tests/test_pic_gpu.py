@@ -23,7 +23,7 @@ def _case(mesh, n, vth, dt, seed=1):
     return E, B, parts, Species([1.0], [1.0], [dt])
 
 
-@pytest.mark.parametrize("mover", [L.MOVER_LAPENTA, L.MOVER_BORIS])
+@pytest.mark.parametrize("mover", [L.MOVER_LAPENTA, L.MOVER_BORIS, L.MOVER_RELATIVISTIC])
 @pytest.mark.parametrize("bc", [L.BC_REFLECT, L.BC_DELETE])
 @pytest.mark.parametrize("amr", [False, True])
 def test_push_single_step_bitexact(gpu_ctx, mover, bc, amr):
@@ -42,6 +42,35 @@ def test_push_single_step_bitexact(gpu_ctx, mover, bc, amr):
     assert set(np.unique(ocode)) >= ({-1, 3, 5} if bc == L.BC_REFLECT else {2, 3, 5})
     assert np.array_equal(g.x, o.x) and np.array_equal(g.v, o.v)
     alive = (ocode == 3) | (ocode == -1)
+    assert np.array_equal(g.blk[alive], o.blk[alive]) and np.array_equal(g.cell[alive], o.cell[alive])
+
+
+@pytest.mark.parametrize("bc", [L.BC_REFLECT, L.BC_DELETE])
+def test_push_relativistic_si_bitexact(gpu_ctx, bc):
+    """Relativistic protons (0.1c-0.95c) in SI units on the 2-level mesh
+    scaled to metres: gyro-limited sub-steps that cross blocks (fields of
+    other blocks read from HBM), exits along the chord, sphere absorption."""
+    c = 299792458.0
+    mesh = M.central_octant_mesh()
+    E, B = M.corner_fields(mesh, E0=(1.0e5, 0.0, 0.0), moment=0.1, bz0=0.05)
+    parts = M.uniform_particles(mesh, 40000, 1.0, seed_x=21, seed_v=22)
+    rng = np.random.default_rng(23)
+    speed = rng.uniform(0.1, 0.95, parts.n) * c
+    parts.v *= speed / np.linalg.norm(parts.v, axis=0)
+    sp = Species([1.602176565e-19], [1.67262192e-27], [1.0e-8])
+    print(f"relativistic SI case bc={bc} n={parts.n}", flush=True)
+    st = PicState(mesh, E, B, sp, ctx=gpu_ctx)
+    st.set_particles(parts)
+    st.MoveParticles(mover=L.MOVER_RELATIVISTIC, bc_mode=bc, r_sphere=0.2, compact=False)
+    g, gcode = st.get_particles()
+    om = O.Mesh(mesh)
+    o, ocode = O.push(om, E, B, sp, parts, mover=L.MOVER_RELATIVISTIC, bc_mode=bc, r_sphere=0.2)
+    g, gcode = _by_id(g, gcode)
+    o, ocode = _by_id(o, ocode)
+    assert np.array_equal(gcode, ocode)
+    assert np.array_equal(g.x, o.x) and np.array_equal(g.v, o.v)
+    alive = (ocode == 3) | (ocode == -1)
+    assert alive.sum() > 1000
     assert np.array_equal(g.blk[alive], o.blk[alive]) and np.array_equal(g.cell[alive], o.cell[alive])
 
 
