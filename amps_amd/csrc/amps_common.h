@@ -105,10 +105,10 @@ struct amps_ctx {
   int* d_ntiles = nullptr;
   int* d_tcount = nullptr;     // per-block tile count / offsets [nb+1]
   int64_t max_tiles = 0;
-  int tile = 2048;
+  int tile = 8192;
   bool offsets_valid = false;  // d_offset describes the current order
   bool tiles_valid = false;    // d_tiles describes the current order
-  int dep_nch = 6;             // deposit output chunks per cell (waves per workgroup)
+  int dep_nch = 12;            // deposit output chunks per cell (waves per workgroup)
   int dep_exact = 0;           // 1: unfused mul+add in the deposit (bit-exact order)
   int dep_kernel = 1;          // cell pass: 0 per-lane particle loop, 1 LDS record batches
   // deposit

@@ -4,10 +4,9 @@ the GPU tracer's own cutoff maps, and the bracketed scan that uses it.
 Features per (location, arrival direction): sin/cos latitude, sin/cos
 longitude, zenith / 90 deg and azimuth / 180 deg - 1.  Target: log10 of the
 upper cutoff rigidity (the allowed side of the highest forbidden -> allowed
-transition on the scan grid).  At inference the predicted bin +-3 is scanned
-downward from its top (CutoffForDirectionUpperScan_GV restricted to the
-bracket); the full scan is the fallback wherever the bracket does not
-contain the transition.
+transition on the scan grid).  At inference only the predicted bin +-3 is
+traced, all bins of all brackets in parallel; the full scan is the fallback
+wherever the bracket does not contain the transition.
 """
 import ctypes as C
 import math
@@ -87,20 +86,32 @@ class Surrogate:
         return y
 
 
+def bracket_bins(pb, nR, half=3):
+    """Fixed-width brackets [lo, lo + 2 half] around the predicted bins,
+    shifted inward at the ends of the grid."""
+    w = 2 * half + 1
+    lo = np.clip(np.asarray(pb) - half, 0, nR - w).astype(np.int32)
+    return lo, (lo + w - 1).astype(np.int32)
+
+
 def bracketed_map(ctx, sur, params, x0, dirs, feats, grid, half=3, precision=L.FP32):
     """Cutoff bins with the surrogate bracket and the full-scan fallback.
-    Returns (bins, fallback mask, predicted bins, total steps)."""
+
+    Every bin of every bracket is traced as an independent trajectory (one
+    persistent launch over nloc x (2 half + 1) items); the bins come from the
+    bracket's samples as CutoffForDirectionUpperScan_GV would read them.
+    Points whose bracket shows no transition (the top sample forbidden below
+    Rmax, or every sample allowed above Rmin) are rescanned over the whole
+    grid.  Returns (bins, fallback mask, predicted bins, total steps)."""
     y = sur.predict(feats)
     pb = predicted_bin(y, grid)
-    nR = len(grid)
-    lo = np.clip(pb - half, 0, nR - 1).astype(np.int32)
-    hi = np.clip(pb + half, 0, nR - 1).astype(np.int32)
-    st, bins, steps = CU.cutoff_scan(ctx, params, x0, dirs, grid, mode=CU.SCAN_UPPER, lo=lo, hi=hi,
-                                     precision=precision)
+    lo, hi = bracket_bins(pb, len(grid), half)
+    _, bins, steps = CU.cutoff_scan(ctx, params, x0, dirs, grid, mode=CU.SCAN_FULL, lo=lo, hi=hi,
+                                    precision=precision)
     fb = bins < -1
     if fb.any():
         idx = np.flatnonzero(fb)
-        _, b2, s2 = CU.cutoff_scan(ctx, params, x0[idx], dirs[idx], grid, mode=CU.SCAN_UPPER,
+        _, b2, s2 = CU.cutoff_scan(ctx, params, x0[idx], dirs[idx], grid, mode=CU.SCAN_FULL,
                                    precision=precision)
         bins = bins.copy()
         bins[idx] = b2

@@ -12,6 +12,7 @@ captured in a hipGraph.  Prints one JSON line.
 import argparse
 import json
 import os
+import re
 import sys
 import time
 
@@ -22,6 +23,75 @@ HBM_PEAK_GBPS = 8000.0
 PUSH_BYTES = 104          # per particle-step: x,v read (48) + write (48) + ids (8)
 DEPOSIT_BYTES_P = 48      # per particle: x, v read
 NODE_BYTES = 246 * 8      # per unique node: J (3) + mass matrix (243) written once
+
+
+COPY_PROBE_BYTES = 1 << 30
+# kernel identifiers of the counter passes -> report keys
+PMC_KERNELS = (("k_push_f64", "push"), ("k_push_f32", "push"), ("k_deposit_cell", "deposit_cell"),
+               ("k_deposit_cell_rec", "deposit_cell"), ("k_deposit_node", "deposit_node"),
+               ("k_scatter", "scatter"), ("k_count_keys", "count_keys"), ("k_copy_probe", "copy"))
+
+
+def _pmc_pass(args, tune, counter, outdir):
+    """One rocprofv3 counter pass over one eager step of the same workload
+    (a child process; the counter in KiB per dispatch, averaged per kernel)."""
+    import csv
+    import glob
+    import subprocess
+
+    cmd = ["timeout", "-s", "KILL", "240", "rocprofv3", "--pmc", counter, "--output-format", "csv",
+           "-d", outdir, "-o", "pmc", "--", sys.executable, os.path.abspath(__file__), "--pmc-child",
+           "--workload", "pic", "--particles-log2", str(args.particles_log2), "--tile", str(tune["tile"]),
+           "--deposit-kernel", str(tune["deposit_kernel"]), "--deposit-chunks", str(tune["deposit_chunks"])]
+    env = dict(os.environ, TMPDIR="/tmp")
+    r = subprocess.run(cmd, cwd="/tmp", env=env, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)
+    files = glob.glob(os.path.join(outdir, "**", "*counter_collection.csv"), recursive=True)
+    if not files:
+        raise RuntimeError(f"rocprofv3 {counter} pass: exit {r.returncode}, no counter file; "
+                           + r.stdout.decode(errors="replace")[-400:])
+    sums = {}
+    for f in files:
+        with open(f) as fh:
+            for row in csv.DictReader(fh):
+                if row.get("Counter_Name") != counter:
+                    continue
+                name = row.get("Kernel_Name", "")
+                for ident, key in PMC_KERNELS:
+                    if re.search(r"\b" + ident + r"\b", name):
+                        tot, cnt = sums.get(key, (0.0, 0))
+                        sums[key] = (tot + float(row["Counter_Value"]), cnt + 1)
+                        break
+    return {k: tot / cnt for k, (tot, cnt) in sums.items()}
+
+
+def pmc_traffic(args, tune):
+    """HBM bytes per launch of the step's kernels from FETCH_SIZE and
+    WRITE_SIZE, each scaled by the factor that makes the copy kernel of known
+    size read and write exactly its bytes."""
+    import shutil
+    import tempfile
+
+    out = {}
+    try:
+        per = {}
+        for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+            d = tempfile.mkdtemp(prefix="amps_pmc_", dir="/tmp")
+            try:
+                per[counter] = _pmc_pass(args, tune, counter, d)
+            finally:
+                shutil.rmtree(d, ignore_errors=True)
+        kib = COPY_PROBE_BYTES / 1024.0
+        f_fetch = kib / per["FETCH_SIZE"]["copy"]
+        f_write = kib / per["WRITE_SIZE"]["copy"]
+        out["calibration"] = {"fetch": round(f_fetch, 4), "write": round(f_write, 4)}
+        for key in per["FETCH_SIZE"]:
+            if key == "copy" or key not in per["WRITE_SIZE"]:
+                continue
+            out[key] = int(1024 * (per["FETCH_SIZE"][key] * f_fetch + per["WRITE_SIZE"][key] * f_write))
+            out[key + "_fetch"] = int(1024 * per["FETCH_SIZE"][key] * f_fetch)
+    except Exception as e:  # the line stays valid without the counter passes
+        return {"error": repr(e)[:300]}
+    return out
 
 
 def cpu_baseline(target_s=12.0):
@@ -91,17 +161,19 @@ def cutoff_map_bench(ctx, n_cu, n_train=1 << 20, epochs=20, cpu_check_points=16,
     # FP64 map (bit-exact with the CPU oracle): the reference map of this run
     t0 = time.perf_counter()
     s64, b64, st64 = CU.cutoff_scan(ctx, P, x0, dirs, grid, mode=CU.SCAN_FULL, precision=L.FP64)
-    out["fp64_full_s"] = time.perf_counter() - t0
-    out["fp64_steps"] = int(st64)
-    # plain FP32 maps: every bin, and the downward UPPER_SCAN
+    out["plain_full_fp64_s"] = time.perf_counter() - t0
+    out["plain_full_fp64_steps"] = int(st64)
+    out["plain_full_fp64_steps_per_s"] = st64 / out["plain_full_fp64_s"]
+    # the same FP64 map from start points moved by 1e-12 (relative): how far
+    # the map itself moves under a perturbation of rounding size
+    _, bp, _ = CU.cutoff_scan(ctx, P, x0 * (1.0 + 1e-12), dirs, grid, mode=CU.SCAN_FULL, precision=L.FP64)
+    out["fp64_perturbed_1e-12_bins_off_gt1"] = float(np.mean(np.abs(bp.astype(int) - b64.astype(int)) > 1))
+    # plain FP32 map, every bin
     t0 = time.perf_counter()
     s32, b32, st32 = CU.cutoff_scan(ctx, P, x0, dirs, grid, mode=CU.SCAN_FULL, precision=L.FP32)
-    out["plain_full_s"] = time.perf_counter() - t0
-    out["plain_full_steps"] = int(st32)
-    out["plain_full_steps_per_s"] = st32 / out["plain_full_s"]
-    t0 = time.perf_counter()
-    _, bu32, stu = CU.cutoff_scan(ctx, P, x0, dirs, grid, mode=CU.SCAN_UPPER, precision=L.FP32)
-    out["plain_upper_s"] = time.perf_counter() - t0
+    out["plain_full_fp32_s"] = time.perf_counter() - t0
+    out["plain_full_fp32_steps"] = int(st32)
+    out["plain_full_fp32_steps_per_s"] = st32 / out["plain_full_fp32_s"]
     out["trajectories_full"] = int(nloc * len(grid))
     out["fp32_states_equal_fp64"] = float(np.mean(s32 == s64))
     out["fp32_bins_off_gt1"] = float(np.mean(np.abs(b32.astype(int) - b64.astype(int)) > 1))
@@ -154,16 +226,17 @@ def cutoff_map_bench(ctx, n_cu, n_train=1 << 20, epochs=20, cpu_check_points=16,
     out["train_steps"] = int(nsteps)
     g, tl0 = sur.grad(4096)
     out["train_loss_end"] = float(tl0)
-    # surrogate-bracketed map (predict + bracket scan + full-scan fallback)
+    # surrogate-bracketed maps (predict + bracket scan + full-scan fallback)
     feats = SG.features(flat_lat, flat_lon, flat_zen, flat_az)
-    t0 = time.perf_counter()
-    bs, fb, pb, sst = SG.bracketed_map(ctx, sur, P, x0, dirs, feats, grid, half=3, precision=L.FP32)
-    out["surrogate_s"] = time.perf_counter() - t0
-    out["surrogate_fallback_frac"] = float(np.mean(fb))
-    out["surrogate_bins_off_gt1"] = float(np.mean(np.abs(bs.astype(int) - b64.astype(int)) > 1))
+    for prec, tag, plain in ((L.FP64, "fp64", "plain_full_fp64_s"), (L.FP32, "fp32", "plain_full_fp32_s")):
+        t0 = time.perf_counter()
+        bs, fb, pb, sst = SG.bracketed_map(ctx, sur, P, x0, dirs, feats, grid, half=3, precision=prec)
+        dt_s = time.perf_counter() - t0
+        out[f"surrogate_{tag}"] = {
+            "map_s": dt_s, "steps": int(sst), "speedup_vs_plain_full": out[plain] / dt_s,
+            "fallback_frac": float(np.mean(fb)),
+            "bins_off_gt1_vs_fp64": float(np.mean(np.abs(bs.astype(int) - b64.astype(int)) > 1))}
     out["surrogate_pred_bin_mae"] = float(np.mean(np.abs(pb.astype(int) - np.where(b64 < 0, 47, b64))))
-    out["speedup_vs_plain_full"] = out["plain_full_s"] / out["surrogate_s"]
-    out["speedup_vs_plain_upper"] = out["plain_upper_s"] / out["surrogate_s"]
     return out
 
 
@@ -179,6 +252,11 @@ def main():
     ap.add_argument("--workload", choices=["all", "pic", "map"], default="all")
     ap.add_argument("--train-samples-log2", type=int, default=20)
     ap.add_argument("--epochs", type=int, default=20)
+    ap.add_argument("--tile", type=int, default=0, help="push tile (skips the tuner with the two below)")
+    ap.add_argument("--deposit-kernel", type=int, default=-1)
+    ap.add_argument("--deposit-chunks", type=int, default=0)
+    ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 FETCH_SIZE/WRITE_SIZE passes")
+    ap.add_argument("--pmc-child", action="store_true", help=argparse.SUPPRESS)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); "
@@ -221,7 +299,20 @@ def main():
     n = st.particle_count()
     opts = dict(mover=L.MOVER_LAPENTA, bc_mode=L.BC_REFLECT, r_sphere=0.0)
     tune = None
-    if not args.no_tune:
+    fixed = args.tile > 0 and args.deposit_kernel >= 0 and args.deposit_chunks > 0
+    if fixed:
+        ctx.set_param("tile", args.tile)
+        autotune.set_deposit_variant(ctx, args.deposit_kernel, args.deposit_chunks)
+        tune = {"tile": args.tile, "deposit_kernel": args.deposit_kernel,
+                "deposit_chunks": args.deposit_chunks, "cached": "given"}
+    if args.pmc_child:
+        # one eager step and the copy kernel of known size, for the counter passes
+        st.run_steps(1, dt, use_graph=False, **opts)
+        ctx.copy_probe(COPY_PROBE_BYTES, reps=1)
+        ctx.sync()
+        ctx.close()
+        return
+    if not args.no_tune and not fixed:
         sig = f"{arch}|pic|nb{mesh.nblocks}|n{n}"
         tune, cached = autotune.tune_pic(ctx, st, opts, dt, sig, cache=args.tune_cache)
         tune = {"tile": tune["tile"], "deposit_kernel": tune["deposit_kernel"],
@@ -251,17 +342,29 @@ def main():
     dep_ms = avg["deposit_cell"] + avg["deposit_node"]
     dep_gbps = dep_bytes / (dep_ms * 1e-3) / 1e9
     pd_gbps = (push_bytes + dep_bytes) / ((avg["push"] + dep_ms) * 1e-3) / 1e9
-    copy_bytes = 1 << 30
+    copy_bytes = COPY_PROBE_BYTES
     copy_ms = ctx.copy_probe(copy_bytes, reps=10)
     copy_gbps = 2 * copy_bytes / (copy_ms * 1e-3) / 1e9
+    traffic = None
+    if not args.no_traffic:
+        if tune is None:
+            tune_used = {"tile": 8192, "deposit_kernel": 1, "deposit_chunks": 12}
+        else:
+            tune_used = tune
+        traffic = pmc_traffic(args, tune_used)
     if avg["push"] >= dep_ms:
         roof = {"kernel": "k_push_f64<Lapenta2017>", "bound": "hbm", "achieved": round(push_gbps, 1),
                 "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": round(push_gbps / HBM_PEAK_GBPS, 4),
-                "traffic": None}
+                "traffic": traffic["push"] if traffic and "push" in traffic else None,
+                "algorithmic_bytes": int(push_bytes)}
     else:
-        roof = {"kernel": "ECSIM deposit (k_deposit_cell + k_deposit_node)", "bound": "hbm",
+        tr = None
+        if traffic and "deposit_cell" in traffic and "deposit_node" in traffic:
+            tr = traffic["deposit_cell"] + traffic["deposit_node"]
+        roof = {"kernel": "ECSIM deposit (deposit cell pass + node gather, one launch each)", "bound": "hbm",
                 "achieved": round(dep_gbps, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-                "frac": round(dep_gbps / HBM_PEAK_GBPS, 4), "traffic": None}
+                "frac": round(dep_gbps / HBM_PEAK_GBPS, 4), "traffic": tr,
+                "algorithmic_bytes": int(dep_bytes)}
     cpu = None if args.no_cpu else cpu_baseline()
     line = {
         "metric": "particle-pushes/s (full ECSIM particle step: Lapenta2017 push + migration + J/mass-matrix deposit, FP64)",
@@ -290,6 +393,7 @@ def main():
         "push_deposit_frac_of_copy": round(pd_gbps / copy_gbps, 4),
         "measured_copy_GBps": round(copy_gbps, 1),
         "autotune": tune,
+        "traffic_bytes_per_launch": traffic,
     }
     if args.workload == "all":
         try:

@@ -36,3 +36,13 @@ def test_bf16_rounding():
     assert MO.bf16_round(1.0 + 2 ** -9) == 1.0           # ties to even
     assert MO.bf16_round(1.0 + 3 * 2 ** -9) == 1.0 + 2 ** -7
     assert MO.bf16_round(-3.14159) == -3.140625
+
+
+def test_bracket_bins_fixed_width_inside_grid():
+    from amps_amd import surrogate as SG
+    pb = np.array([0, 1, 2, 3, 10, 44, 45, 46, 47])
+    lo, hi = SG.bracket_bins(pb, 48, half=3)
+    assert np.all(hi - lo == 6) and lo.min() == 0 and hi.max() == 47
+    inner = (pb >= 3) & (pb <= 44)
+    assert np.array_equal(lo[inner], pb[inner] - 3)
+    assert np.all((pb >= lo) & (pb <= hi))

@@ -111,6 +111,44 @@ def test_cutoff_scan_full_vs_upper_and_oracle(gpu_ctx):
     assert np.array_equal(bf, ob)
 
 
+def test_bracketed_full_scan_matches_full_scan(gpu_ctx):
+    """Bracket-restricted FULL scan (the surrogate's scan): the bracket's
+    samples equal the full scan's on the same bins, and the bin is the full
+    scan's bin wherever the bracket holds the transition and nothing above it
+    is forbidden; otherwise the ABOVE/BELOW fallback codes appear."""
+    from amps_amd import surrogate as SG
+    P = CU.config2_params()
+    P.max_steps = 20000
+    pts, lat, lon = CU.config2_points(nlon=8, nlat=8)
+    zen, az = CU.config2_directions(4)
+    dirs = CU.arrival_dirs(pts, zen, az).reshape(-1, 3)
+    x0 = np.repeat(pts, len(zen), axis=0)
+    grid = CU.build_search_grid(0.1, 20.0, 48)
+    sf, bf, _ = CU.cutoff_scan(gpu_ctx, P, x0, dirs, grid, mode=CU.SCAN_FULL)
+    rng = np.random.default_rng(9)
+    guess = np.where(bf < 0, 47, bf) + rng.integers(-4, 5, len(bf))
+    lo, hi = SG.bracket_bins(np.clip(guess, 0, 47), 48)
+    sb, bb, _ = CU.cutoff_scan(gpu_ctx, P, x0, dirs, grid, mode=CU.SCAN_FULL, lo=lo, hi=hi)
+    inb = np.zeros_like(sf, dtype=bool)
+    for i in range(len(lo)):
+        inb[i, lo[i]:hi[i] + 1] = True
+    assert np.array_equal(sb[inb], sf[inb])
+    assert np.all(sb[~inb] == -1)
+    for i in range(len(lo)):
+        s = sf[i]
+        clean_above = np.all(s[hi[i]:] == 1)
+        if bb[i] >= 0:
+            if clean_above:
+                assert bb[i] == bf[i], i
+        elif bb[i] == CU.BIN_ABOVE:
+            assert s[hi[i]] != 1 and hi[i] < 47
+        elif bb[i] == CU.BIN_BELOW:
+            assert np.all(s[lo[i]:hi[i] + 1] == 1) and lo[i] > 0
+        else:
+            assert bb[i] == -1 and hi[i] == 47 and s[47] != 1
+    print(f"bracket results: {np.mean(bb >= 0):.3f} resolved, {np.mean(bb == bf):.3f} equal to the full scan")
+
+
 def test_fp32_tracer_agreement(gpu_ctx):
     """FP32 arithmetic with FP64 position accumulation vs the FP64 oracle:
     per-trajectory classification and per-direction cutoff bin."""
