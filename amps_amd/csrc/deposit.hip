@@ -1,12 +1,13 @@
 // deposit.hip — ECSIM deposit cell pass, record-batched.
 //
 // One workgroup per 4x4x4 cell sub-tile of a block (64 cells) with NCH
-// wavefronts.  The sub-tile's particles are taken in batches of RB particles
+// wavefronts.  The sub-tile's particles are taken in batches of NCH particles
 // per cell.  A batch first turns each particle into a record — its corner
 // weights (InitStencil), alpha matrix (Lapenta Eq. D.3), alpha v, charge and
-// mass-matrix factor — built once per particle by one of the first 64 RB
-// threads and kept in LDS as structure-of-arrays [field][rank][cell], so the reads below
-// are bank-conflict free.  Lane l of wavefront w then accumulates output chunk
+// mass-matrix factor — built once per particle by one thread (whose next
+// particle is loaded while the batch is accumulated) and kept in LDS as
+// structure-of-arrays [field][rank][cell], so the reads below are
+// bank-conflict free.  Lane l of wavefront w then accumulates output chunk
 // w of cell l (J and the 36 lower-triangular corner-pair 3x3 blocks of
 // ProcessCell, pic_field_solver_ecsim.cpp:2241-2342) from its cell's records.
 //
@@ -69,21 +70,25 @@ __device__ __forceinline__ void dispatch_accum(int w, double* acc, const double*
 }
 
 template <int NCH, int W = 0>
-__device__ __forceinline__ void dispatch_store(int w, const double* acc, double* out, double CellVolume) {
+__device__ __forceinline__ void dispatch_stage(int w, int w0, const double* acc, double* stage, int SW,
+                                               int lane, double CellVolume) {
   if constexpr (W < NCH) {
     constexpr int CH = (CELL_VALS + NCH - 1) / NCH;
     constexpr int O0 = W * CH;
     constexpr int O1 = (O0 + CH < CELL_VALS) ? O0 + CH : CELL_VALS;
     if (w == W) {
+      double* dst = stage + lane * SW + (W - w0) * CH;
 #pragma unroll
-      for (int o = O0; o < O1; o++) out[o] = (o < 24) ? acc[o - O0] / CellVolume : acc[o - O0];
+      for (int o = O0; o < O1; o++) dst[o - O0] = (o < 24) ? acc[o - O0] / CellVolume : acc[o - O0];
     } else {
-      dispatch_store<NCH, W + 1>(w, acc, out, CellVolume);
+      dispatch_stage<NCH, W + 1>(w, w0, acc, stage, SW, lane, CellVolume);
     }
   }
 }
 
-template <int NCH, int RB, bool EXACT>
+// RB = NCH: every thread builds one record per batch, so a batch holds NCH
+// particles of each of the 64 cells.
+template <int NCH, bool EXACT>
 __global__ __launch_bounds__(64 * NCH) void k_deposit_cell_rec(MeshView m, Species sp, double dtTotal,
                                                                 const double* __restrict__ B,
                                                                 const int* __restrict__ offset,
@@ -93,10 +98,16 @@ __global__ __launch_bounds__(64 * NCH) void k_deposit_cell_rec(MeshView m, Speci
                                                                 int vf32,
                                                                 const uint8_t* __restrict__ spec,
                                                                 double* __restrict__ cellpart) {
+  constexpr int RB = NCH;
   constexpr int CH = (CELL_VALS + NCH - 1) / NCH;
   constexpr int FS = RB * 64;  // field stride of a record batch
+  constexpr int REC = NF * FS;
+  // output staging reuses the record buffer: WPR waves' chunks of all 64 cells per round
+  constexpr int WPR = (REC / (64 * (CH + 1)) < NCH) ? REC / (64 * (CH + 1)) : NCH;
+  static_assert(WPR >= 1, "record buffer too small to stage one wave's output");
+  constexpr int SW = (WPR * CH) | 1;  // odd stride (in doubles) between the staged cells
   __shared__ double sB[125 * 3];
-  __shared__ double rec[NF * FS];
+  __shared__ double rec[REC];
   __shared__ double sconst[MAX_SPEC][3];  // chargeQ, QdT_over_2m, matrixConst
   __shared__ int maxcnt;
   const int b = blockIdx.x >> 3;
@@ -132,12 +143,27 @@ __global__ __launch_bounds__(64 * NCH) void k_deposit_cell_rec(MeshView m, Speci
   const int akey = keybase + (sx + (lane & 3)) + NC * ((sy + ((lane >> 2) & 3)) + NC * (sz + (lane >> 4)));
   const int acnt = offset[akey + 1] - offset[akey];
   if (w == 0) atomicMax(&maxcnt, acnt);
-  // record-building role: thread -> (cell, rank within the batch); threads
-  // beyond 64 RB only accumulate
-  const bool builder = threadIdx.x < 64 * RB;
+  // record-building role: thread -> (cell, rank within the batch)
   const int bcell = threadIdx.x / RB, brank = threadIdx.x % RB;
   const int bkey = keybase + (sx + (bcell & 3)) + NC * ((sy + ((bcell >> 2) & 3)) + NC * (sz + (bcell >> 4)));
-  const int bp0 = builder ? offset[bkey] : 0, bcnt = builder ? offset[bkey + 1] - bp0 : 0;
+  const int bp0 = offset[bkey], bcnt = offset[bkey + 1] - bp0;
+  // the particle of the next batch, loaded one batch ahead
+  double nx[3] = {0.0, 0.0, 0.0}, nv[3] = {0.0, 0.0, 0.0};
+  int ns = 0;
+  auto fetch = [&](int rank) {
+    if (rank < bcnt) {
+      const int p = bp0 + rank;
+      nx[0] = x[p], nx[1] = x[cap + p], nx[2] = x[2 * cap + p];
+      if (vf32) {
+        const float* vf = (const float*)v;
+        nv[0] = vf[p], nv[1] = vf[cap + p], nv[2] = vf[2 * cap + p];
+      } else {
+        nv[0] = v[p], nv[1] = v[cap + p], nv[2] = v[2 * cap + p];
+      }
+      ns = spec[p];
+    }
+  };
+  fetch(brank);
   __syncthreads();
   const int nbatch = (maxcnt + RB - 1) / RB;
   double acc[CH];
@@ -146,16 +172,9 @@ __global__ __launch_bounds__(64 * NCH) void k_deposit_cell_rec(MeshView m, Speci
   for (int bt = 0; bt < nbatch; bt++) {
     const int rank = bt * RB + brank;
     if (rank < bcnt) {
-      const int p = bp0 + rank;
-      double xI[3] = {x[p], x[cap + p], x[2 * cap + p]};
-      double vI[3];
-      if (vf32) {
-        const float* vf = (const float*)v;
-        vI[0] = vf[p], vI[1] = vf[cap + p], vI[2] = vf[2 * cap + p];
-      } else {
-        vI[0] = v[p], vI[1] = v[cap + p], vI[2] = v[2 * cap + p];
-      }
-      const int s = spec[p];
+      double xI[3] = {nx[0], nx[1], nx[2]};
+      const double vI[3] = {nv[0], nv[1], nv[2]};
+      const int s = ns;
       double wn[8], table[8];
       int iX[3];
       corner_stencil(g, xI, wn, table, iX);
@@ -206,19 +225,34 @@ __global__ __launch_bounds__(64 * NCH) void k_deposit_cell_rec(MeshView m, Speci
       R[F_CQ * FS] = sconst[s][0];
       R[F_MC * FS] = sconst[s][2];
     }
+    fetch(rank + RB);  // in flight while the batch is accumulated
     __syncthreads();
     int nr = acnt - bt * RB;
     nr = nr < 0 ? 0 : (nr > RB ? RB : nr);
     dispatch_accum<NCH, RB, EXACT>(w, acc, rec, lane, nr);
     __syncthreads();
   }
-  dispatch_store<NCH>(w, acc, cellpart + (size_t)akey * CELL_VALS, CellVolume);
+  // write the 64 cell records through LDS, WPR waves' chunks per round, so
+  // that each cell's values leave as contiguous runs
+  for (int w0 = 0; w0 < NCH; w0 += WPR) {
+    const int w1 = (w0 + WPR < NCH) ? w0 + WPR : NCH;
+    const int v0 = w0 * CH, v1 = (w1 * CH < CELL_VALS) ? w1 * CH : CELL_VALS;
+    const int nv1 = v1 - v0;
+    if (w >= w0 && w < w1) dispatch_stage<NCH>(w, w0, acc, rec, SW, lane, CellVolume);
+    __syncthreads();
+    for (int i = threadIdx.x; i < 64 * nv1; i += blockDim.x) {
+      const int c = i / nv1, j = i - c * nv1;
+      const int key = keybase + (sx + (c & 3)) + NC * ((sy + ((c >> 2) & 3)) + NC * (sz + (c >> 4)));
+      cellpart[(size_t)key * CELL_VALS + v0 + j] = rec[c * SW + j];
+    }
+    __syncthreads();
+  }
 }
 
-template <int NCH, int RB, bool EX>
+template <int NCH, bool EX>
 void launch_rec(amps_ctx* c, double dt) {
   const int k = c->cur;
-  hipLaunchKernelGGL((k_deposit_cell_rec<NCH, RB, EX>), dim3((unsigned)c->mv.nb * 8), dim3(64 * NCH), 0,
+  hipLaunchKernelGGL((k_deposit_cell_rec<NCH, EX>), dim3((unsigned)c->mv.nb * 8), dim3(64 * NCH), 0,
                      c->stream, c->mv, c->sp, dt, c->d_B, c->d_offset, c->cap, c->d_x[k], c->d_v[k],
                      c->precision == AMPS_FP32 ? 1 : 0, c->d_spec[k], c->d_cellpart);
 }
@@ -229,9 +263,9 @@ namespace amps {
 int launch_deposit_cell_rec(amps_ctx* c, double dt) {
   const bool ex = c->dep_exact != 0;
   switch (c->dep_nch) {
-    case 4: ex ? launch_rec<4, 4, true>(c, dt) : launch_rec<4, 4, false>(c, dt); break;
-    case 6: ex ? launch_rec<6, 4, true>(c, dt) : launch_rec<6, 4, false>(c, dt); break;
-    case 12: ex ? launch_rec<12, 4, true>(c, dt) : launch_rec<12, 4, false>(c, dt); break;
+    case 4: ex ? launch_rec<4, true>(c, dt) : launch_rec<4, false>(c, dt); break;
+    case 6: ex ? launch_rec<6, true>(c, dt) : launch_rec<6, false>(c, dt); break;
+    case 12: ex ? launch_rec<12, true>(c, dt) : launch_rec<12, false>(c, dt); break;
     default:
       return fail(AMPS_E_ARG, "deposit_kernel 1 runs with deposit_chunks 4, 6 or 12");
   }

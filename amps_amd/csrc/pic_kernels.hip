@@ -396,30 +396,56 @@ __global__ __launch_bounds__(256) void k_deposit_node(int64_t nnodes,
                                                        const int* __restrict__ inc,
                                                        const double* __restrict__ cellpart,
                                                        double* __restrict__ nodeout) {
+  // soff[q][val]: where output value val of a node that is local corner q of
+  // an incident cell sits in that cell's record (-1: the cell does not touch
+  // that neighbour slot)
+  __shared__ short soff[8][256];
+  for (int i = threadIdx.x; i < 8 * 256; i += blockDim.x) {
+    const int q = i >> 8, val = i & 255;
+    int off = -1;
+    if (val < 3) {
+      off = 3 * q + val;
+    } else if (val < NODE_VALS) {
+      const int s = (val - 3) / 9, k = (val - 3) % 9;
+      const int j = c_inv_im[q][s];
+      if (j >= 0) {
+        const int hi = q > j ? q : j, lo = q > j ? j : q;
+        off = 24 + 9 * (hi * (hi + 1) / 2 + lo) + k;
+      }
+    }
+    soff[q][val] = (short)off;
+  }
+  __syncthreads();
   const int64_t slot = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
   if (slot >= nnodes) return;
   const int64_t node = node_order[slot];
   const int lane = threadIdx.x & 63;
   double acc[4] = {0.0, 0.0, 0.0, 0.0};
-  const int e0 = inc_start[node], e1 = inc_start[node + 1];
-  for (int e = e0; e < e1; e++) {
-    const int ic = inc[e];
-    const int q = ic & 7;
-    const double* P = cellpart + (size_t)(ic >> 3) * CELL_VALS;
+  const int e1 = inc_start[node + 1];
+  // up to 8 incident cells at a time: all loads issued first, then the sums
+  // in ascending cell order (x + -0.0 == x, so absent slots change nothing)
+  for (int e = inc_start[node]; e < e1; e += 8) {
+    const int m = (e1 - e < 8) ? e1 - e : 8;
+    double vv[8][4];
 #pragma unroll
-    for (int r = 0; r < 4; r++) {
-      const int val = lane + 64 * r;
-      if (val < 3) {
-        acc[r] += P[3 * q + val];
-      } else if (val < NODE_VALS) {
-        const int s = (val - 3) / 9, k = (val - 3) % 9;
-        const int j = c_inv_im[q][s];
-        if (j >= 0) {
-          const int hi = q > j ? q : j, lo = q > j ? j : q;
-          acc[r] += P[24 + 9 * (hi * (hi + 1) / 2 + lo) + k];
+    for (int t = 0; t < 8; t++) {
+      if (t < m) {
+        const int ic = inc[e + t];
+        const short* so = soff[ic & 7];
+        const double* P = cellpart + (size_t)(ic >> 3) * CELL_VALS;
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+          const int off = so[lane + 64 * r];
+          vv[t][r] = (off >= 0) ? P[off] : -0.0;
         }
       }
     }
+#pragma unroll
+    for (int t = 0; t < 8; t++)
+      if (t < m) {
+#pragma unroll
+        for (int r = 0; r < 4; r++) acc[r] += vv[t][r];
+      }
   }
   double* o = nodeout + (size_t)node * NODE_VALS;
 #pragma unroll
