@@ -75,8 +75,7 @@ struct amps_ctx {
   int64_t nnodes = 0;
   std::vector<int32_t> h_node_of_corner;
   std::vector<int64_t> h_node_keys;
-  int* d_inc_start = nullptr;  // [nnodes+1]
-  int* d_inc = nullptr;        // cell_gid*8 + corner, ascending cell order per node
+  int* d_inc8 = nullptr;  // [nnodes][8] in node_order: cell_gid*8 + corner, ascending cell order, -1 padded
   int* d_node_order = nullptr; // node processing order of the corner flush
   // fields and species
   bool has_fields = false;

@@ -33,7 +33,7 @@ __global__ void k_tile_emit(int, int, const int*, const int*, Tile*, int*);
 template <int NCH, bool EXACT>
 __global__ void k_deposit_cell(MeshView, Species, double, const double*, const int*, int64_t,
                                const double*, const double*, int, const uint8_t*, double*);
-__global__ void k_deposit_node(int64_t, const int*, const int*, const int*, const double*, double*);
+__global__ void k_deposit_node(int64_t, const int*, const int*, const double*, double*);
 __global__ void k_generate(MeshView, GenArgs, int64_t, double*, double*, uint8_t*, int*, int64_t*, int8_t*);
 __global__ void k_copy_probe(const float4*, float4*, int64_t);
 namespace amps {
@@ -94,8 +94,7 @@ static void free_mesh(amps_ctx* c) {
   (void)hipFree(c->d_dxc);
   (void)hipFree(c->d_dxs);
   (void)hipFree(c->d_leaf);
-  (void)hipFree(c->d_inc_start);
-  (void)hipFree(c->d_inc);
+  (void)hipFree(c->d_inc8);
   (void)hipFree(c->d_node_order);
   c->d_node_order = nullptr;
   (void)hipFree(c->d_offset);
@@ -106,7 +105,7 @@ static void free_mesh(amps_ctx* c) {
   (void)hipFree(c->d_E);
   (void)hipFree(c->d_B);
   c->d_xmin = c->d_xmax = c->d_dxc = c->d_dxs = nullptr;
-  c->d_leaf = c->d_inc_start = c->d_inc = nullptr;
+  c->d_leaf = c->d_inc8 = nullptr;
   c->d_offset = c->d_cursor = c->d_tcount = nullptr;
   c->d_cellpart = c->d_nodeout = nullptr;
   c->d_E = c->d_B = nullptr;
@@ -355,21 +354,28 @@ extern "C" int amps_gpu_set_mesh(amps_ctx* c, const amps_mesh_desc* desc, const 
   });
   HIP_TRY(hipMalloc(&c->d_node_order, sizeof(int) * std::max<size_t>(1, node_order.size())));
   HIP_TRY(hipMemcpy(c->d_node_order, node_order.data(), sizeof(int) * node_order.size(), hipMemcpyHostToDevice));
+  // fixed-width incidence rows in processing order: one 32-byte row per node
+  // (a node is a corner of at most one leaf cell per octant around it)
+  std::vector<int> inc8((size_t)c->nnodes * 8, -1);
+  for (int64_t sl = 0; sl < c->nnodes; sl++) {
+    const int node = node_order[(size_t)sl];
+    const int e0 = inc_start[(size_t)node], e1 = inc_start[(size_t)node + 1];
+    if (e1 - e0 > 8) return fail(AMPS_E_ARG, "amps_gpu_set_mesh: a corner node with more than 8 incident cells");
+    for (int e = e0; e < e1; e++) inc8[(size_t)sl * 8 + (e - e0)] = inc[(size_t)e];
+  }
+  HIP_TRY(hipMalloc(&c->d_inc8, sizeof(int) * std::max<size_t>(8, inc8.size())));
+  if (!inc8.empty()) HIP_TRY(hipMemcpy(c->d_inc8, inc8.data(), sizeof(int) * inc8.size(), hipMemcpyHostToDevice));
   // device upload
   HIP_TRY(hipMalloc(&c->d_xmin, sizeof(double) * 3 * nb));
   HIP_TRY(hipMalloc(&c->d_xmax, sizeof(double) * 3 * nb));
   HIP_TRY(hipMalloc(&c->d_dxc, sizeof(double) * 3 * nb));
   HIP_TRY(hipMalloc(&c->d_dxs, sizeof(double) * 3 * nb));
   HIP_TRY(hipMalloc(&c->d_leaf, sizeof(int) * leaf.size()));
-  HIP_TRY(hipMalloc(&c->d_inc_start, sizeof(int) * inc_start.size()));
-  HIP_TRY(hipMalloc(&c->d_inc, sizeof(int) * std::max<size_t>(1, inc.size())));
   HIP_TRY(hipMemcpy(c->d_xmin, xmin.data(), sizeof(double) * 3 * nb, hipMemcpyHostToDevice));
   HIP_TRY(hipMemcpy(c->d_xmax, xmax.data(), sizeof(double) * 3 * nb, hipMemcpyHostToDevice));
   HIP_TRY(hipMemcpy(c->d_dxc, dxc.data(), sizeof(double) * 3 * nb, hipMemcpyHostToDevice));
   HIP_TRY(hipMemcpy(c->d_dxs, dxs.data(), sizeof(double) * 3 * nb, hipMemcpyHostToDevice));
   HIP_TRY(hipMemcpy(c->d_leaf, leaf.data(), sizeof(int) * leaf.size(), hipMemcpyHostToDevice));
-  HIP_TRY(hipMemcpy(c->d_inc_start, inc_start.data(), sizeof(int) * inc_start.size(), hipMemcpyHostToDevice));
-  if (!inc.empty()) HIP_TRY(hipMemcpy(c->d_inc, inc.data(), sizeof(int) * inc.size(), hipMemcpyHostToDevice));
   const int64_t nkeys = (int64_t)nb * NCELL;
   HIP_TRY(hipMalloc(&c->d_offset, sizeof(int) * (nkeys + 1)));
   HIP_TRY(hipMalloc(&c->d_cursor, sizeof(int) * nkeys));
@@ -719,8 +725,10 @@ static int launch_deposit(amps_ctx* c, double dt) {
   HIP_TRY(hipGetLastError());
   tend(c, AMPS_K_DEPOSIT_CELL);
   tbegin(c);
-  hipLaunchKernelGGL(k_deposit_node, dim3((unsigned)((c->nnodes + 3) / 4)), dim3(256), 0, c->stream,
-                     c->nnodes, c->d_node_order, c->d_inc_start, c->d_inc, c->d_cellpart, c->d_nodeout);
+  // a multiple of 8 workgroups (see xcd_tile); the surplus ones exit
+  const int64_t node_wgs = ((c->nnodes + 3) / 4 + 7) / 8 * 8;
+  hipLaunchKernelGGL(k_deposit_node, dim3((unsigned)node_wgs), dim3(256), 0, c->stream,
+                     c->nnodes, c->d_node_order, c->d_inc8, c->d_cellpart, c->d_nodeout);
   HIP_TRY(hipGetLastError());
   tend(c, AMPS_K_DEPOSIT_NODE);
   return 0;

@@ -110,8 +110,9 @@ __global__ __launch_bounds__(64 * NCH) void k_deposit_cell_rec(MeshView m, Speci
   __shared__ double rec[REC];
   __shared__ double sconst[MAX_SPEC][3];  // chargeQ, QdT_over_2m, matrixConst
   __shared__ int maxcnt;
-  const int b = blockIdx.x >> 3;
-  const int st = blockIdx.x & 7;
+  const int tile = xcd_tile();  // the 8 sub-tiles of a block on one XCD
+  const int b = tile >> 3;
+  const int st = tile & 7;
   const int sx = (st & 1) * 4, sy = ((st >> 1) & 1) * 4, sz = (st >> 2) * 4;
   const double* Bb = B + (size_t)b * NCORNER * 3;
   for (int i = threadIdx.x; i < 125 * 3; i += blockDim.x) {

@@ -24,6 +24,15 @@ __device__ __forceinline__ void load_block_geom(const MeshView& m, int b, BlockG
 
 __device__ __forceinline__ int clampi(int a, int lo, int hi) { return a < lo ? lo : (a > hi ? hi : a); }
 
+// Workgroups are dealt to the 8 XCDs round-robin (XCD = blockIdx.x % 8).
+// This maps them so that each XCD works through one contiguous range of
+// logical tiles, whose shared data then stays in that XCD's L2.  gridDim.x
+// must be a multiple of 8.
+constexpr int N_XCD = 8;
+__device__ __forceinline__ int xcd_tile() {
+  return (int)((blockIdx.x % N_XCD) * (gridDim.x / N_XCD) + blockIdx.x / N_XCD);
+}
+
 // CornerBased::InitStencil — pic_interpolation_routines.cpp:1074-1194.
 // wn[l], l = 4i+2j+k in the reference's stencil loop order, normalized as
 // cStencil::Normalize (pic.h:7255-7262); table[] in the reference's

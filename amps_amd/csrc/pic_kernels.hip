@@ -344,8 +344,9 @@ __global__ __launch_bounds__(64 * NCH) void k_deposit_cell(MeshView m, Species s
                                                             const uint8_t* __restrict__ spec,
                                                             double* __restrict__ cellpart) {
   __shared__ double sB[125 * 3];
-  const int b = blockIdx.x >> 3;
-  const int st = blockIdx.x & 7;
+  const int tile = xcd_tile();  // the 8 sub-tiles of a block on one XCD
+  const int b = tile >> 3;
+  const int st = tile & 7;
   const int sx = (st & 1) * 4, sy = ((st >> 1) & 1) * 4, sz = (st >> 2) * 4;
   const double* Bb = B + (size_t)b * NCORNER * 3;
   for (int i = threadIdx.x; i < 125 * 3; i += blockDim.x) {
@@ -392,8 +393,7 @@ __constant__ signed char c_inv_im[8][27];  // slot -> partner corner j, or -1
 
 __global__ __launch_bounds__(256) void k_deposit_node(int64_t nnodes,
                                                        const int* __restrict__ node_order,
-                                                       const int* __restrict__ inc_start,
-                                                       const int* __restrict__ inc,
+                                                       const int* __restrict__ inc8,
                                                        const double* __restrict__ cellpart,
                                                        double* __restrict__ nodeout) {
   // soff[q][val]: where output value val of a node that is local corner q of
@@ -416,37 +416,38 @@ __global__ __launch_bounds__(256) void k_deposit_node(int64_t nnodes,
     soff[q][val] = (short)off;
   }
   __syncthreads();
-  const int64_t slot = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  // consecutive nodes share incident cells: keep them on one XCD
+  const int64_t slot = (int64_t)xcd_tile() * 4 + (threadIdx.x >> 6);
   if (slot >= nnodes) return;
-  const int64_t node = node_order[slot];
   const int lane = threadIdx.x & 63;
-  double acc[4] = {0.0, 0.0, 0.0, 0.0};
-  const int e1 = inc_start[node + 1];
-  // up to 8 incident cells at a time: all loads issued first, then the sums
-  // in ascending cell order (x + -0.0 == x, so absent slots change nothing)
-  for (int e = inc_start[node]; e < e1; e += 8) {
-    const int m = (e1 - e < 8) ? e1 - e : 8;
-    double vv[8][4];
+  // the node's incident cells (one 32-byte row) and its id: independent loads
+  int ics[8];
 #pragma unroll
-    for (int t = 0; t < 8; t++) {
-      if (t < m) {
-        const int ic = inc[e + t];
-        const short* so = soff[ic & 7];
-        const double* P = cellpart + (size_t)(ic >> 3) * CELL_VALS;
+  for (int t = 0; t < 8; t++) ics[t] = inc8[slot * 8 + t];
+  const int64_t node = node_order[slot];
+  // all loads first, then the sums in ascending cell order (x + -0.0 == x,
+  // so absent slots change nothing)
+  double vv[8][4];
 #pragma unroll
-        for (int r = 0; r < 4; r++) {
-          const int off = so[lane + 64 * r];
-          vv[t][r] = (off >= 0) ? P[off] : -0.0;
-        }
+  for (int t = 0; t < 8; t++) {
+    const int ic = ics[t];
+    if (ic >= 0) {
+      const short* so = soff[ic & 7];
+      const double* P = cellpart + (size_t)(ic >> 3) * CELL_VALS;
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int off = so[lane + 64 * r];
+        vv[t][r] = (off >= 0) ? P[off] : -0.0;
       }
     }
-#pragma unroll
-    for (int t = 0; t < 8; t++)
-      if (t < m) {
-#pragma unroll
-        for (int r = 0; r < 4; r++) acc[r] += vv[t][r];
-      }
   }
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int t = 0; t < 8; t++)
+    if (ics[t] >= 0) {
+#pragma unroll
+      for (int r = 0; r < 4; r++) acc[r] += vv[t][r];
+    }
   double* o = nodeout + (size_t)node * NODE_VALS;
 #pragma unroll
   for (int r = 0; r < 4; r++) {

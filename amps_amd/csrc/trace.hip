@@ -320,38 +320,39 @@ struct Fp32Stepper {
     if (!inside) return 0;
     // time step (normalized)
     const float u2 = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
-    const float gam = sqrtf(1.0f + u2);
-    const float igam = 1.0f / gam;
-    const float vmag = sqrtf(u2) * igam;
+    const float igam = rsqrtf(1.0f + u2);
     float b[3];
     bfield32(a, x, b);
     double dt = L.dtTrace;
     const double trem = L.maxTime - L.t;
     if (trem < dt) dt = trem;
     if (P.adaptive) {
-      const float bm = sqrtf(b[0] * b[0] + b[1] * b[1] + b[2] * b[2]);
-      if (bm > 0.0f) {
-        const float omega = bm * igam;
-        dt = dmin(dt, (double)(a.f_dphi / omega));
+      const float bm2 = b[0] * b[0] + b[1] * b[1] + b[2] * b[2];
+      if (bm2 > 0.0f) {
+        // dphi / omega with omega = |B| / gamma, gamma = (1 + u^2) / sqrt(1 + u^2)
+        dt = dmin(dt, (double)(a.f_dphi * ((1.0f + u2) * igam) * rsqrtf(bm2)));
       }
-      if (P.dist_limiter) {
-        const double r = sqrt(r2);
-        double dIn = 1.0e300, dOut = 1.0e300;
-        if (r > a.rin_n) dIn = r - a.rin_n;
-        if (P.outer_sphere) {
-          if (r < a.rout_n) dOut = a.rout_n - r;
-        } else if (inside) {
-          dOut = dmin(dmin(dmin(P.box_max_Re[0] - x[0], x[0] - P.box_min_Re[0]),
-                           dmin(P.box_max_Re[1] - x[1], x[1] - P.box_min_Re[1])),
-                      dmin(P.box_max_Re[2] - x[2], x[2] - P.box_min_Re[2]));
+      if (P.dist_limiter || P.dt_floor) {
+        const float vmag = sqrtf(u2) * igam;
+        if (P.dist_limiter) {
+          const double r = sqrt(r2);
+          double dIn = 1.0e300, dOut = 1.0e300;
+          if (r > a.rin_n) dIn = r - a.rin_n;
+          if (P.outer_sphere) {
+            if (r < a.rout_n) dOut = a.rout_n - r;
+          } else if (inside) {
+            dOut = dmin(dmin(dmin(P.box_max_Re[0] - x[0], x[0] - P.box_min_Re[0]),
+                             dmin(P.box_max_Re[1] - x[1], x[1] - P.box_min_Re[1])),
+                        dmin(P.box_max_Re[2] - x[2], x[2] - P.box_min_Re[2]));
+          }
+          const double dn = dmin(dIn, dOut);
+          if (vmag > 0.0f && dn < 1.0e299) dt = dmin(dt, 0.20 * dn / (double)vmag);
         }
-        const double dn = dmin(dIn, dOut);
-        if (vmag > 0.0f && dn < 1.0e299) dt = dmin(dt, 0.20 * dn / (double)vmag);
-      }
-      if (P.dt_floor) {
-        const double sv = dmax((double)vmag, 1.0e-12 / C_LIGHT);
-        const double fl = dmax(dmax(a.floor_abs_n, a.floor_rel_n), a.floor_dist_n / sv);
-        dt = dmax(fl, dt);
+        if (P.dt_floor) {
+          const double sv = dmax((double)vmag, 1.0e-12 / C_LIGHT);
+          const double fl = dmax(dmax(a.floor_abs_n, a.floor_rel_n), a.floor_dist_n / sv);
+          dt = dmax(fl, dt);
+        }
       }
       if (trem > 0.0) dt = dmin(dt, trem);
     }
@@ -365,7 +366,7 @@ struct Fp32Stepper {
     const float tc = dtf * 0.5f * igam;
     const float t[3] = {tc * b[0], tc * b[1], tc * b[2]};
     const float t2 = t[0] * t[0] + t[1] * t[1] + t[2] * t[2];
-    const float sc = 2.0f / (1.0f + t2);
+    const float sc = 2.0f * __builtin_amdgcn_rcpf(1.0f + t2);
     const float s[3] = {sc * t[0], sc * t[1], sc * t[2]};
     const float pp[3] = {u[0] + (u[1] * t[2] - u[2] * t[1]), u[1] + (u[2] * t[0] - u[0] * t[2]),
                          u[2] + (u[0] * t[1] - u[1] * t[0])};
@@ -374,14 +375,17 @@ struct Fp32Stepper {
     u[0] = un[0];
     u[1] = un[1];
     u[2] = un[2];
-    const float igam2 = 1.0f / sqrtf(1.0f + u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+    const float igam2 = rsqrtf(1.0f + u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
 #pragma unroll
     for (int d = 0; d < 3; d++) x[d] += (double)(hf * (u[d] * igam2));
-    if (seg_hits(xb, x, a.rin_n) || inside_inner_m(x, a.rin_n)) return 1;
+    const float dd[3] = {(float)(x[0] - xb[0]), (float)(x[1] - xb[1]), (float)(x[2] - xb[2])};
+    const float seg = sqrtf(dd[0] * dd[0] + dd[1] * dd[1] + dd[2] * dd[2]);
+    // the chord can reach the inner sphere only if it starts within its
+    // length of the sphere; the exact FP64 test runs only then
+    if (a.rin_n > 0.0 && sqrtf((float)r2) - seg <= (float)a.rin_n * 1.0001f)
+      if (seg_hits(xb, x, a.rin_n) || inside_inner_m(x, a.rin_n)) return 1;
     if (!fin3(x) || !isfinite(u[0]) || !isfinite(u[1]) || !isfinite(u[2])) return 8;
-    const double dd[3] = {x[0] - xb[0], x[1] - xb[1], x[2] - xb[2]};
-    const double seg = sqrt(dot3(dd, dd));
-    L.s += isfinite(seg) ? seg : 0.0;
+    L.s += isfinite(seg) ? (double)seg : 0.0;
     L.t += dt;
     ++L.n;
     return -1;
@@ -424,6 +428,7 @@ __global__ __launch_bounds__(256) void k_trace_persistent(TraceView a) {
   bool has = false, done = false;
   int64_t my_steps = 0;
   const int lane = __lane_id();
+  const int64_t nloc_full = (a.mode == TRACE_MODE_FULL) ? a.nitems / a.nbin_item : 1;
   for (;;) {
     const bool need = !has && !done;
     const uint64_t mask = __ballot(need);
@@ -442,8 +447,11 @@ __global__ __launch_bounds__(256) void k_trace_persistent(TraceView a) {
             loc = (int)item;
             S::init(a, L, a.x0 + 3 * item, a.dir + 3 * item, a.R[item], 0);
           } else if (a.mode == TRACE_MODE_FULL) {
-            loc = (int)(item / a.nbin_item);
-            bin = (a.lo ? a.lo[loc] : 0) + (int)(item % a.nbin_item);
+            // bin-major: every location's lowest rigidities first, the
+            // longest trajectories (quasi-trapped, up to the step cap)
+            // therefore start first
+            loc = (int)(item % nloc_full);
+            bin = (a.lo ? a.lo[loc] : 0) + (int)(item / nloc_full);
             S::init(a, L, a.x0 + 3 * (int64_t)loc, a.dir + 3 * (int64_t)loc, a.grid[bin], 0);
           } else {
             loc = (int)item;
