@@ -15,6 +15,9 @@ constexpr int NN = 9;                  // corner nodes per block edge
 constexpr int NCELL = NC * NC * NC;    // 512
 constexpr int NCORNER = NN * NN * NN;  // 729
 constexpr int CELL_VALS = 24 + 36 * 9; // per-cell J (8x3) + lower-triangular 8x8 pairs x 9
+// per-cell J (8x3) + 27 weight-product moments x 9: the pair block of corners
+// (i, j) depends only on the per-axis sums of their offset bits
+constexpr int CELL_VALS_MOMENT = 24 + 27 * 9;
 constexpr int NODE_VALS = 3 + 243;     // per-node J + mass-matrix row block
 constexpr int MAX_SPEC = 8;
 

@@ -33,6 +33,7 @@ __global__ void k_tile_emit(int, int, const int*, const int*, Tile*, int*);
 template <int NCH, bool EXACT>
 __global__ void k_deposit_cell(MeshView, Species, double, const double*, const int*, int64_t,
                                const double*, const double*, int, const uint8_t*, double*);
+template <bool MOMENT>
 __global__ void k_deposit_node(int64_t, const int*, const int*, const double*, double*);
 __global__ void k_generate(MeshView, GenArgs, int64_t, double*, double*, uint8_t*, int*, int64_t*, int8_t*);
 __global__ void k_copy_probe(const float4*, float4*, int64_t);
@@ -727,8 +728,13 @@ static int launch_deposit(amps_ctx* c, double dt) {
   tbegin(c);
   // a multiple of 8 workgroups (see xcd_tile); the surplus ones exit
   const int64_t node_wgs = ((c->nnodes + 3) / 4 + 7) / 8 * 8;
-  hipLaunchKernelGGL(k_deposit_node, dim3((unsigned)node_wgs), dim3(256), 0, c->stream,
-                     c->nnodes, c->d_node_order, c->d_inc8, c->d_cellpart, c->d_nodeout);
+  // the record-batched cell pass writes the moment form unless bit-exact
+  if (c->dep_kernel == 1 && !c->dep_exact)
+    hipLaunchKernelGGL(k_deposit_node<true>, dim3((unsigned)node_wgs), dim3(256), 0, c->stream,
+                       c->nnodes, c->d_node_order, c->d_inc8, c->d_cellpart, c->d_nodeout);
+  else
+    hipLaunchKernelGGL(k_deposit_node<false>, dim3((unsigned)node_wgs), dim3(256), 0, c->stream,
+                       c->nnodes, c->d_node_order, c->d_inc8, c->d_cellpart, c->d_nodeout);
   HIP_TRY(hipGetLastError());
   tend(c, AMPS_K_DEPOSIT_NODE);
   return 0;

@@ -11,10 +11,12 @@
 // w of cell l (J and the 36 lower-triangular corner-pair 3x3 blocks of
 // ProcessCell, pic_field_solver_ecsim.cpp:2241-2342) from its cell's records.
 //
-// Every record value is computed by the same expressions, in the same order,
-// as the reference's ProcessCell, and each cell's particles are summed in
-// list order, so with EXACT (separate multiply and add) the cell partials are
-// bit-identical to a serial evaluation; without EXACT the sums use FMA.
+// With EXACT every record value is computed by the same expressions, in the
+// same order, as the reference's ProcessCell (separate multiply and add), and
+// each cell's particles are summed in list order, so the cell partials are
+// bit-identical to a serial evaluation.  Without EXACT the stencil uses the
+// block's reciprocal cell size and one reciprocal of the weight sum, and the
+// sums use FMA (results within a few ulp per term).
 #include <hip/hip_runtime.h>
 
 #include "amps_common.h"
@@ -24,32 +26,66 @@ using namespace amps;
 
 namespace {
 
-constexpr int NF = 22;  // record fields: table[8] alpha[9] chargeQ matrixConst vRot[3]
-constexpr int F_TAB = 0, F_ALPHA = 8, F_CQ = 17, F_MC = 18, F_VROT = 19;
+// Record fields.  EXACT: table[8] alpha[9] chargeQ matrixConst vRot[3].
+// Moment form: the linear factors L[2d + bit] (1 - f_d, f_d) of the three axes
+// instead of the 8 corner weights.
+template <bool EXACT>
+struct Rec {
+  static constexpr int NF = EXACT ? 22 : 20;
+  static constexpr int TAB = 0, LIN = 0;
+  static constexpr int ALPHA = EXACT ? 8 : 6;
+  static constexpr int CQ = ALPHA + 9, MC = ALPHA + 10, VROT = ALPHA + 11;
+  static constexpr int VALS = EXACT ? CELL_VALS : CELL_VALS_MOMENT;
+};
+
+// corner c (InterpolationCoefficientTable order) -> its x, y, z offset bits
+constexpr int cbit(int c, int d) {
+  return d == 0 ? (c == 1 || c == 2 || c == 5 || c == 6) : d == 1 ? (c == 2 || c == 3 || c == 6 || c == 7) : (c >= 4);
+}
 
 template <int O0, int O1, int RB, bool EXACT>
 __device__ __forceinline__ void accum_records(double* acc, const double* rec, int lane, int nr) {
+  using R_ = Rec<EXACT>;
   for (int r = 0; r < nr; r++) {
     const double* R = rec + r * 64 + lane;
 #define FLD(f) R[(f) * RB * 64]
-    double tab[8];
+    const double chargeQ = FLD(R_::CQ), matrixConst = FLD(R_::MC);
+    if constexpr (EXACT) {
+      double tab[8];
 #pragma unroll
-    for (int i = 0; i < 8; i++) tab[i] = FLD(F_TAB + i);
-    const double chargeQ = FLD(F_CQ), matrixConst = FLD(F_MC);
-    double twc[8];
+      for (int i = 0; i < 8; i++) tab[i] = FLD(R_::TAB + i);
+      double twc[8];
 #pragma unroll
-    for (int i = 0; i < 8; i++) twc[i] = matrixConst * tab[i];
+      for (int i = 0; i < 8; i++) twc[i] = matrixConst * tab[i];
 #pragma unroll
-    for (int o = O0; o < O1; o++) {
-      if (o < 24) {
-        const double t = chargeQ * tab[o / 3];
-        if (EXACT) acc[o - O0] += t * FLD(F_VROT + o % 3);
-        else acc[o - O0] = fma(t, FLD(F_VROT + o % 3), acc[o - O0]);
-      } else {
-        const int pr = (o - 24) / 9, k = (o - 24) % 9;
-        const double twp = tab[pair_j(pr)] * twc[pair_i(pr)];
-        if (EXACT) acc[o - O0] += FLD(F_ALPHA + k) * twp;
-        else acc[o - O0] = fma(FLD(F_ALPHA + k), twp, acc[o - O0]);
+      for (int o = O0; o < O1; o++) {
+        if (o < 24) {
+          const double t = chargeQ * tab[o / 3];
+          acc[o - O0] += t * FLD(R_::VROT + o % 3);
+        } else {
+          const int pr = (o - 24) / 9, k = (o - 24) % 9;
+          const double twp = tab[pair_j(pr)] * twc[pair_i(pr)];
+          acc[o - O0] += FLD(R_::ALPHA + k) * twp;
+        }
+      }
+    } else {
+      double Lf[6];
+#pragma unroll
+      for (int i = 0; i < 6; i++) Lf[i] = FLD(R_::LIN + i);
+#pragma unroll
+      for (int o = O0; o < O1; o++) {
+        if (o < 24) {
+          const int c = o / 3;
+          const double w = Lf[cbit(c, 0)] * Lf[2 + cbit(c, 1)] * Lf[4 + cbit(c, 2)];
+          acc[o - O0] = fma(chargeQ * w, FLD(R_::VROT + o % 3), acc[o - O0]);
+        } else {
+          const int t = (o - 24) / 9, k = (o - 24) % 9;
+          const int s0 = t % 3, s1 = (t / 3) % 3, s2 = t / 9;
+          const double g0 = Lf[s0 == 2 ? 1 : 0] * Lf[s0 == 0 ? 0 : 1];
+          const double g1 = Lf[2 + (s1 == 2 ? 1 : 0)] * Lf[2 + (s1 == 0 ? 0 : 1)];
+          const double g2 = Lf[4 + (s2 == 2 ? 1 : 0)] * Lf[4 + (s2 == 0 ? 0 : 1)];
+          acc[o - O0] = fma(FLD(R_::ALPHA + k), ((matrixConst * g0) * g1) * g2, acc[o - O0]);
+        }
       }
     }
 #undef FLD
@@ -59,9 +95,10 @@ __device__ __forceinline__ void accum_records(double* acc, const double* rec, in
 template <int NCH, int RB, bool EXACT, int W = 0>
 __device__ __forceinline__ void dispatch_accum(int w, double* acc, const double* rec, int lane, int nr) {
   if constexpr (W < NCH) {
-    constexpr int CH = (CELL_VALS + NCH - 1) / NCH;
+    constexpr int V = Rec<EXACT>::VALS;
+    constexpr int CH = (V + NCH - 1) / NCH;
     constexpr int O0 = W * CH;
-    constexpr int O1 = (O0 + CH < CELL_VALS) ? O0 + CH : CELL_VALS;
+    constexpr int O1 = (O0 + CH < V) ? O0 + CH : V;
     if (w == W)
       accum_records<O0, O1, RB, EXACT>(acc, rec, lane, nr);
     else
@@ -69,19 +106,20 @@ __device__ __forceinline__ void dispatch_accum(int w, double* acc, const double*
   }
 }
 
-template <int NCH, int W = 0>
+template <int NCH, bool EXACT, int W = 0>
 __device__ __forceinline__ void dispatch_stage(int w, int w0, const double* acc, double* stage, int SW,
                                                int lane, double CellVolume) {
   if constexpr (W < NCH) {
-    constexpr int CH = (CELL_VALS + NCH - 1) / NCH;
+    constexpr int V = Rec<EXACT>::VALS;
+    constexpr int CH = (V + NCH - 1) / NCH;
     constexpr int O0 = W * CH;
-    constexpr int O1 = (O0 + CH < CELL_VALS) ? O0 + CH : CELL_VALS;
+    constexpr int O1 = (O0 + CH < V) ? O0 + CH : V;
     if (w == W) {
       double* dst = stage + lane * SW + (W - w0) * CH;
 #pragma unroll
       for (int o = O0; o < O1; o++) dst[o - O0] = (o < 24) ? acc[o - O0] / CellVolume : acc[o - O0];
     } else {
-      dispatch_stage<NCH, W + 1>(w, w0, acc, stage, SW, lane, CellVolume);
+      dispatch_stage<NCH, EXACT, W + 1>(w, w0, acc, stage, SW, lane, CellVolume);
     }
   }
 }
@@ -98,10 +136,12 @@ __global__ __launch_bounds__(64 * NCH) void k_deposit_cell_rec(MeshView m, Speci
                                                                 int vf32,
                                                                 const uint8_t* __restrict__ spec,
                                                                 double* __restrict__ cellpart) {
+  using R_ = Rec<EXACT>;
   constexpr int RB = NCH;
-  constexpr int CH = (CELL_VALS + NCH - 1) / NCH;
+  constexpr int V = R_::VALS;  // values per cell record
+  constexpr int CH = (V + NCH - 1) / NCH;
   constexpr int FS = RB * 64;  // field stride of a record batch
-  constexpr int REC = NF * FS;
+  constexpr int REC = R_::NF * FS;
   // output staging reuses the record buffer: WPR waves' chunks of all 64 cells per round
   constexpr int WPR = (REC / (64 * (CH + 1)) < NCH) ? REC / (64 * (CH + 1)) : NCH;
   static_assert(WPR >= 1, "record buffer too small to stage one wave's output");
@@ -124,6 +164,9 @@ __global__ __launch_bounds__(64 * NCH) void k_deposit_cell_rec(MeshView m, Speci
   load_block_geom(m, b, g);
   double CellVolume = 1;
   for (int d = 0; d < 3; d++) CellVolume *= (g.xmax[d] - g.xmin[d]) / NC * 1.0;
+  double inv_dxs[3];
+#pragma unroll
+  for (int d = 0; d < 3; d++) inv_dxs[d] = EXACT ? 0.0 : 1.0 / g.dxs[d];
   if (threadIdx.x < sp.n) {
     // per-species factors of ProcessCell, same operations in the same order
     const int s = threadIdx.x;
@@ -176,12 +219,36 @@ __global__ __launch_bounds__(64 * NCH) void k_deposit_cell_rec(MeshView m, Speci
       double xI[3] = {nx[0], nx[1], nx[2]};
       const double vI[3] = {nv[0], nv[1], nv[2]};
       const int s = ns;
-      double wn[8], table[8];
+      double wn[8], table[8], Lf[6];
       int iX[3];
-      corner_stencil(g, xI, wn, table, iX);
-      iX[0] = clampi(iX[0], sx, sx + 3);
-      iX[1] = clampi(iX[1], sy, sy + 3);
-      iX[2] = clampi(iX[2], sz, sz + 3);
+      if constexpr (EXACT) {
+        corner_stencil(g, xI, wn, table, iX);
+        iX[0] = clampi(iX[0], sx, sx + 3);
+        iX[1] = clampi(iX[1], sy, sy + 3);
+        iX[2] = clampi(iX[2], sz, sz + 3);
+      } else {
+        // the same weights with the block's reciprocal cell size, relative
+        // to the particle's own cell (a few ulp from the divided form)
+        iX[0] = sx + (bcell & 3), iX[1] = sy + ((bcell >> 2) & 3), iX[2] = sz + (bcell >> 4);
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+          double xd = xI[d];
+          if (fabs(xd - g.xmax[d]) < 1e-10 * g.dxs[d]) xd = g.xmax[d] - 1e-10 * g.dxs[d];
+          const double t = (xd - g.xmin[d]) * inv_dxs[d] - iX[d];
+          const double f = t < 0.0 ? 0.0 : (t > 1.0 ? 1.0 : t);
+          Lf[2 * d] = 1.0 - f;
+          Lf[2 * d + 1] = f;
+        }
+        double norm = 0.0;
+#pragma unroll
+        for (int l = 0; l < 8; l++) {
+          wn[l] = Lf[l >> 2] * Lf[2 + ((l >> 1) & 1)] * Lf[4 + (l & 1)];
+          norm += wn[l];
+        }
+        const double rn = 1.0 / norm;
+#pragma unroll
+        for (int l = 0; l < 8; l++) wn[l] *= rn;
+      }
       double Bv[3] = {0.0, 0.0, 0.0};
 #pragma unroll
       for (int l = 0; l < 8; l++) {
@@ -217,14 +284,19 @@ __global__ __launch_bounds__(64 * NCH) void k_deposit_cell_rec(MeshView m, Speci
         double vr = 0.0;
 #pragma unroll
         for (int jj = 0; jj < 3; jj++) vr += alpha[3 * i + jj] * vI[jj];
-        R[(F_VROT + i) * FS] = vr;
+        R[(R_::VROT + i) * FS] = vr;
+      }
+      if constexpr (EXACT) {
+#pragma unroll
+        for (int i = 0; i < 8; i++) R[(R_::TAB + i) * FS] = table[i];
+      } else {
+#pragma unroll
+        for (int i = 0; i < 6; i++) R[(R_::LIN + i) * FS] = Lf[i];
       }
 #pragma unroll
-      for (int i = 0; i < 8; i++) R[(F_TAB + i) * FS] = table[i];
-#pragma unroll
-      for (int k = 0; k < 9; k++) R[(F_ALPHA + k) * FS] = alpha[k];
-      R[F_CQ * FS] = sconst[s][0];
-      R[F_MC * FS] = sconst[s][2];
+      for (int k = 0; k < 9; k++) R[(R_::ALPHA + k) * FS] = alpha[k];
+      R[R_::CQ * FS] = sconst[s][0];
+      R[R_::MC * FS] = sconst[s][2];
     }
     fetch(rank + RB);  // in flight while the batch is accumulated
     __syncthreads();
@@ -237,14 +309,14 @@ __global__ __launch_bounds__(64 * NCH) void k_deposit_cell_rec(MeshView m, Speci
   // that each cell's values leave as contiguous runs
   for (int w0 = 0; w0 < NCH; w0 += WPR) {
     const int w1 = (w0 + WPR < NCH) ? w0 + WPR : NCH;
-    const int v0 = w0 * CH, v1 = (w1 * CH < CELL_VALS) ? w1 * CH : CELL_VALS;
+    const int v0 = w0 * CH, v1 = (w1 * CH < V) ? w1 * CH : V;
     const int nv1 = v1 - v0;
-    if (w >= w0 && w < w1) dispatch_stage<NCH>(w, w0, acc, rec, SW, lane, CellVolume);
+    if (w >= w0 && w < w1) dispatch_stage<NCH, EXACT>(w, w0, acc, rec, SW, lane, CellVolume);
     __syncthreads();
     for (int i = threadIdx.x; i < 64 * nv1; i += blockDim.x) {
       const int c = i / nv1, j = i - c * nv1;
       const int key = keybase + (sx + (c & 3)) + NC * ((sy + ((c >> 2) & 3)) + NC * (sz + (c >> 4)));
-      cellpart[(size_t)key * CELL_VALS + v0 + j] = rec[c * SW + j];
+      cellpart[(size_t)key * V + v0 + j] = rec[c * SW + j];
     }
     __syncthreads();
   }

@@ -389,32 +389,27 @@ INST_DEP(6, false)
 // order of the reference's serial cell loop — into J (3) and the 27x9 mass
 // matrix row block (IndexMatrix, pic_field_solver_ecsim.cpp:1377-1379).
 // ======================================================================
-__constant__ signed char c_inv_im[8][27];  // slot -> partner corner j, or -1
+// c_soff[q][val]: where output value val (J[3], then 27 slots x 9) of a node
+// that is local corner q of an incident cell sits in that cell's record
+// (J of corner q at 3q; the pair block (max(q,j), min(q,j)) at
+// 24 + 9 (hi (hi+1)/2 + lo)), -1 where the cell does not touch that
+// neighbour slot.  Built on the host from IndexMatrix.
+// c_soff_m: the same for the moment record (pair (q, j) -> moment of the
+// per-axis bit sums of q and j, at 24 + 9 t).
+__constant__ __align__(16) short c_soff[8][256];
+__constant__ __align__(16) short c_soff_m[8][256];
 
+template <bool MOMENT>
 __global__ __launch_bounds__(256) void k_deposit_node(int64_t nnodes,
                                                        const int* __restrict__ node_order,
                                                        const int* __restrict__ inc8,
                                                        const double* __restrict__ cellpart,
                                                        double* __restrict__ nodeout) {
-  // soff[q][val]: where output value val of a node that is local corner q of
-  // an incident cell sits in that cell's record (-1: the cell does not touch
-  // that neighbour slot)
-  __shared__ short soff[8][256];
-  for (int i = threadIdx.x; i < 8 * 256; i += blockDim.x) {
-    const int q = i >> 8, val = i & 255;
-    int off = -1;
-    if (val < 3) {
-      off = 3 * q + val;
-    } else if (val < NODE_VALS) {
-      const int s = (val - 3) / 9, k = (val - 3) % 9;
-      const int j = c_inv_im[q][s];
-      if (j >= 0) {
-        const int hi = q > j ? q : j, lo = q > j ? j : q;
-        off = 24 + 9 * (hi * (hi + 1) / 2 + lo) + k;
-      }
-    }
-    soff[q][val] = (short)off;
-  }
+  constexpr int STRIDE = MOMENT ? CELL_VALS_MOMENT : CELL_VALS;
+  __shared__ __align__(16) short soff[8][256];
+  static_assert(sizeof(soff) == 256 * 16, "one 16-byte row per thread");
+  reinterpret_cast<int4*>(&soff[0][0])[threadIdx.x] =
+      reinterpret_cast<const int4*>(MOMENT ? &c_soff_m[0][0] : &c_soff[0][0])[threadIdx.x];
   __syncthreads();
   // consecutive nodes share incident cells: keep them on one XCD
   const int64_t slot = (int64_t)xcd_tile() * 4 + (threadIdx.x >> 6);
@@ -433,7 +428,7 @@ __global__ __launch_bounds__(256) void k_deposit_node(int64_t nnodes,
     const int ic = ics[t];
     if (ic >= 0) {
       const short* so = soff[ic & 7];
-      const double* P = cellpart + (size_t)(ic >> 3) * CELL_VALS;
+      const double* P = cellpart + (size_t)(ic >> 3) * STRIDE;
 #pragma unroll
       for (int r = 0; r < 4; r++) {
         const int off = so[lane + 64 * r];
@@ -456,19 +451,43 @@ __global__ __launch_bounds__(256) void k_deposit_node(int64_t nnodes,
   }
 }
 
+template __global__ void k_deposit_node<false>(int64_t, const int*, const int*, const double*, double*);
+template __global__ void k_deposit_node<true>(int64_t, const int*, const int*, const double*, double*);
+
 namespace amps {
 int upload_inv_index_matrix(hipStream_t s) {
+  // IndexMatrix, pic_field_solver_ecsim.cpp:1377-1379
   static const int IM[8][8] = {{0, 2, 8, 6, 18, 20, 26, 24}, {1, 0, 6, 7, 19, 18, 24, 25},
                                {4, 3, 0, 1, 22, 21, 18, 19}, {3, 5, 2, 0, 21, 23, 20, 18},
                                {9, 11, 17, 15, 0, 2, 8, 6},  {10, 9, 15, 16, 1, 0, 6, 7},
                                {13, 12, 9, 10, 4, 3, 0, 1},  {12, 14, 11, 9, 3, 5, 2, 0}};
-  signed char inv[8][27];
+  // corner offset bits (x, y, z) in InterpolationCoefficientTable order
+  static const int CB[8][3] = {{0, 0, 0}, {1, 0, 0}, {1, 1, 0}, {0, 1, 0},
+                               {0, 0, 1}, {1, 0, 1}, {1, 1, 1}, {0, 1, 1}};
+  short soff[8][256], soffm[8][256];
   for (int q = 0; q < 8; q++) {
-    for (int s2 = 0; s2 < 27; s2++) inv[q][s2] = -1;
-    for (int j = 0; j < 8; j++) inv[q][IM[q][j]] = (signed char)j;
+    int inv[27];
+    for (int s2 = 0; s2 < 27; s2++) inv[s2] = -1;
+    for (int j = 0; j < 8; j++) inv[IM[q][j]] = j;
+    for (int val = 0; val < 256; val++) {
+      int off = -1, offm = -1;
+      if (val < 3) {
+        off = offm = 3 * q + val;
+      } else if (val < NODE_VALS) {
+        const int s2 = (val - 3) / 9, k = (val - 3) % 9, j = inv[s2];
+        if (j >= 0) {
+          const int hi = q > j ? q : j, lo = q > j ? j : q;
+          off = 24 + 9 * (hi * (hi + 1) / 2 + lo) + k;
+          const int t = (CB[q][0] + CB[j][0]) + 3 * (CB[q][1] + CB[j][1]) + 9 * (CB[q][2] + CB[j][2]);
+          offm = 24 + 9 * t + k;
+        }
+      }
+      soff[q][val] = (short)off;
+      soffm[q][val] = (short)offm;
+    }
   }
-  HIP_TRY(hipMemcpyToSymbolAsync(HIP_SYMBOL(c_inv_im), inv, sizeof(inv), 0,
-                                 hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyToSymbolAsync(HIP_SYMBOL(c_soff), soff, sizeof(soff), 0, hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyToSymbolAsync(HIP_SYMBOL(c_soff_m), soffm, sizeof(soffm), 0, hipMemcpyHostToDevice, s));
   return 0;
 }
 }  // namespace amps
