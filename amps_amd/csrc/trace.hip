@@ -17,6 +17,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <vector>
 
@@ -681,6 +682,84 @@ extern "C" int amps_gpu_trace_batch(amps_ctx* c, const amps_trace_params* P, int
   HIP_TRY(hipMemcpyAsync(&st, t->d_steps, 8, hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
   if (steps_total) *steps_total = (int64_t)st;
+  return AMPS_OK;
+}
+
+// RefineCutoffTransition (CutoffBandSearch.h:212-243) for many brackets at
+// once: every bisection round traces the midpoints of all still-open brackets
+// in one persistent launch.  Midpoint, tolerance and update are the
+// reference's: tol = max(abs_tol, rel_tol max(|lo|, |hi|)), mid = 0.5 (lo + hi),
+// Allowed -> hi = mid, PhysicalForbidden -> lo = mid; a trace-limit
+// termination is forbidden when trace_limit_forbidden is set (the
+// "FORBIDDEN" traceLimitPolicy, CutoffRigidityGridless.cpp:2685-2694) and
+// otherwise ends that bracket as unresolved; a numerical failure after the
+// retry ends it with status 2 (the reference throws).
+extern "C" int amps_gpu_refine_cutoff(amps_ctx* c, const amps_trace_params* P, int32_t precision,
+                                      int64_t n, const double* x0, const double* dir,
+                                      const double* R_forbid, const double* R_allow, double abs_tol,
+                                      double rel_tol, int32_t trace_limit_forbidden, double* cutoff,
+                                      int32_t* iterations, int32_t* status, int64_t* steps_total) {
+  if (!c) return fail(AMPS_E_ARG, "null context");
+  if (n < 0 || n > ((int64_t)1 << 31)) return fail(AMPS_E_RANGE, "refine_cutoff: n out of range");
+  if (n == 0) return AMPS_OK;
+  if (!x0 || !dir || !R_forbid || !R_allow || !cutoff || !iterations || !status)
+    return fail(AMPS_E_ARG, "refine_cutoff: null array");
+  std::vector<double> lo(R_forbid, R_forbid + n), hi(R_allow, R_allow + n);
+  std::vector<int64_t> act;
+  for (int64_t i = 0; i < n; i++) {
+    iterations[i] = 0;
+    status[i] = 0;
+    cutoff[i] = -1.0;
+    if (!(hi[(size_t)i] > lo[(size_t)i])) continue;  // RefinedCutoffTransition{} (cutoff -1)
+    act.push_back(i);
+  }
+  auto tol_of = [&](int64_t i) {
+    return std::max(abs_tol, rel_tol * std::max(std::fabs(R_forbid[i]), std::fabs(R_allow[i])));
+  };
+  int64_t st_total = 0;
+  std::vector<double> bx, bd, bR;
+  std::vector<amps_trace_result> res;
+  std::vector<int64_t> next;
+  while (!act.empty()) {
+    next.clear();
+    bx.clear();
+    bd.clear();
+    bR.clear();
+    for (int64_t i : act) {
+      if (!((hi[(size_t)i] - lo[(size_t)i]) > tol_of(i))) {
+        cutoff[i] = hi[(size_t)i];
+        continue;
+      }
+      next.push_back(i);
+      for (int d = 0; d < 3; d++) bx.push_back(x0[3 * i + d]), bd.push_back(dir[3 * i + d]);
+      bR.push_back(0.5 * (lo[(size_t)i] + hi[(size_t)i]));
+    }
+    if (next.empty()) break;
+    res.resize(next.size());
+    int64_t st = 0;
+    if (int r = amps_gpu_trace_batch(c, P, precision, (int64_t)next.size(), bx.data(), bd.data(), bR.data(),
+                                     res.data(), &st))
+      return r;
+    st_total += st;
+    act.clear();
+    for (size_t k = 0; k < next.size(); k++) {
+      const int64_t i = next[k];
+      iterations[i]++;
+      const int term = res[k].termination;
+      const double mid = bR[k];
+      if (term == 0) {
+        hi[(size_t)i] = mid;
+      } else if (term == 1 || term == 2 || (trace_limit_forbidden && term >= 3 && term <= 5)) {
+        lo[(size_t)i] = mid;
+      } else {
+        status[i] = (term >= 3 && term <= 5) ? 1 : 2;
+        cutoff[i] = hi[(size_t)i];
+        continue;
+      }
+      act.push_back(i);
+    }
+  }
+  if (steps_total) *steps_total = st_total;
   return AMPS_OK;
 }
 

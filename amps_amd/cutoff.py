@@ -194,5 +194,33 @@ def cutoff_scan(ctx, params, x0, dirs, grid, mode=SCAN_UPPER, lo=None, hi=None, 
     return states, bins, steps.value
 
 
+def refine_cutoff(ctx, params, x0, dirs, r_forbid, r_allow, abs_tol=1.0e-3, rel_tol=1.0e-6,
+                  trace_limit_forbidden=True, precision=L.FP64):
+    """RefineCutoffTransition (CutoffBandSearch.h:212-243) for many brackets:
+    bisection of [r_forbid, r_allow] per (location, direction), one batched
+    trace launch per round.  Returns (cutoff_GV, iterations, status, steps);
+    status 1 = a trace limit ended the bracket (with trace_limit_forbidden
+    off).  A numerical failure after the retry raises, as the reference's
+    classifier throws."""
+    x0 = np.ascontiguousarray(x0, np.float64).reshape(-1, 3)
+    dirs = np.ascontiguousarray(dirs, np.float64).reshape(-1, 3)
+    lo = np.ascontiguousarray(r_forbid, np.float64)
+    hi = np.ascontiguousarray(r_allow, np.float64)
+    n = len(lo)
+    cut = np.zeros(n)
+    it = np.zeros(n, np.int32)
+    stt = np.zeros(n, np.int32)
+    steps = C.c_int64()
+    L.check(ctx.lib.amps_gpu_refine_cutoff(ctx.h, C.byref(params), precision, n, L._ptr(x0, C.c_double),
+                                           L._ptr(dirs, C.c_double), L._ptr(lo, C.c_double),
+                                           L._ptr(hi, C.c_double), abs_tol, rel_tol,
+                                           1 if trace_limit_forbidden else 0, L._ptr(cut, C.c_double),
+                                           L._ptr(it, C.c_int32), L._ptr(stt, C.c_int32),
+                                           C.byref(steps)), "refine_cutoff")
+    if np.any(stt == 2):
+        raise L.AmpsGpuError(f"refine_cutoff: numerical failure after retry in {int(np.sum(stt == 2))} brackets")
+    return cut, it, stt, steps.value
+
+
 def set_trace_param(ctx, name, value):
     L.check(ctx.lib.amps_gpu_trace_set_param(ctx.h, name.encode(), int(value)), f"trace param {name}")

@@ -21,6 +21,7 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBPS = 8000.0
 PUSH_BYTES = 104          # per particle-step: x,v read (48) + write (48) + ids (8)
+PUSH_BYTES_F32 = 80       # FP32 velocities: x read+write (48) + v read+write (24) + ids (8)
 DEPOSIT_BYTES_P = 48      # per particle: x, v read
 NODE_BYTES = 246 * 8      # per unique node: J (3) + mass matrix (243) written once
 
@@ -393,6 +394,28 @@ def main():
                 "achieved": round(dep_gbps, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                 "frac": round(dep_gbps / HBM_PEAK_GBPS, 4), "traffic": tr,
                 "algorithmic_bytes": int(dep_bytes)}
+    # the same step with the FP32 push (FP32 stencil, fields and velocities,
+    # FP64 positions and FP64 deposit): same population, regenerated
+    st.generate_particles(nps, [0.05, 0.005], drift=(0.01, 0.0, 0.0), seed=4, precision=L.FP32)
+    if args.warmup > 0:
+        st.run_steps(args.warmup, dt, use_graph=True, **opts)
+    ctx.sync()
+    t0 = time.perf_counter()
+    st.run_steps(args.steps, dt, use_graph=True, **opts)
+    ctx.sync()
+    el32 = time.perf_counter() - t0
+    ctx.set_timing(True)
+    ctx.reset_timing()
+    st.run_steps(4, dt, use_graph=False, **opts)
+    ctx.set_timing(False)
+    ms32, cnt32 = ctx.timing()
+    avg32 = {k: ms32[i] / max(cnt32[i], 1) for k, i in [("push", L.K_PUSH), ("compact", L.K_COMPACT),
+                                                        ("deposit_cell", L.K_DEPOSIT_CELL),
+                                                        ("deposit_node", L.K_DEPOSIT_NODE)]}
+    push32_bytes = n * PUSH_BYTES_F32 + nb * 729 * 6 * 8
+    fp32_variant = {"value": n * args.steps / el32, "unit": "particle-pushes/s", "ms_per_step": 1e3 * el32 / args.steps,
+                    "kernel_ms": {k: round(v, 4) for k, v in avg32.items()},
+                    "push_GBps": round(push32_bytes / (avg32["push"] * 1e-3) / 1e9, 1)}
     cpu = None if args.no_cpu else cpu_baseline()
     line = {
         "metric": "particle-pushes/s (full ECSIM particle step: Lapenta2017 push + migration + J/mass-matrix deposit, FP64)",
@@ -422,6 +445,7 @@ def main():
         "measured_copy_GBps": round(copy_gbps, 1),
         "autotune": tune,
         "traffic_bytes_per_launch": traffic,
+        "fp32_push_variant": fp32_variant,
     }
     if args.workload == "all":
         try:

@@ -217,6 +217,18 @@ int amps_gpu_cutoff_scan(amps_ctx* ctx, const amps_trace_params* params, int32_t
                          int32_t nR, const double* grid, const int32_t* bin_lo,
                          const int32_t* bin_hi, int8_t* states, int32_t* cutoff_bin,
                          int64_t* steps_total);
+/* RefineCutoffTransition (srcEarth/util/CutoffBandSearch.h:212-243) for n
+ * (location, direction) brackets at once, each [R_forbid, R_allow] with the
+ * forbidden state at R_forbid: bisection to max(abs_tol, rel_tol max(|lo|,
+ * |hi|)), one batched trace launch per round.  cutoff receives the allowed
+ * end (-1 for an empty bracket), iterations the midpoints traced, status 0
+ * resolved, 1 a trace limit ended it (trace_limit_forbidden = 0), 2 a
+ * numerical failure after the retry ended it (the reference throws). */
+int amps_gpu_refine_cutoff(amps_ctx* ctx, const amps_trace_params* params, int32_t precision,
+                           int64_t n, const double* x0, const double* dir, const double* R_forbid,
+                           const double* R_allow, double abs_tol, double rel_tol,
+                           int32_t trace_limit_forbidden, double* cutoff, int32_t* iterations,
+                           int32_t* status, int64_t* steps_total);
 /* Tracer tuning (autotuner): "workgroups" (persistent grid, 0 = 8 per CU),
  * "chunk" (steps between work-queue refills). */
 int amps_gpu_trace_set_param(amps_ctx* ctx, const char* name, int64_t value);

@@ -88,6 +88,20 @@ def ref_available():
     return os.path.exists(REF)
 
 
+REF_CLASSIFY = C.CFUNCTYPE(C.c_int, C.c_double, C.c_void_p)
+
+
+def ref_refine_cutoff(r_forbid, r_allow, classify, abs_tol=1.0e-3, rel_tol=1.0e-6):
+    """The reference's RefineCutoffTransition (CutoffBandSearch.h:212-243)
+    with classify(R_GV) -> 0 physical forbidden, 1 allowed, 2 unresolved.
+    Returns (cutoff_GV, iterations, unresolved)."""
+    cb = REF_CLASSIFY(lambda R, _u: int(classify(R)))
+    c, it, un = C.c_double(), C.c_int(), C.c_int()
+    ref().ref_refine_cutoff(r_forbid, r_allow, abs_tol, rel_tol, cb, None, C.byref(c), C.byref(it),
+                            C.byref(un))
+    return c.value, it.value, bool(un.value)
+
+
 def ref():
     """The reference's compiled gridless mover/dipole/band sources."""
     global _ref
@@ -101,6 +115,8 @@ def ref():
         r.ref_step_checked.argtypes = [P(dbl), P(dbl), dbl, dbl, dbl, P(dbl), dbl]
         r.ref_step_checked.restype = C.c_int
         r.ref_band_topology.argtypes = [C.c_int, P(C.c_int), P(C.c_int)]
+        r.ref_refine_cutoff.argtypes = [dbl, dbl, dbl, dbl, REF_CLASSIFY, C.c_void_p, P(dbl),
+                                        P(C.c_int), P(C.c_int)]
         _ref = r
     return _ref
 

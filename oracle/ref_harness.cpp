@@ -93,4 +93,18 @@ void ref_band_topology(int n, const int* states, int* out) {
   out[13] = t.lowerBracketUnresolved ? 1 : 0;
   out[14] = t.upperBracketUnresolved ? 1 : 0;
 }
+
+// RefineCutoffTransition driven by a caller-supplied classifier (0 physical
+// forbidden, 1 allowed, 2 unresolved).
+typedef int (*ref_classify_fn)(double R_GV, void* user);
+void ref_refine_cutoff(double r_forbid, double r_allow, double abs_tol, double rel_tol,
+                       ref_classify_fn classify, void* user, double* cutoff, int* iterations,
+                       int* unresolved) {
+  auto cls = [&](double R) { return static_cast<EarthUtil::CutoffSampleState>(classify(R, user)); };
+  const EarthUtil::RefinedCutoffTransition r =
+      EarthUtil::RefineCutoffTransition(r_forbid, r_allow, cls, abs_tol, rel_tol);
+  *cutoff = r.cutoff_GV;
+  *iterations = r.iterations;
+  *unresolved = r.unresolved ? 1 : 0;
+}
 }

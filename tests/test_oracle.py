@@ -383,3 +383,14 @@ def test_deposit_mass_matrix_symmetry():
             assert np.array_equal(blk, Mm[pos[kb], 9 * s2:9 * s2 + 9])
             n_checked += 1
     assert n_checked > 10000
+
+
+def test_reference_refine_cutoff_threshold():
+    """The reference's RefineCutoffTransition through the harness: a sharp
+    threshold is bracketed to max(1e-3, 1e-6 R) from the allowed side, in
+    ceil(log2(width / tol)) midpoints; an unresolved midpoint stops it."""
+    rc = 3.14159
+    c, it, un = O.ref_refine_cutoff(1.0, 5.0, lambda R: 1 if R >= rc else 0)
+    assert not un and rc <= c <= rc + 1e-3 and it == int(np.ceil(np.log2(4.0 / 1e-3)))
+    c, it, un = O.ref_refine_cutoff(1.0, 5.0, lambda R: 2 if R < 2.0 else 1)
+    assert un and it == 3 and c == 2.0  # midpoints 3, 2 (allowed), 1.5 (unresolved)

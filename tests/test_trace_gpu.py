@@ -149,6 +149,37 @@ def test_bracketed_full_scan_matches_full_scan(gpu_ctx):
     print(f"bracket results: {np.mean(bb >= 0):.3f} resolved, {np.mean(bb == bf):.3f} equal to the full scan")
 
 
+def test_refine_cutoff_matches_reference_bisection(gpu_ctx):
+    """Batched bisection (FP64) vs the reference's RefineCutoffTransition
+    compiled from CutoffBandSearch.h, driven by the oracle tracer: the same
+    cutoffs and iteration counts, bit for bit."""
+    P = CU.config2_params()
+    P.max_steps = 20000
+    pts, lat, lon = CU.config2_points(nlon=6, nlat=4)
+    zen, az = CU.config2_directions(4)
+    dirs = CU.arrival_dirs(pts, zen, az).reshape(-1, 3)
+    x0 = np.repeat(pts, len(zen), axis=0)
+    grid = CU.build_search_grid(0.1, 20.0, 48)
+    sf, bf, _ = CU.cutoff_scan(gpu_ctx, P, x0, dirs, grid, mode=CU.SCAN_FULL)
+    sel = np.flatnonzero((bf >= 1) & (sf[np.arange(len(bf)), np.maximum(bf - 1, 0)] == 0))[:20]
+    assert len(sel) >= 5
+    lo, hi = grid[bf[sel] - 1], grid[bf[sel]]
+    cut, it, stt, _ = CU.refine_cutoff(gpu_ctx, P, x0[sel], dirs[sel], lo, hi)
+    Q = _oparams(P)
+
+    def classify(i):
+        def f(R):
+            t = int(O.trace_batch(Q, x0[i:i + 1], dirs[i:i + 1], np.array([R]))["termination"][0])
+            return 1 if t == 0 else (0 if t <= 5 else 2)
+        return f
+
+    for k, i in enumerate(sel):
+        c, n_it, un = O.ref_refine_cutoff(lo[k], hi[k], classify(i))
+        assert not un and stt[k] == 0
+        assert cut[k] == c and it[k] == n_it, (k, cut[k], c, it[k], n_it)
+    print(f"refined {len(sel)} upper cutoffs, {int(it.sum())} midpoints")
+
+
 def test_fp32_tracer_agreement(gpu_ctx):
     """FP32 arithmetic with FP64 position accumulation vs the FP64 oracle:
     per-trajectory classification and per-direction cutoff bin."""
