@@ -108,6 +108,8 @@ def load():
         "amps_gpu_generate_particles": (C.c_int, [vp, i64, i32, P(dbl), P(dbl), C.c_uint64, i32]),
         "amps_gpu_run_steps": (C.c_int, [vp, P(PushOpts), dbl, i64, i32]),
         "amps_gpu_copy_probe": (C.c_int, [vp, i64, i32, P(dbl)]),
+        "amps_gpu_set_center_fields": (C.c_int, [vp, P(dbl)]),
+        "amps_gpu_interp_center": (C.c_int, [vp, i64, P(dbl), P(i32), P(dbl)]),
         "amps_gpu_trace_batch": (C.c_int, [vp, P(TraceParams), i32, i64, P(dbl), P(dbl), P(dbl),
                                            vp, P(i64)]),
         "amps_gpu_cutoff_scan": (C.c_int, [vp, P(TraceParams), i32, i32, i64, P(dbl), P(dbl), i32,

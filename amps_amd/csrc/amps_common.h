@@ -28,6 +28,8 @@ struct MeshView {
   const double* blk_xmin;    // [nb*3]
   const double* blk_xmax;    // [nb*3]
   const double* blk_dxc;     // FindCellIndex cell size dxRootBlock/2^L/8, [nb*3]
+  const int* blk_level;      // refinement level, [nb]
+  const int* nbinfo;         // min/max neighbour level and face/edge/corner neighbours, [nb*28]
   const double* blk_dxs;     // InitStencil cell size (xmax-xmin)/8, [nb*3]
   const int* leaf_of;        // [nf^3] finest-level lattice -> block id
 };
@@ -74,6 +76,10 @@ struct amps_ctx {
   std::vector<double> h_xmin, h_xmax;
   double *d_xmin = nullptr, *d_xmax = nullptr, *d_dxc = nullptr, *d_dxs = nullptr;
   int* d_leaf = nullptr;
+  int* d_level = nullptr;   // [nb] block refinement levels
+  int* d_nbinfo = nullptr;  // [nb][28] min/max neighbour level, face[6], edge[12], corner[8] neighbours
+  double* d_Bc = nullptr;   // [nb][10^3][3] cell-centred B with one ghost layer
+  bool has_center = false;
   amps::Faces faces{};
   int64_t nnodes = 0;
   std::vector<int32_t> h_node_of_corner;

@@ -95,6 +95,12 @@ static void free_mesh(amps_ctx* c) {
   (void)hipFree(c->d_dxc);
   (void)hipFree(c->d_dxs);
   (void)hipFree(c->d_leaf);
+  (void)hipFree(c->d_level);
+  (void)hipFree(c->d_nbinfo);
+  (void)hipFree(c->d_Bc);
+  c->d_level = c->d_nbinfo = nullptr;
+  c->d_Bc = nullptr;
+  c->has_center = false;
   (void)hipFree(c->d_inc8);
   (void)hipFree(c->d_node_order);
   c->d_node_order = nullptr;
@@ -386,6 +392,82 @@ extern "C" int amps_gpu_set_mesh(amps_ctx* c, const amps_mesh_desc* desc, const 
   m.blk_dxc = c->d_dxc;
   m.blk_dxs = c->d_dxs;
   m.leaf_of = c->d_leaf;
+  // neighbours of every block for the cell-centred stencil: the block just
+  // across each face, edge and corner (points 1/64 of the block outside it),
+  // and the min/max level over sub-face, sub-edge and corner samples
+  // (SetNeibRefinmentLevelLimits, meshAMRgeneric.h:1018-1046)
+  {
+    auto host_find = [&](const double p[3]) -> int {
+      int ix[3];
+      for (int d = 0; d < 3; d++) {
+        const double t = std::floor((p[d] - m.gxmin[d]) / m.dx_maxref[d]);
+        if (!(t >= 0.0 && t < (double)((int64_t)1 << lmax))) return -1;
+        ix[d] = (int)t;
+      }
+      const int sh = lmax - lf;
+      return leaf[((size_t)(ix[2] >> sh) * nf + (ix[1] >> sh)) * nf + (ix[0] >> sh)];
+    };
+    static const int EdgeAxis[12] = {0, 0, 0, 0, 1, 1, 1, 1, 2, 2, 2, 2};
+    static const int EdgeSide[12][3] = {{-1, 0, 0}, {-1, 0, 1}, {-1, 1, 1}, {-1, 1, 0},
+                                        {0, -1, 0}, {1, -1, 0}, {1, -1, 1}, {0, -1, 1},
+                                        {0, 0, -1}, {1, 0, -1}, {1, 1, -1}, {0, 1, -1}};
+    std::vector<int> nbinfo((size_t)nb * 28);
+    for (int b = 0; b < nb; b++) {
+      int* info = &nbinfo[(size_t)b * 28];
+      double lo[3], hi[3], dl[3];
+      for (int d = 0; d < 3; d++) {
+        lo[d] = xmin[3 * (size_t)b + d], hi[d] = xmax[3 * (size_t)b + d];
+        dl[d] = (hi[d] - lo[d]) / 64.0;
+      }
+      int mn = -1, mx = -1;
+      auto seen = [&](int bb) {
+        if (bb < 0) return;
+        const int L = level[bb];
+        if (mn == -1 || mn > L) mn = L;
+        if (mx < L) mx = L;
+      };
+      for (int f = 0; f < 6; f++) {
+        const int a = f / 2, side = f % 2;
+        double p[3];
+        for (int d = 0; d < 3; d++) p[d] = 0.5 * (lo[d] + hi[d]);
+        p[a] = side ? hi[a] + dl[a] : lo[a] - dl[a];
+        info[2 + f] = host_find(p);
+        const int a1 = (a + 1) % 3, a2 = (a + 2) % 3;
+        for (int q = 0; q < 4; q++) {
+          p[a1] = lo[a1] + (hi[a1] - lo[a1]) * ((q & 1) ? 0.75 : 0.25);
+          p[a2] = lo[a2] + (hi[a2] - lo[a2]) * ((q & 2) ? 0.75 : 0.25);
+          seen(host_find(p));
+        }
+      }
+      for (int e = 0; e < 12; e++) {
+        double p[3];
+        const int ax = EdgeAxis[e];
+        for (int d = 0; d < 3; d++) {
+          if (d == ax) p[d] = 0.5 * (lo[d] + hi[d]);
+          else p[d] = EdgeSide[e][d] ? hi[d] + dl[d] : lo[d] - dl[d];
+        }
+        info[8 + e] = host_find(p);
+        for (int q = 0; q < 2; q++) {
+          p[ax] = lo[ax] + (hi[ax] - lo[ax]) * (q ? 0.75 : 0.25);
+          seen(host_find(p));
+        }
+      }
+      for (int cc = 0; cc < 8; cc++) {
+        double p[3];
+        for (int d = 0; d < 3; d++) p[d] = ((cc >> d) & 1) ? hi[d] + dl[d] : lo[d] - dl[d];
+        info[20 + cc] = host_find(p);
+        seen(info[20 + cc]);
+      }
+      info[0] = mn;
+      info[1] = mx;
+    }
+    HIP_TRY(hipMalloc(&c->d_level, sizeof(int) * nb));
+    HIP_TRY(hipMalloc(&c->d_nbinfo, sizeof(int) * nbinfo.size()));
+    HIP_TRY(hipMemcpy(c->d_level, level, sizeof(int) * nb, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(c->d_nbinfo, nbinfo.data(), sizeof(int) * nbinfo.size(), hipMemcpyHostToDevice));
+    m.blk_level = c->d_level;
+    m.nbinfo = c->d_nbinfo;
+  }
   c->h_level.assign(level, level + nb);
   c->h_ijk.assign(ijk, ijk + 3 * (size_t)nb);
   c->h_xmin = xmin;

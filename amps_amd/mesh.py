@@ -192,6 +192,20 @@ class Particles:
                          self.cell[idx], self.id[idx])
 
 
+def center_fields(mesh, fn):
+    """Cell-centred values with one ghost layer per block, [nb][10][10][10][3]
+    (cell (i,j,k) in [-1,8]^3 at (i+1) + 10((j+1) + 10(k+1))), from fn(points
+    [m,3]) -> [m,3] evaluated at every cell and ghost-cell centre."""
+    g = np.arange(-1, NC + 1) + 0.5
+    dx = mesh.cell_size()  # [nb, 3]
+    X = mesh.blk_xmin[:, None, None, None, 0] + g[None, None, None, :] * dx[:, None, None, None, 0]
+    Y = mesh.blk_xmin[:, None, None, None, 1] + g[None, None, :, None] * dx[:, None, None, None, 1]
+    Z = mesh.blk_xmin[:, None, None, None, 2] + g[None, :, None, None] * dx[:, None, None, None, 2]
+    X, Y, Z = np.broadcast_arrays(X, Y, Z)
+    P = np.stack([X, Y, Z], axis=-1).reshape(-1, 3)
+    return np.ascontiguousarray(fn(P).reshape(mesh.nblocks, (NC + 2) ** 3, 3))
+
+
 def cell_of(mesh, x, blk):
     """FindCellIndex for points known to lie in block blk."""
     dxc = (mesh.xmax - mesh.xmin)[None, :] / (1 << mesh.level[blk]).astype(np.float64)[:, None] / float(NC)

@@ -90,6 +90,23 @@ class PicState:
         L.check(self.lib.amps_gpu_run_steps(self.ctx.h, C.byref(o), deposit_dt, int(nsteps),
                                             1 if use_graph else 0), "run_steps")
 
+    def set_center_fields(self, B_center):
+        """Cell-centred B with one ghost layer per block, [nb][1000][3]."""
+        Bc = np.ascontiguousarray(B_center, np.float64)
+        if Bc.shape != (self.mesh.nblocks, 1000, 3):
+            raise ValueError("B_center must be [nblocks][1000][3]")
+        L.check(self.lib.amps_gpu_set_center_fields(self.ctx.h, L._ptr(Bc, C.c_double)), "set_center_fields")
+
+    def interp_center(self, x, blk):
+        """B at points x[3][n] of blocks blk from the cell-centred field."""
+        x = np.ascontiguousarray(x, np.float64)
+        blk = np.ascontiguousarray(blk, np.int32)
+        n = x.shape[1]
+        B = np.zeros((3, n))
+        L.check(self.lib.amps_gpu_interp_center(self.ctx.h, n, L._ptr(x, C.c_double), L._ptr(blk, C.c_int32),
+                                                L._ptr(B, C.c_double)), "interp_center")
+        return B
+
     def particle_count(self):
         n = int(self.lib.amps_gpu_particle_count(self.ctx.h))
         if n < 0:

@@ -87,6 +87,18 @@ int amps_gpu_get_node_map(amps_ctx* ctx, int32_t* node_of_corner /* nb*729 */,
 /* Corner E^{n+theta} and B per block, [block][k][j][i][3] with local corner
  * i+9(j+9k) (SetBlock_E/SetBlock_B, pic_mover.cpp:84-186). */
 int amps_gpu_set_fields(amps_ctx* ctx, const double* E_corner, const double* B_corner);
+/* Cell-centred B with one ghost layer per block, [block][10][10][10][3],
+ * cell (i,j,k) in [-1,8]^3 at (i+1) + 10((j+1) + 10(k+1)): what SetBlock_B
+ * stages for a centre-based B (pic_mover.cpp:84-186); the caller fills the
+ * ghost cells (its halo exchange).  A ghost cell whose centre is outside the
+ * domain is absent from every stencil. */
+int amps_gpu_set_center_fields(amps_ctx* ctx, const double* B_center);
+/* B at n points x (planes x[d*n+p]) of blocks blk by the cell-centred linear
+ * stencil CellCentered::Linear::InitStencil (pic_interpolation_routines.cpp:
+ * 224-1070): trilinear on cell centres, the multi-block stencil with 2x2x2
+ * fine averages at coarse/fine interfaces and the coarse/fine blend. */
+int amps_gpu_interp_center(amps_ctx* ctx, int64_t n, const double* x, const int32_t* blk, double* B);
+
 /* Normalized charge and mass per species, the mover time step per species
  * (_SPECIES_DEPENDENT_GLOBAL_TIME_STEP_) and the particle weight. */
 int amps_gpu_set_species(amps_ctx* ctx, int32_t nspec, const double* charge,

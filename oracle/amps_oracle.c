@@ -33,6 +33,7 @@ struct orc_mesh {
   int lmax, nblocks, nnodes, cap;
   orc_tnode* t;
   int* block_node;
+  int* nb_info;  /* per block: minNeib, maxNeib, face[6], edge[12], corner[8] (28 ints) */
 };
 
 static int new_tnode(orc_mesh* m, int up, const int64_t xg[3], int64_t size, int level) {
@@ -55,6 +56,8 @@ static int new_tnode(orc_mesh* m, int up, const int64_t xg[3], int64_t size, int
   n->block = -1;
   return m->nnodes++;
 }
+
+static void build_neighbour_info(orc_mesh* m);
 
 orc_mesh* orc_mesh_create(const double gxmin[3], const double gxmax[3], int lmax, int nblocks,
                           const int* level, const int64_t* ijk) {
@@ -110,6 +113,7 @@ orc_mesh* orc_mesh_create(const double gxmin[3], const double gxmax[3], int lmax
     for (int c = 0; c < 8; c++)
       if (m->t[i].child[c] < 0) goto fail;
   }
+  build_neighbour_info(m);
   return m;
 fail:
   orc_mesh_destroy(m);
@@ -120,6 +124,7 @@ void orc_mesh_destroy(orc_mesh* m) {
   if (!m) return;
   free(m->t);
   free(m->block_node);
+  free(m->nb_info);
   free(m);
 }
 
@@ -1091,4 +1096,379 @@ void orc_band_topology(const int* st, int n, int* out) {
   }
   int v[15] = {lF, lA, uF, uA, nT, nAI, nU, allA, allF, lBelow, lAbove, uBelow, uAbove, lUn, uUn};
   for (int i = 0; i < 15; i++) out[i] = v[i];
+}
+
+
+/* ===================================================================== */
+/* Cell-centred linear interpolation                                      */
+/* ===================================================================== */
+/* Edges (pic_interpolation_routines.cpp:421-521): axis and the low/high side
+ * of the two other axes; corners c: x = c&1, y = (c>>1)&1, z = c>>2. */
+static const int EdgeAxis[12] = {0, 0, 0, 0, 1, 1, 1, 1, 2, 2, 2, 2};
+static const int EdgeSide[12][3] = {{-1, 0, 0}, {-1, 0, 1}, {-1, 1, 1}, {-1, 1, 0},
+                                    {0, -1, 0}, {1, -1, 0}, {1, -1, 1}, {0, -1, 1},
+                                    {0, 0, -1}, {1, 0, -1}, {1, 1, -1}, {0, 1, -1}};
+
+static int block_at(const orc_mesh* m, const double x[3]) {
+  int r = find_tnode(m, x, 0);
+  return (r >= 0) ? m->t[r].block : -1;
+}
+
+/* Neighbours by position: a point just outside the block across each face,
+ * edge and corner.  Sub-face and sub-edge points also sample finer
+ * neighbours for the min/max levels (SetNeibRefinmentLevelLimits,
+ * meshAMRgeneric.h:1018-1046). */
+static void build_neighbour_info(orc_mesh* m) {
+  m->nb_info = (int*)malloc(sizeof(int) * 28 * (size_t)m->nblocks);
+  for (int b = 0; b < m->nblocks; b++) {
+    const orc_tnode* n = &m->t[m->block_node[b]];
+    int* info = m->nb_info + 28 * b;
+    double lo[3], hi[3], dl[3];
+    for (int d = 0; d < 3; d++) {
+      lo[d] = n->xmin[d], hi[d] = n->xmax[d];
+      dl[d] = (hi[d] - lo[d]) / 64.0;
+    }
+    int mn = -1, mx = -1;
+#define ORC_SEEN(bb)                                                        \
+  do {                                                                      \
+    if ((bb) >= 0) {                                                        \
+      const int L_ = m->t[m->block_node[bb]].level;                         \
+      if (mn == -1 || mn > L_) mn = L_;                                     \
+      if (mx < L_) mx = L_;                                                 \
+    }                                                                       \
+  } while (0)
+    for (int f = 0; f < 6; f++) {
+      const int a = f / 2, side = f % 2;
+      double p[3];
+      for (int d = 0; d < 3; d++) p[d] = 0.5 * (lo[d] + hi[d]);
+      p[a] = side ? hi[a] + dl[a] : lo[a] - dl[a];
+      info[2 + f] = block_at(m, p);
+      const int a1 = (a + 1) % 3, a2 = (a + 2) % 3;
+      for (int q = 0; q < 4; q++) {
+        p[a1] = lo[a1] + (hi[a1] - lo[a1]) * ((q & 1) ? 0.75 : 0.25);
+        p[a2] = lo[a2] + (hi[a2] - lo[a2]) * ((q & 2) ? 0.75 : 0.25);
+        ORC_SEEN(block_at(m, p));
+      }
+    }
+    for (int e = 0; e < 12; e++) {
+      double p[3];
+      const int ax = EdgeAxis[e];
+      for (int d = 0; d < 3; d++) {
+        if (d == ax) p[d] = 0.5 * (lo[d] + hi[d]);
+        else p[d] = EdgeSide[e][d] ? hi[d] + dl[d] : lo[d] - dl[d];
+      }
+      info[8 + e] = block_at(m, p);
+      for (int q = 0; q < 2; q++) {
+        p[ax] = lo[ax] + (hi[ax] - lo[ax]) * (q ? 0.75 : 0.25);
+        ORC_SEEN(block_at(m, p));
+      }
+    }
+    for (int c = 0; c < 8; c++) {
+      double p[3];
+      for (int d = 0; d < 3; d++) p[d] = ((c >> d) & 1) ? hi[d] + dl[d] : lo[d] - dl[d];
+      info[20 + c] = block_at(m, p);
+      ORC_SEEN(info[20 + c]);
+    }
+#undef ORC_SEEN
+    info[0] = mn;
+    info[1] = mx;
+  }
+}
+
+void orc_block_neighbours(const orc_mesh* m, int block, int* minmax, int* face, int* edge,
+                          int* corner) {
+  const int* info = m->nb_info + 28 * block;
+  minmax[0] = info[0], minmax[1] = info[1];
+  for (int i = 0; i < 6; i++) face[i] = info[2 + i];
+  for (int i = 0; i < 12; i++) edge[i] = info[8 + i];
+  for (int i = 0; i < 8; i++) corner[i] = info[20 + i];
+}
+
+static inline int gidx(int i, int j, int k) { return (i + 1) + ORC_GN * ((j + 1) + ORC_GN * (k + 1)); }
+
+typedef struct {
+  int n;
+  int blk[ORC_CC_MAX], cell[ORC_CC_MAX];
+  double w[ORC_CC_MAX];
+} orc_cst;
+
+static void cst_add(orc_cst* s, double w, int blk, int cell) {
+  s->blk[s->n] = blk, s->cell[s->n] = cell, s->w[s->n] = w;
+  s->n++;
+}
+
+/* Normalize(), pic.h:7254-7261 */
+static void cst_normalize(orc_cst* s) {
+  double norm = 0.0;
+  for (int i = 0; i < s->n; i++) norm += s->w[i];
+  if (norm > 0.0)
+    for (int i = 0; i < s->n; i++) s->w[i] /= norm;
+}
+
+/* a cell of the block's ghost-layered array exists when it is interior or
+ * its centre lies inside the domain */
+static int cell_exists(const orc_mesh* m, const orc_tnode* n, int i, int j, int k) {
+  const int ijk[3] = {i, j, k};
+  for (int d = 0; d < 3; d++) {
+    if (ijk[d] < -1 || ijk[d] > ORC_NC) return 0;
+    if (ijk[d] >= 0 && ijk[d] < ORC_NC) continue;
+    const double dx = (n->xmax[d] - n->xmin[d]) / ORC_NC;
+    const double c = n->xmin[d] + (ijk[d] + 0.5) * dx;
+    if (!(c >= m->gxmin[d] && c < m->gxmax[d])) return 0;
+  }
+  return 1;
+}
+
+/* CellCentered::Constant::InitStencil, :168-219 */
+static void cst_constant(const orc_mesh* m, const double x[3], int tn, orc_cst* s) {
+  s->n = 0;
+  int ijk[3];
+  if (tn < 0 || find_cell_node(m, x, &m->t[tn], ijk) < 0) return;
+  cst_add(s, 1.0, m->t[tn].block, gidx(ijk[0], ijk[1], ijk[2]));
+}
+
+/* GetTriliniarInterpolationStencil, :820-907 (the renormalisation test reads
+ * the stencil being built; the reference reads a shared table there) */
+static void cst_tri(const orc_mesh* m, double iLoc, double jLoc, double kLoc, const double x[3], int tn,
+                    orc_cst* s) {
+  const orc_tnode* n = &m->t[tn];
+  s->n = 0;
+  const int i0 = (iLoc < 0.5) ? -1 : (int)(iLoc - 0.50);
+  const int j0 = (jLoc < 0.5) ? -1 : (int)(jLoc - 0.50);
+  const int k0 = (kLoc < 0.5) ? -1 : (int)(kLoc - 0.50);
+  double w[3];
+  w[0] = iLoc - (i0 + 0.5);
+  w[1] = jLoc - (j0 + 0.5);
+  w[2] = kLoc - (k0 + 0.5);
+  for (int i = 0; i < 2; i++)
+    for (int j = 0; j < 2; j++)
+      for (int k = 0; k < 2; k++) {
+        double W;
+        switch (i + 2 * j + 4 * k) {
+          case 0: W = (1.0 - w[0]) * (1.0 - w[1]) * (1.0 - w[2]); break;
+          case 1: W = w[0] * (1.0 - w[1]) * (1.0 - w[2]); break;
+          case 2: W = (1.0 - w[0]) * w[1] * (1.0 - w[2]); break;
+          case 3: W = w[0] * w[1] * (1.0 - w[2]); break;
+          case 4: W = (1.0 - w[0]) * (1.0 - w[1]) * w[2]; break;
+          case 5: W = w[0] * (1.0 - w[1]) * w[2]; break;
+          case 6: W = (1.0 - w[0]) * w[1] * w[2]; break;
+          default: W = w[0] * w[1] * w[2]; break;
+        }
+        if (cell_exists(m, n, i0 + i, j0 + j, k0 + k)) cst_add(s, W, n->block, gidx(i0 + i, j0 + j, k0 + k));
+      }
+  if (s->n == 0) {
+    cst_constant(m, x, tn, s);
+    return;
+  }
+  if (s->n != 8) cst_normalize(s);
+}
+
+/* GetTriliniarInterpolationMutiBlockStencil, :912-1070 */
+static void cst_multiblock(const orc_mesh* m, const double x[3], int tn, orc_cst* s) {
+  const orc_tnode* n = &m->t[tn];
+  s->n = 0;
+  double dxCell[3], xLower[3], xLoc[3];
+  int ijkMin[3];
+  for (int d = 0; d < 3; d++) dxCell[d] = (n->xmax[d] - n->xmin[d]) / ORC_NC;
+  for (int d = 0; d < 3; d++) {
+    ijkMin[d] = (x[d] - n->xmin[d] < 0.5 * dxCell[d]) ? -1 : (int)((x[d] - n->xmin[d] - 0.5 * dxCell[d]) / dxCell[d]);
+    xLower[d] = n->xmin[d] + (ijkMin[d] + 0.5) * dxCell[d];
+    xLoc[d] = (x[d] - xLower[d]) / dxCell[d];
+    if (xLoc[d] < 0.0) xLoc[d] = 0.0;
+    if (xLoc[d] > 1.0) xLoc[d] = 1.0;
+  }
+  int geometry = 1, physical = 1;
+  for (int di = 0; di < 2; di++)
+    for (int dj = 0; dj < 2; dj++)
+      for (int dk = 0; dk < 2; dk++) {
+        const int ijk[3] = {ijkMin[0] + di, ijkMin[1] + dj, ijkMin[2] + dk};
+        double xl[3];
+        for (int d = 0; d < 3; d++) xl[d] = n->xmin[d] + (ijk[d] + 0.5) * dxCell[d];
+        double W;
+        switch (di + 2 * dj + 4 * dk) {
+          case 0: W = (1.0 - xLoc[0]) * (1.0 - xLoc[1]) * (1.0 - xLoc[2]); break;
+          case 1: W = xLoc[0] * (1.0 - xLoc[1]) * (1.0 - xLoc[2]); break;
+          case 2: W = (1.0 - xLoc[0]) * xLoc[1] * (1.0 - xLoc[2]); break;
+          case 3: W = xLoc[0] * xLoc[1] * (1.0 - xLoc[2]); break;
+          case 4: W = (1.0 - xLoc[0]) * (1.0 - xLoc[1]) * xLoc[2]; break;
+          case 5: W = xLoc[0] * (1.0 - xLoc[1]) * xLoc[2]; break;
+          case 6: W = (1.0 - xLoc[0]) * xLoc[1] * xLoc[2]; break;
+          default: W = xLoc[0] * xLoc[1] * xLoc[2]; break;
+        }
+        const int sn = find_tnode(m, xl, tn);
+        if (sn < 0) {
+          geometry = 0;
+          continue;
+        }
+        const orc_tnode* S = &m->t[sn];
+        if (S->level == n->level) {
+          if (cell_exists(m, n, ijk[0], ijk[1], ijk[2])) cst_add(s, W, n->block, gidx(ijk[0], ijk[1], ijk[2]));
+          else physical = 0;
+        } else {
+          /* a logical coarse centre covered by a finer block: the average of
+           * its 2x2x2 fine cells */
+          int iNeib[3];
+          for (int d = 0; d < 3; d++) iNeib[d] = 2 * ((int)((xl[d] - S->xmin[d]) / dxCell[d]));
+          for (int ii = 0; ii < 2; ii++)
+            for (int jj = 0; jj < 2; jj++)
+              for (int kk = 0; kk < 2; kk++) {
+                const int f0 = iNeib[0] + ii, f1 = iNeib[1] + jj, f2 = iNeib[2] + kk;
+                const double FineCellWeight = (1.0 / 8.0) * W;
+                if (f0 < 0 || f0 >= ORC_NC || f1 < 0 || f1 >= ORC_NC || f2 < 0 || f2 >= ORC_NC) physical = 0;
+                else cst_add(s, FineCellWeight, S->block, gidx(f0, f1, f2));
+              }
+        }
+      }
+  if (!geometry || !physical) cst_constant(m, x, find_tnode(m, x, tn), s);
+}
+
+/* CellCentered::Linear::InitStencil, :224-816, AMPS branch of an AMR mesh */
+static void cst_init(const orc_mesh* m, const double x[3], int tn, orc_cst* s) {
+  const orc_tnode* n = &m->t[tn];
+  const int* info = m->nb_info + 28 * n->block;
+  const int minL = info[0], maxL = info[1];
+  const double iLoc = (x[0] - n->xmin[0]) / (n->xmax[0] - n->xmin[0]) * ORC_NC;
+  const double jLoc = (x[1] - n->xmin[1]) / (n->xmax[1] - n->xmin[1]) * ORC_NC;
+  const double kLoc = (x[2] - n->xmin[2]) / (n->xmax[2] - n->xmin[2]) * ORC_NC;
+  const double N = ORC_NC;
+  if (n->level == minL && n->level == maxL) {
+    cst_tri(m, iLoc, jLoc, kLoc, x, tn, s);
+    return;
+  }
+  if ((1.0 < iLoc) && (iLoc < N - 1) && (1.0 < jLoc) && (jLoc < N - 1) && (1.0 < kLoc) && (kLoc < N - 1)) {
+    cst_tri(m, iLoc, jLoc, kLoc, x, tn, s);
+    return;
+  }
+  if (n->level == minL) {
+    if ((0.5 < iLoc) && (iLoc < N - 0.5) && (0.5 < jLoc) && (jLoc < N - 0.5) && (0.5 < kLoc) && (kLoc < N - 0.5))
+      cst_tri(m, iLoc, jLoc, kLoc, x, tn, s);
+    else
+      cst_multiblock(m, x, tn, s);
+    return;
+  }
+  /* search a coarser block through the faces, edges and corners near x */
+  static const int faceEdges[6][4] = {{4, 11, 7, 8}, {5, 10, 6, 9}, {0, 9, 3, 8},
+                                      {1, 10, 2, 11}, {0, 5, 1, 4}, {3, 6, 2, 7}};
+  static const int FaceNodeMap[6][4] = {{0, 2, 4, 6}, {1, 3, 5, 7}, {0, 1, 4, 5},
+                                        {2, 3, 6, 7}, {0, 1, 2, 3}, {4, 5, 6, 7}};
+  double dxCell[3];
+  for (int d = 0; d < 3; d++) dxCell[d] = (n->xmax[d] - n->xmin[d]) / ORC_NC;
+  const double xLoc[3] = {iLoc, jLoc, kLoc};
+  double dmin = 10.0 * ORC_NC * ORC_NC * ORC_NC;
+  int coarser = -1;
+  int edgeTested[12] = {0}, cornerTested[8] = {0};
+#define ORC_TRY(dist) \
+  if ((dist) < dmin) dmin = (dist), coarser = nb
+  for (int idim = 0; idim < 3; idim++) {
+    int iFace;
+    if (xLoc[idim] <= 1.0) iFace = 2 * idim;
+    else if (xLoc[idim] >= N - 1.0) iFace = 2 * idim + 1;
+    else continue;
+    int nb = info[2 + iFace];
+    if (nb >= 0 && m->t[m->block_node[nb]].level < n->level) {
+      const orc_tnode* B = &m->t[m->block_node[nb]];
+      int cnt = 0;
+      for (int d = 0; d < 3; d++)
+        if ((B->xmin[d] - dxCell[d] <= x[d]) && (B->xmax[d] + dxCell[d] >= x[d])) cnt++;
+      if (cnt == 3) {
+        const int a = iFace / 2;
+        if ((iFace & 1) == 0) {
+          if (xLoc[a] < 1.0) ORC_TRY(xLoc[a]);
+        } else {
+          if (xLoc[a] > N - 1) ORC_TRY(N - xLoc[a]);
+        }
+      }
+    }
+    for (int q = 0; q < 4; q++) {
+      const int e = faceEdges[iFace][q];
+      if (edgeTested[e]) continue;
+      edgeTested[e] = 1;
+      nb = info[8 + e];
+      if (nb < 0 || !(m->t[m->block_node[nb]].level < n->level)) continue;
+      const orc_tnode* B = &m->t[m->block_node[nb]];
+      int cnt = 0;
+      for (int d = 0; d < 3; d++)
+        if ((B->xmin[d] - dxCell[d] <= x[d]) && (B->xmax[d] + dxCell[d] >= x[d])) cnt++;
+      if (cnt != 3) continue;
+      /* both non-axis coordinates within one cell of the edge's faces */
+      int ok = 1;
+      for (int d = 0; d < 3; d++) {
+        if (d == EdgeAxis[e]) continue;
+        if (EdgeSide[e][d] ? !(xLoc[d] > N - 1) : !(xLoc[d] < 1.0)) ok = 0;
+      }
+      if (!ok) continue;
+      for (int d = 0; d < 3; d++) {
+        if (d == EdgeAxis[e]) continue;
+        ORC_TRY(EdgeSide[e][d] ? N - xLoc[d] : xLoc[d]);
+      }
+    }
+    for (int q = 0; q < 4; q++) {
+      const int c = FaceNodeMap[iFace][q];
+      if (cornerTested[c]) continue;
+      cornerTested[c] = 1;
+      nb = info[20 + c];
+      if (nb < 0 || !(m->t[m->block_node[nb]].level < n->level)) continue;
+      const orc_tnode* B = &m->t[m->block_node[nb]];
+      int cnt = 0;
+      for (int d = 0; d < 3; d++)
+        if ((B->xmin[d] - dxCell[d] <= x[d]) && (B->xmax[d] + dxCell[d] >= x[d])) cnt++;
+      if (cnt != 3) continue;
+      int ok = 1;
+      for (int d = 0; d < 3; d++)
+        if (((c >> d) & 1) ? !(xLoc[d] > N - 1) : !(xLoc[d] < 1.0)) ok = 0;
+      if (!ok) continue;
+      for (int d = 0; d < 3; d++) ORC_TRY(((c >> d) & 1) ? N - xLoc[d] : xLoc[d]);
+    }
+  }
+#undef ORC_TRY
+  if (coarser >= 0) {
+    cst_multiblock(m, x, m->block_node[coarser], s);
+    if ((0.5 < dmin) && (dmin <= 1.0)) {
+      orc_cst fine;
+      cst_tri(m, iLoc, jLoc, kLoc, x, tn, &fine);
+      const double cw = 1.0 - (dmin - 0.5) / 0.5, fw = (dmin - 0.5) / 0.5;
+      for (int i = 0; i < s->n; i++) s->w[i] *= cw;
+      for (int i = 0; i < fine.n; i++) fine.w[i] *= fw;
+      /* cStencilGeneric::Add, pic.h:7273-7291 */
+      for (int i = 0; i < fine.n; i++) {
+        int found = 0;
+        for (int j = 0; j < s->n; j++)
+          if (s->blk[j] == fine.blk[i] && s->cell[j] == fine.cell[i]) {
+            s->w[j] += fine.w[i];
+            found = 1;
+            break;
+          }
+        if (!found) cst_add(s, fine.w[i], fine.blk[i], fine.cell[i]);
+      }
+    }
+    return;
+  }
+  cst_tri(m, iLoc, jLoc, kLoc, x, tn, s);
+}
+
+int orc_center_stencil(const orc_mesh* m, const double x[3], int block, int* st_block,
+                       int* st_cell, double* st_w) {
+  orc_cst s;
+  cst_init(m, x, m->block_node[block], &s);
+  for (int i = 0; i < s.n; i++) st_block[i] = s.blk[i], st_cell[i] = s.cell[i], st_w[i] = s.w[i];
+  return s.n;
+}
+
+void orc_interp_center(const orc_mesh* m, const double* Bc, int64_t n, const double* x,
+                       const int* block, double* B, int nthreads) {
+#ifdef _OPENMP
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#pragma omp parallel for schedule(static)
+#endif
+  for (int64_t p = 0; p < n; p++) {
+    const double xp[3] = {x[p], x[n + p], x[2 * n + p]};
+    orc_cst s;
+    cst_init(m, xp, m->block_node[block[p]], &s);
+    double b[3] = {0.0, 0.0, 0.0};
+    for (int i = 0; i < s.n; i++) {
+      const double* v = Bc + ((size_t)s.blk[i] * ORC_GN * ORC_GN * ORC_GN + (size_t)s.cell[i]) * 3;
+      for (int d = 0; d < 3; d++) b[d] += s.w[i] * v[d];
+    }
+    for (int d = 0; d < 3; d++) B[d * n + p] = b[d];
+  }
 }

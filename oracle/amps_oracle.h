@@ -128,6 +128,23 @@ int orc_upper_cutoff_bin(const int* allowed, int n);
  * lowerBelow, lowerAbove, upperBelow, upperAbove, lowerUnres, upperUnres. */
 void orc_band_topology(const int* states, int n, int* out);
 
+/* ---- cell-centred linear interpolation (CellCentered::Linear::InitStencil,
+ * pic_interpolation_routines.cpp:224-1070) ---------------------------------
+ * Cell-centred data per block with one ghost layer: Bc[block][10][10][10][3],
+ * cell (i,j,k) in [-1,8]^3 at ((i+1) + 10((j+1) + 10(k+1))).  A ghost cell
+ * exists when its centre lies inside the domain.  The stencil is returned as
+ * (block, ghost-layer cell index, weight) in the reference's order. */
+#define ORC_GN 10
+#define ORC_CC_MAX 80
+int orc_center_stencil(const orc_mesh* m, const double x[3], int block, int* st_block,
+                       int* st_cell, double* st_w);
+void orc_interp_center(const orc_mesh* m, const double* Bc, int64_t n, const double* x,
+                       const int* block, double* B, int nthreads);
+/* per-block neighbour data used by the stencil: min and max neighbour level,
+ * and the neighbour block across each face (6), edge (12) and corner (8) */
+void orc_block_neighbours(const orc_mesh* m, int block, int* minmax, int* face, int* edge,
+                          int* corner);
+
 #ifdef __cplusplus
 }
 #endif

@@ -80,6 +80,9 @@ def lib():
         l.orc_rigidity_grid.argtypes = [dbl, dbl, i32, P(dbl)]
         l.orc_upper_cutoff_bin.argtypes = [P(C.c_int), i32]
         l.orc_band_topology.argtypes = [P(C.c_int), i32, P(C.c_int)]
+        l.orc_center_stencil.argtypes = [vp, P(dbl), i32, P(C.c_int), P(C.c_int), P(dbl)]
+        l.orc_interp_center.argtypes = [vp, P(dbl), i64, P(dbl), P(C.c_int), P(dbl), i32]
+        l.orc_block_neighbours.argtypes = [vp, i32, P(C.c_int), P(C.c_int), P(C.c_int), P(C.c_int)]
         _lib = l
     return _lib
 
@@ -160,6 +163,36 @@ class Mesh:
         lib().orc_corner_stencil(self.h, _p(x, C.c_double), b, _p(w, C.c_double),
                                  _p(t, C.c_double), _p(c, C.c_int))
         return x, w, t, c
+
+    def center_stencil(self, x, b):
+        """Cell-centred linear stencil: [(block, ghost-layer cell index, weight)]."""
+        x = np.array(x, np.float64)
+        bb = np.zeros(80, np.int32)
+        cc = np.zeros(80, np.int32)
+        ww = np.zeros(80)
+        n = lib().orc_center_stencil(self.h, _p(x, C.c_double), int(b), _p(bb, C.c_int), _p(cc, C.c_int),
+                                     _p(ww, C.c_double))
+        return bb[:n], cc[:n], ww[:n]
+
+    def interp_center(self, Bc, x, blk, nthreads=0):
+        """B at points x[3][n] (blocks blk) from ghost-layered cell data Bc[nb][1000][3]."""
+        Bc = np.ascontiguousarray(Bc, np.float64)
+        x = np.ascontiguousarray(x, np.float64)
+        blk = np.ascontiguousarray(blk, np.int32)
+        n = x.shape[1]
+        B = np.zeros((3, n))
+        lib().orc_interp_center(self.h, _p(Bc, C.c_double), n, _p(x, C.c_double), _p(blk, C.c_int),
+                                _p(B, C.c_double), nthreads)
+        return B
+
+    def neighbours(self, b):
+        mm = np.zeros(2, np.int32)
+        f = np.zeros(6, np.int32)
+        e = np.zeros(12, np.int32)
+        c = np.zeros(8, np.int32)
+        lib().orc_block_neighbours(self.h, int(b), _p(mm, C.c_int), _p(f, C.c_int), _p(e, C.c_int),
+                                   _p(c, C.c_int))
+        return mm, f, e, c
 
     def build_nodes(self):
         noc = np.zeros(self.nblocks * 729, np.int32)

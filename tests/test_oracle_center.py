@@ -1,0 +1,110 @@
+"""Pins the oracle's cell-centred linear interpolation
+(CellCentered::Linear::InitStencil, pic_interpolation_routines.cpp:224-1070)
+with what the mathematics fixes:
+
+* a constant field is reproduced exactly everywhere (the weights of every
+  branch, the 2x2x2 fine averages and the coarse/fine blend sum to one);
+* a linear field is reproduced to rounding wherever no support point leaves
+  the domain: trilinear interpolation of cell centres, the fine averages
+  (their mean is the coarse centre) and the blend of two exact stencils are
+  all exact for linear data, so a wrong index or weight shows up;
+* the AMR branches are exercised: stencils that reach a coarser block, fine
+  2x2x2 averages, and blended stencils all occur on the 2-level mesh.
+"""
+import numpy as np
+import pytest
+
+from amps_amd import mesh as M
+from oracle import oracle as O
+
+A0 = np.array([0.3, -1.2, 2.5])
+G = np.array([[1.5, -0.7, 0.2], [0.4, 2.0, -1.1], [-0.9, 0.3, 0.8]])
+
+
+def linear(P):
+    return A0[None, :] + P @ G.T
+
+
+def _points(mesh, n, margin, seed):
+    rng = np.random.default_rng(seed)
+    x = rng.uniform(np.asarray(mesh.xmin) + margin, np.asarray(mesh.xmax) - margin, size=(n, 3))
+    return x.T.copy(), mesh.locate(x).astype(np.int32)
+
+
+@pytest.mark.parametrize("amr", [False, True])
+def test_constant_field_exact(amr):
+    mesh = M.central_octant_mesh() if amr else M.uniform_mesh(2)
+    om = O.Mesh(mesh)
+    Bc = M.center_fields(mesh, lambda P: np.tile([1.25, -0.5, 3.0], (len(P), 1)))
+    x, blk = _points(mesh, 20000, 0.0, 3)
+    B = om.interp_center(Bc, x, blk)
+    assert np.max(np.abs(B - np.array([[1.25], [-0.5], [3.0]]))) < 1e-14
+
+
+@pytest.mark.parametrize("amr", [False, True])
+def test_linear_field_reproduced(amr):
+    mesh = M.central_octant_mesh() if amr else M.uniform_mesh(2)
+    om = O.Mesh(mesh)
+    Bc = M.center_fields(mesh, linear)
+    # one coarse cell of margin: no support point leaves the domain
+    x, blk = _points(mesh, 20000, 2.0 / 32, 4)
+    B = om.interp_center(Bc, x, blk)
+    err = np.max(np.abs(B - linear(x.T).T))
+    assert err < 1e-12, err
+
+
+def test_amr_branches_occur():
+    mesh = M.central_octant_mesh()
+    om = O.Mesh(mesh)
+    lv = mesh.level
+    x, blk = _points(mesh, 4000, 2.0 / 32, 5)
+    other_block = fine_avg = blended = 0
+    for p in range(x.shape[1]):
+        b, c, w = om.center_stencil(x[:, p], blk[p])
+        assert abs(w.sum() - 1.0) < 1e-12
+        if np.any(b != blk[p]):
+            other_block += 1
+        if np.any(np.isclose(w * 8, np.round(w * 8 * 1e6) / 1e6)) and len(w) > 8:
+            fine_avg += 1
+        if len(set(b.tolist())) > 1 and np.any(b == blk[p]) and len(w) > 8:
+            blended += 1
+    assert other_block > 50 and fine_avg > 50 and blended > 5, (other_block, fine_avg, blended)
+    # neighbour levels: a coarse block next to the refined octant sees a finer
+    # neighbour, a fine block at the octant's surface a coarser one
+    mm = np.array([om.neighbours(b)[0] for b in range(mesh.nblocks)])
+    assert np.any((mm[:, 1] > lv)) and np.any((mm[:, 0] < lv))
+
+
+def test_continuous_across_coarse_fine_zones():
+    """A smooth (quadratic) field interpolates continuously along a line that
+    crosses a face of the refined octant away from block edges: from the
+    fine side's blend (0.5 < dmin <= 1, fine weight (dmin - 0.5)/0.5) through
+    the fine side's coarse multi-block stencil into the coarse side's, which
+    shares its frame.  (Near block edges the reference's stencil choice jumps
+    by construction, and the coarse side's trilinear stencil reads ghost
+    cells, whose fill is the caller's: those stretches are not sampled.)"""
+    mesh = M.central_octant_mesh()
+    om = O.Mesh(mesh)
+
+    def quad(P):
+        return np.stack([P[:, 0] ** 2 + P[:, 1] * P[:, 2], np.sin(3 * P[:, 1]), P[:, 2] ** 2 - P[:, 0]], axis=1)
+
+    Bc = M.center_fields(mesh, quad)
+    rng = np.random.default_rng(7)
+    worst = 0.0
+    for line in range(40):
+        a = int(rng.integers(0, 3))
+        side = float(rng.choice([-0.5, 0.5]))
+        p0 = np.zeros(3)
+        for d in range(3):
+            # inside a coarse block (0.25) and a fine block (0.125), 1+ cells from their faces
+            p0[d] = -0.5 + 0.25 * rng.integers(0, 4) + rng.choice([0.04, 0.165]) + rng.uniform(0.0, 0.045)
+        t = np.linspace(-0.06, 0.012, 1501)
+        x = np.repeat(p0[None, :], len(t), axis=0)
+        x[:, a] = side + t * np.sign(side)  # t < 0: inside the octant
+        B = om.interp_center(Bc, x.T.copy(), mesh.locate(x).astype(np.int32))
+        dB = np.max(np.abs(np.diff(B, axis=1)), axis=0)
+        worst = max(worst, float(np.max(dB) / np.median(dB)))
+    # a smooth interpolant changes by about the same amount per sample; a
+    # jump between the coarse and fine stencils is many sample steps
+    assert worst < 4.0, worst
