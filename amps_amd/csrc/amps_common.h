@@ -119,6 +119,7 @@ struct amps_ctx {
   int dep_nch = 12;            // deposit output chunks per cell (waves per workgroup)
   int dep_exact = 0;           // 1: unfused mul+add in the deposit (bit-exact order)
   int dep_kernel = 1;          // cell pass: 0 per-lane particle loop, 1 LDS record batches
+  int b_center = 0;            // 1: B from the cell-centred field (mover and deposit)
   // deposit
   double* d_cellpart = nullptr;
   double* d_nodeout = nullptr;

@@ -14,10 +14,10 @@
 using namespace amps;
 
 // kernels (pic_kernels.hip)
-template <int MOVER>
+template <int MOVER, bool BC>
 __global__ void k_push_f64(MeshView, Faces, Species, const double*, const double*, const Tile*,
                            const int*, int, double, int64_t, double*, double*, const uint8_t*,
-                           int*, int8_t*);
+                           int*, int8_t*, const double*);
 template <int MOVER>
 __global__ void k_push_f32(MeshView, Faces, Species, const double*, const double*, const Tile*,
                            const int*, int, double, int64_t, double*, float*, const uint8_t*, int*,
@@ -197,6 +197,9 @@ extern "C" int amps_gpu_set_param(amps_ctx* c, const char* name, int64_t value) 
     c->dep_nch = (int)value;
   } else if (n == "deposit_exact") {
     c->dep_exact = value ? 1 : 0;
+  } else if (n == "b_center") {
+    if (value != 0 && value != 1) return fail(AMPS_E_ARG, "b_center must be 0 or 1");
+    c->b_center = (int)value;
   } else if (n == "deposit_kernel") {
     if (value != 0 && value != 1) return fail(AMPS_E_ARG, "deposit_kernel must be 0 or 1");
     if (!deposit_variant_ok((int)value, c->dep_nch))
@@ -716,6 +719,13 @@ static int check_mover(const amps_ctx* c, const amps_push_opts* o, const char* w
     return fail(AMPS_E_ARG, std::string(who) + ": unknown mover");
   if (o->mover == AMPS_MOVER_RELATIVISTIC && c->precision == AMPS_FP32)
     return fail(AMPS_E_ARG, std::string(who) + ": the relativistic mover runs in FP64 only");
+  if (c->b_center) {
+    if (!c->has_center) return fail(AMPS_E_STATE, std::string(who) + ": b_center set but no cell-centred B given");
+    if (c->precision == AMPS_FP32 || o->mover == AMPS_MOVER_RELATIVISTIC)
+      return fail(AMPS_E_ARG, std::string(who) + ": cell-centred B runs with the FP64 Lapenta and Boris movers");
+    if (c->dep_kernel != 1)
+      return fail(AMPS_E_ARG, std::string(who) + ": cell-centred B runs with deposit_kernel 1");
+  }
   return 0;
 }
 
@@ -735,21 +745,21 @@ static int launch_push(amps_ctx* c, const amps_push_opts* o) {
     HIP_TRY(hipGetLastError());
     return 0;
   }
-  if (o->mover == AMPS_MOVER_LAPENTA)
-    hipLaunchKernelGGL(k_push_f64<AMPS_MOVER_LAPENTA>, dim3((unsigned)c->max_tiles), dim3(256), 0,
-                       c->stream, c->mv, c->faces, c->sp, c->d_E, c->d_B, c->d_tiles, c->d_ntiles,
-                       o->bc_mode, o->r_sphere, c->cap, c->d_x[k], c->d_v[k], c->d_spec[k],
-                       c->d_key[k], c->d_code);
-  else if (o->mover == AMPS_MOVER_RELATIVISTIC)
-    hipLaunchKernelGGL(k_push_f64<AMPS_MOVER_RELATIVISTIC>, dim3((unsigned)c->max_tiles), dim3(256), 0,
-                       c->stream, c->mv, c->faces, c->sp, c->d_E, c->d_B, c->d_tiles, c->d_ntiles,
-                       o->bc_mode, o->r_sphere, c->cap, c->d_x[k], c->d_v[k], c->d_spec[k],
-                       c->d_key[k], c->d_code);
-  else
-    hipLaunchKernelGGL(k_push_f64<AMPS_MOVER_BORIS>, dim3((unsigned)c->max_tiles), dim3(256), 0,
-                       c->stream, c->mv, c->faces, c->sp, c->d_E, c->d_B, c->d_tiles, c->d_ntiles,
-                       o->bc_mode, o->r_sphere, c->cap, c->d_x[k], c->d_v[k], c->d_spec[k],
-                       c->d_key[k], c->d_code);
+#define LAUNCH_PUSH64(MV, BCV)                                                                         \
+  hipLaunchKernelGGL((k_push_f64<MV, BCV>), dim3((unsigned)c->max_tiles), dim3(256), 0, c->stream, c->mv,     \
+                     c->faces, c->sp, c->d_E, c->d_B, c->d_tiles, c->d_ntiles, o->bc_mode, o->r_sphere, c->cap, \
+                     c->d_x[k], c->d_v[k], c->d_spec[k], c->d_key[k], c->d_code, c->d_Bc)
+  const bool bc = c->b_center != 0;
+  if (o->mover == AMPS_MOVER_LAPENTA) {
+    if (bc) LAUNCH_PUSH64(AMPS_MOVER_LAPENTA, true);
+    else LAUNCH_PUSH64(AMPS_MOVER_LAPENTA, false);
+  } else if (o->mover == AMPS_MOVER_RELATIVISTIC) {
+    LAUNCH_PUSH64(AMPS_MOVER_RELATIVISTIC, false);
+  } else {
+    if (bc) LAUNCH_PUSH64(AMPS_MOVER_BORIS, true);
+    else LAUNCH_PUSH64(AMPS_MOVER_BORIS, false);
+  }
+#undef LAUNCH_PUSH64
   HIP_TRY(hipGetLastError());
   return 0;
 }
@@ -832,6 +842,8 @@ extern "C" int amps_gpu_update_j_mass_matrix(amps_ctx* c, double dt, double* J_o
   if (!c) return fail(AMPS_E_ARG, "null context");
   if (!c->has_mesh || !c->has_fields || !c->has_species)
     return fail(AMPS_E_STATE, "amps_gpu_update_j_mass_matrix: mesh, fields and species are required");
+  if (c->b_center && (!c->has_center || c->dep_kernel != 1))
+    return fail(AMPS_E_STATE, "amps_gpu_update_j_mass_matrix: b_center needs the cell-centred B and deposit_kernel 1");
   (void)hipSetDevice(c->device);
   if (int r = ensure_deposit_buffers(c)) return r;
   if (c->cap == 0) {

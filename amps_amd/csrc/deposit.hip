@@ -20,6 +20,7 @@
 #include <hip/hip_runtime.h>
 
 #include "amps_common.h"
+#include "center_device.h"
 #include "pic_device.h"
 
 using namespace amps;
@@ -126,7 +127,7 @@ __device__ __forceinline__ void dispatch_stage(int w, int w0, const double* acc,
 
 // RB = NCH: every thread builds one record per batch, so a batch holds NCH
 // particles of each of the 64 cells.
-template <int NCH, bool EXACT>
+template <int NCH, bool EXACT, bool BC>
 __global__ __launch_bounds__(64 * NCH) void k_deposit_cell_rec(MeshView m, Species sp, double dtTotal,
                                                                 const double* __restrict__ B,
                                                                 const int* __restrict__ offset,
@@ -135,7 +136,8 @@ __global__ __launch_bounds__(64 * NCH) void k_deposit_cell_rec(MeshView m, Speci
                                                                 const double* __restrict__ v,
                                                                 int vf32,
                                                                 const uint8_t* __restrict__ spec,
-                                                                double* __restrict__ cellpart) {
+                                                                double* __restrict__ cellpart,
+                                                                const double* __restrict__ Bc) {
   using R_ = Rec<EXACT>;
   constexpr int RB = NCH;
   constexpr int V = R_::VALS;  // values per cell record
@@ -155,10 +157,12 @@ __global__ __launch_bounds__(64 * NCH) void k_deposit_cell_rec(MeshView m, Speci
   const int st = tile & 7;
   const int sx = (st & 1) * 4, sy = ((st >> 1) & 1) * 4, sz = (st >> 2) * 4;
   const double* Bb = B + (size_t)b * NCORNER * 3;
-  for (int i = threadIdx.x; i < 125 * 3; i += blockDim.x) {
-    const int cidx = i / 3, d = i % 3;
-    const int a0 = cidx % 5, a1 = (cidx / 5) % 5, a2 = cidx / 25;
-    sB[i] = Bb[3 * ((sx + a0) + NN * ((sy + a1) + NN * (sz + a2))) + d];
+  if constexpr (!BC) {
+    for (int i = threadIdx.x; i < 125 * 3; i += blockDim.x) {
+      const int cidx = i / 3, d = i % 3;
+      const int a0 = cidx % 5, a1 = (cidx / 5) % 5, a2 = cidx / 25;
+      sB[i] = Bb[3 * ((sx + a0) + NN * ((sy + a1) + NN * (sz + a2))) + d];
+    }
   }
   BlockGeom g;
   load_block_geom(m, b, g);
@@ -220,9 +224,12 @@ __global__ __launch_bounds__(64 * NCH) void k_deposit_cell_rec(MeshView m, Speci
       const double vI[3] = {nv[0], nv[1], nv[2]};
       const int s = ns;
       double wn[8], table[8], Lf[6];
+      double xn[3] = {xI[0], xI[1], xI[2]};  // the position the stencils see (nudged)
       int iX[3];
       if constexpr (EXACT) {
         corner_stencil(g, xI, wn, table, iX);
+#pragma unroll
+        for (int d = 0; d < 3; d++) xn[d] = xI[d];
         iX[0] = clampi(iX[0], sx, sx + 3);
         iX[1] = clampi(iX[1], sy, sy + 3);
         iX[2] = clampi(iX[2], sz, sz + 3);
@@ -234,6 +241,7 @@ __global__ __launch_bounds__(64 * NCH) void k_deposit_cell_rec(MeshView m, Speci
         for (int d = 0; d < 3; d++) {
           double xd = xI[d];
           if (fabs(xd - g.xmax[d]) < 1e-10 * g.dxs[d]) xd = g.xmax[d] - 1e-10 * g.dxs[d];
+          xn[d] = xd;
           const double t = (xd - g.xmin[d]) * inv_dxs[d] - iX[d];
           const double f = t < 0.0 ? 0.0 : (t > 1.0 ? 1.0 : t);
           Lf[2 * d] = 1.0 - f;
@@ -250,11 +258,15 @@ __global__ __launch_bounds__(64 * NCH) void k_deposit_cell_rec(MeshView m, Speci
         for (int l = 0; l < 8; l++) wn[l] *= rn;
       }
       double Bv[3] = {0.0, 0.0, 0.0};
+      if constexpr (BC) {
+        center_B(m, Bc, xn, b, Bv);  // centre-based B, pic_field_solver_ecsim.cpp:2097-2129
+      } else {
 #pragma unroll
-      for (int l = 0; l < 8; l++) {
-        const int c = (iX[0] - sx + (l >> 2)) + 5 * ((iX[1] - sy + ((l >> 1) & 1)) + 5 * (iX[2] - sz + (l & 1)));
+        for (int l = 0; l < 8; l++) {
+          const int c = (iX[0] - sx + (l >> 2)) + 5 * ((iX[1] - sy + ((l >> 1) & 1)) + 5 * (iX[2] - sz + (l & 1)));
 #pragma unroll
-        for (int d = 0; d < 3; d++) Bv[d] += wn[l] * sB[3 * c + d];
+          for (int d = 0; d < 3; d++) Bv[d] += wn[l] * sB[3 * c + d];
+        }
       }
       const double QdT_over_2m = sconst[s][1];
       const double QdT_over_2m_squared = QdT_over_2m * QdT_over_2m;
@@ -322,12 +334,19 @@ __global__ __launch_bounds__(64 * NCH) void k_deposit_cell_rec(MeshView m, Speci
   }
 }
 
-template <int NCH, bool EX>
+template <int NCH, bool EX, bool BC>
 void launch_rec(amps_ctx* c, double dt) {
   const int k = c->cur;
-  hipLaunchKernelGGL((k_deposit_cell_rec<NCH, EX>), dim3((unsigned)c->mv.nb * 8), dim3(64 * NCH), 0,
+  hipLaunchKernelGGL((k_deposit_cell_rec<NCH, EX, BC>), dim3((unsigned)c->mv.nb * 8), dim3(64 * NCH), 0,
                      c->stream, c->mv, c->sp, dt, c->d_B, c->d_offset, c->cap, c->d_x[k], c->d_v[k],
-                     c->precision == AMPS_FP32 ? 1 : 0, c->d_spec[k], c->d_cellpart);
+                     c->precision == AMPS_FP32 ? 1 : 0, c->d_spec[k], c->d_cellpart, c->d_Bc);
+}
+
+template <int NCH>
+void launch_rec2(amps_ctx* c, double dt) {
+  const bool ex = c->dep_exact != 0, bc = c->b_center != 0;
+  if (ex) bc ? launch_rec<NCH, true, true>(c, dt) : launch_rec<NCH, true, false>(c, dt);
+  else bc ? launch_rec<NCH, false, true>(c, dt) : launch_rec<NCH, false, false>(c, dt);
 }
 
 }  // namespace
@@ -336,9 +355,9 @@ namespace amps {
 int launch_deposit_cell_rec(amps_ctx* c, double dt) {
   const bool ex = c->dep_exact != 0;
   switch (c->dep_nch) {
-    case 4: ex ? launch_rec<4, true>(c, dt) : launch_rec<4, false>(c, dt); break;
-    case 6: ex ? launch_rec<6, true>(c, dt) : launch_rec<6, false>(c, dt); break;
-    case 12: ex ? launch_rec<12, true>(c, dt) : launch_rec<12, false>(c, dt); break;
+    case 4: launch_rec2<4>(c, dt); break;
+    case 6: launch_rec2<6>(c, dt); break;
+    case 12: launch_rec2<12>(c, dt); break;
     default:
       return fail(AMPS_E_ARG, "deposit_kernel 1 runs with deposit_chunks 4, 6 or 12");
   }

@@ -214,15 +214,21 @@ __device__ __forceinline__ int relocate(const MeshView& m, const Faces& faces, d
   return code;
 }
 
+// cell-centred linear stencil (center_device.h)
+static __device__ __noinline__ void center_B(const MeshView& m, const double* __restrict__ Bc, const double x[3],
+                                             int b, double B[3]);
+
 // One particle of PIC::Mover::Lapenta2017 (pic_mover_boris.cpp:876-1380) or
 // PIC::Mover::Boris (:126-553, rotation :253-277) with corner E and B from
-// LDS.  Returns the AMPS_CODE_*; xp/vp hold the committed state.
-template <int MOVER>
+// LDS, or with BC the cell-centred B of Bc (the centre-based B mode,
+// :979-1000).  Returns the AMPS_CODE_*; xp/vp hold the committed state.
+template <int MOVER, bool BC = false>
 __device__ __forceinline__ int push_particle(const MeshView& m, const Faces& faces,
                                              const BlockGeom& g, const double* sE,
                                              const double* sB, double q, double mass,
                                              double dtTotal, int bc_mode, double r_sphere,
-                                             double xp[3], double vp[3], int* blk, int* cell) {
+                                             double xp[3], double vp[3], int* blk, int* cell,
+                                             const double* Bc = nullptr) {
   double vInit[3] = {vp[0], vp[1], vp[2]}, xInit[3] = {xp[0], xp[1], xp[2]};
   double vFinal[3], xFinal[3];
   double wn[8], table[8];
@@ -237,9 +243,10 @@ __device__ __forceinline__ int push_particle(const MeshView& m, const Faces& fac
 #pragma unroll
     for (int d = 0; d < 3; d++) {
       E[d] += wn[l] * sE[3 * c + d];
-      B[d] += wn[l] * sB[3 * c + d];
+      if constexpr (!BC) B[d] += wn[l] * sB[3 * c + d];
     }
   }
+  if constexpr (BC) center_B(m, Bc, xInit, *blk, B);
   if (MOVER == AMPS_MOVER_LAPENTA) {
     // :1033-1081
     const double QdT_over_m = q * dtTotal / mass;

@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include "amps_common.h"
+#include "center_device.h"
 #include "pic_device.h"
 
 using namespace amps;
@@ -16,7 +17,7 @@ using namespace amps;
 // corner E and B (729 x 3 doubles each) are staged in LDS (SetBlock_E/B,
 // pic_mover.cpp:84-186), particles are read and written coalesced (SoA).
 // ======================================================================
-template <int MOVER>
+template <int MOVER, bool BC>
 __global__ __launch_bounds__(256) void k_push_f64(MeshView m, Faces faces, Species sp,
                                                    const double* __restrict__ E,
                                                    const double* __restrict__ B,
@@ -24,7 +25,7 @@ __global__ __launch_bounds__(256) void k_push_f64(MeshView m, Faces faces, Speci
                                                    const int* __restrict__ ntiles, int bc_mode,
                                                    double r_sphere, int64_t cap, double* x,
                                                    double* v, const uint8_t* __restrict__ spec,
-                                                   int* key, int8_t* code) {
+                                                   int* key, int8_t* code, const double* __restrict__ Bc) {
   __shared__ double sE[NCORNER * 3];
   __shared__ double sB[NCORNER * 3];
   const int t = blockIdx.x;
@@ -51,8 +52,8 @@ __global__ __launch_bounds__(256) void k_push_f64(MeshView m, Faces faces, Speci
       c = push_particle_rel(m, faces, b, g, sE, sB, E, B, sp.charge[s], sp.mass[s], sp.dt[s], bc_mode,
                             r_sphere, xp, vp, &nb_out, &cell_out);
     else
-      c = push_particle<MOVER>(m, faces, g, sE, sB, sp.charge[s], sp.mass[s], sp.dt[s], bc_mode,
-                               r_sphere, xp, vp, &nb_out, &cell_out);
+      c = push_particle<MOVER, BC>(m, faces, g, sE, sB, sp.charge[s], sp.mass[s], sp.dt[s], bc_mode,
+                                   r_sphere, xp, vp, &nb_out, &cell_out, Bc);
     x[p] = xp[0];
     x[cap + p] = xp[1];
     x[2 * cap + p] = xp[2];
@@ -64,18 +65,15 @@ __global__ __launch_bounds__(256) void k_push_f64(MeshView m, Faces faces, Speci
   }
 }
 
-template __global__ void k_push_f64<AMPS_MOVER_LAPENTA>(MeshView, Faces, Species, const double*,
-                                                         const double*, const Tile*, const int*,
-                                                         int, double, int64_t, double*, double*,
-                                                         const uint8_t*, int*, int8_t*);
-template __global__ void k_push_f64<AMPS_MOVER_BORIS>(MeshView, Faces, Species, const double*,
-                                                       const double*, const Tile*, const int*, int,
-                                                       double, int64_t, double*, double*,
-                                                       const uint8_t*, int*, int8_t*);
-template __global__ void k_push_f64<AMPS_MOVER_RELATIVISTIC>(MeshView, Faces, Species, const double*,
-                                                              const double*, const Tile*, const int*,
-                                                              int, double, int64_t, double*, double*,
-                                                              const uint8_t*, int*, int8_t*);
+#define INST_PUSH64(MV, BCV)                                                                    \
+  template __global__ void k_push_f64<MV, BCV>(MeshView, Faces, Species, const double*, const double*, \
+                                               const Tile*, const int*, int, double, int64_t, double*, \
+                                               double*, const uint8_t*, int*, int8_t*, const double*);
+INST_PUSH64(AMPS_MOVER_LAPENTA, false)
+INST_PUSH64(AMPS_MOVER_BORIS, false)
+INST_PUSH64(AMPS_MOVER_RELATIVISTIC, false)
+INST_PUSH64(AMPS_MOVER_LAPENTA, true)
+INST_PUSH64(AMPS_MOVER_BORIS, true)
 
 // FP32 push: float E/B stencil in LDS (17.5 KB), float velocities, FP64
 // positions.

@@ -60,7 +60,10 @@ int amps_gpu_device_info(amps_ctx* ctx, char* name, int* n_cu, int64_t* hbm_byte
  * (2,3,4,6 waves per deposit workgroup), "deposit_exact" (0/1: unfused
  * mul+add in the deposit accumulation for a bit-exact summation),
  * "deposit_kernel" (0: each lane evaluates its cell's particles itself,
- * 1 (default): particles evaluated once into shared-memory record batches). */
+ * 1 (default): particles evaluated once into shared-memory record batches),
+ * "b_center" (1: the FP64 Lapenta/Boris movers and the deposit take B from
+ * the cell-centred field of amps_gpu_set_center_fields, the centre-based B
+ * mode _PIC_FIELD_SOLVER_B_MODE_; 0 (default): B at the corners). */
 int amps_gpu_set_param(amps_ctx* ctx, const char* name, int64_t value);
 
 /* ---- mesh ------------------------------------------------------------ */

@@ -34,6 +34,7 @@ struct orc_mesh {
   orc_tnode* t;
   int* block_node;
   int* nb_info;  /* per block: minNeib, maxNeib, face[6], edge[12], corner[8] (28 ints) */
+  const double* Bc; /* centre-based B with ghost layer, or NULL */
 };
 
 static int new_tnode(orc_mesh* m, int up, const int64_t xg[3], int64_t size, int level) {
@@ -58,6 +59,7 @@ static int new_tnode(orc_mesh* m, int up, const int64_t xg[3], int64_t size, int
 }
 
 static void build_neighbour_info(orc_mesh* m);
+static void center_B_at(const orc_mesh* m, const double x[3], int tn, double B[3]);
 
 orc_mesh* orc_mesh_create(const double gxmin[3], const double gxmax[3], int lmax, int nblocks,
                           const int* level, const int64_t* ijk) {
@@ -303,6 +305,10 @@ static int push_one(const orc_mesh* m, const orc_face* faces, const double* E_co
       B[d] += w[l] * b[d];
     }
   }
+
+  /* centre-based B: CellCentered::Linear::InitStencil at the (nudged) xInit
+   * (pic_mover_boris.cpp:979-1000) */
+  if (m->Bc) center_B_at(m, xInit, startNode, B);
 
   if (mover == ORC_MOVER_LAPENTA) {
     /* :1033-1081 */
@@ -683,10 +689,11 @@ static const int CornerOff[8][3] = {{0, 0, 0}, {1, 0, 0}, {1, 1, 0}, {0, 1, 0},
 
 /* Per-cell accumulation: ProcessCell scalar path, pic_field_solver_ecsim.cpp:1882-2438
  * (normalized units: B_conv = length_conv = LightSpeed = 1, :193-206). */
-static void process_cell(const orc_tnode* n, const double* Bb, const double* charge,
+static void process_cell(const orc_mesh* m, int tn, const double* Bb, const double* charge,
                          const double* mass, const double* weight, double dtTotal,
                          int64_t np, const int64_t* idx, int64_t ntot, const double* x,
                          const double* v, const int* spec, double* out) {
+  const orc_tnode* n = &m->t[tn];
   double Jg[8][3], MM[8][8][9];
   memset(Jg, 0, sizeof(Jg));
   memset(MM, 0, sizeof(MM));
@@ -701,8 +708,12 @@ static void process_cell(const orc_tnode* n, const double* Bb, const double* cha
     double w[8], table[8], B[3] = {0.0, 0.0, 0.0};
     int cr[8];
     corner_stencil_node(n, xInit, w, table, cr);
-    for (int l = 0; l < 8; l++)
-      for (int d = 0; d < 3; d++) B[d] += w[l] * Bb[3 * cr[l] + d];
+    if (m->Bc) {
+      center_B_at(m, xInit, tn, B); /* :2097-2129, centre-based B */
+    } else {
+      for (int l = 0; l < 8; l++)
+        for (int d = 0; d < 3; d++) B[d] += w[l] * Bb[3 * cr[l] + d];
+    }
     double chargeQ = charge[s], m_ = mass[s];
     chargeQ *= weight[s];
     m_ *= weight[s];
@@ -784,8 +795,7 @@ int orc_deposit(const orc_mesh* m, const double* B_corner, const double* charge,
       int64_t np = start[c + 1] - start[c];
       if (np == 0) continue;
       int b = (int)(c / ORC_NCELL);
-      const orc_tnode* nd = &m->t[m->block_node[b]];
-      process_cell(nd, B_corner + (size_t)b * ORC_NCORNER * 3, charge, mass, weight, dt, np,
+      process_cell(m, m->block_node[b], B_corner + (size_t)b * ORC_NCORNER * 3, charge, mass, weight, dt, np,
                    idx + start[c], n, x, v, spec, buf + (c - c0) * CELL_VALS);
     }
     /* corner flush in cell order (:2362-2369, :2398-2426) */
@@ -1470,5 +1480,17 @@ void orc_interp_center(const orc_mesh* m, const double* Bc, int64_t n, const dou
       for (int d = 0; d < 3; d++) b[d] += s.w[i] * v[d];
     }
     for (int d = 0; d < 3; d++) B[d * n + p] = b[d];
+  }
+}
+
+void orc_set_center_field(orc_mesh* m, const double* Bc) { m->Bc = Bc; }
+
+static void center_B_at(const orc_mesh* m, const double x[3], int tn, double B[3]) {
+  orc_cst s;
+  cst_init(m, x, tn, &s);
+  B[0] = B[1] = B[2] = 0.0;
+  for (int i = 0; i < s.n; i++) {
+    const double* v = m->Bc + ((size_t)s.blk[i] * ORC_GN * ORC_GN * ORC_GN + (size_t)s.cell[i]) * 3;
+    for (int d = 0; d < 3; d++) B[d] += s.w[i] * v[d];
   }
 }

@@ -140,6 +140,11 @@ int orc_center_stencil(const orc_mesh* m, const double x[3], int block, int* st_
                        int* st_cell, double* st_w);
 void orc_interp_center(const orc_mesh* m, const double* Bc, int64_t n, const double* x,
                        const int* block, double* B, int nthreads);
+/* Centre-based B for the mover and the deposit (_PIC_FIELD_SOLVER_B_MODE_ =
+ * centre): when set (non-NULL, the caller keeps the array alive), orc_push's
+ * Lapenta/Boris movers and orc_deposit interpolate B with the cell-centred
+ * stencil instead of the corner values. */
+void orc_set_center_field(orc_mesh* m, const double* Bc);
 /* per-block neighbour data used by the stencil: min and max neighbour level,
  * and the neighbour block across each face (6), edge (12) and corner (8) */
 void orc_block_neighbours(const orc_mesh* m, int block, int* minmax, int* face, int* edge,

@@ -83,6 +83,7 @@ def lib():
         l.orc_center_stencil.argtypes = [vp, P(dbl), i32, P(C.c_int), P(C.c_int), P(dbl)]
         l.orc_interp_center.argtypes = [vp, P(dbl), i64, P(dbl), P(C.c_int), P(dbl), i32]
         l.orc_block_neighbours.argtypes = [vp, i32, P(C.c_int), P(C.c_int), P(C.c_int), P(C.c_int)]
+        l.orc_set_center_field.argtypes = [vp, P(dbl)]
         _lib = l
     return _lib
 
@@ -184,6 +185,12 @@ class Mesh:
         lib().orc_interp_center(self.h, _p(Bc, C.c_double), n, _p(x, C.c_double), _p(blk, C.c_int),
                                 _p(B, C.c_double), nthreads)
         return B
+
+    def set_center_field(self, Bc):
+        """Centre-based B (ghost-layered [nb][1000][3]) for push and deposit;
+        None restores the corner B."""
+        self._Bc = None if Bc is None else np.ascontiguousarray(Bc, np.float64)
+        lib().orc_set_center_field(self.h, None if self._Bc is None else _p(self._Bc, C.c_double))
 
     def neighbours(self, b):
         mm = np.zeros(2, np.int32)
