@@ -19,6 +19,11 @@ constexpr int CELL_VALS = 24 + 36 * 9; // per-cell J (8x3) + lower-triangular 8x
 // (i, j) depends only on the per-axis sums of their offset bits
 constexpr int CELL_VALS_MOMENT = 24 + 27 * 9;
 constexpr int NODE_VALS = 3 + 243;     // per-node J + mass-matrix row block
+// Record strides padded to whole 128-byte lines: a line that a store only
+// partly covers is first read from HBM, so records that start and end on line
+// boundaries and are written by full-line stores cost no reads.
+constexpr int CELL_STRIDE_MOMENT = 272;  // 17 lines
+constexpr int NODE_STRIDE = 256;         // 16 lines
 constexpr int MAX_SPEC = 8;
 
 // Device view of the block mesh (leaf blocks of the AMR octree).
@@ -150,6 +155,7 @@ void mlp_free(amps_ctx* c);
 int scan_exclusive(amps_ctx* c, int* data, int64_t n, int* total_out);
 // ECSIM cell pass with LDS record batches (deposit.hip)
 int launch_deposit_cell_rec(amps_ctx* c, double dt);
+int launch_deposit_cell_mfma(amps_ctx* c, double dt);
 }  // namespace amps
 
 #define HIP_TRY(expr)                                    \

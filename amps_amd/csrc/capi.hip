@@ -179,7 +179,8 @@ extern "C" int amps_gpu_device_info(amps_ctx* c, char* name, int* n_cu, int64_t*
 
 // chunk counts each deposit cell-pass kernel is instantiated for
 static bool deposit_variant_ok(int kernel, int64_t nch) {
-  if (kernel == 1) return nch == 4 || nch == 6 || nch == 12;
+  // kernel 2 keeps the chunk count for its exact mode, which runs kernel 1
+  if (kernel == 1 || kernel == 2) return nch == 4 || nch == 6 || nch == 12;
   return nch == 2 || nch == 3 || nch == 4 || nch == 6;
 }
 
@@ -193,7 +194,7 @@ extern "C" int amps_gpu_set_param(amps_ctx* c, const char* name, int64_t value) 
   } else if (n == "deposit_chunks") {
     if (!deposit_variant_ok(c->dep_kernel, value))
       return fail(AMPS_E_ARG, "deposit_chunks must be 2, 3, 4 or 6 (deposit_kernel 0) or 4, 6 or 12 "
-                              "(deposit_kernel 1)");
+                              "(deposit_kernel 1 and 2)");
     c->dep_nch = (int)value;
   } else if (n == "deposit_exact") {
     c->dep_exact = value ? 1 : 0;
@@ -201,7 +202,7 @@ extern "C" int amps_gpu_set_param(amps_ctx* c, const char* name, int64_t value) 
     if (value != 0 && value != 1) return fail(AMPS_E_ARG, "b_center must be 0 or 1");
     c->b_center = (int)value;
   } else if (n == "deposit_kernel") {
-    if (value != 0 && value != 1) return fail(AMPS_E_ARG, "deposit_kernel must be 0 or 1");
+    if (value < 0 || value > 2) return fail(AMPS_E_ARG, "deposit_kernel must be 0, 1 or 2");
     if (!deposit_variant_ok((int)value, c->dep_nch))
       return fail(AMPS_E_ARG, "deposit_kernel: set deposit_chunks to a count this kernel runs with first");
     c->dep_kernel = (int)value;
@@ -723,8 +724,8 @@ static int check_mover(const amps_ctx* c, const amps_push_opts* o, const char* w
     if (!c->has_center) return fail(AMPS_E_STATE, std::string(who) + ": b_center set but no cell-centred B given");
     if (c->precision == AMPS_FP32 || o->mover == AMPS_MOVER_RELATIVISTIC)
       return fail(AMPS_E_ARG, std::string(who) + ": cell-centred B runs with the FP64 Lapenta and Boris movers");
-    if (c->dep_kernel != 1)
-      return fail(AMPS_E_ARG, std::string(who) + ": cell-centred B runs with deposit_kernel 1");
+    if (c->dep_kernel == 0)
+      return fail(AMPS_E_ARG, std::string(who) + ": cell-centred B runs with deposit_kernel 1 or 2");
   }
   return 0;
 }
@@ -804,7 +805,10 @@ static void launch_dep(amps_ctx* c, double dt) {
 static int launch_deposit(amps_ctx* c, double dt) {
   tbegin(c);
   const bool ex = c->dep_exact != 0;
-  if (c->dep_kernel == 1) {
+  if (c->dep_kernel == 2 && !ex) {
+    if (int r = launch_deposit_cell_mfma(c, dt)) return r;
+  } else if (c->dep_kernel >= 1) {
+    // kernel 2 has no bit-exact form: its exact mode is kernel 1's
     if (int r = launch_deposit_cell_rec(c, dt)) return r;
   } else {
     switch (c->dep_nch) {
@@ -820,8 +824,8 @@ static int launch_deposit(amps_ctx* c, double dt) {
   tbegin(c);
   // a multiple of 8 workgroups (see xcd_tile); the surplus ones exit
   const int64_t node_wgs = ((c->nnodes + 3) / 4 + 7) / 8 * 8;
-  // the record-batched cell pass writes the moment form unless bit-exact
-  if (c->dep_kernel == 1 && !c->dep_exact)
+  // the record-batched and matrix-core cell passes write the moment form unless bit-exact
+  if (c->dep_kernel >= 1 && !c->dep_exact)
     hipLaunchKernelGGL(k_deposit_node<true>, dim3((unsigned)node_wgs), dim3(256), 0, c->stream,
                        c->nnodes, c->d_node_order, c->d_inc8, c->d_cellpart, c->d_nodeout);
   else
@@ -834,7 +838,7 @@ static int launch_deposit(amps_ctx* c, double dt) {
 
 static int ensure_deposit_buffers(amps_ctx* c) {
   if (!c->d_cellpart) HIP_TRY(hipMalloc(&c->d_cellpart, sizeof(double) * (size_t)c->mv.nb * NCELL * CELL_VALS));
-  if (!c->d_nodeout) HIP_TRY(hipMalloc(&c->d_nodeout, sizeof(double) * (size_t)c->nnodes * NODE_VALS));
+  if (!c->d_nodeout) HIP_TRY(hipMalloc(&c->d_nodeout, sizeof(double) * (size_t)c->nnodes * NODE_STRIDE));
   return 0;
 }
 
@@ -842,24 +846,24 @@ extern "C" int amps_gpu_update_j_mass_matrix(amps_ctx* c, double dt, double* J_o
   if (!c) return fail(AMPS_E_ARG, "null context");
   if (!c->has_mesh || !c->has_fields || !c->has_species)
     return fail(AMPS_E_STATE, "amps_gpu_update_j_mass_matrix: mesh, fields and species are required");
-  if (c->b_center && (!c->has_center || c->dep_kernel != 1))
-    return fail(AMPS_E_STATE, "amps_gpu_update_j_mass_matrix: b_center needs the cell-centred B and deposit_kernel 1");
+  if (c->b_center && (!c->has_center || c->dep_kernel == 0))
+    return fail(AMPS_E_STATE, "amps_gpu_update_j_mass_matrix: b_center needs the cell-centred B and deposit_kernel 1 or 2");
   (void)hipSetDevice(c->device);
   if (int r = ensure_deposit_buffers(c)) return r;
   if (c->cap == 0) {
-    HIP_TRY(hipMemsetAsync(c->d_nodeout, 0, sizeof(double) * (size_t)c->nnodes * NODE_VALS, c->stream));
+    HIP_TRY(hipMemsetAsync(c->d_nodeout, 0, sizeof(double) * (size_t)c->nnodes * NODE_STRIDE, c->stream));
   } else {
     if (!c->offsets_valid)
       if (int r = amps_gpu_compact(c)) return r;
     if (int r = launch_deposit(c, dt)) return r;
   }
   if (J_out || M_out) {
-    std::vector<double> h((size_t)c->nnodes * NODE_VALS);
+    std::vector<double> h((size_t)c->nnodes * NODE_STRIDE);
     HIP_TRY(hipStreamSynchronize(c->stream));
     HIP_TRY(hipMemcpy(h.data(), c->d_nodeout, sizeof(double) * h.size(), hipMemcpyDeviceToHost));
     for (int64_t i = 0; i < c->nnodes; i++) {
-      if (J_out) std::memcpy(J_out + 3 * i, &h[(size_t)i * NODE_VALS], 3 * sizeof(double));
-      if (M_out) std::memcpy(M_out + 243 * i, &h[(size_t)i * NODE_VALS + 3], 243 * sizeof(double));
+      if (J_out) std::memcpy(J_out + 3 * i, &h[(size_t)i * NODE_STRIDE], 3 * sizeof(double));
+      if (M_out) std::memcpy(M_out + 243 * i, &h[(size_t)i * NODE_STRIDE + 3], 243 * sizeof(double));
     }
   }
   return AMPS_OK;

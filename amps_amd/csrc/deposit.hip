@@ -37,6 +37,7 @@ struct Rec {
   static constexpr int ALPHA = EXACT ? 8 : 6;
   static constexpr int CQ = ALPHA + 9, MC = ALPHA + 10, VROT = ALPHA + 11;
   static constexpr int VALS = EXACT ? CELL_VALS : CELL_VALS_MOMENT;
+  static constexpr int STRIDE = EXACT ? CELL_VALS : CELL_STRIDE_MOMENT;
 };
 
 // corner c (InterpolationCoefficientTable order) -> its x, y, z offset bits
@@ -328,7 +329,7 @@ __global__ __launch_bounds__(64 * NCH) void k_deposit_cell_rec(MeshView m, Speci
     for (int i = threadIdx.x; i < 64 * nv1; i += blockDim.x) {
       const int c = i / nv1, j = i - c * nv1;
       const int key = keybase + (sx + (c & 3)) + NC * ((sy + ((c >> 2) & 3)) + NC * (sz + (c >> 4)));
-      cellpart[(size_t)key * V + v0 + j] = rec[c * SW + j];
+      cellpart[(size_t)key * R_::STRIDE + v0 + j] = rec[c * SW + j];
     }
     __syncthreads();
   }

@@ -403,17 +403,19 @@ __global__ __launch_bounds__(256) void k_deposit_node(int64_t nnodes,
                                                        const int* __restrict__ inc8,
                                                        const double* __restrict__ cellpart,
                                                        double* __restrict__ nodeout) {
-  constexpr int STRIDE = MOMENT ? CELL_VALS_MOMENT : CELL_VALS;
+  constexpr int STRIDE = MOMENT ? CELL_STRIDE_MOMENT : CELL_VALS;
   __shared__ __align__(16) short soff[8][256];
   static_assert(sizeof(soff) == 256 * 16, "one 16-byte row per thread");
   reinterpret_cast<int4*>(&soff[0][0])[threadIdx.x] =
       reinterpret_cast<const int4*>(MOMENT ? &c_soff_m[0][0] : &c_soff[0][0])[threadIdx.x];
   __syncthreads();
-  // consecutive nodes share incident cells: keep them on one XCD
-  const int64_t slot = (int64_t)xcd_tile() * 4 + (threadIdx.x >> 6);
+  // consecutive nodes share incident cells: keep them on one XCD.  The node,
+  // its incident cells and their records are the same for the whole
+  // wavefront, so their addresses live in scalar registers and each lane adds
+  // only its 32-bit offset.
+  const int64_t slot = (int64_t)xcd_tile() * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   if (slot >= nnodes) return;
   const int lane = threadIdx.x & 63;
-  // the node's incident cells (one 32-byte row) and its id: independent loads
   int ics[8];
 #pragma unroll
   for (int t = 0; t < 8; t++) ics[t] = inc8[slot * 8 + t];
@@ -430,7 +432,8 @@ __global__ __launch_bounds__(256) void k_deposit_node(int64_t nnodes,
 #pragma unroll
       for (int r = 0; r < 4; r++) {
         const int off = so[lane + 64 * r];
-        vv[t][r] = (off >= 0) ? P[off] : -0.0;
+        const double val = P[off < 0 ? 0 : off];
+        vv[t][r] = (off >= 0) ? val : -0.0;
       }
     }
   }
@@ -441,11 +444,12 @@ __global__ __launch_bounds__(256) void k_deposit_node(int64_t nnodes,
 #pragma unroll
       for (int r = 0; r < 4; r++) acc[r] += vv[t][r];
     }
-  double* o = nodeout + (size_t)node * NODE_VALS;
+  // the whole 2 KB record, padding included: four full-line stores
+  double* o = nodeout + (size_t)node * NODE_STRIDE;
 #pragma unroll
   for (int r = 0; r < 4; r++) {
     const int val = lane + 64 * r;
-    if (val < NODE_VALS) o[val] = acc[r];
+    o[val] = (val < NODE_VALS) ? acc[r] : 0.0;
   }
 }
 
@@ -561,6 +565,15 @@ __global__ __launch_bounds__(256) void k_generate(MeshView m, GenArgs ga, int64_
 
 __global__ __launch_bounds__(256) void k_copy_probe(const float4* __restrict__ a, float4* __restrict__ b,
                                                      int64_t n4) {
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n4; i += (int64_t)gridDim.x * blockDim.x)
-    b[i] = a[i];
+  // four independent 16-byte loads in flight per lane before the stores
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n4; i += 4 * stride) {
+    float4 t[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++)
+      if (i + u * stride < n4) t[u] = a[i + u * stride];
+#pragma unroll
+    for (int u = 0; u < 4; u++)
+      if (i + u * stride < n4) b[i + u * stride] = t[u];
+  }
 }

@@ -29,7 +29,8 @@ NODE_BYTES = 246 * 8      # per unique node: J (3) + mass matrix (243) written o
 COPY_PROBE_BYTES = 1 << 30
 # kernel identifiers of the counter passes -> report keys
 PMC_KERNELS = (("k_push_f64", "push"), ("k_push_f32", "push"), ("k_deposit_cell", "deposit_cell"),
-               ("k_deposit_cell_rec", "deposit_cell"), ("k_deposit_node", "deposit_node"),
+               ("k_deposit_cell_rec", "deposit_cell"), ("k_deposit_cell_mfma", "deposit_cell"),
+               ("k_deposit_node", "deposit_node"),
                ("k_scatter", "scatter"), ("k_count_keys", "count_keys"), ("k_copy_probe", "copy"))
 
 

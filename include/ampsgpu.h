@@ -57,10 +57,12 @@ int amps_gpu_device_info(amps_ctx* ctx, char* name, int* n_cu, int64_t* hbm_byte
 
 /* Tuning parameters (no reference counterpart; set by the autotuner):
  * "tile" (particles per push workgroup, 64..2^20), "deposit_chunks"
- * (2,3,4,6 waves per deposit workgroup), "deposit_exact" (0/1: unfused
- * mul+add in the deposit accumulation for a bit-exact summation),
+ * (waves per deposit workgroup: 2,3,4,6 for kernel 0, 4,6,12 for kernels 1
+ * and 2), "deposit_exact" (0/1: unfused mul+add in the deposit accumulation
+ * for a bit-exact summation; kernel 2 then runs kernel 1's exact form),
  * "deposit_kernel" (0: each lane evaluates its cell's particles itself,
- * 1 (default): particles evaluated once into shared-memory record batches),
+ * 1 (default): particles evaluated once into shared-memory record batches,
+ * 2: the per-cell moment sums as FP64 matrix-core products),
  * "b_center" (1: the FP64 Lapenta/Boris movers and the deposit take B from
  * the cell-centred field of amps_gpu_set_center_fields, the centre-based B
  * mode _PIC_FIELD_SOLVER_B_MODE_; 0 (default): B at the corners). */

@@ -130,7 +130,7 @@ def dep_case():
     return mesh, E, B, parts, sp, dt
 
 
-DEPOSIT_VARIANTS = [(1, 6), (1, 4), (1, 12), (0, 4), (0, 2), (0, 3), (0, 6)]
+DEPOSIT_VARIANTS = [(2, 12), (1, 6), (1, 4), (1, 12), (0, 4), (0, 2), (0, 3), (0, 6)]
 
 
 def set_deposit_variant(ctx, kernel, nch):
@@ -164,6 +164,41 @@ def test_deposit_parity(gpu_ctx, dep_case, exact, kernel, nch):
         assert nj.max() <= 1e-12 and nm.max() <= 1e-12, (nj.max(), nm.max())
     gpu_ctx.set_param("deposit_exact", 0)
     set_deposit_variant(gpu_ctx, 1, 6)
+
+
+@pytest.mark.parametrize("kernel", [2, 1])
+def test_deposit_ragged_cells(gpu_ctx, kernel):
+    """Empty cells, cells of 1-3 particles and cells of 65-80 (more than one
+    64-particle chunk of the matrix-core pass) side by side."""
+    mesh = M.central_octant_mesh()
+    E, B = M.corner_fields(mesh)
+    dt = 0.1 / (np.max(np.linalg.norm(B, axis=-1)) / 0.01)
+    sp = Species([-1.0, 1.0], [0.01, 1.0], [dt, dt])
+    full = M.per_cell_particles(mesh, [40, 40], [0.05, 0.005], drift=(0.01, 0.0, 0.0))
+    rng = np.random.default_rng(17)
+    key = full.blk.astype(np.int64) * 512 + full.cell
+    keep_p = np.array([0.0, 0.015, 0.3, 1.0])[rng.integers(0, 4, mesh.nblocks * 512)]
+    parts = full.take(np.flatnonzero(rng.random(full.n) < keep_p[key]))
+    counts = np.bincount(parts.blk.astype(np.int64) * 512 + parts.cell, minlength=mesh.nblocks * 512)
+    print(f"ragged deposit kernel={kernel} n={parts.n} empty={np.sum(counts == 0)} max={counts.max()}", flush=True)
+    assert np.sum(counts == 0) > 1000 and counts.max() > 64 and np.sum((counts > 0) & (counts < 4)) > 100
+    st = PicState(mesh, E, B, sp, ctx=gpu_ctx)
+    st.set_particles(parts)
+    set_deposit_variant(gpu_ctx, kernel, 12)
+    try:
+        J, Mm = st.UpdateJMassMatrix(dt)
+    finally:
+        set_deposit_variant(gpu_ctx, 1, 6)
+    g, _ = st.get_particles()
+    om = O.Mesh(mesh)
+    noc, keys = om.build_nodes()
+    Jo, Mo = O.deposit(om, B, sp, dt, g, noc, len(keys))
+    nj = np.linalg.norm(J - Jo, axis=1) / np.maximum(np.linalg.norm(Jo, axis=1), 1e-300)
+    nm = np.linalg.norm(Mm - Mo, axis=1) / np.maximum(np.linalg.norm(Mo, axis=1), 1e-300)
+    assert nj.max() <= 1e-12 and nm.max() <= 1e-12, (nj.max(), nm.max())
+    # nodes with no particle in any incident cell are exactly zero
+    zero = np.all(Mo == 0, axis=1)
+    assert np.all(Mm[zero] == 0) and np.all(J[zero] == 0)
 
 
 def test_deposit_after_push_and_migration(gpu_ctx, dep_case):
