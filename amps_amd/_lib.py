@@ -116,6 +116,10 @@ def load():
                                            P(dbl), P(i32), P(i32), P(C.c_int8), P(i32), P(i64)]),
         "amps_gpu_refine_cutoff": (C.c_int, [vp, P(TraceParams), i32, i64, P(dbl), P(dbl), P(dbl), P(dbl),
                                              dbl, dbl, i32, P(dbl), P(i32), P(i32), P(i64)]),
+        "amps_gpu_effective_cutoff": (C.c_int, [vp, P(TraceParams), i32, i64, P(dbl), P(dbl), i32, P(dbl),
+                                                P(C.c_int8), P(dbl), P(dbl), dbl, dbl, i32, P(dbl), P(dbl),
+                                                P(i32), P(i32), P(i64)]),
+        "amps_gpu_band_topology": (C.c_int, [i64, i32, P(C.c_int8), P(i32)]),
         "amps_gpu_trace_set_param": (C.c_int, [vp, C.c_char_p, i64]),
         "amps_gpu_mlp_init": (C.c_int, [vp, P(C.c_float)]),
         "amps_gpu_mlp_get_params": (C.c_int, [vp, P(C.c_float), P(i64)]),

@@ -37,6 +37,7 @@ struct MeshView {
   const int* nbinfo;         // min/max neighbour level and face/edge/corner neighbours, [nb*28]
   const double* blk_dxs;     // InitStencil cell size (xmax-xmin)/8, [nb*3]
   const int* leaf_of;        // [nf^3] finest-level lattice -> block id
+  const int* blk_lat;        // [nb*4] block origin on the level-lmax lattice, and its extent
 };
 
 struct Species {
@@ -81,6 +82,7 @@ struct amps_ctx {
   std::vector<double> h_xmin, h_xmax;
   double *d_xmin = nullptr, *d_xmax = nullptr, *d_dxc = nullptr, *d_dxs = nullptr;
   int* d_leaf = nullptr;
+  int* d_blk_lat = nullptr;
   int* d_level = nullptr;   // [nb] block refinement levels
   int* d_nbinfo = nullptr;  // [nb][28] min/max neighbour level, face[6], edge[12], corner[8] neighbours
   double* d_Bc = nullptr;   // [nb][10^3][3] cell-centred B with one ghost layer
@@ -122,6 +124,10 @@ struct amps_ctx {
   bool offsets_valid = false;  // d_offset describes the current order
   bool tiles_valid = false;    // d_tiles describes the current order
   int dep_nch = 12;            // deposit output chunks per cell (waves per workgroup)
+  int node_batch = 8;          // nodes per wavefront in the deposit node gather
+  int node_staged = 1;         // moment-record node gather staged through LDS
+  int dep_paired = 0;          // matrix-core cell pass: two wavefronts per SIMD in opposite phases
+  int push_waves = 1;          // 4: push kernels built for 4 waves per SIMD (128 VGPRs)
   int dep_exact = 0;           // 1: unfused mul+add in the deposit (bit-exact order)
   int dep_kernel = 1;          // cell pass: 0 per-lane particle loop, 1 LDS record batches
   int b_center = 0;            // 1: B from the cell-centred field (mover and deposit)

@@ -14,11 +14,11 @@
 using namespace amps;
 
 // kernels (pic_kernels.hip)
-template <int MOVER, bool BC>
+template <int MOVER, bool BC, int W>
 __global__ void k_push_f64(MeshView, Faces, Species, const double*, const double*, const Tile*,
                            const int*, int, double, int64_t, double*, double*, const uint8_t*,
                            int*, int8_t*, const double*);
-template <int MOVER>
+template <int MOVER, int W>
 __global__ void k_push_f32(MeshView, Faces, Species, const double*, const double*, const Tile*,
                            const int*, int, double, int64_t, double*, float*, const uint8_t*, int*,
                            int8_t*);
@@ -34,7 +34,8 @@ template <int NCH, bool EXACT>
 __global__ void k_deposit_cell(MeshView, Species, double, const double*, const int*, int64_t,
                                const double*, const double*, int, const uint8_t*, double*);
 template <bool MOMENT>
-__global__ void k_deposit_node(int64_t, const int*, const int*, const double*, double*);
+__global__ void k_deposit_node(int64_t, int, const int*, const int*, const double*, double*);
+__global__ void k_deposit_node_staged(int64_t, int, const int*, const int*, const double*, double*);
 __global__ void k_generate(MeshView, GenArgs, int64_t, double*, double*, uint8_t*, int*, int64_t*, int8_t*);
 __global__ void k_copy_probe(const float4*, float4*, int64_t);
 namespace amps {
@@ -95,6 +96,8 @@ static void free_mesh(amps_ctx* c) {
   (void)hipFree(c->d_dxc);
   (void)hipFree(c->d_dxs);
   (void)hipFree(c->d_leaf);
+  (void)hipFree(c->d_blk_lat);
+  c->d_blk_lat = nullptr;
   (void)hipFree(c->d_level);
   (void)hipFree(c->d_nbinfo);
   (void)hipFree(c->d_Bc);
@@ -196,6 +199,18 @@ extern "C" int amps_gpu_set_param(amps_ctx* c, const char* name, int64_t value) 
       return fail(AMPS_E_ARG, "deposit_chunks must be 2, 3, 4 or 6 (deposit_kernel 0) or 4, 6 or 12 "
                               "(deposit_kernel 1 and 2)");
     c->dep_nch = (int)value;
+  } else if (n == "push_waves") {
+    if (value != 1 && value != 4) return fail(AMPS_E_ARG, "push_waves must be 1 or 4");
+    c->push_waves = (int)value;
+  } else if (n == "deposit_paired") {
+    if (value != 0 && value != 1) return fail(AMPS_E_ARG, "deposit_paired must be 0 or 1");
+    c->dep_paired = (int)value;
+  } else if (n == "node_staged") {
+    if (value != 0 && value != 1) return fail(AMPS_E_ARG, "node_staged must be 0 or 1");
+    c->node_staged = (int)value;
+  } else if (n == "node_batch") {
+    if (value < 1 || value > 64) return fail(AMPS_E_ARG, "node_batch must be in [1, 64]");
+    c->node_batch = (int)value;
   } else if (n == "deposit_exact") {
     c->dep_exact = value ? 1 : 0;
   } else if (n == "b_center") {
@@ -387,6 +402,17 @@ extern "C" int amps_gpu_set_mesh(amps_ctx* c, const amps_mesh_desc* desc, const 
   HIP_TRY(hipMemcpy(c->d_dxc, dxc.data(), sizeof(double) * 3 * nb, hipMemcpyHostToDevice));
   HIP_TRY(hipMemcpy(c->d_dxs, dxs.data(), sizeof(double) * 3 * nb, hipMemcpyHostToDevice));
   HIP_TRY(hipMemcpy(c->d_leaf, leaf.data(), sizeof(int) * leaf.size(), hipMemcpyHostToDevice));
+  {
+    // block origin and extent on the level-lmax lattice (find_block_in)
+    std::vector<int> lat((size_t)nb * 4);
+    for (int b = 0; b < nb; b++) {
+      const int sh = m.lmax - level[b];
+      for (int d = 0; d < 3; d++) lat[4 * (size_t)b + d] = (int)(ijk[3 * (size_t)b + d] << sh);
+      lat[4 * (size_t)b + 3] = 1 << sh;
+    }
+    HIP_TRY(hipMalloc(&c->d_blk_lat, sizeof(int) * lat.size()));
+    HIP_TRY(hipMemcpy(c->d_blk_lat, lat.data(), sizeof(int) * lat.size(), hipMemcpyHostToDevice));
+  }
   const int64_t nkeys = (int64_t)nb * NCELL;
   HIP_TRY(hipMalloc(&c->d_offset, sizeof(int) * (nkeys + 1)));
   HIP_TRY(hipMalloc(&c->d_cursor, sizeof(int) * nkeys));
@@ -396,6 +422,7 @@ extern "C" int amps_gpu_set_mesh(amps_ctx* c, const amps_mesh_desc* desc, const 
   m.blk_dxc = c->d_dxc;
   m.blk_dxs = c->d_dxs;
   m.leaf_of = c->d_leaf;
+  m.blk_lat = c->d_blk_lat;
   // neighbours of every block for the cell-centred stencil: the block just
   // across each face, edge and corner (points 1/64 of the block outside it),
   // and the min/max level over sub-face, sub-edge and corner samples
@@ -732,33 +759,38 @@ static int check_mover(const amps_ctx* c, const amps_push_opts* o, const char* w
 
 static int launch_push(amps_ctx* c, const amps_push_opts* o) {
   const int k = c->cur;
+  const bool w4 = c->push_waves == 4;
+#define LAUNCH_PUSH32(MV, W)                                                                              \
+  hipLaunchKernelGGL((k_push_f32<MV, W>), dim3((unsigned)c->max_tiles), dim3(256), 0, c->stream, c->mv,     \
+                     c->faces, c->sp, c->d_E, c->d_B, c->d_tiles, c->d_ntiles, o->bc_mode, o->r_sphere, c->cap, \
+                     c->d_x[k], (float*)c->d_v[k], c->d_spec[k], c->d_key[k], c->d_code)
   if (c->precision == AMPS_FP32) {
-    if (o->mover == AMPS_MOVER_LAPENTA)
-      hipLaunchKernelGGL(k_push_f32<AMPS_MOVER_LAPENTA>, dim3((unsigned)c->max_tiles), dim3(256), 0,
-                         c->stream, c->mv, c->faces, c->sp, c->d_E, c->d_B, c->d_tiles, c->d_ntiles,
-                         o->bc_mode, o->r_sphere, c->cap, c->d_x[k], (float*)c->d_v[k], c->d_spec[k],
-                         c->d_key[k], c->d_code);
-    else
-      hipLaunchKernelGGL(k_push_f32<AMPS_MOVER_BORIS>, dim3((unsigned)c->max_tiles), dim3(256), 0,
-                         c->stream, c->mv, c->faces, c->sp, c->d_E, c->d_B, c->d_tiles, c->d_ntiles,
-                         o->bc_mode, o->r_sphere, c->cap, c->d_x[k], (float*)c->d_v[k], c->d_spec[k],
-                         c->d_key[k], c->d_code);
+    if (o->mover == AMPS_MOVER_LAPENTA) {
+      if (w4) LAUNCH_PUSH32(AMPS_MOVER_LAPENTA, 4);
+      else LAUNCH_PUSH32(AMPS_MOVER_LAPENTA, 1);
+    } else {
+      if (w4) LAUNCH_PUSH32(AMPS_MOVER_BORIS, 4);
+      else LAUNCH_PUSH32(AMPS_MOVER_BORIS, 1);
+    }
     HIP_TRY(hipGetLastError());
     return 0;
   }
-#define LAUNCH_PUSH64(MV, BCV)                                                                         \
-  hipLaunchKernelGGL((k_push_f64<MV, BCV>), dim3((unsigned)c->max_tiles), dim3(256), 0, c->stream, c->mv,     \
+#undef LAUNCH_PUSH32
+#define LAUNCH_PUSH64(MV, BCV, W)                                                                       \
+  hipLaunchKernelGGL((k_push_f64<MV, BCV, W>), dim3((unsigned)c->max_tiles), dim3(256), 0, c->stream, c->mv,  \
                      c->faces, c->sp, c->d_E, c->d_B, c->d_tiles, c->d_ntiles, o->bc_mode, o->r_sphere, c->cap, \
                      c->d_x[k], c->d_v[k], c->d_spec[k], c->d_key[k], c->d_code, c->d_Bc)
   const bool bc = c->b_center != 0;
   if (o->mover == AMPS_MOVER_LAPENTA) {
-    if (bc) LAUNCH_PUSH64(AMPS_MOVER_LAPENTA, true);
-    else LAUNCH_PUSH64(AMPS_MOVER_LAPENTA, false);
+    if (bc) LAUNCH_PUSH64(AMPS_MOVER_LAPENTA, true, 1);
+    else if (w4) LAUNCH_PUSH64(AMPS_MOVER_LAPENTA, false, 4);
+    else LAUNCH_PUSH64(AMPS_MOVER_LAPENTA, false, 1);
   } else if (o->mover == AMPS_MOVER_RELATIVISTIC) {
-    LAUNCH_PUSH64(AMPS_MOVER_RELATIVISTIC, false);
+    LAUNCH_PUSH64(AMPS_MOVER_RELATIVISTIC, false, 1);
   } else {
-    if (bc) LAUNCH_PUSH64(AMPS_MOVER_BORIS, true);
-    else LAUNCH_PUSH64(AMPS_MOVER_BORIS, false);
+    if (bc) LAUNCH_PUSH64(AMPS_MOVER_BORIS, true, 1);
+    else if (w4) LAUNCH_PUSH64(AMPS_MOVER_BORIS, false, 4);
+    else LAUNCH_PUSH64(AMPS_MOVER_BORIS, false, 1);
   }
 #undef LAUNCH_PUSH64
   HIP_TRY(hipGetLastError());
@@ -823,14 +855,18 @@ static int launch_deposit(amps_ctx* c, double dt) {
   tend(c, AMPS_K_DEPOSIT_CELL);
   tbegin(c);
   // a multiple of 8 workgroups (see xcd_tile); the surplus ones exit
-  const int64_t node_wgs = ((c->nnodes + 3) / 4 + 7) / 8 * 8;
+  const int64_t per_wg = 4 * (int64_t)c->node_batch;
+  const int64_t node_wgs = ((c->nnodes + per_wg - 1) / per_wg + 7) / 8 * 8;
   // the record-batched and matrix-core cell passes write the moment form unless bit-exact
-  if (c->dep_kernel >= 1 && !c->dep_exact)
+  if (c->dep_kernel >= 1 && !c->dep_exact && c->node_staged)
+    hipLaunchKernelGGL(k_deposit_node_staged, dim3((unsigned)node_wgs), dim3(256), 0, c->stream,
+                       c->nnodes, c->node_batch, c->d_node_order, c->d_inc8, c->d_cellpart, c->d_nodeout);
+  else if (c->dep_kernel >= 1 && !c->dep_exact)
     hipLaunchKernelGGL(k_deposit_node<true>, dim3((unsigned)node_wgs), dim3(256), 0, c->stream,
-                       c->nnodes, c->d_node_order, c->d_inc8, c->d_cellpart, c->d_nodeout);
+                       c->nnodes, c->node_batch, c->d_node_order, c->d_inc8, c->d_cellpart, c->d_nodeout);
   else
     hipLaunchKernelGGL(k_deposit_node<false>, dim3((unsigned)node_wgs), dim3(256), 0, c->stream,
-                       c->nnodes, c->d_node_order, c->d_inc8, c->d_cellpart, c->d_nodeout);
+                       c->nnodes, c->node_batch, c->d_node_order, c->d_inc8, c->d_cellpart, c->d_nodeout);
   HIP_TRY(hipGetLastError());
   tend(c, AMPS_K_DEPOSIT_NODE);
   return 0;

@@ -22,6 +22,11 @@
 // wavefront issues the MFMA steps cell by cell and writes a finished cell's
 // 267 values (J[8][3] / V, then M[27][9]) to its partial-sum record.  The next
 // chunk's particles are loaded while the current one is accumulated.
+//
+// PAIRED: eight wavefronts, two per SIMD (a z-layer's two row pairs), run
+// in opposite phases locked by workgroup barriers: while one builds the
+// records of its next chunk (VALU), the other issues the matrix products of
+// its current one (MFMA), so the two units overlap instead of taking turns.
 #include <hip/hip_runtime.h>
 
 #include "amps_common.h"
@@ -34,20 +39,22 @@ namespace {
 
 typedef double dbl4 __attribute__((ext_vector_type(4)));
 
-constexpr int MW = 4;       // wavefronts per workgroup: one per z-layer of the sub-tile
-constexpr int RS = 33;      // record stride in doubles (odd: lane-per-record stores hit distinct banks)
+// Records are kept field-major, rec[f * FS + particle]: the record stores
+// (lane = particle) are contiguous, and with FS = 66 the MFMA operand reads
+// (16 fields x 2 particles per 32-lane pass) fall on distinct banks.
+constexpr int NFLD = 32;
+constexpr int FS = 66;
 constexpr int F_A = 0;      // A operand rows 0-15 (row 15 stays zero)
 constexpr int F_ZERO = 15;
 constexpr int F_MA = 16;    // MC alpha[k], k = 0..8
 constexpr int F_MZ = 25;    // mz[tz], tz = 0..2
 constexpr int F_WYZ = 28;   // wy[by] wz[bz] at by + 2bz
-constexpr int F_ONE = 32;   // 1.0, written once
 constexpr int CS = CELL_STRIDE_MOMENT;  // cell record stride: 267 values + padding
 
-// LDS ordering between the lanes of one wavefront: DS instructions of a
-// wavefront complete in order; the wait and the clobber keep the compiler
-// from moving LDS accesses across.
-__device__ __forceinline__ void wave_lds_sync() { asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory"); }
+// LDS ordering between the lanes of one wavefront: the DS instructions of a
+// wavefront execute in issue order, so a compiler barrier that keeps LDS
+// accesses on their side is all that is needed.
+__device__ __forceinline__ void wave_lds_sync() { asm volatile("" ::: "memory"); }
 
 // a value every lane of the wavefront holds, kept in a scalar register
 __device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
@@ -55,8 +62,11 @@ __device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirs
 // InterpolationCoefficientTable corner index of offset bits (bx, by, bz)
 __device__ __forceinline__ int corner_of(int bx, int by, int bz) { return 4 * bz + (by ? (bx ? 2 : 3) : bx); }
 
-template <bool BC>
-__global__ __launch_bounds__(64 * MW) void k_deposit_cell_mfma(MeshView m, Species sp, double dtTotal,
+template <bool PAIRED>
+constexpr int waves_of() { return PAIRED ? 8 : 4; }
+
+template <bool BC, bool PAIRED>
+__global__ __launch_bounds__(64 * waves_of<PAIRED>()) void k_deposit_cell_mfma(MeshView m, Species sp, double dtTotal,
                                                                  const double* __restrict__ B,
                                                                  const int* __restrict__ offset,
                                                                  int64_t cap,
@@ -66,17 +76,22 @@ __global__ __launch_bounds__(64 * MW) void k_deposit_cell_mfma(MeshView m, Speci
                                                                  const uint8_t* __restrict__ spec,
                                                                  double* __restrict__ cellpart,
                                                                  const double* __restrict__ Bc) {
+  constexpr int MW = waves_of<PAIRED>();
+  constexpr int NCW = PAIRED ? 8 : 16;  // cells per wavefront
   __shared__ double sB[125 * 3];
-  __shared__ double rec[MW][64 * RS];
+  __shared__ double rec[MW][NFLD * FS];
   __shared__ double sconst[MAX_SPEC][3];  // chargeQ, QdT_over_2m, matrixConst
-  __shared__ int sfirst[MW][16], scum[MW][17];
+  __shared__ int sfirst[MW][NCW], scum[MW][NCW + 1];
   __shared__ double sstage[MW][CS];  // a finished cell's record, written out in full lines
+  __shared__ int smaxchunks;
   const int tile = xcd_tile();  // the 8 sub-tiles of a block on one XCD
   const int b = tile >> 3;
   const int st = tile & 7;
   const int sx = (st & 1) * 4, sy = ((st >> 1) & 1) * 4, sz = (st >> 2) * 4;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const int zc = sz + w;
+  // wavefront w: z-layer w % 4, rows row0.. of it (cell c: x = c % 4, y = row0 + c / 4)
+  const int zc = sz + (w & 3);
+  const int row0 = PAIRED ? 2 * (w >> 2) : 0;
   const int keybase = b * NCELL;
   if constexpr (!BC) {
     const double* Bb = B + (size_t)b * NCORNER * 3;
@@ -105,38 +120,36 @@ __global__ __launch_bounds__(64 * MW) void k_deposit_cell_mfma(MeshView m, Speci
     sconst[s][1] = QdT_over_2m;
     sconst[s][2] = chargeQ * QdT_over_2m / CellVolume;
   }
-  // this wavefront's 16 cells (cell c = cx + 4cy of layer zc): first particle, running counts
-  if (lane < 16) {
-    const int key = keybase + (sx + (lane & 3)) + NC * ((sy + (lane >> 2)) + NC * zc);
-    sfirst[w][lane] = offset[key];
-  }
-  if (lane < 17) {
+  auto cellkey = [&](int c) { return keybase + (sx + (c & 3)) + NC * ((sy + row0 + (c >> 2)) + NC * zc); };
+  // this wavefront's cells: first particle, running counts
+  if (lane < NCW) sfirst[w][lane] = offset[cellkey(lane)];
+  if (lane < NCW + 1) {
     int cum = 0;
-    for (int c = 0; c < lane; c++) {
-      const int key = keybase + (sx + (c & 3)) + NC * ((sy + (c >> 2)) + NC * zc);
-      cum += offset[key + 1] - offset[key];
-    }
+    for (int c = 0; c < lane; c++) cum += offset[cellkey(c) + 1] - offset[cellkey(c)];
     scum[w][lane] = cum;
   }
+  if (threadIdx.x == 0) smaxchunks = 0;
   __syncthreads();
   const int* cum = scum[w];
-  const int total = uniform(cum[16]);
+  const int total = uniform(cum[NCW]);
   double* R0 = rec[w];
 
   // operand selectors of this lane: A row ia = j and B columns j, 16 + j of particle slot lane / 16
   const int j = lane & 15, slot = lane >> 4;
   const int f0a = F_MZ + j / 9, f0b = F_MA + j % 9;
   int f1a, f1b;
+  bool f1w = false;  // column 27-30: the record field itself
   {
     const int col = 16 + j;
     if (col < 27) {
       f1a = F_MZ + col / 9, f1b = F_MA + col % 9;
     } else if (col < 31) {
-      f1a = F_WYZ + (col - 27), f1b = F_ONE;
+      f1a = F_WYZ + (col - 27), f1b = F_ZERO, f1w = true;
     } else {
       f1a = F_ZERO, f1b = F_ZERO;
     }
   }
+  const int oa = (F_A + j) * FS, o0a = f0a * FS, o0b = f0b * FS, o1a = f1a * FS, o1b = f1b * FS;
   // destinations of the 8 accumulator values; -1: unused.  v_mfma_f64_16x16x4f64
   // holds A[i][k] in lane i + 16k, B[k][j] in lane j + 16k and D[i][j] in
   // lane 16 (i % 4) + j, accumulator register i / 4.
@@ -161,35 +174,37 @@ __global__ __launch_bounds__(64 * MW) void k_deposit_cell_mfma(MeshView m, Speci
     }
   }
   const double invV = 1.0 / CellVolume;
-  R0[lane * RS + F_ONE] = 1.0;
   double* S = sstage[w];
   if (lane < CS - CELL_VALS_MOMENT) S[CELL_VALS_MOMENT + lane] = 0.0;
 
-  // the particle of the next chunk, loaded one chunk ahead
-  double nx[3] = {0.0, 0.0, 0.0}, nv[3] = {0.0, 0.0, 0.0};
-  int ns = 0, nc = 0;
-  auto fetch = [&](int pos) {
+  // each lane's particles of the next two chunks, loaded ahead
+  struct Pre {
+    double x[3], v[3];
+    int s, c;
+  };
+  auto fetch = [&](int pos, Pre& n) {
     if (pos < total) {
       int c = 0;
 #pragma unroll
-      for (int k = 1; k < 16; k++) c += (pos >= cum[k]) ? 1 : 0;
+      for (int k = 1; k < NCW; k++) c += (pos >= cum[k]) ? 1 : 0;
       const int p = sfirst[w][c] + (pos - cum[c]);
-      nc = c;
-      nx[0] = x[p], nx[1] = x[cap + p], nx[2] = x[2 * cap + p];
+      n.c = c;
+      n.x[0] = x[p], n.x[1] = x[cap + p], n.x[2] = x[2 * cap + p];
       if (vf32) {
         const float* vf = (const float*)v;
-        nv[0] = vf[p], nv[1] = vf[cap + p], nv[2] = vf[2 * cap + p];
+        n.v[0] = vf[p], n.v[1] = vf[cap + p], n.v[2] = vf[2 * cap + p];
       } else {
-        nv[0] = v[p], nv[1] = v[cap + p], nv[2] = v[2 * cap + p];
+        n.v[0] = v[p], n.v[1] = v[cap + p], n.v[2] = v[2 * cap + p];
       }
-      ns = spec[p];
+      n.s = spec[p];
     }
   };
+  Pre n1 = {{0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}, 0, 0}, n2 = n1;
 
   dbl4 C0 = {0.0, 0.0, 0.0, 0.0}, C1 = C0;
   int cur = 0;  // the cell being accumulated (uniform)
   auto flush = [&](int c) {
-    const int key = keybase + (sx + (c & 3)) + NC * ((sy + (c >> 2)) + NC * zc);
+    const int key = cellkey(c);
 #pragma unroll
     for (int r = 0; r < 4; r++) {
       if (oidx[r] >= 0) S[oidx[r]] = ojc[r] ? C0[r] * invV : C0[r];
@@ -205,16 +220,23 @@ __global__ __launch_bounds__(64 * MW) void k_deposit_cell_mfma(MeshView m, Speci
     wave_lds_sync();
   };
   // cells with no particles before the first particle
-  while (cur < 16 && uniform(cum[cur + 1]) == 0) flush(cur++);
+  while (cur < NCW && uniform(cum[cur + 1]) == 0) flush(cur++);
 
-  fetch(lane);
-  for (int base = 0; base < total; base += 64) {
+  fetch(lane, n1);
+  fetch(lane + 64, n2);
+  // the records of the chunk starting at particle base (VALU); the particles
+  // two chunks ahead are loaded meanwhile
+  auto build = [&](int base) {
     const int pos = base + lane;
     if (pos < total) {
-      const double xI[3] = {nx[0], nx[1], nx[2]};
-      const double vI[3] = {nv[0], nv[1], nv[2]};
-      const int s = ns;
-      const int iX[3] = {sx + (nc & 3), sy + (nc >> 2), zc};
+      // the record: the reference's expressions, contracted to FMA where
+      // the compiler finds them (this pass is the tolerance-bound one)
+#pragma clang fp contract(fast)
+      const double xI[3] = {n1.x[0], n1.x[1], n1.x[2]};
+      const double vI[3] = {n1.v[0], n1.v[1], n1.v[2]};
+      const int s = n1.s;
+      const int nc = n1.c;
+      const int iX[3] = {sx + (nc & 3), sy + row0 + (nc >> 2), zc};
       double Lf[6], xn[3];
 #pragma unroll
       for (int d = 0; d < 3; d++) {
@@ -283,27 +305,32 @@ __global__ __launch_bounds__(64 * MW) void k_deposit_cell_mfma(MeshView m, Speci
         mo[d][1] = Lf[2 * d] * Lf[2 * d + 1];
         mo[d][2] = Lf[2 * d + 1] * Lf[2 * d + 1];
       }
-      double* R = R0 + lane * RS;
+      double* R = R0 + lane;
+#define FLD(f) R[(f) * FS]
 #pragma unroll
       for (int ty = 0; ty < 3; ty++)
 #pragma unroll
-        for (int tx = 0; tx < 3; tx++) R[F_A + tx + 3 * ty] = mo[0][tx] * mo[1][ty];
+        for (int tx = 0; tx < 3; tx++) FLD(F_A + tx + 3 * ty) = mo[0][tx] * mo[1][ty];
 #pragma unroll
       for (int bx = 0; bx < 2; bx++)
 #pragma unroll
-        for (int d = 0; d < 3; d++) R[F_A + 9 + 3 * bx + d] = Lf[bx] * qv[d];
-      R[F_ZERO] = 0.0;
+        for (int d = 0; d < 3; d++) FLD(F_A + 9 + 3 * bx + d) = Lf[bx] * qv[d];
+      FLD(F_ZERO) = 0.0;
 #pragma unroll
-      for (int k = 0; k < 9; k++) R[F_MA + k] = MC * alpha[k];
+      for (int k = 0; k < 9; k++) FLD(F_MA + k) = MC * alpha[k];
 #pragma unroll
-      for (int tz = 0; tz < 3; tz++) R[F_MZ + tz] = mo[2][tz];
+      for (int tz = 0; tz < 3; tz++) FLD(F_MZ + tz) = mo[2][tz];
 #pragma unroll
       for (int bz = 0; bz < 2; bz++)
 #pragma unroll
-        for (int by = 0; by < 2; by++) R[F_WYZ + by + 2 * bz] = Lf[2 + by] * Lf[4 + bz];
+        for (int by = 0; by < 2; by++) FLD(F_WYZ + by + 2 * bz) = Lf[2 + by] * Lf[4 + bz];
+#undef FLD
     }
-    fetch(pos + 64);  // in flight while this chunk is accumulated
-    wave_lds_sync();
+    n1 = n2;
+    fetch(pos + 128, n2);  // in flight while two chunks are accumulated
+  };
+  // the matrix products of that chunk, cell by cell, and the finished cells' records
+  auto accumulate = [&](int base) {
     const int cend = (base + 64 < total) ? base + 64 : total;
     int q = base;
     while (q < cend) {
@@ -311,27 +338,27 @@ __global__ __launch_bounds__(64 * MW) void k_deposit_cell_mfma(MeshView m, Speci
       const int se = ce < cend ? ce : cend;
       // full steps of 4 particles, the operands of step t + 1 loaded while step t runs
       const int nfull = (se - q) >> 2;
-      const double* R = R0 + (q - base + slot) * RS;
+      const double* R = R0 + (q - base + slot);
       if (nfull > 0) {
-        double a = R[F_A + j], b0 = R[f0a] * R[f0b], b1 = R[f1a] * R[f1b];
+        double a = R[oa], b0 = R[o0a] * R[o0b], b1 = R[o1a] * (f1w ? 1.0 : R[o1b]);
         for (int t = 1; t < nfull; t++) {
-          R += 4 * RS;
-          const double a2 = R[F_A + j], b02 = R[f0a] * R[f0b], b12 = R[f1a] * R[f1b];
+          R += 4;
+          const double a2 = R[oa], b02 = R[o0a] * R[o0b], b12 = R[o1a] * (f1w ? 1.0 : R[o1b]);
           C0 = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b0, C0, 0, 0, 0);
           C1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b1, C1, 0, 0, 0);
           a = a2, b0 = b02, b1 = b12;
         }
         C0 = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b0, C0, 0, 0, 0);
         C1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b1, C1, 0, 0, 0);
-        R += 4 * RS;
+        R += 4;
       }
       if (q + 4 * nfull < se) {
         // the cell's last 1-3 particles in this chunk: the other slots add zero
         double a = 0.0, b0 = 0.0, b1 = 0.0;
         if (q + 4 * nfull + slot < se) {
-          a = R[F_A + j];
-          b0 = R[f0a] * R[f0b];
-          b1 = R[f1a] * R[f1b];
+          a = R[oa];
+          b0 = R[o0a] * R[o0b];
+          b1 = R[o1a] * (f1w ? 1.0 : R[o1b]);
         }
         C0 = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b0, C0, 0, 0, 0);
         C1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b1, C1, 0, 0, 0);
@@ -339,18 +366,43 @@ __global__ __launch_bounds__(64 * MW) void k_deposit_cell_mfma(MeshView m, Speci
       q = se;
       if (q == ce) {
         flush(cur++);
-        while (cur < 16 && uniform(cum[cur + 1]) == q) flush(cur++);  // empty cells
+        while (cur < NCW && uniform(cum[cur + 1]) == q) flush(cur++);  // empty cells
       }
     }
-    wave_lds_sync();  // the records are read before the next chunk overwrites them
+  };
+  if constexpr (!PAIRED) {
+    for (int base = 0; base < total; base += 64) {
+      build(base);
+      wave_lds_sync();
+      accumulate(base);
+      wave_lds_sync();  // the records are read before the next chunk overwrites them
+    }
+  } else {
+    // phase p: waves 0-3 build chunk p/2 at even p and accumulate it at odd
+    // p; waves 4-7 (the other wavefront of each SIMD) one phase later
+    const int nch = (total + 63) / 64;
+    if (lane == 0) atomicMax(&smaxchunks, nch);
+    __syncthreads();
+    const int grp = w >> 2;
+    const int nphase = 2 * smaxchunks + 1;
+    for (int ph = 0; ph < nphase; ph++) {
+      const int k = ph - grp;
+      if (k >= 0 && (k >> 1) < nch) {
+        if ((k & 1) == 0)
+          build(64 * (k >> 1));
+        else
+          accumulate(64 * (k >> 1));
+      }
+      __syncthreads();
+    }
   }
 }
 
-template <bool BC>
+template <bool BC, bool PAIRED>
 void launch(amps_ctx* c, double dt) {
   const int k = c->cur;
-  hipLaunchKernelGGL((k_deposit_cell_mfma<BC>), dim3((unsigned)c->mv.nb * 8), dim3(64 * MW), 0, c->stream, c->mv,
-                     c->sp, dt, c->d_B, c->d_offset, c->cap, c->d_x[k], c->d_v[k],
+  hipLaunchKernelGGL((k_deposit_cell_mfma<BC, PAIRED>), dim3((unsigned)c->mv.nb * 8), dim3(64 * waves_of<PAIRED>()), 0,
+                     c->stream, c->mv, c->sp, dt, c->d_B, c->d_offset, c->cap, c->d_x[k], c->d_v[k],
                      c->precision == AMPS_FP32 ? 1 : 0, c->d_spec[k], c->d_cellpart, c->d_Bc);
 }
 
@@ -358,10 +410,11 @@ void launch(amps_ctx* c, double dt) {
 
 namespace amps {
 int launch_deposit_cell_mfma(amps_ctx* c, double dt) {
+  const bool paired = c->dep_paired != 0;
   if (c->b_center)
-    launch<true>(c, dt);
+    paired ? launch<true, true>(c, dt) : launch<true, false>(c, dt);
   else
-    launch<false>(c, dt);
+    paired ? launch<false, true>(c, dt) : launch<false, false>(c, dt);
   return 0;
 }
 }  // namespace amps

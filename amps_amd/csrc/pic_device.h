@@ -97,6 +97,42 @@ __device__ __forceinline__ int find_block(const MeshView& m, const double x[3]) 
   return res;
 }
 
+// The tile's block on the level-lmax lattice: find_block_in() returns it
+// without reading leaf_of when the lattice point lies in its range (the
+// lookup would return it), and find_cell_in() takes its bounds from the
+// block geometry already in registers (the same values find_cell reads).
+struct BlockLat {
+  int b, lo[3], hi[3];
+};
+
+__device__ __forceinline__ void load_block_lat(const MeshView& m, int b, BlockLat& l) {
+  l.b = b;
+  const int ext = m.blk_lat[4 * b + 3];
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    l.lo[d] = m.blk_lat[4 * b + d];
+    l.hi[d] = l.lo[d] + ext;
+  }
+}
+
+__device__ __forceinline__ int find_block_in(const MeshView& m, const double x[3], const BlockGeom& g,
+                                             const BlockLat& l) {
+  int ix[3];
+#pragma unroll
+  for (int d = 0; d < 3; d++) ix[d] = (int)floor((x[d] - m.gxmin[d]) / m.dx_maxref[d]);
+  bool in = true;
+#pragma unroll
+  for (int d = 0; d < 3; d++) in = in && ix[d] >= l.lo[d] && ix[d] < l.hi[d];
+  if (!in) return find_block(m, x);
+  bool flag = false;
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    if (x[d] < g.xmin[d]) ix[d]--, flag = true;
+    if (x[d] >= g.xmax[d]) ix[d]++, flag = true;
+  }
+  return flag ? lookup_leaf(m, ix) : l.b;
+}
+
 // FindCellIndex — meshAMRgeneric.h:2256-2323 (ExitFlag=false).
 __device__ __forceinline__ int find_cell(const MeshView& m, const double x[3], int b) {
   int ijk[3];
@@ -111,11 +147,27 @@ __device__ __forceinline__ int find_cell(const MeshView& m, const double x[3], i
   return ijk[0] + NC * (ijk[1] + NC * ijk[2]);
 }
 
+__device__ __forceinline__ int find_cell_in(const MeshView& m, const double x[3], int b, const BlockGeom& g,
+                                            const BlockLat& l) {
+  if (b != l.b) return find_cell(m, x, b);
+  int ijk[3];
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    const double xmn = g.xmin[d], xmx = g.xmax[d];
+    if ((x[d] < xmn) || (xmx < x[d])) return -1;
+    int i = (int)((x[d] - xmn) / g.dxc[d]);
+    if (i == NC) i = NC - 1;
+    ijk[d] = clampi(i, 0, NC - 1);
+  }
+  return ijk[0] + NC * (ijk[1] + NC * ijk[2]);
+}
+
 // Sphere, domain-boundary and block/cell relocation of a pushed particle
 // (pic_mover_boris.cpp:1088-1375), shared by the FP64 and FP32 pushes.  On
 // entry xInit/vInit hold the start state, xFinal/vFinal the pushed state;
 // xp/vp receive the committed state.
-__device__ __forceinline__ int relocate(const MeshView& m, const Faces& faces, double dtTotal,
+__device__ __forceinline__ int relocate(const MeshView& m, const Faces& faces, const BlockGeom& g,
+                                        const BlockLat& lat, double dtTotal,
                                         int bc_mode, double r_sphere, double xInit[3],
                                         double vInit[3], double xFinal[3], double vFinal[3],
                                         double xp[3], double vp[3], int* blk, int* cell) {
@@ -131,7 +183,7 @@ __device__ __forceinline__ int relocate(const MeshView& m, const Faces& faces, d
     for (int d = 0; d < 3; d++) xp[d] = xFinal[d], vp[d] = vFinal[d];
     return AMPS_CODE_SPHERE;
   }
-  int newNode = find_block(m, xFinal);
+  int newNode = find_block_in(m, xFinal, g, lat);
   int code = AMPS_CODE_FINISHED;
   if (newNode < 0) {
     if (bc_mode == AMPS_BC_DELETE) {
@@ -205,7 +257,7 @@ __device__ __forceinline__ int relocate(const MeshView& m, const Faces& faces, d
     for (int d = 0; d < 3; d++) vFinal[d] = vInit[d], xFinal[d] = xInit[d];
     code = AMPS_CODE_REFLECTED;
   }
-  const int c_id = find_cell(m, xFinal, newNode);
+  const int c_id = find_cell_in(m, xFinal, newNode, g, lat);
   if (c_id < 0) return AMPS_CODE_LEFT_DOMAIN;
   *blk = newNode;
   *cell = c_id;
@@ -224,7 +276,7 @@ static __device__ __noinline__ void center_B(const MeshView& m, const double* __
 // :979-1000).  Returns the AMPS_CODE_*; xp/vp hold the committed state.
 template <int MOVER, bool BC = false>
 __device__ __forceinline__ int push_particle(const MeshView& m, const Faces& faces,
-                                             const BlockGeom& g, const double* sE,
+                                             const BlockGeom& g, const BlockLat& lat, const double* sE,
                                              const double* sB, double q, double mass,
                                              double dtTotal, int bc_mode, double r_sphere,
                                              double xp[3], double vp[3], int* blk, int* cell,
@@ -307,7 +359,7 @@ __device__ __forceinline__ int push_particle(const MeshView& m, const Faces& fac
     for (int d = 0; d < 3; d++) xFinal[d] = xInit[d] + dtTemp * vFinal[d];
   }
 
-  return relocate(m, faces, dtTotal, bc_mode, r_sphere, xInit, vInit, xFinal, vFinal, xp, vp, blk,
+  return relocate(m, faces, g, lat, dtTotal, bc_mode, r_sphere, xInit, vInit, xFinal, vFinal, xp, vp, blk,
                   cell);
 }
 
@@ -501,7 +553,7 @@ __device__ __forceinline__ int push_particle_rel(const MeshView& m, const Faces&
 // boundary handling run on the FP64 position (relocate()).
 template <int MOVER>
 __device__ __forceinline__ int push_particle_f32(const MeshView& m, const Faces& faces,
-                                                 const BlockGeom& g, const float* sE,
+                                                 const BlockGeom& g, const BlockLat& lat, const float* sE,
                                                  const float* sB, float q, float mass,
                                                  double dtTotal, int bc_mode, double r_sphere,
                                                  double xp[3], float vf[3], int* blk, int* cell) {
@@ -572,7 +624,7 @@ __device__ __forceinline__ int push_particle_f32(const MeshView& m, const Faces&
   for (int d = 0; d < 3; d++) xFinal[d] = xInit[d] + (double)(dt * vF[d]);
   double vo[3];
   const int code =
-      relocate(m, faces, dtTotal, bc_mode, r_sphere, xInit, vInit, xFinal, vFinal, xp, vo, blk, cell);
+      relocate(m, faces, g, lat, dtTotal, bc_mode, r_sphere, xInit, vInit, xFinal, vFinal, xp, vo, blk, cell);
 #pragma unroll
   for (int d = 0; d < 3; d++) vf[d] = (float)vo[d];
   return code;

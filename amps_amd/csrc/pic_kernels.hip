@@ -17,8 +17,9 @@ using namespace amps;
 // corner E and B (729 x 3 doubles each) are staged in LDS (SetBlock_E/B,
 // pic_mover.cpp:84-186), particles are read and written coalesced (SoA).
 // ======================================================================
-template <int MOVER, bool BC>
-__global__ __launch_bounds__(256) void k_push_f64(MeshView m, Faces faces, Species sp,
+// W: the occupancy the register allocation aims at (waves per SIMD).
+template <int MOVER, bool BC, int W>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W))) void k_push_f64(MeshView m, Faces faces, Species sp,
                                                    const double* __restrict__ E,
                                                    const double* __restrict__ B,
                                                    const Tile* __restrict__ tiles,
@@ -41,6 +42,8 @@ __global__ __launch_bounds__(256) void k_push_f64(MeshView m, Faces faces, Speci
   __syncthreads();
   BlockGeom g;
   load_block_geom(m, b, g);
+  BlockLat lat;
+  load_block_lat(m, b, lat);
   const int p1 = tl.start + tl.count;
   for (int p = tl.start + threadIdx.x; p < p1; p += blockDim.x) {
     double xp[3] = {x[p], x[cap + p], x[2 * cap + p]};
@@ -52,7 +55,7 @@ __global__ __launch_bounds__(256) void k_push_f64(MeshView m, Faces faces, Speci
       c = push_particle_rel(m, faces, b, g, sE, sB, E, B, sp.charge[s], sp.mass[s], sp.dt[s], bc_mode,
                             r_sphere, xp, vp, &nb_out, &cell_out);
     else
-      c = push_particle<MOVER, BC>(m, faces, g, sE, sB, sp.charge[s], sp.mass[s], sp.dt[s], bc_mode,
+      c = push_particle<MOVER, BC>(m, faces, g, lat, sE, sB, sp.charge[s], sp.mass[s], sp.dt[s], bc_mode,
                                    r_sphere, xp, vp, &nb_out, &cell_out, Bc);
     x[p] = xp[0];
     x[cap + p] = xp[1];
@@ -65,20 +68,22 @@ __global__ __launch_bounds__(256) void k_push_f64(MeshView m, Faces faces, Speci
   }
 }
 
-#define INST_PUSH64(MV, BCV)                                                                    \
-  template __global__ void k_push_f64<MV, BCV>(MeshView, Faces, Species, const double*, const double*, \
-                                               const Tile*, const int*, int, double, int64_t, double*, \
-                                               double*, const uint8_t*, int*, int8_t*, const double*);
-INST_PUSH64(AMPS_MOVER_LAPENTA, false)
-INST_PUSH64(AMPS_MOVER_BORIS, false)
-INST_PUSH64(AMPS_MOVER_RELATIVISTIC, false)
-INST_PUSH64(AMPS_MOVER_LAPENTA, true)
-INST_PUSH64(AMPS_MOVER_BORIS, true)
+#define INST_PUSH64(MV, BCV, W)                                                                  \
+  template __global__ void k_push_f64<MV, BCV, W>(MeshView, Faces, Species, const double*, const double*, \
+                                                  const Tile*, const int*, int, double, int64_t, double*, \
+                                                  double*, const uint8_t*, int*, int8_t*, const double*);
+INST_PUSH64(AMPS_MOVER_LAPENTA, false, 1)
+INST_PUSH64(AMPS_MOVER_BORIS, false, 1)
+INST_PUSH64(AMPS_MOVER_LAPENTA, false, 4)
+INST_PUSH64(AMPS_MOVER_BORIS, false, 4)
+INST_PUSH64(AMPS_MOVER_RELATIVISTIC, false, 1)
+INST_PUSH64(AMPS_MOVER_LAPENTA, true, 1)
+INST_PUSH64(AMPS_MOVER_BORIS, true, 1)
 
 // FP32 push: float E/B stencil in LDS (17.5 KB), float velocities, FP64
 // positions.
-template <int MOVER>
-__global__ __launch_bounds__(256) void k_push_f32(MeshView m, Faces faces, Species sp,
+template <int MOVER, int W>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W))) void k_push_f32(MeshView m, Faces faces, Species sp,
                                                    const double* __restrict__ E,
                                                    const double* __restrict__ B,
                                                    const Tile* __restrict__ tiles,
@@ -101,13 +106,15 @@ __global__ __launch_bounds__(256) void k_push_f32(MeshView m, Faces faces, Speci
   __syncthreads();
   BlockGeom g;
   load_block_geom(m, b, g);
+  BlockLat lat;
+  load_block_lat(m, b, lat);
   const int p1 = tl.start + tl.count;
   for (int p = tl.start + threadIdx.x; p < p1; p += blockDim.x) {
     double xp[3] = {x[p], x[cap + p], x[2 * cap + p]};
     float vp[3] = {v[p], v[cap + p], v[2 * cap + p]};
     const int s = spec[p];
     int nb_out = b, cell_out = 0;
-    const int c = push_particle_f32<MOVER>(m, faces, g, sE, sB, (float)sp.charge[s], (float)sp.mass[s],
+    const int c = push_particle_f32<MOVER>(m, faces, g, lat, sE, sB, (float)sp.charge[s], (float)sp.mass[s],
                                            sp.dt[s], bc_mode, r_sphere, xp, vp, &nb_out, &cell_out);
     x[p] = xp[0];
     x[cap + p] = xp[1];
@@ -120,14 +127,14 @@ __global__ __launch_bounds__(256) void k_push_f32(MeshView m, Faces faces, Speci
   }
 }
 
-template __global__ void k_push_f32<AMPS_MOVER_LAPENTA>(MeshView, Faces, Species, const double*,
-                                                         const double*, const Tile*, const int*,
-                                                         int, double, int64_t, double*, float*,
-                                                         const uint8_t*, int*, int8_t*);
-template __global__ void k_push_f32<AMPS_MOVER_BORIS>(MeshView, Faces, Species, const double*,
-                                                       const double*, const Tile*, const int*, int,
-                                                       double, int64_t, double*, float*,
-                                                       const uint8_t*, int*, int8_t*);
+#define INST_PUSH32(MV, W)                                                                        \
+  template __global__ void k_push_f32<MV, W>(MeshView, Faces, Species, const double*, const double*, \
+                                             const Tile*, const int*, int, double, int64_t, double*,  \
+                                             float*, const uint8_t*, int*, int8_t*);
+INST_PUSH32(AMPS_MOVER_LAPENTA, 1)
+INST_PUSH32(AMPS_MOVER_BORIS, 1)
+INST_PUSH32(AMPS_MOVER_LAPENTA, 4)
+INST_PUSH32(AMPS_MOVER_BORIS, 4)
 
 // ======================================================================
 // Migration: counting sort by key = block*512 + cell with wavefront
@@ -398,7 +405,7 @@ __constant__ __align__(16) short c_soff[8][256];
 __constant__ __align__(16) short c_soff_m[8][256];
 
 template <bool MOMENT>
-__global__ __launch_bounds__(256) void k_deposit_node(int64_t nnodes,
+__global__ __launch_bounds__(256) void k_deposit_node(int64_t nnodes, int npw,
                                                        const int* __restrict__ node_order,
                                                        const int* __restrict__ inc8,
                                                        const double* __restrict__ cellpart,
@@ -413,9 +420,11 @@ __global__ __launch_bounds__(256) void k_deposit_node(int64_t nnodes,
   // its incident cells and their records are the same for the whole
   // wavefront, so their addresses live in scalar registers and each lane adds
   // only its 32-bit offset.
-  const int64_t slot = (int64_t)xcd_tile() * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  if (slot >= nnodes) return;
+  // Each wavefront takes npw consecutive nodes of the processing order.
+  const int64_t first = ((int64_t)xcd_tile() * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6)) * npw;
+  const int64_t last = (first + npw < nnodes) ? first + npw : nnodes;
   const int lane = threadIdx.x & 63;
+  for (int64_t slot = first; slot < last; slot++) {
   int ics[8];
 #pragma unroll
   for (int t = 0; t < 8; t++) ics[t] = inc8[slot * 8 + t];
@@ -451,10 +460,88 @@ __global__ __launch_bounds__(256) void k_deposit_node(int64_t nnodes,
     const int val = lane + 64 * r;
     o[val] = (val < NODE_VALS) ? acc[r] : 0.0;
   }
+  }
 }
 
-template __global__ void k_deposit_node<false>(int64_t, const int*, const int*, const double*, double*);
-template __global__ void k_deposit_node<true>(int64_t, const int*, const int*, const double*, double*);
+template __global__ void k_deposit_node<false>(int64_t, int, const int*, const int*, const double*, double*);
+template __global__ void k_deposit_node<true>(int64_t, int, const int*, const int*, const double*, double*);
+
+// Node gather for the moment record, staged.  A node that is corner q of an
+// incident cell needs 75 of the cell's 267 values: J of corner q and the
+// moments t = b_q + b_j of its 8 neighbours j, which lie in four runs of 18
+// consecutive values.  The wavefront first copies the 8 x 75 values its node
+// needs into LDS with loads whose lanes walk those runs (a handful of cache
+// lines per load instead of one per lane), then each lane sums its outputs
+// from LDS in ascending cell order.  Same operands, same order: the result is
+// bit-identical to k_deposit_node<true>.
+// c_gtab[q][w]: offset in the cell record of gathered value w (w < 72: run
+// w / 18 of the 2x2x2 moment block above b_q, w >= 72: J of corner q).
+// c_wtab[q][val]: the gathered value that output val takes from such a cell,
+// -1 where the cell does not touch that slot (val >= 246: padding).
+__constant__ __align__(16) short c_gtab[8][80];
+__constant__ __align__(16) short c_wtab[8][256];
+
+constexpr int NODE_G = 75;  // values one incident cell gives a node
+
+__global__ __launch_bounds__(256) void k_deposit_node_staged(int64_t nnodes, int npw,
+                                                             const int* __restrict__ node_order,
+                                                             const int* __restrict__ inc8,
+                                                             const double* __restrict__ cellpart,
+                                                             double* __restrict__ nodeout) {
+  __shared__ __align__(16) short wtab[8][256];
+  __shared__ __align__(16) short gtab[8][80];
+  __shared__ double G[4][8 * NODE_G];
+  reinterpret_cast<int4*>(&wtab[0][0])[threadIdx.x] = reinterpret_cast<const int4*>(&c_wtab[0][0])[threadIdx.x];
+  if (threadIdx.x < 8 * 80 / 8)
+    reinterpret_cast<int4*>(&gtab[0][0])[threadIdx.x] = reinterpret_cast<const int4*>(&c_gtab[0][0])[threadIdx.x];
+  __syncthreads();
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int64_t first = ((int64_t)xcd_tile() * 4 + wv) * npw;
+  const int64_t last = (first + npw < nnodes) ? first + npw : nnodes;
+  const int lane = threadIdx.x & 63;
+  double* Gw = G[wv];
+  // this lane's gathered values e = lane + 64i: incident cell e / 75, value e % 75
+  constexpr int NI = (8 * NODE_G + 63) / 64;
+  for (int64_t slot = first; slot < last; slot++) {
+    int ics[8];
+#pragma unroll
+    for (int t = 0; t < 8; t++) ics[t] = inc8[slot * 8 + t];
+    const int64_t node = node_order[slot];
+#pragma unroll
+    for (int i = 0; i < NI; i++) {
+      const int e = lane + 64 * i;
+      const int t0 = (64 * i) / NODE_G;  // the lanes of step i cover cells t0 and t0 + 1
+      if (e < 8 * NODE_G) {
+        const bool hi = t0 + 1 < 8 && e >= NODE_G * (t0 + 1);
+        const int ic = hi ? ics[t0 + 1 < 8 ? t0 + 1 : 7] : ics[t0];
+        const int w = e - NODE_G * (hi ? t0 + 1 : t0);
+        if (ic >= 0) Gw[e] = cellpart[(size_t)(ic >> 3) * CELL_STRIDE_MOMENT + gtab[ic & 7][w]];
+      }
+    }
+    asm volatile("" ::: "memory");  // the DS ops of a wavefront execute in issue order
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int t = 0; t < 8; t++) {
+      const int ic = ics[t];
+      if (ic >= 0) {
+        const short* wt = wtab[ic & 7];
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+          const int w = wt[lane + 64 * r];
+          const double val = Gw[NODE_G * t + (w < 0 ? 0 : w)];
+          acc[r] += (w >= 0) ? val : -0.0;
+        }
+      }
+    }
+    asm volatile("" ::: "memory");
+    double* o = nodeout + (size_t)node * NODE_STRIDE;
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      const int val = lane + 64 * r;
+      o[val] = (val < NODE_VALS) ? acc[r] : 0.0;
+    }
+  }
+}
 
 namespace amps {
 int upload_inv_index_matrix(hipStream_t s) {
@@ -488,6 +575,35 @@ int upload_inv_index_matrix(hipStream_t s) {
       soffm[q][val] = (short)offm;
     }
   }
+  // staged gather tables for the moment record
+  short gtab[8][80], wtab[8][256];
+  for (int q = 0; q < 8; q++) {
+    for (int w = 0; w < 80; w++) {
+      int off = 0;
+      if (w < 72) {
+        const int run = w / 18, o = w % 18;
+        off = 24 + 9 * (CB[q][0] + 3 * (CB[q][1] + (run & 1)) + 9 * (CB[q][2] + (run >> 1))) + o;
+      } else if (w < 75) {
+        off = 3 * q + (w - 72);
+      }
+      gtab[q][w] = (short)off;
+    }
+    int inv[27];
+    for (int s2 = 0; s2 < 27; s2++) inv[s2] = -1;
+    for (int j = 0; j < 8; j++) inv[IM[q][j]] = j;
+    for (int val = 0; val < 256; val++) {
+      int w = -1;
+      if (val < 3) {
+        w = 72 + val;
+      } else if (val < NODE_VALS) {
+        const int s2 = (val - 3) / 9, k = (val - 3) % 9, j = inv[s2];
+        if (j >= 0) w = 18 * (CB[j][1] + 2 * CB[j][2]) + 9 * CB[j][0] + k;
+      }
+      wtab[q][val] = (short)w;
+    }
+  }
+  HIP_TRY(hipMemcpyToSymbolAsync(HIP_SYMBOL(c_gtab), gtab, sizeof(gtab), 0, hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyToSymbolAsync(HIP_SYMBOL(c_wtab), wtab, sizeof(wtab), 0, hipMemcpyHostToDevice, s));
   HIP_TRY(hipMemcpyToSymbolAsync(HIP_SYMBOL(c_soff), soff, sizeof(soff), 0, hipMemcpyHostToDevice, s));
   HIP_TRY(hipMemcpyToSymbolAsync(HIP_SYMBOL(c_soff_m), soffm, sizeof(soffm), 0, hipMemcpyHostToDevice, s));
   return 0;

@@ -763,6 +763,224 @@ extern "C" int amps_gpu_refine_cutoff(amps_ctx* c, const amps_trace_params* P, i
   return AMPS_OK;
 }
 
+// AnalyzeCutoffBandSamples (CutoffBandSearch.h:96-202) for n state vectors:
+// the lower and upper transition brackets, transition and allowed-interval
+// counts and the range/unresolved flags.  An untraced sample (-1) counts as
+// unresolved.
+extern "C" int amps_gpu_band_topology(int64_t n, int32_t nR, const int8_t* states, int32_t* out) {
+  if (n < 0 || nR < 0 || nR > 4096) return fail(AMPS_E_RANGE, "band_topology: n or nR out of range");
+  if (n > 0 && (!states || !out)) return fail(AMPS_E_ARG, "band_topology: null array");
+  constexpr int8_t FORB = 0, ALLOW = 1;
+  for (int64_t i = 0; i < n; i++) {
+    const int8_t* s = states + i * (int64_t)nR;
+    auto st = [&](int k) { return (s[k] == FORB || s[k] == ALLOW) ? s[k] : (int8_t)2; };
+    int lfi = -1, lai = -1, ufi = -1, uai = -1, first_allowed = -1, last_forbidden = -1;
+    int ntr = 0, nint = 0, nunres = 0;
+    bool all_allowed = false, all_forbidden = false, lower_below = false, lower_above = false;
+    bool upper_below = false, upper_above = false, lower_unres = false, upper_unres = false;
+    if (nR > 0) {
+      bool in_allowed = false;
+      for (int k = 0; k < nR; k++) {
+        const int8_t v = st(k);
+        if (v == ALLOW) {
+          if (first_allowed < 0) first_allowed = k;
+          if (!in_allowed) ++nint, in_allowed = true;
+        } else {
+          in_allowed = false;
+          if (v == FORB) last_forbidden = k;
+          else ++nunres;
+        }
+        if (k > 0) {
+          const int8_t a = st(k - 1);
+          if (a != 2 && v != 2 && a != v) ++ntr;
+        }
+      }
+      all_allowed = first_allowed == 0 && last_forbidden < 0 && nunres == 0;
+      all_forbidden = first_allowed < 0 && last_forbidden == nR - 1 && nunres == 0;
+      if (first_allowed == 0) {
+        lower_below = true;
+      } else if (first_allowed > 0) {
+        bool clean = true;
+        for (int k = 0; k < first_allowed && clean; k++) clean = st(k) == FORB;
+        if (clean) lfi = first_allowed - 1, lai = first_allowed;
+        else lower_unres = true;
+      } else if (nunres > 0) {
+        lower_unres = true;
+      } else {
+        lower_above = true;
+      }
+      if (last_forbidden < 0) {
+        if (nunres > 0) upper_unres = true;
+        else upper_below = true;
+      } else if (last_forbidden == nR - 1) {
+        upper_above = true;
+      } else {
+        bool clean = true;
+        for (int k = last_forbidden + 1; k < nR && clean; k++) clean = st(k) == ALLOW;
+        if (clean) ufi = last_forbidden, uai = last_forbidden + 1;
+        else upper_unres = true;
+      }
+    }
+    int32_t* o = out + 15 * i;
+    o[0] = lfi, o[1] = lai, o[2] = ufi, o[3] = uai, o[4] = ntr, o[5] = nint, o[6] = nunres;
+    o[7] = all_allowed, o[8] = all_forbidden, o[9] = lower_below, o[10] = lower_above;
+    o[11] = upper_below, o[12] = upper_above, o[13] = lower_unres, o[14] = upper_unres;
+  }
+  return AMPS_OK;
+}
+
+// IntegrateEffectiveCutoff (CutoffBandSearch.h:284-366) for n (location,
+// direction) pairs.  Every coarse interval of [lower, upper] whose end states
+// differ is bisected; the bisections of all pairs advance together, one
+// batched trace of their midpoints per round.  The widths are then summed per
+// pair in interval order, stopping where the reference returns early (an
+// unresolved end state or midpoint), so counts and widths match the serial
+// routine.
+extern "C" int amps_gpu_effective_cutoff(amps_ctx* c, const amps_trace_params* P, int32_t precision,
+                                         int64_t n, const double* x0, const double* dir, int32_t nR,
+                                         const double* grid, const int8_t* states, const double* lower,
+                                         const double* upper, double abs_tol, double rel_tol,
+                                         int32_t trace_limit_forbidden, double* effective,
+                                         double* allowed_width, int32_t* refined, int32_t* status,
+                                         int64_t* steps_total) {
+  if (!c) return fail(AMPS_E_ARG, "null context");
+  if (n < 0 || n > ((int64_t)1 << 28) || nR < 0 || nR > 4096)
+    return fail(AMPS_E_RANGE, "effective_cutoff: n or nR out of range");
+  if (steps_total) *steps_total = 0;
+  if (n == 0) return AMPS_OK;
+  if (!x0 || !dir || !grid || !states || !lower || !upper || !effective || !allowed_width || !refined || !status)
+    return fail(AMPS_E_ARG, "effective_cutoff: null array");
+  constexpr int8_t FORB = 0, ALLOW = 1, UNRES = 2;  // CutoffSampleState
+  bool grid_ok = nR >= 2;
+  for (int k = 0; k + 1 < nR; k++) grid_ok = grid_ok && grid[k + 1] > grid[k];
+  // one bisection per straddling interval that the serial routine reaches
+  struct Bis {
+    int64_t pair;
+    int k;
+    double lo, hi, tol;
+    int8_t low_state;
+    int count;
+    bool unresolved, done;
+  };
+  std::vector<Bis> bis;
+  std::vector<int> stop_at((size_t)n, -1);  // first interval with an unresolved end state
+  for (int64_t i = 0; i < n; i++) {
+    effective[i] = -1.0;
+    allowed_width[i] = 0.0;
+    refined[i] = 0;
+    status[i] = 0;
+    const double lo_i = lower[i], up_i = upper[i];
+    if (!grid_ok || !(lo_i >= 0.0) || !(up_i >= lo_i) || up_i == lo_i) continue;
+    const int8_t* st = states + i * (int64_t)nR;
+    for (int k = 0; k + 1 < nR; k++) {
+      const double a = std::max(grid[k], lo_i), b = std::min(grid[k + 1], up_i);
+      if (!(b > a)) continue;
+      const int8_t l = st[k], r = st[k + 1];
+      if (l == UNRES || r == UNRES || l < 0 || r < 0) {
+        stop_at[(size_t)i] = k;
+        break;
+      }
+      if (l == r) continue;
+      const double tol = std::max(abs_tol, rel_tol * std::max(std::fabs(grid[k]), std::fabs(grid[k + 1])));
+      bis.push_back({i, k, grid[k], grid[k + 1], tol, l, 0, false, false});
+    }
+  }
+  int64_t st_total = 0;
+  std::vector<double> bx, bd, bR;
+  std::vector<amps_trace_result> res;
+  std::vector<size_t> act;
+  for (size_t q = 0; q < bis.size(); q++) act.push_back(q);
+  while (!act.empty()) {
+    std::vector<size_t> run;
+    bx.clear(), bd.clear(), bR.clear();
+    for (size_t q : act) {
+      Bis& B = bis[q];
+      if (!((B.hi - B.lo) > B.tol)) {
+        B.done = true;
+        continue;
+      }
+      run.push_back(q);
+      for (int d = 0; d < 3; d++) bx.push_back(x0[3 * B.pair + d]), bd.push_back(dir[3 * B.pair + d]);
+      bR.push_back(0.5 * (B.lo + B.hi));
+    }
+    if (run.empty()) break;
+    res.resize(run.size());
+    int64_t st = 0;
+    if (int r = amps_gpu_trace_batch(c, P, precision, (int64_t)run.size(), bx.data(), bd.data(), bR.data(),
+                                     res.data(), &st))
+      return r;
+    st_total += st;
+    act.clear();
+    for (size_t k = 0; k < run.size(); k++) {
+      Bis& B = bis[run[k]];
+      B.count++;
+      const int term = res[k].termination;
+      int8_t ms;
+      if (term == 0) ms = ALLOW;
+      else if (term == 1 || term == 2 || (trace_limit_forbidden && term >= 3 && term <= 5)) ms = FORB;
+      else ms = UNRES;
+      if (ms == UNRES) {
+        B.unresolved = true;
+        B.done = true;
+        if (term >= 6) status[B.pair] = 2;  // numerical failure after retry (the reference throws)
+        continue;
+      }
+      if (ms == B.low_state) B.lo = bR[k];
+      else B.hi = bR[k];
+      act.push_back(run[k]);
+    }
+  }
+  // serial accumulation in interval order
+  size_t q = 0;
+  for (int64_t i = 0; i < n; i++) {
+    const double lo_i = lower[i], up_i = upper[i];
+    if (!grid_ok || !(lo_i >= 0.0) || !(up_i >= lo_i)) continue;
+    if (up_i == lo_i) {
+      effective[i] = lo_i;
+      continue;
+    }
+    const int8_t* st = states + i * (int64_t)nR;
+    bool unres = false;
+    double width = 0.0;
+    int count = 0;
+    for (int k = 0; k + 1 < nR && !unres; k++) {
+      const double a = std::max(grid[k], lo_i), b = std::min(grid[k + 1], up_i);
+      if (!(b > a)) continue;
+      if (k == stop_at[(size_t)i]) {
+        unres = true;
+        break;
+      }
+      const int8_t l = st[k], r = st[k + 1];
+      if (l == r) {
+        if (l == ALLOW) width += b - a;
+        continue;
+      }
+      const Bis& B = bis[q++];
+      count += B.count;
+      if (B.unresolved) {
+        unres = true;
+        break;
+      }
+      const double tr = 0.5 * (B.lo + B.hi);
+      if (l == ALLOW)
+        width += std::max(0.0, std::min(tr, b) - a);
+      else
+        width += std::max(0.0, b - std::max(tr, a));
+    }
+    // the bisections queued beyond the stop (none: the queue stops at it) are skipped
+    while (q < bis.size() && bis[q].pair == i) q++;
+    allowed_width[i] = width;
+    refined[i] = count;
+    if (unres) {
+      if (status[i] == 0) status[i] = 1;
+      continue;
+    }
+    effective[i] = std::max(lo_i, std::min(up_i, up_i - width));
+  }
+  if (steps_total) *steps_total = st_total;
+  return AMPS_OK;
+}
+
 extern "C" int amps_gpu_cutoff_scan(amps_ctx* c, const amps_trace_params* P, int32_t precision,
                                     int32_t mode, int64_t nloc, const double* x0,
                                     const double* dir, int32_t nR, const double* grid,

@@ -222,5 +222,112 @@ def refine_cutoff(ctx, params, x0, dirs, r_forbid, r_allow, abs_tol=1.0e-3, rel_
     return cut, it, stt, steps.value
 
 
+def effective_cutoff(ctx, params, x0, dirs, grid, states, lower, upper, abs_tol=1.0e-3, rel_tol=1.0e-6,
+                     trace_limit_forbidden=True, precision=L.FP64):
+    """IntegrateEffectiveCutoff (CutoffBandSearch.h:284-366) for many
+    (location, direction) pairs: the penumbra's allowed width between the
+    refined lower and upper cutoffs, each straddling interval of the scan
+    grid bisected (batched).  Returns (effective_GV, allowed_width_GV,
+    refined, status, steps); effective is -1 where the integration is
+    unresolved (status 1).  A numerical failure after the retry raises, as
+    the reference's classifier throws."""
+    x0 = np.ascontiguousarray(x0, np.float64).reshape(-1, 3)
+    dirs = np.ascontiguousarray(dirs, np.float64).reshape(-1, 3)
+    grid = np.ascontiguousarray(grid, np.float64)
+    states = np.ascontiguousarray(states, np.int8).reshape(len(x0), len(grid))
+    lo = np.ascontiguousarray(lower, np.float64)
+    up = np.ascontiguousarray(upper, np.float64)
+    n = len(lo)
+    eff = np.zeros(n)
+    width = np.zeros(n)
+    ref = np.zeros(n, np.int32)
+    stt = np.zeros(n, np.int32)
+    steps = C.c_int64()
+    L.check(ctx.lib.amps_gpu_effective_cutoff(ctx.h, C.byref(params), precision, n, L._ptr(x0, C.c_double),
+                                              L._ptr(dirs, C.c_double), len(grid), L._ptr(grid, C.c_double),
+                                              L._ptr(states, C.c_int8), L._ptr(lo, C.c_double),
+                                              L._ptr(up, C.c_double), abs_tol, rel_tol,
+                                              1 if trace_limit_forbidden else 0, L._ptr(eff, C.c_double),
+                                              L._ptr(width, C.c_double), L._ptr(ref, C.c_int32),
+                                              L._ptr(stt, C.c_int32), C.byref(steps)), "effective_cutoff")
+    if np.any(stt == 2):
+        raise L.AmpsGpuError(f"effective_cutoff: numerical failure after retry in {int(np.sum(stt == 2))} pairs")
+    return eff, width, ref, stt, steps.value
+
+
+TOPOLOGY_FIELDS = ("lowerForbiddenIndex", "lowerAllowedIndex", "upperForbiddenIndex", "upperAllowedIndex",
+                   "nTransitions", "nAllowedIntervals", "nUnresolved", "allAllowed", "allForbidden",
+                   "lowerBelowRange", "lowerAboveRange", "upperBelowRange", "upperAboveRange",
+                   "lowerBracketUnresolved", "upperBracketUnresolved")
+
+
+def band_topology(states):
+    """AnalyzeCutoffBandSamples (CutoffBandSearch.h:96-202) for each row of
+    states (0 physically forbidden, 1 allowed, 2 unresolved): [n, 15] int32
+    in TOPOLOGY_FIELDS order."""
+    st = np.ascontiguousarray(states, np.int8)
+    st2 = st.reshape(-1, st.shape[-1])
+    out = np.zeros((len(st2), 15), np.int32)
+    L.check(L.load().amps_gpu_band_topology(len(st2), st2.shape[1], L._ptr(st2, C.c_int8),
+                                            L._ptr(out, C.c_int32)), "band_topology")
+    return out
+
+
+def penumbra_scan(ctx, params, x0, dirs, rmin_gv, rmax_gv, nscan=48, abs_tol=1.0e-3, rel_tol=1.0e-6,
+                  precision=L.FP64):
+    """The gridless PENUMBRA_SCAN cutoff for many (location, direction)
+    pairs (CutoffForDirectionPenumbraScan_GV, CutoffRigidityGridless.cpp:
+    2704-2772): the full scan of the search grid, the band topology, the
+    lower and upper transitions refined by bisection, and the effective
+    cutoff integrated over the penumbra, each stage batched over all pairs.
+    Trace limits count as forbidden (traceLimitPolicy FORBIDDEN).  Returns a
+    dict of per-pair arrays: lower_GV, upper_GV, effective_GV (-1 where not
+    determined) and the topology counts and flags."""
+    x0 = np.ascontiguousarray(x0, np.float64).reshape(-1, 3)
+    dirs = np.ascontiguousarray(dirs, np.float64).reshape(-1, 3)
+    n = len(x0)
+    out = {"lower_GV": np.full(n, -1.0), "upper_GV": np.full(n, -1.0), "effective_GV": np.full(n, -1.0)}
+    if rmax_gv < rmin_gv:
+        return out
+    grid = build_search_grid(rmin_gv, rmax_gv, nscan)
+    states, _, steps = cutoff_scan(ctx, params, x0, dirs, grid, mode=SCAN_FULL, precision=precision)
+    topo = band_topology(states)
+    for k, f in enumerate(TOPOLOGY_FIELDS):
+        out[f] = topo[:, k].copy()
+    lfi, lai, ufi, uai = topo[:, 0], topo[:, 1], topo[:, 2], topo[:, 3]
+    lower_unres = topo[:, 13].astype(bool)
+    upper_unres = topo[:, 14].astype(bool)
+    out["lower_GV"][topo[:, 9] == 1] = grid[0]
+    sel = np.flatnonzero((topo[:, 9] == 0) & (lfi >= 0) & (lai >= 0))
+    if len(sel):
+        c, _, stt, s2 = refine_cutoff(ctx, params, x0[sel], dirs[sel], grid[lfi[sel]], grid[lai[sel]], abs_tol,
+                                      rel_tol, precision=precision)
+        out["lower_GV"][sel] = c
+        lower_unres[sel[stt != 0]] = True
+        steps += s2
+    out["upper_GV"][topo[:, 7] == 1] = grid[0]
+    sel = np.flatnonzero((topo[:, 7] == 0) & (ufi >= 0) & (uai >= 0))
+    if len(sel):
+        c, _, stt, s2 = refine_cutoff(ctx, params, x0[sel], dirs[sel], grid[ufi[sel]], grid[uai[sel]], abs_tol,
+                                      rel_tol, precision=precision)
+        out["upper_GV"][sel] = c
+        upper_unres[sel[stt != 0]] = True
+        steps += s2
+    out["lowerBracketUnresolved"] = lower_unres.astype(np.int32)
+    out["upperBracketUnresolved"] = upper_unres.astype(np.int32)
+    lo, up = out["lower_GV"], out["upper_GV"]
+    sel = np.flatnonzero((lo >= 0.0) & (up >= lo) & (topo[:, 6] == 0) & ~lower_unres & ~upper_unres)
+    if len(sel):
+        eff, _, _, stt, s3 = effective_cutoff(ctx, params, x0[sel], dirs[sel], grid, states[sel], lo[sel], up[sel],
+                                              abs_tol, rel_tol, precision=precision)
+        ok = stt == 0
+        out["effective_GV"][sel[ok]] = eff[ok]
+        steps += s3
+    out["steps"] = steps
+    out["grid"] = grid
+    out["states"] = states
+    return out
+
+
 def set_trace_param(ctx, name, value):
     L.check(ctx.lib.amps_gpu_trace_set_param(ctx.h, name.encode(), int(value)), f"trace param {name}")

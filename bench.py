@@ -30,7 +30,7 @@ COPY_PROBE_BYTES = 1 << 30
 # kernel identifiers of the counter passes -> report keys
 PMC_KERNELS = (("k_push_f64", "push"), ("k_push_f32", "push"), ("k_deposit_cell", "deposit_cell"),
                ("k_deposit_cell_rec", "deposit_cell"), ("k_deposit_cell_mfma", "deposit_cell"),
-               ("k_deposit_node", "deposit_node"),
+               ("k_deposit_node", "deposit_node"), ("k_deposit_node_staged", "deposit_node"),
                ("k_scatter", "scatter"), ("k_count_keys", "count_keys"), ("k_copy_probe", "copy"))
 
 
@@ -44,7 +44,9 @@ def _pmc_pass(args, tune, counter, outdir):
     cmd = ["timeout", "-s", "KILL", "240", "rocprofv3", "--pmc", counter, "--output-format", "csv",
            "-d", outdir, "-o", "pmc", "--", sys.executable, os.path.abspath(__file__), "--pmc-child",
            "--workload", "pic", "--particles-log2", str(args.particles_log2), "--tile", str(tune["tile"]),
-           "--deposit-kernel", str(tune["deposit_kernel"]), "--deposit-chunks", str(tune["deposit_chunks"])]
+           "--deposit-kernel", str(tune["deposit_kernel"]), "--deposit-chunks", str(tune["deposit_chunks"]),
+           "--node-batch", str(args.node_batch), "--push-waves", str(args.push_waves),
+           "--node-staged", str(args.node_staged), "--deposit-paired", str(args.deposit_paired)]
     env = dict(os.environ, TMPDIR="/tmp")
     r = subprocess.run(cmd, cwd="/tmp", env=env, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)
     files = glob.glob(os.path.join(outdir, "**", "*counter_collection.csv"), recursive=True)
@@ -284,6 +286,10 @@ def main():
     ap.add_argument("--tile", type=int, default=0, help="push tile (skips the tuner with the two below)")
     ap.add_argument("--deposit-kernel", type=int, default=-1)
     ap.add_argument("--deposit-chunks", type=int, default=0)
+    ap.add_argument("--node-batch", type=int, default=0, help="nodes per wavefront in the node gather")
+    ap.add_argument("--push-waves", type=int, default=0, help="push register target: 1 or 4 waves per SIMD")
+    ap.add_argument("--node-staged", type=int, default=-1, help="1: staged node gather, 0: per-value gather")
+    ap.add_argument("--deposit-paired", type=int, default=-1, help="matrix-core cell pass: 1 paired phases, 0 not")
     ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 FETCH_SIZE/WRITE_SIZE passes")
     ap.add_argument("--pmc-child", action="store_true", help=argparse.SUPPRESS)
     args = ap.parse_args()
@@ -330,6 +336,14 @@ def main():
     opts = dict(mover=L.MOVER_LAPENTA, bc_mode=L.BC_REFLECT, r_sphere=0.0)
     tune = None
     fixed = args.tile > 0 and args.deposit_kernel >= 0 and args.deposit_chunks > 0
+    if args.node_batch > 0:
+        ctx.set_param("node_batch", args.node_batch)
+    if args.push_waves > 0:
+        ctx.set_param("push_waves", args.push_waves)
+    if args.node_staged >= 0:
+        ctx.set_param("node_staged", args.node_staged)
+    if args.deposit_paired >= 0:
+        ctx.set_param("deposit_paired", args.deposit_paired)
     if fixed:
         ctx.set_param("tile", args.tile)
         autotune.set_deposit_variant(ctx, args.deposit_kernel, args.deposit_chunks)

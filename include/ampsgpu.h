@@ -62,7 +62,15 @@ int amps_gpu_device_info(amps_ctx* ctx, char* name, int* n_cu, int64_t* hbm_byte
  * for a bit-exact summation; kernel 2 then runs kernel 1's exact form),
  * "deposit_kernel" (0: each lane evaluates its cell's particles itself,
  * 1 (default): particles evaluated once into shared-memory record batches,
- * 2: the per-cell moment sums as FP64 matrix-core products),
+ * 2: the per-cell moment sums as FP64 matrix-core products), "deposit_paired"
+ * (kernel 2: 1 two wavefronts per SIMD in opposite VALU/MFMA phases,
+ * 0 (default): one phase sequence per wavefront), "node_batch"
+ * (1..64 nodes per wavefront in the deposit node gather, default 8),
+ * "node_staged" (1 (default): the node gather of the moment record copies
+ * each node's inputs through shared memory with run-wise loads; 0: one
+ * gathered load per output value),
+ * "push_waves" (1: push kernels with the compiler's register allocation,
+ * 4: held to 128 registers for four wavefronts per SIMD),
  * "b_center" (1: the FP64 Lapenta/Boris movers and the deposit take B from
  * the cell-centred field of amps_gpu_set_center_fields, the centre-based B
  * mode _PIC_FIELD_SOLVER_B_MODE_; 0 (default): B at the corners). */
@@ -246,6 +254,30 @@ int amps_gpu_refine_cutoff(amps_ctx* ctx, const amps_trace_params* params, int32
                            const double* R_allow, double abs_tol, double rel_tol,
                            int32_t trace_limit_forbidden, double* cutoff, int32_t* iterations,
                            int32_t* status, int64_t* steps_total);
+/* AnalyzeCutoffBandSamples (srcEarth/util/CutoffBandSearch.h:96-202) for n
+ * state vectors of nR samples (host code): out[15 i ..] = lowerForbiddenIndex,
+ * lowerAllowedIndex, upperForbiddenIndex, upperAllowedIndex, nTransitions,
+ * nAllowedIntervals, nUnresolved, allAllowed, allForbidden, lowerBelowRange,
+ * lowerAboveRange, upperBelowRange, upperAboveRange, lowerBracketUnresolved,
+ * upperBracketUnresolved. */
+int amps_gpu_band_topology(int64_t n, int32_t nR, const int8_t* states, int32_t* out);
+/* IntegrateEffectiveCutoff (srcEarth/util/CutoffBandSearch.h:284-366), the
+ * PENUMBRA_SCAN effective cutoff, for n (location, direction) pairs: every
+ * coarse interval of grid[nR] inside [lower, upper] whose end states differ
+ * is bisected (all pairs' bisections advance together, one batched trace
+ * launch per round) and the allowed width of the penumbra summed.  states
+ * [n*nR]: the scan's sample states (0 physically forbidden, 1 allowed,
+ * 2 unresolved, as CutoffSampleState; -1 untraced counts as unresolved).
+ * effective receives upper - allowed width clamped to [lower, upper] (-1
+ * when not integrated), allowed_width and refined (midpoints traced) what the
+ * reference accumulates, status 0 integrated, 1 unresolved, 2 a numerical
+ * failure after the retry (the reference throws). */
+int amps_gpu_effective_cutoff(amps_ctx* ctx, const amps_trace_params* params, int32_t precision,
+                              int64_t n, const double* x0, const double* dir, int32_t nR,
+                              const double* grid, const int8_t* states, const double* lower,
+                              const double* upper, double abs_tol, double rel_tol,
+                              int32_t trace_limit_forbidden, double* effective, double* allowed_width,
+                              int32_t* refined, int32_t* status, int64_t* steps_total);
 /* Tracer tuning (autotuner): "workgroups" (persistent grid, 0 = 8 per CU),
  * "chunk" (steps between work-queue refills). */
 int amps_gpu_trace_set_param(amps_ctx* ctx, const char* name, int64_t value);

@@ -106,6 +106,21 @@ def ref_refine_cutoff(r_forbid, r_allow, classify, abs_tol=1.0e-3, rel_tol=1.0e-
     return c.value, it.value, bool(un.value)
 
 
+def ref_effective_cutoff(grid, states, lower, upper, classify, abs_tol=1.0e-3, rel_tol=1.0e-6):
+    """The reference's IntegrateEffectiveCutoff (CutoffBandSearch.h:284-366)
+    on grid/states (0 forbidden, 1 allowed, 2 unresolved) with the same
+    classifier convention.  Returns (cutoff_GV, allowed_width_GV, refined,
+    unresolved)."""
+    cb = REF_CLASSIFY(lambda R, _u: int(classify(R)))
+    g = np.ascontiguousarray(grid, np.float64)
+    s = np.ascontiguousarray(states, np.int32)
+    c, w, it, un = C.c_double(), C.c_double(), C.c_int(), C.c_int()
+    ref().ref_effective_cutoff(len(g), g.ctypes.data_as(C.POINTER(C.c_double)),
+                               s.ctypes.data_as(C.POINTER(C.c_int)), lower, upper, abs_tol, rel_tol, cb, None,
+                               C.byref(c), C.byref(w), C.byref(it), C.byref(un))
+    return c.value, w.value, it.value, bool(un.value)
+
+
 def ref():
     """The reference's compiled gridless mover/dipole/band sources."""
     global _ref
@@ -121,6 +136,8 @@ def ref():
         r.ref_band_topology.argtypes = [C.c_int, P(C.c_int), P(C.c_int)]
         r.ref_refine_cutoff.argtypes = [dbl, dbl, dbl, dbl, REF_CLASSIFY, C.c_void_p, P(dbl),
                                         P(C.c_int), P(C.c_int)]
+        r.ref_effective_cutoff.argtypes = [C.c_int, P(dbl), P(C.c_int), dbl, dbl, dbl, dbl, REF_CLASSIFY,
+                                           C.c_void_p, P(dbl), P(dbl), P(C.c_int), P(C.c_int)]
         _ref = r
     return _ref
 

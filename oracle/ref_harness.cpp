@@ -107,4 +107,21 @@ void ref_refine_cutoff(double r_forbid, double r_allow, double abs_tol, double r
   *iterations = r.iterations;
   *unresolved = r.unresolved ? 1 : 0;
 }
+
+// IntegrateEffectiveCutoff over the grid and states given, with the same
+// classifier convention.
+void ref_effective_cutoff(int n, const double* grid, const int* states, double lower, double upper,
+                          double abs_tol, double rel_tol, ref_classify_fn classify, void* user,
+                          double* cutoff, double* width, int* refined, int* unresolved) {
+  auto cls = [&](double R) { return static_cast<EarthUtil::CutoffSampleState>(classify(R, user)); };
+  std::vector<double> g(grid, grid + n);
+  std::vector<EarthUtil::CutoffSampleState> st(n);
+  for (int i = 0; i < n; i++) st[i] = static_cast<EarthUtil::CutoffSampleState>(states[i]);
+  const EarthUtil::EffectiveCutoffIntegration r =
+      EarthUtil::IntegrateEffectiveCutoff(g, st, lower, upper, cls, abs_tol, rel_tol);
+  *cutoff = r.cutoff_GV;
+  *width = r.allowedWidth_GV;
+  *refined = r.refinedTransitions;
+  *unresolved = r.unresolved ? 1 : 0;
+}
 }

@@ -76,6 +76,25 @@ def test_band_topology_matches_reference():
             assert list(O.band_topology(st)) == list(out), st
 
 
+@needs_ref
+def test_library_band_topology_matches_reference():
+    """The library's AnalyzeCutoffBandSamples (host code, no GPU) against the
+    reference's, over random state vectors with unresolved samples and
+    clean allowed suffixes, batched."""
+    from amps_amd import cutoff as CU
+    r = O.ref()
+    rng = np.random.default_rng(6)
+    for n in [1, 2, 3, 7, 48]:
+        st = rng.choice([0, 1, 2], size=(400, n), p=[0.45, 0.45, 0.1]).astype(np.int32)
+        for row in st[::2]:
+            row[rng.integers(0, n):] = 1
+        got = CU.band_topology(st.astype(np.int8))
+        for i in range(len(st)):
+            out = np.zeros(15, np.int32)
+            r.ref_band_topology(n, O._p(st[i], O.C.c_int), O._p(out, O.C.c_int))
+            assert list(got[i]) == list(out), st[i]
+
+
 def test_upper_cutoff_bin_semantics():
     # CutoffForDirectionUpperScan_GV: -1 if Rmax forbidden, Rmin index when all allowed
     assert O.upper_cutoff_bin([0, 0, 1, 1]) == 2
@@ -383,6 +402,25 @@ def test_deposit_mass_matrix_symmetry():
             assert np.array_equal(blk, Mm[pos[kb], 9 * s2:9 * s2 + 9])
             n_checked += 1
     assert n_checked > 10000
+
+
+def test_reference_effective_cutoff_islands():
+    """The reference's IntegrateEffectiveCutoff through the harness against
+    the closed form Rc_eff = Rc_upper - (allowed width inside the penumbra)
+    for an access function with two allowed islands, each straddling
+    interval bisected to the 1e-3 GV tolerance."""
+    def T(R):
+        return 1 if (2.0 <= R < 2.5) or R >= 3.7 else 0
+    grid = np.linspace(1.0, 5.0, 17)
+    states = [T(R) for R in grid]
+    lower, upper = 2.0, 3.7
+    c, w, it, un = O.ref_effective_cutoff(grid, states, lower, upper, T)
+    assert not un and abs(w - 0.5) < 2e-3 and abs(c - (upper - 0.5)) < 2e-3
+    assert it > 0
+    # an unresolved sample inside the penumbra stops it unresolved
+    states[6] = 2
+    c, w, it, un = O.ref_effective_cutoff(grid, states, lower, upper, T)
+    assert un and c == -1.0
 
 
 def test_reference_refine_cutoff_threshold():

@@ -45,6 +45,36 @@ def test_push_single_step_bitexact(gpu_ctx, mover, bc, amr):
     assert np.array_equal(g.blk[alive], o.blk[alive]) and np.array_equal(g.cell[alive], o.cell[alive])
 
 
+@pytest.mark.parametrize("mover", [L.MOVER_LAPENTA, L.MOVER_BORIS])
+def test_push_waves4_variant(gpu_ctx, mover):
+    """The push kernels held to 128 registers: FP64 bit-exact against the
+    oracle, FP32 bit-identical to the unconstrained build."""
+    mesh = M.central_octant_mesh()
+    E, B, parts, sp = _case(mesh, 40000, 0.5, 0.05, seed=5)
+    om = O.Mesh(mesh)
+    o, ocode = O.push(om, E, B, sp, parts, mover=mover, bc_mode=L.BC_REFLECT, r_sphere=0.2)
+    o, ocode = _by_id(o, ocode)
+    res = {}
+    try:
+        for w in (4, 1):
+            gpu_ctx.set_param("push_waves", w)
+            st = PicState(mesh, E, B, sp, ctx=gpu_ctx)
+            st.set_particles(parts)
+            st.MoveParticles(mover=mover, bc_mode=L.BC_REFLECT, r_sphere=0.2, compact=False)
+            g, gcode = _by_id(*st.get_particles())
+            if w == 4:
+                assert np.array_equal(gcode, ocode)
+                assert np.array_equal(g.x, o.x) and np.array_equal(g.v, o.v)
+            st = PicState(mesh, E, B, sp, ctx=gpu_ctx)
+            st.set_particles(parts, precision=L.FP32)
+            st.MoveParticles(mover=mover, bc_mode=L.BC_REFLECT, r_sphere=0.2, compact=False)
+            res[w] = _by_id(*st.get_particles())
+    finally:
+        gpu_ctx.set_param("push_waves", 1)
+    (a, ca), (b, cb) = res[4], res[1]
+    assert np.array_equal(ca, cb) and np.array_equal(a.x, b.x) and np.array_equal(a.v, b.v)
+
+
 @pytest.mark.parametrize("bc", [L.BC_REFLECT, L.BC_DELETE])
 def test_push_relativistic_si_bitexact(gpu_ctx, bc):
     """Relativistic protons (0.1c-0.95c) in SI units on the 2-level mesh
@@ -166,8 +196,29 @@ def test_deposit_parity(gpu_ctx, dep_case, exact, kernel, nch):
     set_deposit_variant(gpu_ctx, 1, 6)
 
 
-@pytest.mark.parametrize("kernel", [2, 1])
-def test_deposit_ragged_cells(gpu_ctx, kernel):
+def test_node_gather_staged_matches_direct(gpu_ctx, dep_case):
+    """The staged node gather reads the same operands and sums them in the
+    same order as the per-value gather: bit-identical nodes."""
+    mesh, E, B, parts, sp, dt = dep_case
+    st = PicState(mesh, E, B, sp, ctx=gpu_ctx)
+    st.set_particles(parts)
+    out = {}
+    try:
+        for staged in (1, 0):
+            gpu_ctx.set_param("node_staged", staged)
+            for nb in (1, 8):
+                gpu_ctx.set_param("node_batch", nb)
+                out[staged, nb] = st.UpdateJMassMatrix(dt)
+    finally:
+        gpu_ctx.set_param("node_staged", 1)
+        gpu_ctx.set_param("node_batch", 8)
+    J0, M0 = out[0, 1]
+    for k, (J, Mm) in out.items():
+        assert np.array_equal(J, J0) and np.array_equal(Mm, M0), k
+
+
+@pytest.mark.parametrize("kernel,paired", [(2, 1), (2, 0), (1, 0)])
+def test_deposit_ragged_cells(gpu_ctx, kernel, paired):
     """Empty cells, cells of 1-3 particles and cells of 65-80 (more than one
     64-particle chunk of the matrix-core pass) side by side."""
     mesh = M.central_octant_mesh()
@@ -185,10 +236,12 @@ def test_deposit_ragged_cells(gpu_ctx, kernel):
     st = PicState(mesh, E, B, sp, ctx=gpu_ctx)
     st.set_particles(parts)
     set_deposit_variant(gpu_ctx, kernel, 12)
+    gpu_ctx.set_param("deposit_paired", paired)
     try:
         J, Mm = st.UpdateJMassMatrix(dt)
     finally:
         set_deposit_variant(gpu_ctx, 1, 6)
+        gpu_ctx.set_param("deposit_paired", 0)
     g, _ = st.get_particles()
     om = O.Mesh(mesh)
     noc, keys = om.build_nodes()

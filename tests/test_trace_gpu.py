@@ -180,6 +180,46 @@ def test_refine_cutoff_matches_reference_bisection(gpu_ctx):
     print(f"refined {len(sel)} upper cutoffs, {int(it.sum())} midpoints")
 
 
+def test_penumbra_scan_matches_reference_chain(gpu_ctx):
+    """PENUMBRA_SCAN (FP64) for penumbral directions: the GPU chain (full
+    scan, band topology, batched refinement of both transitions, batched
+    effective-cutoff bisection) against the reference's AnalyzeCutoffBand-
+    Samples, RefineCutoffTransition and IntegrateEffectiveCutoff compiled from
+    CutoffBandSearch.h and driven by the oracle tracer: identical lower,
+    upper and effective cutoffs."""
+    P = CU.config2_params()
+    P.max_steps = 20000
+    pts, lat, lon = CU.config2_points(nlon=8, nlat=6)
+    zen, az = CU.config2_directions(4)
+    dirs = CU.arrival_dirs(pts, zen, az).reshape(-1, 3)
+    x0 = np.repeat(pts, len(zen), axis=0)
+    res = CU.penumbra_scan(gpu_ctx, P, x0, dirs, 0.1, 20.0, 24)
+    grid, states = res["grid"], res["states"]
+    pen = np.flatnonzero((res["nTransitions"] > 1) & (res["effective_GV"] >= 0))
+    assert len(pen) >= 3, int(np.sum(res["nTransitions"] > 1))
+    Q = _oparams(P)
+
+    def classify(i):
+        def f(R):
+            t = int(O.trace_batch(Q, x0[i:i + 1], dirs[i:i + 1], np.array([R]))["termination"][0])
+            return 1 if t == 0 else (0 if t <= 5 else 2)
+        return f
+
+    r = O.ref()
+    for i in pen[:8]:
+        st = states[i].astype(np.int32)
+        topo = np.zeros(15, np.int32)
+        r.ref_band_topology(len(st), O._p(st, O.C.c_int), O._p(topo, O.C.c_int))
+        assert list(topo[:7]) == [int(res[f][i]) for f in CU.TOPOLOGY_FIELDS[:7]]
+        lo, _, un_lo = O.ref_refine_cutoff(grid[topo[0]], grid[topo[1]], classify(i))
+        up, _, un_up = O.ref_refine_cutoff(grid[topo[2]], grid[topo[3]], classify(i))
+        assert not un_lo and not un_up
+        assert res["lower_GV"][i] == lo and res["upper_GV"][i] == up
+        eff, width, n_ref, un = O.ref_effective_cutoff(grid, st, lo, up, classify(i))
+        assert not un and res["effective_GV"][i] == eff, (i, res["effective_GV"][i], eff)
+    print(f"penumbra: {len(pen)} of {len(x0)} directions with more than one transition")
+
+
 def test_fp32_tracer_agreement(gpu_ctx):
     """FP32 arithmetic with FP64 position accumulation vs the FP64 oracle:
     per-trajectory classification and per-direction cutoff bin."""
