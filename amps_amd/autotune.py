@@ -16,7 +16,7 @@ from . import _lib as L
 PUSH_TILES = (1024, 2048, 4096, 8192)
 # (cell-pass kernel, chunks per cell): 0 = per-lane particle loop,
 # 1 = shared-memory record batches
-DEPOSIT_VARIANTS = ((1, 6), (1, 12), (1, 4), (0, 6))
+DEPOSIT_VARIANTS = ((2, 4), (1, 4), (1, 6), (1, 12), (0, 6))
 
 
 def set_deposit_variant(ctx, kernel, nch):

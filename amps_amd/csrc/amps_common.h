@@ -124,12 +124,13 @@ struct amps_ctx {
   bool offsets_valid = false;  // d_offset describes the current order
   bool tiles_valid = false;    // d_tiles describes the current order
   int dep_nch = 12;            // deposit output chunks per cell (waves per workgroup)
-  int node_batch = 8;          // nodes per wavefront in the deposit node gather
+  int node_batch = 8;          // nodes per wavefront in the deposit node gather (0: grid-stride)
+  int n_cu = 256;              // compute units of the device
   int node_staged = 1;         // moment-record node gather staged through LDS
   int dep_paired = 0;          // matrix-core cell pass: two wavefronts per SIMD in opposite phases
   int push_waves = 1;          // 4: push kernels built for 4 waves per SIMD (128 VGPRs)
   int dep_exact = 0;           // 1: unfused mul+add in the deposit (bit-exact order)
-  int dep_kernel = 1;          // cell pass: 0 per-lane particle loop, 1 LDS record batches
+  int dep_kernel = 2;          // cell pass: 0 per-lane particle loop, 1 LDS record batches, 2 matrix cores
   int b_center = 0;            // 1: B from the cell-centred field (mover and deposit)
   // deposit
   double* d_cellpart = nullptr;

@@ -130,6 +130,7 @@ extern "C" int amps_gpu_create(int device, amps_ctx** out) {
   HIP_TRY(hipSetDevice(device));
   amps_ctx* c = new amps_ctx();
   c->device = device;
+  HIP_TRY(hipDeviceGetAttribute(&c->n_cu, hipDeviceAttributeMultiprocessorCount, device));
   HIP_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
   HIP_TRY(hipEventCreate(&c->ev0));
   HIP_TRY(hipEventCreate(&c->ev1));
@@ -209,7 +210,7 @@ extern "C" int amps_gpu_set_param(amps_ctx* c, const char* name, int64_t value) 
     if (value != 0 && value != 1) return fail(AMPS_E_ARG, "node_staged must be 0 or 1");
     c->node_staged = (int)value;
   } else if (n == "node_batch") {
-    if (value < 1 || value > 64) return fail(AMPS_E_ARG, "node_batch must be in [1, 64]");
+    if (value < 0 || value > 64) return fail(AMPS_E_ARG, "node_batch must be in [0, 64]");
     c->node_batch = (int)value;
   } else if (n == "deposit_exact") {
     c->dep_exact = value ? 1 : 0;
@@ -855,18 +856,22 @@ static int launch_deposit(amps_ctx* c, double dt) {
   tend(c, AMPS_K_DEPOSIT_CELL);
   tbegin(c);
   // a multiple of 8 workgroups (see xcd_tile); the surplus ones exit
-  const int64_t per_wg = 4 * (int64_t)c->node_batch;
-  const int64_t node_wgs = ((c->nnodes + per_wg - 1) / per_wg + 7) / 8 * 8;
+  // node_batch 0 (staged gather only): a resident grid striding over the nodes
+  const bool stride_grid = c->node_batch == 0 && c->dep_kernel >= 1 && !c->dep_exact && c->node_staged;
+  const int nb_eff = (c->node_batch == 0 && !stride_grid) ? 8 : c->node_batch;
+  const int64_t per_wg = 4 * (int64_t)(nb_eff > 0 ? nb_eff : 1);
+  int64_t node_wgs = ((c->nnodes + per_wg - 1) / per_wg + 7) / 8 * 8;
+  if (stride_grid) node_wgs = std::min<int64_t>(node_wgs, (int64_t)c->n_cu * 8);
   // the record-batched and matrix-core cell passes write the moment form unless bit-exact
   if (c->dep_kernel >= 1 && !c->dep_exact && c->node_staged)
     hipLaunchKernelGGL(k_deposit_node_staged, dim3((unsigned)node_wgs), dim3(256), 0, c->stream,
-                       c->nnodes, c->node_batch, c->d_node_order, c->d_inc8, c->d_cellpart, c->d_nodeout);
+                       c->nnodes, nb_eff, c->d_node_order, c->d_inc8, c->d_cellpart, c->d_nodeout);
   else if (c->dep_kernel >= 1 && !c->dep_exact)
     hipLaunchKernelGGL(k_deposit_node<true>, dim3((unsigned)node_wgs), dim3(256), 0, c->stream,
-                       c->nnodes, c->node_batch, c->d_node_order, c->d_inc8, c->d_cellpart, c->d_nodeout);
+                       c->nnodes, nb_eff, c->d_node_order, c->d_inc8, c->d_cellpart, c->d_nodeout);
   else
     hipLaunchKernelGGL(k_deposit_node<false>, dim3((unsigned)node_wgs), dim3(256), 0, c->stream,
-                       c->nnodes, c->node_batch, c->d_node_order, c->d_inc8, c->d_cellpart, c->d_nodeout);
+                       c->nnodes, nb_eff, c->d_node_order, c->d_inc8, c->d_cellpart, c->d_nodeout);
   HIP_TRY(hipGetLastError());
   tend(c, AMPS_K_DEPOSIT_NODE);
   return 0;

@@ -122,11 +122,21 @@ __global__ __launch_bounds__(64 * waves_of<PAIRED>()) void k_deposit_cell_mfma(M
   }
   auto cellkey = [&](int c) { return keybase + (sx + (c & 3)) + NC * ((sy + row0 + (c >> 2)) + NC * zc); };
   // this wavefront's cells: first particle, running counts
-  if (lane < NCW) sfirst[w][lane] = offset[cellkey(lane)];
-  if (lane < NCW + 1) {
-    int cum = 0;
-    for (int c = 0; c < lane; c++) cum += offset[cellkey(c) + 1] - offset[cellkey(c)];
-    scum[w][lane] = cum;
+  {
+    // counts of lanes' cells, then an inclusive scan across the lanes
+    const int ck = cellkey(lane < NCW ? lane : 0);
+    const int o0 = offset[ck], cnt = lane < NCW ? offset[ck + 1] - o0 : 0;
+    int incl = cnt;
+#pragma unroll
+    for (int d = 1; d < NCW; d <<= 1) {
+      const int t = __shfl_up(incl, d, 64);
+      if (lane >= d) incl += t;
+    }
+    if (lane < NCW) {
+      sfirst[w][lane] = o0;
+      scum[w][lane] = incl - cnt;
+    }
+    if (lane == NCW - 1) scum[w][NCW] = incl;
   }
   if (threadIdx.x == 0) smaxchunks = 0;
   __syncthreads();
@@ -150,6 +160,9 @@ __global__ __launch_bounds__(64 * waves_of<PAIRED>()) void k_deposit_cell_mfma(M
     }
   }
   const int oa = (F_A + j) * FS, o0a = f0a * FS, o0b = f0b * FS, o1a = f1a * FS, o1b = f1b * FS;
+  // B1 factor blend: the loaded factor, or 1 for the wy wz columns (the field read
+  // there is F_ZERO, so loaded * keep + one is exact either way)
+  const double keep = f1w ? 0.0 : 1.0, one = f1w ? 1.0 : 0.0;
   // destinations of the 8 accumulator values; -1: unused.  v_mfma_f64_16x16x4f64
   // holds A[i][k] in lane i + 16k, B[k][j] in lane j + 16k and D[i][j] in
   // lane 16 (i % 4) + j, accumulator register i / 4.
@@ -201,17 +214,21 @@ __global__ __launch_bounds__(64 * waves_of<PAIRED>()) void k_deposit_cell_mfma(M
   };
   Pre n1 = {{0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}, 0, 0}, n2 = n1;
 
-  dbl4 C0 = {0.0, 0.0, 0.0, 0.0}, C1 = C0;
+  // two accumulator pairs (even and odd steps): four independent MFMA chains
+  dbl4 C0 = {0.0, 0.0, 0.0, 0.0}, C1 = C0, D0 = C0, D1 = C0;
   int cur = 0;  // the cell being accumulated (uniform)
   auto flush = [&](int c) {
     const int key = cellkey(c);
 #pragma unroll
     for (int r = 0; r < 4; r++) {
-      if (oidx[r] >= 0) S[oidx[r]] = ojc[r] ? C0[r] * invV : C0[r];
-      if (oidx[4 + r] >= 0) S[oidx[4 + r]] = ojc[4 + r] ? C1[r] * invV : C1[r];
+      const double e0 = C0[r] + D0[r], e1 = C1[r] + D1[r];
+      if (oidx[r] >= 0) S[oidx[r]] = ojc[r] ? e0 * invV : e0;
+      if (oidx[4 + r] >= 0) S[oidx[4 + r]] = ojc[4 + r] ? e1 * invV : e1;
     }
     C0 = dbl4{0.0, 0.0, 0.0, 0.0};
     C1 = C0;
+    D0 = C0;
+    D1 = C0;
     wave_lds_sync();
     // 2176 bytes from a 128-byte boundary: every store covers whole lines
     double* dst = cellpart + (size_t)key * CS;
@@ -340,17 +357,45 @@ __global__ __launch_bounds__(64 * waves_of<PAIRED>()) void k_deposit_cell_mfma(M
       const int nfull = (se - q) >> 2;
       const double* R = R0 + (q - base + slot);
       if (nfull > 0) {
-        double a = R[oa], b0 = R[o0a] * R[o0b], b1 = R[o1a] * (f1w ? 1.0 : R[o1b]);
-        for (int t = 1; t < nfull; t++) {
-          R += 4;
-          const double a2 = R[oa], b02 = R[o0a] * R[o0b], b12 = R[o1a] * (f1w ? 1.0 : R[o1b]);
-          C0 = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b0, C0, 0, 0, 0);
-          C1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b1, C1, 0, 0, 0);
-          a = a2, b0 = b02, b1 = b12;
+        // the raw operands of two steps in flight ahead of the products (the
+        // LDS latency under load exceeds one step's two MFMAs); the factors
+        // are multiplied just before use
+        struct Ops {
+          double a, x0, y0, x1, y1;
+        };
+        auto ld = [&](const double* Rs, Ops& o) {
+          o.a = Rs[oa];
+          o.x0 = Rs[o0a];
+          o.y0 = Rs[o0b];
+          o.x1 = Rs[o1a];
+          o.y1 = Rs[o1b];
+        };
+        auto step = [&](const Ops& o, dbl4& A0, dbl4& A1) {
+          const double b0 = o.x0 * o.y0, b1 = o.x1 * fma(o.y1, keep, one);
+          A0 = __builtin_amdgcn_mfma_f64_16x16x4f64(o.a, b0, A0, 0, 0, 0);
+          A1 = __builtin_amdgcn_mfma_f64_16x16x4f64(o.a, b1, A1, 0, 0, 0);
+        };
+        // two operand sets, unrolled by two so that neither is copied while
+        // its loads are in flight; each is reloaded right after its products
+        // (the loads past the last step re-read the last step's records, so
+        // that every load is unconditional and the waits stay exact)
+        const int last = nfull - 1;
+        Ops pa, pb;
+        ld(R, pa);
+        ld(R + 4 * (last < 1 ? last : 1), pb);
+        __builtin_amdgcn_sched_barrier(0);
+        int t = 0;
+        for (; t + 1 < nfull; t += 2) {
+          // the scheduling barriers keep each reload where it is issued
+          step(pa, C0, C1);
+          ld(R + 4 * (t + 2 < last ? t + 2 : last), pa);
+          __builtin_amdgcn_sched_barrier(0);
+          step(pb, D0, D1);
+          ld(R + 4 * (t + 3 < last ? t + 3 : last), pb);
+          __builtin_amdgcn_sched_barrier(0);
         }
-        C0 = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b0, C0, 0, 0, 0);
-        C1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b1, C1, 0, 0, 0);
-        R += 4;
+        if (t < nfull) step(pa, C0, C1);
+        R += 4 * nfull;
       }
       if (q + 4 * nfull < se) {
         // the cell's last 1-3 particles in this chunk: the other slots add zero
@@ -358,7 +403,7 @@ __global__ __launch_bounds__(64 * waves_of<PAIRED>()) void k_deposit_cell_mfma(M
         if (q + 4 * nfull + slot < se) {
           a = R[oa];
           b0 = R[o0a] * R[o0b];
-          b1 = R[o1a] * (f1w ? 1.0 : R[o1b]);
+          b1 = R[o1a] * fma(R[o1b], keep, one);
         }
         C0 = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b0, C0, 0, 0, 0);
         C1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b1, C1, 0, 0, 0);

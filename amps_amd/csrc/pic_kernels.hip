@@ -496,33 +496,53 @@ __global__ __launch_bounds__(256) void k_deposit_node_staged(int64_t nnodes, int
     reinterpret_cast<int4*>(&gtab[0][0])[threadIdx.x] = reinterpret_cast<const int4*>(&c_gtab[0][0])[threadIdx.x];
   __syncthreads();
   const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int64_t first = ((int64_t)xcd_tile() * 4 + wv) * npw;
-  const int64_t last = (first + npw < nnodes) ? first + npw : nnodes;
+  // npw > 0: wavefront g takes nodes [g npw, (g+1) npw).  npw == 0: the grid
+  // strides over the nodes, wavefront g taking g, g + G, g + 2G, ... (G the
+  // grid's wavefronts), so the nodes in flight at any time form one
+  // contiguous band whose incident cells stay in L2.
+  const int64_t gw = (int64_t)xcd_tile() * 4 + wv;
+  const int64_t first = npw > 0 ? gw * npw : gw;
+  const int64_t last = npw > 0 ? ((first + npw < nnodes) ? first + npw : nnodes) : nnodes;
+  const int64_t stride = npw > 0 ? 1 : (int64_t)gridDim.x * 4;
   const int lane = threadIdx.x & 63;
   double* Gw = G[wv];
   // this lane's gathered values e = lane + 64i: incident cell e / 75, value e % 75
   constexpr int NI = (8 * NODE_G + 63) / 64;
-  for (int64_t slot = first; slot < last; slot++) {
+  // The next node's incidence row and gathered values are loaded into
+  // registers while the current node is summed (two buffers, the loop
+  // unrolled by two so that neither is copied while its loads are in flight;
+  // past the last node the last one is loaded again).
+  struct Buf {
     int ics[8];
+    double v[NI];
+    int64_t node;
+  };
+  auto gather = [&](int64_t slot, Buf& B) {
 #pragma unroll
-    for (int t = 0; t < 8; t++) ics[t] = inc8[slot * 8 + t];
-    const int64_t node = node_order[slot];
+    for (int t = 0; t < 8; t++) B.ics[t] = inc8[slot * 8 + t];
+    B.node = node_order[slot];
 #pragma unroll
     for (int i = 0; i < NI; i++) {
       const int e = lane + 64 * i;
       const int t0 = (64 * i) / NODE_G;  // the lanes of step i cover cells t0 and t0 + 1
-      if (e < 8 * NODE_G) {
-        const bool hi = t0 + 1 < 8 && e >= NODE_G * (t0 + 1);
-        const int ic = hi ? ics[t0 + 1 < 8 ? t0 + 1 : 7] : ics[t0];
-        const int w = e - NODE_G * (hi ? t0 + 1 : t0);
-        if (ic >= 0) Gw[e] = cellpart[(size_t)(ic >> 3) * CELL_STRIDE_MOMENT + gtab[ic & 7][w]];
-      }
+      const bool hi = t0 + 1 < 8 && e >= NODE_G * (t0 + 1);
+      const int ic = hi ? B.ics[t0 + 1 < 8 ? t0 + 1 : 7] : B.ics[t0];
+      const int w = e - NODE_G * (hi ? t0 + 1 : t0);
+      B.v[i] = (e < 8 * NODE_G && ic >= 0) ? cellpart[(size_t)(ic >> 3) * CELL_STRIDE_MOMENT + gtab[ic & 7][w]]
+                                            : 0.0;
+    }
+  };
+  auto finish = [&](const Buf& B) {
+#pragma unroll
+    for (int i = 0; i < NI; i++) {
+      const int e = lane + 64 * i;
+      if (e < 8 * NODE_G) Gw[e] = B.v[i];
     }
     asm volatile("" ::: "memory");  // the DS ops of a wavefront execute in issue order
     double acc[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
     for (int t = 0; t < 8; t++) {
-      const int ic = ics[t];
+      const int ic = B.ics[t];
       if (ic >= 0) {
         const short* wt = wtab[ic & 7];
 #pragma unroll
@@ -534,12 +554,24 @@ __global__ __launch_bounds__(256) void k_deposit_node_staged(int64_t nnodes, int
       }
     }
     asm volatile("" ::: "memory");
-    double* o = nodeout + (size_t)node * NODE_STRIDE;
+    double* o = nodeout + (size_t)B.node * NODE_STRIDE;
 #pragma unroll
     for (int r = 0; r < 4; r++) {
       const int val = lane + 64 * r;
       o[val] = (val < NODE_VALS) ? acc[r] : 0.0;
     }
+  };
+  if (first >= last) return;
+  const int64_t final_slot = first + (last - 1 - first) / stride * stride;
+  auto clampslot = [&](int64_t s2) { return s2 < final_slot ? s2 : final_slot; };
+  Buf A, Bb;
+  gather(first, A);
+  for (int64_t slot = first; slot < last; slot += 2 * stride) {
+    gather(clampslot(slot + stride), Bb);
+    finish(A);
+    if (slot + stride >= last) break;
+    gather(clampslot(slot + 2 * stride), A);
+    finish(Bb);
   }
 }
 

@@ -685,7 +685,7 @@ extern "C" int amps_gpu_trace_batch(amps_ctx* c, const amps_trace_params* P, int
   return AMPS_OK;
 }
 
-// RefineCutoffTransition (CutoffBandSearch.h:212-243) for many brackets at
+// RefineCutoffTransition (CutoffBandSearch.h:212-242) for many brackets at
 // once: every bisection round traces the midpoints of all still-open brackets
 // in one persistent launch.  Midpoint, tolerance and update are the
 // reference's: tol = max(abs_tol, rel_tol max(|lo|, |hi|)), mid = 0.5 (lo + hi),
@@ -763,7 +763,7 @@ extern "C" int amps_gpu_refine_cutoff(amps_ctx* c, const amps_trace_params* P, i
   return AMPS_OK;
 }
 
-// AnalyzeCutoffBandSamples (CutoffBandSearch.h:96-202) for n state vectors:
+// AnalyzeCutoffBandSamples (CutoffBandSearch.h:96-195) for n state vectors:
 // the lower and upper transition brackets, transition and allowed-interval
 // counts and the range/unresolved flags.  An untraced sample (-1) counts as
 // unresolved.

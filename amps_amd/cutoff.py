@@ -196,7 +196,7 @@ def cutoff_scan(ctx, params, x0, dirs, grid, mode=SCAN_UPPER, lo=None, hi=None, 
 
 def refine_cutoff(ctx, params, x0, dirs, r_forbid, r_allow, abs_tol=1.0e-3, rel_tol=1.0e-6,
                   trace_limit_forbidden=True, precision=L.FP64):
-    """RefineCutoffTransition (CutoffBandSearch.h:212-243) for many brackets:
+    """RefineCutoffTransition (CutoffBandSearch.h:212-242) for many brackets:
     bisection of [r_forbid, r_allow] per (location, direction), one batched
     trace launch per round.  Returns (cutoff_GV, iterations, status, steps);
     status 1 = a trace limit ended the bracket (with trace_limit_forbidden
@@ -262,7 +262,7 @@ TOPOLOGY_FIELDS = ("lowerForbiddenIndex", "lowerAllowedIndex", "upperForbiddenIn
 
 
 def band_topology(states):
-    """AnalyzeCutoffBandSamples (CutoffBandSearch.h:96-202) for each row of
+    """AnalyzeCutoffBandSamples (CutoffBandSearch.h:96-195) for each row of
     states (0 physically forbidden, 1 allowed, 2 unresolved): [n, 15] int32
     in TOPOLOGY_FIELDS order."""
     st = np.ascontiguousarray(states, np.int8)

@@ -61,11 +61,13 @@ int amps_gpu_device_info(amps_ctx* ctx, char* name, int* n_cu, int64_t* hbm_byte
  * and 2), "deposit_exact" (0/1: unfused mul+add in the deposit accumulation
  * for a bit-exact summation; kernel 2 then runs kernel 1's exact form),
  * "deposit_kernel" (0: each lane evaluates its cell's particles itself,
- * 1 (default): particles evaluated once into shared-memory record batches,
- * 2: the per-cell moment sums as FP64 matrix-core products), "deposit_paired"
+ * 1: particles evaluated once into shared-memory record batches,
+ * 2 (default): the per-cell moment sums as FP64 matrix-core products), "deposit_paired"
  * (kernel 2: 1 two wavefronts per SIMD in opposite VALU/MFMA phases,
  * 0 (default): one phase sequence per wavefront), "node_batch"
- * (1..64 nodes per wavefront in the deposit node gather, default 8),
+ * (1..64 consecutive nodes per wavefront in the deposit node gather,
+ * default 8; 0: a resident grid whose wavefronts stride over the nodes, for
+ * the staged gather — the others then take 8),
  * "node_staged" (1 (default): the node gather of the moment record copies
  * each node's inputs through shared memory with run-wise loads; 0: one
  * gathered load per output value),
@@ -242,7 +244,7 @@ int amps_gpu_cutoff_scan(amps_ctx* ctx, const amps_trace_params* params, int32_t
                          int32_t nR, const double* grid, const int32_t* bin_lo,
                          const int32_t* bin_hi, int8_t* states, int32_t* cutoff_bin,
                          int64_t* steps_total);
-/* RefineCutoffTransition (srcEarth/util/CutoffBandSearch.h:212-243) for n
+/* RefineCutoffTransition (srcEarth/util/CutoffBandSearch.h:212-242) for n
  * (location, direction) brackets at once, each [R_forbid, R_allow] with the
  * forbidden state at R_forbid: bisection to max(abs_tol, rel_tol max(|lo|,
  * |hi|)), one batched trace launch per round.  cutoff receives the allowed
@@ -254,7 +256,7 @@ int amps_gpu_refine_cutoff(amps_ctx* ctx, const amps_trace_params* params, int32
                            const double* R_allow, double abs_tol, double rel_tol,
                            int32_t trace_limit_forbidden, double* cutoff, int32_t* iterations,
                            int32_t* status, int64_t* steps_total);
-/* AnalyzeCutoffBandSamples (srcEarth/util/CutoffBandSearch.h:96-202) for n
+/* AnalyzeCutoffBandSamples (srcEarth/util/CutoffBandSearch.h:96-195) for n
  * state vectors of nR samples (host code): out[15 i ..] = lowerForbiddenIndex,
  * lowerAllowedIndex, upperForbiddenIndex, upperAllowedIndex, nTransitions,
  * nAllowedIntervals, nUnresolved, allAllowed, allForbidden, lowerBelowRange,

@@ -96,7 +96,7 @@ REF_CLASSIFY = C.CFUNCTYPE(C.c_int, C.c_double, C.c_void_p)
 
 
 def ref_refine_cutoff(r_forbid, r_allow, classify, abs_tol=1.0e-3, rel_tol=1.0e-6):
-    """The reference's RefineCutoffTransition (CutoffBandSearch.h:212-243)
+    """The reference's RefineCutoffTransition (CutoffBandSearch.h:212-242)
     with classify(R_GV) -> 0 physical forbidden, 1 allowed, 2 unresolved.
     Returns (cutoff_GV, iterations, unresolved)."""
     cb = REF_CLASSIFY(lambda R, _u: int(classify(R)))

@@ -206,7 +206,7 @@ def test_node_gather_staged_matches_direct(gpu_ctx, dep_case):
     try:
         for staged in (1, 0):
             gpu_ctx.set_param("node_staged", staged)
-            for nb in (1, 8):
+            for nb in (0, 1, 8):
                 gpu_ctx.set_param("node_batch", nb)
                 out[staged, nb] = st.UpdateJMassMatrix(dt)
     finally:

@@ -211,8 +211,17 @@ def test_penumbra_scan_matches_reference_chain(gpu_ctx):
         topo = np.zeros(15, np.int32)
         r.ref_band_topology(len(st), O._p(st, O.C.c_int), O._p(topo, O.C.c_int))
         assert list(topo[:7]) == [int(res[f][i]) for f in CU.TOPOLOGY_FIELDS[:7]]
-        lo, _, un_lo = O.ref_refine_cutoff(grid[topo[0]], grid[topo[1]], classify(i))
-        up, _, un_up = O.ref_refine_cutoff(grid[topo[2]], grid[topo[3]], classify(i))
+        # CutoffForDirectionPenumbraScan_GV (CutoffRigidityGridless.cpp:2733-2750)
+        lo = up = -1.0
+        un_lo = un_up = False
+        if topo[9]:
+            lo = grid[0]
+        elif topo[0] >= 0 and topo[1] >= 0:
+            lo, _, un_lo = O.ref_refine_cutoff(grid[topo[0]], grid[topo[1]], classify(i))
+        if topo[7]:
+            up = grid[0]
+        elif topo[2] >= 0 and topo[3] >= 0:
+            up, _, un_up = O.ref_refine_cutoff(grid[topo[2]], grid[topo[3]], classify(i))
         assert not un_lo and not un_up
         assert res["lower_GV"][i] == lo and res["upper_GV"][i] == up
         eff, width, n_ref, un = O.ref_effective_cutoff(grid, st, lo, up, classify(i))
