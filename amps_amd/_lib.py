@@ -41,6 +41,25 @@ class PushOpts(C.Structure):
                 ("tile", C.c_int32), ("r_sphere", C.c_double)]
 
 
+class AosLayout(C.Structure):
+    """ParticleBuffer record layout (amps_aos_layout)."""
+    _fields_ = [("record_bytes", C.c_int32), ("species_offset", C.c_int32), ("species_bytes", C.c_int32),
+                ("velocity_offset", C.c_int32), ("position_offset", C.c_int32), ("weight_offset", C.c_int32)]
+
+
+# the mandatory-field offsets of picParticleDataMacro.h in its two state-vector modes
+def aos_layout_aligned(weight=False, extra_bytes=0):
+    """_STATE_VECTOR_MODE_ALIGNED_: next, prev, species (long), v[3], [weight], x[3]."""
+    w = 48 if weight else -1
+    x = 56 if weight else 48
+    return AosLayout(x + 24 + extra_bytes, 16, 8, 24, x, w)
+
+
+def aos_layout_packed(weight=False, extra_bytes=0):
+    """Unaligned state vector: next, prev, species (unsigned char), v[3], x[3], [weight]."""
+    return AosLayout(17 + 48 + (8 if weight else 0) + extra_bytes, 16, 1, 17, 41, 65 if weight else -1)
+
+
 class TraceParams(C.Structure):
     _fields_ = [
         ("q_C", C.c_double), ("qabs_C", C.c_double), ("m0_kg", C.c_double),
@@ -100,6 +119,8 @@ def load():
         "amps_gpu_set_particles": (C.c_int, [vp, i64, P(dbl), P(dbl), P(i32), P(i32), P(i32),
                                              P(i64), i32]),
         "amps_gpu_particle_count": (i64, [vp]),
+        "amps_gpu_get_particles_restart": (C.c_int, [vp, P(AosLayout), P(i32), vp, i64, P(i64)]),
+        "amps_gpu_set_particles_restart": (C.c_int, [vp, P(AosLayout), P(i32), i64, vp, i32]),
         "amps_gpu_get_particles": (C.c_int, [vp, P(dbl), P(dbl), P(i32), P(i32), P(i32),
                                              P(i64), P(i32)]),
         "amps_gpu_move_particles": (C.c_int, [vp, P(PushOpts)]),

@@ -163,6 +163,8 @@ int scan_exclusive(amps_ctx* c, int* data, int64_t n, int* total_out);
 // ECSIM cell pass with LDS record batches (deposit.hip)
 int launch_deposit_cell_rec(amps_ctx* c, double dt);
 int launch_deposit_cell_mfma(amps_ctx* c, double dt);
+int alloc_particles(amps_ctx* c, int64_t n);  // both particle sets for at least n
+int build_tiles(amps_ctx* c);                 // push tiles from the current offsets
 }  // namespace amps
 
 #define HIP_TRY(expr)                                    \

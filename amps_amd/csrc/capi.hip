@@ -552,7 +552,7 @@ extern "C" int amps_gpu_set_species(amps_ctx* c, int32_t nspec, const double* ch
 // ---------------------------------------------------------------------
 // particles
 // ---------------------------------------------------------------------
-static int alloc_particles(amps_ctx* c, int64_t n) {
+int amps::alloc_particles(amps_ctx* c, int64_t n) {
   if (n <= c->cap) return 0;
   free_particles(c);
   const int64_t cap = std::max<int64_t>(n, 64);
@@ -569,7 +569,7 @@ static int alloc_particles(amps_ctx* c, int64_t n) {
   return 0;
 }
 
-static int build_tiles(amps_ctx* c) {
+int amps::build_tiles(amps_ctx* c) {
   const int nb = c->mv.nb;
   const int64_t need = nb + c->cap / c->tile + 2;
   if (need > c->max_tiles) {

@@ -128,6 +128,29 @@ class PicState:
             L._ptr(code, C.c_int32)), "get_particles")
         return Particles(x, v, spec, blk, cell, pid), code
 
+    def get_restart(self, layout):
+        """The particle part of a restart file (SaveParticleDataBlock,
+        pic_restart.cpp:312-370): per-block ParticleNumberTable [nb, 512] in
+        its [i][j][k] order and the records, bytes [n, record_bytes]."""
+        nb = self.mesh.nblocks
+        counts = np.zeros((nb, 512), np.int32)
+        n = C.c_int64()
+        L.check(self.lib.amps_gpu_get_particles_restart(self.ctx.h, C.byref(layout), L._ptr(counts, C.c_int32),
+                                                        None, 0, C.byref(n)), "get_particles_restart")
+        rec = np.zeros((n.value, layout.record_bytes), np.uint8)
+        L.check(self.lib.amps_gpu_get_particles_restart(self.ctx.h, C.byref(layout), L._ptr(counts, C.c_int32),
+                                                        rec.ctypes.data_as(C.c_void_p), n.value, C.byref(n)),
+                "get_particles_restart")
+        return counts, rec
+
+    def set_restart(self, layout, counts, records, precision=L.FP64):
+        """The inverse (ReadParticleDataBlock, pic_restart.cpp:430-512)."""
+        counts = np.ascontiguousarray(counts, np.int32)
+        records = np.ascontiguousarray(records, np.uint8)
+        L.check(self.lib.amps_gpu_set_particles_restart(self.ctx.h, C.byref(layout), L._ptr(counts, C.c_int32),
+                                                        len(records), records.ctypes.data_as(C.c_void_p),
+                                                        precision), "set_particles_restart")
+
     def MoveParticles(self, mover=L.MOVER_LAPENTA, bc_mode=L.BC_REFLECT, r_sphere=0.0,
                       compact=True, tile=0):
         o = L.PushOpts(mover, bc_mode, 1 if compact else 0, tile, r_sphere)

@@ -130,6 +130,40 @@ int64_t amps_gpu_particle_count(amps_ctx* ctx);
 int amps_gpu_get_particles(amps_ctx* ctx, double* x, double* v, int32_t* spec, int32_t* blk,
                            int32_t* cell, int64_t* id, int32_t* code);
 
+/* ParticleBuffer records (picParticleDataMacro.h): ParticleDataLength bytes
+ * per particle, next/prev list links at bytes 0 and 8, and the byte offsets
+ * of the mandatory fields: _PIC_PARTICLE_DATA__SPECIES_ID_OFFSET_ (8 bytes in
+ * the aligned state-vector mode, 1 otherwise), __VELOCITY_OFFSET_ and
+ * __POSITION_OFFSET_ (3 doubles each) and __WEIGHT_CORRECTION_OFFSET_ (-1
+ * when _INDIVIDUAL_PARTICLE_WEIGHT_MODE_ is off; the device carries no
+ * per-particle weight: exported as 1, ignored on import).  Optional fields
+ * are exported as zero bytes and ignored on import. */
+typedef struct {
+  int32_t record_bytes;
+  int32_t species_offset;
+  int32_t species_bytes;
+  int32_t velocity_offset;
+  int32_t position_offset;
+  int32_t weight_offset;
+} amps_aos_layout;
+/* The particle part of a restart file as SaveParticleDataBlock writes it
+ * (pic_restart.cpp:312-370), for all blocks at once: counts[512 b + 64 i +
+ * 8 j + k] = ParticleNumberTable[i][j][k] of block b, and the records of
+ * block 0, 1, ... each cell by cell in that i, j, k order, a cell's particles
+ * from the last of its list to the first (the device order within a cell is
+ * the list order from its head).  The caller writes AMRnodeID and the total
+ * of each block with particles in front of its table.  Compacts first if the
+ * particles are not sorted.  records may be NULL (counts only). */
+int amps_gpu_get_particles_restart(amps_ctx* ctx, const amps_aos_layout* layout, int32_t* counts,
+                                   void* records, int64_t cap_records, int64_t* n_records);
+/* The inverse, ReadParticleDataBlock (pic_restart.cpp:430-512): counts and
+ * records in the order above for all blocks (zero counts for blocks absent
+ * from the file); every record is pushed onto the head of its cell, so the
+ * device order within a cell is the file order reversed.  Particle ids are
+ * the record indices. */
+int amps_gpu_set_particles_restart(amps_ctx* ctx, const amps_aos_layout* layout, const int32_t* counts,
+                                   int64_t n_records, const void* records, int32_t precision);
+
 /* ---- mover ----------------------------------------------------------- */
 typedef struct {
   int32_t mover;      /* AMPS_MOVER_* */

@@ -121,6 +121,30 @@ def ref_effective_cutoff(grid, states, lower, upper, classify, abs_tol=1.0e-3, r
     return c.value, w.value, it.value, bool(un.value)
 
 
+def restart_order(blk, cell, nblocks):
+    """The order in which SaveParticleDataBlock (pic_restart.cpp:312-370)
+    writes particles given in list order (per cell, head first): blocks in
+    id order, cells in i, j, k loop order (k fastest, cell i + 8(j + 8k)),
+    each cell's list from its last particle to its first.  Returns the
+    per-block ParticleNumberTable [nblocks, 512] in [i][j][k] order and the
+    permutation of particle indices in file order."""
+    blk = np.asarray(blk)
+    cell = np.asarray(cell)
+    table = np.zeros((nblocks, 8, 8, 8), np.int32)
+    members = {}
+    for p in range(len(blk)):
+        members.setdefault((int(blk[p]), int(cell[p])), []).append(p)
+    order = []
+    for b in range(nblocks):
+        for i in range(8):
+            for j in range(8):
+                for k in range(8):
+                    lst = members.get((b, i + 8 * (j + 8 * k)), [])
+                    table[b, i, j, k] = len(lst)
+                    order.extend(reversed(lst))
+    return table.reshape(nblocks, 512), np.array(order, np.int64)
+
+
 def ref():
     """The reference's compiled gridless mover/dipole/band sources."""
     global _ref

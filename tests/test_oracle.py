@@ -95,6 +95,18 @@ def test_library_band_topology_matches_reference():
             assert list(got[i]) == list(out), st[i]
 
 
+def test_restart_order_small_case():
+    """SaveParticleDataBlock's order on a hand-checked case: two particles in
+    cell (i,j,k) = (1,0,0) and one in (0,0,1) of block 0, one in cell (0,0,0)
+    of block 1.  The loop visits (0,0,1) before (1,0,0) (k fastest) and
+    writes each cell's list reversed."""
+    blk = [0, 0, 0, 1]
+    cell = [1, 1, 64, 0]      # i + 8(j + 8k): (1,0,0), (1,0,0), (0,0,1), (0,0,0)
+    table, order = O.restart_order(blk, cell, 2)
+    assert table[0, 64 * 1] == 2 and table[0, 1] == 1 and table[1, 0] == 1 and table.sum() == 4
+    assert list(order) == [2, 1, 0, 3]
+
+
 def test_upper_cutoff_bin_semantics():
     # CutoffForDirectionUpperScan_GV: -1 if Rmax forbidden, Rmin index when all allowed
     assert O.upper_cutoff_bin([0, 0, 1, 1]) == 2

@@ -152,6 +152,10 @@ def test_config0_workload_steps(gpu_ctx):
 
 @pytest.fixture(scope="module")
 def dep_case():
+    return _dep_case_data()
+
+
+def _dep_case_data():
     mesh = M.central_octant_mesh()
     E, B = M.corner_fields(mesh)
     parts = M.per_cell_particles(mesh, [4, 4], [0.05, 0.005], drift=(0.01, 0.0, 0.0))
@@ -336,3 +340,36 @@ def test_push_fp32_error_against_fp64_oracle(gpu_ctx, mover):
     assert v_rel < 1e-4 and x_abs < 1e-5  # FP32 eps ~ 6e-8, 20 steps of rotation
     assert same_code > 0.999 and same_cell > 0.999
     assert abs(g.n - o.n) <= max(2, o.n // 5000)
+
+
+@pytest.mark.parametrize("layout_name,weight", [("aligned", False), ("aligned", True), ("packed", True)])
+def test_restart_records_round_trip(gpu_ctx, layout_name, weight):
+    """ParticleBuffer records in restart order: the export follows
+    SaveParticleDataBlock's order (oracle restatement) with the fields at the
+    picParticleDataMacro.h offsets; the import restores the device state
+    (positions, velocities, species, blocks, cells and the order within each
+    cell), after migration steps have shuffled the in-cell order."""
+    mesh, E, B, parts, sp, dt = _dep_case_data()
+    st = PicState(mesh, E, B, sp, ctx=gpu_ctx)
+    st.set_particles(parts)
+    for _ in range(2):
+        st.MoveParticles(mover=L.MOVER_LAPENTA, bc_mode=L.BC_REFLECT, r_sphere=0.0, compact=True)
+    g, _ = st.get_particles()
+    layout = L.aos_layout_aligned(weight) if layout_name == "aligned" else L.aos_layout_packed(weight)
+    counts, rec = st.get_restart(layout)
+    table, order = O.restart_order(g.blk, g.cell, mesh.nblocks)
+    assert np.array_equal(counts, table)
+    assert rec.shape == (g.n, layout.record_bytes)
+    def field(off, n, dt_):
+        return np.frombuffer(np.ascontiguousarray(rec[:, off:off + 8 * n]).tobytes(), dt_).reshape(-1, n)
+    assert np.array_equal(field(layout.velocity_offset, 3, np.float64), g.v[:, order].T)
+    assert np.array_equal(field(layout.position_offset, 3, np.float64), g.x[:, order].T)
+    assert np.array_equal(rec[:, layout.species_offset], g.spec[order].astype(np.uint8))
+    assert np.all(field(0, 2, np.int64) == -1)
+    if weight:
+        assert np.all(field(layout.weight_offset, 1, np.float64) == 1.0)
+    st2 = PicState(mesh, E, B, sp, ctx=gpu_ctx)
+    st2.set_restart(layout, counts, rec)
+    h, _ = st2.get_particles()
+    assert np.array_equal(h.x, g.x) and np.array_equal(h.v, g.v) and np.array_equal(h.spec, g.spec)
+    assert np.array_equal(h.blk, g.blk) and np.array_equal(h.cell, g.cell)
