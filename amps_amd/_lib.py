@@ -66,7 +66,7 @@ class TraceParams(C.Structure):
         ("dt_trace_s", C.c_double), ("max_time_s", C.c_double), ("max_steps", C.c_int64),
         ("max_distance_m", C.c_double), ("dphi_max", C.c_double), ("adaptive", C.c_int32),
         ("dist_limiter", C.c_int32), ("dt_floor", C.c_int32), ("outer_sphere", C.c_int32),
-        ("retry", C.c_int32), ("pad_", C.c_int32), ("box_min_Re", C.c_double * 3),
+        ("retry", C.c_int32), ("mover", C.c_int32), ("box_min_Re", C.c_double * 3),
         ("box_max_Re", C.c_double * 3), ("r_outer_Re", C.c_double), ("r_inner_Re", C.c_double),
         ("earth_radius_m", C.c_double), ("moment_Am2", C.c_double), ("m_hat", C.c_double * 3),
         ("B_ext_T", C.c_double * 3),

@@ -121,6 +121,161 @@ __device__ __forceinline__ bool seg_hits(const double* a, const double* b, doubl
   return dot3(q, q) <= r * r;
 }
 
+// Full-orbit movers of StepParticleChecked (MoverType numbering).
+constexpr int MV_BORIS = 0, MV_HC4 = 1, MV_RK2 = 2, MV_RK4 = 3, MV_RK6 = 4;
+
+// HigueraCaryMagneticMidpointStep, GridlessParticleMovers.cpp:723-757: the
+// Boris rotation with B sampled at the half-drift midpoint.
+__device__ __forceinline__ void hc_mid64(const amps_trace_params& P, double* x, double* p, double dt) {
+  const double m0 = P.m0_kg, q = P.q_C;
+  const double g0 = gamma64(p, m0);
+  const double iv0 = 1.0 / (g0 * m0);
+  const double h = 0.5 * dt;
+  const double xm[3] = {x[0] + h * (iv0 * p[0]), x[1] + h * (iv0 * p[1]), x[2] + h * (iv0 * p[2])};
+  double B[3];
+  dipole64(P, xm, B);
+  const double tc = (q * dt) / (2.0 * g0 * m0);
+  const double t[3] = {tc * B[0], tc * B[1], tc * B[2]};
+  const double t2 = dot3(t, t);
+  const double sc = 2.0 / (1.0 + t2);
+  const double s[3] = {sc * t[0], sc * t[1], sc * t[2]};
+  const double pm[3] = {p[0], p[1], p[2]};
+  const double pp[3] = {pm[0] + (pm[1] * t[2] - pm[2] * t[1]), pm[1] + (pm[2] * t[0] - pm[0] * t[2]),
+                        pm[2] + (pm[0] * t[1] - pm[1] * t[0])};
+  p[0] = pm[0] + (pp[1] * s[2] - pp[2] * s[1]);
+  p[1] = pm[1] + (pp[2] * s[0] - pp[0] * s[2]);
+  p[2] = pm[2] + (pp[0] * s[1] - pp[1] * s[0]);
+  const double iv1 = 1.0 / (gamma64(p, m0) * m0);
+#pragma unroll
+  for (int d = 0; d < 3; d++) x[d] = xm[d] + h * (iv1 * p[d]);
+}
+
+// Yoshida triple-jump weights of HC4StepUnchecked (GridlessParticleMovers.cpp:
+// 759-779): w1 = 1/(2 - 2^(1/3)), w0 = -2^(1/3)/(2 - 2^(1/3)).
+constexpr double HC4_CBRT2 = 1.25992104989487316477;
+constexpr double HC4_W1 = 1.0 / (2.0 - HC4_CBRT2);
+constexpr double HC4_W0 = -HC4_CBRT2 / (2.0 - HC4_CBRT2);
+
+__device__ __forceinline__ void hc4_64(const amps_trace_params& P, double* x, double* p, double dt) {
+  hc_mid64(P, x, p, HC4_W1 * dt);
+  hc_mid64(P, x, p, HC4_W0 * dt);
+  hc_mid64(P, x, p, HC4_W1 * dt);
+}
+
+// Butcher tables of the RK2 / RK4 / RK6 cases, GridlessParticleMovers.cpp:
+// 1061-1098 (midpoint, classical, Butcher's seven-stage sixth order).
+template <int MV>
+struct RKTable;
+template <>
+struct RKTable<MV_RK2> {
+  static constexpr int NS = 2;
+  static constexpr double a[2][2] = {{0.0, 0.0}, {0.5, 0.0}};
+  static constexpr double b[2] = {0.0, 1.0};
+};
+template <>
+struct RKTable<MV_RK4> {
+  static constexpr int NS = 4;
+  static constexpr double a[4][4] = {
+      {0.0, 0.0, 0.0, 0.0}, {0.5, 0.0, 0.0, 0.0}, {0.0, 0.5, 0.0, 0.0}, {0.0, 0.0, 1.0, 0.0}};
+  static constexpr double b[4] = {1.0 / 6.0, 1.0 / 3.0, 1.0 / 3.0, 1.0 / 6.0};
+};
+template <>
+struct RKTable<MV_RK6> {
+  static constexpr int NS = 7;
+  static constexpr double a[7][7] = {{0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0},
+                                     {1.0 / 3.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0},
+                                     {0.0, 2.0 / 3.0, 0.0, 0.0, 0.0, 0.0, 0.0},
+                                     {1.0 / 12.0, 1.0 / 3.0, -1.0 / 12.0, 0.0, 0.0, 0.0, 0.0},
+                                     {-1.0 / 16.0, 9.0 / 8.0, -3.0 / 16.0, -3.0 / 8.0, 0.0, 0.0, 0.0},
+                                     {0.0, 9.0 / 8.0, -3.0 / 8.0, -3.0 / 4.0, 0.5, 0.0, 0.0},
+                                     {9.0 / 44.0, -9.0 / 11.0, 63.0 / 44.0, 18.0 / 11.0, 0.0, -16.0 / 11.0, 0.0}};
+  static constexpr double b[7] = {11.0 / 120.0, 0.0, 27.0 / 40.0, 27.0 / 40.0, -4.0 / 15.0, -4.0 / 15.0, 11.0 / 120.0};
+};
+
+// RKStepGeneric, GridlessParticleMovers.cpp:655-692.  Every product of the
+// table is formed, zeros included, so that the sums round as the reference's.
+// A stage chord meeting the sphere leaves x at the stage point and p at its
+// start value; the final chord leaves both at y1.
+template <int MV>
+__device__ __forceinline__ bool rk_checked64(const amps_trace_params& P, double* x, double* p, double dt,
+                                             double rIn) {
+  using T = RKTable<MV>;
+  double kx[T::NS][3], kp[T::NS][3];
+  const double y0x[3] = {x[0], x[1], x[2]}, y0p[3] = {p[0], p[1], p[2]};
+  double prev[3] = {x[0], x[1], x[2]};
+#pragma unroll
+  for (int i = 0; i < T::NS; i++) {
+    double yx[3] = {y0x[0], y0x[1], y0x[2]}, yp[3] = {y0p[0], y0p[1], y0p[2]};
+#pragma unroll
+    for (int j = 0; j < i; j++) {
+      const double c = dt * T::a[i][j];
+#pragma unroll
+      for (int d = 0; d < 3; d++) {
+        yx[d] = yx[d] + c * kx[j][d];
+        yp[d] = yp[d] + c * kp[j][d];
+      }
+    }
+    if (seg_hits(prev, yx, rIn)) {
+#pragma unroll
+      for (int d = 0; d < 3; d++) x[d] = yx[d];
+      return false;
+    }
+    if (inside_inner_m(yx, rIn)) {
+#pragma unroll
+      for (int d = 0; d < 3; d++) x[d] = yx[d], p[d] = yp[d];
+      return false;
+    }
+#pragma unroll
+    for (int d = 0; d < 3; d++) prev[d] = yx[d];
+    // LorentzRhs, GridlessParticleMovers.cpp:151-157
+    double B[3];
+    dipole64(P, yx, B);
+    const double iv = 1.0 / (gamma64(yp, P.m0_kg) * P.m0_kg);
+    const double v[3] = {iv * yp[0], iv * yp[1], iv * yp[2]};
+    kx[i][0] = v[0], kx[i][1] = v[1], kx[i][2] = v[2];
+    kp[i][0] = P.q_C * (v[1] * B[2] - v[2] * B[1]);
+    kp[i][1] = P.q_C * (v[2] * B[0] - v[0] * B[2]);
+    kp[i][2] = P.q_C * (v[0] * B[1] - v[1] * B[0]);
+  }
+  double y1x[3] = {y0x[0], y0x[1], y0x[2]}, y1p[3] = {y0p[0], y0p[1], y0p[2]};
+#pragma unroll
+  for (int i = 0; i < T::NS; i++) {
+    const double c = dt * T::b[i];
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+      y1x[d] = y1x[d] + c * kx[i][d];
+      y1p[d] = y1p[d] + c * kp[i][d];
+    }
+  }
+#pragma unroll
+  for (int d = 0; d < 3; d++) x[d] = y1x[d], p[d] = y1p[d];
+  return !seg_hits(prev, y1x, rIn) && !inside_inner_m(y1x, rIn);
+}
+
+// StepParticleChecked, GridlessParticleMovers.cpp:1035-1098 (full-orbit
+// movers): false when the step met the inner sphere.
+template <int MV>
+__device__ __forceinline__ bool step_checked64(const amps_trace_params& P, double* x, double* p, double dt,
+                                               double rIn) {
+  if constexpr (MV == MV_BORIS) {
+    const double x0[3] = {x[0], x[1], x[2]};
+    boris64(P, x, p, dt);
+    return !seg_hits(x0, x, rIn) && !inside_inner_m(x, rIn);
+  } else if constexpr (MV == MV_HC4) {
+    // four positive macro-steps, each chord checked; the composition's
+    // negative inner stage is never classified
+    const double dtSub = dt / 4.0;
+    for (int i = 0; i < 4; i++) {
+      const double x0[3] = {x[0], x[1], x[2]};
+      hc4_64(P, x, p, dtSub);
+      if (seg_hits(x0, x, rIn) || inside_inner_m(x, rIn)) return false;
+    }
+    return true;
+  } else {
+    return rk_checked64<MV>(P, x, p, dt, rIn);
+  }
+}
+
 // SelectLegacyCutoffTraceDt, CutoffRigidityGridless.cpp:1086-1152 (switchable parts)
 __device__ __forceinline__ double select_dt64(const amps_trace_params& P, const double* x,
                                               const double* p, double dtTrace, double timeRemaining) {
@@ -169,6 +324,7 @@ __device__ __forceinline__ double select_dt64(const amps_trace_params& P, const 
   return dt;
 }
 
+template <int MV>
 struct Fp64Stepper {
   using Lane = Fp64Lane;
   static __device__ __forceinline__ void init(const TraceView& a, Lane& L, const double* x0,
@@ -220,8 +376,7 @@ struct Fp64Stepper {
     const double dt = select_dt64(P, x, p, L.dtTrace, L.maxTime - L.t);
     if (!(dt > 0.0) || !isfinite(dt)) return 6;
     const double xb[3] = {x[0], x[1], x[2]};
-    boris64(P, x, p, dt);
-    if (seg_hits(xb, x, rIn) || inside_inner_m(x, rIn)) return 1;
+    if (!step_checked64<MV>(P, x, p, dt, rIn)) return 1;
     if (!fin3(x) || !fin3(p)) return 8;
     const double dd[3] = {x[0] - xb[0], x[1] - xb[1], x[2] - xb[2]};
     const double seg = sqrt(dot3(dd, dd));
@@ -271,6 +426,24 @@ __device__ __forceinline__ void bfield32(const TraceView& a, const double* xd, f
   b[2] = a.f_cb * (k5 * z - a.f_mhat[2] * ir3) + a.f_bext[2];
 }
 
+// Boris rotation of u by t = tc b, in place (FP32)
+__device__ __forceinline__ void rotate32(float* u, const float* b, float tc) {
+#pragma clang fp contract(fast)
+  const float t[3] = {tc * b[0], tc * b[1], tc * b[2]};
+  const float t2 = t[0] * t[0] + t[1] * t[1] + t[2] * t[2];
+  const float sc = 2.0f * __builtin_amdgcn_rcpf(1.0f + t2);
+  const float s[3] = {sc * t[0], sc * t[1], sc * t[2]};
+  const float pp[3] = {u[0] + (u[1] * t[2] - u[2] * t[1]), u[1] + (u[2] * t[0] - u[0] * t[2]),
+                       u[2] + (u[0] * t[1] - u[1] * t[0])};
+  const float un[3] = {u[0] + (pp[1] * s[2] - pp[2] * s[1]), u[1] + (pp[2] * s[0] - pp[0] * s[2]),
+                       u[2] + (pp[0] * s[1] - pp[1] * s[0])};
+  u[0] = un[0];
+  u[1] = un[1];
+  u[2] = un[2];
+}
+
+// MV: MV_BORIS or MV_HC4 (the FP32 path has no RK movers)
+template <int MV>
 struct Fp32Stepper {
   using Lane = Fp32Lane;
   static __device__ __forceinline__ void init(const TraceView& a, Lane& L, const double* x0,
@@ -359,32 +532,52 @@ struct Fp32Stepper {
     }
     if (!(dt > 0.0) || !isfinite(dt)) return 6;
     const float dtf = (float)dt;
-    const float hf = 0.5f * dtf;
     const double xb[3] = {x[0], x[1], x[2]};
-    // drift - kick - drift (BorisStep with B sampled at the start point)
+    if constexpr (MV == MV_HC4) {
+      // four positive macro-steps of the triple jump H2(w1) H2(w0) H2(w1),
+      // H2 = half drift, B at the midpoint, rotation, half drift; each
+      // macro-step chord is checked against the inner sphere
+      const float dq = 0.25f * dtf;
+      float ig = igam;
+      for (int m = 0; m < 4; m++) {
+        const double xm0[3] = {x[0], x[1], x[2]};
+        const float rm = sqrtf((float)(x[0] * x[0] + x[1] * x[1] + x[2] * x[2]));
 #pragma unroll
-    for (int d = 0; d < 3; d++) x[d] += (double)(hf * (u[d] * igam));
-    const float tc = dtf * 0.5f * igam;
-    const float t[3] = {tc * b[0], tc * b[1], tc * b[2]};
-    const float t2 = t[0] * t[0] + t[1] * t[1] + t[2] * t[2];
-    const float sc = 2.0f * __builtin_amdgcn_rcpf(1.0f + t2);
-    const float s[3] = {sc * t[0], sc * t[1], sc * t[2]};
-    const float pp[3] = {u[0] + (u[1] * t[2] - u[2] * t[1]), u[1] + (u[2] * t[0] - u[0] * t[2]),
-                         u[2] + (u[0] * t[1] - u[1] * t[0])};
-    const float un[3] = {u[0] + (pp[1] * s[2] - pp[2] * s[1]), u[1] + (pp[2] * s[0] - pp[0] * s[2]),
-                         u[2] + (pp[0] * s[1] - pp[1] * s[0])};
-    u[0] = un[0];
-    u[1] = un[1];
-    u[2] = un[2];
-    const float igam2 = rsqrtf(1.0f + u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+        for (int k = 0; k < 3; k++) {
+          const float dk = (k == 1 ? (float)HC4_W0 : (float)HC4_W1) * dq;
+          const float hk = 0.5f * dk;
 #pragma unroll
-    for (int d = 0; d < 3; d++) x[d] += (double)(hf * (u[d] * igam2));
+          for (int d = 0; d < 3; d++) x[d] += (double)(hk * (u[d] * ig));
+          float bm[3];
+          bfield32(a, x, bm);
+          rotate32(u, bm, dk * 0.5f * ig);
+          ig = rsqrtf(1.0f + u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+#pragma unroll
+          for (int d = 0; d < 3; d++) x[d] += (double)(hk * (u[d] * ig));
+        }
+        const float dm[3] = {(float)(x[0] - xm0[0]), (float)(x[1] - xm0[1]), (float)(x[2] - xm0[2])};
+        const float segm = sqrtf(dm[0] * dm[0] + dm[1] * dm[1] + dm[2] * dm[2]);
+        if (a.rin_n > 0.0 && rm - segm <= (float)a.rin_n * 1.0001f)
+          if (seg_hits(xm0, x, a.rin_n) || inside_inner_m(x, a.rin_n)) return 1;
+      }
+    } else {
+      // drift - kick - drift (BorisStep with B sampled at the start point)
+      const float hf = 0.5f * dtf;
+#pragma unroll
+      for (int d = 0; d < 3; d++) x[d] += (double)(hf * (u[d] * igam));
+      rotate32(u, b, dtf * 0.5f * igam);
+      const float igam2 = rsqrtf(1.0f + u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+#pragma unroll
+      for (int d = 0; d < 3; d++) x[d] += (double)(hf * (u[d] * igam2));
+    }
     const float dd[3] = {(float)(x[0] - xb[0]), (float)(x[1] - xb[1]), (float)(x[2] - xb[2])};
     const float seg = sqrtf(dd[0] * dd[0] + dd[1] * dd[1] + dd[2] * dd[2]);
     // the chord can reach the inner sphere only if it starts within its
     // length of the sphere; the exact FP64 test runs only then
-    if (a.rin_n > 0.0 && sqrtf((float)r2) - seg <= (float)a.rin_n * 1.0001f)
-      if (seg_hits(xb, x, a.rin_n) || inside_inner_m(x, a.rin_n)) return 1;
+    if constexpr (MV == MV_BORIS) {
+      if (a.rin_n > 0.0 && sqrtf((float)r2) - seg <= (float)a.rin_n * 1.0001f)
+        if (seg_hits(xb, x, a.rin_n) || inside_inner_m(x, a.rin_n)) return 1;
+    }
     if (!fin3(x) || !isfinite(u[0]) || !isfinite(u[1]) || !isfinite(u[2])) return 8;
     L.s += isfinite(seg) ? (double)seg : 0.0;
     L.t += dt;
@@ -577,6 +770,9 @@ static int check_params(const amps_trace_params* P) {
   if (!(P->m0_kg > 0.0) || !(P->qabs_C > 0.0)) return fail(AMPS_E_ARG, "trace: mass and |charge| must be > 0");
   if (!(P->earth_radius_m > 0.0)) return fail(AMPS_E_ARG, "trace: earth_radius_m must be > 0");
   if (P->max_steps <= 0) return fail(AMPS_E_ARG, "trace: max_steps must be > 0");
+  if (P->mover < MV_BORIS || P->mover > MV_RK6)
+    return fail(AMPS_E_ARG, "trace: mover must be 0 BORIS, 1 HC4, 2 RK2, 3 RK4 or 4 RK6 "
+                            "(the guiding-centre and hybrid movers are not built)");
   return 0;
 }
 
@@ -612,6 +808,8 @@ static void fill_view(amps_ctx* c, TraceState* t, const amps_trace_params* P, Tr
 }
 
 static int launch(amps_ctx* c, TraceState* t, TraceView& a, int precision) {
+  if (precision == AMPS_FP32 && a.P.mover != MV_BORIS && a.P.mover != MV_HC4)
+    return fail(AMPS_E_ARG, "trace: the FP32 path runs the BORIS and HC4 movers only");
   HIP_TRY(hipMemsetAsync(t->d_queue, 0, 8, c->stream));
   int grid = t->grid_waves;
   if (grid <= 0) {
@@ -620,10 +818,21 @@ static int launch(amps_ctx* c, TraceState* t, TraceView& a, int precision) {
     grid = p.multiProcessorCount * 8;
   }
   tbegin(c);
-  if (precision == AMPS_FP32)
-    hipLaunchKernelGGL(k_trace_persistent<Fp32Stepper>, dim3(grid), dim3(256), 0, c->stream, a);
-  else
-    hipLaunchKernelGGL(k_trace_persistent<Fp64Stepper>, dim3(grid), dim3(256), 0, c->stream, a);
+  const dim3 g(grid), blk(256);
+  if (precision == AMPS_FP32) {
+    if (a.P.mover == MV_HC4)
+      hipLaunchKernelGGL(k_trace_persistent<Fp32Stepper<MV_HC4>>, g, blk, 0, c->stream, a);
+    else
+      hipLaunchKernelGGL(k_trace_persistent<Fp32Stepper<MV_BORIS>>, g, blk, 0, c->stream, a);
+  } else {
+    switch (a.P.mover) {
+      case MV_HC4: hipLaunchKernelGGL(k_trace_persistent<Fp64Stepper<MV_HC4>>, g, blk, 0, c->stream, a); break;
+      case MV_RK2: hipLaunchKernelGGL(k_trace_persistent<Fp64Stepper<MV_RK2>>, g, blk, 0, c->stream, a); break;
+      case MV_RK4: hipLaunchKernelGGL(k_trace_persistent<Fp64Stepper<MV_RK4>>, g, blk, 0, c->stream, a); break;
+      case MV_RK6: hipLaunchKernelGGL(k_trace_persistent<Fp64Stepper<MV_RK6>>, g, blk, 0, c->stream, a); break;
+      default: hipLaunchKernelGGL(k_trace_persistent<Fp64Stepper<MV_BORIS>>, g, blk, 0, c->stream, a); break;
+    }
+  }
   HIP_TRY(hipGetLastError());
   tend(c, AMPS_K_TRACE);
   return 0;

@@ -33,6 +33,10 @@ TERM_NAMES = ["OUTER_BOUNDARY_ALLOWED", "INNER_BOUNDARY_FORBIDDEN",
               "MAGNETICALLY_TRAPPED_FORBIDDEN", "TIME_LIMIT", "STEP_LIMIT", "DISTANCE_LIMIT",
               "INVALID_TIME_STEP", "INVALID_FIELD", "NUMERICAL_FAILURE"]
 
+# MoverType of the full-orbit movers (GridlessParticleMovers.h:300-310); the
+# FP32 tracer runs BORIS and HC4.
+MOVERS = {"BORIS": 0, "HC4": 1, "RK2": 2, "RK4": 3, "RK6": 4}
+
 SCAN_FULL = 0
 SCAN_UPPER = 1
 BIN_ABOVE = -2
@@ -49,8 +53,12 @@ def make_params(*, tilt_deg=0.0, moment_scale=1.0, bz_T=0.0, mass_amu=PROTON_AMU
                 reversed_charge=False, dt_trace=1.0, max_time=600.0, max_steps=300000,
                 max_distance_Re=400.0, dphi=0.15, adaptive=True, dist_limiter=True, dt_floor=True,
                 outer_sphere=False, box_Re=35.0, r_outer_Re=35.0, r_inner_Re=1.0,
-                earth_radius_m=RE_AMPS, retry=True):
+                earth_radius_m=RE_AMPS, retry=True, mover="BORIS"):
     P = L.TraceParams()
+    if mover not in MOVERS:
+        raise ValueError("mover must be one of %s (guiding-centre and hybrid movers are not built)"
+                         % ", ".join(MOVERS))
+    P.mover = MOVERS[mover]
     q = charge_e * ELECTRON_CHARGE
     P.q_C = -q if reversed_charge else q
     P.qabs_C = abs(q)

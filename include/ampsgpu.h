@@ -215,10 +215,13 @@ int amps_gpu_update_j_mass_matrix(amps_ctx* ctx, double dt, double* J_out, doubl
 /* ---- gridless cutoff tracer ------------------------------------------ */
 /* Parameters of one trace (SI).  The FP64 path is TraceTrajectoryImpl +
  * TraceTrajectoryWithSingleRetry (CutoffRigidityGridless.cpp:1246-1496) with
- * the BORIS mover (GridlessParticleMovers.cpp:781-801, 1035-1046), the
- * legacy step selector (:1086-1152) whose gyro budget, distance limiter and
- * floor are switchable, the analytic dipole (DipoleInterface.h:99-125) plus a
- * uniform field, and a box or spherical outer boundary. */
+ * StepParticleChecked (GridlessParticleMovers.cpp:1035-1098) of the selected
+ * full-orbit mover: BORIS (:781-801), HC4 (:723-779, four checked positive
+ * macro-steps) or RK2/RK4/RK6 (RKStepGeneric :655-692, stage chords
+ * checked); the legacy step selector (CutoffRigidityGridless.cpp:1086-1152)
+ * whose gyro budget, distance limiter and floor are switchable, the analytic
+ * dipole (DipoleInterface.h:99-125) plus a uniform field, and a box or
+ * spherical outer boundary.  The FP32 path runs BORIS and HC4. */
 typedef struct {
   double q_C;            /* signed backtrace charge [C] */
   double qabs_C;         /* |physical charge| for rigidity conversion [C] */
@@ -233,7 +236,8 @@ typedef struct {
   int32_t dt_floor;      /* legacy max(1e-12, 1e-9*DT, 100 km / v) floor */
   int32_t outer_sphere;  /* 0: box, 1: sphere of radius r_outer_Re */
   int32_t retry;         /* single retry on numerical failure */
-  int32_t pad_;
+  int32_t mover;         /* MoverType (GridlessParticleMovers.h:300-310): 0 BORIS,
+                            1 HC4, 2 RK2, 3 RK4, 4 RK6; GC2..HYBRID rejected */
   double box_min_Re[3], box_max_Re[3];
   double r_outer_Re, r_inner_Re;
   double earth_radius_m; /* length of one Re for all geometry conversions */

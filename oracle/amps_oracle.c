@@ -914,6 +914,147 @@ static int segment_hits_inner(const double a[3], const double b[3], double r) {
   return dot3(q, q) <= r * r;
 }
 
+/* HigueraCaryMagneticMidpointStep — GridlessParticleMovers.cpp:723-757: half
+ * drift with v(p_n), B at the midpoint, Boris/Cayley rotation with gamma(p_n),
+ * half drift with v(p_{n+1}). */
+static void hc_midpoint_step(const orc_trace_params* P, double x[3], double p[3], double dt) {
+  double m0 = P->m0_kg, q = P->q_C;
+  double g0 = gamma_p(p, m0);
+  double iv0 = 1.0 / (g0 * m0);
+  double v0[3] = {iv0 * p[0], iv0 * p[1], iv0 * p[2]};
+  double h = 0.5 * dt;
+  double xm[3] = {x[0] + h * v0[0], x[1] + h * v0[1], x[2] + h * v0[2]};
+  double B[3];
+  orc_dipole_B(P, xm, B);
+  double gamma0 = gamma_p(p, m0);
+  double tc = (q * dt) / (2.0 * gamma0 * m0);
+  double t[3] = {tc * B[0], tc * B[1], tc * B[2]};
+  double t2 = dot3(t, t);
+  double sc = 2.0 / (1.0 + t2);
+  double s[3] = {sc * t[0], sc * t[1], sc * t[2]};
+  double pm[3] = {p[0], p[1], p[2]};
+  double pp[3] = {pm[0] + (pm[1] * t[2] - pm[2] * t[1]), pm[1] + (pm[2] * t[0] - pm[0] * t[2]),
+                  pm[2] + (pm[0] * t[1] - pm[1] * t[0])};
+  p[0] = pm[0] + (pp[1] * s[2] - pp[2] * s[1]);
+  p[1] = pm[1] + (pp[2] * s[0] - pp[0] * s[2]);
+  p[2] = pm[2] + (pp[0] * s[1] - pp[1] * s[0]);
+  double g1 = gamma_p(p, m0);
+  double iv1 = 1.0 / (g1 * m0);
+  double v1[3] = {iv1 * p[0], iv1 * p[1], iv1 * p[2]};
+  for (int d = 0; d < 3; d++) x[d] = xm[d] + h * v1[d];
+}
+
+/* HC4StepUnchecked — GridlessParticleMovers.cpp:759-779: Yoshida triple jump
+ * H2(w1 dt) H2(w0 dt) H2(w1 dt), w1 = 1/(2 - 2^(1/3)), w0 = -2^(1/3)/(2 - 2^(1/3)). */
+void orc_hc4_step(const orc_trace_params* P, double x[3], double p[3], double dt) {
+  const double cbrt2 = 1.25992104989487316477;
+  const double w1 = 1.0 / (2.0 - cbrt2);
+  const double w0 = -cbrt2 / (2.0 - cbrt2);
+  hc_midpoint_step(P, x, p, w1 * dt);
+  hc_midpoint_step(P, x, p, w0 * dt);
+  hc_midpoint_step(P, x, p, w1 * dt);
+}
+
+/* LorentzRhs — GridlessParticleMovers.cpp:151-157: (dx/dt, dp/dt) = (v, q v x B). */
+static void lorentz_rhs(const orc_trace_params* P, const double x[3], const double p[3], double kx[3],
+                        double kp[3]) {
+  double B[3];
+  orc_dipole_B(P, x, B);
+  double iv = 1.0 / (gamma_p(p, P->m0_kg) * P->m0_kg);
+  double v[3] = {iv * p[0], iv * p[1], iv * p[2]};
+  double q = P->q_C;
+  kx[0] = v[0], kx[1] = v[1], kx[2] = v[2];
+  kp[0] = q * (v[1] * B[2] - v[2] * B[1]);
+  kp[1] = q * (v[2] * B[0] - v[0] * B[2]);
+  kp[2] = q * (v[0] * B[1] - v[1] * B[0]);
+}
+
+/* Butcher tables of StepParticleChecked's RK2 (midpoint), RK4 (classical)
+ * and RK6 (Butcher's seven-stage sixth-order method),
+ * GridlessParticleMovers.cpp:1061-1098.  a is ns x ns, row-major. */
+static const double RK2_A[4] = {0.0, 0.0, 0.5, 0.0};
+static const double RK2_B[2] = {0.0, 1.0};
+static const double RK4_A[16] = {0.0, 0.0, 0.0, 0.0, 0.5, 0.0, 0.0, 0.0, 0.0, 0.5, 0.0, 0.0, 0.0, 0.0, 1.0, 0.0};
+static const double RK4_B[4] = {1.0 / 6.0, 1.0 / 3.0, 1.0 / 3.0, 1.0 / 6.0};
+static const double RK6_A[49] = {
+    0.0,         0.0,        0.0,          0.0,         0.0, 0.0,          0.0,
+    1.0 / 3.0,   0.0,        0.0,          0.0,         0.0, 0.0,          0.0,
+    0.0,         2.0 / 3.0,  0.0,          0.0,         0.0, 0.0,          0.0,
+    1.0 / 12.0,  1.0 / 3.0,  -1.0 / 12.0,  0.0,         0.0, 0.0,          0.0,
+    -1.0 / 16.0, 9.0 / 8.0,  -3.0 / 16.0,  -3.0 / 8.0,  0.0, 0.0,          0.0,
+    0.0,         9.0 / 8.0,  -3.0 / 8.0,   -3.0 / 4.0,  0.5, 0.0,          0.0,
+    9.0 / 44.0,  -9.0 / 11.0, 63.0 / 44.0, 18.0 / 11.0, 0.0, -16.0 / 11.0, 0.0};
+static const double RK6_B[7] = {11.0 / 120.0, 0.0, 27.0 / 40.0, 27.0 / 40.0, -4.0 / 15.0, -4.0 / 15.0, 11.0 / 120.0};
+
+/* RKStepGeneric — GridlessParticleMovers.cpp:655-692: explicit RK with the
+ * chord between consecutive stage positions and the final chord checked
+ * against the inner sphere.  On a stage chord hit x takes the stage position
+ * and p is left at its start value; on the final chord both take y1. */
+static int rk_step_checked(const orc_trace_params* P, int ns, const double* a, const double* b, double x[3],
+                           double p[3], double dt, double rIn) {
+  double kx[7][3], kp[7][3];
+  double y0x[3] = {x[0], x[1], x[2]}, y0p[3] = {p[0], p[1], p[2]};
+  double prev[3] = {x[0], x[1], x[2]};
+  for (int i = 0; i < ns; i++) {
+    double yx[3] = {y0x[0], y0x[1], y0x[2]}, yp[3] = {y0p[0], y0p[1], y0p[2]};
+    for (int j = 0; j < i; j++) {
+      double c = dt * a[i * ns + j];
+      for (int d = 0; d < 3; d++) {
+        yx[d] = yx[d] + c * kx[j][d];
+        yp[d] = yp[d] + c * kp[j][d];
+      }
+    }
+    if (segment_hits_inner(prev, yx, rIn)) {
+      for (int d = 0; d < 3; d++) x[d] = yx[d];
+      return 0;
+    }
+    if (inside_inner_m(yx, rIn)) {
+      for (int d = 0; d < 3; d++) x[d] = yx[d], p[d] = yp[d];
+      return 0;
+    }
+    for (int d = 0; d < 3; d++) prev[d] = yx[d];
+    lorentz_rhs(P, yx, yp, kx[i], kp[i]);
+  }
+  double y1x[3] = {y0x[0], y0x[1], y0x[2]}, y1p[3] = {y0p[0], y0p[1], y0p[2]};
+  for (int i = 0; i < ns; i++) {
+    double c = dt * b[i];
+    for (int d = 0; d < 3; d++) {
+      y1x[d] = y1x[d] + c * kx[i][d];
+      y1p[d] = y1p[d] + c * kp[i][d];
+    }
+  }
+  for (int d = 0; d < 3; d++) x[d] = y1x[d], p[d] = y1p[d];
+  if (segment_hits_inner(prev, y1x, rIn)) return 0;
+  if (inside_inner_m(y1x, rIn)) return 0;
+  return 1;
+}
+
+/* StepParticleChecked — GridlessParticleMovers.cpp:1035-1098 for the
+ * full-orbit movers.  BORIS: one step, chord checked.  HC4: four positive
+ * macro-steps of dt/4, each chord checked (the composition's negative inner
+ * stage is never classified). */
+int orc_step_checked(const orc_trace_params* P, double x[3], double p[3], double dt, double rIn) {
+  switch (P->mover) {
+    case 1: {
+      const double dtSub = dt / 4.0;
+      for (int i = 0; i < 4; i++) {
+        double x0[3] = {x[0], x[1], x[2]};
+        orc_hc4_step(P, x, p, dtSub);
+        if (segment_hits_inner(x0, x, rIn) || inside_inner_m(x, rIn)) return 0;
+      }
+      return 1;
+    }
+    case 2: return rk_step_checked(P, 2, RK2_A, RK2_B, x, p, dt, rIn);
+    case 3: return rk_step_checked(P, 4, RK4_A, RK4_B, x, p, dt, rIn);
+    case 4: return rk_step_checked(P, 7, RK6_A, RK6_B, x, p, dt, rIn);
+    default: {
+      double x0[3] = {x[0], x[1], x[2]};
+      orc_boris_step(P, x, p, dt);
+      return !segment_hits_inner(x0, x, rIn) && !inside_inner_m(x, rIn);
+    }
+  }
+}
+
 /* SelectLegacyCutoffTraceDt — CutoffRigidityGridless.cpp:1086-1152.  The
  * gyro budget, limiter and floor are switchable; with outer_sphere the
  * boundary distance is the distance to the outer sphere. */
@@ -967,8 +1108,8 @@ static double select_dt(const orc_trace_params* P, const double x[3], const doub
 
 static int isfin3(const double a[3]) { return isfinite(a[0]) && isfinite(a[1]) && isfinite(a[2]); }
 
-/* TraceTrajectoryImpl — CutoffRigidityGridless.cpp:1246-1457 (BORIS mover,
- * legacy policy, trap detector off).  Termination values follow
+/* TraceTrajectoryImpl — CutoffRigidityGridless.cpp:1246-1457 (full-orbit
+ * mover P->mover, legacy policy, trap detector off).  Termination values follow
  * util/TrajectoryTermination.h:10-21. */
 static void trace_impl(const orc_trace_params* P, const double x0[3], const double dir[3],
                        double R_GV, double dtTrace, int64_t maxSteps, double maxTime,
@@ -999,8 +1140,7 @@ static void trace_impl(const orc_trace_params* P, const double x0[3], const doub
     double dt = select_dt(P, x, p, dtTrace, maxTime - t);
     if (!(dt > 0.0) || !isfinite(dt)) { term = 6; break; }
     double xb[3] = {x[0], x[1], x[2]};
-    orc_boris_step(P, x, p, dt);
-    if (segment_hits_inner(xb, x, rInner) || inside_inner_m(x, rInner)) { term = 1; break; }
+    if (!orc_step_checked(P, x, p, dt, rInner)) { term = 1; break; }
     if (!isfin3(x) || !isfin3(p)) { term = 8; break; }
     double dd[3] = {x[0] - xb[0], x[1] - xb[1], x[2] - xb[2]};
     double seg = sqrt(dot3(dd, dd));

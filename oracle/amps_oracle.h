@@ -92,7 +92,7 @@ typedef struct {
   int32_t dt_floor;      /* legacy max(1e-12, 1e-9*DT, 100 km / v) floor */
   int32_t outer_sphere;  /* 0: box, 1: sphere of radius r_outer_Re */
   int32_t retry;         /* single retry on numerical failure */
-  int32_t pad_;
+  int32_t mover;         /* MoverType: 0 BORIS, 1 HC4, 2 RK2, 3 RK4, 4 RK6 */
   double box_min_Re[3], box_max_Re[3];
   double r_outer_Re, r_inner_Re;
   double earth_radius_m; /* length of one Re for all geometry conversions */
@@ -111,6 +111,12 @@ typedef struct {
 
 void orc_dipole_B(const orc_trace_params* P, const double x[3], double B[3]);
 void orc_boris_step(const orc_trace_params* P, double x[3], double p[3], double dt);
+/* HC4Step (unchecked fourth-order composition), in place. */
+void orc_hc4_step(const orc_trace_params* P, double x[3], double p[3], double dt);
+/* StepParticleChecked with mover P->mover: 1 when the step completed, 0 when
+ * it met the inner sphere of radius r_inner_m (x, p then hold what the
+ * reference leaves in them). */
+int orc_step_checked(const orc_trace_params* P, double x[3], double p[3], double dt, double r_inner_m);
 void orc_trace(const orc_trace_params* P, const double x0[3], const double dir_unit[3],
                double R_GV, orc_trace_result* res);
 void orc_trace_batch(const orc_trace_params* P, int64_t n, const double* x0,

@@ -28,7 +28,7 @@ class TraceParams(C.Structure):
         ("dt_trace_s", C.c_double), ("max_time_s", C.c_double), ("max_steps", C.c_int64),
         ("max_distance_m", C.c_double), ("dphi_max", C.c_double), ("adaptive", C.c_int32),
         ("dist_limiter", C.c_int32), ("dt_floor", C.c_int32), ("outer_sphere", C.c_int32),
-        ("retry", C.c_int32), ("pad_", C.c_int32), ("box_min_Re", C.c_double * 3),
+        ("retry", C.c_int32), ("mover", C.c_int32), ("box_min_Re", C.c_double * 3),
         ("box_max_Re", C.c_double * 3), ("r_outer_Re", C.c_double), ("r_inner_Re", C.c_double),
         ("earth_radius_m", C.c_double), ("moment_Am2", C.c_double), ("m_hat", C.c_double * 3),
         ("B_ext_T", C.c_double * 3),
@@ -76,6 +76,9 @@ def lib():
                                   P(dbl), i32]
         l.orc_dipole_B.argtypes = [P(TraceParams), P(dbl), P(dbl)]
         l.orc_boris_step.argtypes = [P(TraceParams), P(dbl), P(dbl), dbl]
+        l.orc_hc4_step.argtypes = [P(TraceParams), P(dbl), P(dbl), dbl]
+        l.orc_step_checked.argtypes = [P(TraceParams), P(dbl), P(dbl), dbl, dbl]
+        l.orc_step_checked.restype = i32
         l.orc_trace_batch.argtypes = [P(TraceParams), i64, P(dbl), P(dbl), P(dbl), vp, i32]
         l.orc_rigidity_grid.argtypes = [dbl, dbl, i32, P(dbl)]
         l.orc_upper_cutoff_bin.argtypes = [P(C.c_int), i32]
@@ -157,6 +160,9 @@ def ref():
         r.ref_boris_steps.argtypes = [C.c_int, P(dbl), P(dbl), dbl, dbl, dbl, P(dbl), P(dbl)]
         r.ref_step_checked.argtypes = [P(dbl), P(dbl), dbl, dbl, dbl, P(dbl), dbl]
         r.ref_step_checked.restype = C.c_int
+        r.ref_mover_steps.argtypes = [C.c_int, C.c_int, P(dbl), P(dbl), dbl, dbl, dbl, P(dbl), P(dbl)]
+        r.ref_step_checked_mover.argtypes = [C.c_int, P(dbl), P(dbl), dbl, dbl, dbl, P(dbl), dbl]
+        r.ref_step_checked_mover.restype = C.c_int
         r.ref_band_topology.argtypes = [C.c_int, P(C.c_int), P(C.c_int)]
         r.ref_refine_cutoff.argtypes = [dbl, dbl, dbl, dbl, REF_CLASSIFY, C.c_void_p, P(dbl),
                                         P(C.c_int), P(C.c_int)]

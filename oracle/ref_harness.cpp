@@ -70,6 +70,36 @@ int ref_step_checked(double x[3], double p[3], double q, double m0, double dt,
   return ok ? 1 : 0;
 }
 
+// n consecutive StepParticle(mover) calls (MoverType numbering: 0 BORIS,
+// 1 HC4, 2 RK2, 3 RK4, 4 RK6), trajectory as in ref_boris_steps.
+void ref_mover_steps(int mover, int n, double x[3], double p[3], double q, double m0, double dt,
+                     const double bext[3], double* traj) {
+  DipolePlusUniform f;
+  for (int d = 0; d < 3; d++) f.bext[d] = bext[d];
+  V3 xv{x[0], x[1], x[2]}, pv{p[0], p[1], p[2]};
+  for (int i = 0; i < n; i++) {
+    StepParticle(static_cast<MoverType>(mover), xv, pv, q, m0, dt, f);
+    if (traj) {
+      traj[6 * i + 0] = xv.x, traj[6 * i + 1] = xv.y, traj[6 * i + 2] = xv.z;
+      traj[6 * i + 3] = pv.x, traj[6 * i + 4] = pv.y, traj[6 * i + 5] = pv.z;
+    }
+  }
+  x[0] = xv.x, x[1] = xv.y, x[2] = xv.z;
+  p[0] = pv.x, p[1] = pv.y, p[2] = pv.z;
+}
+
+// StepParticleChecked(mover) once; x,p updated in place.
+int ref_step_checked_mover(int mover, double x[3], double p[3], double q, double m0, double dt,
+                           const double bext[3], double r_inner) {
+  DipolePlusUniform f;
+  for (int d = 0; d < 3; d++) f.bext[d] = bext[d];
+  V3 xv{x[0], x[1], x[2]}, pv{p[0], p[1], p[2]};
+  bool ok = StepParticleChecked(static_cast<MoverType>(mover), xv, pv, q, m0, dt, f, r_inner);
+  x[0] = xv.x, x[1] = xv.y, x[2] = xv.z;
+  p[0] = pv.x, p[1] = pv.y, p[2] = pv.z;
+  return ok ? 1 : 0;
+}
+
 // AnalyzeCutoffBandSamples over states numbered as CutoffSampleState
 // (0 physical forbidden, 1 allowed, 2 unresolved).
 void ref_band_topology(int n, const int* states, int* out) {

@@ -62,6 +62,110 @@ def test_boris_step_matches_reference_bitwise(tilt, bz):
         assert np.array_equal(B1, B2)
 
 
+MOVERS = {"BORIS": 0, "HC4": 1, "RK2": 2, "RK4": 3, "RK6": 4}
+
+
+def _random_state(rng):
+    x = rng.normal(size=3)
+    x *= rng.uniform(1.05, 8.0) * TS.RE_DIPOLE / np.linalg.norm(x)
+    d = rng.normal(size=3)
+    d /= np.linalg.norm(d)
+    R = 10 ** rng.uniform(-1, 1.3)
+    return x, d * (R * 1e9 * TS.QE) / TS.C
+
+
+@needs_ref
+@pytest.mark.parametrize("mover", list(MOVERS))
+def test_checked_movers_match_reference_bitwise(mover):
+    """StepParticleChecked for every full-orbit mover, bit for bit, with inner
+    spheres placed so that about half the steps meet them (stage-chord,
+    final-chord and macro-step hits leave x, p as the reference does)."""
+    r = O.ref()
+    r.ref_set_dipole(1.0, 17.0)
+    P = TS.params(tilt_deg=17.0, moment_scale=1.0, bz_T=-3e-9)
+    P.mover = MOVERS[mover]
+    bext = np.array([0.0, 0.0, -3e-9])
+    rng = np.random.default_rng(40 + MOVERS[mover])
+    hits = 0
+    for trial in range(200):
+        x, p = _random_state(rng)
+        dt = rng.uniform(2e-3, 1e-1)
+        rin = np.linalg.norm(x) * (rng.uniform(0.97, 0.9999) if trial % 2 else 0.5)
+        xr, pr, xo, po = x.copy(), p.copy(), x.copy(), p.copy()
+        a = r.ref_step_checked_mover(P.mover, O._p(xr, O.C.c_double), O._p(pr, O.C.c_double), P.q_C, P.m0_kg,
+                                     dt, O._p(bext, O.C.c_double), rin)
+        b = O.lib().orc_step_checked(O.C.byref(P), O._p(xo, O.C.c_double), O._p(po, O.C.c_double), dt, rin)
+        assert a == b and np.array_equal(xr, xo) and np.array_equal(pr, po), (trial, a, b)
+        hits += a == 0
+    assert 20 <= hits <= 180
+
+
+@needs_ref
+def test_hc4_step_matches_reference_bitwise():
+    """The public HC4Step (unchecked) over 40 consecutive steps."""
+    r = O.ref()
+    r.ref_set_dipole(1.0, 0.0)
+    P = TS.params(tilt_deg=0.0, moment_scale=1.0)
+    bext = np.zeros(3)
+    rng = np.random.default_rng(12)
+    for trial in range(10):
+        x, p = _random_state(rng)
+        dt = rng.uniform(1e-3, 2e-2)
+        xr, pr = x.copy(), p.copy()
+        traj = np.zeros(6 * 40)
+        r.ref_mover_steps(1, 40, O._p(xr, O.C.c_double), O._p(pr, O.C.c_double), P.q_C, P.m0_kg, dt,
+                          O._p(bext, O.C.c_double), O._p(traj, O.C.c_double))
+        for i in range(40):
+            O.lib().orc_hc4_step(O.C.byref(P), O._p(x, O.C.c_double), O._p(p, O.C.c_double), dt)
+            assert np.array_equal(x, traj[6 * i:6 * i + 3]) and np.array_equal(p, traj[6 * i + 3:6 * i + 6])
+
+
+def _orbit(P, mover, n, T):
+    P.mover = mover
+    x = np.array([3.0, 0.5, 0.8]) * TS.RE_DIPOLE
+    p = np.array([0.3, -0.5, 0.8])
+    p *= (0.5e9 * TS.QE / TS.C) / np.linalg.norm(p)
+    for _ in range(n):
+        O.lib().orc_step_checked(O.C.byref(P), O._p(x, O.C.c_double), O._p(p, O.C.c_double), T / n, 0.0)
+    return x, p
+
+
+def test_mover_convergence_orders():
+    """Global position error at a fixed time (20 gyro-radians of a 0.5 GV
+    proton at 3 Re in a tilted dipole) against an RK6 solution with 16x
+    smaller steps: halving the step divides it by 2^order.  BORIS samples B at
+    the step start (first order in a non-uniform field); HC4's triple jump of
+    the symmetric midpoint map is fourth order only with Yoshida's weights;
+    the RK tables give orders 2, 4 and 6."""
+    P = TS.params(tilt_deg=10.0, moment_scale=1.0)
+    T = 0.2
+    xr, _ = _orbit(P, 4, 12800, T)
+    expect = {"BORIS": (0.8, 1.5), "HC4": (3.8, 4.2), "RK2": (1.8, 2.9), "RK4": (3.7, 4.7), "RK6": (5.6, 7.0)}
+    for name, (lo, hi) in expect.items():
+        e = []
+        for n in (200, 400, 800):
+            x, _ = _orbit(P, MOVERS[name], n, T)
+            e.append(np.linalg.norm(x - xr))
+        order = math.log2(e[1] / e[2])
+        assert lo <= order <= hi, (name, e, order)
+
+
+def test_hc4_reversible_and_norm_preserving():
+    """HC4 in a static field: |p| kept to roundoff, and a step of -dt undoes
+    a step of dt (the composition is symmetric)."""
+    P = TS.params(tilt_deg=23.0, moment_scale=1.0, bz_T=4e-9)
+    rng = np.random.default_rng(2)
+    for _ in range(20):
+        x, p = _random_state(rng)
+        dt = rng.uniform(1e-3, 5e-2)
+        x1, p1 = x.copy(), p.copy()
+        O.lib().orc_hc4_step(O.C.byref(P), O._p(x1, O.C.c_double), O._p(p1, O.C.c_double), dt)
+        assert abs(np.linalg.norm(p1) / np.linalg.norm(p) - 1) < 1e-14
+        O.lib().orc_hc4_step(O.C.byref(P), O._p(x1, O.C.c_double), O._p(p1, O.C.c_double), -dt)
+        assert np.linalg.norm(x1 - x) < 1e-12 * np.linalg.norm(x)
+        assert np.linalg.norm(p1 - p) < 1e-12 * np.linalg.norm(p)
+
+
 @needs_ref
 def test_band_topology_matches_reference():
     r = O.ref()
@@ -205,6 +309,36 @@ def test_c12_boris_stormer_rows():
         assert abs(rc / rc_ref - 1) <= tol, (lat, rc, rc_ref)
         n += 1
     assert n >= 12
+
+
+@pytest.mark.parametrize("mover,rows_of", [("HC4", "RK4"), ("RK4", "RK4"), ("RK6", "RK6")])
+def test_c12_high_order_mover_rows(mover, rows_of):
+    """C12 rows for the fourth- and sixth-order movers (HC4 has no rows of
+    its own and is held to the RK4 rows).  At the equator, where vertical
+    incidence has no penumbra, the upper cutoff meets the table's own rel_tol
+    (1e-3 RK4, 5e-4 RK6).  Off the equator the highest forbidden->allowed
+    transition lies in the penumbra and moves by several percent between
+    movers and step sizes, so those rows are held to the C1 deck's 2.5e-1."""
+    import csv
+    P = TS.c1_params()
+    P.dphi_max = 0.02
+    P.mover = MOVERS[mover]
+    rmin, rmax = TS.c1_rigidity_range()
+    with open(os.path.join(GOLD, "C12_stormer_movers.csv")) as f:
+        rows = [r for r in csv.DictReader(f) if r["mover"] == rows_of]
+    assert len(rows) == 14
+    n_eq = 0
+    for r in rows:
+        lat = float(r["lat_deg"])
+        rc_ref = float(r["Rc_stormer_GV"])
+        if rc_ref < rmin or lat < 0.0:
+            continue
+        x0 = TS.shell_point(0.0, lat, float(r["alt_km"]), TS.RE_AMPS)
+        rc = TS.oracle_cutoff(P, x0, -x0 / np.linalg.norm(x0), rmin, rmax, 80, rel_tol=1e-5)
+        tol = float(r["rel_tol"]) if lat == 0.0 else 2.5e-1
+        assert abs(rc / rc_ref - 1) <= tol, (lat, rc, rc_ref)
+        n_eq += lat == 0.0
+    assert n_eq == 2
 
 
 def test_c4_exit_classification_and_invariants():

@@ -247,3 +247,67 @@ def test_fp32_tracer_agreement(gpu_ctx):
     print(f"FP32 vs FP64: state agreement {agree:.5f}, bins off by >1: {frac_gt1:.5f}")
     assert agree >= 0.97
     assert frac_gt1 <= 0.03
+
+
+@pytest.mark.parametrize("mover", ["HC4", "RK2", "RK4", "RK6"])
+def test_trace_movers_fp64_bitexact(gpu_ctx, mover):
+    """Every full-orbit mover of StepParticleChecked through the persistent
+    kernel, bit-exact with the oracle: the legacy deck (inner-sphere hits
+    through stage and macro-step chords, outer-box exits, time limits) with
+    starts spread from 1.05 to 3 Re."""
+    P = CU.legacy_params(dphi=0.15, mover=mover)
+    P.max_time_s = 20.0
+    x, d, R = _random_tasks(1500, 7 + CU.MOVERS[mover], rhi=3.0)
+    print(f"trace mover {mover} n={len(R)}", flush=True)
+    g, steps = CU.trace_batch(gpu_ctx, P, x, d, R, precision=L.FP64)
+    o = O.trace_batch(_oparams(P), x, d, R)
+    _same(g, o)
+    assert steps == int(o["steps"].sum())
+    terms = set(np.unique(o["termination"]).tolist())
+    assert {0, 1} <= terms, terms
+
+
+def test_trace_mover_reaches_scan(gpu_ctx):
+    """The mover reaches the scan kernel: an RK4 FULL scan of vertical
+    directions on the 9000 km shell equals the oracle's traces of the same
+    samples, state for state."""
+    P = CU.legacy_params(dphi=0.05, mover="RK4")
+    P.max_time_s = 60.0
+    pts = np.array([TS.shell_point(0.0, lat, 9000.0, TS.RE_AMPS) for lat in (0.0, 20.0, 45.0)])
+    dirs = -pts / np.linalg.norm(pts, axis=1)[:, None]
+    grid = CU.build_search_grid(0.5, 20.0, 24)
+    states, cut, _ = CU.cutoff_scan(gpu_ctx, P, pts, dirs, grid, mode=CU.SCAN_FULL)
+    for i in range(len(pts)):
+        o = O.trace_batch(_oparams(P), np.tile(pts[i], (len(grid), 1)), np.tile(dirs[i], (len(grid), 1)), grid)
+        st = np.where(o["termination"] == 0, 1, np.where(o["termination"] <= 5, 0, 2))
+        assert np.array_equal(states[i], st), i
+    assert 0 < int((states == 1).sum()) < states.size
+
+
+def test_trace_fp32_hc4_agreement(gpu_ctx):
+    """FP32 HC4 against the FP64 HC4 path: classification agreement over a
+    full scan of the batched-map parameterisation."""
+    P = CU.config2_params()
+    P.mover = CU.MOVERS["HC4"]
+    P.max_steps = 20000
+    pts, lat, lon = CU.config2_points(nlon=8, nlat=6)
+    zen, az = CU.config2_directions(4)
+    dirs = CU.arrival_dirs(pts, zen, az).reshape(-1, 3)
+    x0 = np.repeat(pts, len(zen), axis=0)
+    grid = CU.build_search_grid(0.1, 20.0, 48)
+    s64, b64, _ = CU.cutoff_scan(gpu_ctx, P, x0, dirs, grid, mode=CU.SCAN_FULL, precision=L.FP64)
+    s32, b32, _ = CU.cutoff_scan(gpu_ctx, P, x0, dirs, grid, mode=CU.SCAN_FULL, precision=L.FP32)
+    agree = np.mean(s64 == s32)
+    print(f"FP32 vs FP64 HC4: state agreement {agree:.5f}")
+    assert agree >= 0.97
+
+
+def test_trace_rejects_unbuilt_movers(gpu_ctx):
+    P = CU.legacy_params()
+    x, d, R = _random_tasks(4, 1)
+    P.mover = 5  # GC2
+    with pytest.raises(RuntimeError, match="mover"):
+        CU.trace_batch(gpu_ctx, P, x, d, R)
+    P.mover = CU.MOVERS["RK4"]
+    with pytest.raises(RuntimeError, match="FP32"):
+        CU.trace_batch(gpu_ctx, P, x, d, R, precision=L.FP32)

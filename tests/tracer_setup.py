@@ -75,7 +75,9 @@ def shell_point(lon_deg, lat_deg, alt_km, re_m):
     return np.array([r * math.cos(lat) * math.cos(lon), r * math.cos(lat) * math.sin(lon), r * math.sin(lat)])
 
 
-def oracle_cutoff(P, x0, dirv, rmin, rmax, nscan):
+def oracle_cutoff(P, x0, dirv, rmin, rmax, nscan, rel_tol=None):
+    """UPPER_SCAN bin + bisection; the bracket closes at max(1e-3 GV, 1e-6 R)
+    as the reference's default, or at rel_tol * R when given."""
     grid = O.rigidity_grid(rmin, rmax, nscan)
     res = O.trace_batch(P, np.tile(x0, (nscan, 1)), np.tile(dirv, (nscan, 1)), grid)
     allowed = (res["termination"] == 0).astype(np.int32)
@@ -85,7 +87,7 @@ def oracle_cutoff(P, x0, dirv, rmin, rmax, nscan):
     if b == 0:
         return rmin
     lo, hi = grid[b - 1], grid[b]
-    tol = max(1e-3, 1e-6 * max(abs(lo), abs(hi)))
+    tol = max(1e-3, 1e-6 * max(abs(lo), abs(hi))) if rel_tol is None else rel_tol * hi
     while hi - lo > tol:
         mid = 0.5 * (lo + hi)
         r = O.trace_batch(P, x0[None], dirv[None], np.array([mid]))
