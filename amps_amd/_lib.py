@@ -69,7 +69,7 @@ class TraceParams(C.Structure):
         ("retry", C.c_int32), ("mover", C.c_int32), ("box_min_Re", C.c_double * 3),
         ("box_max_Re", C.c_double * 3), ("r_outer_Re", C.c_double), ("r_inner_Re", C.c_double),
         ("earth_radius_m", C.c_double), ("moment_Am2", C.c_double), ("m_hat", C.c_double * 3),
-        ("B_ext_T", C.c_double * 3),
+        ("B_ext_T", C.c_double * 3), ("field_model", C.c_int32), ("dt_selector", C.c_int32),
     ]
 
 
@@ -131,6 +131,8 @@ def load():
         "amps_gpu_copy_probe": (C.c_int, [vp, i64, i32, P(dbl)]),
         "amps_gpu_set_center_fields": (C.c_int, [vp, P(dbl)]),
         "amps_gpu_interp_center": (C.c_int, [vp, i64, P(dbl), P(i32), P(dbl)]),
+        "amps_gpu_set_mode3d_field": (C.c_int, [vp, P(dbl)]),
+        "amps_gpu_mode3d_field": (C.c_int, [vp, i64, P(dbl), P(dbl)]),
         "amps_gpu_trace_batch": (C.c_int, [vp, P(TraceParams), i32, i64, P(dbl), P(dbl), P(dbl),
                                            vp, P(i64)]),
         "amps_gpu_cutoff_scan": (C.c_int, [vp, P(TraceParams), i32, i32, i64, P(dbl), P(dbl), i32,

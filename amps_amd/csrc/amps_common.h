@@ -87,6 +87,8 @@ struct amps_ctx {
   int* d_nbinfo = nullptr;  // [nb][28] min/max neighbour level, face[6], edge[12], corner[8] neighbours
   double* d_Bc = nullptr;   // [nb][10^3][3] cell-centred B with one ghost layer
   bool has_center = false;
+  double* d_Bm = nullptr;   // [nb][512][3] Mode3D compact field of interior cells
+  bool has_mode3d = false;
   amps::Faces faces{};
   int64_t nnodes = 0;
   std::vector<int32_t> h_node_of_corner;

@@ -104,6 +104,9 @@ static void free_mesh(amps_ctx* c) {
   c->d_level = c->d_nbinfo = nullptr;
   c->d_Bc = nullptr;
   c->has_center = false;
+  (void)hipFree(c->d_Bm);
+  c->d_Bm = nullptr;
+  c->has_mode3d = false;
   (void)hipFree(c->d_inc8);
   (void)hipFree(c->d_node_order);
   c->d_node_order = nullptr;

@@ -231,10 +231,12 @@ __constant__ static const signed char cc_face_edges[6][4] = {{4, 11, 7, 8}, {5, 
 __constant__ static const signed char cc_face_nodes[6][4] = {{0, 2, 4, 6}, {1, 3, 5, 7}, {0, 1, 4, 5},
                                                              {2, 3, 6, 7}, {0, 1, 2, 3}, {4, 5, 6, 7}};
 
-// B at x in block b from the ghost-layered cell-centred field Bc.
-static __device__ __noinline__ void center_B(const MeshView& m, const double* __restrict__ Bc, const double x[3], int b,
-                                      double B[3]) {
-  B[0] = B[1] = B[2] = 0.0;
+// Branch of CellCentered::Linear::InitStencil (:224-816, AMPS branch of an
+// AMR mesh) at x in block b: 0 tri-linear in b, 1 multi-block stencil of b,
+// 2 multi-block stencil of the coarser block `coarser` (blended with b's
+// tri-linear stencil when 0.5 < dmin <= 1).  xLoc receives (iLoc, jLoc, kLoc).
+__device__ __forceinline__ int cc_branch(const MeshView& m, const double x[3], int b, double xLoc[3], int& coarser,
+                                         double& dmin) {
   double lo[3], hi[3], dxCell[3];
 #pragma unroll
   for (int d = 0; d < 3; d++) {
@@ -242,32 +244,23 @@ static __device__ __noinline__ void center_B(const MeshView& m, const double* __
     hi[d] = m.blk_xmax[3 * b + d];
     dxCell[d] = (hi[d] - lo[d]) / NC;
   }
-  const double xLoc[3] = {(x[0] - lo[0]) / (hi[0] - lo[0]) * NC, (x[1] - lo[1]) / (hi[1] - lo[1]) * NC,
-                          (x[2] - lo[2]) / (hi[2] - lo[2]) * NC};
+#pragma unroll
+  for (int d = 0; d < 3; d++) xLoc[d] = (x[d] - lo[d]) / (hi[d] - lo[d]) * NC;
   const double N = NC;
   const int* info = m.nbinfo + 28 * b;
   const int L = m.blk_level[b], minL = info[0], maxL = info[1];
-  CcTri T;
   const bool deep1 = (1.0 < xLoc[0]) && (xLoc[0] < N - 1) && (1.0 < xLoc[1]) && (xLoc[1] < N - 1) &&
                      (1.0 < xLoc[2]) && (xLoc[2] < N - 1);
-  if ((L == minL && L == maxL) || deep1) {
-    cc_tri(m, xLoc[0], xLoc[1], xLoc[2], x, b, T);
-    cc_accum_tri(Bc, T, B);
-    return;
-  }
+  coarser = -1;
+  dmin = 10.0 * NC * NC * NC;
+  if ((L == minL && L == maxL) || deep1) return 0;
   if (L == minL) {
     if ((0.5 < xLoc[0]) && (xLoc[0] < N - 0.5) && (0.5 < xLoc[1]) && (xLoc[1] < N - 0.5) && (0.5 < xLoc[2]) &&
-        (xLoc[2] < N - 0.5)) {
-      cc_tri(m, xLoc[0], xLoc[1], xLoc[2], x, b, T);
-      cc_accum_tri(Bc, T, B);
-    } else {
-      cc_multiblock(m, Bc, x, b, false, 1.0, 0.0, T, B);
-    }
-    return;
+        (xLoc[2] < N - 0.5))
+      return 0;
+    return 1;
   }
   // a coarser block near x, through the faces, edges and corners it is near
-  double dmin = 10.0 * NC * NC * NC;
-  int coarser = -1;
   unsigned edgeTested = 0, cornerTested = 0;
   auto contains = [&](int nb) {
     int cnt = 0;
@@ -323,19 +316,238 @@ static __device__ __noinline__ void center_B(const MeshView& m, const double* __
       }
     }
   }
-  if (coarser >= 0) {
-    const bool blend = (0.5 < dmin) && (dmin <= 1.0);
-    double cw = 1.0, fw = 0.0;
-    if (blend) {
-      cc_tri(m, xLoc[0], xLoc[1], xLoc[2], x, b, T);
-      cw = 1.0 - (dmin - 0.5) / 0.5;
-      fw = (dmin - 0.5) / 0.5;
-    }
-    cc_multiblock(m, Bc, x, coarser, blend, cw, fw, T, B);
+  return coarser >= 0 ? 2 : 0;
+}
+
+// B at x in block b from the ghost-layered cell-centred field Bc.
+static __device__ __noinline__ void center_B(const MeshView& m, const double* __restrict__ Bc, const double x[3], int b,
+                                             double B[3]) {
+  B[0] = B[1] = B[2] = 0.0;
+  double xLoc[3], dmin;
+  int coarser;
+  const int br = cc_branch(m, x, b, xLoc, coarser, dmin);
+  CcTri T;
+  if (br == 0) {
+    cc_tri(m, xLoc[0], xLoc[1], xLoc[2], x, b, T);
+    cc_accum_tri(Bc, T, B);
     return;
   }
-  cc_tri(m, xLoc[0], xLoc[1], xLoc[2], x, b, T);
-  cc_accum_tri(Bc, T, B);
+  if (br == 1) {
+    cc_multiblock(m, Bc, x, b, false, 1.0, 0.0, T, B);
+    return;
+  }
+  const bool blend = (0.5 < dmin) && (dmin <= 1.0);
+  double cw = 1.0, fw = 0.0;
+  if (blend) {
+    cc_tri(m, xLoc[0], xLoc[1], xLoc[2], x, b, T);
+    cw = 1.0 - (dmin - 0.5) / 0.5;
+    fw = (dmin - 0.5) / 0.5;
+  }
+  cc_multiblock(m, Bc, x, coarser, blend, cw, fw, T, B);
+}
+
+// ---------------------------------------------------------------------
+// Mode3D compact-array field.  The decomposition-independent row stencil
+// (cRowStencil, pic.h:7295-7420) of the same InitStencil: every support
+// cell, ghost indices included, is taken to the leaf holding its centre and
+// the interior cell under it (AddRowStencilCell / ...ByCoordinate,
+// pic_interpolation_routines.cpp:33-105), entries naming one cell merge, and
+// the field is the weighted sum in row order over Bm[(block*512 + i +
+// 8(j + 8k))*3 + d] (InterpolateField_, GlobalMagneticField.cpp:292-330).
+// ---------------------------------------------------------------------
+constexpr int CC_ROW_MAX = 72;  // 8 support points x 8 fine cells + 8 blended fine entries
+
+template <int CAP>
+struct RowList {
+  int n;
+  int key[CAP];  // block * 512 + interior cell
+  double w[CAP];
+};
+
+// cRowStencil::AddCell (pic.h:7351-7375)
+template <int CAP>
+__device__ __forceinline__ void row_addkey(RowList<CAP>& r, double w, int key) {
+  for (int e = 0; e < r.n; e++)
+    if (r.key[e] == key) {
+      r.w[e] += w;
+      return;
+    }
+  if (r.n < CAP) {
+    r.key[r.n] = key;
+    r.w[r.n] = w;
+    r.n++;
+  }
+}
+
+// AddRowStencilCellByCoordinate (pic_interpolation_routines.cpp:33-82, non-periodic)
+template <int CAP>
+__device__ __forceinline__ bool row_add_coord(const MeshView& m, RowList<CAP>& r, double w, const double x[3]) {
+#pragma unroll
+  for (int d = 0; d < 3; d++)
+    if ((x[d] < m.gxmin[d]) || (x[d] > m.gxmax[d])) return false;
+  const int b = find_block(m, x);
+  if (b < 0) return false;
+  int ijk[3];
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    const double xmn = m.blk_xmin[3 * b + d];
+    const double blockLength = m.blk_xmax[3 * b + d] - xmn;
+    const double xl = (x[d] - xmn) / blockLength * NC;
+    ijk[d] = (int)floor(xl);
+    if ((ijk[d] < 0) && (xl > -1.0e-10)) ijk[d] = 0;
+    if ((ijk[d] >= NC) && (xl < NC + 1.0e-10)) ijk[d] = NC - 1;
+    if ((ijk[d] < 0) || (ijk[d] >= NC)) return false;
+  }
+  row_addkey(r, w, b * NCELL + ijk[0] + NC * (ijk[1] + NC * ijk[2]));
+  return true;
+}
+
+// AddRowStencilCell (:91-105): indices of block s, ghosts included
+template <int CAP>
+__device__ __forceinline__ bool row_add_cell(const MeshView& m, RowList<CAP>& r, double w, int s, int i, int j,
+                                             int k) {
+  const int ijk[3] = {i, j, k};
+  double x[3];
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    const double dx = (m.blk_xmax[3 * s + d] - m.blk_xmin[3 * s + d]) / NC;
+    x[d] = m.blk_xmin[3 * s + d] + (ijk[d] + 0.5) * dx;
+  }
+  return row_add_coord(m, r, w, x);
+}
+
+// GetTriliniarInterpolationStencil, row part (:820-896)
+template <int CAP>
+__device__ __forceinline__ void row_tri(const MeshView& m, const double loc[3], int b, RowList<CAP>& r) {
+  r.n = 0;
+  const int i0 = (loc[0] < 0.5) ? -1 : (int)(loc[0] - 0.50);
+  const int j0 = (loc[1] < 0.5) ? -1 : (int)(loc[1] - 0.50);
+  const int k0 = (loc[2] < 0.5) ? -1 : (int)(loc[2] - 0.50);
+  double w[3];
+  w[0] = loc[0] - (i0 + 0.5);
+  w[1] = loc[1] - (j0 + 0.5);
+  w[2] = loc[2] - (k0 + 0.5);
+  for (int s = 0; s < 8; s++) {
+    const int i = s >> 2, j = (s >> 1) & 1, k = s & 1;
+    row_add_cell(m, r, cc_weight(i + 2 * j + 4 * k, w), b, i0 + i, j0 + j, k0 + k);
+  }
+  if ((r.n > 0) && (r.n != 8)) {
+    double norm = 0.0;
+    for (int e = 0; e < r.n; e++) norm += r.w[e];
+    if (norm > 0.0)
+      for (int e = 0; e < r.n; e++) r.w[e] /= norm;
+  }
+}
+
+// GetTriliniarInterpolationMutiBlockStencil, row part (:912-1054)
+__device__ __forceinline__ void row_multiblock(const MeshView& m, const double x[3], int b,
+                                               RowList<CC_ROW_MAX>& r) {
+  r.n = 0;
+  double dxCell[3], xmn[3], xLoc[3];
+  int ijkMin[3];
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    xmn[d] = m.blk_xmin[3 * b + d];
+    dxCell[d] = (m.blk_xmax[3 * b + d] - xmn[d]) / NC;
+  }
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    ijkMin[d] = (x[d] - xmn[d] < 0.5 * dxCell[d]) ? -1 : (int)((x[d] - xmn[d] - 0.5 * dxCell[d]) / dxCell[d]);
+    const double xLower = xmn[d] + (ijkMin[d] + 0.5) * dxCell[d];
+    xLoc[d] = (x[d] - xLower) / dxCell[d];
+    if (xLoc[d] < 0.0) xLoc[d] = 0.0;
+    if (xLoc[d] > 1.0) xLoc[d] = 1.0;
+  }
+  const int L = m.blk_level[b];
+  bool geometry = true;
+  for (int q = 0; q < 8; q++) {
+    const int di = q >> 2, dj = (q >> 1) & 1, dk = q & 1;
+    const int ijk[3] = {ijkMin[0] + di, ijkMin[1] + dj, ijkMin[2] + dk};
+    double xl[3];
+#pragma unroll
+    for (int d = 0; d < 3; d++) xl[d] = xmn[d] + (ijk[d] + 0.5) * dxCell[d];
+    const double W = cc_weight(di + 2 * dj + 4 * dk, xLoc);
+    const int S = find_block(m, xl);
+    if (S < 0) {
+      geometry = false;
+      continue;
+    }
+    if (m.blk_level[S] == L) {
+      row_add_cell(m, r, W, b, ijk[0], ijk[1], ijk[2]);
+    } else {
+      // a logical coarse centre covered by a finer block: its 2x2x2 fine cells
+      int iNeib[3];
+#pragma unroll
+      for (int d = 0; d < 3; d++) iNeib[d] = 2 * ((int)((xl[d] - m.blk_xmin[3 * S + d]) / dxCell[d]));
+      const double FineCellWeight = (1.0 / 8.0) * W;
+      for (int f = 0; f < 8; f++)
+        row_add_cell(m, r, FineCellWeight, S, iNeib[0] + (f >> 2), iNeib[1] + ((f >> 1) & 1), iNeib[2] + (f & 1));
+    }
+  }
+  if (!geometry) {
+    // BuildConstantRowStencil (:109-115) at the leaf holding x
+    r.n = 0;
+    row_add_coord(m, r, 1.0, x);
+  }
+}
+
+// cMode3DMeshFieldEval::GetB_T mesh path (CutoffRigidityMode3D.cpp:702-800):
+// zero outside the mesh, else the row's weighted sum.  A stencil whose eight
+// support cells are interior cells of the block holding x (the common case)
+// is summed directly: each such cell's centre lies inside that block and maps
+// back to itself, so the row is the same eight entries in the same order.
+static __device__ __noinline__ void mode3d_B(const MeshView& m, const double* __restrict__ Bm, const double x[3],
+                                             double B[3]) {
+  B[0] = B[1] = B[2] = 0.0;
+  const int b = find_block(m, x);
+  if (b < 0) return;
+  double loc[3], dmin;
+  int coarser;
+  const int br = cc_branch(m, x, b, loc, coarser, dmin);
+  if (br == 0) {
+    const int i0 = (loc[0] < 0.5) ? -1 : (int)(loc[0] - 0.50);
+    const int j0 = (loc[1] < 0.5) ? -1 : (int)(loc[1] - 0.50);
+    const int k0 = (loc[2] < 0.5) ? -1 : (int)(loc[2] - 0.50);
+    if (i0 >= 0 && i0 <= NC - 2 && j0 >= 0 && j0 <= NC - 2 && k0 >= 0 && k0 <= NC - 2) {
+      double w[3];
+      w[0] = loc[0] - (i0 + 0.5);
+      w[1] = loc[1] - (j0 + 0.5);
+      w[2] = loc[2] - (k0 + 0.5);
+      const double* base = Bm + (size_t)b * NCELL * 3;
+#pragma unroll
+      for (int s = 0; s < 8; s++) {
+        const int i = s >> 2, j = (s >> 1) & 1, k = s & 1;
+        const double W = cc_weight(i + 2 * j + 4 * k, w);
+        const double* v = base + 3 * ((i0 + i) + NC * ((j0 + j) + NC * (k0 + k)));
+        B[0] += W * v[0];
+        B[1] += W * v[1];
+        B[2] += W * v[2];
+      }
+      return;
+    }
+  }
+  RowList<CC_ROW_MAX> r;
+  if (br == 0) {
+    row_tri(m, loc, b, r);
+  } else if (br == 1) {
+    row_multiblock(m, x, b, r);
+  } else {
+    row_multiblock(m, x, coarser, r);
+    if ((0.5 < dmin) && (dmin <= 1.0)) {
+      RowList<8> f;
+      row_tri(m, loc, b, f);
+      const double cw = 1.0 - (dmin - 0.5) / 0.5, fw = (dmin - 0.5) / 0.5;
+      for (int e = 0; e < r.n; e++) r.w[e] *= cw;
+      for (int e = 0; e < f.n; e++) f.w[e] *= fw;
+      for (int e = 0; e < f.n; e++) row_addkey(r, f.w[e], f.key[e]);
+    }
+  }
+  for (int e = 0; e < r.n; e++) {
+    const double* v = Bm + (size_t)r.key[e] * 3;
+    B[0] += r.w[e] * v[0];
+    B[1] += r.w[e] * v[1];
+    B[2] += r.w[e] * v[2];
+  }
 }
 
 }  // namespace amps

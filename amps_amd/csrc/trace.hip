@@ -22,6 +22,7 @@
 #include <vector>
 
 #include "amps_common.h"
+#include "center_device.h"
 #include "trace_common.h"
 
 using namespace amps;
@@ -75,10 +76,22 @@ __device__ __forceinline__ double gamma64(const double* p, double m0) {
   return sqrt(1.0 + p2 / (mc * mc));
 }
 
+// the tracer's field: the analytic dipole plus the uniform field (FM 0) or
+// the Mode3D mesh field of the compact cell-centred array (FM 1)
+template <int FM>
+__device__ __forceinline__ void field64(const TraceView& a, const double* x, double* B) {
+  if constexpr (FM == 1)
+    mode3d_B(a.mesh, a.Bm, x, B);
+  else
+    dipole64(a.P, x, B);
+}
+
 // BorisStep, GridlessParticleMovers.cpp:781-801
-__device__ __forceinline__ void boris64(const amps_trace_params& P, double* x, double* p, double dt) {
+template <int FM>
+__device__ __forceinline__ void boris64(const TraceView& a, double* x, double* p, double dt) {
+  const amps_trace_params& P = a.P;
   double B[3];
-  dipole64(P, x, B);
+  field64<FM>(a, x, B);
   const double m0 = P.m0_kg, q = P.q_C;
   const double gam = gamma64(p, m0);
   const double iv = 1.0 / (gam * m0);
@@ -121,19 +134,61 @@ __device__ __forceinline__ bool seg_hits(const double* a, const double* b, doubl
   return dot3(q, q) <= r * r;
 }
 
+// SelectLegacyCutoffTraceDt3D, CutoffRigidityMode3D.cpp:881-936: the same
+// rules with the box and inner-sphere distances in metres (no box limit
+// outside the box; inner distance max(0, |x| - r_inner)); limiter and floor
+// always on.
+template <int FM>
+__device__ __forceinline__ double select_dt3d(const TraceView& a, const double* x, const double* p, double dtTrace,
+                                              double timeRemaining) {
+  const amps_trace_params& P = a.P;
+  double dt = dtTrace;
+  if (timeRemaining < dt) dt = timeRemaining;
+  if (!P.adaptive) return dt;
+  const double m0 = P.m0_kg;
+  const double p2 = dot3(p, p);
+  const double mc = m0 * C_LIGHT;
+  const double gamma = sqrt(1.0 + p2 / (mc * mc));
+  const double pMag = sqrt(dmax(0.0, p2));
+  const double vMag = (gamma > 0.0 && m0 > 0.0) ? pMag / (gamma * m0) : 0.0;
+  double B[3];
+  field64<FM>(a, x, B);
+  const double Bmag = sqrt(dot3(B, B));
+  if (Bmag > 0.0) {
+    const double omega = fabs(P.q_C) * Bmag / (gamma * m0);
+    if (omega > 0.0) dt = dmin(dt, P.dphi_max / omega);
+  }
+  const double dxp = a.box_max_m[0] - x[0], dxm = x[0] - a.box_min_m[0];
+  const double dyp = a.box_max_m[1] - x[1], dym = x[1] - a.box_min_m[1];
+  const double dzp = a.box_max_m[2] - x[2], dzm = x[2] - a.box_min_m[2];
+  const double dBox = dmin(dmin(dmin(dmin(dmin(dxp, dxm), dyp), dym), dzp), dzm);
+  const double dBoxSafe = (dBox < 0.0) ? 1.0e300 : dBox;
+  const double dInner = dmax(0.0, sqrt(dot3(x, x)) - a.r_inner_m);
+  const double dNear = dmin(dBoxSafe, dInner);
+  const double vFor = vMag;
+  if (vFor > 0.0 && dNear < 1.0e299) dt = dmin(dt, 0.20 * dNear / vFor);
+  const double safeSpeed = dmax(vFor, 1.0e-12);
+  const double dtFloor = dmax(dmax(1.0e-12, 1.0e-9 * dmax(dtTrace, 1.0)), 100.0e3 / safeSpeed);
+  dt = dmax(dtFloor, dt);
+  if (timeRemaining > 0.0) dt = dmin(dt, timeRemaining);
+  return dt;
+}
+
 // Full-orbit movers of StepParticleChecked (MoverType numbering).
 constexpr int MV_BORIS = 0, MV_HC4 = 1, MV_RK2 = 2, MV_RK4 = 3, MV_RK6 = 4;
 
 // HigueraCaryMagneticMidpointStep, GridlessParticleMovers.cpp:723-757: the
 // Boris rotation with B sampled at the half-drift midpoint.
-__device__ __forceinline__ void hc_mid64(const amps_trace_params& P, double* x, double* p, double dt) {
+template <int FM>
+__device__ __forceinline__ void hc_mid64(const TraceView& a, double* x, double* p, double dt) {
+  const amps_trace_params& P = a.P;
   const double m0 = P.m0_kg, q = P.q_C;
   const double g0 = gamma64(p, m0);
   const double iv0 = 1.0 / (g0 * m0);
   const double h = 0.5 * dt;
   const double xm[3] = {x[0] + h * (iv0 * p[0]), x[1] + h * (iv0 * p[1]), x[2] + h * (iv0 * p[2])};
   double B[3];
-  dipole64(P, xm, B);
+  field64<FM>(a, xm, B);
   const double tc = (q * dt) / (2.0 * g0 * m0);
   const double t[3] = {tc * B[0], tc * B[1], tc * B[2]};
   const double t2 = dot3(t, t);
@@ -156,10 +211,11 @@ constexpr double HC4_CBRT2 = 1.25992104989487316477;
 constexpr double HC4_W1 = 1.0 / (2.0 - HC4_CBRT2);
 constexpr double HC4_W0 = -HC4_CBRT2 / (2.0 - HC4_CBRT2);
 
-__device__ __forceinline__ void hc4_64(const amps_trace_params& P, double* x, double* p, double dt) {
-  hc_mid64(P, x, p, HC4_W1 * dt);
-  hc_mid64(P, x, p, HC4_W0 * dt);
-  hc_mid64(P, x, p, HC4_W1 * dt);
+template <int FM>
+__device__ __forceinline__ void hc4_64(const TraceView& a, double* x, double* p, double dt) {
+  hc_mid64<FM>(a, x, p, HC4_W1 * dt);
+  hc_mid64<FM>(a, x, p, HC4_W0 * dt);
+  hc_mid64<FM>(a, x, p, HC4_W1 * dt);
 }
 
 // Butcher tables of the RK2 / RK4 / RK6 cases, GridlessParticleMovers.cpp:
@@ -196,9 +252,9 @@ struct RKTable<MV_RK6> {
 // table is formed, zeros included, so that the sums round as the reference's.
 // A stage chord meeting the sphere leaves x at the stage point and p at its
 // start value; the final chord leaves both at y1.
-template <int MV>
-__device__ __forceinline__ bool rk_checked64(const amps_trace_params& P, double* x, double* p, double dt,
-                                             double rIn) {
+template <int MV, int FM>
+__device__ __forceinline__ bool rk_checked64(const TraceView& a, double* x, double* p, double dt, double rIn) {
+  const amps_trace_params& P = a.P;
   using T = RKTable<MV>;
   double kx[T::NS][3], kp[T::NS][3];
   const double y0x[3] = {x[0], x[1], x[2]}, y0p[3] = {p[0], p[1], p[2]};
@@ -229,7 +285,7 @@ __device__ __forceinline__ bool rk_checked64(const amps_trace_params& P, double*
     for (int d = 0; d < 3; d++) prev[d] = yx[d];
     // LorentzRhs, GridlessParticleMovers.cpp:151-157
     double B[3];
-    dipole64(P, yx, B);
+    field64<FM>(a, yx, B);
     const double iv = 1.0 / (gamma64(yp, P.m0_kg) * P.m0_kg);
     const double v[3] = {iv * yp[0], iv * yp[1], iv * yp[2]};
     kx[i][0] = v[0], kx[i][1] = v[1], kx[i][2] = v[2];
@@ -254,12 +310,11 @@ __device__ __forceinline__ bool rk_checked64(const amps_trace_params& P, double*
 
 // StepParticleChecked, GridlessParticleMovers.cpp:1035-1098 (full-orbit
 // movers): false when the step met the inner sphere.
-template <int MV>
-__device__ __forceinline__ bool step_checked64(const amps_trace_params& P, double* x, double* p, double dt,
-                                               double rIn) {
+template <int MV, int FM>
+__device__ __forceinline__ bool step_checked64(const TraceView& a, double* x, double* p, double dt, double rIn) {
   if constexpr (MV == MV_BORIS) {
     const double x0[3] = {x[0], x[1], x[2]};
-    boris64(P, x, p, dt);
+    boris64<FM>(a, x, p, dt);
     return !seg_hits(x0, x, rIn) && !inside_inner_m(x, rIn);
   } else if constexpr (MV == MV_HC4) {
     // four positive macro-steps, each chord checked; the composition's
@@ -267,18 +322,20 @@ __device__ __forceinline__ bool step_checked64(const amps_trace_params& P, doubl
     const double dtSub = dt / 4.0;
     for (int i = 0; i < 4; i++) {
       const double x0[3] = {x[0], x[1], x[2]};
-      hc4_64(P, x, p, dtSub);
+      hc4_64<FM>(a, x, p, dtSub);
       if (seg_hits(x0, x, rIn) || inside_inner_m(x, rIn)) return false;
     }
     return true;
   } else {
-    return rk_checked64<MV>(P, x, p, dt, rIn);
+    return rk_checked64<MV, FM>(a, x, p, dt, rIn);
   }
 }
 
 // SelectLegacyCutoffTraceDt, CutoffRigidityGridless.cpp:1086-1152 (switchable parts)
-__device__ __forceinline__ double select_dt64(const amps_trace_params& P, const double* x,
-                                              const double* p, double dtTrace, double timeRemaining) {
+template <int FM>
+__device__ __forceinline__ double select_dt64(const TraceView& a, const double* x, const double* p, double dtTrace,
+                                              double timeRemaining) {
+  const amps_trace_params& P = a.P;
   double dt = dtTrace;
   if (timeRemaining < dt) dt = timeRemaining;
   if (!P.adaptive) return dt;
@@ -289,7 +346,7 @@ __device__ __forceinline__ double select_dt64(const amps_trace_params& P, const 
   const double pMag = sqrt(dmax(0.0, p2));
   const double vMag = (gamma > 0.0 && m0 > 0.0) ? pMag / (gamma * m0) : 0.0;
   double B[3];
-  dipole64(P, x, B);
+  field64<FM>(a, x, B);
   const double Bmag = sqrt(dot3(B, B));
   if (Bmag > 0.0) {
     const double omega = fabs(P.q_C) * Bmag / (gamma * m0);
@@ -324,7 +381,7 @@ __device__ __forceinline__ double select_dt64(const amps_trace_params& P, const 
   return dt;
 }
 
-template <int MV>
+template <int MV, int FM>
 struct Fp64Stepper {
   using Lane = Fp64Lane;
   static __device__ __forceinline__ void init(const TraceView& a, Lane& L, const double* x0,
@@ -373,10 +430,11 @@ struct Fp64Stepper {
       inside = x[0] >= a.box_min_m[0] - 0.0 && x[0] <= a.box_max_m[0] + 0.0 && x[1] >= a.box_min_m[1] - 0.0 &&
                x[1] <= a.box_max_m[1] + 0.0 && x[2] >= a.box_min_m[2] - 0.0 && x[2] <= a.box_max_m[2] + 0.0;
     if (!inside) return 0;
-    const double dt = select_dt64(P, x, p, L.dtTrace, L.maxTime - L.t);
+    const double dt = (P.dt_selector == 1) ? select_dt3d<FM>(a, x, p, L.dtTrace, L.maxTime - L.t)
+                                           : select_dt64<FM>(a, x, p, L.dtTrace, L.maxTime - L.t);
     if (!(dt > 0.0) || !isfinite(dt)) return 6;
     const double xb[3] = {x[0], x[1], x[2]};
-    if (!step_checked64<MV>(P, x, p, dt, rIn)) return 1;
+    if (!step_checked64<MV, FM>(a, x, p, dt, rIn)) return 1;
     if (!fin3(x) || !fin3(p)) return 8;
     const double dd[3] = {x[0] - xb[0], x[1] - xb[1], x[2] - xb[2]};
     const double seg = sqrt(dot3(dd, dd));
@@ -770,6 +828,8 @@ static int check_params(const amps_trace_params* P) {
   if (!(P->m0_kg > 0.0) || !(P->qabs_C > 0.0)) return fail(AMPS_E_ARG, "trace: mass and |charge| must be > 0");
   if (!(P->earth_radius_m > 0.0)) return fail(AMPS_E_ARG, "trace: earth_radius_m must be > 0");
   if (P->max_steps <= 0) return fail(AMPS_E_ARG, "trace: max_steps must be > 0");
+  if (P->field_model < 0 || P->field_model > 1) return fail(AMPS_E_ARG, "trace: field_model must be 0 or 1");
+  if (P->dt_selector < 0 || P->dt_selector > 1) return fail(AMPS_E_ARG, "trace: dt_selector must be 0 or 1");
   if (P->mover < MV_BORIS || P->mover > MV_RK6)
     return fail(AMPS_E_ARG, "trace: mover must be 0 BORIS, 1 HC4, 2 RK2, 3 RK4 or 4 RK6 "
                             "(the guiding-centre and hybrid movers are not built)");
@@ -807,9 +867,27 @@ static void fill_view(amps_ctx* c, TraceState* t, const amps_trace_params* P, Tr
   a.chunk = t->chunk;
 }
 
+template <int FM>
+static void launch64(amps_ctx* c, const TraceView& a, dim3 g, dim3 blk) {
+  switch (a.P.mover) {
+    case MV_HC4: hipLaunchKernelGGL((k_trace_persistent<Fp64Stepper<MV_HC4, FM>>), g, blk, 0, c->stream, a); break;
+    case MV_RK2: hipLaunchKernelGGL((k_trace_persistent<Fp64Stepper<MV_RK2, FM>>), g, blk, 0, c->stream, a); break;
+    case MV_RK4: hipLaunchKernelGGL((k_trace_persistent<Fp64Stepper<MV_RK4, FM>>), g, blk, 0, c->stream, a); break;
+    case MV_RK6: hipLaunchKernelGGL((k_trace_persistent<Fp64Stepper<MV_RK6, FM>>), g, blk, 0, c->stream, a); break;
+    default: hipLaunchKernelGGL((k_trace_persistent<Fp64Stepper<MV_BORIS, FM>>), g, blk, 0, c->stream, a); break;
+  }
+}
+
 static int launch(amps_ctx* c, TraceState* t, TraceView& a, int precision) {
   if (precision == AMPS_FP32 && a.P.mover != MV_BORIS && a.P.mover != MV_HC4)
     return fail(AMPS_E_ARG, "trace: the FP32 path runs the BORIS and HC4 movers only");
+  if (precision == AMPS_FP32 && (a.P.field_model != 0 || a.P.dt_selector != 0))
+    return fail(AMPS_E_ARG, "trace: the FP32 path runs the analytic field and the gridless step selector only");
+  if (a.P.field_model == 1) {
+    if (!c->has_mode3d) return fail(AMPS_E_STATE, "trace: field_model 1 needs amps_gpu_set_mode3d_field");
+    a.mesh = c->mv;
+    a.Bm = c->d_Bm;
+  }
   HIP_TRY(hipMemsetAsync(t->d_queue, 0, 8, c->stream));
   int grid = t->grid_waves;
   if (grid <= 0) {
@@ -824,14 +902,10 @@ static int launch(amps_ctx* c, TraceState* t, TraceView& a, int precision) {
       hipLaunchKernelGGL(k_trace_persistent<Fp32Stepper<MV_HC4>>, g, blk, 0, c->stream, a);
     else
       hipLaunchKernelGGL(k_trace_persistent<Fp32Stepper<MV_BORIS>>, g, blk, 0, c->stream, a);
+  } else if (a.P.field_model == 1) {
+    launch64<1>(c, a, g, blk);
   } else {
-    switch (a.P.mover) {
-      case MV_HC4: hipLaunchKernelGGL(k_trace_persistent<Fp64Stepper<MV_HC4>>, g, blk, 0, c->stream, a); break;
-      case MV_RK2: hipLaunchKernelGGL(k_trace_persistent<Fp64Stepper<MV_RK2>>, g, blk, 0, c->stream, a); break;
-      case MV_RK4: hipLaunchKernelGGL(k_trace_persistent<Fp64Stepper<MV_RK4>>, g, blk, 0, c->stream, a); break;
-      case MV_RK6: hipLaunchKernelGGL(k_trace_persistent<Fp64Stepper<MV_RK6>>, g, blk, 0, c->stream, a); break;
-      default: hipLaunchKernelGGL(k_trace_persistent<Fp64Stepper<MV_BORIS>>, g, blk, 0, c->stream, a); break;
-    }
+    launch64<0>(c, a, g, blk);
   }
   HIP_TRY(hipGetLastError());
   tend(c, AMPS_K_TRACE);

@@ -2,7 +2,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
-#include "../../include/ampsgpu.h"
+#include "amps_common.h"
 
 namespace amps {
 
@@ -29,6 +29,9 @@ struct TraceView {
   amps_trace_result* out;
   int8_t* states;     // [nloc][nR]: -1 untraced, 0 forbidden, 1 allowed, 2 failed
   int* cutoff;        // UPPER mode raw result
+  // mesh field (field_model 1): the block mesh and the compact interior-cell B
+  MeshView mesh;
+  const double* Bm;
 };
 
 }  // namespace amps

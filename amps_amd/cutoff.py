@@ -53,12 +53,14 @@ def make_params(*, tilt_deg=0.0, moment_scale=1.0, bz_T=0.0, mass_amu=PROTON_AMU
                 reversed_charge=False, dt_trace=1.0, max_time=600.0, max_steps=300000,
                 max_distance_Re=400.0, dphi=0.15, adaptive=True, dist_limiter=True, dt_floor=True,
                 outer_sphere=False, box_Re=35.0, r_outer_Re=35.0, r_inner_Re=1.0,
-                earth_radius_m=RE_AMPS, retry=True, mover="BORIS"):
+                earth_radius_m=RE_AMPS, retry=True, mover="BORIS", field_model=0, dt_selector=0):
     P = L.TraceParams()
     if mover not in MOVERS:
         raise ValueError("mover must be one of %s (guiding-centre and hybrid movers are not built)"
                          % ", ".join(MOVERS))
     P.mover = MOVERS[mover]
+    P.field_model = int(field_model)
+    P.dt_selector = int(dt_selector)
     q = charge_e * ELECTRON_CHARGE
     P.q_C = -q if reversed_charge else q
     P.qabs_C = abs(q)
@@ -102,6 +104,47 @@ def config2_params():
                        dphi=2.0 * math.pi * 0.01, dist_limiter=False, dt_floor=False,
                        outer_sphere=True, r_outer_Re=25.0, r_inner_Re=1.0,
                        earth_radius_m=RE_DIPOLE)
+
+
+def mode3d_params(**kw):
+    """TraceAllowed3D on the Mode3D mesh field (CutoffRigidityMode3D.cpp:
+    1048-1300): field_model 1 and the Mode3D legacy step selector (0.15 rad
+    gyro budget, box and inner distances in metres, limiter and floor on)."""
+    kw.setdefault("dphi", 0.15)
+    return make_params(field_model=1, dt_selector=1, **kw)
+
+
+class Mode3DField:
+    """The compact Mode3D field on a block mesh, resident in the context:
+    B of every leaf's interior cells, [nb][512][3] with cell i + 8(j + 8k)
+    (GlobalMagneticField.h:18-30, block id = Temp_ID)."""
+
+    def __init__(self, ctx, mesh, B_cells):
+        self.ctx, self.mesh = ctx, mesh
+        d = L.MeshDesc()
+        for i in range(3):
+            d.xmin[i] = mesh.xmin[i]
+            d.xmax[i] = mesh.xmax[i]
+        d.lmax = mesh.lmax
+        d.nblocks = mesh.nblocks
+        lv, ijk = mesh.desc_arrays()
+        L.check(ctx.lib.amps_gpu_set_mesh(ctx.h, C.byref(d), L._ptr(lv, C.c_int32), L._ptr(ijk, C.c_int64)),
+                "set_mesh")
+        self.set(B_cells)
+
+    def set(self, B_cells):
+        Bm = np.ascontiguousarray(B_cells, np.float64)
+        if Bm.shape != (self.mesh.nblocks, 512, 3):
+            raise ValueError("B_cells must be [nblocks][512][3]")
+        L.check(self.ctx.lib.amps_gpu_set_mode3d_field(self.ctx.h, L._ptr(Bm, C.c_double)), "set_mode3d_field")
+
+    def at(self, x):
+        """B [n][3] at points x [n][3] (cMode3DMeshFieldEval::GetB_T)."""
+        x = np.ascontiguousarray(x, np.float64).reshape(-1, 3)
+        B = np.zeros_like(x)
+        L.check(self.ctx.lib.amps_gpu_mode3d_field(self.ctx.h, len(x), L._ptr(x, C.c_double),
+                                                   L._ptr(B, C.c_double)), "mode3d_field")
+        return B
 
 
 def build_search_grid(rmin, rmax, n):

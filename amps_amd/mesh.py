@@ -206,6 +206,14 @@ def center_fields(mesh, fn):
     return np.ascontiguousarray(fn(P).reshape(mesh.nblocks, (NC + 2) ** 3, 3))
 
 
+def interior_center_fields(mesh, fn):
+    """Values at interior cell centres, [nb][512][3] with cell i + 8(j + 8k):
+    the compact per-leaf layout of the Mode3D global field arrays
+    (GlobalMagneticField.h:18-30, block id = Temp_ID)."""
+    g = center_fields(mesh, fn).reshape(mesh.nblocks, NC + 2, NC + 2, NC + 2, 3)
+    return np.ascontiguousarray(g[:, 1:NC + 1, 1:NC + 1, 1:NC + 1].reshape(mesh.nblocks, NC ** 3, 3))
+
+
 def cell_of(mesh, x, blk):
     """FindCellIndex for points known to lie in block blk."""
     dxc = (mesh.xmax - mesh.xmin)[None, :] / (1 << mesh.level[blk]).astype(np.float64)[:, None] / float(NC)

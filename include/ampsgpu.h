@@ -113,6 +113,18 @@ int amps_gpu_set_center_fields(amps_ctx* ctx, const double* B_center);
  * 224-1070): trilinear on cell centres, the multi-block stencil with 2x2x2
  * fine averages at coarse/fine interfaces and the coarse/fine blend. */
 int amps_gpu_interp_center(amps_ctx* ctx, int64_t n, const double* x, const int32_t* blk, double* B);
+/* Mode3D compact field (GlobalMagneticField.h:18-30): B of every block's
+ * interior cells, [block][512][3] with cell i + 8(j + 8k); the block id is
+ * the leaf's dense Temp_ID.  The tracer's mesh field (field_model 1) reads
+ * it. */
+int amps_gpu_set_mode3d_field(amps_ctx* ctx, const double* B_cells);
+/* B at n points x[3n] from that field as cMode3DMeshFieldEval::GetB_T
+ * (CutoffRigidityMode3D.cpp:702-800): zero outside the mesh, otherwise the
+ * row stencil of CellCentered::Linear::InitStencil (pic_interpolation_
+ * routines.cpp:224-1070 with a cRowStencil, pic.h:7295-7420), every support
+ * cell taken to the leaf and interior cell holding its centre
+ * (:33-105), summed as InterpolateField_ (GlobalMagneticField.cpp:292-330). */
+int amps_gpu_mode3d_field(amps_ctx* ctx, int64_t n, const double* x, double* B);
 
 /* Normalized charge and mass per species, the mover time step per species
  * (_SPECIES_DEPENDENT_GLOBAL_TIME_STEP_) and the particle weight. */
@@ -244,6 +256,12 @@ typedef struct {
   double moment_Am2;     /* dipole |m| */
   double m_hat[3];
   double B_ext_T[3];     /* uniform field added to the dipole */
+  int32_t field_model;   /* 0: analytic dipole + B_ext_T; 1: Mode3D mesh field
+                            (amps_gpu_set_mode3d_field, cMode3DMeshFieldEval) */
+  int32_t dt_selector;   /* 0: SelectLegacyCutoffTraceDt (gridless); 1: its Mode3D
+                            form SelectLegacyCutoffTraceDt3D (CutoffRigidityMode3D.cpp:
+                            881-936: box and inner distances in metres, limiter
+                            and floor always on) */
 } amps_trace_params;
 
 /* TrajectoryResult (termination values of util/TrajectoryTermination.h:10-21) */

@@ -872,10 +872,12 @@ static inline double gamma_p(const double p[3], double m0) {
   return sqrt(1.0 + p2 / (mc * mc));
 }
 
+static void trace_field(const orc_trace_params* P, const double x[3], double B[3]);
+
 /* BorisStep — GridlessParticleMovers.cpp:781-801 */
 void orc_boris_step(const orc_trace_params* P, double x[3], double p[3], double dt) {
   double B[3];
-  orc_dipole_B(P, x, B);
+  trace_field(P, x, B);
   double m0 = P->m0_kg, q = P->q_C;
   double gam = gamma_p(p, m0);
   double iv = 1.0 / (gam * m0);
@@ -925,7 +927,7 @@ static void hc_midpoint_step(const orc_trace_params* P, double x[3], double p[3]
   double h = 0.5 * dt;
   double xm[3] = {x[0] + h * v0[0], x[1] + h * v0[1], x[2] + h * v0[2]};
   double B[3];
-  orc_dipole_B(P, xm, B);
+  trace_field(P, xm, B);
   double gamma0 = gamma_p(p, m0);
   double tc = (q * dt) / (2.0 * gamma0 * m0);
   double t[3] = {tc * B[0], tc * B[1], tc * B[2]};
@@ -959,7 +961,7 @@ void orc_hc4_step(const orc_trace_params* P, double x[3], double p[3], double dt
 static void lorentz_rhs(const orc_trace_params* P, const double x[3], const double p[3], double kx[3],
                         double kp[3]) {
   double B[3];
-  orc_dipole_B(P, x, B);
+  trace_field(P, x, B);
   double iv = 1.0 / (gamma_p(p, P->m0_kg) * P->m0_kg);
   double v[3] = {iv * p[0], iv * p[1], iv * p[2]};
   double q = P->q_C;
@@ -1070,7 +1072,7 @@ static double select_dt(const orc_trace_params* P, const double x[3], const doub
   double pMag = sqrt(dmax(0.0, p2));
   double vMag = (gamma > 0.0 && m0 > 0.0) ? pMag / (gamma * m0) : 0.0;
   double B[3];
-  orc_dipole_B(P, x, B);
+  trace_field(P, x, B);
   double Bmag = sqrt(dot3(B, B));
   if (Bmag > 0.0) {
     double omega = fabs(P->q_C) * Bmag / (gamma * m0);
@@ -1106,6 +1108,44 @@ static double select_dt(const orc_trace_params* P, const double x[3], const doub
   return dt;
 }
 
+/* SelectLegacyCutoffTraceDt3D — CutoffRigidityMode3D.cpp:881-936: the
+ * gridless selector's rules with the box and inner-sphere distances taken in
+ * metres (a point outside the box has no box limit; the inner distance is
+ * max(0, |x| - r_inner)); the limiter and the floor are always on. */
+static double select_dt3d(const orc_trace_params* P, const double x[3], const double p[3], double dtTrace,
+                          double timeRemaining, const double boxmin[3], const double boxmax[3], double rInner) {
+  double dt = dtTrace;
+  if (timeRemaining < dt) dt = timeRemaining;
+  if (!P->adaptive) return dt;
+  double m0 = P->m0_kg;
+  double p2 = dot3(p, p);
+  double mc = m0 * ORC_C;
+  double gamma = sqrt(1.0 + p2 / (mc * mc));
+  double pMag = sqrt(dmax(0.0, p2));
+  double vMag = (gamma > 0.0 && m0 > 0.0) ? pMag / (gamma * m0) : 0.0;
+  double B[3];
+  trace_field(P, x, B);
+  double Bmag = sqrt(dot3(B, B));
+  if (Bmag > 0.0) {
+    double omega = fabs(P->q_C) * Bmag / (gamma * m0);
+    if (omega > 0.0) dt = dmin(dt, P->dphi_max / omega);
+  }
+  double dxp = boxmax[0] - x[0], dxm = x[0] - boxmin[0];
+  double dyp = boxmax[1] - x[1], dym = x[1] - boxmin[1];
+  double dzp = boxmax[2] - x[2], dzm = x[2] - boxmin[2];
+  double dBox = dmin(dmin(dmin(dmin(dmin(dxp, dxm), dyp), dym), dzp), dzm);
+  double dBoxSafe = (dBox < 0.0) ? 1.0e300 : dBox;
+  double dInner = dmax(0.0, sqrt(dot3(x, x)) - rInner);
+  double dNear = dmin(dBoxSafe, dInner);
+  double vFor = vMag;
+  if (vFor > 0.0 && dNear < 1.0e299) dt = dmin(dt, 0.20 * dNear / vFor);
+  double safeSpeed = dmax(vFor, 1.0e-12);
+  double dtFloor = dmax(dmax(1.0e-12, 1.0e-9 * dmax(dtTrace, 1.0)), 100.0e3 / safeSpeed);
+  dt = dmax(dtFloor, dt);
+  if (timeRemaining > 0.0) dt = dmin(dt, timeRemaining);
+  return dt;
+}
+
 static int isfin3(const double a[3]) { return isfinite(a[0]) && isfinite(a[1]) && isfinite(a[2]); }
 
 /* TraceTrajectoryImpl — CutoffRigidityGridless.cpp:1246-1457 (full-orbit
@@ -1137,7 +1177,8 @@ static void trace_impl(const orc_trace_params* P, const double x0[3], const doub
       inside = x[0] >= boxmin[0] - 0.0 && x[0] <= boxmax[0] + 0.0 && x[1] >= boxmin[1] - 0.0 &&
                x[1] <= boxmax[1] + 0.0 && x[2] >= boxmin[2] - 0.0 && x[2] <= boxmax[2] + 0.0;
     if (!inside) { term = 0; break; }
-    double dt = select_dt(P, x, p, dtTrace, maxTime - t);
+    double dt = P->dt_selector == 1 ? select_dt3d(P, x, p, dtTrace, maxTime - t, boxmin, boxmax, rInner)
+                                    : select_dt(P, x, p, dtTrace, maxTime - t);
     if (!(dt > 0.0) || !isfinite(dt)) { term = 6; break; }
     double xb[3] = {x[0], x[1], x[2]};
     if (!orc_step_checked(P, x, p, dt, rInner)) { term = 1; break; }
@@ -1472,8 +1513,12 @@ static void cst_multiblock(const orc_mesh* m, const double x[3], int tn, orc_cst
   if (!geometry || !physical) cst_constant(m, x, find_tnode(m, x, tn), s);
 }
 
-/* CellCentered::Linear::InitStencil, :224-816, AMPS branch of an AMR mesh */
-static void cst_init(const orc_mesh* m, const double x[3], int tn, orc_cst* s) {
+/* Branch of CellCentered::Linear::InitStencil, :224-816 (AMPS branch of an
+ * AMR mesh): 0 tri-linear in tn, 1 multi-block stencil of tn, 2 multi-block
+ * stencil of the coarser block *coarser (blended with tn's tri-linear
+ * stencil when 0.5 < *dmin <= 1).  loc receives (iLoc, jLoc, kLoc). */
+static int cc_branch(const orc_mesh* m, const double x[3], int tn, double loc[3], int* coarser_out,
+                     double* dmin_out) {
   const orc_tnode* n = &m->t[tn];
   const int* info = m->nb_info + 28 * n->block;
   const int minL = info[0], maxL = info[1];
@@ -1481,20 +1526,13 @@ static void cst_init(const orc_mesh* m, const double x[3], int tn, orc_cst* s) {
   const double jLoc = (x[1] - n->xmin[1]) / (n->xmax[1] - n->xmin[1]) * ORC_NC;
   const double kLoc = (x[2] - n->xmin[2]) / (n->xmax[2] - n->xmin[2]) * ORC_NC;
   const double N = ORC_NC;
-  if (n->level == minL && n->level == maxL) {
-    cst_tri(m, iLoc, jLoc, kLoc, x, tn, s);
-    return;
-  }
-  if ((1.0 < iLoc) && (iLoc < N - 1) && (1.0 < jLoc) && (jLoc < N - 1) && (1.0 < kLoc) && (kLoc < N - 1)) {
-    cst_tri(m, iLoc, jLoc, kLoc, x, tn, s);
-    return;
-  }
+  loc[0] = iLoc, loc[1] = jLoc, loc[2] = kLoc;
+  if (n->level == minL && n->level == maxL) return 0;
+  if ((1.0 < iLoc) && (iLoc < N - 1) && (1.0 < jLoc) && (jLoc < N - 1) && (1.0 < kLoc) && (kLoc < N - 1)) return 0;
   if (n->level == minL) {
     if ((0.5 < iLoc) && (iLoc < N - 0.5) && (0.5 < jLoc) && (jLoc < N - 0.5) && (0.5 < kLoc) && (kLoc < N - 0.5))
-      cst_tri(m, iLoc, jLoc, kLoc, x, tn, s);
-    else
-      cst_multiblock(m, x, tn, s);
-    return;
+      return 0;
+    return 1;
   }
   /* search a coarser block through the faces, edges and corners near x */
   static const int faceEdges[6][4] = {{4, 11, 7, 8}, {5, 10, 6, 9}, {0, 9, 3, 8},
@@ -1571,29 +1609,43 @@ static void cst_init(const orc_mesh* m, const double x[3], int tn, orc_cst* s) {
     }
   }
 #undef ORC_TRY
-  if (coarser >= 0) {
-    cst_multiblock(m, x, m->block_node[coarser], s);
-    if ((0.5 < dmin) && (dmin <= 1.0)) {
-      orc_cst fine;
-      cst_tri(m, iLoc, jLoc, kLoc, x, tn, &fine);
-      const double cw = 1.0 - (dmin - 0.5) / 0.5, fw = (dmin - 0.5) / 0.5;
-      for (int i = 0; i < s->n; i++) s->w[i] *= cw;
-      for (int i = 0; i < fine.n; i++) fine.w[i] *= fw;
-      /* cStencilGeneric::Add, pic.h:7273-7291 */
-      for (int i = 0; i < fine.n; i++) {
-        int found = 0;
-        for (int j = 0; j < s->n; j++)
-          if (s->blk[j] == fine.blk[i] && s->cell[j] == fine.cell[i]) {
-            s->w[j] += fine.w[i];
-            found = 1;
-            break;
-          }
-        if (!found) cst_add(s, fine.w[i], fine.blk[i], fine.cell[i]);
-      }
-    }
+  *coarser_out = coarser;
+  *dmin_out = dmin;
+  return coarser >= 0 ? 2 : 0;
+}
+
+/* CellCentered::Linear::InitStencil, :224-816, AMPS branch of an AMR mesh */
+static void cst_init(const orc_mesh* m, const double x[3], int tn, orc_cst* s) {
+  double loc[3], dmin = 0.0;
+  int coarser = -1;
+  const int br = cc_branch(m, x, tn, loc, &coarser, &dmin);
+  if (br == 0) {
+    cst_tri(m, loc[0], loc[1], loc[2], x, tn, s);
     return;
   }
-  cst_tri(m, iLoc, jLoc, kLoc, x, tn, s);
+  if (br == 1) {
+    cst_multiblock(m, x, tn, s);
+    return;
+  }
+  cst_multiblock(m, x, m->block_node[coarser], s);
+  if ((0.5 < dmin) && (dmin <= 1.0)) {
+    orc_cst fine;
+    cst_tri(m, loc[0], loc[1], loc[2], x, tn, &fine);
+    const double cw = 1.0 - (dmin - 0.5) / 0.5, fw = (dmin - 0.5) / 0.5;
+    for (int i = 0; i < s->n; i++) s->w[i] *= cw;
+    for (int i = 0; i < fine.n; i++) fine.w[i] *= fw;
+    /* cStencilGeneric::Add, pic.h:7273-7291 */
+    for (int i = 0; i < fine.n; i++) {
+      int found = 0;
+      for (int j = 0; j < s->n; j++)
+        if (s->blk[j] == fine.blk[i] && s->cell[j] == fine.cell[i]) {
+          s->w[j] += fine.w[i];
+          found = 1;
+          break;
+        }
+      if (!found) cst_add(s, fine.w[i], fine.blk[i], fine.cell[i]);
+    }
+  }
 }
 
 int orc_center_stencil(const orc_mesh* m, const double x[3], int block, int* st_block,
@@ -1633,4 +1685,238 @@ static void center_B_at(const orc_mesh* m, const double x[3], int tn, double B[3
     const double* v = m->Bc + ((size_t)s.blk[i] * ORC_GN * ORC_GN * ORC_GN + (size_t)s.cell[i]) * 3;
     for (int d = 0; d < 3; d++) B[d] += s.w[i] * v[d];
   }
+}
+
+/* ===================================================================== */
+/* Mode3D compact-array field: the decomposition-independent row stencil  */
+/* (cRowStencil, pic.h:7295-7420) built by CellCentered::Linear::        */
+/* InitStencil with a row (pic_interpolation_routines.cpp:224-1070), and  */
+/* GlobalMagneticField's InterpolateField_ (GlobalMagneticField.cpp:      */
+/* 292-330) over B[(block*512 + i + 8(j + 8k))*3 + d] of interior cells.  */
+/* ===================================================================== */
+#define ORC_ROW_MAX 128 /* nMaxRowStencilLength = 2 nMaxStencilLength, pic.h:7185, 7199 */
+typedef struct {
+  int n;
+  int blk[ORC_ROW_MAX], cell[ORC_ROW_MAX]; /* cell = i + 8(j + 8k), interior */
+  double w[ORC_ROW_MAX];
+} orc_row;
+
+/* cRowStencil::AddCell, pic.h:7351-7375: an entry of the same block and cell
+ * takes the weight; otherwise a new entry is appended */
+static void row_addcell(orc_row* r, double w, int blk, int cell) {
+  for (int e = 0; e < r->n; e++)
+    if (r->blk[e] == blk && r->cell[e] == cell) {
+      r->w[e] += w;
+      return;
+    }
+  if (r->n == ORC_ROW_MAX) abort();
+  r->blk[r->n] = blk, r->cell[r->n] = cell, r->w[r->n] = w;
+  r->n++;
+}
+
+/* cRowStencil::Normalize, pic.h:7396-7403 */
+static void row_normalize(orc_row* r) {
+  double norm = 0.0;
+  for (int e = 0; e < r->n; e++) norm += r->w[e];
+  if (norm > 0.0)
+    for (int e = 0; e < r->n; e++) r->w[e] /= norm;
+}
+
+/* AddRowStencilCellByCoordinate, pic_interpolation_routines.cpp:33-82
+ * (non-periodic): a centre outside the global box is omitted; otherwise the
+ * leaf holding it and the interior cell under it, with the 1e-10 face
+ * clamps. */
+static int row_add_coord(const orc_mesh* m, orc_row* r, double w, const double xin[3], int start) {
+  const double x[3] = {xin[0], xin[1], xin[2]};
+  for (int d = 0; d < 3; d++)
+    if ((x[d] < m->gxmin[d]) || (x[d] > m->gxmax[d])) return 0;
+  const int tn = find_tnode(m, x, start < 0 ? 0 : start);
+  if (tn < 0) return 0;
+  const orc_tnode* n = &m->t[tn];
+  int ijk[3];
+  for (int d = 0; d < 3; d++) {
+    const double blockLength = n->xmax[d] - n->xmin[d];
+    const double xLoc = (x[d] - n->xmin[d]) / blockLength * ORC_NC;
+    ijk[d] = (int)floor(xLoc);
+    if ((ijk[d] < 0) && (xLoc > -1.0e-10)) ijk[d] = 0;
+    if ((ijk[d] >= ORC_NC) && (xLoc < ORC_NC + 1.0e-10)) ijk[d] = ORC_NC - 1;
+    if ((ijk[d] < 0) || (ijk[d] >= ORC_NC)) return 0;
+  }
+  row_addcell(r, w, n->block, ijk[0] + ORC_NC * (ijk[1] + ORC_NC * ijk[2]));
+  return 1;
+}
+
+/* AddRowStencilCell, :91-105: (possibly ghost) indices of tn to the centre */
+static int row_add_cell(const orc_mesh* m, orc_row* r, double w, int tn, int i, int j, int k) {
+  const orc_tnode* S = &m->t[tn];
+  const int ijk[3] = {i, j, k};
+  double x[3];
+  for (int d = 0; d < 3; d++) {
+    const double dx = (S->xmax[d] - S->xmin[d]) / ORC_NC;
+    x[d] = S->xmin[d] + (ijk[d] + 0.5) * dx;
+  }
+  return row_add_coord(m, r, w, x, tn);
+}
+
+/* GetTriliniarInterpolationStencil, row part (:820-896) */
+static void row_tri(const orc_mesh* m, double iLoc, double jLoc, double kLoc, int tn, orc_row* r) {
+  r->n = 0;
+  const int i0 = (iLoc < 0.5) ? -1 : (int)(iLoc - 0.50);
+  const int j0 = (jLoc < 0.5) ? -1 : (int)(jLoc - 0.50);
+  const int k0 = (kLoc < 0.5) ? -1 : (int)(kLoc - 0.50);
+  double w[3];
+  w[0] = iLoc - (i0 + 0.5);
+  w[1] = jLoc - (j0 + 0.5);
+  w[2] = kLoc - (k0 + 0.5);
+  for (int i = 0; i < 2; i++)
+    for (int j = 0; j < 2; j++)
+      for (int k = 0; k < 2; k++) {
+        double W;
+        switch (i + 2 * j + 4 * k) {
+          case 0: W = (1.0 - w[0]) * (1.0 - w[1]) * (1.0 - w[2]); break;
+          case 1: W = w[0] * (1.0 - w[1]) * (1.0 - w[2]); break;
+          case 2: W = (1.0 - w[0]) * w[1] * (1.0 - w[2]); break;
+          case 3: W = w[0] * w[1] * (1.0 - w[2]); break;
+          case 4: W = (1.0 - w[0]) * (1.0 - w[1]) * w[2]; break;
+          case 5: W = w[0] * (1.0 - w[1]) * w[2]; break;
+          case 6: W = (1.0 - w[0]) * w[1] * w[2]; break;
+          default: W = w[0] * w[1] * w[2]; break;
+        }
+        row_add_cell(m, r, W, tn, i0 + i, j0 + j, k0 + k);
+      }
+  if ((r->n > 0) && (r->n != 8)) row_normalize(r);
+}
+
+/* GetTriliniarInterpolationMutiBlockStencil, row part (:912-1054) */
+static void row_multiblock(const orc_mesh* m, const double x[3], int tn, orc_row* r) {
+  const orc_tnode* n = &m->t[tn];
+  r->n = 0;
+  double dxCell[3], xLower[3], xLoc[3];
+  int ijkMin[3];
+  for (int d = 0; d < 3; d++) dxCell[d] = (n->xmax[d] - n->xmin[d]) / ORC_NC;
+  for (int d = 0; d < 3; d++) {
+    ijkMin[d] = (x[d] - n->xmin[d] < 0.5 * dxCell[d]) ? -1 : (int)((x[d] - n->xmin[d] - 0.5 * dxCell[d]) / dxCell[d]);
+    xLower[d] = n->xmin[d] + (ijkMin[d] + 0.5) * dxCell[d];
+    xLoc[d] = (x[d] - xLower[d]) / dxCell[d];
+    if (xLoc[d] < 0.0) xLoc[d] = 0.0;
+    if (xLoc[d] > 1.0) xLoc[d] = 1.0;
+  }
+  int geometry = 1;
+  for (int di = 0; di < 2; di++)
+    for (int dj = 0; dj < 2; dj++)
+      for (int dk = 0; dk < 2; dk++) {
+        const int ijk[3] = {ijkMin[0] + di, ijkMin[1] + dj, ijkMin[2] + dk};
+        double xl[3];
+        for (int d = 0; d < 3; d++) xl[d] = n->xmin[d] + (ijk[d] + 0.5) * dxCell[d];
+        double W;
+        switch (di + 2 * dj + 4 * dk) {
+          case 0: W = (1.0 - xLoc[0]) * (1.0 - xLoc[1]) * (1.0 - xLoc[2]); break;
+          case 1: W = xLoc[0] * (1.0 - xLoc[1]) * (1.0 - xLoc[2]); break;
+          case 2: W = (1.0 - xLoc[0]) * xLoc[1] * (1.0 - xLoc[2]); break;
+          case 3: W = xLoc[0] * xLoc[1] * (1.0 - xLoc[2]); break;
+          case 4: W = (1.0 - xLoc[0]) * (1.0 - xLoc[1]) * xLoc[2]; break;
+          case 5: W = xLoc[0] * (1.0 - xLoc[1]) * xLoc[2]; break;
+          case 6: W = (1.0 - xLoc[0]) * xLoc[1] * xLoc[2]; break;
+          default: W = xLoc[0] * xLoc[1] * xLoc[2]; break;
+        }
+        const int sn = find_tnode(m, xl, tn);
+        if (sn < 0) {
+          geometry = 0;
+          continue;
+        }
+        const orc_tnode* S = &m->t[sn];
+        if (S->level == n->level) {
+          row_add_cell(m, r, W, tn, ijk[0], ijk[1], ijk[2]);
+        } else {
+          int iNeib[3];
+          for (int d = 0; d < 3; d++) iNeib[d] = 2 * ((int)((xl[d] - S->xmin[d]) / dxCell[d]));
+          for (int ii = 0; ii < 2; ii++)
+            for (int jj = 0; jj < 2; jj++)
+              for (int kk = 0; kk < 2; kk++) {
+                const double FineCellWeight = (1.0 / 8.0) * W;
+                row_add_cell(m, r, FineCellWeight, sn, iNeib[0] + ii, iNeib[1] + jj, iNeib[2] + kk);
+              }
+        }
+      }
+  if (!geometry) {
+    /* BuildConstantRowStencil (:109-115) at the leaf holding x */
+    const int xn = find_tnode(m, x, tn);
+    r->n = 0;
+    row_add_coord(m, r, 1.0, x, xn);
+  }
+}
+
+/* CellCentered::Linear::InitStencil(x, node, row) (pic.h:7528-7532) */
+static void row_init(const orc_mesh* m, const double x[3], int tn, orc_row* r) {
+  double loc[3], dmin = 0.0;
+  int coarser = -1;
+  const int br = cc_branch(m, x, tn, loc, &coarser, &dmin);
+  if (br == 0) {
+    row_tri(m, loc[0], loc[1], loc[2], tn, r);
+    return;
+  }
+  if (br == 1) {
+    row_multiblock(m, x, tn, r);
+    return;
+  }
+  row_multiblock(m, x, m->block_node[coarser], r);
+  if ((0.5 < dmin) && (dmin <= 1.0)) {
+    /* the coarse/fine blend of :660-681: MultiplyScalar on both rows, then
+     * cRowStencil::Add (AddCell per fine entry, merging) */
+    orc_row fine;
+    row_tri(m, loc[0], loc[1], loc[2], tn, &fine);
+    const double cw = 1.0 - (dmin - 0.5) / 0.5, fw = (dmin - 0.5) / 0.5;
+    for (int e = 0; e < r->n; e++) r->w[e] *= cw;
+    for (int e = 0; e < fine.n; e++) fine.w[e] *= fw;
+    for (int e = 0; e < fine.n; e++) row_addcell(r, fine.w[e], fine.blk[e], fine.cell[e]);
+  }
+}
+
+int orc_row_stencil(const orc_mesh* m, const double x[3], int* st_block, int* st_cell, double* st_w) {
+  const int tn = find_tnode(m, x, 0);
+  if (tn < 0) return 0;
+  orc_row r;
+  row_init(m, x, tn, &r);
+  for (int e = 0; e < r.n; e++) st_block[e] = r.blk[e], st_cell[e] = r.cell[e], st_w[e] = r.w[e];
+  return r.n;
+}
+
+/* cMode3DMeshFieldEval::GetB_T, CutoffRigidityMode3D.cpp:702-800, mesh path:
+ * zero outside the mesh; otherwise InterpolateField_ (GlobalMagneticField.cpp:
+ * 292-330), the row's weighted sum in row order.  Returns 0 for an empty
+ * row (the reference exits there). */
+static int mode3d_B(const orc_mesh* m, const double* Bm, const double x[3], double B[3]) {
+  B[0] = B[1] = B[2] = 0.0;
+  const int tn = find_tnode(m, x, 0);
+  if (tn < 0) return 1;
+  orc_row r;
+  row_init(m, x, tn, &r);
+  if (r.n <= 0) return 0;
+  for (int e = 0; e < r.n; e++) {
+    const double* v = Bm + ((size_t)r.blk[e] * ORC_NCELL + (size_t)r.cell[e]) * 3;
+    B[0] += r.w[e] * v[0];
+    B[1] += r.w[e] * v[1];
+    B[2] += r.w[e] * v[2];
+  }
+  return 1;
+}
+
+int orc_mode3d_field(const orc_mesh* m, const double* Bm, int64_t n, const double* x, double* B) {
+  int ok = 1;
+  for (int64_t p = 0; p < n; p++) ok &= mode3d_B(m, Bm, x + 3 * p, B + 3 * p);
+  return ok;
+}
+
+static const orc_mesh* g_trace_mesh = NULL;
+static const double* g_trace_Bm = NULL;
+void orc_set_trace_mesh(const orc_mesh* m, const double* Bm) {
+  g_trace_mesh = m;
+  g_trace_Bm = Bm;
+}
+
+/* the tracer's field: the analytic dipole plus the uniform field, or the
+ * Mode3D mesh field of orc_set_trace_mesh */
+static void trace_field(const orc_trace_params* P, const double x[3], double B[3]) {
+  if (P->field_model == 1) mode3d_B(g_trace_mesh, g_trace_Bm, x, B);
+  else orc_dipole_B(P, x, B);
 }

@@ -99,6 +99,8 @@ typedef struct {
   double moment_Am2;     /* dipole |m| */
   double m_hat[3];
   double B_ext_T[3];     /* uniform field added to the dipole */
+  int32_t field_model;   /* 0 dipole + uniform, 1 Mode3D mesh (orc_set_trace_mesh) */
+  int32_t dt_selector;   /* 0 SelectLegacyCutoffTraceDt, 1 its Mode3D form (metres) */
 } orc_trace_params;
 
 typedef struct {
@@ -151,6 +153,15 @@ void orc_interp_center(const orc_mesh* m, const double* Bc, int64_t n, const dou
  * Lapenta/Boris movers and orc_deposit interpolate B with the cell-centred
  * stencil instead of the corner values. */
 void orc_set_center_field(orc_mesh* m, const double* Bc);
+/* Mode3D compact-array field: row stencil at x (entries: block, interior
+ * cell i + 8(j + 8k), weight; returns the length) and the field at n points
+ * x[3n] from Bm[(block*512 + cell)*3 + d] into B[3n] (0 where a row is
+ * empty). */
+int orc_row_stencil(const orc_mesh* m, const double x[3], int* st_block, int* st_cell, double* st_w);
+int orc_mode3d_field(const orc_mesh* m, const double* Bm, int64_t n, const double* x, double* B);
+/* mesh and field of the tracer's field_model 1 (not thread-safe to change
+ * during a trace batch) */
+void orc_set_trace_mesh(const orc_mesh* m, const double* Bm);
 /* per-block neighbour data used by the stencil: min and max neighbour level,
  * and the neighbour block across each face (6), edge (12) and corner (8) */
 void orc_block_neighbours(const orc_mesh* m, int block, int* minmax, int* face, int* edge,

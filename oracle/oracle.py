@@ -31,7 +31,7 @@ class TraceParams(C.Structure):
         ("retry", C.c_int32), ("mover", C.c_int32), ("box_min_Re", C.c_double * 3),
         ("box_max_Re", C.c_double * 3), ("r_outer_Re", C.c_double), ("r_inner_Re", C.c_double),
         ("earth_radius_m", C.c_double), ("moment_Am2", C.c_double), ("m_hat", C.c_double * 3),
-        ("B_ext_T", C.c_double * 3),
+        ("B_ext_T", C.c_double * 3), ("field_model", C.c_int32), ("dt_selector", C.c_int32),
     ]
 
 
@@ -87,6 +87,11 @@ def lib():
         l.orc_interp_center.argtypes = [vp, P(dbl), i64, P(dbl), P(C.c_int), P(dbl), i32]
         l.orc_block_neighbours.argtypes = [vp, i32, P(C.c_int), P(C.c_int), P(C.c_int), P(C.c_int)]
         l.orc_set_center_field.argtypes = [vp, P(dbl)]
+        l.orc_row_stencil.argtypes = [vp, P(dbl), P(C.c_int), P(C.c_int), P(dbl)]
+        l.orc_row_stencil.restype = i32
+        l.orc_mode3d_field.argtypes = [vp, P(dbl), i64, P(dbl), P(dbl)]
+        l.orc_mode3d_field.restype = i32
+        l.orc_set_trace_mesh.argtypes = [vp, P(dbl)]
         _lib = l
     return _lib
 
@@ -238,6 +243,31 @@ class Mesh:
         None restores the corner B."""
         self._Bc = None if Bc is None else np.ascontiguousarray(Bc, np.float64)
         lib().orc_set_center_field(self.h, None if self._Bc is None else _p(self._Bc, C.c_double))
+
+    def row_stencil(self, x):
+        """Mode3D row stencil at x: (blocks, interior cells i + 8(j + 8k), weights)."""
+        x = np.ascontiguousarray(x, np.float64)
+        bb = np.zeros(128, np.int32)
+        cc = np.zeros(128, np.int32)
+        ww = np.zeros(128)
+        n = lib().orc_row_stencil(self.h, _p(x, C.c_double), _p(bb, C.c_int), _p(cc, C.c_int),
+                                  _p(ww, C.c_double))
+        return bb[:n], cc[:n], ww[:n]
+
+    def mode3d_field(self, Bm, x):
+        """Mode3D mesh field at points x[n][3] from interior-cell data Bm[nb][512][3]."""
+        Bm = np.ascontiguousarray(Bm, np.float64)
+        x = np.ascontiguousarray(x, np.float64).reshape(-1, 3)
+        B = np.zeros_like(x)
+        ok = lib().orc_mode3d_field(self.h, _p(Bm, C.c_double), len(x), _p(x, C.c_double), _p(B, C.c_double))
+        if not ok:
+            raise ValueError("empty row stencil")
+        return B
+
+    def set_trace_field(self, Bm):
+        """Mesh field of the tracer's field_model 1 (kept referenced here)."""
+        self._Bm = np.ascontiguousarray(Bm, np.float64)
+        lib().orc_set_trace_mesh(self.h, _p(self._Bm, C.c_double))
 
     def neighbours(self, b):
         mm = np.zeros(2, np.int32)

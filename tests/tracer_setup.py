@@ -96,3 +96,39 @@ def oracle_cutoff(P, x0, dirv, rmin, rmax, nscan, rel_tol=None):
         else:
             lo = mid
     return hi
+
+
+def geospace_mesh(half_Re=16.0, base=3, r_refine_Re=5.0, re_m=RE_AMPS):
+    """Block mesh over [-half, half]^3 Re in metres: level-`base` blocks, those
+    within r_refine of the origin refined once (a coarse/fine shell around the
+    Earth for the Mode3D field tests)."""
+    from amps_amd import mesh as M
+    L = half_Re * re_m
+    m = M.uniform_mesh(base, xmin=(-L, -L, -L), xmax=(L, L, L))
+    h = 2.0 * L / (1 << base)
+
+    def pred(lv, i, j, k):
+        lo = -L + np.array([i, j, k]) * h
+        c = np.clip(0.0, lo, lo + h)
+        return float(np.sum(c * c)) < (r_refine_Re * re_m) ** 2
+
+    lv, ijk = M.refine(m.level, m.ijk, pred)
+    return M.BlockMesh(lv, ijk, xmin=(-L, -L, -L), xmax=(L, L, L), lmax=m.lmax)
+
+
+def dipole_cells(mesh, P, soft_Re=0.5):
+    """The analytic dipole + uniform field of P sampled at every interior cell
+    centre, [nb][512][3] (r clamped to soft_Re inside the Earth)."""
+    from amps_amd import mesh as M
+    Re = P.earth_radius_m
+
+    def f(X):
+        r = np.linalg.norm(X, axis=1, keepdims=True)
+        Xs = np.where(r < soft_Re * Re, X * (soft_Re * Re / np.maximum(r, 1e-30)), X)
+        r = np.linalg.norm(Xs, axis=1, keepdims=True)
+        m = P.moment_Am2 * np.array(P.m_hat[:])
+        mdr = Xs @ m
+        B = 1e-7 * (3.0 * Xs * mdr[:, None] / r ** 5 - m[None, :] / r ** 3)
+        return B + np.array(P.B_ext_T[:])[None, :]
+
+    return M.interior_center_fields(mesh, f)
