@@ -87,11 +87,14 @@ __device__ __forceinline__ void field64(const TraceView& a, const double* x, dou
 }
 
 // BorisStep, GridlessParticleMovers.cpp:781-801
+// Bx: B at x when the step selector has already evaluated it (have_B)
 template <int FM>
-__device__ __forceinline__ void boris64(const TraceView& a, double* x, double* p, double dt) {
+__device__ __forceinline__ void boris64(const TraceView& a, double* x, double* p, double dt, const double* Bx,
+                                        bool have_B) {
   const amps_trace_params& P = a.P;
   double B[3];
-  field64<FM>(a, x, B);
+  if (have_B) B[0] = Bx[0], B[1] = Bx[1], B[2] = Bx[2];
+  else field64<FM>(a, x, B);
   const double m0 = P.m0_kg, q = P.q_C;
   const double gam = gamma64(p, m0);
   const double iv = 1.0 / (gam * m0);
@@ -140,7 +143,7 @@ __device__ __forceinline__ bool seg_hits(const double* a, const double* b, doubl
 // always on.
 template <int FM>
 __device__ __forceinline__ double select_dt3d(const TraceView& a, const double* x, const double* p, double dtTrace,
-                                              double timeRemaining) {
+                                              double timeRemaining, double* B, bool& have_B) {
   const amps_trace_params& P = a.P;
   double dt = dtTrace;
   if (timeRemaining < dt) dt = timeRemaining;
@@ -151,8 +154,8 @@ __device__ __forceinline__ double select_dt3d(const TraceView& a, const double* 
   const double gamma = sqrt(1.0 + p2 / (mc * mc));
   const double pMag = sqrt(dmax(0.0, p2));
   const double vMag = (gamma > 0.0 && m0 > 0.0) ? pMag / (gamma * m0) : 0.0;
-  double B[3];
   field64<FM>(a, x, B);
+  have_B = true;
   const double Bmag = sqrt(dot3(B, B));
   if (Bmag > 0.0) {
     const double omega = fabs(P.q_C) * Bmag / (gamma * m0);
@@ -253,7 +256,8 @@ struct RKTable<MV_RK6> {
 // A stage chord meeting the sphere leaves x at the stage point and p at its
 // start value; the final chord leaves both at y1.
 template <int MV, int FM>
-__device__ __forceinline__ bool rk_checked64(const TraceView& a, double* x, double* p, double dt, double rIn) {
+__device__ __forceinline__ bool rk_checked64(const TraceView& a, double* x, double* p, double dt, double rIn,
+                                             const double* Bx, bool have_B) {
   const amps_trace_params& P = a.P;
   using T = RKTable<MV>;
   double kx[T::NS][3], kp[T::NS][3];
@@ -284,8 +288,10 @@ __device__ __forceinline__ bool rk_checked64(const TraceView& a, double* x, doub
 #pragma unroll
     for (int d = 0; d < 3; d++) prev[d] = yx[d];
     // LorentzRhs, GridlessParticleMovers.cpp:151-157
+    // the first stage sits at x itself (no table terms), where B may be known
     double B[3];
-    field64<FM>(a, yx, B);
+    if (i == 0 && have_B) B[0] = Bx[0], B[1] = Bx[1], B[2] = Bx[2];
+    else field64<FM>(a, yx, B);
     const double iv = 1.0 / (gamma64(yp, P.m0_kg) * P.m0_kg);
     const double v[3] = {iv * yp[0], iv * yp[1], iv * yp[2]};
     kx[i][0] = v[0], kx[i][1] = v[1], kx[i][2] = v[2];
@@ -311,10 +317,11 @@ __device__ __forceinline__ bool rk_checked64(const TraceView& a, double* x, doub
 // StepParticleChecked, GridlessParticleMovers.cpp:1035-1098 (full-orbit
 // movers): false when the step met the inner sphere.
 template <int MV, int FM>
-__device__ __forceinline__ bool step_checked64(const TraceView& a, double* x, double* p, double dt, double rIn) {
+__device__ __forceinline__ bool step_checked64(const TraceView& a, double* x, double* p, double dt, double rIn,
+                                               const double* Bx, bool have_B) {
   if constexpr (MV == MV_BORIS) {
     const double x0[3] = {x[0], x[1], x[2]};
-    boris64<FM>(a, x, p, dt);
+    boris64<FM>(a, x, p, dt, Bx, have_B);
     return !seg_hits(x0, x, rIn) && !inside_inner_m(x, rIn);
   } else if constexpr (MV == MV_HC4) {
     // four positive macro-steps, each chord checked; the composition's
@@ -327,14 +334,14 @@ __device__ __forceinline__ bool step_checked64(const TraceView& a, double* x, do
     }
     return true;
   } else {
-    return rk_checked64<MV, FM>(a, x, p, dt, rIn);
+    return rk_checked64<MV, FM>(a, x, p, dt, rIn, Bx, have_B);
   }
 }
 
 // SelectLegacyCutoffTraceDt, CutoffRigidityGridless.cpp:1086-1152 (switchable parts)
 template <int FM>
 __device__ __forceinline__ double select_dt64(const TraceView& a, const double* x, const double* p, double dtTrace,
-                                              double timeRemaining) {
+                                              double timeRemaining, double* B, bool& have_B) {
   const amps_trace_params& P = a.P;
   double dt = dtTrace;
   if (timeRemaining < dt) dt = timeRemaining;
@@ -345,8 +352,8 @@ __device__ __forceinline__ double select_dt64(const TraceView& a, const double* 
   const double gamma = sqrt(1.0 + p2 / (mc * mc));
   const double pMag = sqrt(dmax(0.0, p2));
   const double vMag = (gamma > 0.0 && m0 > 0.0) ? pMag / (gamma * m0) : 0.0;
-  double B[3];
   field64<FM>(a, x, B);
+  have_B = true;
   const double Bmag = sqrt(dot3(B, B));
   if (Bmag > 0.0) {
     const double omega = fabs(P.q_C) * Bmag / (gamma * m0);
@@ -430,11 +437,15 @@ struct Fp64Stepper {
       inside = x[0] >= a.box_min_m[0] - 0.0 && x[0] <= a.box_max_m[0] + 0.0 && x[1] >= a.box_min_m[1] - 0.0 &&
                x[1] <= a.box_max_m[1] + 0.0 && x[2] >= a.box_min_m[2] - 0.0 && x[2] <= a.box_max_m[2] + 0.0;
     if (!inside) return 0;
-    const double dt = (P.dt_selector == 1) ? select_dt3d<FM>(a, x, p, L.dtTrace, L.maxTime - L.t)
-                                           : select_dt64<FM>(a, x, p, L.dtTrace, L.maxTime - L.t);
+    // B at the start point: evaluated once by the adaptive step selector and
+    // reused by the step (the same function of the same point)
+    double Bx[3];
+    bool have_B = false;
+    const double dt = (P.dt_selector == 1) ? select_dt3d<FM>(a, x, p, L.dtTrace, L.maxTime - L.t, Bx, have_B)
+                                           : select_dt64<FM>(a, x, p, L.dtTrace, L.maxTime - L.t, Bx, have_B);
     if (!(dt > 0.0) || !isfinite(dt)) return 6;
     const double xb[3] = {x[0], x[1], x[2]};
-    if (!step_checked64<MV, FM>(a, x, p, dt, rIn)) return 1;
+    if (!step_checked64<MV, FM>(a, x, p, dt, rIn, Bx, have_B)) return 1;
     if (!fin3(x) || !fin3(p)) return 8;
     const double dd[3] = {x[0] - xb[0], x[1] - xb[1], x[2] - xb[2]};
     const double seg = sqrt(dot3(dd, dd));
