@@ -17,11 +17,11 @@ using namespace amps;
 template <int MOVER, bool BC, int W>
 __global__ void k_push_f64(MeshView, Faces, Species, const double*, const double*, const Tile*,
                            const int*, int, double, int64_t, double*, double*, const uint8_t*,
-                           int*, int8_t*, const double*);
+                           int*, int8_t*, const double*, int*);
 template <int MOVER, int W>
 __global__ void k_push_f32(MeshView, Faces, Species, const double*, const double*, const Tile*,
                            const int*, int, double, int64_t, double*, float*, const uint8_t*, int*,
-                           int8_t*);
+                           int8_t*, int*);
 __global__ void k_fill_int(int*, int64_t, int);
 __global__ void k_count_keys(const int*, const int*, const int8_t*, int*);
 __global__ void k_scatter(const int*, const int*, const int8_t*, const int*, int*, int64_t,
@@ -710,15 +710,24 @@ extern "C" int amps_gpu_get_particles(amps_ctx* c, double* x, double* v, int32_t
 // ---------------------------------------------------------------------
 // migration
 // ---------------------------------------------------------------------
-static int compact_impl(amps_ctx* c) {
+// zeroed per-key counters (d_offset) and scatter cursors
+static void clear_counts(amps_ctx* c) {
+  const int64_t nkeys = (int64_t)c->mv.nb * NCELL;
+  hipLaunchKernelGGL(k_fill_int, dim3(1024), dim3(256), 0, c->stream, c->d_offset, nkeys + 1, 0);
+  hipLaunchKernelGGL(k_fill_int, dim3(1024), dim3(256), 0, c->stream, c->d_cursor, nkeys, 0);
+}
+
+// counted: the push has already counted the survivors into a cleared d_offset
+static int compact_impl(amps_ctx* c, bool counted = false) {
   const int64_t nkeys = (int64_t)c->mv.nb * NCELL;
   const int64_t cap = c->cap;
   const unsigned gridp = (unsigned)((cap + 255) / 256);
   const int a = c->cur, b = 1 - c->cur;
-  hipLaunchKernelGGL(k_fill_int, dim3(1024), dim3(256), 0, c->stream, c->d_offset, nkeys + 1, 0);
-  hipLaunchKernelGGL(k_fill_int, dim3(1024), dim3(256), 0, c->stream, c->d_cursor, nkeys, 0);
-  hipLaunchKernelGGL(k_count_keys, dim3(gridp), dim3(256), 0, c->stream, c->d_n, c->d_key[a],
-                     c->d_code, c->d_offset);
+  if (!counted) {
+    clear_counts(c);
+    hipLaunchKernelGGL(k_count_keys, dim3(gridp), dim3(256), 0, c->stream, c->d_n, c->d_key[a],
+                       c->d_code, c->d_offset);
+  }
   if (int r = scan_exclusive(c, c->d_offset, nkeys + 1, nullptr)) return r;
   hipLaunchKernelGGL(k_scatter, dim3(gridp), dim3(256), 0, c->stream, c->d_n, c->d_key[a],
                      c->d_code, c->d_offset, c->d_cursor, cap, c->d_x[a], c->d_v[a],
@@ -761,13 +770,15 @@ static int check_mover(const amps_ctx* c, const amps_push_opts* o, const char* w
   return 0;
 }
 
-static int launch_push(amps_ctx* c, const amps_push_opts* o) {
+// count: when non-null, the push also counts the surviving particles per key
+// (the migration's counting pass), into a zeroed array
+static int launch_push(amps_ctx* c, const amps_push_opts* o, int* count = nullptr) {
   const int k = c->cur;
   const bool w4 = c->push_waves == 4;
 #define LAUNCH_PUSH32(MV, W)                                                                              \
   hipLaunchKernelGGL((k_push_f32<MV, W>), dim3((unsigned)c->max_tiles), dim3(256), 0, c->stream, c->mv,     \
                      c->faces, c->sp, c->d_E, c->d_B, c->d_tiles, c->d_ntiles, o->bc_mode, o->r_sphere, c->cap, \
-                     c->d_x[k], (float*)c->d_v[k], c->d_spec[k], c->d_key[k], c->d_code)
+                     c->d_x[k], (float*)c->d_v[k], c->d_spec[k], c->d_key[k], c->d_code, count)
   if (c->precision == AMPS_FP32) {
     if (o->mover == AMPS_MOVER_LAPENTA) {
       if (w4) LAUNCH_PUSH32(AMPS_MOVER_LAPENTA, 4);
@@ -783,7 +794,7 @@ static int launch_push(amps_ctx* c, const amps_push_opts* o) {
 #define LAUNCH_PUSH64(MV, BCV, W)                                                                       \
   hipLaunchKernelGGL((k_push_f64<MV, BCV, W>), dim3((unsigned)c->max_tiles), dim3(256), 0, c->stream, c->mv,  \
                      c->faces, c->sp, c->d_E, c->d_B, c->d_tiles, c->d_ntiles, o->bc_mode, o->r_sphere, c->cap, \
-                     c->d_x[k], c->d_v[k], c->d_spec[k], c->d_key[k], c->d_code, c->d_Bc)
+                     c->d_x[k], c->d_v[k], c->d_spec[k], c->d_key[k], c->d_code, c->d_Bc, count)
   const bool bc = c->b_center != 0;
   if (o->mover == AMPS_MOVER_LAPENTA) {
     if (bc) LAUNCH_PUSH64(AMPS_MOVER_LAPENTA, true, 1);
@@ -978,11 +989,13 @@ extern "C" int amps_gpu_generate_particles(amps_ctx* c, int64_t n_per_species, i
 }
 
 static int one_step(amps_ctx* c, const amps_push_opts* o, double dt) {
+  // the push counts the survivors per key for the migration that follows
+  clear_counts(c);
   tbegin(c);
-  if (int r = launch_push(c, o)) return r;
+  if (int r = launch_push(c, o, c->d_offset)) return r;
   tend(c, AMPS_K_PUSH);
   tbegin(c);
-  if (int r = compact_impl(c)) return r;
+  if (int r = compact_impl(c, true)) return r;
   tend(c, AMPS_K_COMPACT);
   return launch_deposit(c, dt);
 }

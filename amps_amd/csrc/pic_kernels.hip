@@ -17,130 +17,6 @@ using namespace amps;
 // corner E and B (729 x 3 doubles each) are staged in LDS (SetBlock_E/B,
 // pic_mover.cpp:84-186), particles are read and written coalesced (SoA).
 // ======================================================================
-// W: the occupancy the register allocation aims at (waves per SIMD).
-template <int MOVER, bool BC, int W>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W))) void k_push_f64(MeshView m, Faces faces, Species sp,
-                                                   const double* __restrict__ E,
-                                                   const double* __restrict__ B,
-                                                   const Tile* __restrict__ tiles,
-                                                   const int* __restrict__ ntiles, int bc_mode,
-                                                   double r_sphere, int64_t cap, double* x,
-                                                   double* v, const uint8_t* __restrict__ spec,
-                                                   int* key, int8_t* code, const double* __restrict__ Bc) {
-  __shared__ double sE[NCORNER * 3];
-  __shared__ double sB[NCORNER * 3];
-  const int t = blockIdx.x;
-  if (t >= *ntiles) return;
-  const Tile tl = tiles[t];
-  const int b = tl.block;
-  const double* Eb = E + (size_t)b * NCORNER * 3;
-  const double* Bb = B + (size_t)b * NCORNER * 3;
-  for (int i = threadIdx.x; i < NCORNER * 3; i += blockDim.x) {
-    sE[i] = Eb[i];
-    sB[i] = Bb[i];
-  }
-  __syncthreads();
-  BlockGeom g;
-  load_block_geom(m, b, g);
-  BlockLat lat;
-  load_block_lat(m, b, lat);
-  const int p1 = tl.start + tl.count;
-  for (int p = tl.start + threadIdx.x; p < p1; p += blockDim.x) {
-    double xp[3] = {x[p], x[cap + p], x[2 * cap + p]};
-    double vp[3] = {v[p], v[cap + p], v[2 * cap + p]};
-    const int s = spec[p];
-    int nb_out = b, cell_out = 0;
-    int c;
-    if constexpr (MOVER == AMPS_MOVER_RELATIVISTIC)
-      c = push_particle_rel(m, faces, b, g, sE, sB, E, B, sp.charge[s], sp.mass[s], sp.dt[s], bc_mode,
-                            r_sphere, xp, vp, &nb_out, &cell_out);
-    else
-      c = push_particle<MOVER, BC>(m, faces, g, lat, sE, sB, sp.charge[s], sp.mass[s], sp.dt[s], bc_mode,
-                                   r_sphere, xp, vp, &nb_out, &cell_out, Bc);
-    x[p] = xp[0];
-    x[cap + p] = xp[1];
-    x[2 * cap + p] = xp[2];
-    v[p] = vp[0];
-    v[cap + p] = vp[1];
-    v[2 * cap + p] = vp[2];
-    if (c == AMPS_CODE_FINISHED || c == AMPS_CODE_REFLECTED) key[p] = nb_out * NCELL + cell_out;
-    code[p] = (int8_t)c;
-  }
-}
-
-#define INST_PUSH64(MV, BCV, W)                                                                  \
-  template __global__ void k_push_f64<MV, BCV, W>(MeshView, Faces, Species, const double*, const double*, \
-                                                  const Tile*, const int*, int, double, int64_t, double*, \
-                                                  double*, const uint8_t*, int*, int8_t*, const double*);
-INST_PUSH64(AMPS_MOVER_LAPENTA, false, 1)
-INST_PUSH64(AMPS_MOVER_BORIS, false, 1)
-INST_PUSH64(AMPS_MOVER_LAPENTA, false, 4)
-INST_PUSH64(AMPS_MOVER_BORIS, false, 4)
-INST_PUSH64(AMPS_MOVER_RELATIVISTIC, false, 1)
-INST_PUSH64(AMPS_MOVER_LAPENTA, true, 1)
-INST_PUSH64(AMPS_MOVER_BORIS, true, 1)
-
-// FP32 push: float E/B stencil in LDS (17.5 KB), float velocities, FP64
-// positions.
-template <int MOVER, int W>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W))) void k_push_f32(MeshView m, Faces faces, Species sp,
-                                                   const double* __restrict__ E,
-                                                   const double* __restrict__ B,
-                                                   const Tile* __restrict__ tiles,
-                                                   const int* __restrict__ ntiles, int bc_mode,
-                                                   double r_sphere, int64_t cap, double* x,
-                                                   float* v, const uint8_t* __restrict__ spec,
-                                                   int* key, int8_t* code) {
-  __shared__ float sE[NCORNER * 3];
-  __shared__ float sB[NCORNER * 3];
-  const int t = blockIdx.x;
-  if (t >= *ntiles) return;
-  const Tile tl = tiles[t];
-  const int b = tl.block;
-  const double* Eb = E + (size_t)b * NCORNER * 3;
-  const double* Bb = B + (size_t)b * NCORNER * 3;
-  for (int i = threadIdx.x; i < NCORNER * 3; i += blockDim.x) {
-    sE[i] = (float)Eb[i];
-    sB[i] = (float)Bb[i];
-  }
-  __syncthreads();
-  BlockGeom g;
-  load_block_geom(m, b, g);
-  BlockLat lat;
-  load_block_lat(m, b, lat);
-  const int p1 = tl.start + tl.count;
-  for (int p = tl.start + threadIdx.x; p < p1; p += blockDim.x) {
-    double xp[3] = {x[p], x[cap + p], x[2 * cap + p]};
-    float vp[3] = {v[p], v[cap + p], v[2 * cap + p]};
-    const int s = spec[p];
-    int nb_out = b, cell_out = 0;
-    const int c = push_particle_f32<MOVER>(m, faces, g, lat, sE, sB, (float)sp.charge[s], (float)sp.mass[s],
-                                           sp.dt[s], bc_mode, r_sphere, xp, vp, &nb_out, &cell_out);
-    x[p] = xp[0];
-    x[cap + p] = xp[1];
-    x[2 * cap + p] = xp[2];
-    v[p] = vp[0];
-    v[cap + p] = vp[1];
-    v[2 * cap + p] = vp[2];
-    if (c == AMPS_CODE_FINISHED || c == AMPS_CODE_REFLECTED) key[p] = nb_out * NCELL + cell_out;
-    code[p] = (int8_t)c;
-  }
-}
-
-#define INST_PUSH32(MV, W)                                                                        \
-  template __global__ void k_push_f32<MV, W>(MeshView, Faces, Species, const double*, const double*, \
-                                             const Tile*, const int*, int, double, int64_t, double*,  \
-                                             float*, const uint8_t*, int*, int8_t*);
-INST_PUSH32(AMPS_MOVER_LAPENTA, 1)
-INST_PUSH32(AMPS_MOVER_BORIS, 1)
-INST_PUSH32(AMPS_MOVER_LAPENTA, 4)
-INST_PUSH32(AMPS_MOVER_BORIS, 4)
-
-// ======================================================================
-// Migration: counting sort by key = block*512 + cell with wavefront
-// run-length aggregation (ballot + lane masks) so that one atomic serves a
-// whole run of equal keys; deleted particles are dropped.
-// ======================================================================
 __device__ __forceinline__ bool alive_code(int c) {
   return c == AMPS_CODE_FINISHED || c == AMPS_CODE_REFLECTED;
 }
@@ -171,6 +47,157 @@ __device__ __forceinline__ RunInfo wave_runs(bool valid, int k) {
   return r;
 }
 
+// W: the occupancy the register allocation aims at (waves per SIMD).
+template <int MOVER, bool BC, int W>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W))) void k_push_f64(MeshView m, Faces faces, Species sp,
+                                                   const double* __restrict__ E,
+                                                   const double* __restrict__ B,
+                                                   const Tile* __restrict__ tiles,
+                                                   const int* __restrict__ ntiles, int bc_mode,
+                                                   double r_sphere, int64_t cap, double* x,
+                                                   double* v, const uint8_t* __restrict__ spec,
+                                                   int* key, int8_t* code, const double* __restrict__ Bc,
+                                                   int* __restrict__ count) {
+  __shared__ double sE[NCORNER * 3];
+  __shared__ double sB[NCORNER * 3];
+  const int t = blockIdx.x;
+  if (t >= *ntiles) return;
+  const Tile tl = tiles[t];
+  const int b = tl.block;
+  const double* Eb = E + (size_t)b * NCORNER * 3;
+  const double* Bb = B + (size_t)b * NCORNER * 3;
+  for (int i = threadIdx.x; i < NCORNER * 3; i += blockDim.x) {
+    sE[i] = Eb[i];
+    sB[i] = Bb[i];
+  }
+  __syncthreads();
+  BlockGeom g;
+  load_block_geom(m, b, g);
+  BlockLat lat;
+  load_block_lat(m, b, lat);
+  const int p1 = tl.start + tl.count;
+  // every lane runs the same number of passes, so that the wavefront's run
+  // aggregation of the migration count sees all 64 lanes
+  for (int p0 = tl.start; p0 < p1; p0 += blockDim.x) {
+    const int p = p0 + threadIdx.x;
+    bool alive = false;
+    int knew = -1;
+    if (p < p1) {
+    double xp[3] = {x[p], x[cap + p], x[2 * cap + p]};
+    double vp[3] = {v[p], v[cap + p], v[2 * cap + p]};
+    const int s = spec[p];
+    int nb_out = b, cell_out = 0;
+    int c;
+    if constexpr (MOVER == AMPS_MOVER_RELATIVISTIC)
+      c = push_particle_rel(m, faces, b, g, sE, sB, E, B, sp.charge[s], sp.mass[s], sp.dt[s], bc_mode,
+                            r_sphere, xp, vp, &nb_out, &cell_out);
+    else
+      c = push_particle<MOVER, BC>(m, faces, g, lat, sE, sB, sp.charge[s], sp.mass[s], sp.dt[s], bc_mode,
+                                   r_sphere, xp, vp, &nb_out, &cell_out, Bc);
+    x[p] = xp[0];
+    x[cap + p] = xp[1];
+    x[2 * cap + p] = xp[2];
+    v[p] = vp[0];
+    v[cap + p] = vp[1];
+    v[2 * cap + p] = vp[2];
+    alive = alive_code(c);
+    knew = nb_out * NCELL + cell_out;
+    if (alive) key[p] = knew;
+    code[p] = (int8_t)c;
+    }
+    // migration count of the particles that stay: one atomic per run of
+    // equal keys (count is null when the caller counts separately)
+    if (count) {
+      const RunInfo r = wave_runs(alive, knew);
+      if (r.head) atomicAdd(&count[knew], r.len);
+    }
+  }
+}
+
+#define INST_PUSH64(MV, BCV, W)                                                                  \
+  template __global__ void k_push_f64<MV, BCV, W>(MeshView, Faces, Species, const double*, const double*, \
+                                                  const Tile*, const int*, int, double, int64_t, double*, \
+                                                  double*, const uint8_t*, int*, int8_t*, const double*, int*);
+INST_PUSH64(AMPS_MOVER_LAPENTA, false, 1)
+INST_PUSH64(AMPS_MOVER_BORIS, false, 1)
+INST_PUSH64(AMPS_MOVER_LAPENTA, false, 4)
+INST_PUSH64(AMPS_MOVER_BORIS, false, 4)
+INST_PUSH64(AMPS_MOVER_RELATIVISTIC, false, 1)
+INST_PUSH64(AMPS_MOVER_LAPENTA, true, 1)
+INST_PUSH64(AMPS_MOVER_BORIS, true, 1)
+
+// FP32 push: float E/B stencil in LDS (17.5 KB), float velocities, FP64
+// positions.
+template <int MOVER, int W>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W))) void k_push_f32(MeshView m, Faces faces, Species sp,
+                                                   const double* __restrict__ E,
+                                                   const double* __restrict__ B,
+                                                   const Tile* __restrict__ tiles,
+                                                   const int* __restrict__ ntiles, int bc_mode,
+                                                   double r_sphere, int64_t cap, double* x,
+                                                   float* v, const uint8_t* __restrict__ spec,
+                                                   int* key, int8_t* code, int* __restrict__ count) {
+  __shared__ float sE[NCORNER * 3];
+  __shared__ float sB[NCORNER * 3];
+  const int t = blockIdx.x;
+  if (t >= *ntiles) return;
+  const Tile tl = tiles[t];
+  const int b = tl.block;
+  const double* Eb = E + (size_t)b * NCORNER * 3;
+  const double* Bb = B + (size_t)b * NCORNER * 3;
+  for (int i = threadIdx.x; i < NCORNER * 3; i += blockDim.x) {
+    sE[i] = (float)Eb[i];
+    sB[i] = (float)Bb[i];
+  }
+  __syncthreads();
+  BlockGeom g;
+  load_block_geom(m, b, g);
+  BlockLat lat;
+  load_block_lat(m, b, lat);
+  const int p1 = tl.start + tl.count;
+  for (int p0 = tl.start; p0 < p1; p0 += blockDim.x) {
+    const int p = p0 + threadIdx.x;
+    bool alive = false;
+    int knew = -1;
+    if (p < p1) {
+    double xp[3] = {x[p], x[cap + p], x[2 * cap + p]};
+    float vp[3] = {v[p], v[cap + p], v[2 * cap + p]};
+    const int s = spec[p];
+    int nb_out = b, cell_out = 0;
+    const int c = push_particle_f32<MOVER>(m, faces, g, lat, sE, sB, (float)sp.charge[s], (float)sp.mass[s],
+                                           sp.dt[s], bc_mode, r_sphere, xp, vp, &nb_out, &cell_out);
+    x[p] = xp[0];
+    x[cap + p] = xp[1];
+    x[2 * cap + p] = xp[2];
+    v[p] = vp[0];
+    v[cap + p] = vp[1];
+    v[2 * cap + p] = vp[2];
+    alive = alive_code(c);
+    knew = nb_out * NCELL + cell_out;
+    if (alive) key[p] = knew;
+    code[p] = (int8_t)c;
+    }
+    if (count) {
+      const RunInfo r = wave_runs(alive, knew);
+      if (r.head) atomicAdd(&count[knew], r.len);
+    }
+  }
+}
+
+#define INST_PUSH32(MV, W)                                                                        \
+  template __global__ void k_push_f32<MV, W>(MeshView, Faces, Species, const double*, const double*, \
+                                             const Tile*, const int*, int, double, int64_t, double*,  \
+                                             float*, const uint8_t*, int*, int8_t*, int*);
+INST_PUSH32(AMPS_MOVER_LAPENTA, 1)
+INST_PUSH32(AMPS_MOVER_BORIS, 1)
+INST_PUSH32(AMPS_MOVER_LAPENTA, 4)
+INST_PUSH32(AMPS_MOVER_BORIS, 4)
+
+// ======================================================================
+// Migration: counting sort by key = block*512 + cell with wavefront
+// run-length aggregation (ballot + lane masks) so that one atomic serves a
+// whole run of equal keys; deleted particles are dropped.
+// ======================================================================
 __global__ __launch_bounds__(256) void k_fill_int(int* a, int64_t n, int val) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x)
@@ -554,11 +581,13 @@ __global__ __launch_bounds__(256) void k_deposit_node_staged(int64_t nnodes, int
       }
     }
     asm volatile("" ::: "memory");
+    // node records are not read again by this step: non-temporal stores keep
+    // them from displacing the cell records still being gathered
     double* o = nodeout + (size_t)B.node * NODE_STRIDE;
 #pragma unroll
     for (int r = 0; r < 4; r++) {
       const int val = lane + 64 * r;
-      o[val] = (val < NODE_VALS) ? acc[r] : 0.0;
+      __builtin_nontemporal_store((val < NODE_VALS) ? acc[r] : 0.0, &o[val]);
     }
   };
   if (first >= last) return;
