@@ -135,6 +135,60 @@ def cpu_baseline(target_s=12.0):
                       f"steps, {t_push + t_dep:.1f} s (push {t_push:.1f} s, deposit {t_dep:.1f} s)"}
 
 
+MLP_FWD_FLOP = 2 * (6 * 128 + 128 * 128 + 128)  # per sample, 6 -> 128 -> 128 -> 1
+BF16_PEAK_TFLOPS = 2500.0                          # MI355X dense BF16 matrix peak
+
+
+def mlp_bench(ctx, arch, args):
+    """The surrogate network alone: Adam training steps (batch 4096) and
+    batched inference on 2^20 synthetic samples, timed with the device
+    synchronised on both sides.  FLOPs count the network's multiply-adds
+    (forward 34.6 kFLOP per sample, training 3x that: forward, input and
+    weight gradients)."""
+    import numpy as np
+
+    from amps_amd import surrogate as SG
+
+    rng = np.random.default_rng(3)
+    n = 1 << args.train_samples_log2
+    lat, lon = rng.uniform(-90, 90, n), rng.uniform(-180, 180, n)
+    zen, az = rng.uniform(0, 60, n), rng.uniform(0, 360, n)
+    X = SG.features(lat, lon, zen, az)
+    T = (np.cos(np.radians(lat)) ** 4 * (1.0 + 0.1 * np.sin(np.radians(lon))) + 0.05).astype(np.float32)
+    sur = SG.Surrogate(ctx, seed=1)
+    sur.set_data(X, T)
+    batch = 4096
+    sur.train(max(args.warmup, 1) * 16, batch=batch)
+    ctx.sync()
+    nsteps = 256 * args.steps
+    t0 = time.perf_counter()
+    sur.train(nsteps, batch=batch)
+    ctx.sync()
+    t_train = time.perf_counter() - t0
+    sur.predict(X[:4096])
+    ctx.sync()
+    t0 = time.perf_counter()
+    reps = 4
+    for _ in range(reps):
+        sur.predict(X)
+    ctx.sync()
+    t_inf = (time.perf_counter() - t0) / reps
+    train_tflops = 3 * MLP_FWD_FLOP * batch * nsteps / t_train / 1e12
+    inf_tflops = MLP_FWD_FLOP * n / t_inf / 1e12
+    return {"metric": "surrogate MLP training samples/s (BF16 MFMA, FP32 accumulate, Adam)",
+            "value": batch * nsteps / t_train, "unit": "samples/s", "n_gpus": 1, "steps": nsteps,
+            "warmup": max(args.warmup, 1) * 16, "ms_per_step": 1e3 * t_train / nsteps, "higher_is_better": True,
+            "scaling": "strong", "vs_baseline": None, "dtype": "bf16", "data": "synthetic features and targets",
+            "config": {"workload": "MLP 6-128-128-1 GELU, batch 4096, 2^%d samples" % args.train_samples_log2,
+                       "device": arch},
+            "roofline": {"kernel": "k_mlp_train + k_mlp_adam (training step)", "bound": "mfma",
+                         "achieved": round(train_tflops, 3), "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
+                         "frac": round(train_tflops / BF16_PEAK_TFLOPS, 6), "traffic": None},
+            "inference": {"samples": n, "s": t_inf, "samples_per_s": n / t_inf, "TFLOPs": round(inf_tflops, 3),
+                          "frac_of_bf16_peak": round(inf_tflops / BF16_PEAK_TFLOPS, 6)},
+            "train_step_us": 1e6 * t_train / nsteps}
+
+
 def cutoff_map_bench(ctx, n_cu, n_train=1 << 20, epochs=20, cpu_check_points=16, tune_cache=None, n_batch=16):
     """Batched backward cutoff map (64x32 points x 8 directions x 48 bins)
     and the surrogate-bracketed map.  Returns a dict of timings and errors."""
@@ -280,7 +334,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--tune-cache", default=os.path.join(ROOT, "gpurun_out", "tuning.json"))
-    ap.add_argument("--workload", choices=["all", "pic", "map"], default="all")
+    ap.add_argument("--workload", choices=["all", "pic", "map", "mlp"], default="all")
     ap.add_argument("--train-samples-log2", type=int, default=20)
     ap.add_argument("--epochs", type=int, default=20)
     ap.add_argument("--tile", type=int, default=0, help="push tile (skips the tuner with the two below)")
@@ -322,6 +376,10 @@ def main():
                            "device": arch},
                 "cutoff_map": mp, "wall_s": time.perf_counter() - t0}
         print(json.dumps(line), flush=True)
+        ctx.close()
+        return
+    if args.workload == "mlp":
+        print(json.dumps(mlp_bench(ctx, arch, args)), flush=True)
         ctx.close()
         return
     mesh = M.three_level_mesh()
