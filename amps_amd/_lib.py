@@ -70,6 +70,7 @@ class TraceParams(C.Structure):
         ("box_max_Re", C.c_double * 3), ("r_outer_Re", C.c_double), ("r_inner_Re", C.c_double),
         ("earth_radius_m", C.c_double), ("moment_Am2", C.c_double), ("m_hat", C.c_double * 3),
         ("B_ext_T", C.c_double * 3), ("field_model", C.c_int32), ("dt_selector", C.c_int32),
+        ("policy", C.c_int32), ("pad2_", C.c_int32), ("event_tol_m", C.c_double),
     ]
 
 
@@ -83,6 +84,8 @@ TRACE_RESULT_DTYPE = np.dtype([("termination", np.int32), ("retry_count", np.int
                                ("steps", np.int64), ("time_s", np.float64),
                                ("distance_m", np.float64), ("x", np.float64, 3),
                                ("p", np.float64, 3)])
+TRACE_EXIT_DTYPE = np.dtype([("x_exit_m", np.float64, 3), ("v_exit_unit", np.float64, 3),
+                             ("cos_alpha", np.float64), ("valid", np.int32), ("pad_", np.int32)])
 
 _lib = None
 
@@ -135,6 +138,10 @@ def load():
         "amps_gpu_mode3d_field": (C.c_int, [vp, i64, P(dbl), P(dbl)]),
         "amps_gpu_trace_batch": (C.c_int, [vp, P(TraceParams), i32, i64, P(dbl), P(dbl), P(dbl),
                                            vp, P(i64)]),
+        "amps_gpu_trace_batch_ex": (C.c_int, [vp, P(TraceParams), i32, i64, P(dbl), P(dbl), P(dbl),
+                                              vp, vp, P(i64)]),
+        "amps_gpu_transmission": (C.c_int, [vp, P(TraceParams), i32, i64, P(dbl), i32, P(dbl), i32, P(dbl),
+                                            P(dbl), P(i32), P(i32), P(i32), P(i32), P(i64)]),
         "amps_gpu_cutoff_scan": (C.c_int, [vp, P(TraceParams), i32, i32, i64, P(dbl), P(dbl), i32,
                                            P(dbl), P(i32), P(i32), P(C.c_int8), P(i32), P(i64)]),
         "amps_gpu_refine_cutoff": (C.c_int, [vp, P(TraceParams), i32, i64, P(dbl), P(dbl), P(dbl), P(dbl),

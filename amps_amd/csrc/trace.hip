@@ -47,6 +47,9 @@ struct Fp64Lane {
   double x[3], p[3], t, s, dtTrace, maxTime;
   int64_t n, maxSteps;
   int retry;
+  // exit state of an allowed structured trace (x_exit, v_exit, cos alpha)
+  double ex[7];
+  int ex_valid;
 };
 
 // Dipole::GetB_Tesla (DipoleInterface.h:99-125) plus a uniform field
@@ -388,14 +391,177 @@ __device__ __forceinline__ double select_dt64(const TraceView& a, const double* 
   return dt;
 }
 
+// SelectStructuredTraceDt (CutoffRigidityGridless.cpp:1154-1228) with
+// FinalizeUpperBoundedStep (util/TrajectoryTimeStep.h:15-24): DT_TRACE capped
+// by the gyro-angle budget; no boundary limiter, no floor.
+__device__ __forceinline__ double finalize_upper_bounded(double dt, double trem) {
+  if (!(dt > 0.0) || !isfinite(dt)) return __builtin_nan("");
+  if (trem > 0.0) dt = dmin(dt, trem);
+  if (!(dt > 0.0) || !isfinite(dt)) return __builtin_nan("");
+  return dt;
+}
+template <int FM>
+__device__ __forceinline__ double select_dt_structured(const TraceView& a, const double* x, const double* p,
+                                                       double dtTrace, double trem, double* B, bool& have_B) {
+  const amps_trace_params& P = a.P;
+  double dt = dtTrace;
+  if (!P.adaptive) return finalize_upper_bounded(dt, trem);
+  const double m0 = P.m0_kg;
+  const double p2 = dot3(p, p);
+  const double mc = m0 * C_LIGHT;
+  const double gamma = sqrt(1.0 + p2 / (mc * mc));
+  field64<FM>(a, x, B);
+  have_B = true;
+  const double Bmag = sqrt(dot3(B, B));
+  if (Bmag > 0.0) {
+    const double omega = fabs(P.q_C) * Bmag / (gamma * m0);
+    if (omega > 0.0) dt = dmin(dt, P.dphi_max / omega);
+  }
+  return finalize_upper_bounded(dt, trem);
+}
+
+// TrajectoryBoundary::FindFirstEvent (util/TrajectoryBoundary.h:46-141): the
+// first inner-sphere (type 1) or outer-box (type 2) crossing on the chord.
+__device__ __forceinline__ void find_first_event(const double* a, const double* z, const double* bmin,
+                                                 const double* bmax, double rInner, double tol, int& type,
+                                                 double& frac, double* pos) {
+  type = 0;
+  frac = __builtin_inf();
+  if (!fin3(a) || !fin3(z)) return;
+  const double d[3] = {z[0] - a[0], z[1] - a[1], z[2] - a[2]};
+  if (rInner > 0.0) {
+    const double r0 = dmax(0.0, rInner + 0.0);
+    if (a[0] * a[0] + a[1] * a[1] + a[2] * a[2] <= r0 * r0) {
+      type = 1, frac = 0.0;
+      pos[0] = a[0], pos[1] = a[1], pos[2] = a[2];
+    } else {
+      const double aa = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+      if (aa > 0.0) {
+        const double segmentLength = sqrt(aa);
+        const double fractionTol = (segmentLength > 0.0) ? dmax(0.0, tol) / segmentLength : 0.0;
+        const double bb = 2.0 * (a[0] * d[0] + a[1] * d[1] + a[2] * d[2]);
+        const double cc = a[0] * a[0] + a[1] * a[1] + a[2] * a[2] - rInner * rInner;
+        double disc = bb * bb - 4.0 * aa * cc;
+        if (!(disc < 0.0)) {
+          disc = dmax(0.0, disc);
+          const double root = sqrt(disc);
+          const double cand[2] = {(-bb - root) / (2.0 * aa), (-bb + root) / (2.0 * aa)};
+#pragma unroll
+          for (int c = 0; c < 2; c++) {
+            if (cand[c] < -fractionTol || cand[c] > 1.0 + fractionTol) continue;
+            const double t = dmax(0.0, dmin(1.0, cand[c]));
+            if (t >= frac) continue;
+            type = 1, frac = t;
+#pragma unroll
+            for (int i = 0; i < 3; i++) pos[i] = a[i] + frac * d[i];
+          }
+        }
+      }
+    }
+  }
+  const bool ain = a[0] >= bmin[0] && a[0] <= bmax[0] && a[1] >= bmin[1] && a[1] <= bmax[1] && a[2] >= bmin[2] &&
+                   a[2] <= bmax[2];
+  if (!ain) {
+    if (0.0 < frac) {
+      type = 2, frac = 0.0;
+      pos[0] = a[0], pos[1] = a[1], pos[2] = a[2];
+    }
+    return;
+  }
+  const bool zin = z[0] >= bmin[0] && z[0] <= bmax[0] && z[1] >= bmin[1] && z[1] <= bmax[1] && z[2] >= bmin[2] &&
+                   z[2] <= bmax[2];
+  if (zin) return;
+  const double segmentLength = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+  const double fractionTol = (segmentLength > 0.0) ? dmax(0.0, tol) / segmentLength : 0.0;
+  for (int axis = 0; axis < 3; axis++) {
+    if (d[axis] == 0.0) continue;
+    const int side = (d[axis] > 0.0) ? 1 : 0;
+    const double plane = side ? bmax[axis] : bmin[axis];
+    const double cand = (plane - a[axis]) / d[axis];
+    if (cand < -fractionTol || cand > 1.0 + fractionTol) continue;
+    const double t = dmax(0.0, dmin(1.0, cand));
+    if (t >= frac) continue;
+    const double q[3] = {a[0] + t * d[0], a[1] + t * d[1], a[2] + t * d[2]};
+    bool onFace = true;
+    for (int j = 0; j < 3; j++) {
+      if (j == axis) continue;
+      if (q[j] < bmin[j] - tol || q[j] > bmax[j] + tol) {
+        onFace = false;
+        break;
+      }
+    }
+    if (!onFace) continue;
+    type = 2, frac = t;
+    pos[0] = q[0], pos[1] = q[1], pos[2] = q[2];
+  }
+}
+
+// PopulateOuterExit (CutoffRigidityGridless.cpp:1323-1358) into the lane:
+// the event position, the direction of the momentum interpolated to the
+// event, cos(pitch angle) against B a tolerance inside the box.  False for an
+// invalid field or momentum (InvalidField).
+template <int FM>
+__device__ __forceinline__ bool exit_state(const TraceView& a, Fp64Lane& L, const double* pos, double fraction,
+                                           const double* xb, const double* xa, const double* pb, const double* pa) {
+  const amps_trace_params& P = a.P;
+#pragma unroll
+  for (int d = 0; d < 3; d++) L.ex[d] = pos[d];
+  const double f = dmax(0.0, dmin(1.0, fraction));
+  const double pc[3] = {pb[0] + f * (pa[0] - pb[0]), pb[1] + f * (pa[1] - pb[1]), pb[2] + f * (pa[2] - pb[2])};
+  const double p2n = dot3(pc, pc);
+  const double mc = P.m0_kg * C_LIGHT;
+  const double g = sqrt(1.0 + p2n / (mc * mc));
+  if (!(g > 0.0) || !isfinite(g)) return false;
+  const double iv = 1.0 / (g * P.m0_kg);
+  double v[3] = {iv * pc[0], iv * pc[1], iv * pc[2]};
+  const double nv = sqrt(dot3(v, v));
+  if (nv > 0.0) {
+    const double in = 1.0 / nv;
+#pragma unroll
+    for (int d = 0; d < 3; d++) v[d] = in * v[d];
+  } else {
+    v[0] = v[1] = v[2] = 0.0;
+  }
+#pragma unroll
+  for (int d = 0; d < 3; d++) L.ex[3 + d] = v[d];
+  const double ch[3] = {xa[0] - xb[0], xa[1] - xb[1], xa[2] - xb[2]};
+  const double chl = sqrt(dot3(ch, ch));
+  double inset = 0.0;
+  if (chl > 0.0) inset = dmin(f, dmax(1.0, P.event_tol_m) / chl);
+  const double fe = dmax(0.0, f - inset);
+  const double xe[3] = {xb[0] + fe * ch[0], xb[1] + fe * ch[1], xb[2] + fe * ch[2]};
+  double B[3];
+  field64<FM>(a, xe, B);
+  const double Bn = sqrt(dot3(B, B));
+  if (!(Bn > 0.0) || !isfinite(Bn)) return false;
+  const double ib = 1.0 / Bn;
+  const double bh[3] = {ib * B[0], ib * B[1], ib * B[2]};
+  L.ex[6] = dot3(v, bh);
+  L.ex_valid = isfinite(L.ex[6]) ? 1 : 0;
+  return L.ex_valid != 0;
+}
+
 template <int MV, int FM>
 struct Fp64Stepper {
   using Lane = Fp64Lane;
+  static constexpr bool HAS_EXIT = true;
   static __device__ __forceinline__ void init(const TraceView& a, Lane& L, const double* x0,
                                               const double* dir, double R, int attempt) {
     const amps_trace_params& P = a.P;
     const double pMag = (R * 1.0e9 * P.qabs_C) / C_LIGHT;
-    const double v0[3] = {-1.0 * dir[0], -1.0 * dir[1], -1.0 * dir[2]};
+    double v0[3] = {-1.0 * dir[0], -1.0 * dir[1], -1.0 * dir[2]};
+    if (P.policy == 1) {
+      // TraceTrajectoryShared(Ex) (CutoffRigidityGridless.cpp:1563-1585)
+      // normalise the start direction
+      const double nv = sqrt(dot3(v0, v0));
+      if (nv > 0.0) {
+        const double in = 1.0 / nv;
+#pragma unroll
+        for (int d = 0; d < 3; d++) v0[d] = in * v0[d];
+      } else {
+        v0[0] = v0[1] = v0[2] = 0.0;
+      }
+    }
 #pragma unroll
     for (int d = 0; d < 3; d++) {
       L.p[d] = pMag * v0[d];
@@ -405,6 +571,7 @@ struct Fp64Stepper {
     L.s = 0.0;
     L.n = 0;
     L.retry = attempt;
+    L.ex_valid = 0;
     if (attempt == 0) {
       L.dtTrace = P.dt_trace_s;
       L.maxSteps = P.max_steps;
@@ -436,22 +603,38 @@ struct Fp64Stepper {
     else
       inside = x[0] >= a.box_min_m[0] - 0.0 && x[0] <= a.box_max_m[0] + 0.0 && x[1] >= a.box_min_m[1] - 0.0 &&
                x[1] <= a.box_max_m[1] + 0.0 && x[2] >= a.box_min_m[2] - 0.0 && x[2] <= a.box_max_m[2] + 0.0;
-    if (!inside) return 0;
+    if (!inside) {
+      // structured policy: an OuterBox event at the step start (fraction 0
+      // on a zero chord)
+      if (P.policy == 1 && a.exits) return exit_state<FM>(a, L, x, 0.0, x, x, p, p) ? 0 : 7;
+      return 0;
+    }
     // B at the start point: evaluated once by the adaptive step selector and
     // reused by the step (the same function of the same point)
     double Bx[3];
     bool have_B = false;
-    const double dt = (P.dt_selector == 1) ? select_dt3d<FM>(a, x, p, L.dtTrace, L.maxTime - L.t, Bx, have_B)
-                                           : select_dt64<FM>(a, x, p, L.dtTrace, L.maxTime - L.t, Bx, have_B);
+    double dt;
+    if (P.policy == 1)
+      dt = select_dt_structured<FM>(a, x, p, L.dtTrace, L.maxTime - L.t, Bx, have_B);
+    else if (P.dt_selector == 1)
+      dt = select_dt3d<FM>(a, x, p, L.dtTrace, L.maxTime - L.t, Bx, have_B);
+    else
+      dt = select_dt64<FM>(a, x, p, L.dtTrace, L.maxTime - L.t, Bx, have_B);
     if (!(dt > 0.0) || !isfinite(dt)) return 6;
     const double xb[3] = {x[0], x[1], x[2]};
+    const double pb[3] = {p[0], p[1], p[2]};
     if (!step_checked64<MV, FM>(a, x, p, dt, rIn, Bx, have_B)) return 1;
     if (!fin3(x) || !fin3(p)) return 8;
+    int etype = 0;
+    double efrac = 0.0, epos[3] = {0.0, 0.0, 0.0};
+    if (P.policy == 1) find_first_event(xb, x, a.box_min_m, a.box_max_m, rIn, P.event_tol_m, etype, efrac, epos);
     const double dd[3] = {x[0] - xb[0], x[1] - xb[1], x[2] - xb[2]};
     const double seg = sqrt(dot3(dd, dd));
     L.s += isfinite(seg) ? seg : 0.0;
     L.t += dt;
     ++L.n;
+    if (etype == 1) return 1;
+    if (etype == 2) return (!a.exits || exit_state<FM>(a, L, epos, efrac, xb, x, pb, p)) ? 0 : 7;
     return -1;
   }
   static __device__ __forceinline__ void result(const TraceView& a, const Lane& L, int term,
@@ -466,6 +649,17 @@ struct Fp64Stepper {
       r.x[d] = L.x[d];
       r.p[d] = L.p[d];
     }
+  }
+  static __device__ __forceinline__ void exit_out(const Lane& L, int term, amps_trace_exit& e) {
+    const bool v = term == 0 && L.ex_valid;
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+      e.x_exit_m[d] = v ? L.ex[d] : 0.0;
+      e.v_exit_unit[d] = v ? L.ex[3 + d] : 0.0;
+    }
+    e.cos_alpha = v ? L.ex[6] : 0.0;
+    e.valid = v ? 1 : 0;
+    e.pad_ = 0;
   }
 };
 
@@ -515,6 +709,7 @@ __device__ __forceinline__ void rotate32(float* u, const float* b, float tc) {
 template <int MV>
 struct Fp32Stepper {
   using Lane = Fp32Lane;
+  static constexpr bool HAS_EXIT = false;
   static __device__ __forceinline__ void init(const TraceView& a, Lane& L, const double* x0,
                                               const double* dir, double R, int attempt) {
     const amps_trace_params& P = a.P;
@@ -670,7 +865,9 @@ struct Fp32Stepper {
   }
 };
 
-__device__ __forceinline__ bool retryable(int t) { return t >= 6 && t <= 8; }
+// retry 1: numerical failures (TraceTrajectoryWithSingleRetry); retry 2:
+// every unresolved termination (TraceDensityDirection, DensityGridless.cpp:849-879)
+__device__ __forceinline__ bool retryable(int mode, int t) { return mode == 2 ? t > 2 : (t >= 6 && t <= 8); }
 __device__ __forceinline__ int8_t state_of(int term) {
   // 1 allowed; 0 cutoff-forbidden (physical or trace limit); 2 numerical failure
   return term == 0 ? 1 : (term <= 5 ? 0 : 2);
@@ -731,7 +928,7 @@ __global__ __launch_bounds__(256) void k_trace_persistent(TraceView a) {
     for (int k = 0; k < a.chunk && has; k++) {
       const int term = S::step(a, L);
       if (term < 0) continue;
-      if (a.P.retry && attempt == 0 && retryable(term)) {
+      if (a.P.retry && attempt == 0 && retryable(a.P.retry, term)) {
         attempt = 1;
         my_steps += L.n;
         if (a.mode == TRACE_MODE_LIST)
@@ -745,6 +942,13 @@ __global__ __launch_bounds__(256) void k_trace_persistent(TraceView a) {
         amps_trace_result r;
         S::result(a, L, term, r);
         a.out[item] = r;
+        if constexpr (S::HAS_EXIT) {
+          if (a.exits) {
+            amps_trace_exit e;
+            S::exit_out(L, term, e);
+            a.exits[item] = e;
+          }
+        }
         has = false;
       } else if (a.mode == TRACE_MODE_FULL) {
         a.states[(int64_t)loc * a.nR + bin] = state_of(term);
@@ -788,7 +992,8 @@ struct TraceState {
   int* d_cut = nullptr;
   int8_t* d_states = nullptr;
   amps_trace_result* d_out = nullptr;
-  size_t b_x0 = 0, b_dir = 0, b_R = 0, b_lo = 0, b_hi = 0, b_cut = 0, b_states = 0, b_out = 0;
+  amps_trace_exit* d_exits = nullptr;
+  size_t b_x0 = 0, b_dir = 0, b_R = 0, b_lo = 0, b_hi = 0, b_cut = 0, b_states = 0, b_out = 0, b_exits = 0;
   int grid_waves = 0;  // persistent grid size (workgroups); 0 = 8 per CU
   int chunk = 32;      // steps between queue refills
 };
@@ -808,6 +1013,7 @@ void trace_free(amps_ctx* c) {
   (void)hipFree(t->d_cut);
   (void)hipFree(t->d_states);
   (void)hipFree(t->d_out);
+  (void)hipFree(t->d_exits);
   delete t;
   c->trace_state = nullptr;
 }
@@ -840,6 +1046,10 @@ static int check_params(const amps_trace_params* P) {
   if (!(P->earth_radius_m > 0.0)) return fail(AMPS_E_ARG, "trace: earth_radius_m must be > 0");
   if (P->max_steps <= 0) return fail(AMPS_E_ARG, "trace: max_steps must be > 0");
   if (P->field_model < 0 || P->field_model > 1) return fail(AMPS_E_ARG, "trace: field_model must be 0 or 1");
+  if (P->retry < 0 || P->retry > 2) return fail(AMPS_E_ARG, "trace: retry must be 0, 1 or 2");
+  if (P->policy < 0 || P->policy > 1) return fail(AMPS_E_ARG, "trace: policy must be 0 (legacy) or 1 (structured)");
+  if (P->policy == 1 && (P->outer_sphere || P->dt_selector != 0))
+    return fail(AMPS_E_ARG, "trace: the structured policy runs with the box boundary and its own step selector");
   if (P->dt_selector < 0 || P->dt_selector > 1) return fail(AMPS_E_ARG, "trace: dt_selector must be 0 or 1");
   if (P->mover < MV_BORIS || P->mover > MV_RK6)
     return fail(AMPS_E_ARG, "trace: mover must be 0 BORIS, 1 HC4, 2 RK2, 3 RK4 or 4 RK6 "
@@ -892,8 +1102,8 @@ static void launch64(amps_ctx* c, const TraceView& a, dim3 g, dim3 blk) {
 static int launch(amps_ctx* c, TraceState* t, TraceView& a, int precision) {
   if (precision == AMPS_FP32 && a.P.mover != MV_BORIS && a.P.mover != MV_HC4)
     return fail(AMPS_E_ARG, "trace: the FP32 path runs the BORIS and HC4 movers only");
-  if (precision == AMPS_FP32 && (a.P.field_model != 0 || a.P.dt_selector != 0))
-    return fail(AMPS_E_ARG, "trace: the FP32 path runs the analytic field and the gridless step selector only");
+  if (precision == AMPS_FP32 && (a.P.field_model != 0 || a.P.dt_selector != 0 || a.P.policy != 0))
+    return fail(AMPS_E_ARG, "trace: the FP32 path runs the analytic field and the legacy gridless policy only");
   if (a.P.field_model == 1) {
     if (!c->has_mode3d) return fail(AMPS_E_STATE, "trace: field_model 1 needs amps_gpu_set_mode3d_field");
     a.mesh = c->mv;
@@ -941,10 +1151,9 @@ extern "C" int amps_gpu_trace_set_param(amps_ctx* c, const char* name, int64_t v
   return AMPS_OK;
 }
 
-extern "C" int amps_gpu_trace_batch(amps_ctx* c, const amps_trace_params* P, int32_t precision,
-                                    int64_t n, const double* x0, const double* dir,
-                                    const double* R_GV, amps_trace_result* out,
-                                    int64_t* steps_total) {
+static int trace_list(amps_ctx* c, const amps_trace_params* P, int32_t precision, int64_t n, const double* x0,
+                      const double* dir, const double* R_GV, amps_trace_result* out, amps_trace_exit* exits,
+                      int64_t* steps_total) {
   if (!c) return fail(AMPS_E_ARG, "null context");
   if (int r = check_params(P)) return r;
   if (precision != AMPS_FP64 && precision != AMPS_FP32) return fail(AMPS_E_ARG, "trace: bad precision");
@@ -958,6 +1167,8 @@ extern "C" int amps_gpu_trace_batch(amps_ctx* c, const amps_trace_params* P, int
   if (int r = ensure((void**)&t->d_dir, &t->b_dir, sizeof(double) * 3 * n)) return r;
   if (int r = ensure((void**)&t->d_R, &t->b_R, sizeof(double) * n)) return r;
   if (int r = ensure((void**)&t->d_out, &t->b_out, sizeof(amps_trace_result) * n)) return r;
+  if (exits)
+    if (int r = ensure((void**)&t->d_exits, &t->b_exits, sizeof(amps_trace_exit) * n)) return r;
   HIP_TRY(hipMemcpyAsync(t->d_x0, x0, sizeof(double) * 3 * n, hipMemcpyHostToDevice, c->stream));
   HIP_TRY(hipMemcpyAsync(t->d_dir, dir, sizeof(double) * 3 * n, hipMemcpyHostToDevice, c->stream));
   HIP_TRY(hipMemcpyAsync(t->d_R, R_GV, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
@@ -970,12 +1181,79 @@ extern "C" int amps_gpu_trace_batch(amps_ctx* c, const amps_trace_params* P, int
   a.dir = t->d_dir;
   a.R = t->d_R;
   a.out = t->d_out;
+  a.exits = exits ? t->d_exits : nullptr;
   if (int r = launch(c, t, a, precision)) return r;
   HIP_TRY(hipMemcpyAsync(out, t->d_out, sizeof(amps_trace_result) * n, hipMemcpyDeviceToHost, c->stream));
+  if (exits)
+    HIP_TRY(hipMemcpyAsync(exits, t->d_exits, sizeof(amps_trace_exit) * n, hipMemcpyDeviceToHost, c->stream));
   unsigned long long st = 0;
   HIP_TRY(hipMemcpyAsync(&st, t->d_steps, 8, hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
   if (steps_total) *steps_total = (int64_t)st;
+  return AMPS_OK;
+}
+
+extern "C" int amps_gpu_trace_batch(amps_ctx* c, const amps_trace_params* P, int32_t precision,
+                                    int64_t n, const double* x0, const double* dir,
+                                    const double* R_GV, amps_trace_result* out,
+                                    int64_t* steps_total) {
+  return trace_list(c, P, precision, n, x0, dir, R_GV, out, nullptr, steps_total);
+}
+
+// TraceTrajectorySharedEx batched: the exit states of the allowed
+// structured traces come back with the results.
+extern "C" int amps_gpu_trace_batch_ex(amps_ctx* c, const amps_trace_params* P, int32_t precision,
+                                       int64_t n, const double* x0, const double* dir,
+                                       const double* R_GV, amps_trace_result* out, amps_trace_exit* exits,
+                                       int64_t* steps_total) {
+  if (!exits && n > 0) return fail(AMPS_E_ARG, "trace_batch_ex: null exits");
+  return trace_list(c, P, precision, n, x0, dir, R_GV, out, exits, steps_total);
+}
+
+// ComputeWeightBlockAtEnergy's isotropic branch (DensityGridless.cpp:
+// 880-912) for every (point, rigidity) at once: one launch traces all
+// (point, rigidity, direction) tasks, the per-(point, rigidity) counts are
+// then summed on the host in direction order.
+extern "C" int amps_gpu_transmission(amps_ctx* c, const amps_trace_params* P, int32_t precision, int64_t npts,
+                                     const double* x0, int32_t nR, const double* R_GV, int32_t ndir,
+                                     const double* dirs, double* weight_sum, int32_t* sampled, int32_t* resolved,
+                                     int32_t* retried, int32_t* term_counts, int64_t* steps_total) {
+  if (!c) return fail(AMPS_E_ARG, "null context");
+  if (npts < 0 || nR < 0 || ndir < 0) return fail(AMPS_E_ARG, "transmission: negative sizes");
+  const int64_t n = npts * (int64_t)nR * (int64_t)ndir;
+  if (n > ((int64_t)1 << 31)) return fail(AMPS_E_RANGE, "transmission: more than 2^31 trajectories");
+  if (n == 0) return AMPS_OK;
+  if (!x0 || !R_GV || !dirs) return fail(AMPS_E_ARG, "transmission: null array");
+  std::vector<double> tx((size_t)n * 3), td((size_t)n * 3), tr((size_t)n);
+  for (int64_t p = 0; p < npts; p++)
+    for (int32_t e = 0; e < nR; e++)
+      for (int32_t k = 0; k < ndir; k++) {
+        const size_t i = (size_t)((p * nR + e) * ndir + k);
+        for (int d = 0; d < 3; d++) tx[3 * i + d] = x0[3 * p + d], td[3 * i + d] = dirs[3 * k + d];
+        tr[i] = R_GV[e];
+      }
+  std::vector<amps_trace_result> out((size_t)n);
+  if (int r = trace_list(c, P, precision, n, tx.data(), td.data(), tr.data(), out.data(), nullptr, steps_total))
+    return r;
+  for (int64_t g = 0; g < npts * nR; g++) {
+    double ws = 0.0;
+    int32_t ns = 0, nres = 0, nret = 0, tc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (int32_t k = 0; k < ndir; k++) {
+      const amps_trace_result& r = out[(size_t)(g * ndir + k)];
+      ++ns;
+      if (r.retry_count > 0) ++nret;
+      if (r.termination >= 0 && r.termination < 9) ++tc[r.termination];
+      if (r.termination > 2) continue;  // unresolved
+      ++nres;
+      if (r.termination == 0) ws += 1.0;
+    }
+    if (weight_sum) weight_sum[g] = ws;
+    if (sampled) sampled[g] = ns;
+    if (resolved) resolved[g] = nres;
+    if (retried) retried[g] = nret;
+    if (term_counts)
+      for (int t2 = 0; t2 < 9; t2++) term_counts[9 * g + t2] = tc[t2];
+  }
   return AMPS_OK;
 }
 

@@ -27,6 +27,7 @@ struct TraceView {
   const int* lo;      // per-location bracket (may be null: full range)
   const int* hi;
   amps_trace_result* out;
+  amps_trace_exit* exits;  // LIST mode exit states (may be null)
   int8_t* states;     // [nloc][nR]: -1 untraced, 0 forbidden, 1 allowed, 2 failed
   int* cutoff;        // UPPER mode raw result
   // mesh field (field_model 1): the block mesh and the compact interior-cell B

@@ -53,7 +53,8 @@ def make_params(*, tilt_deg=0.0, moment_scale=1.0, bz_T=0.0, mass_amu=PROTON_AMU
                 reversed_charge=False, dt_trace=1.0, max_time=600.0, max_steps=300000,
                 max_distance_Re=400.0, dphi=0.15, adaptive=True, dist_limiter=True, dt_floor=True,
                 outer_sphere=False, box_Re=35.0, r_outer_Re=35.0, r_inner_Re=1.0,
-                earth_radius_m=RE_AMPS, retry=True, mover="BORIS", field_model=0, dt_selector=0):
+                earth_radius_m=RE_AMPS, retry=True, mover="BORIS", field_model=0, dt_selector=0,
+                policy=0, event_tol_m=1.0):
     P = L.TraceParams()
     if mover not in MOVERS:
         raise ValueError("mover must be one of %s (guiding-centre and hybrid movers are not built)"
@@ -61,6 +62,8 @@ def make_params(*, tilt_deg=0.0, moment_scale=1.0, bz_T=0.0, mass_amu=PROTON_AMU
     P.mover = MOVERS[mover]
     P.field_model = int(field_model)
     P.dt_selector = int(dt_selector)
+    P.policy = int(policy)
+    P.event_tol_m = float(event_tol_m)
     q = charge_e * ELECTRON_CHARGE
     P.q_C = -q if reversed_charge else q
     P.qabs_C = abs(q)
@@ -74,7 +77,7 @@ def make_params(*, tilt_deg=0.0, moment_scale=1.0, bz_T=0.0, mass_amu=PROTON_AMU
     P.dist_limiter = int(dist_limiter)
     P.dt_floor = int(dt_floor)
     P.outer_sphere = int(outer_sphere)
-    P.retry = int(retry)
+    P.retry = int(retry)  # True/1: numerical failures; 2: every unresolved trace
     for d in range(3):
         P.box_min_Re[d] = -box_Re
         P.box_max_Re[d] = box_Re
@@ -104,6 +107,54 @@ def config2_params():
                        dphi=2.0 * math.pi * 0.01, dist_limiter=False, dt_floor=False,
                        outer_sphere=True, r_outer_Re=25.0, r_inner_Re=1.0,
                        earth_radius_m=RE_DIPOLE)
+
+
+def density_params(retry_unresolved=False, **kw):
+    """TraceTrajectoryShared(Ex) as the density / spectrum solver calls it
+    (TraceDensityDirection, DensityGridless.cpp:849-879): the StructuredAccurate
+    policy (gyro budget 0.15 rad, no boundary limiter or floor, exact chord
+    events with boundaryEventTolerance 1 m), no retry unless
+    DS_RETRY_UNRESOLVED, which retries every unresolved trace once."""
+    kw.setdefault("dphi", 0.15)
+    return make_params(policy=1, retry=2 if retry_unresolved else 0, **kw)
+
+
+def trace_batch_ex(ctx, params, x0, dirs, R, precision=L.FP64):
+    """trace_batch with the exit states of the allowed structured traces."""
+    x0 = np.ascontiguousarray(x0, np.float64).reshape(-1, 3)
+    dirs = np.ascontiguousarray(dirs, np.float64).reshape(-1, 3)
+    R = np.ascontiguousarray(R, np.float64)
+    out = np.zeros(len(R), L.TRACE_RESULT_DTYPE)
+    ex = np.zeros(len(R), L.TRACE_EXIT_DTYPE)
+    steps = C.c_int64()
+    L.check(ctx.lib.amps_gpu_trace_batch_ex(ctx.h, C.byref(params), precision, len(R),
+                                            L._ptr(x0, C.c_double), L._ptr(dirs, C.c_double),
+                                            L._ptr(R, C.c_double), out.ctypes.data_as(C.c_void_p),
+                                            ex.ctypes.data_as(C.c_void_p), C.byref(steps)), "trace_batch_ex")
+    return out, ex, steps.value
+
+
+def transmission(ctx, params, x0, R_GV, dirs, precision=L.FP64):
+    """Isotropic transmission T = weight_sum / resolved per (point, rigidity)
+    over the arrival directions dirs (ComputeT_atEnergy, DensityGridless.cpp:
+    915-925).  Returns a dict of [npts, nR] arrays and the step count."""
+    x0 = np.ascontiguousarray(x0, np.float64).reshape(-1, 3)
+    R = np.ascontiguousarray(R_GV, np.float64).ravel()
+    dirs = np.ascontiguousarray(dirs, np.float64).reshape(-1, 3)
+    g = (len(x0), len(R))
+    ws = np.zeros(g)
+    ns, nres, nret = (np.zeros(g, np.int32) for _ in range(3))
+    tc = np.zeros(g + (9,), np.int32)
+    steps = C.c_int64()
+    L.check(ctx.lib.amps_gpu_transmission(ctx.h, C.byref(params), precision, len(x0), L._ptr(x0, C.c_double),
+                                          len(R), L._ptr(R, C.c_double), len(dirs), L._ptr(dirs, C.c_double),
+                                          L._ptr(ws, C.c_double), L._ptr(ns, C.c_int32), L._ptr(nres, C.c_int32),
+                                          L._ptr(nret, C.c_int32), L._ptr(tc, C.c_int32), C.byref(steps)),
+            "transmission")
+    with np.errstate(invalid="ignore", divide="ignore"):
+        T = np.where(nres > 0, ws / np.maximum(nres, 1), 0.0)
+    return {"T": T, "weight_sum": ws, "sampled": ns, "resolved": nres, "retried": nret,
+            "term_counts": tc, "steps": steps.value}
 
 
 def mode3d_params(**kw):

@@ -247,7 +247,9 @@ typedef struct {
   int32_t dist_limiter;  /* 0.20 * d_near / v */
   int32_t dt_floor;      /* legacy max(1e-12, 1e-9*DT, 100 km / v) floor */
   int32_t outer_sphere;  /* 0: box, 1: sphere of radius r_outer_Re */
-  int32_t retry;         /* single retry on numerical failure */
+  int32_t retry;         /* 0 none; 1 single retry on a numerical failure
+                            (TraceTrajectoryWithSingleRetry); 2 single retry of
+                            every unresolved trace (DS_RETRY_UNRESOLVED) */
   int32_t mover;         /* MoverType (GridlessParticleMovers.h:300-310): 0 BORIS,
                             1 HC4, 2 RK2, 3 RK4, 4 RK6; GC2..HYBRID rejected */
   double box_min_Re[3], box_max_Re[3];
@@ -262,7 +264,25 @@ typedef struct {
                             form SelectLegacyCutoffTraceDt3D (CutoffRigidityMode3D.cpp:
                             881-936: box and inner distances in metres, limiter
                             and floor always on) */
+  int32_t policy;        /* 0: the legacy cutoff policy (boundaries tested at the
+                            step start); 1: StructuredAccurate, the density /
+                            transmission policy of TraceTrajectoryShared(Ex)
+                            (CutoffRigidityGridless.cpp:1154-1228, 1403-1440,
+                            1563-1585): SelectStructuredTraceDt, the first
+                            boundary event on every accepted chord
+                            (util/TrajectoryBoundary.h:133-141), a normalised
+                            start direction; box boundary, FP64 only */
+  int32_t pad2_;
+  double event_tol_m;    /* boundaryEventTolerance_m (default 1 m) */
 } amps_trace_params;
+
+/* TrajectoryExitState (CutoffRigidityGridless.h:258-269) of an allowed
+ * structured trace (PopulateOuterExit, CutoffRigidityGridless.cpp:1323-1358);
+ * valid = 0 for every other trajectory. */
+typedef struct {
+  double x_exit_m[3], v_exit_unit[3], cos_alpha;
+  int32_t valid, pad_;
+} amps_trace_exit;
 
 /* TrajectoryResult (termination values of util/TrajectoryTermination.h:10-21) */
 typedef struct {
@@ -281,6 +301,26 @@ typedef struct {
 int amps_gpu_trace_batch(amps_ctx* ctx, const amps_trace_params* params, int32_t precision,
                          int64_t n, const double* x0, const double* dir, const double* R_GV,
                          amps_trace_result* out, int64_t* steps_total);
+
+/* amps_gpu_trace_batch with the exit states (TraceTrajectorySharedEx,
+ * CutoffRigidityGridless.cpp:1575-1585) of the allowed structured traces in
+ * exits[n]. */
+int amps_gpu_trace_batch_ex(amps_ctx* ctx, const amps_trace_params* params, int32_t precision,
+                            int64_t n, const double* x0, const double* dir, const double* R_GV,
+                            amps_trace_result* out, amps_trace_exit* exits, int64_t* steps_total);
+
+/* Isotropic transmission of the density / spectrum solver
+ * (ComputeWeightBlockAtEnergy / ComputeT_atEnergy, DensityGridless.cpp:
+ * 880-925, through TraceDensityDirection :849-879): for every point x0[3p] and
+ * rigidity R_GV[e], all ndir arrival directions dirs[3 ndir] are traced (one
+ * batched launch) and, per (p, e), sampled, resolved (allowed or physically
+ * forbidden), retried, weight_sum (allowed count) and the termination counts
+ * term_counts[9 (p nR + e) + t] are reported; T = weight_sum / resolved.  Any
+ * output array may be NULL. */
+int amps_gpu_transmission(amps_ctx* ctx, const amps_trace_params* params, int32_t precision, int64_t npts,
+                          const double* x0, int32_t nR, const double* R_GV, int32_t ndir, const double* dirs,
+                          double* weight_sum, int32_t* sampled, int32_t* resolved, int32_t* retried,
+                          int32_t* term_counts, int64_t* steps_total);
 
 #define AMPS_SCAN_FULL 0   /* trace every bin of the bracket */
 #define AMPS_SCAN_UPPER 1  /* downward from the bracket top to the first forbidden bin */

@@ -1148,23 +1148,183 @@ static double select_dt3d(const orc_trace_params* P, const double x[3], const do
 
 static int isfin3(const double a[3]) { return isfinite(a[0]) && isfinite(a[1]) && isfinite(a[2]); }
 
+/* FinalizeUpperBoundedStep — util/TrajectoryTimeStep.h:15-24 */
+double orc_finalize_step(double dt, double timeRemaining) {
+  if (!(dt > 0.0) || !isfinite(dt)) return NAN;
+  if (timeRemaining > 0.0) dt = dmin(dt, timeRemaining);
+  if (!(dt > 0.0) || !isfinite(dt)) return NAN;
+  return dt;
+}
+
+/* SelectStructuredTraceDt — CutoffRigidityGridless.cpp:1154-1228: DT_TRACE
+ * capped by the gyro-angle budget, no boundary limiter, no floor. */
+static double select_dt_structured(const orc_trace_params* P, const double x[3], const double p[3],
+                                   double dtTrace, double timeRemaining) {
+  double dt = dtTrace;
+  if (!P->adaptive) return orc_finalize_step(dt, timeRemaining);
+  double m0 = P->m0_kg;
+  double p2 = dot3(p, p);
+  double mc = m0 * ORC_C;
+  double gamma = sqrt(1.0 + p2 / (mc * mc));
+  double B[3];
+  trace_field(P, x, B);
+  double Bmag = sqrt(dot3(B, B));
+  if (Bmag > 0.0) {
+    double omega = fabs(P->q_C) * Bmag / (gamma * m0);
+    if (omega > 0.0) dt = dmin(dt, P->dphi_max / omega);
+  }
+  return orc_finalize_step(dt, timeRemaining);
+}
+
+/* TrajectoryBoundary::FindFirstEvent — util/TrajectoryBoundary.h:46-141:
+ * the first inner-sphere or outer-box crossing on the chord start -> end
+ * (type 0 none, 1 inner sphere, 2 outer box). */
+void orc_find_first_event(const double a[3], const double z[3], const double bmin[3], const double bmax[3],
+                          double rInner, double tol, int* type, double* fraction, double pos[3], int* face) {
+  int bt = 0, bface = -1;
+  double bf = INFINITY, bp[3] = {0.0, 0.0, 0.0};
+  if (isfin3(a) && isfin3(z)) {
+    const double d[3] = {z[0] - a[0], z[1] - a[1], z[2] - a[2]};
+    /* ConsiderInnerSphere :50-85 */
+    if (rInner > 0.0) {
+      const double r0 = dmax(0.0, rInner + 0.0);
+      if (a[0] * a[0] + a[1] * a[1] + a[2] * a[2] <= r0 * r0) {
+        bt = 1, bf = 0.0, bface = -1;
+        bp[0] = a[0], bp[1] = a[1], bp[2] = a[2];
+      } else {
+        const double aa = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+        if (aa > 0.0) {
+          const double segmentLength = sqrt(aa);
+          const double fractionTol = (segmentLength > 0.0) ? dmax(0.0, tol) / segmentLength : 0.0;
+          const double bb = 2.0 * (a[0] * d[0] + a[1] * d[1] + a[2] * d[2]);
+          const double cc = a[0] * a[0] + a[1] * a[1] + a[2] * a[2] - rInner * rInner;
+          double disc = bb * bb - 4.0 * aa * cc;
+          if (!(disc < 0.0)) {
+            disc = dmax(0.0, disc);
+            const double root = sqrt(disc);
+            const double cand[2] = {(-bb - root) / (2.0 * aa), (-bb + root) / (2.0 * aa)};
+            for (int c = 0; c < 2; c++) {
+              if (cand[c] < -fractionTol || cand[c] > 1.0 + fractionTol) continue;
+              const double t = dmax(0.0, dmin(1.0, cand[c]));
+              if (t >= bf) continue;
+              bt = 1, bf = t, bface = -1;
+              for (int i = 0; i < 3; i++) bp[i] = a[i] + bf * d[i];
+            }
+          }
+        }
+      }
+    }
+    /* ConsiderOuterBox :87-131 */
+    const int ain = a[0] >= bmin[0] && a[0] <= bmax[0] && a[1] >= bmin[1] && a[1] <= bmax[1] &&
+                    a[2] >= bmin[2] && a[2] <= bmax[2];
+    if (!ain) {
+      if (0.0 < bf) {
+        bt = 2, bf = 0.0, bface = -1;
+        bp[0] = a[0], bp[1] = a[1], bp[2] = a[2];
+      }
+    } else {
+      const int zin = z[0] >= bmin[0] && z[0] <= bmax[0] && z[1] >= bmin[1] && z[1] <= bmax[1] &&
+                      z[2] >= bmin[2] && z[2] <= bmax[2];
+      if (!zin) {
+        const double segmentLength = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+        const double fractionTol = (segmentLength > 0.0) ? dmax(0.0, tol) / segmentLength : 0.0;
+        for (int axis = 0; axis < 3; axis++) {
+          if (d[axis] == 0.0) continue;
+          const int side = (d[axis] > 0.0) ? 1 : 0;
+          const double plane = side ? bmax[axis] : bmin[axis];
+          const double cand = (plane - a[axis]) / d[axis];
+          if (cand < -fractionTol || cand > 1.0 + fractionTol) continue;
+          const double t = dmax(0.0, dmin(1.0, cand));
+          if (t >= bf) continue;
+          const double q[3] = {a[0] + t * d[0], a[1] + t * d[1], a[2] + t * d[2]};
+          int onFace = 1;
+          for (int j = 0; j < 3; j++) {
+            if (j == axis) continue;
+            if (q[j] < bmin[j] - tol || q[j] > bmax[j] + tol) {
+              onFace = 0;
+              break;
+            }
+          }
+          if (!onFace) continue;
+          bt = 2, bf = t, bface = 2 * axis + side;
+          bp[0] = q[0], bp[1] = q[1], bp[2] = q[2];
+        }
+      }
+    }
+  }
+  *type = bt, *fraction = bf, *face = bface;
+  pos[0] = bp[0], pos[1] = bp[1], pos[2] = bp[2];
+}
+
+/* PopulateOuterExit — CutoffRigidityGridless.cpp:1323-1358: position of the
+ * event, velocity direction from the momentum interpolated to the event
+ * fraction, cos(pitch angle) against B a tolerance inside the box. */
+static int populate_exit(const orc_trace_params* P, const double pos[3], double fraction, const double xb[3],
+                         const double xa[3], const double pb[3], const double pa[3], orc_trace_exit* ex) {
+  for (int d = 0; d < 3; d++) ex->x_exit_m[d] = pos[d];
+  const double a = dmax(0.0, dmin(1.0, fraction));
+  const double pc[3] = {pb[0] + a * (pa[0] - pb[0]), pb[1] + a * (pa[1] - pb[1]), pb[2] + a * (pa[2] - pb[2])};
+  const double p2n = dot3(pc, pc);
+  const double mc = P->m0_kg * ORC_C;
+  const double g = sqrt(1.0 + p2n / (mc * mc));
+  if (!(g > 0.0) || !isfinite(g)) return 0;
+  const double iv = 1.0 / (g * P->m0_kg);
+  double v[3] = {iv * pc[0], iv * pc[1], iv * pc[2]};
+  const double n = sqrt(dot3(v, v));
+  if (n > 0.0) {
+    const double in = 1.0 / n;
+    for (int d = 0; d < 3; d++) v[d] = in * v[d];
+  } else {
+    v[0] = v[1] = v[2] = 0.0;
+  }
+  for (int d = 0; d < 3; d++) ex->v_exit_unit[d] = v[d];
+  const double ch[3] = {xa[0] - xb[0], xa[1] - xb[1], xa[2] - xb[2]};
+  const double chl = sqrt(dot3(ch, ch));
+  double inset = 0.0;
+  if (chl > 0.0) inset = dmin(a, dmax(1.0, P->event_tol_m) / chl);
+  const double f = dmax(0.0, a - inset);
+  const double xe[3] = {xb[0] + f * ch[0], xb[1] + f * ch[1], xb[2] + f * ch[2]};
+  double B[3];
+  trace_field(P, xe, B);
+  const double Bn = sqrt(dot3(B, B));
+  if (!(Bn > 0.0) || !isfinite(Bn)) return 0;
+  const double ib = 1.0 / Bn;
+  const double bh[3] = {ib * B[0], ib * B[1], ib * B[2]};
+  ex->cos_alpha = dot3(v, bh);
+  return isfinite(ex->cos_alpha);
+}
+
 /* TraceTrajectoryImpl — CutoffRigidityGridless.cpp:1246-1457 (full-orbit
- * mover P->mover, legacy policy, trap detector off).  Termination values follow
- * util/TrajectoryTermination.h:10-21. */
+ * mover P->mover, trap detector off).  policy 0: the legacy cutoff policy
+ * (SelectLegacyCutoffTraceDt, boundaries checked at the step start);
+ * policy 1: StructuredAccurate (SelectStructuredTraceDt and the first
+ * boundary event on each accepted chord, with the exit state when ex is
+ * given).  Termination values follow util/TrajectoryTermination.h:10-21. */
 static void trace_impl(const orc_trace_params* P, const double x0[3], const double dir[3],
                        double R_GV, double dtTrace, int64_t maxSteps, double maxTime,
-                       orc_trace_result* res) {
+                       orc_trace_result* res, orc_trace_exit* ex) {
   double Re = P->earth_radius_m;
   double boxmin[3], boxmax[3];
   for (int d = 0; d < 3; d++) boxmin[d] = P->box_min_Re[d] * Re, boxmax[d] = P->box_max_Re[d] * Re;
   double rInner = P->r_inner_Re * Re, rOuter = P->r_outer_Re * Re;
   double pMag = (R_GV * 1.0e9 * P->qabs_C) / ORC_C;
   double v0[3] = {-1.0 * dir[0], -1.0 * dir[1], -1.0 * dir[2]};
+  if (P->policy == 1) {
+    /* TraceTrajectoryShared(Ex) (:1563-1585) normalise the start direction */
+    const double n = sqrt(dot3(v0, v0));
+    if (n > 0.0) {
+      const double in = 1.0 / n;
+      for (int d = 0; d < 3; d++) v0[d] = in * v0[d];
+    } else {
+      v0[0] = v0[1] = v0[2] = 0.0;
+    }
+  }
   double p[3] = {pMag * v0[0], pMag * v0[1], pMag * v0[2]};
   double x[3] = {x0[0], x0[1], x0[2]};
   double t = 0.0, s = 0.0;
   int64_t nSteps = 0;
   int term = -1;
+  if (ex) memset(ex, 0, sizeof(*ex));
   while (nSteps < maxSteps && t < maxTime && (P->max_distance_m <= 0.0 || s < P->max_distance_m)) {
     if (!isfin3(x) || !isfin3(p)) { term = 8; break; }
     if (rInner > 0.0 && x[0] * x[0] + x[1] * x[1] + x[2] * x[2] <= dmax(0.0, rInner + 0.0) * dmax(0.0, rInner + 0.0)) {
@@ -1176,18 +1336,41 @@ static void trace_impl(const orc_trace_params* P, const double x0[3], const doub
     else
       inside = x[0] >= boxmin[0] - 0.0 && x[0] <= boxmax[0] + 0.0 && x[1] >= boxmin[1] - 0.0 &&
                x[1] <= boxmax[1] + 0.0 && x[2] >= boxmin[2] - 0.0 && x[2] <= boxmax[2] + 0.0;
-    if (!inside) { term = 0; break; }
-    double dt = P->dt_selector == 1 ? select_dt3d(P, x, p, dtTrace, maxTime - t, boxmin, boxmax, rInner)
-                                    : select_dt(P, x, p, dtTrace, maxTime - t);
+    if (!inside) {
+      term = 0;
+      if (P->policy == 1 && ex) {
+        /* an OuterBox event at the step start: fraction 0 on a zero chord */
+        if (!populate_exit(P, x, 0.0, x, x, p, p, ex)) term = 7;
+        else ex->valid = 1;
+      }
+      break;
+    }
+    double dt;
+    if (P->policy == 1) dt = select_dt_structured(P, x, p, dtTrace, maxTime - t);
+    else if (P->dt_selector == 1) dt = select_dt3d(P, x, p, dtTrace, maxTime - t, boxmin, boxmax, rInner);
+    else dt = select_dt(P, x, p, dtTrace, maxTime - t);
     if (!(dt > 0.0) || !isfinite(dt)) { term = 6; break; }
     double xb[3] = {x[0], x[1], x[2]};
+    double pb[3] = {p[0], p[1], p[2]};
     if (!orc_step_checked(P, x, p, dt, rInner)) { term = 1; break; }
     if (!isfin3(x) || !isfin3(p)) { term = 8; break; }
+    int etype = 0, eface = -1;
+    double efrac = 0.0, epos[3] = {0.0, 0.0, 0.0};
+    if (P->policy == 1) orc_find_first_event(xb, x, boxmin, boxmax, rInner, P->event_tol_m, &etype, &efrac, epos, &eface);
     double dd[3] = {x[0] - xb[0], x[1] - xb[1], x[2] - xb[2]};
     double seg = sqrt(dot3(dd, dd));
     s += isfinite(seg) ? seg : 0.0;
     t += dt;
     ++nSteps;
+    if (etype == 1) { term = 1; break; }
+    if (etype == 2) {
+      term = 0;
+      if (ex) {
+        if (!populate_exit(P, epos, efrac, xb, x, pb, p, ex)) term = 7;
+        else ex->valid = 1;
+      }
+      break;
+    }
   }
   if (term < 0) {
     if (nSteps >= maxSteps) term = 4;
@@ -1195,6 +1378,7 @@ static void trace_impl(const orc_trace_params* P, const double x0[3], const doub
     else if (t >= maxTime) term = 3;
     else term = 8;
   }
+  if (ex && term != 0) ex->valid = 0;
   res->termination = term;
   res->retry_count = 0;
   res->steps = nSteps;
@@ -1203,25 +1387,38 @@ static void trace_impl(const orc_trace_params* P, const double x0[3], const doub
   for (int d = 0; d < 3; d++) res->x[d] = x[d], res->p[d] = p[d];
 }
 
-/* TraceTrajectoryWithSingleRetry — CutoffRigidityGridless.cpp:1459-1496 */
-void orc_trace(const orc_trace_params* P, const double x0[3], const double dir[3], double R_GV,
-               orc_trace_result* res) {
-  trace_impl(P, x0, dir, R_GV, P->dt_trace_s, P->max_steps, P->max_time_s, res);
+/* TraceTrajectoryWithSingleRetry — CutoffRigidityGridless.cpp:1459-1496
+ * (retry 1: numerical failures), or TraceDensityDirection's retry —
+ * DensityGridless.cpp:849-879 (retry 2: every unresolved termination) */
+void orc_trace_ex(const orc_trace_params* P, const double x0[3], const double dir[3], double R_GV,
+                  orc_trace_result* res, orc_trace_exit* ex) {
+  trace_impl(P, x0, dir, R_GV, P->dt_trace_s, P->max_steps, P->max_time_s, res, ex);
   int t = res->termination;
-  if (!P->retry || t <= 5) return;
+  if (!P->retry) return;
+  if (P->retry == 2 ? t <= 2 : t <= 5) return;
   double dt2 = dmax(1.0e-12, 0.5 * P->dt_trace_s);
   int64_t ms2 = (P->max_steps <= (int64_t)2147483647 / 2) ? 2 * P->max_steps : (int64_t)2147483647;
-  trace_impl(P, x0, dir, R_GV, dt2, ms2, 2.0 * P->max_time_s, res);
+  trace_impl(P, x0, dir, R_GV, dt2, ms2, 2.0 * P->max_time_s, res, ex);
   res->retry_count = 1;
+}
+
+void orc_trace(const orc_trace_params* P, const double x0[3], const double dir[3], double R_GV,
+               orc_trace_result* res) {
+  orc_trace_ex(P, x0, dir, R_GV, res, NULL);
 }
 
 void orc_trace_batch(const orc_trace_params* P, int64_t n, const double* x0, const double* dir,
                      const double* R_GV, orc_trace_result* res, int nthreads) {
+  orc_trace_batch_ex(P, n, x0, dir, R_GV, res, NULL, nthreads);
+}
+
+void orc_trace_batch_ex(const orc_trace_params* P, int64_t n, const double* x0, const double* dir,
+                        const double* R_GV, orc_trace_result* res, orc_trace_exit* ex, int nthreads) {
 #ifdef _OPENMP
   if (nthreads > 0) omp_set_num_threads(nthreads);
 #pragma omp parallel for schedule(dynamic, 1)
 #endif
-  for (int64_t i = 0; i < n; i++) orc_trace(P, x0 + 3 * i, dir + 3 * i, R_GV[i], &res[i]);
+  for (int64_t i = 0; i < n; i++) orc_trace_ex(P, x0 + 3 * i, dir + 3 * i, R_GV[i], &res[i], ex ? &ex[i] : NULL);
 }
 
 /* BuildCutoffSearchGrid_GV (LOG spacing) — CutoffRigidityGridless.cpp:2578-2617 */

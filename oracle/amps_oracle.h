@@ -101,6 +101,9 @@ typedef struct {
   double B_ext_T[3];     /* uniform field added to the dipole */
   int32_t field_model;   /* 0 dipole + uniform, 1 Mode3D mesh (orc_set_trace_mesh) */
   int32_t dt_selector;   /* 0 SelectLegacyCutoffTraceDt, 1 its Mode3D form (metres) */
+  int32_t policy;        /* 0 legacy cutoff policy, 1 StructuredAccurate (density) */
+  int32_t pad_;
+  double event_tol_m;    /* boundaryEventTolerance_m */
 } orc_trace_params;
 
 typedef struct {
@@ -110,6 +113,13 @@ typedef struct {
   double time_s, distance_m;
   double x[3], p[3];
 } orc_trace_result;
+
+/* TrajectoryExitState (CutoffRigidityGridless.h:258-269) of an allowed
+ * structured trace; valid 0 otherwise. */
+typedef struct {
+  double x_exit_m[3], v_exit_unit[3], cos_alpha;
+  int32_t valid, pad_;
+} orc_trace_exit;
 
 void orc_dipole_B(const orc_trace_params* P, const double x[3], double B[3]);
 void orc_boris_step(const orc_trace_params* P, double x[3], double p[3], double dt);
@@ -121,6 +131,13 @@ void orc_hc4_step(const orc_trace_params* P, double x[3], double p[3], double dt
 int orc_step_checked(const orc_trace_params* P, double x[3], double p[3], double dt, double r_inner_m);
 void orc_trace(const orc_trace_params* P, const double x0[3], const double dir_unit[3],
                double R_GV, orc_trace_result* res);
+void orc_trace_ex(const orc_trace_params* P, const double x0[3], const double dir_unit[3],
+                  double R_GV, orc_trace_result* res, orc_trace_exit* ex);
+void orc_trace_batch_ex(const orc_trace_params* P, int64_t n, const double* x0, const double* dir,
+                        const double* R_GV, orc_trace_result* res, orc_trace_exit* ex, int nthreads);
+double orc_finalize_step(double dt, double time_remaining);
+void orc_find_first_event(const double a[3], const double z[3], const double bmin[3], const double bmax[3],
+                          double rInner, double tol, int* type, double* fraction, double pos[3], int* face);
 void orc_trace_batch(const orc_trace_params* P, int64_t n, const double* x0,
                      const double* dir, const double* R_GV, orc_trace_result* res,
                      int nthreads);

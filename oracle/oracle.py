@@ -32,6 +32,7 @@ class TraceParams(C.Structure):
         ("box_max_Re", C.c_double * 3), ("r_outer_Re", C.c_double), ("r_inner_Re", C.c_double),
         ("earth_radius_m", C.c_double), ("moment_Am2", C.c_double), ("m_hat", C.c_double * 3),
         ("B_ext_T", C.c_double * 3), ("field_model", C.c_int32), ("dt_selector", C.c_int32),
+        ("policy", C.c_int32), ("pad_", C.c_int32), ("event_tol_m", C.c_double),
     ]
 
 
@@ -80,6 +81,10 @@ def lib():
         l.orc_step_checked.argtypes = [P(TraceParams), P(dbl), P(dbl), dbl, dbl]
         l.orc_step_checked.restype = i32
         l.orc_trace_batch.argtypes = [P(TraceParams), i64, P(dbl), P(dbl), P(dbl), vp, i32]
+        l.orc_trace_batch_ex.argtypes = [P(TraceParams), i64, P(dbl), P(dbl), P(dbl), vp, vp, i32]
+        l.orc_find_first_event.argtypes = [P(dbl), P(dbl), P(dbl), P(dbl), dbl, dbl, P(i32), P(dbl), P(dbl), P(i32)]
+        l.orc_finalize_step.argtypes = [dbl, dbl]
+        l.orc_finalize_step.restype = dbl
         l.orc_rigidity_grid.argtypes = [dbl, dbl, i32, P(dbl)]
         l.orc_upper_cutoff_bin.argtypes = [P(C.c_int), i32]
         l.orc_band_topology.argtypes = [P(C.c_int), i32, P(C.c_int)]
@@ -168,6 +173,10 @@ def ref():
         r.ref_mover_steps.argtypes = [C.c_int, C.c_int, P(dbl), P(dbl), dbl, dbl, dbl, P(dbl), P(dbl)]
         r.ref_step_checked_mover.argtypes = [C.c_int, P(dbl), P(dbl), dbl, dbl, dbl, P(dbl), dbl]
         r.ref_step_checked_mover.restype = C.c_int
+        r.ref_find_first_event.argtypes = [P(dbl), P(dbl), P(dbl), P(dbl), dbl, dbl, P(C.c_int), P(dbl), P(dbl),
+                                           P(C.c_int)]
+        r.ref_finalize_step.argtypes = [dbl, dbl]
+        r.ref_finalize_step.restype = dbl
         r.ref_band_topology.argtypes = [C.c_int, P(C.c_int), P(C.c_int)]
         r.ref_refine_cutoff.argtypes = [dbl, dbl, dbl, dbl, REF_CLASSIFY, C.c_void_p, P(dbl),
                                         P(C.c_int), P(C.c_int)]
@@ -310,6 +319,36 @@ def deposit(omesh, B, species, dt, parts, node_of_corner, nnodes, nthreads=0):
                       _p(parts.blk, C.c_int), _p(parts.cell, C.c_int), _p(noc, C.c_int), nnodes,
                       _p(J, C.c_double), _p(M, C.c_double), nthreads)
     return J, M
+
+
+EXIT_DTYPE = np.dtype([("x_exit_m", "<f8", 3), ("v_exit_unit", "<f8", 3), ("cos_alpha", "<f8"),
+                       ("valid", "<i4"), ("pad_", "<i4")])
+
+
+def trace_batch_ex(params, x0, dirs, R, nthreads=0):
+    """Traces with the exit states of allowed structured traces."""
+    x0 = np.ascontiguousarray(x0, np.float64).reshape(-1, 3)
+    dirs = np.ascontiguousarray(dirs, np.float64).reshape(-1, 3)
+    R = np.ascontiguousarray(R, np.float64)
+    out = np.zeros(len(R), RESULT_DTYPE)
+    ex = np.zeros(len(R), EXIT_DTYPE)
+    lib().orc_trace_batch_ex(C.byref(params), len(R), _p(x0, C.c_double), _p(dirs, C.c_double),
+                             _p(R, C.c_double), out.ctypes.data_as(C.c_void_p), ex.ctypes.data_as(C.c_void_p),
+                             nthreads)
+    return out, ex
+
+
+def find_first_event(a, z, bmin, bmax, r_inner, tol):
+    """The first boundary event on the chord a -> z: (type, fraction, position, face)."""
+    a, z, bmin, bmax = (np.ascontiguousarray(v, np.float64) for v in (a, z, bmin, bmax))
+    t, fr, face, pos = C.c_int(), C.c_double(), C.c_int(), np.zeros(3)
+    lib().orc_find_first_event(_p(a, C.c_double), _p(z, C.c_double), _p(bmin, C.c_double), _p(bmax, C.c_double),
+                               r_inner, tol, C.byref(t), C.byref(fr), _p(pos, C.c_double), C.byref(face))
+    return t.value, fr.value, pos, face.value
+
+
+def finalize_step(dt, time_remaining):
+    return lib().orc_finalize_step(dt, time_remaining)
 
 
 def trace_batch(params, x0, dirs, R, nthreads=0):

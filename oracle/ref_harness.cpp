@@ -7,6 +7,8 @@
 #include "CutoffBandSearch.h"
 #include "DipoleInterface.h"
 #include "GridlessParticleMovers.h"
+#include "TrajectoryBoundary.h"
+#include "TrajectoryTimeStep.h"
 
 namespace {
 struct DipolePlusUniform : public IGridlessFieldEvaluator {
@@ -98,6 +100,23 @@ int ref_step_checked_mover(int mover, double x[3], double p[3], double q, double
   x[0] = xv.x, x[1] = xv.y, x[2] = xv.z;
   p[0] = pv.x, p[1] = pv.y, p[2] = pv.z;
   return ok ? 1 : 0;
+}
+
+// TrajectoryBoundary::FindFirstEvent on the chord a -> z.
+void ref_find_first_event(const double a[3], const double z[3], const double bmin[3], const double bmax[3],
+                          double r_inner, double tol, int* type, double* fraction, double pos[3], int* face) {
+  Earth::TrajectoryBoundary::Box b;
+  for (int d = 0; d < 3; d++) b.min[d] = bmin[d], b.max[d] = bmax[d];
+  b.innerRadius = r_inner;
+  const Earth::TrajectoryBoundary::Event e = Earth::TrajectoryBoundary::FindFirstEvent(a, z, b, tol);
+  *type = static_cast<int>(e.type);
+  *fraction = e.fraction;
+  for (int d = 0; d < 3; d++) pos[d] = e.position[d];
+  *face = e.outerFace;
+}
+
+double ref_finalize_step(double dt, double time_remaining) {
+  return Earth::TrajectoryTimeStep::FinalizeUpperBoundedStep(dt, time_remaining);
 }
 
 // AnalyzeCutoffBandSamples over states numbered as CutoffSampleState

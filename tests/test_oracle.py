@@ -167,6 +167,84 @@ def test_hc4_reversible_and_norm_preserving():
 
 
 @needs_ref
+def test_find_first_event_matches_reference_bitwise():
+    """TrajectoryBoundary::FindFirstEvent and FinalizeUpperBoundedStep of the
+    compiled reference against the oracle: chords inside, across faces,
+    edges and the inner sphere, starting outside, degenerate, with and
+    without a tolerance."""
+    r = O.ref()
+    l = O.lib()
+    D = O.C.c_double
+    rng = np.random.default_rng(1)
+    bmin, bmax = np.array([-5.0, -4.0, -6.0]), np.array([5.0, 4.0, 6.0])
+    seen = set()
+    for i in range(20000):
+        a = rng.uniform(-7, 7, 3) if i % 3 else rng.uniform(-1.5, 1.5, 3)
+        z = a + rng.normal(size=3) * rng.choice([0.01, 0.5, 3.0])
+        if i % 7 == 0:
+            z = a.copy()
+        if i % 11 == 0:
+            a[rng.integers(0, 3)] = bmax[0]  # on a face plane
+        rin = 1.0 if i % 2 else 0.0
+        tol = float(rng.choice([0.0, 1e-9, 0.1]))
+        res = []
+        for f in (r.ref_find_first_event, l.orc_find_first_event):
+            t, fr, face, pos = O.C.c_int(), O.C.c_double(), O.C.c_int(), np.zeros(3)
+            f(O._p(a, D), O._p(z, D), O._p(bmin, D), O._p(bmax, D), rin, tol, O.C.byref(t), O.C.byref(fr),
+              O._p(pos, D), O.C.byref(face))
+            res.append((t.value, fr.value, tuple(pos), face.value))
+        assert res[0] == res[1], (i, res)
+        seen.add(res[0][0])
+    assert seen == {0, 1, 2}
+    for dt, trem in [(1.0, 0.5), (-1.0, 1.0), (2.0, -1.0), (1e-300, 1e-320), (np.inf, 1.0), (0.3, 0.0)]:
+        assert O.finalize_step(dt, trem) == r.ref_finalize_step(dt, trem) or (
+            np.isnan(O.finalize_step(dt, trem)) and np.isnan(r.ref_finalize_step(dt, trem)))
+
+
+def test_find_first_event_closed_forms():
+    ev = O.find_first_event
+    bmin, bmax = -np.ones(3), np.ones(3)
+    # leaving through x = +1 half-way along the chord
+    t, f, pos, face = ev([0, 0, 0], [2, 0, 0], bmin, bmax, 0.0, 0.0)
+    assert (t, f, face) == (2, 0.5, 1) and list(pos) == [1.0, 0.0, 0.0]
+    # the inner sphere first when the chord meets it before the box
+    t, f, pos, face = ev([0.9, 0, 0], [-3.9, 0, 0], bmin * 5, bmax * 5, 0.5, 0.0)
+    assert t == 1 and abs(f - 0.4 / 4.8) < 1e-15 and abs(pos[0] - 0.5) < 1e-15
+    # starting outside the box is an event at fraction 0
+    t, f, pos, face = ev([1.5, 0, 0], [0, 0, 0], bmin, bmax, 0.0, 0.0)
+    assert (t, f, face) == (2, 0.0, -1)
+    # a chord inside both: no event
+    t, f, pos, face = ev([0.1, 0.1, 0.1], [0.2, 0.3, 0.1], bmin, bmax, 0.05, 0.0)
+    assert t == 0 and f == np.inf
+
+
+def test_structured_exit_state_uniform_field():
+    """A uniform field along z: the pitch-angle cosine is an invariant, the
+    exit point lies on the box face the chord crosses, the exit direction is
+    a unit vector, and the retry flag never fires for resolved traces."""
+    P = TS.params(moment_scale=0.0, bz_T=2e-8)
+    P.moment_Am2 = 0.0
+    P.policy, P.event_tol_m, P.dist_limiter, P.dt_floor, P.retry = 1, 1.0, 0, 0, 0
+    P.r_inner_Re = 0.0
+    rng = np.random.default_rng(9)
+    n = 200
+    x0 = rng.uniform(-2, 2, (n, 3)) * TS.RE_AMPS
+    d = rng.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1)[:, None]
+    R = np.full(n, 0.05)
+    res, ex = O.trace_batch_ex(P, x0, d, R)
+    ok = res["termination"] == 0
+    assert ok.mean() > 0.8 and np.all(ex["valid"][ok] == 1) and np.all(ex["valid"][~ok] == 0)
+    v = ex["v_exit_unit"][ok]
+    assert np.max(np.abs(np.linalg.norm(v, axis=1) - 1)) < 1e-14
+    mu0 = -d[ok, 2]  # start direction -d, B along +z
+    assert np.max(np.abs(ex["cos_alpha"][ok] - mu0)) < 5e-3
+    box = 35.0 * TS.RE_AMPS
+    on_face = np.isclose(np.max(np.abs(ex["x_exit_m"][ok]), axis=1), box, rtol=0, atol=1e-6)
+    assert on_face.all()
+
+
+@needs_ref
 def test_band_topology_matches_reference():
     r = O.ref()
     rng = np.random.default_rng(5)

@@ -311,3 +311,65 @@ def test_trace_rejects_unbuilt_movers(gpu_ctx):
     P.mover = CU.MOVERS["RK4"]
     with pytest.raises(RuntimeError, match="FP32"):
         CU.trace_batch(gpu_ctx, P, x, d, R, precision=L.FP32)
+
+
+def _same_exit(g, o):
+    assert np.array_equal(g["valid"], o["valid"])
+    for f in ["x_exit_m", "v_exit_unit", "cos_alpha"]:
+        assert np.array_equal(g[f], o[f]), f
+
+
+@pytest.mark.parametrize("mover,retry", [("BORIS", False), ("RK4", True)])
+def test_structured_trace_and_exit_states_bitexact(gpu_ctx, mover, retry):
+    """The density / transmission policy (TraceTrajectorySharedEx): the first
+    boundary event on every chord, the exit state of the allowed traces and,
+    with DS_RETRY_UNRESOLVED, the retry of every unresolved trace, bit-exact
+    with the oracle (terminations, steps, time, path, x, p, exit x, direction
+    and pitch cosine)."""
+    P = CU.density_params(retry_unresolved=retry, mover=mover, tilt_deg=10.0, bz_T=-5e-9)
+    P.max_time_s = 30.0
+    if retry:
+        P.max_steps = 3000  # step limits that the retry (2x steps, 2x time) revisits
+    x, d, R = _random_tasks(1500, 23, rhi=4.0)
+    print(f"structured trace case mover={mover} retry={retry} n={len(R)}", flush=True)
+    g, ge, steps = CU.trace_batch_ex(gpu_ctx, P, x, d, R)
+    o, oe = O.trace_batch_ex(_oparams(P), x, d, R)
+    _same(g, o)
+    _same_exit(ge, oe)
+    # the step count covers every step taken, the discarded first attempts too
+    first = 0
+    rt = np.flatnonzero(o["retry_count"] == 1)
+    if len(rt):
+        P0 = _oparams(P)
+        P0.retry = 0
+        first = int(O.trace_batch(P0, x[rt], d[rt], R[rt])["steps"].sum())
+    assert steps == int(o["steps"].sum()) + first
+    assert (o["termination"] == 0).sum() > 100 and (o["termination"] == 1).sum() > 100
+    if retry:
+        assert len(rt) > 10
+
+
+def test_transmission_matches_oracle_counts(gpu_ctx):
+    """Isotropic transmission per (point, rigidity): the library's counts are
+    those of the oracle's traces of the same tasks, reduced over directions."""
+    P = CU.density_params(tilt_deg=10.0)
+    P.max_time_s = 30.0
+    pts = np.array([TS.shell_point(lon, lat, 500.0, TS.RE_AMPS) for lon, lat in [(0, 0), (90, 40), (200, -65)]])
+    R = np.array([0.3, 1.0, 3.0, 10.0])
+    zen, az = CU.config2_directions(24)
+    dirs = np.array([CU.arrival_dirs(p[None], zen, az).reshape(-1, 3) for p in pts])
+    # one direction set for all points: the local frame of the first point
+    dset = dirs[0]
+    print(f"transmission case pts={len(pts)} nR={len(R)} ndir={len(dset)}", flush=True)
+    t = CU.transmission(gpu_ctx, P, pts, R, dset)
+    x0 = np.repeat(pts, len(R) * len(dset), axis=0)
+    Rt = np.tile(np.repeat(R, len(dset)), len(pts))
+    dt_ = np.tile(dset, (len(pts) * len(R), 1))
+    o, _ = O.trace_batch_ex(_oparams(P), x0, dt_, Rt)
+    term = o["termination"].reshape(len(pts), len(R), len(dset))
+    assert np.array_equal(t["sampled"], np.full((len(pts), len(R)), len(dset)))
+    assert np.array_equal(t["resolved"], (term <= 2).sum(-1))
+    assert np.array_equal(t["weight_sum"], (term == 0).sum(-1).astype(float))
+    for k in range(9):
+        assert np.array_equal(t["term_counts"][..., k], (term == k).sum(-1))
+    assert t["T"].min() >= 0.0 and t["T"].max() <= 1.0 and t["T"].max() > 0.5 and t["T"].min() < 0.5
