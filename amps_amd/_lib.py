@@ -71,6 +71,9 @@ class TraceParams(C.Structure):
         ("earth_radius_m", C.c_double), ("moment_Am2", C.c_double), ("m_hat", C.c_double * 3),
         ("B_ext_T", C.c_double * 3), ("field_model", C.c_int32), ("dt_selector", C.c_int32),
         ("policy", C.c_int32), ("pad2_", C.c_int32), ("event_tol_m", C.c_double),
+        ("trap", C.c_int32), ("trap_min_mirror", C.c_int32), ("trap_min_bounce", C.c_int32), ("pad3_", C.c_int32),
+        ("trap_outer_margin_Re", C.c_double), ("trap_radial_tol_Re", C.c_double), ("trap_energy_tol", C.c_double),
+        ("trap_deadband", C.c_double),
     ]
 
 

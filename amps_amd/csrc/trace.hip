@@ -19,6 +19,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <type_traits>
 #include <vector>
 
 #include "amps_common.h"
@@ -47,9 +48,15 @@ struct Fp64Lane {
   double x[3], p[3], t, s, dtTrace, maxTime;
   int64_t n, maxSteps;
   int retry;
+};
+// the lane of the structured policy and the trap detector
+struct Fp64LaneX : Fp64Lane {
   // exit state of an allowed structured trace (x_exit, v_exit, cos alpha)
   double ex[7];
   int ex_valid;
+  // trap detector (TrajectoryTrap::Detector, util/TrajectoryTrapDetector.h:23-153)
+  int tr_init, tr_trapped, tr_sign, tr_mirror, tr_bounce, tr_stable, tr_prev;
+  double tr_pref, tr_pmin, tr_pmax, tr_rmin, tr_rmax, tr_prmin, tr_prmax, tr_margin;
 };
 
 // Dipole::GetB_Tesla (DipoleInterface.h:99-125) plus a uniform field
@@ -501,7 +508,7 @@ __device__ __forceinline__ void find_first_event(const double* a, const double* 
 // event, cos(pitch angle) against B a tolerance inside the box.  False for an
 // invalid field or momentum (InvalidField).
 template <int FM>
-__device__ __forceinline__ bool exit_state(const TraceView& a, Fp64Lane& L, const double* pos, double fraction,
+__device__ __forceinline__ bool exit_state(const TraceView& a, Fp64LaneX& L, const double* pos, double fraction,
                                            const double* xb, const double* xa, const double* pb, const double* pa) {
   const amps_trace_params& P = a.P;
 #pragma unroll
@@ -541,16 +548,82 @@ __device__ __forceinline__ bool exit_state(const TraceView& a, Fp64Lane& L, cons
   return L.ex_valid != 0;
 }
 
-template <int MV, int FM>
+// TrajectoryTrap::Detector::Update (util/TrajectoryTrapDetector.h:36-103):
+// mirror points are sign changes of p.B beyond the deadband, two per bounce
+// cycle; the orbit is trapped once enough cycles have a stable radial
+// envelope, stay the margin inside the box and keep |p| to the tolerance.
+__device__ __forceinline__ void trap_reset(Fp64LaneX& L) {
+  L.tr_init = L.tr_trapped = L.tr_sign = L.tr_mirror = L.tr_bounce = L.tr_stable = L.tr_prev = 0;
+  L.tr_pref = L.tr_pmax = L.tr_rmax = L.tr_prmin = L.tr_prmax = 0.0;
+  L.tr_pmin = L.tr_rmin = L.tr_margin = __builtin_inf();
+}
+__device__ __forceinline__ double trap_margin(const TraceView& a, const double* x) {
+  return dmin(dmin(dmin(x[0] - a.box_min_m[0], a.box_max_m[0] - x[0]), dmin(x[1] - a.box_min_m[1], a.box_max_m[1] - x[1])),
+              dmin(x[2] - a.box_min_m[2], a.box_max_m[2] - x[2]));
+}
+__device__ __forceinline__ bool trap_update(const TraceView& a, Fp64LaneX& L, const double* x, const double* p,
+                                            const double* B) {
+  const amps_trace_params& P = a.P;
+  if (L.tr_trapped) return true;
+  if (!fin3(x) || !fin3(p) || !fin3(B)) return false;
+  const double pMag = sqrt(p[0] * p[0] + p[1] * p[1] + p[2] * p[2]);
+  const double bMag = sqrt(B[0] * B[0] + B[1] * B[1] + B[2] * B[2]);
+  const double r = sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2]);
+  if (!(pMag > 0.0) || !(bMag > 0.0) || !isfinite(r)) return false;
+  if (!L.tr_init) {
+    L.tr_init = 1;
+    L.tr_pref = pMag, L.tr_pmin = pMag, L.tr_pmax = pMag;
+    L.tr_rmin = r, L.tr_rmax = r;
+    L.tr_margin = dmin(L.tr_margin, trap_margin(a, x));
+  } else {
+    L.tr_pmin = dmin(L.tr_pmin, pMag);
+    L.tr_pmax = dmax(L.tr_pmax, pMag);
+    L.tr_rmin = dmin(L.tr_rmin, r);
+    L.tr_rmax = dmax(L.tr_rmax, r);
+    L.tr_margin = dmin(L.tr_margin, trap_margin(a, x));
+  }
+  const double mu = (p[0] * B[0] + p[1] * B[1] + p[2] * B[2]) / (pMag * bMag);
+  const int sign = (mu > P.trap_deadband) ? 1 : ((mu < -P.trap_deadband) ? -1 : 0);
+  if (sign == 0) return false;
+  if (L.tr_sign == 0) {
+    L.tr_sign = sign;
+    return false;
+  }
+  if (sign == L.tr_sign) return false;
+  L.tr_sign = sign;
+  ++L.tr_mirror;
+  if ((L.tr_mirror % 2) != 0) return false;
+  ++L.tr_bounce;
+  if (L.tr_prev) {
+    const double ch = dmax(fabs(L.tr_rmin - L.tr_prmin), fabs(L.tr_rmax - L.tr_prmax));
+    if (ch <= P.trap_radial_tol_Re * P.earth_radius_m) ++L.tr_stable;
+    else L.tr_stable = 0;
+  }
+  L.tr_prmin = L.tr_rmin;
+  L.tr_prmax = L.tr_rmax;
+  L.tr_prev = 1;
+  const double spread = (L.tr_pref > 0.0) ? (L.tr_pmax - L.tr_pmin) / L.tr_pref : __builtin_inf();
+  L.tr_trapped = (L.tr_mirror >= P.trap_min_mirror && L.tr_bounce >= P.trap_min_bounce && L.tr_stable >= 2 &&
+                  L.tr_margin >= P.trap_outer_margin_Re * P.earth_radius_m && spread <= P.trap_energy_tol)
+                     ? 1
+                     : 0;
+  L.tr_rmin = r, L.tr_rmax = r;
+  L.tr_margin = dmin(L.tr_margin, trap_margin(a, x));
+  return L.tr_trapped != 0;
+}
+
+// X: the structured policy, exit states and the trap detector are compiled
+// in (the legacy cutoff kernels stay lean: twice the wavefronts per SIMD)
+template <int MV, int FM, bool X>
 struct Fp64Stepper {
-  using Lane = Fp64Lane;
-  static constexpr bool HAS_EXIT = true;
+  using Lane = std::conditional_t<X, Fp64LaneX, Fp64Lane>;
+  static constexpr bool HAS_EXIT = X;
   static __device__ __forceinline__ void init(const TraceView& a, Lane& L, const double* x0,
                                               const double* dir, double R, int attempt) {
     const amps_trace_params& P = a.P;
     const double pMag = (R * 1.0e9 * P.qabs_C) / C_LIGHT;
     double v0[3] = {-1.0 * dir[0], -1.0 * dir[1], -1.0 * dir[2]};
-    if (P.policy == 1) {
+    if (X && P.policy == 1) {
       // TraceTrajectoryShared(Ex) (CutoffRigidityGridless.cpp:1563-1585)
       // normalise the start direction
       const double nv = sqrt(dot3(v0, v0));
@@ -571,7 +644,15 @@ struct Fp64Stepper {
     L.s = 0.0;
     L.n = 0;
     L.retry = attempt;
-    L.ex_valid = 0;
+    if constexpr (X) {
+      L.ex_valid = 0;
+      trap_reset(L);
+      if (P.trap) {
+        double B0[3];
+        field64<FM>(a, L.x, B0);
+        trap_update(a, L, L.x, L.p, B0);
+      }
+    }
     if (attempt == 0) {
       L.dtTrace = P.dt_trace_s;
       L.maxSteps = P.max_steps;
@@ -606,7 +687,8 @@ struct Fp64Stepper {
     if (!inside) {
       // structured policy: an OuterBox event at the step start (fraction 0
       // on a zero chord)
-      if (P.policy == 1 && a.exits) return exit_state<FM>(a, L, x, 0.0, x, x, p, p) ? 0 : 7;
+      if constexpr (X)
+        if (P.policy == 1 && a.exits) return exit_state<FM>(a, L, x, 0.0, x, x, p, p) ? 0 : 7;
       return 0;
     }
     // B at the start point: evaluated once by the adaptive step selector and
@@ -614,7 +696,7 @@ struct Fp64Stepper {
     double Bx[3];
     bool have_B = false;
     double dt;
-    if (P.policy == 1)
+    if (X && P.policy == 1)
       dt = select_dt_structured<FM>(a, x, p, L.dtTrace, L.maxTime - L.t, Bx, have_B);
     else if (P.dt_selector == 1)
       dt = select_dt3d<FM>(a, x, p, L.dtTrace, L.maxTime - L.t, Bx, have_B);
@@ -627,14 +709,22 @@ struct Fp64Stepper {
     if (!fin3(x) || !fin3(p)) return 8;
     int etype = 0;
     double efrac = 0.0, epos[3] = {0.0, 0.0, 0.0};
-    if (P.policy == 1) find_first_event(xb, x, a.box_min_m, a.box_max_m, rIn, P.event_tol_m, etype, efrac, epos);
+    if constexpr (X)
+      if (P.policy == 1) find_first_event(xb, x, a.box_min_m, a.box_max_m, rIn, P.event_tol_m, etype, efrac, epos);
     const double dd[3] = {x[0] - xb[0], x[1] - xb[1], x[2] - xb[2]};
     const double seg = sqrt(dot3(dd, dd));
     L.s += isfinite(seg) ? seg : 0.0;
     L.t += dt;
     ++L.n;
-    if (etype == 1) return 1;
-    if (etype == 2) return (!a.exits || exit_state<FM>(a, L, epos, efrac, xb, x, pb, p)) ? 0 : 7;
+    if constexpr (X) {
+      if (etype == 1) return 1;
+      if (etype == 2) return (!a.exits || exit_state<FM>(a, L, epos, efrac, xb, x, pb, p)) ? 0 : 7;
+      if (P.trap) {
+        double Bt[3];
+        field64<FM>(a, x, Bt);
+        if (trap_update(a, L, x, p, Bt)) return 2;  // MagneticallyTrappedForbidden
+      }
+    }
     return -1;
   }
   static __device__ __forceinline__ void result(const TraceView& a, const Lane& L, int term,
@@ -651,6 +741,7 @@ struct Fp64Stepper {
     }
   }
   static __device__ __forceinline__ void exit_out(const Lane& L, int term, amps_trace_exit& e) {
+    static_assert(X, "exit states come from the extended lane");
     const bool v = term == 0 && L.ex_valid;
 #pragma unroll
     for (int d = 0; d < 3; d++) {
@@ -1048,6 +1139,8 @@ static int check_params(const amps_trace_params* P) {
   if (P->field_model < 0 || P->field_model > 1) return fail(AMPS_E_ARG, "trace: field_model must be 0 or 1");
   if (P->retry < 0 || P->retry > 2) return fail(AMPS_E_ARG, "trace: retry must be 0, 1 or 2");
   if (P->policy < 0 || P->policy > 1) return fail(AMPS_E_ARG, "trace: policy must be 0 (legacy) or 1 (structured)");
+  if (P->trap && (P->trap_min_mirror < 0 || P->trap_min_bounce < 0 || P->outer_sphere))
+    return fail(AMPS_E_ARG, "trace: trap detection needs non-negative minima and the box boundary");
   if (P->policy == 1 && (P->outer_sphere || P->dt_selector != 0))
     return fail(AMPS_E_ARG, "trace: the structured policy runs with the box boundary and its own step selector");
   if (P->dt_selector < 0 || P->dt_selector > 1) return fail(AMPS_E_ARG, "trace: dt_selector must be 0 or 1");
@@ -1088,21 +1181,21 @@ static void fill_view(amps_ctx* c, TraceState* t, const amps_trace_params* P, Tr
   a.chunk = t->chunk;
 }
 
-template <int FM>
+template <int FM, bool X>
 static void launch64(amps_ctx* c, const TraceView& a, dim3 g, dim3 blk) {
   switch (a.P.mover) {
-    case MV_HC4: hipLaunchKernelGGL((k_trace_persistent<Fp64Stepper<MV_HC4, FM>>), g, blk, 0, c->stream, a); break;
-    case MV_RK2: hipLaunchKernelGGL((k_trace_persistent<Fp64Stepper<MV_RK2, FM>>), g, blk, 0, c->stream, a); break;
-    case MV_RK4: hipLaunchKernelGGL((k_trace_persistent<Fp64Stepper<MV_RK4, FM>>), g, blk, 0, c->stream, a); break;
-    case MV_RK6: hipLaunchKernelGGL((k_trace_persistent<Fp64Stepper<MV_RK6, FM>>), g, blk, 0, c->stream, a); break;
-    default: hipLaunchKernelGGL((k_trace_persistent<Fp64Stepper<MV_BORIS, FM>>), g, blk, 0, c->stream, a); break;
+    case MV_HC4: hipLaunchKernelGGL((k_trace_persistent<Fp64Stepper<MV_HC4, FM, X>>), g, blk, 0, c->stream, a); break;
+    case MV_RK2: hipLaunchKernelGGL((k_trace_persistent<Fp64Stepper<MV_RK2, FM, X>>), g, blk, 0, c->stream, a); break;
+    case MV_RK4: hipLaunchKernelGGL((k_trace_persistent<Fp64Stepper<MV_RK4, FM, X>>), g, blk, 0, c->stream, a); break;
+    case MV_RK6: hipLaunchKernelGGL((k_trace_persistent<Fp64Stepper<MV_RK6, FM, X>>), g, blk, 0, c->stream, a); break;
+    default: hipLaunchKernelGGL((k_trace_persistent<Fp64Stepper<MV_BORIS, FM, X>>), g, blk, 0, c->stream, a); break;
   }
 }
 
 static int launch(amps_ctx* c, TraceState* t, TraceView& a, int precision) {
   if (precision == AMPS_FP32 && a.P.mover != MV_BORIS && a.P.mover != MV_HC4)
     return fail(AMPS_E_ARG, "trace: the FP32 path runs the BORIS and HC4 movers only");
-  if (precision == AMPS_FP32 && (a.P.field_model != 0 || a.P.dt_selector != 0 || a.P.policy != 0))
+  if (precision == AMPS_FP32 && (a.P.field_model != 0 || a.P.dt_selector != 0 || a.P.policy != 0 || a.P.trap))
     return fail(AMPS_E_ARG, "trace: the FP32 path runs the analytic field and the legacy gridless policy only");
   if (a.P.field_model == 1) {
     if (!c->has_mode3d) return fail(AMPS_E_STATE, "trace: field_model 1 needs amps_gpu_set_mode3d_field");
@@ -1123,10 +1216,12 @@ static int launch(amps_ctx* c, TraceState* t, TraceView& a, int precision) {
       hipLaunchKernelGGL(k_trace_persistent<Fp32Stepper<MV_HC4>>, g, blk, 0, c->stream, a);
     else
       hipLaunchKernelGGL(k_trace_persistent<Fp32Stepper<MV_BORIS>>, g, blk, 0, c->stream, a);
-  } else if (a.P.field_model == 1) {
-    launch64<1>(c, a, g, blk);
   } else {
-    launch64<0>(c, a, g, blk);
+    const bool x = a.P.policy == 1 || a.P.trap;
+    if (a.P.field_model == 1)
+      x ? launch64<1, true>(c, a, g, blk) : launch64<1, false>(c, a, g, blk);
+    else
+      x ? launch64<0, true>(c, a, g, blk) : launch64<0, false>(c, a, g, blk);
   }
   HIP_TRY(hipGetLastError());
   tend(c, AMPS_K_TRACE);

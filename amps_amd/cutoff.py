@@ -54,7 +54,8 @@ def make_params(*, tilt_deg=0.0, moment_scale=1.0, bz_T=0.0, mass_amu=PROTON_AMU
                 max_distance_Re=400.0, dphi=0.15, adaptive=True, dist_limiter=True, dt_floor=True,
                 outer_sphere=False, box_Re=35.0, r_outer_Re=35.0, r_inner_Re=1.0,
                 earth_radius_m=RE_AMPS, retry=True, mover="BORIS", field_model=0, dt_selector=0,
-                policy=0, event_tol_m=1.0):
+                policy=0, event_tol_m=1.0, trap=False, trap_min_mirror=8, trap_min_bounce=4,
+                trap_outer_margin_Re=1.0, trap_radial_tol_Re=0.05, trap_energy_tol=1.0e-4, trap_deadband=1.0e-6):
     P = L.TraceParams()
     if mover not in MOVERS:
         raise ValueError("mover must be one of %s (guiding-centre and hybrid movers are not built)"
@@ -64,6 +65,11 @@ def make_params(*, tilt_deg=0.0, moment_scale=1.0, bz_T=0.0, mass_amu=PROTON_AMU
     P.dt_selector = int(dt_selector)
     P.policy = int(policy)
     P.event_tol_m = float(event_tol_m)
+    # TrajectoryTrap::Config (util/TrajectoryTrapDetector.h:13-21), parser defaults
+    P.trap = int(trap)
+    P.trap_min_mirror, P.trap_min_bounce = int(trap_min_mirror), int(trap_min_bounce)
+    P.trap_outer_margin_Re, P.trap_radial_tol_Re = float(trap_outer_margin_Re), float(trap_radial_tol_Re)
+    P.trap_energy_tol, P.trap_deadband = float(trap_energy_tol), float(trap_deadband)
     q = charge_e * ELECTRON_CHARGE
     P.q_C = -q if reversed_charge else q
     P.qabs_C = abs(q)

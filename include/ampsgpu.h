@@ -274,6 +274,14 @@ typedef struct {
                             start direction; box boundary, FP64 only */
   int32_t pad2_;
   double event_tol_m;    /* boundaryEventTolerance_m (default 1 m) */
+  /* TrajectoryTrap::Detector (util/TrajectoryTrapDetector.h:23-153), the
+   * MagneticallyTrappedForbidden termination of both policies: TRAP_DETECTION,
+   * TRAP_MIN_MIRROR_POINTS (8), TRAP_MIN_BOUNCE_CYCLES (4), TRAP_OUTER_MARGIN
+   * (1 Re), TRAP_RADIAL_GROWTH_TOLERANCE (0.05 Re), TRAP_ENERGY_RELATIVE_
+   * TOLERANCE (1e-4), TRAP_PARALLEL_DEADBAND (1e-6); lengths in earth radii */
+  int32_t trap;
+  int32_t trap_min_mirror, trap_min_bounce, pad3_;
+  double trap_outer_margin_Re, trap_radial_tol_Re, trap_energy_tol, trap_deadband;
 } amps_trace_params;
 
 /* TrajectoryExitState (CutoffRigidityGridless.h:258-269) of an allowed

@@ -1294,6 +1294,72 @@ static int populate_exit(const orc_trace_params* P, const double pos[3], double 
   return isfinite(ex->cos_alpha);
 }
 
+/* TrajectoryTrap::Detector — util/TrajectoryTrapDetector.h:23-153: counts
+ * mirror points (sign changes of p.B beyond a deadband), two per bounce
+ * cycle, and calls the orbit trapped once enough cycles have a stable radial
+ * envelope, kept off the box by the margin, with |p| unchanged to the
+ * tolerance. */
+void orc_trap_init(orc_trap* t) {
+  memset(t, 0, sizeof(*t));
+  t->p_min = INFINITY;
+  t->r_min = INFINITY;
+  t->min_margin = INFINITY;
+}
+
+static double trap_margin(const orc_trace_params* P, const double x[3]) {
+  double bmin[3], bmax[3];
+  for (int d = 0; d < 3; d++) bmin[d] = P->box_min_Re[d] * P->earth_radius_m, bmax[d] = P->box_max_Re[d] * P->earth_radius_m;
+  return dmin(dmin(dmin(x[0] - bmin[0], bmax[0] - x[0]), dmin(x[1] - bmin[1], bmax[1] - x[1])),
+              dmin(x[2] - bmin[2], bmax[2] - x[2]));
+}
+
+int orc_trap_update(const orc_trace_params* P, orc_trap* t, const double x[3], const double p[3], const double B[3]) {
+  if (!P->trap || t->trapped) return t->trapped;
+  if (!isfin3(x) || !isfin3(p) || !isfin3(B)) return 0;
+  const double pMag = sqrt(p[0] * p[0] + p[1] * p[1] + p[2] * p[2]);
+  const double bMag = sqrt(B[0] * B[0] + B[1] * B[1] + B[2] * B[2]);
+  const double r = sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2]);
+  if (!(pMag > 0.0) || !(bMag > 0.0) || !isfinite(r)) return 0;
+  if (!t->initialized) {
+    t->initialized = 1;
+    t->p_ref = pMag, t->p_min = pMag, t->p_max = pMag;
+    t->r_min = r, t->r_max = r;
+    t->min_margin = dmin(t->min_margin, trap_margin(P, x));
+  } else {
+    t->p_min = dmin(t->p_min, pMag);
+    t->p_max = dmax(t->p_max, pMag);
+    t->r_min = dmin(t->r_min, r);
+    t->r_max = dmax(t->r_max, r);
+    t->min_margin = dmin(t->min_margin, trap_margin(P, x));
+  }
+  const double mu = (p[0] * B[0] + p[1] * B[1] + p[2] * B[2]) / (pMag * bMag);
+  const int sign = (mu > P->trap_deadband) ? 1 : ((mu < -P->trap_deadband) ? -1 : 0);
+  if (sign == 0) return t->trapped;
+  if (t->last_sign == 0) {
+    t->last_sign = sign;
+    return t->trapped;
+  }
+  if (sign == t->last_sign) return t->trapped;
+  t->last_sign = sign;
+  ++t->mirror;
+  if ((t->mirror % 2) != 0) return t->trapped;
+  ++t->bounce;
+  if (t->have_prev) {
+    const double ch = dmax(fabs(t->r_min - t->prev_r_min), fabs(t->r_max - t->prev_r_max));
+    if (ch <= P->trap_radial_tol_Re * P->earth_radius_m) ++t->stable;
+    else t->stable = 0;
+  }
+  t->prev_r_min = t->r_min;
+  t->prev_r_max = t->r_max;
+  t->have_prev = 1;
+  const double spread = (t->p_ref > 0.0) ? (t->p_max - t->p_min) / t->p_ref : INFINITY;
+  t->trapped = (t->mirror >= P->trap_min_mirror && t->bounce >= P->trap_min_bounce && t->stable >= 2 &&
+                t->min_margin >= P->trap_outer_margin_Re * P->earth_radius_m && spread <= P->trap_energy_tol);
+  t->r_min = r, t->r_max = r;
+  t->min_margin = dmin(t->min_margin, trap_margin(P, x));
+  return t->trapped;
+}
+
 /* TraceTrajectoryImpl — CutoffRigidityGridless.cpp:1246-1457 (full-orbit
  * mover P->mover, trap detector off).  policy 0: the legacy cutoff policy
  * (SelectLegacyCutoffTraceDt, boundaries checked at the step start);
@@ -1325,6 +1391,13 @@ static void trace_impl(const orc_trace_params* P, const double x0[3], const doub
   int64_t nSteps = 0;
   int term = -1;
   if (ex) memset(ex, 0, sizeof(*ex));
+  orc_trap trap;
+  orc_trap_init(&trap);
+  if (P->trap) {
+    double B0[3];
+    trace_field(P, x, B0);
+    orc_trap_update(P, &trap, x, p, B0);
+  }
   while (nSteps < maxSteps && t < maxTime && (P->max_distance_m <= 0.0 || s < P->max_distance_m)) {
     if (!isfin3(x) || !isfin3(p)) { term = 8; break; }
     if (rInner > 0.0 && x[0] * x[0] + x[1] * x[1] + x[2] * x[2] <= dmax(0.0, rInner + 0.0) * dmax(0.0, rInner + 0.0)) {
@@ -1370,6 +1443,11 @@ static void trace_impl(const orc_trace_params* P, const double x0[3], const doub
         else ex->valid = 1;
       }
       break;
+    }
+    if (P->trap) {
+      double Bt[3];
+      trace_field(P, x, Bt);
+      if (orc_trap_update(P, &trap, x, p, Bt)) { term = 2; break; }
     }
   }
   if (term < 0) {

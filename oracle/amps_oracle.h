@@ -104,7 +104,19 @@ typedef struct {
   int32_t policy;        /* 0 legacy cutoff policy, 1 StructuredAccurate (density) */
   int32_t pad_;
   double event_tol_m;    /* boundaryEventTolerance_m */
+  int32_t trap;          /* trapDetection */
+  int32_t trap_min_mirror, trap_min_bounce, pad3_;
+  double trap_outer_margin_Re, trap_radial_tol_Re, trap_energy_tol, trap_deadband;
 } orc_trace_params;
+
+/* TrajectoryTrap::Detector state (util/TrajectoryTrapDetector.h:23-153) */
+typedef struct {
+  int initialized, trapped, last_sign, mirror, bounce, stable, have_prev, pad_;
+  double p_ref, p_min, p_max, r_min, r_max, prev_r_min, prev_r_max, min_margin;
+} orc_trap;
+void orc_trap_init(orc_trap* t);
+/* Detector::Update with the trace parameters' trap configuration; returns trapped */
+int orc_trap_update(const orc_trace_params* P, orc_trap* t, const double x[3], const double p[3], const double B[3]);
 
 typedef struct {
   int32_t termination;   /* TrajectoryTermination value */

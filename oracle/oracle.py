@@ -33,6 +33,9 @@ class TraceParams(C.Structure):
         ("earth_radius_m", C.c_double), ("moment_Am2", C.c_double), ("m_hat", C.c_double * 3),
         ("B_ext_T", C.c_double * 3), ("field_model", C.c_int32), ("dt_selector", C.c_int32),
         ("policy", C.c_int32), ("pad_", C.c_int32), ("event_tol_m", C.c_double),
+        ("trap", C.c_int32), ("trap_min_mirror", C.c_int32), ("trap_min_bounce", C.c_int32), ("pad3_", C.c_int32),
+        ("trap_outer_margin_Re", C.c_double), ("trap_radial_tol_Re", C.c_double), ("trap_energy_tol", C.c_double),
+        ("trap_deadband", C.c_double),
     ]
 
 
@@ -83,6 +86,9 @@ def lib():
         l.orc_trace_batch.argtypes = [P(TraceParams), i64, P(dbl), P(dbl), P(dbl), vp, i32]
         l.orc_trace_batch_ex.argtypes = [P(TraceParams), i64, P(dbl), P(dbl), P(dbl), vp, vp, i32]
         l.orc_find_first_event.argtypes = [P(dbl), P(dbl), P(dbl), P(dbl), dbl, dbl, P(i32), P(dbl), P(dbl), P(i32)]
+        l.orc_trap_init.argtypes = [vp]
+        l.orc_trap_update.argtypes = [P(TraceParams), vp, P(dbl), P(dbl), P(dbl)]
+        l.orc_trap_update.restype = i32
         l.orc_finalize_step.argtypes = [dbl, dbl]
         l.orc_finalize_step.restype = dbl
         l.orc_rigidity_grid.argtypes = [dbl, dbl, i32, P(dbl)]
@@ -175,6 +181,8 @@ def ref():
         r.ref_step_checked_mover.restype = C.c_int
         r.ref_find_first_event.argtypes = [P(dbl), P(dbl), P(dbl), P(dbl), dbl, dbl, P(C.c_int), P(dbl), P(dbl),
                                            P(C.c_int)]
+        r.ref_trap_sequence.argtypes = [C.c_int, P(dbl), P(dbl), P(dbl), C.c_int, C.c_int, dbl, dbl, dbl, dbl,
+                                         P(dbl), P(dbl), P(C.c_int), P(C.c_int), P(C.c_int)]
         r.ref_finalize_step.argtypes = [dbl, dbl]
         r.ref_finalize_step.restype = dbl
         r.ref_band_topology.argtypes = [C.c_int, P(C.c_int), P(C.c_int)]
@@ -345,6 +353,26 @@ def find_first_event(a, z, bmin, bmax, r_inner, tol):
     lib().orc_find_first_event(_p(a, C.c_double), _p(z, C.c_double), _p(bmin, C.c_double), _p(bmax, C.c_double),
                                r_inner, tol, C.byref(t), C.byref(fr), _p(pos, C.c_double), C.byref(face))
     return t.value, fr.value, pos, face.value
+
+
+TRAP_DTYPE = np.dtype([("initialized", "<i4"), ("trapped", "<i4"), ("last_sign", "<i4"), ("mirror", "<i4"),
+                       ("bounce", "<i4"), ("stable", "<i4"), ("have_prev", "<i4"), ("pad_", "<i4"),
+                       ("p_ref", "<f8"), ("p_min", "<f8"), ("p_max", "<f8"), ("r_min", "<f8"), ("r_max", "<f8"),
+                       ("prev_r_min", "<f8"), ("prev_r_max", "<f8"), ("min_margin", "<f8")])
+
+
+def trap_sequence(params, x, p, B):
+    """The oracle's trap detector over consecutive (x, p, B) samples:
+    (trapped, mirror points, bounce cycles) after each sample."""
+    st = np.zeros(1, TRAP_DTYPE)
+    lib().orc_trap_init(st.ctypes.data_as(C.c_void_p))
+    out = np.zeros((len(x), 3), np.int32)
+    for i in range(len(x)):
+        xi, pi, bi = (np.ascontiguousarray(v[i], np.float64) for v in (x, p, B))
+        out[i, 0] = lib().orc_trap_update(C.byref(params), st.ctypes.data_as(C.c_void_p), _p(xi, C.c_double),
+                                          _p(pi, C.c_double), _p(bi, C.c_double))
+        out[i, 1], out[i, 2] = st["mirror"][0], st["bounce"][0]
+    return out
 
 
 def finalize_step(dt, time_remaining):

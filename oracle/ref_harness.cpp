@@ -9,6 +9,7 @@
 #include "GridlessParticleMovers.h"
 #include "TrajectoryBoundary.h"
 #include "TrajectoryTimeStep.h"
+#include "TrajectoryTrapDetector.h"
 
 namespace {
 struct DipolePlusUniform : public IGridlessFieldEvaluator {
@@ -113,6 +114,30 @@ void ref_find_first_event(const double a[3], const double z[3], const double bmi
   *fraction = e.fraction;
   for (int d = 0; d < 3; d++) pos[d] = e.position[d];
   *face = e.outerFace;
+}
+
+// TrajectoryTrap::Detector::Update over n consecutive (x, p, B) samples;
+// trapped[i], mirror[i], bounce[i] after sample i.
+void ref_trap_sequence(int n, const double* x, const double* p, const double* B, int min_mirror, int min_bounce,
+                       double outer_margin_m, double radial_tol_m, double energy_tol, double deadband,
+                       const double bmin[3], const double bmax[3], int* trapped, int* mirror, int* bounce) {
+  Earth::TrajectoryTrap::Config c;
+  c.enabled = true;
+  c.minMirrorPoints = min_mirror;
+  c.minBounceCycles = min_bounce;
+  c.outerMargin_m = outer_margin_m;
+  c.radialEnvelopeTolerance_m = radial_tol_m;
+  c.energyRelativeTolerance = energy_tol;
+  c.parallelDeadband = deadband;
+  Earth::TrajectoryBoundary::Box b;
+  for (int d = 0; d < 3; d++) b.min[d] = bmin[d], b.max[d] = bmax[d];
+  b.innerRadius = 0.0;
+  Earth::TrajectoryTrap::Detector det(c, b);
+  for (int i = 0; i < n; i++) {
+    trapped[i] = det.Update(x + 3 * i, p + 3 * i, B + 3 * i) ? 1 : 0;
+    mirror[i] = det.mirrorPoints();
+    bounce[i] = det.bounceCycles();
+  }
 }
 
 double ref_finalize_step(double dt, double time_remaining) {

@@ -201,6 +201,96 @@ def test_find_first_event_matches_reference_bitwise():
             np.isnan(O.finalize_step(dt, trem)) and np.isnan(r.ref_finalize_step(dt, trem)))
 
 
+def _bounce_orbit(P, n, pitch_deg=40.0, R_GV=0.05, dt=2e-3):
+    """A mirroring proton orbit at 3 Re in the dipole: (x, p, B) after each of
+    n fixed Boris steps of the oracle."""
+    D = O.C.c_double
+    x = np.array([3.0, 0.0, 0.0]) * TS.RE_AMPS
+    a = np.radians(pitch_deg)
+    p = np.array([0.0, np.sin(a), np.cos(a)]) * (R_GV * 1e9 * TS.QE / TS.C)
+    xs, ps, Bs = np.zeros((n, 3)), np.zeros((n, 3)), np.zeros((n, 3))
+    l = O.lib()
+    for i in range(n):
+        B = np.zeros(3)
+        l.orc_dipole_B(O.C.byref(P), O._p(x, D), O._p(B, D))
+        xs[i], ps[i], Bs[i] = x, p, B
+        l.orc_boris_step(O.C.byref(P), O._p(x, D), O._p(p, D), dt)
+    return xs, ps, Bs
+
+
+@needs_ref
+def test_trap_detector_matches_reference_bitwise():
+    """TrajectoryTrap::Detector of the compiled reference against the oracle
+    over a bounce orbit: trapped flag, mirror points and bounce cycles after
+    every sample, for configurations that trap, trap late, and fail on each
+    of the margin, radial-envelope, energy-spread and deadband conditions."""
+    P = TS.params()
+    xs, ps, Bs = _bounce_orbit(P, 24000)
+    # |p| modulated at 1e-3 on a copy: the energy-spread condition must fail
+    pw = ps * (1.0 + 1e-3 * np.sin(np.arange(len(ps)) * 0.01))[:, None]
+    cases = [  # (min mirror, min bounce, margin Re, radial tol Re, energy tol, deadband, momenta, trapped?)
+        (8, 4, 1.0, 0.05, 1e-4, 1e-6, ps, True),
+        (30, 14, 1.0, 0.05, 1e-4, 1e-6, ps, None),
+        (8, 4, 40.0, 0.05, 1e-4, 1e-6, ps, False),
+        (8, 4, 1.0, 1e-12, 1e-4, 1e-6, ps, False),
+        (8, 4, 1.0, 0.05, 1e-4, 1e-6, pw, False),
+        (8, 4, 1.0, 0.05, 1e-4, 0.6, ps, None),
+    ]
+    r = O.ref()
+    D, I = O.C.c_double, O.C.c_int
+    bmin = np.array(P.box_min_Re[:]) * P.earth_radius_m
+    bmax = np.array(P.box_max_Re[:]) * P.earth_radius_m
+    for mm, mb, mg, rt, et, db, pp, want in cases:
+        P.trap, P.trap_min_mirror, P.trap_min_bounce = 1, mm, mb
+        P.trap_outer_margin_Re, P.trap_radial_tol_Re, P.trap_energy_tol, P.trap_deadband = mg, rt, et, db
+        o = O.trap_sequence(P, xs, pp, Bs)
+        n = len(xs)
+        tr, mi, bo = np.zeros(n, np.int32), np.zeros(n, np.int32), np.zeros(n, np.int32)
+        pc = np.ascontiguousarray(pp)
+        r.ref_trap_sequence(n, O._p(xs, D), O._p(pc, D), O._p(Bs, D), mm, mb, mg * P.earth_radius_m,
+                            rt * P.earth_radius_m, et, db, O._p(bmin, D), O._p(bmax, D), O._p(tr, I), O._p(mi, I),
+                            O._p(bo, I))
+        assert np.array_equal(o[:, 0], tr) and np.array_equal(o[:, 1], mi) and np.array_equal(o[:, 2], bo), (mm, mg)
+        assert o[-1, 1] >= 8
+        if want is not None:
+            assert bool(o[-1, 0]) == want, (mm, mb, mg, rt, et, db)
+
+
+def test_trace_trap_terminations():
+    """With the detector on, mirroring orbits started at 2-4 Re end as
+    MagneticallyTrappedForbidden (code 2) under the energy-conserving Boris
+    mover; the same traces without it take more steps and mostly run to a
+    limit, and no trace that ends otherwise changes its outcome."""
+    from amps_amd import cutoff as CU
+    rng = np.random.default_rng(5)
+    n = 120
+    x = rng.normal(size=(n, 3))
+    x[:, 2] *= 0.3
+    x *= (rng.uniform(2.0, 4.0, n) * CU.RE_DIPOLE / np.linalg.norm(x, axis=1))[:, None]
+    d = rng.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1)[:, None]
+    R = np.exp(rng.uniform(np.log(0.01), np.log(1.0), n))
+    res = {}
+    for trap in (False, True):
+        P = O.TraceParams()
+        Pl = CU.make_params(trap=trap, max_steps=60000)
+        for name, _t in Pl._fields_:
+            v = getattr(Pl, name)
+            if hasattr(v, "__len__"):
+                for i in range(len(v)):
+                    getattr(P, name)[i] = v[i]
+            else:
+                setattr(P, name, v)
+        res[trap] = O.trace_batch(P, x, d, R)
+    t0, t1 = res[False]["termination"], res[True]["termination"]
+    assert (t1 == 2).sum() > 40 and (t0 == 2).sum() == 0
+    # without the detector most of them run to a limit (3 time, 4 distance, 5 steps)
+    assert np.isin(t0[t1 == 2], [3, 4, 5]).mean() > 0.8
+    same = t1 != 2
+    assert np.array_equal(t0[same], t1[same])
+    assert np.all(res[True]["steps"][t1 == 2] < res[False]["steps"][t1 == 2])
+
+
 def test_find_first_event_closed_forms():
     ev = O.find_first_event
     bmin, bmax = -np.ones(3), np.ones(3)

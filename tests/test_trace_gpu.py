@@ -373,3 +373,31 @@ def test_transmission_matches_oracle_counts(gpu_ctx):
     for k in range(9):
         assert np.array_equal(t["term_counts"][..., k], (term == k).sum(-1))
     assert t["T"].min() >= 0.0 and t["T"].max() <= 1.0 and t["T"].max() > 0.5 and t["T"].min() < 0.5
+
+
+@pytest.mark.parametrize("mover,policy", [("BORIS", 0), ("HC4", 1)])
+def test_trap_detector_traces_bitexact(gpu_ctx, mover, policy):
+    """MagneticallyTrappedForbidden: mirroring orbits started at 2-4 Re with
+    the trap detector on (TrajectoryTrap::Detector after every accepted
+    step), bit-exact with the oracle under the legacy and the structured
+    policy, with trapped, forbidden, allowed and limit terminations present."""
+    P = CU.make_params(trap=True, mover=mover, policy=policy, max_steps=60000)
+    rng = np.random.default_rng(5 + policy)
+    n = 600
+    x = rng.normal(size=(n, 3))
+    x[:, 2] *= 0.3
+    x *= (rng.uniform(2.0, 4.0, n) * CU.RE_DIPOLE / np.linalg.norm(x, axis=1))[:, None]
+    d = rng.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1)[:, None]
+    R = np.exp(rng.uniform(np.log(0.01), np.log(1.0), n))
+    print(f"trap trace case mover={mover} policy={policy} n={n}", flush=True)
+    if policy == 1:
+        g, ge, steps = CU.trace_batch_ex(gpu_ctx, P, x, d, R)
+        o, oe = O.trace_batch_ex(_oparams(P), x, d, R)
+        _same_exit(ge, oe)
+    else:
+        g, steps = CU.trace_batch(gpu_ctx, P, x, d, R, precision=L.FP64)
+        o = O.trace_batch(_oparams(P), x, d, R)
+    _same(g, o)
+    t = o["termination"]
+    assert (t == 2).sum() > 100 and (t == 1).sum() > 10 and (t == 0).sum() > 5
