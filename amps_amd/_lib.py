@@ -23,8 +23,8 @@ BC_DELETE = 0
 BC_REFLECT = 1
 FP64 = 0
 FP32 = 1
-K_PUSH, K_COMPACT, K_DEPOSIT_CELL, K_DEPOSIT_NODE, K_TRACE, K_MLP = range(6)
-K_COUNT = 6
+K_PUSH, K_COMPACT, K_DEPOSIT_CELL, K_DEPOSIT_NODE, K_TRACE, K_MLP, K_ECSIM = range(7)
+K_COUNT = 7
 
 
 class AmpsGpuError(RuntimeError):
@@ -132,6 +132,7 @@ def load():
         "amps_gpu_move_particles": (C.c_int, [vp, P(PushOpts)]),
         "amps_gpu_compact": (C.c_int, [vp]),
         "amps_gpu_update_j_mass_matrix": (C.c_int, [vp, dbl, P(dbl), P(dbl)]),
+        "amps_gpu_ecsim_mass_apply": (C.c_int, [vp, P(dbl), dbl, dbl, P(dbl), i32]),
         "amps_gpu_generate_particles": (C.c_int, [vp, i64, i32, P(dbl), P(dbl), C.c_uint64, i32]),
         "amps_gpu_run_steps": (C.c_int, [vp, P(PushOpts), dbl, i64, i32]),
         "amps_gpu_copy_probe": (C.c_int, [vp, i64, i32, P(dbl)]),

@@ -137,6 +137,7 @@ struct amps_ctx {
   // deposit
   double* d_cellpart = nullptr;
   double* d_nodeout = nullptr;
+  int* d_ecsim_nbr = nullptr;  // [nnodes][27] neighbour nodes of the ECSIM stencil, -1 outside
   // timing
   bool timing = false;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;

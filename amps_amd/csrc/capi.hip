@@ -115,6 +115,8 @@ static void free_mesh(amps_ctx* c) {
   (void)hipFree(c->d_tcount);
   (void)hipFree(c->d_cellpart);
   (void)hipFree(c->d_nodeout);
+  (void)hipFree(c->d_ecsim_nbr);
+  c->d_ecsim_nbr = nullptr;
   (void)hipFree(c->d_E);
   (void)hipFree(c->d_B);
   c->d_xmin = c->d_xmax = c->d_dxc = c->d_dxs = nullptr;

@@ -1,0 +1,141 @@
+// ecsim.hip — the mass-matrix term of the ECSIM field equation applied on
+// the device to a corner-node E, from the J/M that the deposit left
+// resident (one NODE_STRIDE row per unique node: J[3], M[243]).
+//
+// For node n and component c the reference accumulates, over its 27
+// neighbours ii = a(di) + 3a(dj) + 9a(dk), a(0,-1,+1) = (0,1,2)
+// (GetStencil, pic_field_solver_ecsim.cpp:1061-1074):
+//   res += (Ex[ii] M[9ii+3c] + Ey[ii] M[9ii+3c+1] + Ez[ii] M[9ii+3c+2]) * c1 * c2
+// with M indexed through MassMatrixOffsetTable (:621-639) and
+// c1 = -4 pi dt theta, c2 = E_conv in UpdateRhs_Legacy (:6116-6124).  The
+// same products are the mass part of every matrix element that
+// UpdateMatrixElement forms (:6004-6016), so this is also the mass part of
+// the operator the GMRES iterations apply.
+//
+// Neighbour ii of a node is the node one cell of the node's finest incident
+// cell size away on the level-lmax lattice (inside one block the block's own
+// cell size, as GetCornerNode(i+-1, j+-1, k+-1) of the owning block); a
+// neighbour outside the domain contributes nothing (a boundary row in the
+// reference's system).  The terms are summed in the reference's order, so
+// the result is bit-exact with it.  The M rows are read once by consecutive
+// lanes, the E of the neighbours through the caches: the kernel is HBM-bound
+// at 1944 + 108 + 24 + 24 bytes per node.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "amps_common.h"
+
+using namespace amps;
+
+namespace {
+
+__device__ __forceinline__ double lane_value(double v, int lane) {
+  const int lo = __builtin_amdgcn_readlane(__double2loint(v), lane);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), lane);
+  return __hiloint2double(hi, lo);
+}
+
+// Two nodes per wavefront: lane s (0..26) of each half reads slot s of its
+// node's M row (the row is read by consecutive lanes, 72 bytes each) and the
+// E of that slot's neighbour, and forms the three slot terms; the terms are
+// then added in slot order through lane reads, as the reference sums them.
+__global__ __launch_bounds__(256) void k_ecsim_mass(const double* __restrict__ rows, const int* __restrict__ nbr,
+                                                    const double* __restrict__ E, double c1, double c2,
+                                                    int64_t nnodes, double* __restrict__ out) {
+  const int lane = threadIdx.x & 63;
+  const int half = lane >> 5, s = lane & 31;
+  const int64_t n = ((int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * 2 + half;
+  const bool act = n < nnodes && s < 27;
+  double t[3] = {0.0, 0.0, 0.0};
+  int j = -1;
+  if (act) {
+    j = nbr[n * 27 + s];
+    if (j >= 0) {
+      const double* m = rows + n * NODE_STRIDE + 3 + 9 * s;
+      const double* e = E + 3 * (int64_t)j;
+      const double ex = e[0], ey = e[1], ez = e[2];
+#pragma unroll
+      for (int c = 0; c < 3; c++) t[c] = ((ex * m[3 * c] + ey * m[3 * c + 1]) + ez * m[3 * c + 2]) * c1 * c2;
+    }
+  }
+  const uint64_t present = __ballot(act && j >= 0);
+  double r0[3] = {0.0, 0.0, 0.0}, r1[3] = {0.0, 0.0, 0.0};
+  for (int q = 0; q < 27; q++) {
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      if ((present >> q) & 1) r0[c] += lane_value(t[c], q);
+      if ((present >> (32 + q)) & 1) r1[c] += lane_value(t[c], 32 + q);
+    }
+  }
+  if (s < 3 && n < nnodes) out[3 * n + s] = half ? (s == 0 ? r1[0] : s == 1 ? r1[1] : r1[2])
+                                                  : (s == 0 ? r0[0] : s == 1 ? r0[1] : r0[2]);
+}
+
+// neighbour table [nnodes][27] on the level-lmax lattice (-1: outside)
+int build_nbr(amps_ctx* c) {
+  if (c->d_ecsim_nbr) return 0;
+  const int nb = c->mv.nb, lmax = c->mv.lmax;
+  const int64_t S = ((int64_t)1 << lmax) + 1;
+  const int64_t nn = c->nnodes;
+  std::vector<int64_t> h((size_t)nn, INT64_MAX);
+  for (int b = 0; b < nb; b++) {
+    const int64_t cs = ((int64_t)1 << (lmax - c->h_level[(size_t)b])) / NC;
+    for (int q = 0; q < NCORNER; q++) {
+      int64_t& hn = h[(size_t)c->h_node_of_corner[(size_t)b * NCORNER + q]];
+      hn = std::min(hn, cs);
+    }
+  }
+  const std::vector<int64_t>& keys = c->h_node_keys;
+  std::vector<int> nbr((size_t)nn * 27, -1);
+  static const int A[3] = {0, -1, 1};
+  for (int64_t n = 0; n < nn; n++) {
+    const int64_t k = keys[(size_t)n];
+    const int64_t g[3] = {k % S, (k / S) % S, k / (S * S)};
+    for (int s = 0; s < 27; s++) {
+      const int64_t o[3] = {g[0] + A[s % 3] * h[(size_t)n], g[1] + A[(s / 3) % 3] * h[(size_t)n],
+                            g[2] + A[s / 9] * h[(size_t)n]};
+      if (o[0] < 0 || o[1] < 0 || o[2] < 0 || o[0] >= S || o[1] >= S || o[2] >= S) continue;
+      const int64_t kk = (o[2] * S + o[1]) * S + o[0];
+      const auto it = std::lower_bound(keys.begin(), keys.end(), kk);
+      if (it != keys.end() && *it == kk) nbr[(size_t)n * 27 + s] = (int)(it - keys.begin());
+    }
+  }
+  HIP_TRY(hipMalloc(&c->d_ecsim_nbr, sizeof(int) * std::max<size_t>(27, nbr.size())));
+  HIP_TRY(hipMemcpy(c->d_ecsim_nbr, nbr.data(), sizeof(int) * nbr.size(), hipMemcpyHostToDevice));
+  return 0;
+}
+
+}  // namespace
+
+extern "C" int amps_gpu_ecsim_mass_apply(amps_ctx* c, const double* E_nodes, double c1, double c2, double* out,
+                                         int32_t reps) {
+  if (!c) return fail(AMPS_E_ARG, "null context");
+  if (!c->has_mesh || !c->d_nodeout)
+    return fail(AMPS_E_STATE, "amps_gpu_ecsim_mass_apply: run amps_gpu_update_j_mass_matrix first");
+  if (!E_nodes || !out) return fail(AMPS_E_ARG, "amps_gpu_ecsim_mass_apply: null array");
+  if (reps < 1 || reps > 1000) return fail(AMPS_E_RANGE, "amps_gpu_ecsim_mass_apply: reps must be in 1..1000");
+  (void)hipSetDevice(c->device);
+  if (int r = build_nbr(c)) return r;
+  const int64_t nn = c->nnodes;
+  if (nn == 0) return AMPS_OK;
+  if (nn > ((int64_t)1 << 27)) return fail(AMPS_E_RANGE, "amps_gpu_ecsim_mass_apply: more than 2^27 nodes");
+  double *dE = nullptr, *dO = nullptr;
+  HIP_TRY(hipMalloc(&dE, sizeof(double) * 3 * nn));
+  HIP_TRY(hipMalloc(&dO, sizeof(double) * 3 * nn));
+  HIP_TRY(hipMemcpyAsync(dE, E_nodes, sizeof(double) * 3 * nn, hipMemcpyHostToDevice, c->stream));
+  const dim3 grid((unsigned)((nn + 7) / 8));  // 4 wavefronts of 2 nodes per workgroup
+  for (int r = 0; r < reps; r++) {
+    tbegin(c);
+    hipLaunchKernelGGL(k_ecsim_mass, grid, dim3(256), 0, c->stream, c->d_nodeout, c->d_ecsim_nbr, dE, c1, c2, nn,
+                       dO);
+    HIP_TRY(hipGetLastError());
+    tend(c, AMPS_K_ECSIM);
+  }
+  HIP_TRY(hipMemcpyAsync(out, dO, sizeof(double) * 3 * nn, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  (void)hipFree(dE);
+  (void)hipFree(dO);
+  return AMPS_OK;
+}

@@ -168,3 +168,18 @@ class PicState:
         L.check(self.lib.amps_gpu_update_j_mass_matrix(self.ctx.h, dt, L._ptr(J, C.c_double),
                                                        L._ptr(M, C.c_double)), "deposit")
         return J, M
+
+    def MassMatrixApply(self, E_nodes, c1, c2=1.0, reps=1):
+        """The ECSIM mass-matrix term over the resident M of the last
+        UpdateJMassMatrix (UpdateRhs_Legacy, pic_field_solver_ecsim.cpp:
+        6116-6124): out[n, c] = sum over the 27 neighbours of
+        (E . M[n, slot, c, :]) * c1 * c2, with c1 = -4 pi dt theta and
+        c2 = E_conv in the reference.  E_nodes is [nnodes, 3] in node-key
+        order."""
+        E = np.ascontiguousarray(E_nodes, np.float64)
+        if E.shape != (self.nnodes, 3):
+            raise ValueError("E_nodes must be [nnodes, 3]")
+        out = np.zeros((self.nnodes, 3))
+        L.check(self.lib.amps_gpu_ecsim_mass_apply(self.ctx.h, L._ptr(E, C.c_double), float(c1), float(c2),
+                                                   L._ptr(out, C.c_double), int(reps)), "ecsim_mass_apply")
+        return out

@@ -24,6 +24,7 @@ PUSH_BYTES = 104          # per particle-step: x,v read (48) + write (48) + ids 
 PUSH_BYTES_F32 = 80       # FP32 velocities: x read+write (48) + v read+write (24) + ids (8)
 DEPOSIT_BYTES_P = 48      # per particle: x, v read
 NODE_BYTES = 246 * 8      # per unique node: J (3) + mass matrix (243) written once
+ECSIM_NODE_BYTES = 243 * 8 + 27 * 4 + 24 + 24  # mass-term: M row, neighbour ids, own E once, result
 
 
 COPY_PROBE_BYTES = 1 << 30
@@ -31,7 +32,8 @@ COPY_PROBE_BYTES = 1 << 30
 PMC_KERNELS = (("k_push_f64", "push"), ("k_push_f32", "push"), ("k_deposit_cell", "deposit_cell"),
                ("k_deposit_cell_rec", "deposit_cell"), ("k_deposit_cell_mfma", "deposit_cell"),
                ("k_deposit_node", "deposit_node"), ("k_deposit_node_staged", "deposit_node"),
-               ("k_scatter", "scatter"), ("k_count_keys", "count_keys"), ("k_copy_probe", "copy"))
+               ("k_scatter", "scatter"), ("k_count_keys", "count_keys"), ("k_copy_probe", "copy"),
+               ("k_ecsim_mass", "ecsim_mass"))
 
 
 def _pmc_pass(args, tune, counter, outdir):
@@ -410,6 +412,7 @@ def main():
     if args.pmc_child:
         # one eager step and the copy kernel of known size, for the counter passes
         st.run_steps(1, dt, use_graph=False, **opts)
+        st.MassMatrixApply(np.random.default_rng(7).normal(size=(st.nnodes, 3)), -4 * np.pi * dt * 0.5)
         ctx.copy_probe(COPY_PROBE_BYTES, reps=1)
         ctx.sync()
         ctx.close()
@@ -467,6 +470,25 @@ def main():
                 "achieved": round(dep_gbps, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                 "frac": round(dep_gbps / HBM_PEAK_GBPS, 4), "traffic": tr,
                 "algorithmic_bytes": int(dep_bytes)}
+    # the ECSIM mass-matrix term over the resident M of the last deposit
+    # (27-neighbour stencil, 243 entries per node): HIP-event time per launch
+    ecsim = None
+    try:
+        En = np.random.default_rng(7).normal(size=(nnodes, 3))
+        st.MassMatrixApply(En, -4 * np.pi * dt * 0.5)  # builds the neighbour table once
+        ctx.set_timing(True)
+        ctx.reset_timing()
+        st.MassMatrixApply(En, -4 * np.pi * dt * 0.5, reps=20)
+        ctx.set_timing(False)
+        mse, cnte = ctx.timing()
+        e_ms = mse[L.K_ECSIM] / max(cnte[L.K_ECSIM], 1)
+        e_bytes = nnodes * ECSIM_NODE_BYTES
+        ecsim = {"kernel": "k_ecsim_mass", "ms": round(e_ms, 4), "nodes": int(nnodes),
+                 "traffic": traffic.get("ecsim_mass") if traffic else None,
+                 "algorithmic_bytes": int(e_bytes), "GBps": round(e_bytes / (e_ms * 1e-3) / 1e9, 1),
+                 "frac_of_hbm_peak": round(e_bytes / (e_ms * 1e-3) / 1e9 / HBM_PEAK_GBPS, 4)}
+    except Exception as e:  # the particle-step line stays valid
+        ecsim = {"error": repr(e)}
     # the same step with the FP32 push (FP32 stencil, fields and velocities,
     # FP64 positions and FP64 deposit): same population, regenerated
     st.generate_particles(nps, [0.05, 0.005], drift=(0.01, 0.0, 0.0), seed=4, precision=L.FP32)
@@ -519,6 +541,7 @@ def main():
         "autotune": tune,
         "traffic_bytes_per_launch": traffic,
         "fp32_push_variant": fp32_variant,
+        "ecsim_mass_term": ecsim,
     }
     if args.workload == "all":
         try:

@@ -224,6 +224,18 @@ int amps_gpu_copy_probe(amps_ctx* ctx, int64_t nbytes, int32_t reps, double* ms)
  * the dense [node][246] result stays resident either way. */
 int amps_gpu_update_j_mass_matrix(amps_ctx* ctx, double dt, double* J_out, double* M_out);
 
+/* The mass-matrix term of the ECSIM field equation over the resident M of
+ * the last amps_gpu_update_j_mass_matrix: for every node n and component c,
+ *   out[3n+c] = sum over slots s of ((Ex M[9s+3c] + Ey M[9s+3c+1]) + Ez M[9s+3c+2]) * c1 * c2
+ * with E of the slot's neighbour node (UpdateRhs_Legacy mass part,
+ * pic_field_solver_ecsim.cpp:6116-6124, c1 = -4 pi dt theta, c2 = E_conv;
+ * the neighbour order of GetStencil :1061-1074).  E_nodes and out are
+ * [nnodes][3] in node-key order.  A neighbour is one cell of the node's
+ * finest incident cell size away; one outside the domain contributes 0.
+ * reps > 1 repeats the launch (for timing). */
+int amps_gpu_ecsim_mass_apply(amps_ctx* ctx, const double* E_nodes, double c1, double c2, double* out,
+                              int32_t reps);
+
 /* ---- gridless cutoff tracer ------------------------------------------ */
 /* Parameters of one trace (SI).  The FP64 path is TraceTrajectoryImpl +
  * TraceTrajectoryWithSingleRetry (CutoffRigidityGridless.cpp:1246-1496) with
@@ -414,7 +426,8 @@ int amps_gpu_mlp_predict(amps_ctx* ctx, int64_t n, const float* X, float* y);
 #define AMPS_K_DEPOSIT_NODE 3
 #define AMPS_K_TRACE 4
 #define AMPS_K_MLP 5
-#define AMPS_K_COUNT 6
+#define AMPS_K_ECSIM 6
+#define AMPS_K_COUNT 7
 int amps_gpu_set_timing(amps_ctx* ctx, int32_t enable);
 int amps_gpu_get_timing(amps_ctx* ctx, double* ms /* AMPS_K_COUNT */, int64_t* launches);
 int amps_gpu_reset_timing(amps_ctx* ctx);

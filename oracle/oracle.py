@@ -316,6 +316,48 @@ def push(omesh, E, B, species, parts, mover=0, bc_mode=1, r_sphere=0.0, nthreads
     return p, code
 
 
+def ecsim_neighbours(mesh, node_of_corner, keys):
+    """[nnodes][27] neighbour nodes of the ECSIM corner stencil, -1 outside:
+    slot ii = a(di) + 3a(dj) + 9a(dk), a(0,-1,+1) = (0,1,2), one cell of the
+    node's finest incident cell size away (GetStencil,
+    pic_field_solver_ecsim.cpp:1061-1074)."""
+    S = (1 << mesh.lmax) + 1
+    nn = len(keys)
+    h = np.full(nn, np.iinfo(np.int64).max, np.int64)
+    cs = (np.int64(1) << (mesh.lmax - mesh.level.astype(np.int64))) // 8
+    np.minimum.at(h, np.asarray(node_of_corner).reshape(-1, 729), cs[:, None])
+    g = np.stack([keys % S, (keys // S) % S, keys // (S * S)], 1)
+    a = np.array([0, -1, 1])
+    nbr = -np.ones((nn, 27), np.int64)
+    for s in range(27):
+        off = np.array([a[s % 3], a[(s // 3) % 3], a[s // 9]])
+        o = g + off[None, :] * h[:, None]
+        inside = ((o >= 0) & (o < S)).all(1)
+        kk = (o[:, 2] * S + o[:, 1]) * S + o[:, 0]
+        pos = np.minimum(np.searchsorted(keys, kk), nn - 1)
+        found = inside & (keys[pos] == kk)
+        nbr[found, s] = pos[found]
+    return nbr
+
+
+def ecsim_mass_apply(nbr, Mm, E, c1, c2):
+    """The mass-matrix part of UpdateRhs_Legacy (pic_field_solver_ecsim.cpp:
+    6116-6124): res_c += (Ex M[9ii+3c] + Ey M[9ii+3c+1] + Ez M[9ii+3c+2])
+    * c1 * c2 over the 27 neighbours in slot order (MassMatrixOffsetTable,
+    :621-639), absent neighbours skipped."""
+    nn = len(Mm)
+    out = np.zeros((nn, 3))
+    for s in range(27):
+        j = nbr[:, s]
+        ok = j >= 0
+        e = E[np.where(ok, j, 0)]
+        for c in range(3):
+            m = Mm[:, 9 * s + 3 * c:9 * s + 3 * c + 3]
+            term = ((e[:, 0] * m[:, 0] + e[:, 1] * m[:, 1]) + e[:, 2] * m[:, 2]) * c1 * c2
+            out[:, c] = np.where(ok, out[:, c] + term, out[:, c])
+    return out
+
+
 def deposit(omesh, B, species, dt, parts, node_of_corner, nnodes, nthreads=0):
     J = np.zeros((nnodes, 3))
     M = np.zeros((nnodes, 243))

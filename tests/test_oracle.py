@@ -686,6 +686,74 @@ def test_deposit_symmetry_and_linearity(amr):
     assert np.allclose(M1 + M2, Mm, rtol=1e-12, atol=1e-12 * np.abs(Mm).max())
 
 
+def test_ecsim_mass_apply_single_particle_closed_form():
+    """One particle in a dipole B (alpha not symmetric): the mass matrix is
+    M_ij = (q h / V) W_i W_j alpha, so the mass term at node i is
+    (q h / V) W_i alpha E(x_p) c1 c2 with E(x_p) the trilinear interpolation
+    of the node E, and zero at every node the particle's cell does not
+    touch.  Pins the neighbour slots, the row/column order of the 3x3
+    blocks and the coefficients."""
+    m = M.uniform_mesh(1)
+    om = O.Mesh(m)
+    E, B = M.corner_fields(m, bz0=0.3)
+    noc, keys = om.build_nodes()
+    x = np.array([[0.123], [-0.456], [0.789]])
+    blk = m.locate(x.T)
+    parts = M.Particles(x, np.array([[0.01], [0.02], [-0.03]]), [0], blk, M.cell_of(m, x, blk))
+    sp = Species([-1.0], [0.01], [1.0], [1.0])
+    dt = 1e-3
+    J, Mm = O.deposit(om, B, sp, dt, parts, noc, len(keys))
+    xs, w, t, c = om.stencil(x[:, 0], int(blk[0]))
+    Bi = (w[:, None] * B[blk[0]][c]).sum(0)
+    q, mass = -1.0, 0.01
+    h = q * dt / (2 * mass)
+    c0 = 1 / (1 + h * h * Bi @ Bi)
+    alpha = c0 * (np.eye(3) + h * h * np.outer(Bi, Bi) - h * np.array([[0, -Bi[2], Bi[1]], [Bi[2], 0, -Bi[0]], [-Bi[1], Bi[0], 0]]))
+    assert np.abs(alpha - alpha.T).max() > 1e-3 * np.abs(alpha).max()
+    V = np.prod((m.blk_xmax[blk[0]] - m.blk_xmin[blk[0]]) / 8)
+    rng = np.random.default_rng(3)
+    En = rng.normal(size=(len(keys), 3))
+    nodes = noc[int(blk[0]) * 729 + c]
+    Ep = (w[:, None] * En[nodes]).sum(0)
+    c1, c2 = -4 * np.pi * dt * 0.5, 3.3
+    out = O.ecsim_mass_apply(O.ecsim_neighbours(m, noc, keys), Mm, En, c1, c2)
+    want = np.zeros_like(out)
+    for wi, ni in zip(w, nodes):
+        want[ni] = (q * h / V) * wi * (alpha @ Ep) * c1 * c2
+    assert np.allclose(out, want, rtol=1e-11, atol=1e-13 * np.abs(want).max())
+    assert np.count_nonzero(np.abs(out).sum(1)) == 8
+
+
+def test_ecsim_neighbours_amr(amr):
+    """On the two-level mesh: every slot's neighbour sits at the slot offset
+    times the node's finest incident cell size; the centre slot is the node
+    itself; a node and its neighbour see each other in opposite slots when
+    both have the same cell size."""
+    m, om = amr
+    noc, keys = om.build_nodes()
+    nbr = O.ecsim_neighbours(m, noc, keys)
+    assert np.array_equal(nbr[:, 0], np.arange(len(keys)))
+    S = (1 << m.lmax) + 1
+    g = np.stack([keys % S, (keys // S) % S, keys // (S * S)], 1)
+    cs = (np.int64(1) << (m.lmax - m.level.astype(np.int64))) // 8
+    hmin = np.full(len(keys), np.iinfo(np.int64).max)
+    for b in range(m.nblocks):
+        hmin[noc[b * 729:(b + 1) * 729]] = np.minimum(hmin[noc[b * 729:(b + 1) * 729]], cs[b])
+    a = [0, -1, 1]
+    inv = {0: 0, -1: 1, 1: 2}
+    nrec = 0
+    for s in range(27):
+        off = np.array([a[s % 3], a[(s // 3) % 3], a[s // 9]])
+        j = nbr[:, s]
+        ok = j >= 0
+        assert np.array_equal(g[j[ok]], g[ok] + off * hmin[ok, None])
+        s2 = inv[-off[0]] + 3 * inv[-off[1]] + 9 * inv[-off[2]]
+        same = ok & (hmin == hmin[np.where(ok, j, 0)])
+        assert np.array_equal(nbr[j[same], s2], np.flatnonzero(same))
+        nrec += same.sum()
+    assert nrec > 10000 and (nbr < 0).any() and len(np.unique(hmin)) == 2
+
+
 def test_deposit_mass_matrix_symmetry():
     """On a one-level mesh, node a's block for neighbour b equals node b's block
     for neighbour a (same 3x3 block scattered to both, :2398-2426)."""

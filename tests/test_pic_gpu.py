@@ -373,3 +373,27 @@ def test_restart_records_round_trip(gpu_ctx, layout_name, weight):
     h, _ = st2.get_particles()
     assert np.array_equal(h.x, g.x) and np.array_equal(h.v, g.v) and np.array_equal(h.spec, g.spec)
     assert np.array_equal(h.blk, g.blk) and np.array_equal(h.cell, g.cell)
+
+
+def test_ecsim_mass_apply_bitexact(gpu_ctx, dep_case):
+    """The ECSIM mass-matrix term over the resident M of the deposit,
+    bit-exact with the oracle's restatement over the same M (two-level mesh:
+    resolution-change and boundary nodes included); linear in E; repeated
+    launches give the same result."""
+    mesh, E, B, parts, sp, dt = dep_case
+    print(f"ecsim mass term case n={parts.n}", flush=True)
+    st = PicState(mesh, E, B, sp, ctx=gpu_ctx)
+    st.set_particles(parts)
+    J, Mm = st.UpdateJMassMatrix(dt)
+    noc, keys = st.node_map()
+    nbr = O.ecsim_neighbours(mesh, noc, keys)
+    rng = np.random.default_rng(11)
+    E1, E2 = rng.normal(size=(2, len(keys), 3))
+    c1, c2 = -4 * np.pi * dt * 0.51, 1.7
+    g1 = st.MassMatrixApply(E1, c1, c2)
+    assert np.array_equal(g1, O.ecsim_mass_apply(nbr, Mm, E1, c1, c2))
+    g2 = st.MassMatrixApply(E2, c1, c2, reps=3)
+    assert np.array_equal(g2, O.ecsim_mass_apply(nbr, Mm, E2, c1, c2))
+    g12 = st.MassMatrixApply(E1 + E2, c1, c2)
+    assert np.allclose(g12, g1 + g2, rtol=1e-12, atol=1e-12 * np.abs(g12).max())
+    assert np.count_nonzero(g1) > 0.9 * g1.size
