@@ -31,27 +31,24 @@ using namespace amps;
 
 namespace {
 
-__device__ __forceinline__ double lane_value(double v, int lane) {
-  const int lo = __builtin_amdgcn_readlane(__double2loint(v), lane);
-  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), lane);
-  return __hiloint2double(hi, lo);
-}
-
-// Two nodes per wavefront: lane s (0..26) of each half reads slot s of its
-// node's M row (the row is read by consecutive lanes, 72 bytes each) and the
-// E of that slot's neighbour, and forms the three slot terms; the terms are
-// then added in slot order through lane reads, as the reference sums them.
+// Eight nodes per workgroup, two per wavefront: lane s (0..26) of each half
+// reads slot s of its node's M row (consecutive lanes, 72 bytes each) and
+// the E of that slot's neighbour, and writes the three slot terms to LDS;
+// then 24 lanes, one per (node, component), add the 27 terms in slot order,
+// as the reference sums them.
 __global__ __launch_bounds__(256) void k_ecsim_mass(const double* __restrict__ rows, const int* __restrict__ nbr,
                                                     const double* __restrict__ E, double c1, double c2,
                                                     int64_t nnodes, double* __restrict__ out) {
-  const int lane = threadIdx.x & 63;
+  __shared__ double term[8][27][3];
+  __shared__ uint64_t present[4];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int half = lane >> 5, s = lane & 31;
-  const int64_t n = ((int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * 2 + half;
+  const int64_t n = (int64_t)blockIdx.x * 8 + wave * 2 + half;
   const bool act = n < nnodes && s < 27;
-  double t[3] = {0.0, 0.0, 0.0};
   int j = -1;
   if (act) {
     j = nbr[n * 27 + s];
+    double t[3] = {0.0, 0.0, 0.0};
     if (j >= 0) {
       const double* m = rows + n * NODE_STRIDE + 3 + 9 * s;
       const double* e = E + 3 * (int64_t)j;
@@ -59,18 +56,23 @@ __global__ __launch_bounds__(256) void k_ecsim_mass(const double* __restrict__ r
 #pragma unroll
       for (int c = 0; c < 3; c++) t[c] = ((ex * m[3 * c] + ey * m[3 * c + 1]) + ez * m[3 * c + 2]) * c1 * c2;
     }
-  }
-  const uint64_t present = __ballot(act && j >= 0);
-  double r0[3] = {0.0, 0.0, 0.0}, r1[3] = {0.0, 0.0, 0.0};
-  for (int q = 0; q < 27; q++) {
 #pragma unroll
-    for (int c = 0; c < 3; c++) {
-      if ((present >> q) & 1) r0[c] += lane_value(t[c], q);
-      if ((present >> (32 + q)) & 1) r1[c] += lane_value(t[c], 32 + q);
+    for (int c = 0; c < 3; c++) term[wave * 2 + half][s][c] = t[c];
+  }
+  const uint64_t pm = __ballot(act && j >= 0);
+  if (lane == 0) present[wave] = pm;
+  __syncthreads();
+  if (threadIdx.x < 24) {
+    const int ln = threadIdx.x / 3, c = threadIdx.x - 3 * ln;
+    const int64_t nn = (int64_t)blockIdx.x * 8 + ln;
+    if (nn < nnodes) {
+      const uint64_t p = present[ln >> 1] >> (32 * (ln & 1));
+      double res = 0.0;
+      for (int q = 0; q < 27; q++)
+        if ((p >> q) & 1) res += term[ln][q][c];
+      out[3 * nn + c] = res;
     }
   }
-  if (s < 3 && n < nnodes) out[3 * n + s] = half ? (s == 0 ? r1[0] : s == 1 ? r1[1] : r1[2])
-                                                  : (s == 0 ? r0[0] : s == 1 ? r0[1] : r0[2]);
 }
 
 // neighbour table [nnodes][27] on the level-lmax lattice (-1: outside)
