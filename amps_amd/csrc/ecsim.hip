@@ -41,9 +41,15 @@ __global__ __launch_bounds__(256) void k_ecsim_mass(const double* __restrict__ r
                                                     int64_t nnodes, double* __restrict__ out) {
   __shared__ double term[8][27][3];
   __shared__ uint64_t present[4];
+  // workgroups go round-robin over the 8 XCDs: give each XCD one contiguous
+  // eighth of the nodes, so the neighbours' E stay in that XCD's L2 (the grid
+  // is a multiple of 8; a whole workgroup past the end leaves at once)
+  const int64_t per = gridDim.x >> 3;
+  const int64_t chunk = (int64_t)(blockIdx.x & 7) * per + (blockIdx.x >> 3);
+  if (chunk * 8 >= nnodes) return;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int half = lane >> 5, s = lane & 31;
-  const int64_t n = (int64_t)blockIdx.x * 8 + wave * 2 + half;
+  const int64_t n = chunk * 8 + wave * 2 + half;
   const bool act = n < nnodes && s < 27;
   int j = -1;
   if (act) {
@@ -64,7 +70,7 @@ __global__ __launch_bounds__(256) void k_ecsim_mass(const double* __restrict__ r
   __syncthreads();
   if (threadIdx.x < 24) {
     const int ln = threadIdx.x / 3, c = threadIdx.x - 3 * ln;
-    const int64_t nn = (int64_t)blockIdx.x * 8 + ln;
+    const int64_t nn = chunk * 8 + ln;
     if (nn < nnodes) {
       const uint64_t p = present[ln >> 1] >> (32 * (ln & 1));
       double res = 0.0;
@@ -127,7 +133,8 @@ extern "C" int amps_gpu_ecsim_mass_apply(amps_ctx* c, const double* E_nodes, dou
   HIP_TRY(hipMalloc(&dE, sizeof(double) * 3 * nn));
   HIP_TRY(hipMalloc(&dO, sizeof(double) * 3 * nn));
   HIP_TRY(hipMemcpyAsync(dE, E_nodes, sizeof(double) * 3 * nn, hipMemcpyHostToDevice, c->stream));
-  const dim3 grid((unsigned)((nn + 7) / 8));  // 4 wavefronts of 2 nodes per workgroup
+  // 4 wavefronts of 2 nodes per workgroup, the workgroup count rounded up to 8
+  const dim3 grid((unsigned)((((nn + 7) / 8) + 7) / 8 * 8));
   for (int r = 0; r < reps; r++) {
     tbegin(c);
     hipLaunchKernelGGL(k_ecsim_mass, grid, dim3(256), 0, c->stream, c->d_nodeout, c->d_ecsim_nbr, dE, c1, c2, nn,
