@@ -397,3 +397,38 @@ def test_ecsim_mass_apply_bitexact(gpu_ctx, dep_case):
     g12 = st.MassMatrixApply(E1 + E2, c1, c2)
     assert np.allclose(g12, g1 + g2, rtol=1e-12, atol=1e-12 * np.abs(g12).max())
     assert np.count_nonzero(g1) > 0.9 * g1.size
+
+
+def test_ecsim_mass_apply_single_particle_closed_form(gpu_ctx):
+    """Deposit and mass term both on the device for one particle in a dipole
+    B: (q h / V) W_i alpha E(x_p) c1 c2 at the 8 corners of its cell, zero
+    elsewhere (alpha = (I + h^2 B B^T - h [B]x) / (1 + h^2 |B|^2),
+    h = q dt / 2m)."""
+    mesh = M.uniform_mesh(1)
+    E, B = M.corner_fields(mesh, bz0=0.3)
+    x = np.array([[0.123], [-0.456], [0.789]])
+    blk = mesh.locate(x.T)
+    parts = M.Particles(x, np.array([[0.01], [0.02], [-0.03]]), [0], blk, M.cell_of(mesh, x, blk))
+    q, mass, dt = -1.0, 0.01, 1e-3
+    sp = Species([q], [mass], [dt], [1.0])
+    print("ecsim single-particle closed form", flush=True)
+    st = PicState(mesh, E, B, sp, ctx=gpu_ctx)
+    st.set_particles(parts)
+    st.UpdateJMassMatrix(dt, copy_out=False)
+    noc, keys = st.node_map()
+    xs, w, t, c = O.Mesh(mesh).stencil(x[:, 0], int(blk[0]))  # trilinear weights of the corners c
+    Bi = (w[:, None] * B[blk[0]][c]).sum(0)
+    h = q * dt / (2 * mass)
+    Bx = np.array([[0, -Bi[2], Bi[1]], [Bi[2], 0, -Bi[0]], [-Bi[1], Bi[0], 0]])
+    alpha = (np.eye(3) + h * h * np.outer(Bi, Bi) - h * Bx) / (1 + h * h * Bi @ Bi)
+    V = np.prod((mesh.blk_xmax[blk[0]] - mesh.blk_xmin[blk[0]]) / 8)
+    En = np.random.default_rng(5).normal(size=(len(keys), 3))
+    nodes = noc[int(blk[0]) * 729 + c]
+    Ep = (w[:, None] * En[nodes]).sum(0)
+    c1, c2 = -4 * np.pi * dt * 0.5, 2.0
+    out = st.MassMatrixApply(En, c1, c2)
+    want = np.zeros_like(out)
+    for wi, ni in zip(w, nodes):
+        want[ni] = (q * h / V) * wi * (alpha @ Ep) * c1 * c2
+    assert np.allclose(out, want, rtol=1e-11, atol=1e-13 * np.abs(want).max())
+    assert np.count_nonzero(np.abs(out).sum(1)) == 8
