@@ -16,11 +16,13 @@ CODE_REFLECTED = -1
 CODE_LEFT_DOMAIN = 2
 CODE_FINISHED = 3
 CODE_SPHERE = 5
+CODE_EXIT_FACE0 = 8  # BC_USER: left through face (code - 8), queued for ProcessOutsideDomainParticles
 MOVER_LAPENTA = 0
 MOVER_BORIS = 1
 MOVER_RELATIVISTIC = 2
 BC_DELETE = 0
 BC_REFLECT = 1
+BC_USER = 2
 FP64 = 0
 FP32 = 1
 K_PUSH, K_COMPACT, K_DEPOSIT_CELL, K_DEPOSIT_NODE, K_TRACE, K_MLP, K_ECSIM = range(7)
@@ -131,8 +133,10 @@ def load():
                                              P(i64), P(i32)]),
         "amps_gpu_move_particles": (C.c_int, [vp, P(PushOpts)]),
         "amps_gpu_compact": (C.c_int, [vp]),
+        "amps_gpu_get_exits": (C.c_int, [vp, i64, P(i64), P(i64), P(dbl), P(dbl), P(i32), P(i32), P(i32), i32]),
         "amps_gpu_update_j_mass_matrix": (C.c_int, [vp, dbl, P(dbl), P(dbl)]),
-        "amps_gpu_ecsim_mass_apply": (C.c_int, [vp, P(dbl), dbl, dbl, P(dbl), i32]),
+        "amps_gpu_ecsim_mass_apply": (C.c_int, [vp, P(dbl), dbl, dbl, P(dbl), i32, i32]),
+        "amps_gpu_particle_energy_cfl": (C.c_int, [vp, dbl, P(dbl), P(dbl)]),
         "amps_gpu_generate_particles": (C.c_int, [vp, i64, i32, P(dbl), P(dbl), C.c_uint64, i32]),
         "amps_gpu_run_steps": (C.c_int, [vp, P(PushOpts), dbl, i64, i32]),
         "amps_gpu_copy_probe": (C.c_int, [vp, i64, i32, P(dbl)]),
@@ -155,6 +159,9 @@ def load():
                                                 P(i32), P(i32), P(i64)]),
         "amps_gpu_band_topology": (C.c_int, [i64, i32, P(C.c_int8), P(i32)]),
         "amps_gpu_trace_set_param": (C.c_int, [vp, C.c_char_p, i64]),
+        "amps_gpu_uniform_sphere_dirs": (C.c_int, [i32, P(dbl)]),
+        "amps_gpu_local_dirs": (C.c_int, [i64, P(dbl), i32, P(dbl), P(dbl), P(dbl), P(dbl)]),
+        "amps_gpu_local_dirs_paired": (C.c_int, [i64, P(dbl), P(dbl), P(dbl), P(dbl)]),
         "amps_gpu_mlp_init": (C.c_int, [vp, P(C.c_float)]),
         "amps_gpu_mlp_get_params": (C.c_int, [vp, P(C.c_float), P(i64)]),
         "amps_gpu_mlp_set_data": (C.c_int, [vp, i64, P(C.c_float), P(C.c_float)]),

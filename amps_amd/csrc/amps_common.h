@@ -138,6 +138,15 @@ struct amps_ctx {
   double* d_cellpart = nullptr;
   double* d_nodeout = nullptr;
   int* d_ecsim_nbr = nullptr;  // [nnodes][27] neighbour nodes of the ECSIM stencil, -1 outside
+  // exit list of the USER_FUNCTION boundary mode (exits.hip)
+  int64_t exit_cap = (int64_t)1 << 20;  // records the list holds
+  int64_t exit_alloc = 0;               // records allocated
+  int64_t* d_ex_id = nullptr;           // [exit_alloc]
+  double* d_ex_x = nullptr;             // [exit_alloc][3]
+  double* d_ex_v = nullptr;             // [exit_alloc][3]
+  int* d_ex_meta = nullptr;             // [exit_alloc][3]: species, face, block
+  int* d_ex_wg = nullptr;               // per-workgroup exit counts, then offsets
+  int64_t* d_ex_state = nullptr;        // [0] records in the list, [1] records dropped
   // timing
   bool timing = false;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -168,6 +177,11 @@ int launch_deposit_cell_rec(amps_ctx* c, double dt);
 int launch_deposit_cell_mfma(amps_ctx* c, double dt);
 int alloc_particles(amps_ctx* c, int64_t n);  // both particle sets for at least n
 int build_tiles(amps_ctx* c);                 // push tiles from the current offsets
+// append the exits of the last push (codes AMPS_CODE_EXIT_FACE0..+5) to the
+// exit list (exits.hip); no host synchronisation, capturable
+int collect_exits(amps_ctx* c);
+int prepare_exits(amps_ctx* c);  // allocates the list (outside any capture)
+void exits_free(amps_ctx* c);
 }  // namespace amps
 
 #define HIP_TRY(expr)                                    \

@@ -155,6 +155,7 @@ extern "C" int amps_gpu_destroy(amps_ctx* c) {
   free_mesh(c);
   trace_free(c);
   mlp_free(c);
+  exits_free(c);
   (void)hipFree(c->d_n);
   (void)hipFree(c->d_ntiles);
   (void)hipFree(c->d_tiles);
@@ -217,6 +218,10 @@ extern "C" int amps_gpu_set_param(amps_ctx* c, const char* name, int64_t value) 
   } else if (n == "node_batch") {
     if (value < 0 || value > 64) return fail(AMPS_E_ARG, "node_batch must be in [0, 64]");
     c->node_batch = (int)value;
+  } else if (n == "exit_capacity") {
+    if (value < 1 || value > (1 << 30)) return fail(AMPS_E_ARG, "exit_capacity must be in [1, 2^30]");
+    // the list is reallocated (emptied) before the next USER_FUNCTION push
+    c->exit_cap = value;
   } else if (n == "deposit_exact") {
     c->dep_exact = value ? 1 : 0;
   } else if (n == "b_center") {
@@ -819,10 +824,12 @@ extern "C" int amps_gpu_move_particles(amps_ctx* c, const amps_push_opts* o) {
   if (!c->has_mesh || !c->has_fields || !c->has_species)
     return fail(AMPS_E_STATE, "amps_gpu_move_particles: mesh, fields and species are required");
   if (int r = check_mover(c, o, "amps_gpu_move_particles")) return r;
-  if (o->bc_mode != AMPS_BC_DELETE && o->bc_mode != AMPS_BC_REFLECT)
+  if (o->bc_mode != AMPS_BC_DELETE && o->bc_mode != AMPS_BC_REFLECT && o->bc_mode != AMPS_BC_USER)
     return fail(AMPS_E_ARG, "amps_gpu_move_particles: unknown boundary mode");
   if (c->cap == 0) return AMPS_OK;
   (void)hipSetDevice(c->device);
+  if (o->bc_mode == AMPS_BC_USER)
+    if (int r = prepare_exits(c)) return r;
   if (o->tile > 0 && o->tile != c->tile) {
     if (int r = amps_gpu_set_param(c, "tile", o->tile)) return r;
   }
@@ -836,6 +843,8 @@ extern "C" int amps_gpu_move_particles(amps_ctx* c, const amps_push_opts* o) {
   tend(c, AMPS_K_PUSH);
   c->offsets_valid = false;
   c->tiles_valid = false;
+  if (o->bc_mode == AMPS_BC_USER)
+    if (int r = collect_exits(c)) return r;
   if (o->compact) return amps_gpu_compact(c);
   return AMPS_OK;
 }
@@ -914,15 +923,14 @@ extern "C" int amps_gpu_update_j_mass_matrix(amps_ctx* c, double dt, double* J_o
       if (int r = amps_gpu_compact(c)) return r;
     if (int r = launch_deposit(c, dt)) return r;
   }
-  if (J_out || M_out) {
-    std::vector<double> h((size_t)c->nnodes * NODE_STRIDE);
-    HIP_TRY(hipStreamSynchronize(c->stream));
-    HIP_TRY(hipMemcpy(h.data(), c->d_nodeout, sizeof(double) * h.size(), hipMemcpyDeviceToHost));
-    for (int64_t i = 0; i < c->nnodes; i++) {
-      if (J_out) std::memcpy(J_out + 3 * i, &h[(size_t)i * NODE_STRIDE], 3 * sizeof(double));
-      if (M_out) std::memcpy(M_out + 243 * i, &h[(size_t)i * NODE_STRIDE + 3], 243 * sizeof(double));
-    }
-  }
+  // strided copies of the J and M columns of the resident [node][NODE_STRIDE] rows
+  if (J_out || M_out) HIP_TRY(hipStreamSynchronize(c->stream));
+  if (J_out && c->nnodes > 0)
+    HIP_TRY(hipMemcpy2D(J_out, 3 * sizeof(double), c->d_nodeout, NODE_STRIDE * sizeof(double), 3 * sizeof(double),
+                        (size_t)c->nnodes, hipMemcpyDeviceToHost));
+  if (M_out && c->nnodes > 0)
+    HIP_TRY(hipMemcpy2D(M_out, 243 * sizeof(double), c->d_nodeout + 3, NODE_STRIDE * sizeof(double),
+                        243 * sizeof(double), (size_t)c->nnodes, hipMemcpyDeviceToHost));
   return AMPS_OK;
 }
 
@@ -996,6 +1004,8 @@ static int one_step(amps_ctx* c, const amps_push_opts* o, double dt) {
   tbegin(c);
   if (int r = launch_push(c, o, c->d_offset)) return r;
   tend(c, AMPS_K_PUSH);
+  if (o->bc_mode == AMPS_BC_USER)
+    if (int r = collect_exits(c)) return r;
   tbegin(c);
   if (int r = compact_impl(c, true)) return r;
   tend(c, AMPS_K_COMPACT);
@@ -1008,10 +1018,12 @@ extern "C" int amps_gpu_run_steps(amps_ctx* c, const amps_push_opts* o, double d
   if (!c->has_mesh || !c->has_fields || !c->has_species || c->cap == 0)
     return fail(AMPS_E_STATE, "amps_gpu_run_steps: mesh, fields, species and particles are required");
   if (int r = check_mover(c, o, "amps_gpu_run_steps")) return r;
-  if (o->bc_mode != AMPS_BC_DELETE && o->bc_mode != AMPS_BC_REFLECT)
+  if (o->bc_mode != AMPS_BC_DELETE && o->bc_mode != AMPS_BC_REFLECT && o->bc_mode != AMPS_BC_USER)
     return fail(AMPS_E_ARG, "amps_gpu_run_steps: unknown boundary mode");
   if (nsteps <= 0) return AMPS_OK;
   (void)hipSetDevice(c->device);
+  if (o->bc_mode == AMPS_BC_USER)
+    if (int r = prepare_exits(c)) return r;
   if (o->tile > 0 && o->tile != c->tile)
     if (int r = amps_gpu_set_param(c, "tile", o->tile)) return r;
   if (int r = ensure_deposit_buffers(c)) return r;

@@ -16,8 +16,9 @@
 // cell size away on the level-lmax lattice (inside one block the block's own
 // cell size, as GetCornerNode(i+-1, j+-1, k+-1) of the owning block); a
 // neighbour outside the domain contributes nothing (a boundary row in the
-// reference's system).  The terms are summed in the reference's order, so
-// the result is bit-exact with it.  The M rows are read once by consecutive
+// reference's system).  The terms are summed in the reference's order onto
+// the caller's partial row when one is given (accumulate), so the result is
+// bit-exact with the reference's res after its mass-matrix loop.  The M rows are read once by consecutive
 // lanes, the E of the neighbours through the caches: the kernel is HBM-bound
 // at 1944 + 108 + 24 + 24 bytes per node.
 #include <hip/hip_runtime.h>
@@ -38,7 +39,8 @@ namespace {
 // as the reference sums them.
 __global__ __launch_bounds__(256) void k_ecsim_mass(const double* __restrict__ rows, const int* __restrict__ nbr,
                                                     const double* __restrict__ E, double c1, double c2,
-                                                    int64_t nnodes, double* __restrict__ out) {
+                                                    int64_t nnodes, const double* __restrict__ seed,
+                                                    double* __restrict__ out) {
   __shared__ double term[8][27][3];
   __shared__ uint64_t present[4];
   // workgroups go round-robin over the 8 XCDs: give each XCD one contiguous
@@ -73,7 +75,9 @@ __global__ __launch_bounds__(256) void k_ecsim_mass(const double* __restrict__ r
     const int64_t nn = chunk * 8 + ln;
     if (nn < nnodes) {
       const uint64_t p = present[ln >> 1] >> (32 * (ln & 1));
-      double res = 0.0;
+      // res starts from the part of the row the caller has already summed
+      // (UpdateRhs_Legacy adds the 27 mass terms onto it one at a time)
+      double res = seed ? seed[3 * nn + c] : 0.0;
       for (int q = 0; q < 27; q++)
         if ((p >> q) & 1) res += term[ln][q][c];
       out[3 * nn + c] = res;
@@ -87,6 +91,7 @@ int build_nbr(amps_ctx* c) {
   const int nb = c->mv.nb, lmax = c->mv.lmax;
   const int64_t S = ((int64_t)1 << lmax) + 1;
   const int64_t nn = c->nnodes;
+  try {
   std::vector<int64_t> h((size_t)nn, INT64_MAX);
   for (int b = 0; b < nb; b++) {
     const int64_t cs = ((int64_t)1 << (lmax - c->h_level[(size_t)b])) / NC;
@@ -112,39 +117,61 @@ int build_nbr(amps_ctx* c) {
   }
   HIP_TRY(hipMalloc(&c->d_ecsim_nbr, sizeof(int) * std::max<size_t>(27, nbr.size())));
   HIP_TRY(hipMemcpy(c->d_ecsim_nbr, nbr.data(), sizeof(int) * nbr.size(), hipMemcpyHostToDevice));
+  } catch (const std::bad_alloc&) {
+    return fail(AMPS_E_RANGE, "amps_gpu_ecsim_mass_apply: out of host memory for the neighbour table");
+  }
   return 0;
+}
+
+// one refinement level: the reference's ECSIM solver works on a uniform
+// block mesh (UpdateJMassMatrix takes the resolution of BlockTable[0],
+// pic_field_solver_ecsim.cpp:3566); at a coarse/fine node the M slots of
+// cells of two sizes would refer to different neighbours
+bool single_level(const amps_ctx* c) {
+  for (size_t b = 1; b < c->h_level.size(); b++)
+    if (c->h_level[b] != c->h_level[0]) return false;
+  return true;
 }
 
 }  // namespace
 
 extern "C" int amps_gpu_ecsim_mass_apply(amps_ctx* c, const double* E_nodes, double c1, double c2, double* out,
-                                         int32_t reps) {
+                                         int32_t accumulate, int32_t reps) {
   if (!c) return fail(AMPS_E_ARG, "null context");
   if (!c->has_mesh || !c->d_nodeout)
     return fail(AMPS_E_STATE, "amps_gpu_ecsim_mass_apply: run amps_gpu_update_j_mass_matrix first");
   if (!E_nodes || !out) return fail(AMPS_E_ARG, "amps_gpu_ecsim_mass_apply: null array");
   if (reps < 1 || reps > 1000) return fail(AMPS_E_RANGE, "amps_gpu_ecsim_mass_apply: reps must be in 1..1000");
-  (void)hipSetDevice(c->device);
-  if (int r = build_nbr(c)) return r;
+  if (!single_level(c))
+    return fail(AMPS_E_STATE, "amps_gpu_ecsim_mass_apply: the ECSIM field equation needs a single-level mesh "
+                              "(the reference solver runs on one resolution level)");
   const int64_t nn = c->nnodes;
   if (nn == 0) return AMPS_OK;
   if (nn > ((int64_t)1 << 27)) return fail(AMPS_E_RANGE, "amps_gpu_ecsim_mass_apply: more than 2^27 nodes");
-  double *dE = nullptr, *dO = nullptr;
-  HIP_TRY(hipMalloc(&dE, sizeof(double) * 3 * nn));
-  HIP_TRY(hipMalloc(&dO, sizeof(double) * 3 * nn));
-  HIP_TRY(hipMemcpyAsync(dE, E_nodes, sizeof(double) * 3 * nn, hipMemcpyHostToDevice, c->stream));
+  (void)hipSetDevice(c->device);
+  if (int r = build_nbr(c)) return r;
+  double *dE = nullptr, *dO = nullptr, *dS = nullptr;
+  int rc = AMPS_OK;
+  hipError_t e = hipMalloc(&dE, sizeof(double) * 3 * nn);
+  if (e == hipSuccess) e = hipMalloc(&dO, sizeof(double) * 3 * nn);
+  if (e == hipSuccess && accumulate) e = hipMalloc(&dS, sizeof(double) * 3 * nn);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(dE, E_nodes, sizeof(double) * 3 * nn, hipMemcpyHostToDevice, c->stream);
+  if (e == hipSuccess && dS) e = hipMemcpyAsync(dS, out, sizeof(double) * 3 * nn, hipMemcpyHostToDevice, c->stream);
   // 4 wavefronts of 2 nodes per workgroup, the workgroup count rounded up to 8
   const dim3 grid((unsigned)((((nn + 7) / 8) + 7) / 8 * 8));
-  for (int r = 0; r < reps; r++) {
+  for (int r = 0; r < reps && e == hipSuccess; r++) {
     tbegin(c);
     hipLaunchKernelGGL(k_ecsim_mass, grid, dim3(256), 0, c->stream, c->d_nodeout, c->d_ecsim_nbr, dE, c1, c2, nn,
-                       dO);
-    HIP_TRY(hipGetLastError());
+                       dS, dO);
+    e = hipGetLastError();
     tend(c, AMPS_K_ECSIM);
   }
-  HIP_TRY(hipMemcpyAsync(out, dO, sizeof(double) * 3 * nn, hipMemcpyDeviceToHost, c->stream));
-  HIP_TRY(hipStreamSynchronize(c->stream));
+  if (e == hipSuccess) e = hipMemcpyAsync(out, dO, sizeof(double) * 3 * nn, hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  if (e != hipSuccess) rc = hip_fail(e, "amps_gpu_ecsim_mass_apply");
   (void)hipFree(dE);
   (void)hipFree(dO);
-  return AMPS_OK;
+  (void)hipFree(dS);
+  return rc;
 }

@@ -241,11 +241,21 @@ __device__ __forceinline__ int relocate(const MeshView& m, const Faces& faces, c
         if (m.gxmax[d] <= xInit[d]) xInit[d] = m.gxmax[d] - m.eps;
       }
       newNode = find_block(m, xInit);
-      if (newNode < 0) {
+      if (newNode < 0 && bc_mode != AMPS_BC_USER) {
 #pragma unroll
         for (int d = 0; d < 3; d++) xp[d] = xInit[d], vp[d] = vInit[d];
         return AMPS_CODE_LEFT_DOMAIN;
       }
+    }
+    if (bc_mode == AMPS_BC_USER) {
+      // ProcessOutsideDomainParticles(ptr, xInit, vInit, nIntersectionFace,
+      // newNode) (:1231): the state moved onto the face is what the callback
+      // receives; the particle is then deleted (:1262-1265)
+#pragma unroll
+      for (int d = 0; d < 3; d++) xp[d] = xInit[d], vp[d] = vInit[d];
+      *blk = newNode;
+      *cell = 0;
+      return AMPS_CODE_EXIT_FACE0 + nIntersectionFace;
     }
     // specular reflection (:1236-1253), committed
     double cc = 0.0;
@@ -512,11 +522,19 @@ __device__ __forceinline__ int push_particle_rel(const MeshView& m, const Faces&
           if (m.gxmax[d] <= xInit[d]) xInit[d] = m.gxmax[d] - m.eps;
         }
         newNode = find_block(m, xInit);
-        if (newNode < 0) {
+        if (newNode < 0 && bc_mode != AMPS_BC_USER) {
 #pragma unroll
           for (int d = 0; d < 3; d++) xp[d] = xInit[d], vp[d] = vInit[d];
           return AMPS_CODE_LEFT_DOMAIN;
         }
+      }
+      if (bc_mode == AMPS_BC_USER) {
+        // ProcessOutsideDomainParticles (pic_mover_relativistic_boris.cpp:418-420)
+#pragma unroll
+        for (int d = 0; d < 3; d++) xp[d] = xInit[d], vp[d] = vInit[d];
+        *blk = newNode;
+        *cell = 0;
+        return AMPS_CODE_EXIT_FACE0 + nIntersectionFace;
       }
       double cc = 0.0;
 #pragma unroll

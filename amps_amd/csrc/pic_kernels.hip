@@ -102,7 +102,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W))) void k
     v[2 * cap + p] = vp[2];
     alive = alive_code(c);
     knew = nb_out * NCELL + cell_out;
-    if (alive) key[p] = knew;
+    // an exit of the USER_FUNCTION mode keeps the block holding its face state
+    if (alive || c >= AMPS_CODE_EXIT_FACE0) key[p] = knew;
     code[p] = (int8_t)c;
     }
     // migration count of the particles that stay: one atomic per run of
@@ -174,7 +175,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W))) void k
     v[2 * cap + p] = vp[2];
     alive = alive_code(c);
     knew = nb_out * NCELL + cell_out;
-    if (alive) key[p] = knew;
+    // an exit of the USER_FUNCTION mode keeps the block holding its face state
+    if (alive || c >= AMPS_CODE_EXIT_FACE0) key[p] = knew;
     code[p] = (int8_t)c;
     }
     if (count) {

@@ -5,8 +5,8 @@
 * ``cutoff_scan``   — CutoffForDirectionUpperScan_GV on the
   BuildCutoffSearchGrid_GV grid (.cpp:2578-2834), full or downward scan,
   optionally restricted to a per-direction bracket of bins.
-* ``build_search_grid``, ``local_enu``, ``local_dir`` — the grid and
-  direction helpers (.cpp:2578-2617, 1757-1800).
+* ``build_search_grid``, ``uniform_sphere_dirs``, ``local_dirs`` — the grid
+  and direction helpers (.cpp:2578-2617, 1652-1674, 1757-1800).
 * ``config2_*``     — the batched-map workload: a 64x32 lat/lon grid at
   r = 1.05 Re, 8 arrival directions per point, 48 log-spaced rigidities
   0.1..20 GV, tilted dipole (10 deg, B0 = 3.1e-5 T) plus Bz = -5 nT, steps of
@@ -212,29 +212,51 @@ def build_search_grid(rmin, rmax, n):
     return g
 
 
-def _unit(v):
-    n = math.sqrt(float(v @ v))
-    return v / n if n > 0 else v * 0.0
+def uniform_sphere_dirs(n):
+    """BuildUniformSphereDirs (CutoffRigidityGridless.cpp:1652-1674): n
+    near-uniform unit vectors, [n, 3] (library host code)."""
+    out = np.zeros((int(n), 3))
+    L.check(L.load().amps_gpu_uniform_sphere_dirs(int(n), L._ptr(out, C.c_double)), "uniform_sphere_dirs")
+    return out
+
+
+def local_dirs(x0, lon_deg, lat_deg):
+    """BuildLocalENU_GSM + LocalLonLatToDir_GSM (CutoffRigidityGridless.cpp:
+    1757-1800) through the library: for every location x0[p] and sky
+    direction (lon_deg[k] from local north toward east, lat_deg[k] toward up),
+    the unit vector dirs[p, k]; frames[p] = (E, N, U) rows."""
+    x0 = np.ascontiguousarray(x0, np.float64).reshape(-1, 3)
+    lon = np.ascontiguousarray(np.atleast_1d(lon_deg), np.float64)
+    lat = np.ascontiguousarray(np.atleast_1d(lat_deg), np.float64)
+    if lon.shape != lat.shape:
+        raise ValueError("lon_deg and lat_deg must have the same length")
+    dirs = np.zeros((len(x0), len(lon), 3))
+    frames = np.zeros((len(x0), 3, 3))
+    L.check(L.load().amps_gpu_local_dirs(len(x0), L._ptr(x0, C.c_double), len(lon), L._ptr(lon, C.c_double),
+                                         L._ptr(lat, C.c_double), L._ptr(dirs, C.c_double),
+                                         L._ptr(frames, C.c_double)), "local_dirs")
+    return dirs, frames
+
+
+def local_dirs_paired(x0, lon_deg, lat_deg):
+    """local_dirs for (location, sky direction) pairs: dirs[p] for x0[p],
+    lon_deg[p], lat_deg[p]."""
+    x0 = np.ascontiguousarray(x0, np.float64).reshape(-1, 3)
+    lon = np.ascontiguousarray(lon_deg, np.float64).ravel()
+    lat = np.ascontiguousarray(lat_deg, np.float64).ravel()
+    if not (len(x0) == len(lon) == len(lat)):
+        raise ValueError("x0, lon_deg and lat_deg must have the same length")
+    dirs = np.zeros((len(x0), 3))
+    L.check(L.load().amps_gpu_local_dirs_paired(len(x0), L._ptr(x0, C.c_double), L._ptr(lon, C.c_double),
+                                                L._ptr(lat, C.c_double), L._ptr(dirs, C.c_double)),
+            "local_dirs_paired")
+    return dirs
 
 
 def local_enu(x0):
-    """BuildLocalENU_GSM (CutoffRigidityGridless.cpp:1757-1773)."""
-    U = _unit(np.asarray(x0, np.float64))
-    E = np.cross([0.0, 0.0, 1.0], U)
-    if math.sqrt(E @ E) < 1e-12:
-        E = np.cross([1.0, 0.0, 0.0], U)
-    E = _unit(E)
-    N = _unit(np.cross(U, E))
-    return E, N, U
-
-
-def local_dir(frame, lon_deg, lat_deg):
-    """LocalLonLatToDir_GSM (.cpp:1786-1800): azimuth from +N toward +E,
-    elevation toward +U."""
-    E, N, U = frame
-    lon, lat = math.radians(lon_deg), math.radians(lat_deg)
-    cl = math.cos(lat)
-    return _unit(cl * math.sin(lon) * E + cl * math.cos(lon) * N + math.sin(lat) * U)
+    """BuildLocalENU_GSM (CutoffRigidityGridless.cpp:1757-1773): (E, N, U)."""
+    _, f = local_dirs(x0, [0.0], [0.0])
+    return f[0, 0], f[0, 1], f[0, 2]
 
 
 def config2_points(nlon=64, nlat=32, r_Re=1.05, re_m=RE_DIPOLE):
@@ -263,12 +285,8 @@ def arrival_dirs(x0, zen, az):
     """Arrival direction (unit, pointing down along the particle's motion) for
     every point and every (zenith, azimuth): the backtrace starts along
     LocalLonLatToDir(azimuth, 90 - zenith)."""
-    out = np.zeros((len(x0), len(zen), 3))
-    for i, x in enumerate(x0):
-        fr = local_enu(x)
-        for k in range(len(zen)):
-            out[i, k] = -local_dir(fr, az[k], 90.0 - zen[k])
-    return out
+    d, _ = local_dirs(x0, np.asarray(az, np.float64), 90.0 - np.asarray(zen, np.float64))
+    return -d
 
 
 def trace_batch(ctx, params, x0, dirs, R, precision=L.FP64):

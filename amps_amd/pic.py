@@ -156,30 +156,70 @@ class PicState:
         o = L.PushOpts(mover, bc_mode, 1 if compact else 0, tile, r_sphere)
         L.check(self.lib.amps_gpu_move_particles(self.ctx.h, C.byref(o)), "move_particles")
 
+    def get_exits(self, clear=True):
+        """The exit list of the USER_FUNCTION boundary mode: the arguments of
+        PIC::Mover::ProcessOutsideDomainParticles (pic.h:6043-6044) for every
+        particle that left through an external face since the last clear, as
+        a dict of arrays: id, x [n, 3] and v [n, 3] on the face, spec, face,
+        blk (-1: no block holds the face state)."""
+        n = C.c_int64()
+        L.check(self.lib.amps_gpu_get_exits(self.ctx.h, 0, C.byref(n), None, None, None, None, None, None, 0),
+                "get_exits")
+        m = n.value
+        out = {"id": np.zeros(m, np.int64), "x": np.zeros((m, 3)), "v": np.zeros((m, 3)),
+               "spec": np.zeros(m, np.int32), "face": np.zeros(m, np.int32), "blk": np.zeros(m, np.int32)}
+        L.check(self.lib.amps_gpu_get_exits(self.ctx.h, m, C.byref(n), L._ptr(out["id"], C.c_int64),
+                                            L._ptr(out["x"], C.c_double), L._ptr(out["v"], C.c_double),
+                                            L._ptr(out["spec"], C.c_int32), L._ptr(out["face"], C.c_int32),
+                                            L._ptr(out["blk"], C.c_int32), 1 if clear else 0), "get_exits")
+        return out
+
     def compact(self):
         L.check(self.lib.amps_gpu_compact(self.ctx.h), "compact")
 
     def UpdateJMassMatrix(self, dt, copy_out=True):
+        """copy_out: True (J and M), False (neither) or "J" (J only)."""
         if not copy_out:
             L.check(self.lib.amps_gpu_update_j_mass_matrix(self.ctx.h, dt, None, None), "deposit")
             return None, None
         J = np.zeros((self.nnodes, 3))
+        if copy_out == "J":
+            L.check(self.lib.amps_gpu_update_j_mass_matrix(self.ctx.h, dt, L._ptr(J, C.c_double), None), "deposit")
+            return J, None
         M = np.zeros((self.nnodes, 243))
         L.check(self.lib.amps_gpu_update_j_mass_matrix(self.ctx.h, dt, L._ptr(J, C.c_double),
                                                        L._ptr(M, C.c_double)), "deposit")
         return J, M
 
-    def MassMatrixApply(self, E_nodes, c1, c2=1.0, reps=1):
+    def ParticleEnergyCfl(self, dt):
+        """Total particle energy and per-species max CFL of UpdateJMassMatrix
+        (pic_field_solver_ecsim.cpp:2229-2238, 2355-2358, 3925-3929,
+        3972-3987) over the resident particles."""
+        e = C.c_double()
+        cfl = np.zeros(self.species.n)
+        L.check(self.lib.amps_gpu_particle_energy_cfl(self.ctx.h, float(dt), C.byref(e), L._ptr(cfl, C.c_double)),
+                "particle_energy_cfl")
+        return e.value, cfl
+
+    def MassMatrixApply(self, E_nodes, c1, c2=1.0, reps=1, seed=None):
         """The ECSIM mass-matrix term over the resident M of the last
         UpdateJMassMatrix (UpdateRhs_Legacy, pic_field_solver_ecsim.cpp:
         6116-6124): out[n, c] = sum over the 27 neighbours of
         (E . M[n, slot, c, :]) * c1 * c2, with c1 = -4 pi dt theta and
         c2 = E_conv in the reference.  E_nodes is [nnodes, 3] in node-key
-        order."""
+        order; seed, when given, is the row sum the terms are added onto
+        (the curl and current parts of UpdateRhs_Legacy).  Single-level
+        meshes only, as the reference's solver."""
         E = np.ascontiguousarray(E_nodes, np.float64)
         if E.shape != (self.nnodes, 3):
             raise ValueError("E_nodes must be [nnodes, 3]")
-        out = np.zeros((self.nnodes, 3))
+        if seed is None:
+            out = np.zeros((self.nnodes, 3))
+        else:
+            out = np.array(seed, np.float64, copy=True, order="C")
+            if out.shape != (self.nnodes, 3):
+                raise ValueError("seed must be [nnodes, 3]")
         L.check(self.lib.amps_gpu_ecsim_mass_apply(self.ctx.h, L._ptr(E, C.c_double), float(c1), float(c2),
-                                                   L._ptr(out, C.c_double), int(reps)), "ecsim_mass_apply")
+                                                   L._ptr(out, C.c_double), 0 if seed is None else 1, int(reps)),
+                "ecsim_mass_apply")
         return out

@@ -86,6 +86,32 @@ class Surrogate:
         return y
 
 
+def random_tasks(n, seed=6, r_Re=1.05, max_zenith_deg=60.0):
+    """n random (location, arrival direction) pairs of the surrogate's
+    training set: locations uniform on the sphere of radius r_Re dipole
+    radii, zenith uniform in cos over [0, max_zenith], azimuth uniform.
+    Returns (x0 [n, 3] m, dirs [n, 3], features [n, 6])."""
+    rng = np.random.default_rng(seed)
+    lat = np.degrees(np.arcsin(rng.uniform(-1, 1, n)))
+    lon = rng.uniform(0, 360, n)
+    zen = np.degrees(np.arccos(rng.uniform(math.cos(math.radians(max_zenith_deg)), 1.0, n)))
+    az = rng.uniform(0, 360, n)
+    la, lo = np.radians(lat), np.radians(lon)
+    r0 = r_Re * CU.RE_DIPOLE
+    x0 = np.stack([r0 * np.cos(la) * np.cos(lo), r0 * np.cos(la) * np.sin(lo), r0 * np.sin(la)], 1)
+    dirs = -CU.local_dirs_paired(x0, az, 90.0 - zen)
+    return x0, dirs, features(lat, lon, zen, az)
+
+
+def training_set(ctx, params, n, grid, seed=6, precision=L.FP32):
+    """The surrogate's training set from the tracer itself: the upper cutoff
+    bin of n random pairs by the downward scan.  Returns (features, targets,
+    bins, steps)."""
+    x0, dirs, X = random_tasks(n, seed)
+    _, bins, steps = CU.cutoff_scan(ctx, params, x0, dirs, grid, mode=CU.SCAN_UPPER, precision=precision)
+    return X, bin_target(bins, grid), bins, steps
+
+
 def bracket_bins(pb, nR, half=3):
     """Fixed-width brackets [lo, lo + 2 half] around the predicted bins,
     shifted inward at the ends of the grid."""

@@ -70,6 +70,28 @@ def _pmc_pass(args, tune, counter, outdir):
     return {k: tot / cnt for k, (tot, cnt) in sums.items()}
 
 
+def ecsim_mass_setup():
+    """A second context holding the single-level mesh of the ECSIM field
+    solve (16^3 level-4 blocks, 2.1 M cells, 16 + 16 particles per cell) and
+    the M of one deposit on it."""
+    import numpy as np
+
+    from amps_amd import _lib as L
+    from amps_amd import mesh as M
+    from amps_amd.pic import PicState, Species
+
+    mesh = M.uniform_mesh(4)
+    E, B = M.corner_fields(mesh)
+    dt = 0.1 / (float(np.max(np.linalg.norm(B, axis=-1))) / 0.01)
+    sp = Species([-1.0, 1.0], [0.01, 1.0], [dt, dt])
+    c2 = L.Context(0)
+    st = PicState(mesh, E, B, sp, ctx=c2)
+    st.generate_particles(mesh.ncells * 16, [0.05, 0.005], drift=(0.01, 0.0, 0.0), seed=5)
+    st.UpdateJMassMatrix(dt, copy_out=False)
+    En = np.random.default_rng(7).normal(size=(st.nnodes, 3))
+    return c2, st, En, dt
+
+
 def pmc_traffic(args, tune):
     """HBM bytes per launch of the step's kernels from FETCH_SIZE and
     WRITE_SIZE, each scaled by the factor that makes the copy kernel of known
@@ -255,27 +277,10 @@ def cutoff_map_bench(ctx, n_cu, n_train=1 << 20, epochs=20, cpu_check_points=16,
         ost = (r["termination"] == 0).astype(np.int8).reshape(-1, 48)
         out["cpu_oracle_check"] = {"points": int(len(idx)), "states_bitexact": bool(np.array_equal(ost, s64[idx]))}
     # surrogate: training set from the tracer itself (UPPER_SCAN, FP32)
-    rng = np.random.default_rng(6)
-    tl = np.degrees(np.arcsin(rng.uniform(-1, 1, n_train)))
-    tg = rng.uniform(0, 360, n_train)
-    tz = np.degrees(np.arccos(rng.uniform(np.cos(np.radians(60.0)), 1.0, n_train)))
-    ta = rng.uniform(0, 360, n_train)
-    la, lo = np.radians(tl), np.radians(tg)
-    r0 = 1.05 * CU.RE_DIPOLE
-    tx = np.stack([r0 * np.cos(la) * np.cos(lo), r0 * np.cos(la) * np.sin(lo), r0 * np.sin(la)], 1)
-    U = tx / np.linalg.norm(tx, axis=1)[:, None]
-    E = np.cross(np.array([0.0, 0.0, 1.0]), U)
-    E /= np.maximum(np.linalg.norm(E, axis=1), 1e-300)[:, None]
-    N = np.cross(U, E)
-    el, azr = np.radians(90.0 - tz), np.radians(ta)
-    d = (np.cos(el) * np.sin(azr))[:, None] * E + (np.cos(el) * np.cos(azr))[:, None] * N + np.sin(el)[:, None] * U
-    tdir = -d / np.linalg.norm(d, axis=1)[:, None]
     t0 = time.perf_counter()
-    _, tb, tsteps = CU.cutoff_scan(ctx, P, tx, tdir, grid, mode=CU.SCAN_UPPER, precision=L.FP32)
+    Xtr, Ttr, tb, tsteps = SG.training_set(ctx, P, n_train, grid, seed=6, precision=L.FP32)
     out["train_data_s"] = time.perf_counter() - t0
     out["train_samples"] = int(n_train)
-    Xtr = SG.features(tl, tg, tz, ta)
-    Ttr = SG.bin_target(tb, grid)
     sur = SG.Surrogate(ctx, seed=0)
     sur.set_data(Xtr, Ttr)
     nsteps = max(1, epochs * n_train // 4096)
@@ -412,7 +417,9 @@ def main():
     if args.pmc_child:
         # one eager step and the copy kernel of known size, for the counter passes
         st.run_steps(1, dt, use_graph=False, **opts)
-        st.MassMatrixApply(np.random.default_rng(7).normal(size=(st.nnodes, 3)), -4 * np.pi * dt * 0.5)
+        c2, st2, En2, dt2 = ecsim_mass_setup()
+        st2.MassMatrixApply(En2, -4 * np.pi * dt2 * 0.5)
+        c2.close()
         ctx.copy_probe(COPY_PROBE_BYTES, reps=1)
         ctx.sync()
         ctx.close()
@@ -470,23 +477,27 @@ def main():
                 "achieved": round(dep_gbps, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                 "frac": round(dep_gbps / HBM_PEAK_GBPS, 4), "traffic": tr,
                 "algorithmic_bytes": int(dep_bytes)}
-    # the ECSIM mass-matrix term over the resident M of the last deposit
-    # (27-neighbour stencil, 243 entries per node): HIP-event time per launch
+    # the ECSIM mass-matrix term over the resident M of a deposit on the
+    # single-level mesh the reference's field solver runs on (27-neighbour
+    # stencil, 243 entries per node): HIP-event time per launch
     ecsim = None
     try:
-        En = np.random.default_rng(7).normal(size=(nnodes, 3))
-        st.MassMatrixApply(En, -4 * np.pi * dt * 0.5)  # builds the neighbour table once
-        ctx.set_timing(True)
-        ctx.reset_timing()
-        st.MassMatrixApply(En, -4 * np.pi * dt * 0.5, reps=20)
-        ctx.set_timing(False)
-        mse, cnte = ctx.timing()
+        c2, st2, En, dt2 = ecsim_mass_setup()
+        st2.MassMatrixApply(En, -4 * np.pi * dt2 * 0.5)  # builds the neighbour table once
+        c2.set_timing(True)
+        c2.reset_timing()
+        st2.MassMatrixApply(En, -4 * np.pi * dt2 * 0.5, reps=20)
+        c2.set_timing(False)
+        mse, cnte = c2.timing()
         e_ms = mse[L.K_ECSIM] / max(cnte[L.K_ECSIM], 1)
-        e_bytes = nnodes * ECSIM_NODE_BYTES
-        ecsim = {"kernel": "k_ecsim_mass", "ms": round(e_ms, 4), "nodes": int(nnodes),
+        e_nodes = st2.nnodes
+        e_bytes = e_nodes * ECSIM_NODE_BYTES
+        ecsim = {"kernel": "k_ecsim_mass", "mesh": "uniform level 4 (16^3 blocks of 8^3 cells), 16+16 per cell",
+                 "ms": round(e_ms, 4), "nodes": int(e_nodes),
                  "traffic": traffic.get("ecsim_mass") if traffic else None,
                  "algorithmic_bytes": int(e_bytes), "GBps": round(e_bytes / (e_ms * 1e-3) / 1e9, 1),
                  "frac_of_hbm_peak": round(e_bytes / (e_ms * 1e-3) / 1e9 / HBM_PEAK_GBPS, 4)}
+        c2.close()
     except Exception as e:  # the particle-step line stays valid
         ecsim = {"error": repr(e)}
     # the same step with the FP32 push (FP32 stencil, fields and velocities,

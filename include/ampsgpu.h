@@ -30,6 +30,10 @@ extern "C" {
 #define AMPS_CODE_LEFT_DOMAIN 2   /* _PARTICLE_LEFT_THE_DOMAIN_: deleted on an outer face */
 #define AMPS_CODE_FINISHED 3      /* _PARTICLE_MOTION_FINISHED_ */
 #define AMPS_CODE_SPHERE 5        /* absorbed by the internal sphere (reference: 2) */
+/* AMPS_BC_USER: left the domain through external face f = code - 8 (0..5,
+ * ExternalBoundaryFaceTable order), the state moved onto the face and the
+ * particle queued for ProcessOutsideDomainParticles, then deleted. */
+#define AMPS_CODE_EXIT_FACE0 8
 
 #define AMPS_MOVER_LAPENTA 0      /* PIC::Mover::Lapenta2017, pic_mover_boris.cpp:876-1380 */
 #define AMPS_MOVER_BORIS 1        /* PIC::Mover::Boris, pic_mover_boris.cpp:126-553 */
@@ -37,6 +41,8 @@ extern "C" {
 
 #define AMPS_BC_DELETE 0          /* _PIC_PARTICLE_DOMAIN_BOUNDARY_INTERSECTION_PROCESSING_MODE__DELETE_ */
 #define AMPS_BC_REFLECT 1         /* ..._SPECULAR_REFLECTION_ (pic_mover_boris.cpp:1236-1253) */
+#define AMPS_BC_USER 2            /* ..._USER_FUNCTION_ (pic_mover_boris.cpp:1156-1231): exits go
+                                     to the exit list (amps_gpu_get_exits) */
 
 #define AMPS_FP64 0
 #define AMPS_FP32 1               /* FP32 push arithmetic and velocities; FP64 positions,
@@ -75,7 +81,9 @@ int amps_gpu_device_info(amps_ctx* ctx, char* name, int* n_cu, int64_t* hbm_byte
  * 4: held to 128 registers for four wavefronts per SIMD),
  * "b_center" (1: the FP64 Lapenta/Boris movers and the deposit take B from
  * the cell-centred field of amps_gpu_set_center_fields, the centre-based B
- * mode _PIC_FIELD_SOLVER_B_MODE_; 0 (default): B at the corners). */
+ * mode _PIC_FIELD_SOLVER_B_MODE_; 0 (default): B at the corners),
+ * "exit_capacity" (records of the AMPS_BC_USER exit list, 1..2^30, default
+ * 2^20; a new value empties the list at the next USER_FUNCTION push). */
 int amps_gpu_set_param(amps_ctx* ctx, const char* name, int64_t value);
 
 /* ---- mesh ------------------------------------------------------------ */
@@ -191,6 +199,24 @@ typedef struct {
  * compaction.  Exits (AMPS_CODE_LEFT_DOMAIN) are what the reference hands
  * to ProcessOutsideDomainParticles (pic.h:6043-6044). */
 int amps_gpu_move_particles(amps_ctx* ctx, const amps_push_opts* opts);
+/* The exit list of AMPS_BC_USER: every particle that left the domain through
+ * an external face since the list was last cleared, in device order and step
+ * order, with the arguments the reference passes to
+ * PIC::Mover::ProcessOutsideDomainParticles(ptr, xInit, vInit,
+ * nIntersectionFace, newNode) (pic.h:6043-6044; pic_mover_boris.cpp:1156-1231,
+ * pic_mover_relativistic_boris.cpp:317-420): the particle id, the state moved
+ * onto the exit face (x[3i..3i+2] at the face intersection minus EPS times
+ * the face normal, v[3i..3i+2] advanced to the intersection time), its
+ * species, the face index and the block holding x (-1 when none, where the
+ * reference aborts).  The list is built on the device (per-workgroup counts,
+ * a scan, ordered writes) after each push, so only the exits cross PCIe; its
+ * capacity is the parameter "exit_capacity" (default 2^20 records).
+ * *n_exits receives the count; min(count, cap) records are copied (NULL
+ * arrays: count only); clear = 1 empties the list.  Returns AMPS_E_RANGE when
+ * exits were dropped because the list was full. */
+int amps_gpu_get_exits(amps_ctx* ctx, int64_t cap, int64_t* n_exits, int64_t* id, double* x, double* v,
+                       int32_t* spec, int32_t* face, int32_t* blk, int32_t clear);
+
 /* Migration: stable-in-wave counting sort by (block, cell) with deleted
  * particles dropped (replaces the temp-list swap, pic_mover.cpp:982-992). */
 int amps_gpu_compact(amps_ctx* ctx);
@@ -224,17 +250,30 @@ int amps_gpu_copy_probe(amps_ctx* ctx, int64_t nbytes, int32_t reps, double* ms)
  * the dense [node][246] result stays resident either way. */
 int amps_gpu_update_j_mass_matrix(amps_ctx* ctx, double dt, double* J_out, double* M_out);
 
+/* The particle diagnostics of ECSIM::UpdateJMassMatrix (pic_field_solver_
+ * ecsim.cpp:2229-2238, 2355-2358, 3925-3929, 3972-3987) over the resident
+ * particles: *energy = the sum of 0.5 m w |v|^2 (ParticleEnergy), cfl[s] = the
+ * maximum over cells of (sum over the cell's particles of species s of
+ * |v| dt) / (count |dx|) (the per-species max CFL), cells summed in particle
+ * order, blocks in block order.  cfl may be NULL. */
+int amps_gpu_particle_energy_cfl(amps_ctx* ctx, double dt, double* energy, double* cfl);
+
 /* The mass-matrix term of the ECSIM field equation over the resident M of
  * the last amps_gpu_update_j_mass_matrix: for every node n and component c,
  *   out[3n+c] = sum over slots s of ((Ex M[9s+3c] + Ey M[9s+3c+1]) + Ez M[9s+3c+2]) * c1 * c2
  * with E of the slot's neighbour node (UpdateRhs_Legacy mass part,
  * pic_field_solver_ecsim.cpp:6116-6124, c1 = -4 pi dt theta, c2 = E_conv;
  * the neighbour order of GetStencil :1061-1074).  E_nodes and out are
- * [nnodes][3] in node-key order.  A neighbour is one cell of the node's
- * finest incident cell size away; one outside the domain contributes 0.
- * reps > 1 repeats the launch (for timing). */
+ * [nnodes][3] in node-key order.  accumulate = 1: out holds on entry the
+ * part of each row already summed (UpdateRhs_Legacy's res before its mass
+ * loop, :6040-6115) and the 27 terms are added onto it one at a time, as the
+ * reference does; 0: the sum starts from zero.  A neighbour is one cell away;
+ * one outside the domain contributes 0.  The reference's ECSIM solver runs on
+ * one resolution level (UpdateJMassMatrix takes BlockTable[0]'s level,
+ * :3566): a mesh of several levels is refused with AMPS_E_STATE.  reps > 1
+ * repeats the launch (for timing). */
 int amps_gpu_ecsim_mass_apply(amps_ctx* ctx, const double* E_nodes, double c1, double c2, double* out,
-                              int32_t reps);
+                              int32_t accumulate, int32_t reps);
 
 /* ---- gridless cutoff tracer ------------------------------------------ */
 /* Parameters of one trace (SI).  The FP64 path is TraceTrajectoryImpl +
@@ -396,6 +435,21 @@ int amps_gpu_effective_cutoff(amps_ctx* ctx, const amps_trace_params* params, in
                               const double* upper, double abs_tol, double rel_tol,
                               int32_t trace_limit_forbidden, double* effective, double* allowed_width,
                               int32_t* refined, int32_t* status, int64_t* steps_total);
+/* BuildUniformSphereDirs (srcEarth/gridless/CutoffRigidityGridless.cpp:
+ * 1652-1674): n > 0 near-uniform unit vectors dirs[3k..3k+2] on the sphere
+ * (z uniform in [-1, 1], azimuth advanced by the golden angle).  Host code. */
+int amps_gpu_uniform_sphere_dirs(int32_t n, double* dirs);
+/* BuildLocalENU_GSM + LocalLonLatToDir_GSM (CutoffRigidityGridless.cpp:
+ * 1757-1800) for npts locations x0[3p] and ndir sky directions (lon_deg[k]
+ * azimuth from local north toward east, lat_deg[k] elevation toward up):
+ * dirs[3 (p ndir + k) + d]; frames[9p..9p+8] = E, N, U when non-NULL.  Host
+ * code. */
+int amps_gpu_local_dirs(int64_t npts, const double* x0, int32_t ndir, const double* lon_deg,
+                        const double* lat_deg, double* dirs, double* frames);
+/* The same for n (location, sky direction) pairs: dirs[3p..3p+2] for x0[3p],
+ * lon_deg[p], lat_deg[p].  Host code. */
+int amps_gpu_local_dirs_paired(int64_t n, const double* x0, const double* lon_deg, const double* lat_deg,
+                               double* dirs);
 /* Tracer tuning (autotuner): "workgroups" (persistent grid, 0 = 8 per CU),
  * "chunk" (steps between work-queue refills). */
 int amps_gpu_trace_set_param(amps_ctx* ctx, const char* name, int64_t value);

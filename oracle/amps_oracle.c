@@ -421,10 +421,20 @@ static int push_one(const orc_mesh* m, const orc_face* faces, const double* E_co
         if (m->gxmax[d] <= xInit[d]) xInit[d] = m->gxmax[d] - m->eps;
       }
       newNode = find_tnode(m, xInit, startNode);
-      if (newNode < 0) {
+      if (newNode < 0 && bc_mode != ORC_BC_USER) {
         for (int d = 0; d < 3; d++) xp[d] = xInit[d], vp[d] = vInit[d];
         return ORC_CODE_LEFT_DOMAIN;
       }
+    }
+    /* USER_FUNCTION (:1230-1231): ProcessOutsideDomainParticles(ptr, xInit,
+     * vInit, nIntersectionFace, newNode) receives the state on the face; the
+     * particle is then deleted (:1262-1265).  newNode is -1 where the
+     * reference aborts ("cannot find the node"). */
+    if (bc_mode == ORC_BC_USER) {
+      for (int d = 0; d < 3; d++) xp[d] = xInit[d], vp[d] = vInit[d];
+      *blk = newNode >= 0 ? m->t[newNode].block : -1;
+      *cell = 0;
+      return ORC_CODE_EXIT_FACE0 + nIntersectionFace;
     }
     /* specular reflection (:1236-1253); the reference then aborts in its
      * switch(code) — here the reflected particle is committed. */
@@ -572,10 +582,17 @@ static int push_rel_one(const orc_mesh* m, const orc_face* faces, const double* 
           if (m->gxmax[d] <= xInit[d]) xInit[d] = m->gxmax[d] - m->eps;
         }
         newNode = find_tnode(m, xInit, startNode);
-        if (newNode < 0) {
+        if (newNode < 0 && bc_mode != ORC_BC_USER) {
           for (int d = 0; d < 3; d++) xp[d] = xInit[d], vp[d] = vInit[d];
           return ORC_CODE_LEFT_DOMAIN;
         }
+      }
+      /* USER_FUNCTION (pic_mover_relativistic_boris.cpp:418-420) */
+      if (bc_mode == ORC_BC_USER) {
+        for (int d = 0; d < 3; d++) xp[d] = xInit[d], vp[d] = vInit[d];
+        *blk = newNode >= 0 ? m->t[newNode].block : -1;
+        *cell = 0;
+        return ORC_CODE_EXIT_FACE0 + nIntersectionFace;
       }
       double cc = 0.0;
       for (int d = 0; d < 3; d++) cc += faces[nIntersectionFace].norm[d] * vInit[d];
@@ -830,6 +847,68 @@ int orc_deposit(const orc_mesh* m, const double* B_corner, const double* charge,
     }
   }
   free(buf);
+  free(start);
+  free(idx);
+  return 0;
+}
+
+/* Particle energy and CFL numbers of UpdateJMassMatrix (pic_field_solver_
+ * ecsim.cpp:2229-2238 per particle, :2355-2358 per cell, :3925-3929 and
+ * :3972-3987 the reductions).  Per cell, in the order the particles are given
+ * (grouped by (block, cell)): ParticleEnergyCell += 0.5 mass vsqr with the
+ * weighted mass, vmean[spec] += sqrt(vsqr) dt, and cflCell[spec] =
+ * vmean[spec] / (count[spec] sqrt(dx^2+dy^2+dz^2)).  The totals are reduced
+ * in a fixed order: each block's cells in cell order, then the blocks in
+ * order (energy), and the maximum over cells of each species' cflCell
+ * starting from 0, a cell without particles of the species (0/0) never
+ * winning, as in the reference's comparison.  cell_energy[ncells] and
+ * cell_cfl[ncells][nspec] may be NULL. */
+int orc_energy_cfl(const orc_mesh* m, const double* mass, const double* weight, int nspec, double dt,
+                   int64_t n, const double* v, const int* spec, const int* blk, const int* cell,
+                   double* energy, double* cfl, double* cell_energy, double* cell_cfl) {
+  const int64_t ncells = (int64_t)m->nblocks * ORC_NCELL;
+  int64_t* start = (int64_t*)calloc((size_t)ncells + 1, sizeof(int64_t));
+  int64_t* idx = (int64_t*)malloc(sizeof(int64_t) * (size_t)(n > 0 ? n : 1));
+  for (int64_t p = 0; p < n; p++) start[(int64_t)blk[p] * ORC_NCELL + cell[p] + 1]++;
+  for (int64_t c = 0; c < ncells; c++) start[c + 1] += start[c];
+  int64_t* fill = (int64_t*)malloc(sizeof(int64_t) * (size_t)ncells);
+  memcpy(fill, start, sizeof(int64_t) * (size_t)ncells);
+  for (int64_t p = 0; p < n; p++) idx[fill[(int64_t)blk[p] * ORC_NCELL + cell[p]]++] = p;
+  free(fill);
+  for (int s = 0; s < nspec; s++) cfl[s] = 0.0;
+  double total = 0.0;
+  for (int b = 0; b < m->nblocks; b++) {
+    const orc_tnode* t = &m->t[m->block_node[b]];
+    double dx[3];
+    for (int d = 0; d < 3; d++) dx[d] = (t->xmax[d] - t->xmin[d]) / ORC_NC;
+    const double ldx = sqrt(dx[0] * dx[0] + dx[1] * dx[1] + dx[2] * dx[2]);
+    double eblk = 0.0;
+    for (int lc = 0; lc < ORC_NCELL; lc++) {
+      const int64_t c = (int64_t)b * ORC_NCELL + lc;
+      double ecell = 0.0, vmean[ORC_MAX_SPEC];
+      int cnt[ORC_MAX_SPEC];
+      for (int s = 0; s < nspec; s++) vmean[s] = 0.0, cnt[s] = 0;
+      for (int64_t q = start[c]; q < start[c + 1]; q++) {
+        const int64_t p = idx[q];
+        const int s = spec[p];
+        const double vx = v[p], vy = v[n + p], vz = v[2 * n + p];
+        const double vsqr = vx * vx + vy * vy + vz * vz;
+        const double ms = mass[s] * weight[s];
+        vmean[s] += sqrt(vsqr) * dt;
+        ecell += 0.5 * ms * vsqr;
+        cnt[s]++;
+      }
+      eblk += ecell;
+      if (cell_energy) cell_energy[c] = ecell;
+      for (int s = 0; s < nspec; s++) {
+        const double cc = vmean[s] / (cnt[s] * ldx);
+        if (cell_cfl) cell_cfl[c * nspec + s] = cc;
+        if (cfl[s] < cc) cfl[s] = cc;
+      }
+    }
+    total += eblk;
+  }
+  *energy = total;
   free(start);
   free(idx);
   return 0;
@@ -1147,6 +1226,16 @@ static double select_dt3d(const orc_trace_params* P, const double x[3], const do
 }
 
 static int isfin3(const double a[3]) { return isfinite(a[0]) && isfinite(a[1]) && isfinite(a[2]); }
+
+/* the two legacy step selectors, for their closed-form tests */
+double orc_select_dt(const orc_trace_params* P, const double x[3], const double p[3], double dtTrace,
+                     double timeRemaining) {
+  return select_dt(P, x, p, dtTrace, timeRemaining);
+}
+double orc_select_dt3d(const orc_trace_params* P, const double x[3], const double p[3], double dtTrace,
+                       double timeRemaining, const double boxmin[3], const double boxmax[3], double rInner) {
+  return select_dt3d(P, x, p, dtTrace, timeRemaining, boxmin, boxmax, rInner);
+}
 
 /* FinalizeUpperBoundedStep — util/TrajectoryTimeStep.h:15-24 */
 double orc_finalize_step(double dt, double timeRemaining) {

@@ -30,9 +30,12 @@ extern "C" {
 #define ORC_CODE_LEFT_DOMAIN 2    /* deleted on an outer face */
 #define ORC_CODE_FINISHED 3       /* _PARTICLE_MOTION_FINISHED_ */
 #define ORC_CODE_SPHERE 5         /* absorbed by the internal sphere */
+#define ORC_CODE_EXIT_FACE0 8     /* USER_FUNCTION mode: left through face (code - 8), state on the face */
 
+#define ORC_MAX_SPEC 8
 #define ORC_BC_DELETE 0
 #define ORC_BC_REFLECT 1
+#define ORC_BC_USER 2             /* ..._USER_FUNCTION_: ProcessOutsideDomainParticles gets the face state */
 
 #define ORC_MOVER_LAPENTA 0   /* PIC::Mover::Lapenta2017 (alpha-matrix form) */
 #define ORC_MOVER_BORIS 1     /* PIC::Mover::Boris (h/s rotation form) */
@@ -71,6 +74,10 @@ int64_t orc_build_nodes(const orc_mesh* m, int* node_of_corner, int64_t* keys);
 
 /* ECSIM J + mass matrix.  Particles may be in any order; within a cell they
  * are processed in array order.  J[nnodes*3], M[nnodes*243] are overwritten. */
+/* Particle energy and per-species CFL of UpdateJMassMatrix (see the .c) */
+int orc_energy_cfl(const orc_mesh* m, const double* mass, const double* weight, int nspec, double dt,
+                   int64_t n, const double* v, const int* spec, const int* blk, const int* cell,
+                   double* energy, double* cfl, double* cell_energy, double* cell_cfl);
 int orc_deposit(const orc_mesh* m, const double* B_corner, const double* charge,
                 const double* mass, const double* weight, double dt, int64_t n,
                 const double* x, const double* v, const int* spec, const int* blk,
@@ -195,6 +202,12 @@ void orc_set_trace_mesh(const orc_mesh* m, const double* Bm);
  * and the neighbour block across each face (6), edge (12) and corner (8) */
 void orc_block_neighbours(const orc_mesh* m, int block, int* minmax, int* face, int* edge,
                           int* corner);
+/* SelectLegacyCutoffTraceDt (CutoffRigidityGridless.cpp:1086-1152) and its
+ * Mode3D form (CutoffRigidityMode3D.cpp:881-936) at one state */
+double orc_select_dt(const orc_trace_params* P, const double x[3], const double p[3], double dtTrace,
+                     double timeRemaining);
+double orc_select_dt3d(const orc_trace_params* P, const double x[3], const double p[3], double dtTrace,
+                       double timeRemaining, const double boxmin[3], const double boxmax[3], double rInner);
 
 #ifdef __cplusplus
 }

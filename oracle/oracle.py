@@ -78,7 +78,13 @@ def lib():
         l.orc_deposit.argtypes = [vp, P(dbl), P(dbl), P(dbl), P(dbl), dbl, i64, P(dbl), P(dbl),
                                   P(C.c_int), P(C.c_int), P(C.c_int), P(C.c_int), i64, P(dbl),
                                   P(dbl), i32]
+        l.orc_energy_cfl.argtypes = [vp, P(dbl), P(dbl), i32, dbl, i64, P(dbl), P(C.c_int), P(C.c_int),
+                                     P(C.c_int), P(dbl), P(dbl), P(dbl), P(dbl)]
         l.orc_dipole_B.argtypes = [P(TraceParams), P(dbl), P(dbl)]
+        l.orc_select_dt.argtypes = [P(TraceParams), P(dbl), P(dbl), dbl, dbl]
+        l.orc_select_dt.restype = dbl
+        l.orc_select_dt3d.argtypes = [P(TraceParams), P(dbl), P(dbl), dbl, dbl, P(dbl), P(dbl), dbl]
+        l.orc_select_dt3d.restype = dbl
         l.orc_boris_step.argtypes = [P(TraceParams), P(dbl), P(dbl), dbl]
         l.orc_hc4_step.argtypes = [P(TraceParams), P(dbl), P(dbl), dbl]
         l.orc_step_checked.argtypes = [P(TraceParams), P(dbl), P(dbl), dbl, dbl]
@@ -316,6 +322,23 @@ def push(omesh, E, B, species, parts, mover=0, bc_mode=1, r_sphere=0.0, nthreads
     return p, code
 
 
+def energy_cfl(omesh, species, dt, parts, nthreads=0):
+    """Total particle energy, per-species max CFL and the per-cell values of
+    UpdateJMassMatrix (pic_field_solver_ecsim.cpp:2229-2238, 2355-2358,
+    3925-3929, 3972-3987), cells summed in the given particle order."""
+    ns = species.n
+    e = C.c_double()
+    cfl = np.zeros(ns)
+    nc = omesh.nblocks * 512
+    ce = np.zeros(nc)
+    cc = np.zeros((nc, ns))
+    lib().orc_energy_cfl(omesh.h, _p(species.mass, C.c_double), _p(species.weight, C.c_double), ns, float(dt),
+                         parts.n, _p(parts.v, C.c_double), _p(parts.spec, C.c_int), _p(parts.blk, C.c_int),
+                         _p(parts.cell, C.c_int), C.byref(e), _p(cfl, C.c_double), _p(ce, C.c_double),
+                         _p(cc, C.c_double))
+    return e.value, cfl, ce, cc
+
+
 def ecsim_neighbours(mesh, node_of_corner, keys):
     """[nnodes][27] neighbour nodes of the ECSIM corner stencil, -1 outside:
     slot ii = a(di) + 3a(dj) + 9a(dk), a(0,-1,+1) = (0,1,2), one cell of the
@@ -340,13 +363,14 @@ def ecsim_neighbours(mesh, node_of_corner, keys):
     return nbr
 
 
-def ecsim_mass_apply(nbr, Mm, E, c1, c2):
+def ecsim_mass_apply(nbr, Mm, E, c1, c2, seed=None):
     """The mass-matrix part of UpdateRhs_Legacy (pic_field_solver_ecsim.cpp:
     6116-6124): res_c += (Ex M[9ii+3c] + Ey M[9ii+3c+1] + Ez M[9ii+3c+2])
     * c1 * c2 over the 27 neighbours in slot order (MassMatrixOffsetTable,
-    :621-639), absent neighbours skipped."""
+    :621-639), absent neighbours skipped; res starts from seed (the row's
+    earlier terms) when given."""
     nn = len(Mm)
-    out = np.zeros((nn, 3))
+    out = np.zeros((nn, 3)) if seed is None else np.array(seed, np.float64, copy=True)
     for s in range(27):
         j = nbr[:, s]
         ok = j >= 0
@@ -415,6 +439,19 @@ def trap_sequence(params, x, p, B):
                                           _p(pi, C.c_double), _p(bi, C.c_double))
         out[i, 1], out[i, 2] = st["mirror"][0], st["bounce"][0]
     return out
+
+
+def select_dt(params, x, p, dt_trace, time_remaining):
+    """SelectLegacyCutoffTraceDt at one state (x, p in SI)."""
+    x, p = (np.ascontiguousarray(a, np.float64) for a in (x, p))
+    return lib().orc_select_dt(C.byref(params), _p(x, C.c_double), _p(p, C.c_double), dt_trace, time_remaining)
+
+
+def select_dt3d(params, x, p, dt_trace, time_remaining, boxmin, boxmax, r_inner):
+    """SelectLegacyCutoffTraceDt3D at one state (box and inner radius in metres)."""
+    x, p, bmin, bmax = (np.ascontiguousarray(a, np.float64) for a in (x, p, boxmin, boxmax))
+    return lib().orc_select_dt3d(C.byref(params), _p(x, C.c_double), _p(p, C.c_double), dt_trace, time_remaining,
+                                 _p(bmin, C.c_double), _p(bmax, C.c_double), r_inner)
 
 
 def finalize_step(dt, time_remaining):

@@ -4,6 +4,7 @@ import ctypes
 import os
 import re
 
+import numpy as np
 import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -36,3 +37,57 @@ def test_error_path_without_context():
     lib.amps_gpu_last_error.restype = ctypes.c_char_p
     assert lib.amps_gpu_synchronize(None) != 0
     assert b"null context" in lib.amps_gpu_last_error()
+
+
+def test_uniform_sphere_dirs_closed_forms():
+    """BuildUniformSphereDirs: unit vectors with z = 1 - 2(k + 1/2)/n and the
+    azimuth advancing by the golden angle 2 pi (1 - 1/phi); near-uniform
+    (every octant within 3 % of n/8, mean vector ~0)."""
+    import math
+
+    from amps_amd import _lib as L
+    from amps_amd import cutoff as CU
+
+    n = 4000
+    d = CU.uniform_sphere_dirs(n)
+    k = np.arange(n)
+    assert np.allclose(np.linalg.norm(d, axis=1), 1.0, rtol=0, atol=1e-15)
+    assert np.allclose(d[:, 2], 1.0 - 2.0 * (k + 0.5) / n, rtol=0, atol=1e-15)
+    ga = 2 * math.pi * (1 - 2 / (1 + math.sqrt(5)))
+    phi = np.arctan2(d[:, 1], d[:, 0])
+    step = np.mod(np.diff(phi) - ga + math.pi, 2 * math.pi) - math.pi
+    assert np.abs(step).max() < 1e-9
+    octant = (d[:, 0] > 0) * 4 + (d[:, 1] > 0) * 2 + (d[:, 2] > 0)
+    assert np.all(np.abs(np.bincount(octant, minlength=8) - n / 8) < 0.03 * n / 8)
+    assert np.linalg.norm(d.mean(0)) < 2e-3
+    assert np.array_equal(CU.uniform_sphere_dirs(1), [[1.0, 0.0, 0.0]])
+    with pytest.raises(L.AmpsGpuError):
+        CU.uniform_sphere_dirs(0)
+
+
+def test_local_enu_frame_and_sky_directions():
+    """BuildLocalENU_GSM / LocalLonLatToDir_GSM: a right-handed orthonormal
+    frame with U radial and E horizontal (perpendicular to +Z), the +X
+    fallback at the pole; lon 0 / lat 0 is +N, lon 90 is +E, lon 180 is -N,
+    lat 90 is +U, and a general (lon, lat) has the stated components."""
+    from amps_amd import cutoff as CU
+
+    rng = np.random.default_rng(3)
+    x0 = rng.normal(size=(200, 3)) * 7e6
+    x0[0] = [0.0, 0.0, 6.4e6]
+    lon = np.array([0.0, 90.0, 180.0, 0.0, 37.0])
+    lat = np.array([0.0, 0.0, 0.0, 90.0, -21.0])
+    d, f = CU.local_dirs(x0, lon, lat)
+    E, N, U = f[:, 0], f[:, 1], f[:, 2]
+    for a in (E, N, U):
+        assert np.allclose(np.linalg.norm(a, axis=1), 1.0, atol=1e-15)
+    assert np.allclose((E * N).sum(1), 0, atol=1e-15) and np.allclose((E * U).sum(1), 0, atol=1e-15)
+    assert np.allclose(np.cross(E, N), U, atol=1e-15)
+    assert np.allclose(U, x0 / np.linalg.norm(x0, axis=1)[:, None], atol=1e-15)
+    assert np.allclose(E[1:, 2], 0.0, atol=1e-15)
+    assert np.allclose(E[0], [0.0, -1.0, 0.0], atol=1e-15)  # unit(X x U) at the pole
+    assert np.allclose(d[:, 0], N, atol=1e-15) and np.allclose(d[:, 1], E, atol=1e-15)
+    assert np.allclose(d[:, 2], -N, atol=1e-15) and np.allclose(d[:, 3], U, atol=1e-15)
+    cl, sl = np.cos(np.radians(-21.0)), np.sin(np.radians(-21.0))
+    want = cl * np.sin(np.radians(37.0)) * E + cl * np.cos(np.radians(37.0)) * N + sl * U
+    assert np.allclose(d[:, 4], want, atol=1e-15)

@@ -644,6 +644,109 @@ def test_push_reflect_and_absorb_classification():
     assert np.array_equal(code3 == 2, refl)
 
 
+def _exit_case(E0):
+    """Particles aimed at each of the six faces of the root box from inside,
+    in a uniform E and zero B, with a step long enough to leave."""
+    m = M.uniform_mesh(2)
+    E = np.broadcast_to(np.asarray(E0, float), (m.nblocks, 729, 3)).copy()
+    B = np.zeros_like(E)
+    rng = np.random.default_rng(41)
+    xs, vs = [], []
+    for f in range(6):
+        d, sgn = f // 2, (1 if f % 2 else -1)
+        for _ in range(40):
+            x = rng.uniform(-0.8, 0.8, 3)
+            x[d] = sgn * rng.uniform(0.9, 0.98)
+            v = rng.normal(0, 0.05, 3)
+            v[d] = sgn * rng.uniform(1.0, 3.0)
+            xs.append(x), vs.append(v)
+    x = np.array(xs).T.copy()
+    v = np.array(vs).T.copy()
+    blk = m.locate(x.T)
+    return m, E, B, M.Particles(x, v, np.zeros(x.shape[1], np.int32), blk, M.cell_of(m, x, blk))
+
+
+@pytest.mark.parametrize("mover", [0, 1])
+def test_exit_state_closed_form(mover):
+    """USER_FUNCTION exits (pic_mover_boris.cpp:1156-1231) in a uniform E, zero
+    B: with vFinal = v + (q dt / m) E the exit face f is the first face the
+    chord xInit + t vMiddle meets, the state handed to the callback is
+    x = xInit + t_f vMiddle - EPS n_f and v = vInit + (t_f / dt)(vFinal - vInit),
+    t_f = (x_face - xInit) / vMiddle (closed form, to rounding).  The
+    specular-reflection mode places the particle at the same x with
+    v - 2 (v . n) n (the reflection this build commits)."""
+    E0 = np.array([0.3, -0.2, 0.1])
+    m, E, B, parts = _exit_case(E0)
+    om = O.Mesh(m)
+    dt = 0.2
+    sp = Species([1.0], [1.0], [dt])
+    u, cu = O.push(om, E, B, sp, parts, mover=mover, bc_mode=2)
+    r, cr = O.push(om, E, B, sp, parts, mover=mover, bc_mode=1)
+    assert np.all((cu >= 8) & (cu < 14))
+    face = cu - 8
+    v0 = parts.v
+    vF = v0 + dt * E0[:, None]
+    vM = 0.5 * (v0 + vF)
+    # first face reached by the chord: the smallest positive crossing time
+    xf = np.array([-1.0, 1.0])
+    t = np.full((6, parts.n), np.inf)
+    for f in range(6):
+        d = f // 2
+        tf = (xf[f % 2] - parts.x[d]) / vM[d]
+        t[f] = np.where(tf > 0, tf, np.inf)
+    assert np.array_equal(face, np.argmin(t, 0))
+    tf = t[face, np.arange(parts.n)]
+    n = np.zeros((3, parts.n))
+    n[face // 2, np.arange(parts.n)] = np.where(face % 2, 1.0, -1.0)
+    want_x = parts.x + tf * vM - om.eps * n
+    want_v = v0 + np.minimum(tf / dt, 1.0) * (vF - v0)
+    assert np.allclose(u.x, want_x, rtol=0, atol=1e-14)
+    assert np.allclose(u.v, want_v, rtol=1e-13, atol=1e-15)
+    # the face state sits EPS inside the face and in the block the mesh gives
+    on = want_x[face // 2, np.arange(parts.n)]
+    assert np.allclose(np.abs(on), 1.0 - om.eps, rtol=0, atol=1e-15)
+    assert np.array_equal(u.blk, m.locate(u.x.T))
+    # specular reflection: same place, normal velocity reversed
+    assert np.all(cr == -1)
+    assert np.array_equal(r.x, u.x)
+    vn = (u.v * n).sum(0)
+    assert np.allclose(r.v, u.v - 2 * vn * n, rtol=0, atol=1e-15)
+    assert np.all((r.v * n).sum(0) < 0)
+    # delete mode: the same particles are deleted with their pushed state
+    d_, cd = O.push(om, E, B, sp, parts, mover=mover, bc_mode=0)
+    assert np.all(cd == 2)
+
+
+def test_exit_state_relativistic_closed_form():
+    """The relativistic mover's USER_FUNCTION exits (pic_mover_relativistic_
+    boris.cpp:317-420) in zero fields: straight chords, the face crossing at
+    fraction t = (x_face - xInit) / (xFinal - xInit) of the sub-step, the
+    callback state x = xInit + t (xFinal - xInit) - EPS n with v unchanged."""
+    m, E, B, parts = _exit_case([0.0, 0.0, 0.0])
+    E[:] = 0.0
+    om = O.Mesh(m)
+    c = 299792458.0
+    parts.v *= 0.3 * c / np.linalg.norm(parts.v, axis=0)
+    dt = 2.0 / (0.3 * c)
+    sp = Species([1.0], [1.0], [dt])
+    u, cu = O.push(om, E, B, sp, parts, mover=2, bc_mode=2)
+    assert np.all((cu >= 8) & (cu < 14))
+    face = cu - 8
+    xF = parts.x + dt * parts.v
+    xf = np.array([-1.0, 1.0])
+    t = np.full((6, parts.n), np.inf)
+    for f in range(6):
+        d = f // 2
+        tf = (xf[f % 2] - parts.x[d]) / (xF[d] - parts.x[d])
+        t[f] = np.where(tf > 0, tf, np.inf)
+    assert np.array_equal(face, np.argmin(t, 0))
+    tf = t[face, np.arange(parts.n)]
+    n = np.zeros((3, parts.n))
+    n[face // 2, np.arange(parts.n)] = np.where(face % 2, 1.0, -1.0)
+    assert np.allclose(u.x, parts.x + tf * (xF - parts.x) - om.eps * n, rtol=0, atol=1e-14)
+    assert np.allclose(u.v, parts.v, rtol=1e-14, atol=0)
+
+
 def test_deposit_single_particle_closed_form():
     """One particle: J_c = q W_c (alpha v) / V, and the mass-matrix blocks sum
     over all (i, slot) pairs to (q^2 dt / 2 m V) alpha (sum W)^2."""
@@ -669,6 +772,58 @@ def test_deposit_single_particle_closed_form():
     assert np.allclose(J.sum(0), want_Jsum, rtol=1e-12)
     Ms = Mm.reshape(-1, 27, 9).sum(axis=(0, 1)).reshape(3, 3)
     assert np.allclose(Ms, q * h / V * alpha * t.sum() ** 2, rtol=1e-12)
+    # per corner: J_c = q W_c (alpha v) / V at the corner's node, and the
+    # (node i, slot of node j) block is (q h / V) W_i W_j alpha, with the slot
+    # a(dx) + 3 a(dy) + 9 a(dz), a(0, -1, +1) = (0, 1, 2) of the offset from
+    # corner i to corner j; every other entry is zero
+    nodes = noc[int(blk[0]) * 729 + c]
+    ijk = np.stack([c % 9, (c // 9) % 9, c // 81], 1)
+    a = {0: 0, -1: 1, 1: 2}
+    wantJ = np.zeros_like(J)
+    wantM = np.zeros_like(Mm)
+    for li in range(8):
+        wantJ[nodes[li]] = q * w[li] * (alpha @ parts.v[:, 0]) / V
+        for lj in range(8):
+            d = ijk[lj] - ijk[li]
+            slot = a[d[0]] + 3 * a[d[1]] + 9 * a[d[2]]
+            wantM[nodes[li], 9 * slot:9 * slot + 9] = (q * h / V * w[li] * w[lj] * alpha).ravel()
+    assert np.allclose(J, wantJ, rtol=1e-12, atol=1e-14 * np.abs(wantJ).max())
+    assert np.allclose(Mm, wantM, rtol=1e-12, atol=1e-14 * np.abs(wantM).max())
+    assert np.count_nonzero(np.abs(J).sum(1)) == 8
+
+
+def test_energy_cfl_closed_forms():
+    """UpdateJMassMatrix's particle energy is sum 0.5 m w |v|^2; each species'
+    CFL is the largest over cells of mean(|v|) dt / |dx| over the cell's
+    particles of that species (numpy restatement), and a cell without the
+    species never wins; equal speeds give u dt / |dx| of the finest cell."""
+    m = M.central_octant_mesh()
+    om = O.Mesh(m)
+    parts = M.per_cell_particles(m, [3, 2], [0.05, 0.005], drift=(0.01, 0, 0))
+    sp = Species([-1.0, 1.0], [0.01, 1.0], [1e-3, 1e-3], [2.0, 3.0])
+    dt = 0.7
+    e, cfl, ce, cc = O.energy_cfl(om, sp, dt, parts)
+    vs = (parts.v ** 2).sum(0)
+    assert np.isclose(e, (0.5 * sp.mass[parts.spec] * sp.weight[parts.spec] * vs).sum(), rtol=1e-13)
+    key = parts.blk.astype(np.int64) * 512 + parts.cell
+    dx = (m.blk_xmax - m.blk_xmin) / 8
+    ldx = np.sqrt((dx ** 2).sum(1))
+    for s in range(2):
+        sel = parts.spec == s
+        num = np.bincount(key[sel], np.sqrt(vs[sel]) * dt, minlength=m.nblocks * 512)
+        cnt = np.bincount(key[sel], minlength=m.nblocks * 512)
+        ok = cnt > 0
+        want = num[ok] / (cnt[ok] * np.repeat(ldx, 512)[ok])
+        assert np.isclose(cfl[s], want.max(), rtol=1e-14)
+        assert np.all(np.isnan(cc[~ok, s]))
+    # only species 0 present: species 1's CFL stays 0
+    one = parts.take(np.flatnonzero(parts.spec == 0))
+    u = 0.03
+    one.v = np.vstack([np.full(one.n, u), np.zeros(one.n), np.zeros(one.n)])
+    e1, cfl1, _, _ = O.energy_cfl(om, sp, dt, one)
+    assert cfl1[1] == 0.0
+    assert np.isclose(cfl1[0], u * dt / ldx.min(), rtol=1e-14)
+    assert np.isclose(e1, one.n * 0.5 * sp.mass[0] * sp.weight[0] * u * u, rtol=1e-13)
 
 
 def test_deposit_symmetry_and_linearity(amr):
@@ -814,3 +969,72 @@ def test_reference_refine_cutoff_threshold():
     assert not un and rc <= c <= rc + 1e-3 and it == int(np.ceil(np.log2(4.0 / 1e-3)))
     c, it, un = O.ref_refine_cutoff(1.0, 5.0, lambda R: 2 if R < 2.0 else 1)
     assert un and it == 3 and c == 2.0  # midpoints 3, 2 (allowed), 1.5 (unresolved)
+
+
+def _dt_state(R_GV=1.0, B0=1e-6):
+    import tests.tracer_setup as T
+
+    P = T.params()
+    P.moment_Am2 = 0.0
+    P.B_ext_T[2] = B0
+    m, q = P.m0_kg, P.q_C
+    p = R_GV * 1e9 * q / T.C
+    gamma = np.sqrt(1.0 + (p / (m * T.C)) ** 2)
+    v = p / (gamma * m)
+    omega = q * B0 / (gamma * m)
+    return P, np.array([0.0, p, 0.0]), v, omega, T.RE_AMPS
+
+
+def test_select_legacy_dt_closed_forms():
+    """SelectLegacyCutoffTraceDt (CutoffRigidityGridless.cpp:1086-1152) in a
+    uniform field: far from boundaries dt = 0.15/omega; near the inner sphere
+    or the box face dt = 0.2 d/v (d in Re times _EARTH__RADIUS_); the floor
+    max(1e-12, 1e-9 max(DT_TRACE, 1), 100 km / v); the remaining-time clamp;
+    no adaptation when adaptiveDt is off.  A wrong limiter factor, floor
+    distance or floor coefficient fails one of these."""
+    P, p, v, om, Re = _dt_state(B0=1e-6)
+    far = np.array([10.0 * Re, 0.0, 0.0])
+    assert np.isclose(O.select_dt(P, far, p, 1.0, 600.0), 0.15 / om, rtol=1e-14)
+    P, p, v, om, Re = _dt_state(B0=1e-9)  # gyro budget above DT_TRACE
+    assert O.select_dt(P, far, p, 1.0, 600.0) > 0.2 * 9.0 * Re / v * (1 - 1e-12)
+    near_in = np.array([1.3 * Re, 0.0, 0.0])
+    assert np.isclose(O.select_dt(P, near_in, p, 1.0, 600.0), 0.2 * (1.3 - 1.0) * Re / v, rtol=1e-12)
+    near_box = np.array([34.5 * Re, 0.0, 0.0])
+    assert np.isclose(O.select_dt(P, near_box, p, 1.0, 600.0), 0.2 * (35.0 - 34.5) * Re / v, rtol=1e-12)
+    near_box_z = np.array([3.0 * Re, 0.0, -34.9 * Re])
+    assert np.isclose(O.select_dt(P, near_box_z, p, 1.0, 600.0), 0.2 * 0.1 * Re / v, rtol=1e-10)
+    touch = np.array([1.001 * Re, 0.0, 0.0])
+    assert np.isclose(O.select_dt(P, touch, p, 1.0, 600.0), 100.0e3 / v, rtol=1e-14)
+    assert np.isclose(O.select_dt(P, touch, p, 1.0e7, 600.0), 1e-9 * 1.0e7, rtol=1e-14)
+    assert O.select_dt(P, touch, p, 1.0, 1e-5) == 1e-5
+    assert O.select_dt(P, far, p, 1.0, 0.01) == 0.01
+    P.adaptive = 0
+    assert O.select_dt(P, touch, p, 1.0, 600.0) == 1.0 and O.select_dt(P, touch, p, 1.0, 0.25) == 0.25
+    P.adaptive, P.dt_floor = 1, 0
+    assert np.isclose(O.select_dt(P, touch, p, 1.0, 600.0), 0.2 * 0.001 * Re / v, rtol=1e-9)
+    P.dist_limiter = 0
+    assert O.select_dt(P, touch, p, 1.0, 600.0) == 1.0
+
+
+def test_select_legacy_dt3d_closed_forms():
+    """SelectLegacyCutoffTraceDt3D (CutoffRigidityMode3D.cpp:881-936): the same
+    rules with the box and inner distances in metres, the inner distance
+    max(0, |x| - rInner), no box limit outside the box, and the limiter and
+    floor always on."""
+    P, p, v, om, Re = _dt_state(B0=1e-6)
+    bmin, bmax, rin = np.full(3, -20.0 * Re), np.full(3, 20.0 * Re), 1.0 * Re
+    far = np.array([10.0 * Re, 0.0, 0.0])
+    assert np.isclose(O.select_dt3d(P, far, p, 1.0, 600.0, bmin, bmax, rin), 0.15 / om, rtol=1e-14)
+    P, p, v, om, Re = _dt_state(B0=1e-9)
+    P.dist_limiter = P.dt_floor = 0  # ignored by the Mode3D selector
+    x = np.array([1.3 * Re, 0.0, 0.0])
+    assert np.isclose(O.select_dt3d(P, x, p, 1.0, 600.0, bmin, bmax, rin), 0.2 * 0.3 * Re / v, rtol=1e-12)
+    x = np.array([0.0, 19.5 * Re, 0.0])
+    assert np.isclose(O.select_dt3d(P, x, p, 1.0, 600.0, bmin, bmax, rin), 0.2 * 0.5 * Re / v, rtol=1e-12)
+    outside = np.array([0.0, 0.0, 25.0 * Re])  # dBox < 0: only the inner distance limits
+    assert np.isclose(O.select_dt3d(P, outside, p, 1.0, 600.0, bmin, bmax, rin), min(1.0, 0.2 * 24.0 * Re / v),
+                      rtol=1e-12)
+    inside_sphere = np.array([0.5 * Re, 0.0, 0.0])  # dInner = 0: the floor
+    assert np.isclose(O.select_dt3d(P, inside_sphere, p, 1.0, 600.0, bmin, bmax, rin), 100.0e3 / v, rtol=1e-14)
+    assert np.isclose(O.select_dt3d(P, inside_sphere, p, 1.0e7, 600.0, bmin, bmax, rin), 1e-2, rtol=1e-14)
+    assert O.select_dt3d(P, inside_sphere, p, 1.0, 2e-6, bmin, bmax, rin) == 2e-6

@@ -104,6 +104,81 @@ def test_push_relativistic_si_bitexact(gpu_ctx, bc):
     assert np.array_equal(g.blk[alive], o.blk[alive]) and np.array_equal(g.cell[alive], o.cell[alive])
 
 
+def _exit_cmp(ex, o, ocode):
+    sel = np.flatnonzero(ocode >= L.CODE_EXIT_FACE0)
+    assert len(sel) > 100
+    oi = sel[np.argsort(o.id[sel])]
+    gi = np.argsort(ex["id"])
+    assert np.array_equal(ex["id"][gi], o.id[oi])
+    assert np.array_equal(ex["x"][gi], o.x[:, oi].T) and np.array_equal(ex["v"][gi], o.v[:, oi].T)
+    assert np.array_equal(ex["face"][gi], ocode[oi] - L.CODE_EXIT_FACE0)
+    assert np.array_equal(ex["blk"][gi], o.blk[oi]) and np.array_equal(ex["spec"][gi], o.spec[oi])
+
+
+@pytest.mark.parametrize("mover", [L.MOVER_LAPENTA, L.MOVER_BORIS, L.MOVER_RELATIVISTIC])
+def test_user_mode_exit_list_bitexact(gpu_ctx, mover):
+    """USER_FUNCTION boundary mode: the device exit list holds exactly the
+    oracle's exits with the arguments of ProcessOutsideDomainParticles
+    (state moved onto the face, face index, block), bit for bit; the
+    survivors match the oracle too."""
+    mesh = M.central_octant_mesh()
+    if mover == L.MOVER_RELATIVISTIC:
+        c = 299792458.0
+        E, B = M.corner_fields(mesh, E0=(1.0e5, 0.0, 0.0), moment=0.1, bz0=0.05)
+        parts = M.uniform_particles(mesh, 40000, 1.0, seed_x=31, seed_v=32)
+        speed = np.random.default_rng(33).uniform(0.1, 0.95, parts.n) * c
+        parts.v *= speed / np.linalg.norm(parts.v, axis=0)
+        sp = Species([1.602176565e-19], [1.67262192e-27], [1.0e-8])
+    else:
+        E, B, parts, sp = _case(mesh, 40000, 0.5, 0.05, seed=31)
+    print(f"USER exit list mover={mover} n={parts.n}", flush=True)
+    st = PicState(mesh, E, B, sp, ctx=gpu_ctx)
+    st.set_particles(parts)
+    st.get_exits()
+    st.MoveParticles(mover=mover, bc_mode=L.BC_USER, r_sphere=0.2, compact=True)
+    ex = st.get_exits()
+    o, ocode = O.push(O.Mesh(mesh), E, B, sp, parts, mover=mover, bc_mode=L.BC_USER, r_sphere=0.2)
+    _exit_cmp(ex, o, ocode)
+    assert np.all((ex["face"] >= 0) & (ex["face"] < 6)) and np.all(ex["blk"] >= 0)
+    g, _ = _by_id(st.get_particles()[0])
+    alive = np.flatnonzero((ocode == 3) | (ocode == -1))
+    oa = o.take(alive[np.argsort(o.id[alive])])
+    assert np.array_equal(g.id, oa.id) and np.array_equal(g.x, oa.x) and np.array_equal(g.v, oa.v)
+    assert st.get_exits()["id"].size == 0  # cleared
+
+
+def test_user_mode_exit_list_graph_steps_and_capacity(gpu_ctx):
+    """The exit list accumulates over graph-captured steps exactly as over
+    eager ones (every particle lost without a sphere is an exit), and a full
+    list reports the dropped exits."""
+    mesh = M.central_octant_mesh()
+    E, B = M.corner_fields(mesh)
+    dt = 0.1 / (np.max(np.linalg.norm(B, axis=-1)) / 0.01)
+    sp = Species([-1.0, 1.0], [0.01, 1.0], [dt * 50, dt * 50])
+    res = []
+    for graph in (False, True):
+        st = PicState(mesh, E, B, sp, ctx=gpu_ctx)
+        st.generate_particles(mesh.ncells * 3, [0.05, 0.005], drift=(0.01, 0.0, 0.0), seed=19)
+        n0 = st.particle_count()
+        st.get_exits()
+        st.run_steps(6, dt * 50, bc_mode=L.BC_USER, r_sphere=0.0, use_graph=graph)
+        ex = st.get_exits()
+        assert n0 - st.particle_count() == len(ex["id"]) > 100
+        o = np.argsort(ex["id"])
+        res.append({k: v[o] for k, v in ex.items()})
+    for k in res[0]:
+        assert np.array_equal(res[0][k], res[1][k]), k
+    st = PicState(mesh, E, B, sp, ctx=gpu_ctx)
+    st.generate_particles(mesh.ncells * 3, [0.05, 0.005], drift=(0.01, 0.0, 0.0), seed=19)
+    gpu_ctx.set_param("exit_capacity", 16)
+    try:
+        st.run_steps(2, dt * 50, bc_mode=L.BC_USER, r_sphere=0.0, use_graph=False)
+        with pytest.raises(L.AmpsGpuError, match="dropped"):
+            st.get_exits()
+    finally:
+        gpu_ctx.set_param("exit_capacity", 1 << 20)
+
+
 def test_push_multistep_with_migration(gpu_ctx):
     mesh = M.central_octant_mesh()
     E, B, parts, sp = _case(mesh, 60000, 0.2, 0.05, seed=7)
@@ -198,6 +273,23 @@ def test_deposit_parity(gpu_ctx, dep_case, exact, kernel, nch):
         assert nj.max() <= 1e-12 and nm.max() <= 1e-12, (nj.max(), nm.max())
     gpu_ctx.set_param("deposit_exact", 0)
     set_deposit_variant(gpu_ctx, 1, 6)
+
+
+@pytest.mark.parametrize("precision", [L.FP64, L.FP32])
+def test_particle_energy_cfl_bitexact(gpu_ctx, dep_case, precision):
+    """UpdateJMassMatrix's particle energy and per-species max CFL on the
+    device, bit-exact with the oracle over the device order (FP32 mode: the
+    same sums over the FP32 velocities)."""
+    mesh, E, B, parts, sp, dt = dep_case
+    st = PicState(mesh, E, B, sp, ctx=gpu_ctx)
+    st.set_particles(parts, precision=precision)
+    st.MoveParticles(mover=L.MOVER_LAPENTA, bc_mode=L.BC_REFLECT, r_sphere=0.2, compact=True)
+    e, cfl = st.ParticleEnergyCfl(dt)
+    g, _ = st.get_particles()
+    eo, cflo, _, _ = O.energy_cfl(O.Mesh(mesh), sp, dt, g)
+    print(f"energy {e!r} vs {eo!r}, cfl {cfl} vs {cflo}", flush=True)
+    assert e == eo and np.array_equal(cfl, cflo)
+    assert e > 0 and np.all(cfl > 0)
 
 
 def test_node_gather_staged_matches_direct(gpu_ctx, dep_case):
@@ -375,12 +467,17 @@ def test_restart_records_round_trip(gpu_ctx, layout_name, weight):
     assert np.array_equal(h.blk, g.blk) and np.array_equal(h.cell, g.cell)
 
 
-def test_ecsim_mass_apply_bitexact(gpu_ctx, dep_case):
-    """The ECSIM mass-matrix term over the resident M of the deposit,
-    bit-exact with the oracle's restatement over the same M (two-level mesh:
-    resolution-change and boundary nodes included); linear in E; repeated
-    launches give the same result."""
-    mesh, E, B, parts, sp, dt = dep_case
+def test_ecsim_mass_apply_bitexact(gpu_ctx):
+    """The ECSIM mass-matrix term over the resident M of the deposit on a
+    single-level mesh (the reference solver's), bit-exact with the oracle's
+    restatement over the same M (boundary nodes included), also when added
+    onto a caller's partial row; linear in E; repeated launches give the same
+    result; a multi-level mesh is refused."""
+    mesh = M.uniform_mesh(2)
+    E, B = M.corner_fields(mesh)
+    parts = M.per_cell_particles(mesh, [4, 4], [0.05, 0.005], drift=(0.01, 0.0, 0.0))
+    dt = 0.1 / (np.max(np.linalg.norm(B, axis=-1)) / 0.01)
+    sp = Species([-1.0, 1.0], [0.01, 1.0], [dt, dt])
     print(f"ecsim mass term case n={parts.n}", flush=True)
     st = PicState(mesh, E, B, sp, ctx=gpu_ctx)
     st.set_particles(parts)
@@ -388,15 +485,24 @@ def test_ecsim_mass_apply_bitexact(gpu_ctx, dep_case):
     noc, keys = st.node_map()
     nbr = O.ecsim_neighbours(mesh, noc, keys)
     rng = np.random.default_rng(11)
-    E1, E2 = rng.normal(size=(2, len(keys), 3))
+    E1, E2, S = rng.normal(size=(3, len(keys), 3))
     c1, c2 = -4 * np.pi * dt * 0.51, 1.7
     g1 = st.MassMatrixApply(E1, c1, c2)
     assert np.array_equal(g1, O.ecsim_mass_apply(nbr, Mm, E1, c1, c2))
     g2 = st.MassMatrixApply(E2, c1, c2, reps=3)
     assert np.array_equal(g2, O.ecsim_mass_apply(nbr, Mm, E2, c1, c2))
+    gs = st.MassMatrixApply(E1, c1, c2, seed=S)
+    assert np.array_equal(gs, O.ecsim_mass_apply(nbr, Mm, E1, c1, c2, seed=S))
     g12 = st.MassMatrixApply(E1 + E2, c1, c2)
     assert np.allclose(g12, g1 + g2, rtol=1e-12, atol=1e-12 * np.abs(g12).max())
     assert np.count_nonzero(g1) > 0.9 * g1.size
+    amr = M.central_octant_mesh()
+    Ea, Ba = M.corner_fields(amr)
+    st2 = PicState(amr, Ea, Ba, sp, ctx=gpu_ctx)
+    st2.set_particles(M.per_cell_particles(amr, [1, 1], [0.05, 0.005]))
+    st2.UpdateJMassMatrix(dt, copy_out=False)
+    with pytest.raises(L.AmpsGpuError, match="single-level"):
+        st2.MassMatrixApply(np.zeros((st2.nnodes, 3)), c1, c2)
 
 
 def test_ecsim_mass_apply_single_particle_closed_form(gpu_ctx):
