@@ -153,15 +153,17 @@ def test_user_mode_exit_list_graph_steps_and_capacity(gpu_ctx):
     list reports the dropped exits."""
     mesh = M.central_octant_mesh()
     E, B = M.corner_fields(mesh)
-    dt = 0.1 / (np.max(np.linalg.norm(B, axis=-1)) / 0.01)
-    sp = Species([-1.0, 1.0], [0.01, 1.0], [dt * 50, dt * 50])
+    # thermal speeds of 0.3 and steps of 0.05 move a particle ~0.015 per
+    # step, so the cells next to the box (1/16 wide) empty within 6 steps
+    dt = 0.05
+    sp = Species([-1.0, 1.0], [0.01, 1.0], [dt, dt])
     res = []
     for graph in (False, True):
         st = PicState(mesh, E, B, sp, ctx=gpu_ctx)
-        st.generate_particles(mesh.ncells * 3, [0.05, 0.005], drift=(0.01, 0.0, 0.0), seed=19)
+        st.generate_particles(mesh.ncells * 3, [0.3, 0.3], drift=(0.01, 0.0, 0.0), seed=19)
         n0 = st.particle_count()
         st.get_exits()
-        st.run_steps(6, dt * 50, bc_mode=L.BC_USER, r_sphere=0.0, use_graph=graph)
+        st.run_steps(6, dt, bc_mode=L.BC_USER, r_sphere=0.0, use_graph=graph)
         ex = st.get_exits()
         assert n0 - st.particle_count() == len(ex["id"]) > 100
         o = np.argsort(ex["id"])
@@ -169,10 +171,10 @@ def test_user_mode_exit_list_graph_steps_and_capacity(gpu_ctx):
     for k in res[0]:
         assert np.array_equal(res[0][k], res[1][k]), k
     st = PicState(mesh, E, B, sp, ctx=gpu_ctx)
-    st.generate_particles(mesh.ncells * 3, [0.05, 0.005], drift=(0.01, 0.0, 0.0), seed=19)
+    st.generate_particles(mesh.ncells * 3, [0.3, 0.3], drift=(0.01, 0.0, 0.0), seed=19)
     gpu_ctx.set_param("exit_capacity", 16)
     try:
-        st.run_steps(2, dt * 50, bc_mode=L.BC_USER, r_sphere=0.0, use_graph=False)
+        st.run_steps(2, dt, bc_mode=L.BC_USER, r_sphere=0.0, use_graph=False)
         with pytest.raises(L.AmpsGpuError, match="dropped"):
             st.get_exits()
     finally:
