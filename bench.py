@@ -340,7 +340,8 @@ def main():
     ap.add_argument("--particles-log2", type=int, default=27)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true")
-    ap.add_argument("--tune-cache", default=os.path.join(ROOT, "gpurun_out", "tuning.json"))
+    ap.add_argument("--tune-cache", default=None,
+                    help="JSON file of tuning results to reuse (default: tune on every run)")
     ap.add_argument("--workload", choices=["all", "pic", "map", "mlp"], default="all")
     ap.add_argument("--train-samples-log2", type=int, default=20)
     ap.add_argument("--epochs", type=int, default=20)
@@ -368,7 +369,8 @@ def main():
 
     ctx = L.Context(0)
     arch, ncu, hbm = ctx.device_info()
-    os.makedirs(os.path.dirname(args.tune_cache), exist_ok=True)
+    if args.tune_cache:
+        os.makedirs(os.path.dirname(os.path.abspath(args.tune_cache)), exist_ok=True)
     if args.workload == "map":
         t0 = time.perf_counter()
         mp = cutoff_map_bench(ctx, ncu, n_train=1 << args.train_samples_log2, epochs=args.epochs,
