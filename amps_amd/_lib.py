@@ -137,6 +137,7 @@ def load():
         "amps_gpu_update_j_mass_matrix": (C.c_int, [vp, dbl, P(dbl), P(dbl)]),
         "amps_gpu_ecsim_mass_apply": (C.c_int, [vp, P(dbl), dbl, dbl, P(dbl), i32, i32]),
         "amps_gpu_particle_energy_cfl": (C.c_int, [vp, dbl, P(dbl), P(dbl)]),
+        "amps_gpu_particle_energy_cfl_noalloc": (C.c_int, [vp, dbl, C.c_void_p]),
         "amps_gpu_generate_particles": (C.c_int, [vp, i64, i32, P(dbl), P(dbl), C.c_uint64, i32]),
         "amps_gpu_run_steps": (C.c_int, [vp, P(PushOpts), dbl, i64, i32]),
         "amps_gpu_copy_probe": (C.c_int, [vp, i64, i32, P(dbl)]),

@@ -182,6 +182,8 @@ int build_tiles(amps_ctx* c);                 // push tiles from the current off
 int collect_exits(amps_ctx* c);
 int prepare_exits(amps_ctx* c);  // allocates the list (outside any capture)
 void exits_free(amps_ctx* c);
+// looks up this device's address of the energy/CFL scratch array (diag.hip)
+int energy_scratch_init(int device);
 }  // namespace amps
 
 #define HIP_TRY(expr)                                    \

@@ -135,6 +135,10 @@ extern "C" int amps_gpu_create(int device, amps_ctx** out) {
   HIP_TRY(hipSetDevice(device));
   amps_ctx* c = new amps_ctx();
   c->device = device;
+  if (int r = energy_scratch_init(device)) {
+    delete c;
+    return r;
+  }
   HIP_TRY(hipDeviceGetAttribute(&c->n_cu, hipDeviceAttributeMultiprocessorCount, device));
   HIP_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
   HIP_TRY(hipEventCreate(&c->ev0));

@@ -15,13 +15,28 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <mutex>
 #include <vector>
 
 #include "amps_common.h"
 
 using namespace amps;
 
+// Block partials of the allocation-free variant: [nb][1 + nspec] doubles for
+// at most NOALLOC_MAX_BLOCKS blocks of MAX_SPEC species (2.25 MiB).  Every
+// GPU the library runs on holds its own copy until the process ends, and
+// every call on that GPU shares it: calls that overlap on one GPU race.  Each
+// call writes every entry it reads, so nothing has to start at zero.
+constexpr int NOALLOC_MAX_BLOCKS = 32768;
+static __device__ __align__(256) double scratch[(size_t)NOALLOC_MAX_BLOCKS * (1 + MAX_SPEC)];
+
 namespace {
+
+// device address of this GPU's scratch array (host code must use it, never
+// the array's name), looked up once per device at context creation
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[MAX_DEVICES] = {};
 
 __global__ __launch_bounds__(64) void k_energy_cfl_block(MeshView m, Species sp, double dt,
                                                           const int* __restrict__ offset, int64_t cap,
@@ -102,6 +117,51 @@ __global__ void k_energy_cfl_total(int nb, int ns, const double* __restrict__ bl
 }
 
 }  // namespace
+
+namespace amps {
+// called by amps_gpu_create with the context's device current
+int energy_scratch_init(int device) {
+  if (device < 0 || device >= MAX_DEVICES) return fail(AMPS_E_ARG, "energy_scratch_init: device index out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_ptr[device]) return AMPS_OK;
+  void* ptr = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !ptr) return hip_fail(e != hipSuccess ? e : hipErrorInvalidSymbol, "hipGetSymbolAddress(scratch)");
+  g_scratch_ptr[device] = static_cast<double*>(ptr);
+  return AMPS_OK;
+}
+}  // namespace amps
+
+// The reductions of amps_gpu_particle_energy_cfl with no allocation: the block
+// partials go to the library's static scratch array and the result to the
+// caller's device array out[1 + nspec] (out[0] energy, out[1 + s] the CFL of
+// species s), on the context's stream, without synchronising.
+extern "C" int amps_gpu_particle_energy_cfl_noalloc(amps_ctx* c, double dt, double* out) {
+  if (!c || !out) return fail(AMPS_E_ARG, "amps_gpu_particle_energy_cfl_noalloc: null argument");
+  if (!c->has_mesh || !c->has_species)
+    return fail(AMPS_E_STATE, "amps_gpu_particle_energy_cfl_noalloc: mesh and species are required");
+  const int nb = c->mv.nb, ns = c->sp.n;
+  if (nb > NOALLOC_MAX_BLOCKS)
+    return fail(AMPS_E_RANGE, "amps_gpu_particle_energy_cfl_noalloc: " + std::to_string(nb) +
+                                  " blocks, more than the scratch array's " + std::to_string(NOALLOC_MAX_BLOCKS));
+  if (c->cap > 0 && !c->offsets_valid)
+    return fail(AMPS_E_STATE, "amps_gpu_particle_energy_cfl_noalloc: particles not sorted (call amps_gpu_compact)");
+  double* sc = (c->device >= 0 && c->device < MAX_DEVICES) ? g_scratch_ptr[c->device] : nullptr;
+  if (!sc) return fail(AMPS_E_STATE, "amps_gpu_particle_energy_cfl_noalloc: scratch not set up on this device");
+  (void)hipSetDevice(c->device);
+  if (c->cap == 0) {
+    // no particle arrays: energy and every CFL are 0
+    hipLaunchKernelGGL(k_energy_cfl_total, dim3(1), dim3(64), 0, c->stream, 0, ns, sc, out);
+    HIP_TRY(hipGetLastError());
+    return AMPS_OK;
+  }
+  const int k = c->cur;
+  hipLaunchKernelGGL(k_energy_cfl_block, dim3((unsigned)nb), dim3(64), 0, c->stream, c->mv, c->sp, dt, c->d_offset,
+                     c->cap, c->d_v[k], c->precision == AMPS_FP32 ? 1 : 0, c->d_spec[k], sc);
+  hipLaunchKernelGGL(k_energy_cfl_total, dim3(1), dim3(64), 0, c->stream, nb, ns, sc, out);
+  HIP_TRY(hipGetLastError());
+  return AMPS_OK;
+}
 
 extern "C" int amps_gpu_particle_energy_cfl(amps_ctx* c, double dt, double* energy, double* cfl) {
   if (!c || !energy) return fail(AMPS_E_ARG, "amps_gpu_particle_energy_cfl: null argument");

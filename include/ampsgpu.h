@@ -258,6 +258,19 @@ int amps_gpu_update_j_mass_matrix(amps_ctx* ctx, double dt, double* J_out, doubl
  * order, blocks in block order.  cfl may be NULL. */
 int amps_gpu_particle_energy_cfl(amps_ctx* ctx, double dt, double* energy, double* cfl);
 
+/* No counterpart in the reference: amps_gpu_particle_energy_cfl for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight
+ * loops, harnesses that forbid hipMalloc).  It calls no hipMalloc/hipFree:
+ * the block partials live in a static device array of the library sized for
+ * at most 32768 blocks (2.25 MiB per GPU, held until the process ends; more
+ * blocks give AMPS_E_RANGE), and the result goes to the caller's DEVICE array
+ * out[1 + nspec] (out[0] the energy, out[1 + s] the CFL of species s),
+ * asynchronously on the context's stream (amps_gpu_synchronize before
+ * reading it).  The particles must be sorted (amps_gpu_compact, or any step
+ * that compacts), else AMPS_E_STATE.  All calls on one GPU share the scratch
+ * array: calls that overlap on one GPU race. */
+int amps_gpu_particle_energy_cfl_noalloc(amps_ctx* ctx, double dt, double* out);
+
 /* The mass-matrix term of the ECSIM field equation over the resident M of
  * the last amps_gpu_update_j_mass_matrix: for every node n and component c,
  *   out[3n+c] = sum over slots s of ((Ex M[9s+3c] + Ey M[9s+3c+1]) + Ez M[9s+3c+2]) * c1 * c2
