@@ -98,6 +98,9 @@ _lib = None
 def _ptr(a, ctype):
     if a is None:
         return None
+    if not a.flags["C_CONTIGUOUS"]:
+        raise ValueError("arrays passed to the library must be C-contiguous (a strided view would be read "
+                         "as if packed)")
     return a.ctypes.data_as(C.POINTER(ctype))
 
 

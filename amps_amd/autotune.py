@@ -14,14 +14,15 @@ import numpy as np
 from . import _lib as L
 
 PUSH_TILES = (1024, 2048, 4096, 8192)
-# (cell-pass kernel, chunks per cell): 0 = per-lane particle loop,
-# 1 = shared-memory record batches
-DEPOSIT_VARIANTS = ((2, 4), (1, 4), (1, 6), (1, 12), (0, 6))
+# (cell-pass kernel, chunks): 0 = per-lane particle loop, 1 = shared-memory
+# record batches, 2 = FP64 matrix cores, 3 = eight lanes per cell (chunks =
+# particles per cell per record batch)
+DEPOSIT_VARIANTS = ((3, 8), (3, 4), (2, 4), (1, 4), (1, 6), (1, 12), (0, 6))
 
 
 def set_deposit_variant(ctx, kernel, nch):
-    """Select a deposit cell-pass variant (4 and 6 chunks suit both kernels)."""
-    ctx.set_param("deposit_chunks", 6)
+    """Select a deposit cell-pass variant (a chunk count of 4 suits every kernel)."""
+    ctx.set_param("deposit_chunks", 4)
     ctx.set_param("deposit_kernel", kernel)
     ctx.set_param("deposit_chunks", nch)
 

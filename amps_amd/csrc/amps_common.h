@@ -129,10 +129,12 @@ struct amps_ctx {
   int node_batch = 8;          // nodes per wavefront in the deposit node gather (0: grid-stride)
   int n_cu = 256;              // compute units of the device
   int node_staged = 1;         // moment-record node gather staged through LDS
+  int dep_waves = 2;           // eight-lanes-per-cell pass: register budget in wavefronts per SIMD (2 or 3)
   int dep_paired = 0;          // matrix-core cell pass: two wavefronts per SIMD in opposite phases
   int push_waves = 1;          // 4: push kernels built for 4 waves per SIMD (128 VGPRs)
   int dep_exact = 0;           // 1: unfused mul+add in the deposit (bit-exact order)
-  int dep_kernel = 2;          // cell pass: 0 per-lane particle loop, 1 LDS record batches, 2 matrix cores
+  int dep_kernel = 2;          // cell pass: 0 per-lane particle loop, 1 LDS record batches, 2 matrix cores,
+                               // 3 eight lanes per cell
   int b_center = 0;            // 1: B from the cell-centred field (mover and deposit)
   // deposit
   double* d_cellpart = nullptr;
@@ -173,8 +175,9 @@ void mlp_free(amps_ctx* c);
 // device-wide exclusive scan of n ints in place; total written to total_out
 int scan_exclusive(amps_ctx* c, int* data, int64_t n, int* total_out);
 // ECSIM cell pass with LDS record batches (deposit.hip)
-int launch_deposit_cell_rec(amps_ctx* c, double dt);
+int launch_deposit_cell_rec(amps_ctx* c, double dt, int nch);
 int launch_deposit_cell_mfma(amps_ctx* c, double dt);
+int launch_deposit_cell_lanes(amps_ctx* c, double dt);
 int alloc_particles(amps_ctx* c, int64_t n);  // both particle sets for at least n
 int build_tiles(amps_ctx* c);                 // push tiles from the current offsets
 // append the exits of the last push (codes AMPS_CODE_EXIT_FACE0..+5) to the

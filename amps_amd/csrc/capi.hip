@@ -195,6 +195,8 @@ extern "C" int amps_gpu_device_info(amps_ctx* c, char* name, int* n_cu, int64_t*
 static bool deposit_variant_ok(int kernel, int64_t nch) {
   // kernel 2 keeps the chunk count for its exact mode, which runs kernel 1
   if (kernel == 1 || kernel == 2) return nch == 4 || nch == 6 || nch == 12;
+  // kernel 3: particles per cell per record batch (its exact mode runs kernel 1 with 4 chunks)
+  if (kernel == 3) return nch == 4 || nch == 8;
   return nch == 2 || nch == 3 || nch == 4 || nch == 6;
 }
 
@@ -207,12 +209,15 @@ extern "C" int amps_gpu_set_param(amps_ctx* c, const char* name, int64_t value) 
     c->tiles_valid = false;
   } else if (n == "deposit_chunks") {
     if (!deposit_variant_ok(c->dep_kernel, value))
-      return fail(AMPS_E_ARG, "deposit_chunks must be 2, 3, 4 or 6 (deposit_kernel 0) or 4, 6 or 12 "
-                              "(deposit_kernel 1 and 2)");
+      return fail(AMPS_E_ARG, "deposit_chunks must be 2, 3, 4 or 6 (deposit_kernel 0), 4, 6 or 12 "
+                              "(deposit_kernel 1 and 2) or 4 or 8 (deposit_kernel 3)");
     c->dep_nch = (int)value;
   } else if (n == "push_waves") {
     if (value != 1 && value != 4) return fail(AMPS_E_ARG, "push_waves must be 1 or 4");
     c->push_waves = (int)value;
+  } else if (n == "deposit_waves") {
+    if (value != 2 && value != 3) return fail(AMPS_E_ARG, "deposit_waves must be 2 or 3");
+    c->dep_waves = (int)value;
   } else if (n == "deposit_paired") {
     if (value != 0 && value != 1) return fail(AMPS_E_ARG, "deposit_paired must be 0 or 1");
     c->dep_paired = (int)value;
@@ -232,7 +237,7 @@ extern "C" int amps_gpu_set_param(amps_ctx* c, const char* name, int64_t value) 
     if (value != 0 && value != 1) return fail(AMPS_E_ARG, "b_center must be 0 or 1");
     c->b_center = (int)value;
   } else if (n == "deposit_kernel") {
-    if (value < 0 || value > 2) return fail(AMPS_E_ARG, "deposit_kernel must be 0, 1 or 2");
+    if (value < 0 || value > 3) return fail(AMPS_E_ARG, "deposit_kernel must be 0, 1, 2 or 3");
     if (!deposit_variant_ok((int)value, c->dep_nch))
       return fail(AMPS_E_ARG, "deposit_kernel: set deposit_chunks to a count this kernel runs with first");
     c->dep_kernel = (int)value;
@@ -776,7 +781,7 @@ static int check_mover(const amps_ctx* c, const amps_push_opts* o, const char* w
     if (c->precision == AMPS_FP32 || o->mover == AMPS_MOVER_RELATIVISTIC)
       return fail(AMPS_E_ARG, std::string(who) + ": cell-centred B runs with the FP64 Lapenta and Boris movers");
     if (c->dep_kernel == 0)
-      return fail(AMPS_E_ARG, std::string(who) + ": cell-centred B runs with deposit_kernel 1 or 2");
+      return fail(AMPS_E_ARG, std::string(who) + ": cell-centred B runs with deposit_kernel 1, 2 or 3");
   }
   return 0;
 }
@@ -869,9 +874,11 @@ static int launch_deposit(amps_ctx* c, double dt) {
   const bool ex = c->dep_exact != 0;
   if (c->dep_kernel == 2 && !ex) {
     if (int r = launch_deposit_cell_mfma(c, dt)) return r;
+  } else if (c->dep_kernel == 3 && !ex) {
+    if (int r = launch_deposit_cell_lanes(c, dt)) return r;
   } else if (c->dep_kernel >= 1) {
-    // kernel 2 has no bit-exact form: its exact mode is kernel 1's
-    if (int r = launch_deposit_cell_rec(c, dt)) return r;
+    // kernels 2 and 3 have no bit-exact form: their exact mode is kernel 1's
+    if (int r = launch_deposit_cell_rec(c, dt, c->dep_kernel == 3 ? 4 : c->dep_nch)) return r;
   } else {
     switch (c->dep_nch) {
       case 2: ex ? launch_dep<2, true>(c, dt) : launch_dep<2, false>(c, dt); break;

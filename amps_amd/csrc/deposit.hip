@@ -353,9 +353,8 @@ void launch_rec2(amps_ctx* c, double dt) {
 }  // namespace
 
 namespace amps {
-int launch_deposit_cell_rec(amps_ctx* c, double dt) {
-  const bool ex = c->dep_exact != 0;
-  switch (c->dep_nch) {
+int launch_deposit_cell_rec(amps_ctx* c, double dt, int nch) {
+  switch (nch) {
     case 4: launch_rec2<4>(c, dt); break;
     case 6: launch_rec2<6>(c, dt); break;
     case 12: launch_rec2<12>(c, dt); break;

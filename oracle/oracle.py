@@ -53,7 +53,11 @@ def build():
 
 
 def _p(a, t):
-    return None if a is None else a.ctypes.data_as(C.POINTER(t))
+    if a is None:
+        return None
+    if not a.flags["C_CONTIGUOUS"]:
+        raise ValueError("oracle arrays must be C-contiguous (a strided view would be read as if packed)")
+    return a.ctypes.data_as(C.POINTER(t))
 
 
 def lib():

@@ -91,3 +91,23 @@ def test_local_enu_frame_and_sky_directions():
     cl, sl = np.cos(np.radians(-21.0)), np.sin(np.radians(-21.0))
     want = cl * np.sin(np.radians(37.0)) * E + cl * np.cos(np.radians(37.0)) * N + sl * U
     assert np.allclose(d[:, 4], want, atol=1e-15)
+
+
+def test_pointer_helpers_reject_strided_arrays():
+    """A strided view handed to the C ABI would be read as if packed: the
+    binding and the oracle refuse it instead."""
+    import numpy as np
+
+    from amps_amd import _lib as L
+    from oracle import oracle as O
+
+    a = np.zeros((3, 8))[:, ::2]
+    with pytest.raises(ValueError):
+        L._ptr(a, L.C.c_double)
+    with pytest.raises(ValueError):
+        O._p(a, L.C.c_double)
+    b = np.zeros((3, 8))[:, np.arange(4)]  # advanced indexing can return a Fortran-ordered copy
+    if not b.flags["C_CONTIGUOUS"]:
+        with pytest.raises(ValueError):
+            L._ptr(b, L.C.c_double)
+    assert L._ptr(np.ascontiguousarray(b), L.C.c_double) is not None

@@ -241,12 +241,12 @@ def _dep_case_data():
     return mesh, E, B, parts, sp, dt
 
 
-DEPOSIT_VARIANTS = [(2, 12), (1, 6), (1, 4), (1, 12), (0, 4), (0, 2), (0, 3), (0, 6)]
+DEPOSIT_VARIANTS = [(3, 8), (3, 4), (2, 12), (1, 6), (1, 4), (1, 12), (0, 4), (0, 2), (0, 3), (0, 6)]
 
 
 def set_deposit_variant(ctx, kernel, nch):
-    # chunk counts 4 and 6 are valid for both kernels, so pass through 6
-    ctx.set_param("deposit_chunks", 6)
+    # a chunk count of 4 is valid for every kernel, so pass through it
+    ctx.set_param("deposit_chunks", 4)
     ctx.set_param("deposit_kernel", kernel)
     ctx.set_param("deposit_chunks", nch)
 
@@ -275,6 +275,48 @@ def test_deposit_parity(gpu_ctx, dep_case, exact, kernel, nch):
         assert nj.max() <= 1e-12 and nm.max() <= 1e-12, (nj.max(), nm.max())
     gpu_ctx.set_param("deposit_exact", 0)
     set_deposit_variant(gpu_ctx, 1, 6)
+
+
+@pytest.mark.parametrize("kernel,nch", [(3, 8), (2, 12), (1, 6)])
+def test_deposit_fp32_mode_error(gpu_ctx, dep_case, kernel, nch):
+    """FP32 mode keeps velocities in FP32 and deposits in FP64: against the
+    FP64 oracle on the original FP64 velocities, M (which depends on x and B
+    only) stays within 1e-12 and J carries the velocity rounding (stated
+    here); against the oracle fed the same FP32-rounded velocities both are
+    within 1e-12."""
+    mesh, E, B, parts, sp, dt = dep_case
+    st = PicState(mesh, E, B, sp, ctx=gpu_ctx)
+    st.set_particles(parts, precision=L.FP32)
+    set_deposit_variant(gpu_ctx, kernel, nch)
+    try:
+        J, Mm = st.UpdateJMassMatrix(dt)
+    finally:
+        set_deposit_variant(gpu_ctx, 1, 6)
+    g, _ = st.get_particles()  # device order, velocities widened from FP32
+    om = O.Mesh(mesh)
+    noc, keys = om.build_nodes()
+    by_id = np.argsort(parts.id)
+    g64 = g.take(np.arange(g.n))
+    g64.v = np.ascontiguousarray(parts.v[:, by_id[np.searchsorted(parts.id[by_id], g.id)]])
+    assert np.array_equal(g64.id, g.id) and np.array_equal(g64.x, g.x)
+    Jo, Mo = O.deposit(om, B, sp, dt, g64, noc, len(keys))
+    Jr, Mr = O.deposit(om, B, sp, dt, g, noc, len(keys))
+
+    def rel(a, b):
+        return (np.linalg.norm(a - b, axis=1) / np.maximum(np.linalg.norm(b, axis=1), 1e-300)).max()
+
+    # J of the two species nearly cancels at many nodes (same drift, opposite
+    # charges), so its FP32-velocity error is stated relative to the largest
+    # node current; M does not depend on v
+    ej = np.abs(J - Jo).max() / np.abs(Jo).max()
+    em = rel(Mm, Mo)
+    ejr = np.abs(J - Jr).max() / np.abs(Jr).max()
+    dv = np.abs(g.v - g64.v).max() / np.abs(g64.v).max()
+    print(f"FP32 mode deposit kernel={kernel}: J error {ej:.2e} of max |J|, M rel error {em:.2e} "
+          f"against FP64 velocities (J {ejr:.2e} against the FP32-rounded ones; velocity rounding {dv:.2e})",
+          flush=True)
+    assert em <= 1e-12 and ej <= 1e-6
+    assert ejr <= 1e-12 and rel(Mm, Mr) <= 1e-12
 
 
 @pytest.mark.parametrize("precision", [L.FP64, L.FP32])
@@ -315,8 +357,8 @@ def test_node_gather_staged_matches_direct(gpu_ctx, dep_case):
         assert np.array_equal(J, J0) and np.array_equal(Mm, M0), k
 
 
-@pytest.mark.parametrize("kernel,paired", [(2, 1), (2, 0), (1, 0)])
-def test_deposit_ragged_cells(gpu_ctx, kernel, paired):
+@pytest.mark.parametrize("kernel,paired,nch", [(3, 0, 8), (3, 0, 4), (2, 1, 12), (2, 0, 12), (1, 0, 12)])
+def test_deposit_ragged_cells(gpu_ctx, kernel, paired, nch):
     """Empty cells, cells of 1-3 particles and cells of 65-80 (more than one
     64-particle chunk of the matrix-core pass) side by side."""
     mesh = M.central_octant_mesh()
@@ -333,7 +375,7 @@ def test_deposit_ragged_cells(gpu_ctx, kernel, paired):
     assert np.sum(counts == 0) > 1000 and counts.max() > 64 and np.sum((counts > 0) & (counts < 4)) > 100
     st = PicState(mesh, E, B, sp, ctx=gpu_ctx)
     st.set_particles(parts)
-    set_deposit_variant(gpu_ctx, kernel, 12)
+    set_deposit_variant(gpu_ctx, kernel, nch)
     gpu_ctx.set_param("deposit_paired", paired)
     try:
         J, Mm = st.UpdateJMassMatrix(dt)
