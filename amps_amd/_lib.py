@@ -62,6 +62,12 @@ def aos_layout_packed(weight=False, extra_bytes=0):
     return AosLayout(17 + 48 + (8 if weight else 0) + extra_bytes, 16, 1, 17, 41, 65 if weight else -1)
 
 
+class EcsimParams(C.Structure):
+    """amps_ecsim_params: dtTotal, theta, c dt, E_conv, B_conv, length_conv."""
+    _fields_ = [("dt", C.c_double), ("theta", C.c_double), ("cDt", C.c_double), ("E_conv", C.c_double),
+                ("B_conv", C.c_double), ("length_conv", C.c_double)]
+
+
 class TraceParams(C.Structure):
     _fields_ = [
         ("q_C", C.c_double), ("qabs_C", C.c_double), ("m0_kg", C.c_double),
@@ -140,6 +146,12 @@ def load():
         "amps_gpu_update_j_mass_matrix": (C.c_int, [vp, dbl, P(dbl), P(dbl)]),
         "amps_gpu_ecsim_mass_apply": (C.c_int, [vp, P(dbl), dbl, dbl, P(dbl), i32, i32]),
         "amps_gpu_particle_energy_cfl": (C.c_int, [vp, dbl, P(dbl), P(dbl)]),
+        "amps_gpu_ecsim_operator": (C.c_int, [vp, P(EcsimParams), P(dbl), P(dbl), i32]),
+        "amps_gpu_ecsim_rhs": (C.c_int, [vp, P(EcsimParams), P(dbl), P(dbl), P(dbl)]),
+        "amps_gpu_ecsim_solve": (C.c_int, [vp, P(EcsimParams), P(dbl), dbl, i32, i32, P(dbl), P(i32), P(dbl)]),
+        "amps_gpu_ecsim_update_b": (C.c_int, [vp, P(EcsimParams), P(dbl), P(dbl), P(dbl)]),
+        "amps_gpu_ecsim_time_step": (C.c_int, [vp, P(EcsimParams), P(dbl), P(dbl), P(dbl), dbl, i32, i32, P(i32),
+                                               P(dbl)]),
         "amps_gpu_particle_energy_cfl_noalloc": (C.c_int, [vp, dbl, C.c_void_p]),
         "amps_gpu_generate_particles": (C.c_int, [vp, i64, i32, P(dbl), P(dbl), C.c_uint64, i32]),
         "amps_gpu_run_steps": (C.c_int, [vp, P(PushOpts), dbl, i64, i32]),

@@ -140,6 +140,12 @@ struct amps_ctx {
   double* d_cellpart = nullptr;
   double* d_nodeout = nullptr;
   int* d_ecsim_nbr = nullptr;  // [nnodes][27] neighbour nodes of the ECSIM stencil, -1 outside
+  // ECSIM field equation (ecsim_field.hip)
+  uint8_t* d_ecsim_bnd = nullptr;  // [nnodes] 1: corner on the domain boundary
+  int* d_ecsim_cells8 = nullptr;   // [nnodes][8] cells at node + (ci, cj, ck), ci, cj, ck in {-1, 0}
+  int* d_ecsim_noc = nullptr;      // [nblocks][729] node of each block corner
+  double* d_ecsim_work = nullptr;  // field vectors, the Krylov basis and reduction partials
+  size_t ecsim_work_len = 0;       // doubles allocated at d_ecsim_work
   // exit list of the USER_FUNCTION boundary mode (exits.hip)
   int64_t exit_cap = (int64_t)1 << 20;  // records the list holds
   int64_t exit_alloc = 0;               // records allocated
@@ -187,6 +193,12 @@ int prepare_exits(amps_ctx* c);  // allocates the list (outside any capture)
 void exits_free(amps_ctx* c);
 // looks up this device's address of the energy/CFL scratch array (diag.hip)
 int energy_scratch_init(int device);
+// ECSIM corner-stencil neighbour table [nnodes][27] (ecsim.hip) and the test
+// that the mesh has one refinement level, as the field solver needs
+int build_nbr(amps_ctx* c);
+bool single_level(const amps_ctx* c);
+// field-solver tables and work buffers (ecsim_field.hip)
+void ecsim_field_free(amps_ctx* c);
 }  // namespace amps
 
 #define HIP_TRY(expr)                                    \

@@ -116,6 +116,7 @@ static void free_mesh(amps_ctx* c) {
   (void)hipFree(c->d_cellpart);
   (void)hipFree(c->d_nodeout);
   (void)hipFree(c->d_ecsim_nbr);
+  ecsim_field_free(c);
   c->d_ecsim_nbr = nullptr;
   (void)hipFree(c->d_E);
   (void)hipFree(c->d_B);

@@ -85,6 +85,9 @@ __global__ __launch_bounds__(256) void k_ecsim_mass(const double* __restrict__ r
   }
 }
 
+}  // namespace
+
+namespace amps {
 // neighbour table [nnodes][27] on the level-lmax lattice (-1: outside)
 int build_nbr(amps_ctx* c) {
   if (c->d_ecsim_nbr) return 0;
@@ -133,7 +136,7 @@ bool single_level(const amps_ctx* c) {
   return true;
 }
 
-}  // namespace
+}  // namespace amps
 
 extern "C" int amps_gpu_ecsim_mass_apply(amps_ctx* c, const double* E_nodes, double c1, double c2, double* out,
                                          int32_t accumulate, int32_t reps) {

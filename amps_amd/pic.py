@@ -201,6 +201,64 @@ class PicState:
                 "particle_energy_cfl")
         return e.value, cfl
 
+    # ---- the ECSIM field equation over the resident J and M ---------------
+    @staticmethod
+    def ecsim_params(dt, theta=0.5, cDt=None, E_conv=1.0, B_conv=1.0, length_conv=1.0):
+        """amps_ecsim_params; cDt defaults to dt (c = 1, normalized units)."""
+        return L.EcsimParams(dt, theta, dt if cDt is None else cDt, E_conv, B_conv, length_conv)
+
+    def EcsimOperator(self, params, x, reps=1):
+        """A x of GetStencil_Legacy + UpdateMatrixElement
+        (pic_field_solver_ecsim.cpp:740-1228, 6004-6016), [nnodes][3]."""
+        x = np.ascontiguousarray(x, np.float64)
+        y = np.zeros((self.nnodes, 3))
+        L.check(self.lib.amps_gpu_ecsim_operator(self.ctx.h, C.byref(params), L._ptr(x, C.c_double),
+                                                 L._ptr(y, C.c_double), int(reps)), "ecsim_operator")
+        return y
+
+    def EcsimRhs(self, params, E_n, B_c):
+        """UpdateRhs_Legacy (pic_field_solver_ecsim.cpp:6040-6137), [nnodes][3]."""
+        E_n = np.ascontiguousarray(E_n, np.float64)
+        B_c = np.ascontiguousarray(B_c, np.float64)
+        out = np.zeros((self.nnodes, 3))
+        L.check(self.lib.amps_gpu_ecsim_rhs(self.ctx.h, C.byref(params), L._ptr(E_n, C.c_double),
+                                            L._ptr(B_c, C.c_double), L._ptr(out, C.c_double)), "ecsim_rhs")
+        return out
+
+    def EcsimSolve(self, params, rhs, tol=1e-10, max_iter=500, restart=40):
+        """GMRES on the device: (dE, iterations, relative residual)."""
+        rhs = np.ascontiguousarray(rhs, np.float64)
+        dE = np.zeros((self.nnodes, 3))
+        it = C.c_int32()
+        rr = C.c_double()
+        L.check(self.lib.amps_gpu_ecsim_solve(self.ctx.h, C.byref(params), L._ptr(rhs, C.c_double), float(tol),
+                                              int(max_iter), int(restart), L._ptr(dE, C.c_double), C.byref(it),
+                                              C.byref(rr)), "ecsim_solve")
+        return dE, it.value, rr.value
+
+    def EcsimUpdateB(self, params, E_theta, B_c):
+        """UpdateB (pic_field_solver_ecsim.cpp:5160-5270), [nblocks * 512][3]."""
+        E_theta = np.ascontiguousarray(E_theta, np.float64)
+        B_c = np.ascontiguousarray(B_c, np.float64)
+        out = np.zeros_like(B_c)
+        L.check(self.lib.amps_gpu_ecsim_update_b(self.ctx.h, C.byref(params), L._ptr(E_theta, C.c_double),
+                                                 L._ptr(B_c, C.c_double), L._ptr(out, C.c_double)), "ecsim_update_b")
+        return out
+
+    def EcsimTimeStep(self, params, E, B_c, tol=1e-10, max_iter=500, restart=40):
+        """One field step (ECSIM::TimeStep): returns (E^{n+1}, E^{n+theta},
+        B^{n+1}, iterations, relative residual)."""
+        E = np.array(E, np.float64, copy=True, order="C")
+        B = np.array(B_c, np.float64, copy=True, order="C")
+        Eth = np.zeros_like(E)
+        it = C.c_int32()
+        rr = C.c_double()
+        L.check(self.lib.amps_gpu_ecsim_time_step(self.ctx.h, C.byref(params), L._ptr(E, C.c_double),
+                                                  L._ptr(Eth, C.c_double), L._ptr(B, C.c_double), float(tol),
+                                                  int(max_iter), int(restart), C.byref(it), C.byref(rr)),
+                "ecsim_time_step")
+        return E, Eth, B, it.value, rr.value
+
     def MassMatrixApply(self, E_nodes, c1, c2=1.0, reps=1, seed=None):
         """The ECSIM mass-matrix term over the resident M of the last
         UpdateJMassMatrix (UpdateRhs_Legacy, pic_field_solver_ecsim.cpp:

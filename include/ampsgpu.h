@@ -288,6 +288,64 @@ int amps_gpu_particle_energy_cfl_noalloc(amps_ctx* ctx, double dt, double* out);
 int amps_gpu_ecsim_mass_apply(amps_ctx* ctx, const double* E_nodes, double c1, double c2, double* out,
                               int32_t accumulate, int32_t reps);
 
+/* ---- ECSIM implicit field equation (single-level mesh) ---------------- */
+/* Units and step of the field solve: dtTotal, theta (0.5 in the reference),
+ * cDt = LightSpeed * dtTotal in the solver's units, and the conversions
+ * E_conv, B_conv, length_conv of pic_field_solver_ecsim.cpp:202-212.  The
+ * div-E correction (corrCoeff, the 375-point stencil) is not built: it is 0,
+ * the reference's default (:99). */
+typedef struct {
+  double dt;
+  double theta;
+  double cDt;
+  double E_conv;
+  double B_conv;
+  double length_conv;
+} amps_ecsim_params;
+
+/* y = A x, the matrix of GetStencil_Legacy (pic_field_solver_ecsim.cpp:
+ * 740-1228) with the mass term of UpdateMatrixElement (:6004-6016) over the
+ * resident M: identity - (c dt theta)^2 (Laplacian - grad div) + 4 pi dt
+ * theta M on the 27 corner neighbours; rows of corners on the domain
+ * boundary are the identity, and such corners are dropped from the other
+ * rows (:782-799, :1032).  x, y: [nnodes][3] in node-key order.  Needs the
+ * J/M of amps_gpu_update_j_mass_matrix and a single-level mesh (the
+ * reference's solver runs on one level), else AMPS_E_STATE.  reps > 1
+ * repeats the launch (timing). */
+int amps_gpu_ecsim_operator(amps_ctx* ctx, const amps_ecsim_params* p, const double* x, double* y, int32_t reps);
+
+/* The right-hand side of UpdateRhs_Legacy (:6040-6137) for every row: the
+ * 27 neighbours' E^n times the Laplacian / grad-div coefficients, the mass
+ * term and the node's J times -4 pi dt theta, and c dt theta curl B from the
+ * 8 + 8 cell-centred B of :1234-1375.  E_n: [nnodes][3]; B_c: [nblocks][512]
+ * [3] cell-centred B (cell i + 8j + 64k of each block); rhs: [nnodes][3], 0
+ * on boundary rows. */
+int amps_gpu_ecsim_rhs(amps_ctx* ctx, const amps_ecsim_params* p, const double* E_n, const double* B_c, double* rhs);
+
+/* Solves A dE = rhs on the device with restarted GMRES (zero initial guess,
+ * SetInitialGuess :6296-6301; stops when ||rhs - A dE|| <= tol ||rhs|| or
+ * after max_iter Arnoldi steps).  The reference calls SWMF's
+ * linear_solver_wrapper("GMRES") (srcInterface/LinearSystemCornerNode.h:3282);
+ * its iterates are not reproduced, its converged solution is.  iters: the
+ * Arnoldi steps taken; rel_residual: ||rhs - A dE|| / ||rhs|| of the result. */
+int amps_gpu_ecsim_solve(amps_ctx* ctx, const amps_ecsim_params* p, const double* rhs, double tol, int32_t max_iter,
+                         int32_t restart, double* dE, int32_t* iters, double* rel_residual);
+
+/* UpdateB (:5160-5270) for every cell: B^{n+1} = B^n - c dt curl E^{n+theta}
+ * from the 2x2 face averages of the cell's corners (the interior formula in
+ * every cell; the reference's Dirichlet/Neumann treatment of domain-boundary
+ * cells is left to the caller). */
+int amps_gpu_ecsim_update_b(amps_ctx* ctx, const amps_ecsim_params* p, const double* E_theta, const double* B_c,
+                            double* B_out);
+
+/* One field step of ECSIM::TimeStep (:6179-6270) on the device: the
+ * right-hand side, GMRES, ProcessFinalSolution (E^{n+theta} = dE / E_conv +
+ * E^n, :6311-6318), UpdateE (E^{n+1} = (E^{n+theta} - (1 - theta) E^n) /
+ * theta, :5909-5960) and UpdateB.  E: E^n in, E^{n+1} out; E_theta (may be
+ * NULL): E^{n+theta} out; B_c: B^n in, B^{n+1} out. */
+int amps_gpu_ecsim_time_step(amps_ctx* ctx, const amps_ecsim_params* p, double* E, double* E_theta, double* B_c,
+                             double tol, int32_t max_iter, int32_t restart, int32_t* iters, double* rel_residual);
+
 /* ---- gridless cutoff tracer ------------------------------------------ */
 /* Parameters of one trace (SI).  The FP64 path is TraceTrajectoryImpl +
  * TraceTrajectoryWithSingleRetry (CutoffRigidityGridless.cpp:1246-1496) with

@@ -488,3 +488,288 @@ def band_topology(states):
     out = np.zeros(15, np.int32)
     lib().orc_band_topology(_p(s, C.c_int), len(s), _p(out, C.c_int))
     return out
+
+
+# ----------------------------------------------------------------------------
+# ECSIM implicit field equation on a single-level mesh (GetStencil_Legacy,
+# UpdateMatrixElement, UpdateRhs_Legacy, ProcessFinalSolution, UpdateE,
+# UpdateB of pic_field_solver_ecsim.cpp).  The unknown is dE = E^{n+theta} -
+# E^n (times E_conv) at the corner nodes; domain-boundary corners are rows of
+# the identity with a zero right-hand side (:782-799) and are dropped from
+# the other rows (:1032).
+# ----------------------------------------------------------------------------
+def _st_add(st, other, di=0, dj=0, dk=0, f=1.0):
+    for (i, j, k), a in other.items():
+        key = (i + di, j + dj, k + dk)
+        st[key] = st.get(key, 0.0) + f * a
+
+
+def ecsim_stencils():
+    """The Laplacian and grad-div stencils of the corner field solver as the
+    reference composes them (pic_field_solver_ecsim.cpp:7185-7300): xEdge
+    averages of 4 points, d2E/dx2 = the x second difference of xEdge[0]
+    averaged over 2x2 (y, z) shifts, the mixed derivatives from differences
+    of xEdge[4] and xEdge[8].  Returns (lap[3], gd[3][3]) as dicts
+    {(di, dj, dk): coefficient} with offsets in {-1, 0, 1}."""
+    def edge(ir, jr, kr):
+        return {(i, j, k): 0.25 for i in ir for j in jr for k in kr}
+
+    xe0, xe4, xe8 = edge((0,), (-1, 0), (-1, 0)), edge((-1, 0), (0,), (-1, 0)), edge((-1, 0), (-1, 0), (0,))
+    f0 = {}
+    _st_add(f0, xe0, 1, 0, 0)
+    _st_add(f0, xe0, 0, 0, 0, -2.0)
+    _st_add(f0, xe0, -1, 0, 0)
+    dxx = {}
+    for i in range(2):
+        for j in range(2):
+            _st_add(dxx, f0, 0, i, j, 0.25)
+    dyy = {(j, i, k): a for (i, j, k), a in dxx.items()}
+    dzz = {(k, j, i): a for (i, j, k), a in dxx.items()}
+    dy0 = dict(xe4)
+    _st_add(dy0, xe4, 0, -1, 0, -1.0)
+    dz0 = dict(xe8)
+    _st_add(dz0, xe8, 0, 0, -1, -1.0)
+    dxy, dxz, dyz = {}, {}, {}
+    for l in range(2):
+        for m in range(2):
+            _st_add(dxy, dy0, 1, l, m, 0.25)
+            _st_add(dxy, dy0, 0, l, m, -0.25)
+            _st_add(dxz, dz0, 1, l, m, 0.25)
+            _st_add(dxz, dz0, 0, l, m, -0.25)
+            _st_add(dyz, dz0, l, 1, m, 0.25)
+            _st_add(dyz, dz0, l, 0, m, -0.25)
+    lap = [dxx, dyy, dzz]
+    gd = [[dxx, dxy, dxz], [dxy, dyy, dyz], [dxz, dyz, dzz]]
+    return lap, gd
+
+
+def _slot(di, dj, dk):
+    a = {0: 0, -1: 1, 1: 2}
+    return a[di] + 3 * a[dj] + 9 * a[dk]
+
+
+def ecsim_coefficients(dx, cDt, theta, length_conv=1.0, corr=0.0):
+    """The constant part of the matrix rows (identity, minus the Laplacian
+    times coeff^2, plus the grad-div times coeff_iVar coeff_iVarIndex;
+    GetStencil_Legacy :873-910) and the E^n coefficients of the right-hand
+    side (plus Laplacian, minus grad-div; :1152-1185), as [iVar][iVarIndex]
+    [slot] tables, accumulated in the reference's loop order.  dx: the cell
+    size per axis."""
+    coeff = [cDt / (dx[d] * length_conv) * theta for d in range(3)]
+    coeffSqr = [c * c for c in coeff]
+    lap, gd = ecsim_stencils()
+    K = np.zeros((3, 3, 27))
+    R = np.zeros((3, 3, 27))
+    for iVar in range(3):
+        for idim in range(3):
+            for (i, j, k), a in lap[idim].items():
+                K[iVar, iVar, _slot(i, j, k)] -= a * coeffSqr[idim]
+                R[iVar, iVar, _slot(i, j, k)] += a * coeffSqr[idim]
+        K[iVar, iVar, 0] += 1
+        for iv in range(3):
+            for (i, j, k), a in gd[iVar][iv].items():
+                K[iVar, iv, _slot(i, j, k)] += (1 - corr) * a * coeff[iVar] * coeff[iv]
+                R[iVar, iv, _slot(i, j, k)] -= (1 - corr) * a * coeff[iVar] * coeff[iv]
+    return K, R, coeff
+
+
+def ecsim_boundary(mesh, keys):
+    """Corner nodes on the domain boundary (isBoundaryCorner, :7011-7056, for
+    a mesh whose leaves cover the root box)."""
+    S = (1 << mesh.lmax) + 1
+    g = np.stack([keys % S, (keys // S) % S, keys // (S * S)], 1)
+    return ((g == 0) | (g == S - 1)).any(1)
+
+
+def ecsim_cells_around(mesh, node_of_corner, nbr):
+    """[nnodes][8]: the cell whose lower corner is node + (ci, cj, ck),
+    ci, cj, ck in {-1, 0}, at (ci+1) + 2(cj+1) + 4(ck+1); -1 outside.  Cell
+    ids are block * 512 + i + 8 j + 64 k."""
+    nn = len(nbr)
+    noc = np.asarray(node_of_corner).reshape(-1, 729)
+    lower = -np.ones(nn, np.int64)
+    ii, jj, kk = np.meshgrid(np.arange(8), np.arange(8), np.arange(8), indexing="ij")
+    corner = (ii + 9 * jj + 81 * kk).ravel()
+    cell = (ii + 8 * jj + 64 * kk).ravel()
+    for b in range(noc.shape[0]):
+        lower[noc[b, corner]] = b * 512 + cell
+    out = -np.ones((nn, 8), np.int64)
+    for ck in (-1, 0):
+        for cj in (-1, 0):
+            for ci in (-1, 0):
+                m = nbr[:, _slot(ci, cj, ck)]
+                out[:, (ci + 1) + 2 * (cj + 1) + 4 * (ck + 1)] = np.where(m >= 0, lower[np.maximum(m, 0)], -1)
+    return out
+
+
+def ecsim_operator(nbr, bnd, Mm, K, x, fourPiDtTheta):
+    """y = A x: boundary rows are the identity; row (n, iVar) of an interior
+    node sums over iVarIndex (outer) and slot (inner), as the solver's row
+    elements are ordered (iVarIndex 27 + slot), the value K + M
+    fourPiDtTheta (UpdateMatrixElement, :6004-6016) times x of the neighbour;
+    neighbours outside the domain or on its boundary are dropped."""
+    nn = len(nbr)
+    y = np.zeros((nn, 3))
+    for iVar in range(3):
+        acc = np.zeros(nn)
+        for iv in range(3):
+            for s in range(27):
+                j = nbr[:, s]
+                ok = (j >= 0) & ~bnd[np.maximum(j, 0)]
+                val = K[iVar, iv, s] + Mm[:, 9 * s + 3 * iVar + iv] * fourPiDtTheta
+                acc = np.where(ok, acc + val * x[np.maximum(j, 0), iv], acc)
+        y[:, iVar] = np.where(bnd, x[:, iVar], acc)
+    return y
+
+
+# cells (ci, cj, ck) offsets and signs of the curl-B terms of UpdateRhs_Legacy
+# (:1234-1375): per iVar, 16 entries (group of 4 x 4, ii outer, jj inner),
+# the first 8 for B component (iVar + 2) % 3, the next 8 for (iVar + 4) % 3;
+# each entry (coefficient axis, sign, ci, cj, ck)
+def _curlb_table():
+    a = (-1, 0)
+    t = {0: [], 1: [], 2: []}
+    for ii in range(2):
+        for jj in range(2):
+            t[0].append((1, +1, a[ii], 0, a[jj]))
+    for ii in range(2):
+        for jj in range(2):
+            t[0].append((1, -1, a[ii], -1, a[jj]))
+    for ii in range(2):
+        for jj in range(2):
+            t[0].append((2, -1, a[ii], a[jj], 0))
+    for ii in range(2):
+        for jj in range(2):
+            t[0].append((2, +1, a[ii], a[jj], -1))
+    for ii in range(2):
+        for jj in range(2):
+            t[1].append((2, +1, a[ii], a[jj], 0))
+    for ii in range(2):
+        for jj in range(2):
+            t[1].append((2, -1, a[ii], a[jj], -1))
+    for ii in range(2):
+        for jj in range(2):
+            t[1].append((0, -1, 0, a[jj], a[ii]))
+    for ii in range(2):
+        for jj in range(2):
+            t[1].append((0, +1, -1, a[jj], a[ii]))
+    for ii in range(2):
+        for jj in range(2):
+            t[2].append((0, +1, 0, a[jj], a[ii]))
+    for ii in range(2):
+        for jj in range(2):
+            t[2].append((0, -1, -1, a[jj], a[ii]))
+    for ii in range(2):
+        for jj in range(2):
+            t[2].append((1, -1, a[jj], 0, a[ii]))
+    for ii in range(2):
+        for jj in range(2):
+            t[2].append((1, +1, a[jj], -1, a[ii]))
+    return t
+
+
+CURLB_TABLE = _curlb_table()
+
+
+def ecsim_rhs(nbr, bnd, cells8, Mm, J, R, coeff, En, Bc, dtTotal, theta, E_conv=1.0, B_conv=1.0):
+    """UpdateRhs_Legacy (:6040-6137) for every interior row, in its order:
+    the 27 neighbours' E^n times the Laplacian / grad-div coefficients, the
+    mass-matrix part times -4 pi dt theta, the current times -4 pi dt theta,
+    then the 16 curl-B terms (coefficients +-coeff/4 on the 8 + 8 cell-centred
+    B of :1234-1375).  Boundary rows are 0."""
+    nn = len(nbr)
+    fpdt = 4 * np.pi * dtTotal * theta
+    c4 = [0.25 * c for c in coeff]
+    out = np.zeros((nn, 3))
+    for iVar in range(3):
+        res = np.zeros(nn)
+        for s in range(27):
+            j = np.maximum(nbr[:, s], 0)
+            e = En[j]
+            res = res + (e[:, 0] * R[iVar, 0, s] + e[:, 1] * R[iVar, 1, s] + e[:, 2] * R[iVar, 2, s]) * E_conv
+        for s in range(27):
+            j = np.maximum(nbr[:, s], 0)
+            e = En[j]
+            m = Mm[:, 9 * s + 3 * iVar:9 * s + 3 * iVar + 3]
+            res = res + (e[:, 0] * m[:, 0] + e[:, 1] * m[:, 1] + e[:, 2] * m[:, 2]) * (-fpdt) * E_conv
+        res = res + J[:, iVar] * (-4 * np.pi * dtTotal * theta)
+        for t, (ax, sg, ci, cj, ck) in enumerate(CURLB_TABLE[iVar]):
+            comp = (iVar + 2) % 3 if t < 8 else (iVar + 4) % 3
+            cl = cells8[:, (ci + 1) + 2 * (cj + 1) + 4 * (ck + 1)]
+            res = res + Bc[np.maximum(cl, 0), comp] * (sg * c4[ax]) * B_conv
+        out[:, iVar] = np.where(bnd, 0.0, res)
+    return out
+
+
+def ecsim_gmres(apply, b, tol, maxiter, restart):
+    """Restarted GMRES with modified Gram-Schmidt from x0 = 0, stopping at
+    ||r|| <= tol ||b|| (a CPU check of the device solver)."""
+    n = b.size
+    x = np.zeros(n)
+    bn = np.linalg.norm(b)
+    if bn == 0:
+        return x, 0, 0.0
+    it = 0
+    while it < maxiter:
+        r = b - apply(x)
+        beta = np.linalg.norm(r)
+        if beta <= tol * bn:
+            break
+        V = np.zeros((restart + 1, n))
+        H = np.zeros((restart + 1, restart))
+        V[0] = r / beta
+        k = 0
+        for k in range(restart):
+            w = apply(V[k])
+            for j in range(k + 1):
+                H[j, k] = w @ V[j]
+                w = w - H[j, k] * V[j]
+            H[k + 1, k] = np.linalg.norm(w)
+            it += 1
+            if H[k + 1, k] > 0:
+                V[k + 1] = w / H[k + 1, k]
+            e1 = np.zeros(k + 2)
+            e1[0] = beta
+            yk, *_ = np.linalg.lstsq(H[:k + 2, :k + 1], e1, rcond=None)
+            res = np.linalg.norm(e1 - H[:k + 2, :k + 1] @ yk)
+            if res <= tol * bn or it >= maxiter or H[k + 1, k] == 0:
+                break
+        x = x + V[:k + 1].T @ yk
+    return x, it, np.linalg.norm(b - apply(x)) / bn
+
+
+def ecsim_update_e(En, dE, theta, E_conv=1.0):
+    """ProcessFinalSolution (:6311-6318) then UpdateE (:5909-5960):
+    E^{n+theta} = dE / E_conv + E^n, E^{n+1} = (E^{n+theta} - (1 - theta) E^n) / theta."""
+    Eth = dE / E_conv + En
+    return Eth, (Eth - (1.0 - theta) * En) / theta
+
+
+def ecsim_update_b(node_of_corner, Eth, Bc, dx, cDt, E_conv=1.0, B_conv=1.0, length_conv=1.0):
+    """UpdateB (:5160-5270) for every cell: B^{n+1} = B^n - c dt curl E^{n+theta}
+    from the 2x2 face averages of the cell's 8 corners (coeff4 = c dt / dx / 4)."""
+    noc = np.asarray(node_of_corner).reshape(-1, 729)
+    nb = noc.shape[0]
+    c4 = [0.25 * (cDt / (dx[d] * length_conv)) for d in range(3)]
+    out = np.array(Bc, np.float64, copy=True)
+    ii, jj, kk = np.meshgrid(np.arange(8), np.arange(8), np.arange(8), indexing="ij")
+    for b in range(nb):
+        cell = (ii + 8 * jj + 64 * kk).ravel()
+        E = {}
+        for a in range(2):
+            for bb in range(2):
+                for cc in range(2):
+                    nd = noc[b, ((ii + a) + 9 * (jj + bb) + 81 * (kk + cc)).ravel()]
+                    E[(a, bb, cc)] = Eth[nd] * E_conv
+        t = np.zeros((len(cell), 3))
+        for p in range(2):
+            for q in range(2):
+                t[:, 0] += -c4[1] * (E[(p, 1, q)][:, 2] - E[(p, 0, q)][:, 2]) + c4[2] * (E[(p, q, 1)][:, 1] - E[(p, q, 0)][:, 1])
+        for p in range(2):
+            for q in range(2):
+                t[:, 1] += -c4[2] * (E[(p, q, 1)][:, 0] - E[(p, q, 0)][:, 0]) + c4[0] * (E[(1, p, q)][:, 2] - E[(0, p, q)][:, 2])
+        for p in range(2):
+            for q in range(2):
+                t[:, 2] += -c4[0] * (E[(1, p, q)][:, 1] - E[(0, p, q)][:, 1]) + c4[1] * (E[(p, 1, q)][:, 0] - E[(p, 0, q)][:, 0])
+        out[b * 512 + cell] = Bc[b * 512 + cell] + t / B_conv
+    return out
