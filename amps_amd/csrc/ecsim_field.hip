@@ -158,29 +158,64 @@ Coefs coefficients(const double dx[3], const amps_ecsim_params& p) {
 // ---------------------------------------------------------------------------
 // kernels: one thread per (node, component) row
 // ---------------------------------------------------------------------------
+// Eight nodes per workgroup, two per wavefront (as the mass term of
+// ecsim.hip): lane s (0..26) of each half reads slot s of its node's M row (72
+// contiguous bytes, consecutive lanes) and its neighbour's x, and writes the
+// 9 products (K + M 4 pi dt theta) x of the slot to LDS; then 24 lanes, one
+// per (node, component) row, add the row's 81 products in the row's element
+// order (iVarIndex outer, slot inner), skipping the dropped neighbours.
 __global__ __launch_bounds__(256) void k_ecsim_op(int64_t nn, const double* __restrict__ rows,
                                                   const int* __restrict__ nbr, const uint8_t* __restrict__ bnd,
                                                   const Coefs* __restrict__ C, double fpdt,
                                                   const double* __restrict__ x, double* __restrict__ y) {
-  const int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (r >= 3 * nn) return;
-  const int64_t n = r / 3;
-  const int iVar = (int)(r - 3 * n);
-  if (bnd[n]) {
-    y[r] = x[r];
-    return;
-  }
-  const double* M = rows + n * NODE_STRIDE + 3;
-  const int* nb = nbr + n * 27;
-  double acc = 0.0;
-  for (int iv = 0; iv < 3; iv++)
-    for (int s = 0; s < 27; s++) {
-      const int j = nb[s];
-      if (j < 0 || bnd[j]) continue;
-      const double val = C->K[iVar][iv][s] + M[9 * s + 3 * iVar + iv] * fpdt;
-      acc = acc + val * x[3 * (int64_t)j + iv];
+  __shared__ double term[8][3][3][27];
+  __shared__ uint64_t present[4];
+  const int64_t per = gridDim.x >> 3;
+  const int64_t chunk = (int64_t)(blockIdx.x & 7) * per + (blockIdx.x >> 3);
+  if (chunk * 8 >= nn) return;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int half = lane >> 5, s = lane & 31;
+  const int ln = wave * 2 + half;
+  const int64_t n = chunk * 8 + ln;
+  const bool act = n < nn && s < 27;
+  bool keep = false;
+  if (act) {
+    const int j = nbr[n * 27 + s];
+    keep = j >= 0 && !bnd[j];
+    if (keep) {
+      const double* m = rows + n * NODE_STRIDE + 3 + 9 * s;
+      const double* xj = x + 3 * (int64_t)j;
+      const double xv[3] = {xj[0], xj[1], xj[2]};
+#pragma unroll
+      for (int iVar = 0; iVar < 3; iVar++)
+#pragma unroll
+        for (int iv = 0; iv < 3; iv++) {
+          const double val = C->K[iVar][iv][s] + m[3 * iVar + iv] * fpdt;
+          term[ln][iVar][iv][s] = val * xv[iv];
+        }
     }
-  y[r] = acc;
+  }
+  const uint64_t pm = __ballot(keep);
+  if (lane == 0) present[wave] = pm;
+  __syncthreads();
+  if (threadIdx.x < 24) {
+    const int r = threadIdx.x / 3, iVar = threadIdx.x - 3 * r;
+    const int64_t nr = chunk * 8 + r;
+    if (nr < nn) {
+      double out;
+      if (bnd[nr]) {
+        out = x[3 * nr + iVar];
+      } else {
+        const uint64_t p = present[r >> 1] >> (32 * (r & 1));
+        double acc = 0.0;
+        for (int iv = 0; iv < 3; iv++)
+          for (int q = 0; q < 27; q++)
+            if ((p >> q) & 1) acc = acc + term[r][iVar][iv][q];
+        out = acc;
+      }
+      y[3 * nr + iVar] = out;
+    }
+  }
 }
 
 __global__ __launch_bounds__(256) void k_ecsim_rhs(int64_t nn, const double* __restrict__ rows,
@@ -466,8 +501,10 @@ Work carve(amps_ctx* c, int restart) {
 
 void launch_op(amps_ctx* c, const Work& W, double fpdt, const double* x, double* y) {
   const int64_t nn = c->nnodes;
-  hipLaunchKernelGGL(k_ecsim_op, grid_of(3 * nn), dim3(256), 0, c->stream, nn, c->d_nodeout, c->d_ecsim_nbr,
-                     c->d_ecsim_bnd, W.C, fpdt, x, y);
+  // 8 nodes per workgroup, the workgroup count rounded up to 8 (XCD chunks)
+  const dim3 grid((unsigned)((((nn + 7) / 8) + 7) / 8 * 8));
+  hipLaunchKernelGGL(k_ecsim_op, grid, dim3(256), 0, c->stream, nn, c->d_nodeout, c->d_ecsim_nbr, c->d_ecsim_bnd,
+                     W.C, fpdt, x, y);
 }
 
 void launch_dot(amps_ctx* c, const Work& W, const double* a, const double* b, double* out, bool sqrt_out) {

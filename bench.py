@@ -33,7 +33,8 @@ PMC_KERNELS = (("k_push_f64", "push"), ("k_push_f32", "push"), ("k_deposit_cell"
                ("k_deposit_cell_rec", "deposit_cell"), ("k_deposit_cell_mfma", "deposit_cell"),
                ("k_deposit_node", "deposit_node"), ("k_deposit_node_staged", "deposit_node"),
                ("k_scatter", "scatter"), ("k_count_keys", "count_keys"), ("k_copy_probe", "copy"),
-               ("k_ecsim_mass", "ecsim_mass"))
+               ("k_ecsim_mass", "ecsim_mass"), ("k_ecsim_op", "ecsim_op"),
+               ("k_deposit_cell_lanes", "deposit_cell"))
 
 
 def _pmc_pass(args, tune, counter, outdir):
@@ -90,6 +91,43 @@ def ecsim_mass_setup():
     st.UpdateJMassMatrix(dt, copy_out=False)
     En = np.random.default_rng(7).normal(size=(st.nnodes, 3))
     return c2, st, En, dt
+
+
+ECSIM_OP_NODE_BYTES = 243 * 8 + 27 * 4 + 27 + 24 + 24  # M row, neighbours, their boundary flags, own x, result
+
+
+def ecsim_field_bench(ctx2, st2, En, dt):
+    """The ECSIM field equation on the same single-level mesh and resident
+    J/M: the operator's time per launch (HIP events) against its algorithmic
+    bytes, and one field step (right-hand side, GMRES to 1e-8, E and B
+    updates) timed whole, with c dt theta / dx = 1."""
+    import numpy as np
+
+    from amps_amd import _lib as L
+
+    dx = float(st2.mesh.cell_size()[0][0])
+    p = st2.ecsim_params(dt, theta=0.5, cDt=2 * dx)
+    x = np.random.default_rng(3).normal(size=(st2.nnodes, 3))
+    st2.EcsimOperator(p, x)  # tables and work space
+    ctx2.set_timing(True)
+    ctx2.reset_timing()
+    st2.EcsimOperator(p, x, reps=20)
+    ctx2.set_timing(False)
+    ms, cnt = ctx2.timing()
+    op_ms = ms[L.K_ECSIM] / max(cnt[L.K_ECSIM], 1)
+    op_bytes = st2.nnodes * ECSIM_OP_NODE_BYTES
+    Bc = np.random.default_rng(4).normal(scale=1e-3, size=(st2.mesh.nblocks * 512, 3))
+    st2.EcsimTimeStep(p, En, Bc, tol=1e-8, max_iter=400, restart=40)
+    ctx2.sync()
+    t0 = time.perf_counter()
+    _, _, _, it, rr = st2.EcsimTimeStep(p, En, Bc, tol=1e-8, max_iter=400, restart=40)
+    t_step = time.perf_counter() - t0
+    return {"kernel": "k_ecsim_op", "operator_ms": round(op_ms, 4), "operator_GBps": round(op_bytes / (op_ms * 1e-3) / 1e9, 1),
+            "operator_frac_of_hbm_peak": round(op_bytes / (op_ms * 1e-3) / 1e9 / HBM_PEAK_GBPS, 4),
+            "operator_algorithmic_bytes": int(op_bytes), "step_s": round(t_step, 4), "gmres_iterations": int(it),
+            "relative_residual": rr, "c_dt_theta_over_dx": 1.0,
+            "note": "field step = host arrays in and out (PCIe included), right-hand side, GMRES(40) to 1e-8, "
+                    "E and B updates"}
 
 
 def pmc_traffic(args, tune):
@@ -421,6 +459,7 @@ def main():
         st.run_steps(1, dt, use_graph=False, **opts)
         c2, st2, En2, dt2 = ecsim_mass_setup()
         st2.MassMatrixApply(En2, -4 * np.pi * dt2 * 0.5)
+        st2.EcsimOperator(st2.ecsim_params(dt2, cDt=2 * float(st2.mesh.cell_size()[0][0])), En2)
         c2.close()
         ctx.copy_probe(COPY_PROBE_BYTES, reps=1)
         ctx.sync()
@@ -499,6 +538,7 @@ def main():
                  "traffic": traffic.get("ecsim_mass") if traffic else None,
                  "algorithmic_bytes": int(e_bytes), "GBps": round(e_bytes / (e_ms * 1e-3) / 1e9, 1),
                  "frac_of_hbm_peak": round(e_bytes / (e_ms * 1e-3) / 1e9 / HBM_PEAK_GBPS, 4)}
+        ecsim["field_solve"] = ecsim_field_bench(c2, st2, En, dt2)
         c2.close()
     except Exception as e:  # the particle-step line stays valid
         ecsim = {"error": repr(e)}
