@@ -129,6 +129,8 @@ struct amps_ctx {
   int node_batch = 8;          // nodes per wavefront in the deposit node gather (0: grid-stride)
   int n_cu = 256;              // compute units of the device
   int node_staged = 1;         // moment-record node gather staged through LDS
+  int deterministic = 1;       // migration: in-key order independent of atomic scheduling
+  int mig_small = 64;          // movers per key sorted by one lane (more: a workgroup merge sort)
   int dep_waves = 2;           // eight-lanes-per-cell pass: register budget in wavefronts per SIMD (2 or 3)
   int dep_paired = 0;          // matrix-core cell pass: two wavefronts per SIMD in opposite phases
   int push_waves = 1;          // 4: push kernels built for 4 waves per SIMD (128 VGPRs)
@@ -146,6 +148,16 @@ struct amps_ctx {
   int* d_ecsim_noc = nullptr;      // [nblocks][729] node of each block corner
   double* d_ecsim_work = nullptr;  // field vectors, the Krylov basis and reduction partials
   size_t ecsim_work_len = 0;       // doubles allocated at d_ecsim_work
+  // deterministic migration: the offsets of the current array layout, keys
+  // with many movers, the movers' destinations and source positions, and the
+  // stayer count (scanned) and mask of every wavefront of the scatter grid
+  int* d_off_layout = nullptr;  // [nkeys + 1]
+  int* d_big = nullptr;         // [nkeys + 1]: keys with many movers, count last
+  int* d_dest = nullptr;        // [cap]
+  int* d_tmpsrc = nullptr;      // [cap]
+  int* d_wcnt = nullptr;        // [ceil(cap / 256) * 4 + 1]
+  uint64_t* d_wmask = nullptr;  // [ceil(cap / 256) * 4]
+  bool layout_valid = false;
   // exit list of the USER_FUNCTION boundary mode (exits.hip)
   int64_t exit_cap = (int64_t)1 << 20;  // records the list holds
   int64_t exit_alloc = 0;               // records allocated

@@ -28,6 +28,14 @@ __global__ void k_scatter(const int*, const int*, const int8_t*, const int*, int
                           const double*, const double*, int, const uint8_t*, const int64_t*,
                           double*, double*, uint8_t*, int*, int64_t*);
 __global__ void k_copy_total(const int*, int*);
+__global__ void k_mig_classify(const int*, const int*, const int8_t*, const int*, int, const int*, int*, int*, int*,
+                               uint64_t*);
+__global__ void k_mig_order(int64_t, int, const int*, const int*, int*, int*, int*);
+__global__ void k_mig_order_big(int64_t, const int*, const int*, int*, int*, int*, const int*);
+__global__ void k_mig_scatter(const int*, const int*, const int8_t*, const int*, const int*, const int*,
+                              const uint64_t*, const int*, int64_t, const double*, const double*, int, const uint8_t*,
+                              const int64_t*, double*, double*, uint8_t*, int*, int64_t*);
+__global__ void k_copy_ints(int64_t, const int*, int*);
 __global__ void k_tile_count(int, int, const int*, int*);
 __global__ void k_tile_emit(int, int, const int*, const int*, Tile*, int*);
 template <int NCH, bool EXACT>
@@ -86,6 +94,13 @@ static void free_particles(amps_ctx* c) {
   c->d_code = nullptr;
   (void)hipFree(c->d_code2);
   c->d_code2 = nullptr;
+  (void)hipFree(c->d_dest);
+  (void)hipFree(c->d_tmpsrc);
+  (void)hipFree(c->d_wcnt);
+  (void)hipFree(c->d_wmask);
+  c->d_dest = c->d_tmpsrc = c->d_wcnt = nullptr;
+  c->d_wmask = nullptr;
+  c->layout_valid = false;
   c->cap = 0;
   c->n = 0;
 }
@@ -111,6 +126,10 @@ static void free_mesh(amps_ctx* c) {
   (void)hipFree(c->d_node_order);
   c->d_node_order = nullptr;
   (void)hipFree(c->d_offset);
+  (void)hipFree(c->d_off_layout);
+  (void)hipFree(c->d_big);
+  c->d_off_layout = c->d_big = nullptr;
+  c->layout_valid = false;
   (void)hipFree(c->d_cursor);
   (void)hipFree(c->d_tcount);
   (void)hipFree(c->d_cellpart);
@@ -216,6 +235,12 @@ extern "C" int amps_gpu_set_param(amps_ctx* c, const char* name, int64_t value) 
   } else if (n == "push_waves") {
     if (value != 1 && value != 4) return fail(AMPS_E_ARG, "push_waves must be 1 or 4");
     c->push_waves = (int)value;
+  } else if (n == "deterministic") {
+    if (value != 0 && value != 1) return fail(AMPS_E_ARG, "deterministic must be 0 or 1");
+    c->deterministic = (int)value;
+  } else if (n == "migration_sort_small") {
+    if (value < 1 || value > 64) return fail(AMPS_E_ARG, "migration_sort_small must be in [1, 64]");
+    c->mig_small = (int)value;
   } else if (n == "deposit_waves") {
     if (value != 2 && value != 3) return fail(AMPS_E_ARG, "deposit_waves must be 2 or 3");
     c->dep_waves = (int)value;
@@ -436,7 +461,10 @@ extern "C" int amps_gpu_set_mesh(amps_ctx* c, const amps_mesh_desc* desc, const 
   }
   const int64_t nkeys = (int64_t)nb * NCELL;
   HIP_TRY(hipMalloc(&c->d_offset, sizeof(int) * (nkeys + 1)));
+  HIP_TRY(hipMalloc(&c->d_off_layout, sizeof(int) * (nkeys + 1)));
+  HIP_TRY(hipMalloc(&c->d_big, sizeof(int) * (nkeys + 1)));
   HIP_TRY(hipMalloc(&c->d_cursor, sizeof(int) * nkeys));
+  c->layout_valid = false;
   HIP_TRY(hipMalloc(&c->d_tcount, sizeof(int) * (nb + 1)));
   m.blk_xmin = c->d_xmin;
   m.blk_xmax = c->d_xmax;
@@ -585,7 +613,15 @@ int amps::alloc_particles(amps_ctx* c, int64_t n) {
   }
   HIP_TRY(hipMalloc(&c->d_code, (cap + 4) & ~(int64_t)3));
   HIP_TRY(hipMalloc(&c->d_code2, (cap + 4) & ~(int64_t)3));
+  HIP_TRY(hipMalloc(&c->d_dest, sizeof(int) * cap));
+  HIP_TRY(hipMalloc(&c->d_tmpsrc, sizeof(int) * cap));
+  {
+    const int64_t nw = (cap + 255) / 256 * 4;
+    HIP_TRY(hipMalloc(&c->d_wcnt, sizeof(int) * (nw + 1)));
+    HIP_TRY(hipMalloc(&c->d_wmask, sizeof(uint64_t) * nw));
+  }
   c->cap = cap;
+  c->layout_valid = false;
   return 0;
 }
 
@@ -662,6 +698,8 @@ extern "C" int amps_gpu_set_particles(amps_ctx* c, int64_t n, const double* x, c
   HIP_TRY(hipMemcpy(c->d_id[0], hid.data(), sizeof(int64_t) * cap, hipMemcpyHostToDevice));
   HIP_TRY(hipMemset(c->d_code, AMPS_CODE_FINISHED, cap));
   HIP_TRY(hipMemcpy(c->d_offset, cnt.data(), sizeof(int) * (nkeys + 1), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(c->d_off_layout, cnt.data(), sizeof(int) * (nkeys + 1), hipMemcpyHostToDevice));
+  c->layout_valid = true;
   const int nn = (int)n;
   HIP_TRY(hipMemcpy(c->d_n, &nn, sizeof(int), hipMemcpyHostToDevice));
   c->n = n;
@@ -746,10 +784,35 @@ static int compact_impl(amps_ctx* c, bool counted = false) {
                        c->d_code, c->d_offset);
   }
   if (int r = scan_exclusive(c, c->d_offset, nkeys + 1, nullptr)) return r;
-  hipLaunchKernelGGL(k_scatter, dim3(gridp), dim3(256), 0, c->stream, c->d_n, c->d_key[a],
-                     c->d_code, c->d_offset, c->d_cursor, cap, c->d_x[a], c->d_v[a],
-                     c->precision == AMPS_FP32 ? 1 : 0, c->d_spec[a], c->d_id[a], c->d_x[b], c->d_v[b], c->d_spec[b], c->d_key[b],
-                     c->d_id[b]);
+  const int vf32 = c->precision == AMPS_FP32 ? 1 : 0;
+  if (c->deterministic) {
+    // in every key, the particles that stayed in array order, then those that
+    // moved in, in array order (every particle a mover when the arrays have no
+    // layout of a previous compaction yet, e.g. a generated population)
+    const unsigned gridk = (unsigned)((nkeys + 255) / 256);
+    const int64_t nw = (int64_t)gridp * 4;
+    hipLaunchKernelGGL(k_mig_classify, dim3(gridp), dim3(256), 0, c->stream, c->d_n, c->d_key[a], c->d_code,
+                       c->d_off_layout, c->layout_valid ? 1 : 0, c->d_offset, c->d_cursor, c->d_tmpsrc, c->d_wcnt,
+                       c->d_wmask);
+    if (int r = scan_exclusive(c, c->d_wcnt, nw + 1, nullptr)) return r;
+    hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, c->stream, c->d_big + nkeys, 1, 0);
+    hipLaunchKernelGGL(k_mig_order, dim3(gridk), dim3(256), 0, c->stream, nkeys, c->mig_small, c->d_offset,
+                       c->d_cursor, c->d_tmpsrc, c->d_dest, c->d_big);
+    // spare array of the merge sort: the other buffer's keys, written later
+    hipLaunchKernelGGL(k_mig_order_big, dim3(512), dim3(256), 0, c->stream, nkeys, c->d_offset, c->d_cursor,
+                       c->d_tmpsrc, c->d_key[b], c->d_dest, c->d_big);
+    hipLaunchKernelGGL(k_mig_scatter, dim3(gridp), dim3(256), 0, c->stream, c->d_n, c->d_key[a], c->d_code,
+                       c->d_offset, c->d_off_layout, c->d_wcnt, c->d_wmask, c->d_dest, cap, c->d_x[a], c->d_v[a],
+                       vf32, c->d_spec[a], c->d_id[a], c->d_x[b], c->d_v[b], c->d_spec[b], c->d_key[b], c->d_id[b]);
+  } else {
+    hipLaunchKernelGGL(k_scatter, dim3(gridp), dim3(256), 0, c->stream, c->d_n, c->d_key[a], c->d_code, c->d_offset,
+                       c->d_cursor, cap, c->d_x[a], c->d_v[a], vf32, c->d_spec[a], c->d_id[a], c->d_x[b], c->d_v[b],
+                       c->d_spec[b], c->d_key[b], c->d_id[b]);
+  }
+  // the new layout, for the next compaction
+  hipLaunchKernelGGL(k_copy_ints, dim3((unsigned)((nkeys + 1 + 255) / 256)), dim3(256), 0, c->stream, nkeys + 1,
+                     c->d_offset, c->d_off_layout);
+  c->layout_valid = true;
   hipLaunchKernelGGL(k_copy_total, dim3(1), dim3(1), 0, c->stream, c->d_offset + nkeys, c->d_n);
   c->cur = b;
   c->offsets_valid = true;
@@ -998,6 +1061,7 @@ extern "C" int amps_gpu_generate_particles(amps_ctx* c, int64_t n_per_species, i
   ga.seed = seed;
   ga.vf32 = precision == AMPS_FP32 ? 1 : 0;
   c->cur = 0;
+  c->layout_valid = false;
   hipLaunchKernelGGL(k_generate, dim3((unsigned)((ntot + 255) / 256)), dim3(256), 0, c->stream, c->mv, ga,
                      c->cap, c->d_x[0], c->d_v[0], c->d_spec[0], c->d_key[0], c->d_id[0], c->d_code);
   HIP_TRY(hipGetLastError());

@@ -251,6 +251,168 @@ __global__ __launch_bounds__(256) void k_scatter(const int* __restrict__ n_dev,
   ib[d] = ia[p];
 }
 
+// ----------------------------------------------------------------------
+// Deterministic migration.  The particles enter a compaction in the layout of
+// the previous one (sorted by key; lay[] its offsets), so the particles that
+// keep their key (stayers) are exactly those that still sit inside their
+// key's old run.  Each key's new run takes its stayers first, in array order,
+// then the particles that moved in (movers), in array order, whatever order
+// the atomics land in:
+//   classify: one stayer mask and count per wavefront; each mover lists its
+//             position in its new key's region of tmpsrc (atomic order);
+//   order:    per key, the movers' positions sorted, their destinations
+//             written to dest[] (one lane per key, a workgroup merge sort for
+//             keys with many movers);
+//   scatter:  a stayer's destination is its key's offset plus the stayers
+//             between its key's old run start and itself, from the scanned
+//             wavefront counts and masks; a mover's is dest[p].
+// ----------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_mig_classify(const int* __restrict__ n_dev, const int* __restrict__ key,
+                                                       const int8_t* __restrict__ code, const int* __restrict__ lay,
+                                                       int has_layout, const int* __restrict__ offset, int* cursor,
+                                                       int* __restrict__ tmpsrc, int* __restrict__ wcnt,
+                                                       uint64_t* __restrict__ wmask) {
+  const int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int n = *n_dev;
+  const bool valid = (p < n) && alive_code(code[p]);
+  const int k = valid ? key[p] : -1;
+  const bool st = valid && has_layout && p >= lay[k] && p < lay[k + 1];
+  const uint64_t m = __ballot(st);
+  const bool mover = valid && !st;
+  const RunInfo ru = wave_runs(mover, mover ? k : -1);
+  int base = 0;
+  if (ru.head) base = atomicAdd(&cursor[k], ru.len);
+  base = __shfl(base, ru.head_lane);
+  if (__lane_id() == 0) {
+    wcnt[p >> 6] = __popcll(m);
+    wmask[p >> 6] = m;
+  }
+  if (mover) tmpsrc[(int64_t)offset[k] + base + (__lane_id() - ru.head_lane)] = (int)p;
+}
+
+// per key: the movers' source positions (cursor[k] of them from offset[k] in
+// tmpsrc) in ascending order, destinations after the key's stayers.  Up to
+// `small` (at most 64) are sorted by the key's own lane; keys with more are
+// listed in big[] (count in big[nkeys]) for k_mig_order_big.
+__global__ __launch_bounds__(256) void k_mig_order(int64_t nkeys, int small, const int* __restrict__ offset,
+                                                    const int* __restrict__ cursor, int* __restrict__ tmpsrc,
+                                                    int* __restrict__ dest, int* __restrict__ big) {
+  const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (k >= nkeys) return;
+  const int m = cursor[k];
+  if (m == 0) return;
+  if (m > small) {
+    big[atomicAdd(&big[nkeys], 1)] = (int)k;
+    return;
+  }
+  int* q = tmpsrc + offset[k];
+  const int d0 = offset[k + 1] - m;
+  for (int i = 1; i < m; i++) {
+    const int t = q[i];
+    int j = i - 1;
+    while (j >= 0 && q[j] > t) {
+      q[j + 1] = q[j];
+      j--;
+    }
+    q[j + 1] = t;
+  }
+  for (int i = 0; i < m; i++) dest[q[i]] = d0 + i;
+}
+
+// number of entries of the ascending run r[0, n) below t
+__device__ __forceinline__ int count_below(const int* r, int n, int t) {
+  int lo = 0, hi = n;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (r[mid] < t)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
+// the keys with many movers, one workgroup each (a grid-stride loop over the
+// list): a bottom-up merge sort of the distinct source positions, every
+// element placed by a binary search in the partner run, ping-ponging between
+// tmpsrc and a spare array of the same size
+__global__ __launch_bounds__(256) void k_mig_order_big(int64_t nkeys, const int* __restrict__ offset,
+                                                        const int* __restrict__ cursor, int* tmpsrc, int* spare,
+                                                        int* __restrict__ dest, const int* __restrict__ big) {
+  const int nbig = big[nkeys];
+  for (int e = blockIdx.x; e < nbig; e += gridDim.x) {
+    const int k = big[e];
+    const int m = cursor[k];
+    int* src = tmpsrc + offset[k];
+    int* dst = spare + offset[k];
+    for (int w = 1; w < m; w *= 2) {
+      for (int i = threadIdx.x; i < m; i += blockDim.x) {
+        const int r = i / w, s = (r >> 1) * 2 * w;
+        const int t = src[i];
+        int pos;
+        if ((r & 1) == 0) {
+          const int b0 = min(s + w, m), bn = min(s + 2 * w, m) - b0;
+          pos = (i - s) + count_below(src + b0, bn, t);
+        } else {
+          pos = (i - s - w) + count_below(src + s, w, t);
+        }
+        dst[s + pos] = t;
+      }
+      __syncthreads();
+      int* tmp = src;
+      src = dst;
+      dst = tmp;
+    }
+    const int d0 = offset[k + 1] - m;
+    for (int i = threadIdx.x; i < m; i += blockDim.x) dest[src[i]] = d0 + i;
+    __syncthreads();
+  }
+}
+
+__global__ __launch_bounds__(256) void k_mig_scatter(const int* __restrict__ n_dev, const int* __restrict__ key,
+                                                      const int8_t* __restrict__ code, const int* __restrict__ offset,
+                                                      const int* __restrict__ lay, const int* __restrict__ wscan,
+                                                      const uint64_t* __restrict__ wmask,
+                                                      const int* __restrict__ dest, int64_t cap,
+                                                      const double* __restrict__ xa, const double* __restrict__ va,
+                                                      int vf32, const uint8_t* __restrict__ sa,
+                                                      const int64_t* __restrict__ ia, double* __restrict__ xb,
+                                                      double* __restrict__ vb, uint8_t* __restrict__ sb,
+                                                      int* __restrict__ kb, int64_t* __restrict__ ib) {
+  const int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int n = *n_dev;
+  const bool valid = (p < n) && alive_code(code[p]);
+  if (!valid) return;
+  const int k = key[p];
+  const int lane = __lane_id();
+  const uint64_t mw = wmask[p >> 6];
+  int64_t d;
+  if ((mw >> lane) & 1ull) {
+    const int q = lay[k];
+    const int sp = wscan[p >> 6] + __popcll(mw & ((1ull << lane) - 1ull));
+    const int sq = wscan[q >> 6] + __popcll(wmask[q >> 6] & ((1ull << (q & 63)) - 1ull));
+    d = (int64_t)offset[k] + (sp - sq);
+  } else {
+    d = dest[p];
+  }
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    xb[c * cap + d] = xa[c * cap + p];
+    if (vf32)
+      ((float*)vb)[c * cap + d] = ((const float*)va)[c * cap + p];
+    else
+      vb[c * cap + d] = va[c * cap + p];
+  }
+  sb[d] = sa[p];
+  kb[d] = k;
+  ib[d] = ia[p];
+}
+
+__global__ __launch_bounds__(256) void k_copy_ints(int64_t n, const int* __restrict__ src, int* __restrict__ dst) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = src[i];
+}
+
 __global__ void k_copy_total(const int* __restrict__ src, int* dst) { *dst = *src; }
 
 // tiles of at most T particles, never straddling a block

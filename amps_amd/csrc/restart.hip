@@ -243,6 +243,8 @@ extern "C" int amps_gpu_set_particles_restart(amps_ctx* c, const amps_aos_layout
   c->cur = 0;
   HIP_TRY(hipMemset(c->d_code, AMPS_CODE_FINISHED, cap));
   HIP_TRY(hipMemcpy(c->d_offset, off.data(), sizeof(int) * (nkeys + 1), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(c->d_off_layout, off.data(), sizeof(int) * (nkeys + 1), hipMemcpyHostToDevice));
+  c->layout_valid = true;
   if (n_records > 0) {
     int64_t* d_rb = nullptr;
     unsigned char* d_in = nullptr;

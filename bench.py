@@ -49,7 +49,8 @@ def _pmc_pass(args, tune, counter, outdir):
            "--workload", "pic", "--particles-log2", str(args.particles_log2), "--tile", str(tune["tile"]),
            "--deposit-kernel", str(tune["deposit_kernel"]), "--deposit-chunks", str(tune["deposit_chunks"]),
            "--node-batch", str(args.node_batch), "--push-waves", str(args.push_waves),
-           "--node-staged", str(args.node_staged), "--deposit-paired", str(args.deposit_paired)]
+           "--node-staged", str(args.node_staged), "--deposit-paired", str(args.deposit_paired),
+           "--deterministic", str(args.deterministic)]
     env = dict(os.environ, TMPDIR="/tmp")
     r = subprocess.run(cmd, cwd="/tmp", env=env, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)
     files = glob.glob(os.path.join(outdir, "**", "*counter_collection.csv"), recursive=True)
@@ -390,6 +391,8 @@ def main():
     ap.add_argument("--push-waves", type=int, default=0, help="push register target: 1 or 4 waves per SIMD")
     ap.add_argument("--node-staged", type=int, default=-1, help="1: staged node gather, 0: per-value gather")
     ap.add_argument("--deposit-paired", type=int, default=-1, help="matrix-core cell pass: 1 paired phases, 0 not")
+    ap.add_argument("--deterministic", type=int, default=1,
+                    help="1: migration layout independent of atomic order (default), 0: atomic order")
     ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 FETCH_SIZE/WRITE_SIZE passes")
     ap.add_argument("--pmc-child", action="store_true", help=argparse.SUPPRESS)
     args = ap.parse_args()
@@ -443,6 +446,7 @@ def main():
     fixed = args.tile > 0 and args.deposit_kernel >= 0 and args.deposit_chunks > 0
     if args.node_batch > 0:
         ctx.set_param("node_batch", args.node_batch)
+    ctx.set_param("deterministic", args.deterministic)
     if args.push_waves > 0:
         ctx.set_param("push_waves", args.push_waves)
     if args.node_staged >= 0:

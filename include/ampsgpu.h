@@ -64,11 +64,20 @@ int amps_gpu_device_info(amps_ctx* ctx, char* name, int* n_cu, int64_t* hbm_byte
 /* Tuning parameters (no reference counterpart; set by the autotuner):
  * "tile" (particles per push workgroup, 64..2^20), "deposit_chunks"
  * (waves per deposit workgroup: 2,3,4,6 for kernel 0, 4,6,12 for kernels 1
- * and 2), "deposit_exact" (0/1: unfused mul+add in the deposit accumulation
- * for a bit-exact summation; kernel 2 then runs kernel 1's exact form),
+ * and 2; particles per cell per record batch, 4 or 8, for kernel 3),
+ * "deposit_exact" (0/1: unfused mul+add in the deposit accumulation
+ * for a bit-exact summation; kernels 2 and 3 then run kernel 1's exact form),
  * "deposit_kernel" (0: each lane evaluates its cell's particles itself,
  * 1: particles evaluated once into shared-memory record batches,
- * 2 (default): the per-cell moment sums as FP64 matrix-core products), "deposit_paired"
+ * 2 (default): the per-cell moment sums as FP64 matrix-core products,
+ * 3: eight lanes per cell on the vector ALUs), "deposit_waves" (kernel 3:
+ * 2 (default) or 3 wavefronts per SIMD), "deterministic" (1 (default): the
+ * migration keeps, in every cell, the particles that stayed in their
+ * previous order followed by those that moved in, in array order, so that
+ * repeated runs give the same layout and bit-identical deposits; 0: the
+ * order in which the atomics of the scatter land), "migration_sort_small"
+ * (1..64, default 64: cells receiving more movers than this are ordered by a
+ * workgroup merge sort instead of one lane), "deposit_paired"
  * (kernel 2: 1 two wavefronts per SIMD in opposite VALU/MFMA phases,
  * 0 (default): one phase sequence per wavefront), "node_batch"
  * (1..64 consecutive nodes per wavefront in the deposit node gather,
