@@ -241,14 +241,16 @@ def uniform_particles(mesh, n, vth, seed_x=1, seed_v=2, r_excl=0.2, mass_spec=No
     return Particles(x, v, spec, blk, cell)
 
 
-def per_cell_particles(mesh, ppc, vth, drift=(0.0, 0.0, 0.0), seeds=(3, 4)):
-    """ppc[s] particles of species s uniformly inside every cell, Maxwellian
-    velocities with thermal speed vth[s] and a common drift."""
+def per_cell_particles(mesh, ppc, vth, drift=(0.0, 0.0, 0.0), seeds=(3, 4), blocks=None):
+    """ppc[s] particles of species s uniformly inside every cell (of the
+    given blocks, default all), Maxwellian velocities with thermal speed
+    vth[s] and a common drift."""
     nsp = len(ppc)
-    ncell = mesh.ncells
+    bl = np.arange(mesh.nblocks, dtype=np.int32) if blocks is None else np.asarray(blocks, np.int32)
+    ncell = len(bl) * NC ** 3
     dx = mesh.cell_size()
-    blk_all = np.repeat(np.arange(mesh.nblocks, dtype=np.int32), NC ** 3)
-    lc = np.tile(np.arange(NC ** 3, dtype=np.int32), mesh.nblocks)
+    blk_all = np.repeat(bl, NC ** 3)
+    lc = np.tile(np.arange(NC ** 3, dtype=np.int32), len(bl))
     ci = np.stack([lc % NC, (lc // NC) % NC, lc // (NC * NC)], axis=1)
     lo_all = mesh.blk_xmin[blk_all] + ci * dx[blk_all]
     xs, vs, sps, bs, cs = [], [], [], [], []

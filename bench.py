@@ -161,41 +161,59 @@ def pmc_traffic(args, tune):
     return out
 
 
-def cpu_baseline(target_s=12.0):
-    """The CPU oracle (FP64 C restatement, OpenMP) timed on this host's cores
-    on a bounded sample of the same per-cell workload: the 2-level AMR mesh
-    (120 blocks) with 16+16 particles per cell, push + deposit per step."""
+def cpu_baseline(target_s=10.0, nblk=256):
+    """The CPU oracle (FP64 C restatement built -O3 with AVX2, OpenMP) timed on
+    this host on a bounded sample of the headline step: the same 3-level AMR
+    mesh (7456 blocks), fields, species, time step and particle density (18 +
+    17 per cell, as 2^27 particles give) in nblk blocks spread over the mesh.
+    A step is the Lapenta push with reflection (particles in parallel) and the
+    deposit in the reference's parity-set arrangement (the cells of a set in
+    parallel, each added straight into its corners).  Timed on all host cores
+    and on one core, about target_s each; the migration is the deposit's
+    counting sort of the particles by cell, inside the deposit time."""
     import numpy as np
 
     from amps_amd import mesh as M
     from amps_amd.pic import Species
     from oracle import oracle as O
 
-    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    m = M.central_octant_mesh()
+    m = M.three_level_mesh()
     E, B = M.corner_fields(m)
-    parts = M.per_cell_particles(m, [16, 16], [0.05, 0.005], drift=(0.01, 0.0, 0.0))
     bmax = float(np.max(np.linalg.norm(B, axis=-1)))
     dt = 0.1 / (bmax / 0.01)
     sp = Species([-1.0, 1.0], [0.01, 1.0], [dt, dt])
     om = O.Mesh(m)
     noc, keys = om.build_nodes()
-    steps, t_push, t_dep = 0, 0.0, 0.0
-    p = parts
-    while t_push + t_dep < target_s or steps < 2:
-        t0 = time.perf_counter()
-        p, code = O.push(om, E, B, sp, p, mover=0, bc_mode=1, r_sphere=0.0, nthreads=threads)
-        t1 = time.perf_counter()
-        O.deposit(om, B, sp, dt, p, noc, len(keys), nthreads=threads)
-        t2 = time.perf_counter()
-        t_push += t1 - t0
-        t_dep += t2 - t1
-        steps += 1
-    n = parts.n
-    return {"value": n * steps / (t_push + t_dep), "unit": "particle-pushes/s", "cores": threads,
-            "kind": "port",
-            "sample": f"{n} particles (2-level AMR, 120 blocks, 16+16 per cell), {steps} push+deposit "
-                      f"steps, {t_push + t_dep:.1f} s (push {t_push:.1f} s, deposit {t_dep:.1f} s)"}
+    blocks = np.unique(np.linspace(0, m.nblocks - 1, nblk).astype(np.int32))
+    parts0 = M.per_cell_particles(m, [18, 17], [0.05, 0.005], drift=(0.01, 0.0, 0.0), blocks=blocks)
+    out = (np.empty((len(keys), 3)), np.empty((len(keys), 243)))
+    code = np.zeros(parts0.n, np.int32)
+    host_cores = os.cpu_count() or 1
+    all_threads = int(os.environ.get("OMP_NUM_THREADS", host_cores))
+
+    def run(threads, target):
+        p = parts0.copy()
+        steps, t_push, t_dep = 0, 0.0, 0.0
+        while t_push + t_dep < target or steps < 2:
+            t0 = time.perf_counter()
+            O.push(om, E, B, sp, p, mover=0, bc_mode=1, r_sphere=0.0, nthreads=threads, copy=False, code=code)
+            t1 = time.perf_counter()
+            O.deposit_colored(om, B, sp, dt, p, noc, len(keys), nthreads=threads, out=out)
+            t2 = time.perf_counter()
+            t_push += t1 - t0
+            t_dep += t2 - t1
+            steps += 1
+        return p.n * steps / (t_push + t_dep), steps, t_push, t_dep
+
+    v_all, s_all, tp_all, td_all = run(all_threads, target_s)
+    v_one, s_one, tp_one, td_one = run(1, target_s)
+    n = parts0.n
+    return {"value": v_all, "unit": "particle-pushes/s", "cores": all_threads, "kind": "port",
+            "value_1core": v_one, "host_cores": host_cores,
+            "sample": f"{n} particles (35 per cell in {len(blocks)} of the 7456 blocks of the 3-level mesh), "
+                      f"push + parity-set deposit: {all_threads} threads {s_all} steps in "
+                      f"{tp_all + td_all:.1f} s (push {tp_all:.1f} s, deposit {td_all:.1f} s); 1 thread "
+                      f"{s_one} steps in {tp_one + td_one:.1f} s (push {tp_one:.1f} s, deposit {td_one:.1f} s)"}
 
 
 MLP_FWD_FLOP = 2 * (6 * 128 + 128 * 128 + 128)  # per sample, 6 -> 128 -> 128 -> 1
@@ -310,11 +328,20 @@ def cutoff_map_bench(ctx, n_cu, n_train=1 << 20, epochs=20, cpu_check_points=16,
             else:
                 setattr(Q, name, v)
         idx = np.linspace(0, nloc - 1, cpu_check_points).astype(int)
+        cpu_threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        t0 = time.perf_counter()
         r = O.trace_batch(Q, np.repeat(x0[idx], 48, axis=0), np.repeat(dirs[idx], 48, axis=0),
-                          np.tile(grid, len(idx)),
-                          nthreads=int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)))
+                          np.tile(grid, len(idx)), nthreads=cpu_threads)
+        t_cpu = time.perf_counter() - t0
         ost = (r["termination"] == 0).astype(np.int8).reshape(-1, 48)
-        out["cpu_oracle_check"] = {"points": int(len(idx)), "states_bitexact": bool(np.array_equal(ost, s64[idx]))}
+        cpu_steps = int(r["steps"].sum())
+        # the same FP64 scan on the device runs every bin of every point: its
+        # per-step rate against the CPU oracle's on this subsample
+        out["cpu_oracle_check"] = {"points": int(len(idx)), "states_bitexact": bool(np.array_equal(ost, s64[idx])),
+                                   "threads": cpu_threads, "s": t_cpu, "trajectories": int(len(r)),
+                                   "steps": cpu_steps, "steps_per_s": cpu_steps / t_cpu,
+                                   "gpu_fp64_steps_per_s_ratio": out["plain_full_fp64_steps_per_s"]
+                                   / (cpu_steps / t_cpu)}
     # surrogate: training set from the tracer itself (UPPER_SCAN, FP32)
     t0 = time.perf_counter()
     Xtr, Ttr, tb, tsteps = SG.training_set(ctx, P, n_train, grid, seed=6, precision=L.FP32)

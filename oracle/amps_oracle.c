@@ -852,6 +852,124 @@ int orc_deposit(const orc_mesh* m, const double* B_corner, const double* charge,
   return 0;
 }
 
+/* The same deposit arranged as the reference runs it in parallel
+ * (UpdateJMassMatrix, pic_field_solver_ecsim.cpp:3585-3625 builds the sets,
+ * the cell loop runs them one after the other): the cells fall into eight
+ * sets by the parity of their global lattice index, and the cells of one set
+ * share no corner node, so each thread adds a finished cell straight into
+ * its corners.  The reference builds the sets on one resolution level; here
+ * the sets are formed per level as well, so cells of different sizes that
+ * share a node never run together.  Only cells holding particles are
+ * visited, and only the rows of their corner nodes are zeroed and written:
+ * J and M rows of other nodes are left as they are.  The per-node summation
+ * order is the set order, not the cell order of orc_deposit, so the results
+ * agree with it up to rounding.  This is the CPU baseline's deposit. */
+int orc_deposit_colored(const orc_mesh* m, const double* B_corner, const double* charge, const double* mass,
+                        const double* weight, double dt, int64_t n, const double* x, const double* v,
+                        const int* spec, const int* blk, const int* cell, const int* node_of_corner,
+                        int64_t nnodes, double* J, double* M, int nthreads) {
+  (void)nnodes;
+  const int64_t ncells = (int64_t)m->nblocks * ORC_NCELL;
+  int64_t* start = (int64_t*)calloc((size_t)ncells + 1, sizeof(int64_t));
+  int64_t* idx = (int64_t*)malloc(sizeof(int64_t) * (size_t)(n > 0 ? n : 1));
+  for (int64_t p = 0; p < n; p++) start[(int64_t)blk[p] * ORC_NCELL + cell[p] + 1]++;
+  for (int64_t c = 0; c < ncells; c++) start[c + 1] += start[c];
+  int64_t* fill = (int64_t*)malloc(sizeof(int64_t) * (size_t)ncells);
+  memcpy(fill, start, sizeof(int64_t) * (size_t)ncells);
+  for (int64_t p = 0; p < n; p++) idx[fill[(int64_t)blk[p] * ORC_NCELL + cell[p]]++] = p;
+  free(fill);
+  /* the occupied cells, grouped by (level, parity set) */
+  int lmin = 1 << 30, lmax = -1;
+  for (int b = 0; b < m->nblocks; b++) {
+    const int L = m->t[m->block_node[b]].level;
+    lmin = L < lmin ? L : lmin;
+    lmax = L > lmax ? L : lmax;
+  }
+  const int nlev = lmax >= lmin ? lmax - lmin + 1 : 0, nset = 8 * nlev;
+  int64_t* sstart = (int64_t*)calloc((size_t)nset + 1, sizeof(int64_t));
+  int64_t nocc = 0;
+  int* cset = (int*)malloc(sizeof(int) * (size_t)(ncells > 0 ? ncells : 1));
+  for (int64_t c = 0; c < ncells; c++) {
+    cset[c] = -1;
+    if (start[c + 1] == start[c]) continue;
+    const int b = (int)(c / ORC_NCELL), lc = (int)(c % ORC_NCELL);
+    const orc_tnode* tn = &m->t[m->block_node[b]];
+    const int64_t cs = tn->size_g / ORC_NC;
+    const int64_t gi = tn->xmin_g[0] / cs + lc % ORC_NC, gj = tn->xmin_g[1] / cs + (lc / ORC_NC) % ORC_NC,
+                  gk = tn->xmin_g[2] / cs + lc / (ORC_NC * ORC_NC);
+    cset[c] = 8 * (tn->level - lmin) + (int)((gi & 1) + 2 * (gj & 1) + 4 * (gk & 1));
+    sstart[cset[c] + 1]++;
+    nocc++;
+  }
+  for (int s2 = 0; s2 < nset; s2++) sstart[s2 + 1] += sstart[s2];
+  int64_t* sfill = (int64_t*)malloc(sizeof(int64_t) * (size_t)(nset > 0 ? nset : 1));
+  memcpy(sfill, sstart, sizeof(int64_t) * (size_t)nset);
+  int64_t* order = (int64_t*)malloc(sizeof(int64_t) * (size_t)(nocc > 0 ? nocc : 1));
+  for (int64_t c = 0; c < ncells; c++)
+    if (cset[c] >= 0) order[sfill[cset[c]]++] = c;
+  free(sfill);
+  free(cset);
+#ifdef _OPENMP
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#endif
+  for (int s2 = 0; s2 < nset; s2++) {
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic, 16)
+#endif
+    for (int64_t e = sstart[s2]; e < sstart[s2 + 1]; e++) {
+      const int64_t c = order[e];
+      const int b = (int)(c / ORC_NCELL), lc = (int)(c % ORC_NCELL);
+      const int ci = lc % ORC_NC, cj = (lc / ORC_NC) % ORC_NC, ck = lc / (ORC_NC * ORC_NC);
+      for (int q = 0; q < 8; q++) {
+        const int node = node_of_corner[(int64_t)b * ORC_NCORNER + (ci + CornerOff[q][0]) +
+                                        ORC_NN * ((cj + CornerOff[q][1]) + ORC_NN * (ck + CornerOff[q][2]))];
+        memset(J + (int64_t)node * 3, 0, sizeof(double) * 3);
+        memset(M + (int64_t)node * 243, 0, sizeof(double) * 243);
+      }
+    }
+  }
+  for (int s2 = 0; s2 < nset; s2++) {
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic, 16)
+#endif
+    for (int64_t e = sstart[s2]; e < sstart[s2 + 1]; e++) {
+      double o[CELL_VALS];
+      const int64_t c = order[e];
+      const int b = (int)(c / ORC_NCELL), lc = (int)(c % ORC_NCELL);
+      process_cell(m, m->block_node[b], B_corner + (size_t)b * ORC_NCORNER * 3, charge, mass, weight, dt,
+                   start[c + 1] - start[c], idx + start[c], n, x, v, spec, o);
+      const int ci = lc % ORC_NC, cj = (lc / ORC_NC) % ORC_NC, ck = lc / (ORC_NC * ORC_NC);
+      int nodeid[8];
+      for (int q = 0; q < 8; q++)
+        nodeid[q] = node_of_corner[(int64_t)b * ORC_NCORNER + (ci + CornerOff[q][0]) +
+                                   ORC_NN * ((cj + CornerOff[q][1]) + ORC_NN * (ck + CornerOff[q][2]))];
+      for (int q = 0; q < 8; q++)
+        for (int d = 0; d < 3; d++) J[(int64_t)nodeid[q] * 3 + d] += o[3 * q + d];
+      int pr = 0;
+      for (int i = 0; i < 8; i++)
+        for (int j = 0; j <= i; j++, pr++) {
+          const double* mm = o + 24 + 9 * pr;
+          if (i == j) {
+            double* cm = M + (int64_t)nodeid[i] * 243;
+            for (int k = 0; k < 9; k++) cm[k] += mm[k];
+          } else {
+            double* mi = M + (int64_t)nodeid[i] * 243 + 9 * IndexMatrix[i][j];
+            double* mj = M + (int64_t)nodeid[j] * 243 + 9 * IndexMatrix[j][i];
+            for (int k = 0; k < 9; k++) {
+              mi[k] += mm[k];
+              mj[k] += mm[k];
+            }
+          }
+        }
+    }
+  }
+  free(order);
+  free(sstart);
+  free(start);
+  free(idx);
+  return 0;
+}
+
 /* Particle energy and CFL numbers of UpdateJMassMatrix (pic_field_solver_
  * ecsim.cpp:2229-2238 per particle, :2355-2358 per cell, :3925-3929 and
  * :3972-3987 the reductions).  Per cell, in the order the particles are given

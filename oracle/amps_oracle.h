@@ -84,6 +84,15 @@ int orc_deposit(const orc_mesh* m, const double* B_corner, const double* charge,
                 const int* cell, const int* node_of_corner, int64_t nnodes, double* J,
                 double* M, int nthreads);
 
+/* The deposit in the reference's parallel arrangement (eight parity sets of
+ * cells per level, each cell added straight into its corners); equal to
+ * orc_deposit up to rounding; only the rows of nodes around occupied cells
+ * are written.  The CPU baseline's deposit. */
+int orc_deposit_colored(const orc_mesh* m, const double* B_corner, const double* charge, const double* mass,
+                        const double* weight, double dt, int64_t n, const double* x, const double* v,
+                        const int* spec, const int* blk, const int* cell, const int* node_of_corner,
+                        int64_t nnodes, double* J, double* M, int nthreads);
+
 /* ---------------- gridless cutoff tracer (SI units) ---------------- */
 typedef struct {
   double q_C;            /* signed backtrace charge [C] */

@@ -82,6 +82,7 @@ def lib():
         l.orc_deposit.argtypes = [vp, P(dbl), P(dbl), P(dbl), P(dbl), dbl, i64, P(dbl), P(dbl),
                                   P(C.c_int), P(C.c_int), P(C.c_int), P(C.c_int), i64, P(dbl),
                                   P(dbl), i32]
+        l.orc_deposit_colored.argtypes = l.orc_deposit.argtypes
         l.orc_energy_cfl.argtypes = [vp, P(dbl), P(dbl), i32, dbl, i64, P(dbl), P(C.c_int), P(C.c_int),
                                      P(C.c_int), P(dbl), P(dbl), P(dbl), P(dbl)]
         l.orc_dipole_B.argtypes = [P(TraceParams), P(dbl), P(dbl)]
@@ -312,10 +313,11 @@ class Mesh:
         return noc, keys[:nn]
 
 
-def push(omesh, E, B, species, parts, mover=0, bc_mode=1, r_sphere=0.0, nthreads=0):
-    """One oracle push of a Particles object; returns (new Particles, code)."""
-    p = parts.copy()
-    code = np.zeros(p.n, np.int32)
+def push(omesh, E, B, species, parts, mover=0, bc_mode=1, r_sphere=0.0, nthreads=0, copy=True, code=None):
+    """One oracle push of a Particles object; returns (new Particles, code).
+    copy=False pushes parts itself (and fills code if given)."""
+    p = parts.copy() if copy else parts
+    code = np.zeros(p.n, np.int32) if code is None else code
     E = np.ascontiguousarray(E, np.float64)
     B = np.ascontiguousarray(B, np.float64)
     lib().orc_push(omesh.h, _p(E, C.c_double), _p(B, C.c_double),
@@ -396,6 +398,24 @@ def deposit(omesh, B, species, dt, parts, node_of_corner, nnodes, nthreads=0):
                       _p(parts.x, C.c_double), _p(parts.v, C.c_double), _p(parts.spec, C.c_int),
                       _p(parts.blk, C.c_int), _p(parts.cell, C.c_int), _p(noc, C.c_int), nnodes,
                       _p(J, C.c_double), _p(M, C.c_double), nthreads)
+    return J, M
+
+
+def deposit_colored(omesh, B, species, dt, parts, node_of_corner, nnodes, nthreads=0, out=None):
+    """The deposit in the reference's parallel arrangement (parity sets of
+    cells, UpdateJMassMatrix): equal to deposit() up to rounding.  Rows of
+    nodes with no occupied incident cell are left unset (np.empty, or the
+    given out=(J, M) arrays), so a sample population on a large mesh touches
+    only its own nodes."""
+    J, M = (np.empty((nnodes, 3)), np.empty((nnodes, 243))) if out is None else out
+    assert J.shape == (nnodes, 3) and M.shape == (nnodes, 243)
+    B = np.ascontiguousarray(B, np.float64)
+    noc = np.ascontiguousarray(node_of_corner, np.int32)
+    lib().orc_deposit_colored(omesh.h, _p(B, C.c_double), _p(species.charge, C.c_double),
+                              _p(species.mass, C.c_double), _p(species.weight, C.c_double), dt, parts.n,
+                              _p(parts.x, C.c_double), _p(parts.v, C.c_double), _p(parts.spec, C.c_int),
+                              _p(parts.blk, C.c_int), _p(parts.cell, C.c_int), _p(noc, C.c_int), nnodes,
+                              _p(J, C.c_double), _p(M, C.c_double), nthreads)
     return J, M
 
 

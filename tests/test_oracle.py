@@ -841,6 +841,35 @@ def test_deposit_symmetry_and_linearity(amr):
     assert np.allclose(M1 + M2, Mm, rtol=1e-12, atol=1e-12 * np.abs(Mm).max())
 
 
+def test_deposit_colored_matches_deposit(amr):
+    """The CPU baseline's deposit (parity sets of cells per level, threads
+    adding cells straight into their corners) gives the oracle's J and M up
+    to summation order, on the two-level mesh with a partial population
+    (only the rows around occupied cells are written)."""
+    m, om = amr
+    E, B = M.corner_fields(m)
+    noc, keys = om.build_nodes()
+    pa = M.per_cell_particles(m, [3, 2], [0.05, 0.005], drift=(0.01, 0, 0))
+    keep = np.flatnonzero(pa.blk % 3 != 1)  # a population on two thirds of the blocks
+    pa = pa.take(keep)
+    sp = Species([-1.0, 1.0], [0.01, 1.0], [1e-3, 1e-3])
+    J, Mm = O.deposit(om, B, sp, 1e-3, pa, noc, len(keys), nthreads=4)
+    Jc, Mc = O.deposit_colored(om, B, sp, 1e-3, pa, noc, len(keys), nthreads=4)
+    occ = np.unique(pa.blk.astype(np.int64) * 512 + pa.cell)
+    touched = np.zeros(len(keys), bool)
+    b, lc = occ // 512, occ % 512
+    ci, cj, ck = lc % 8, (lc // 8) % 8, lc // 64
+    for q in range(8):
+        ox, oy, oz = q & 1, (q >> 1) & 1, q >> 2
+        touched[noc[b * 729 + (ci + ox) + 9 * ((cj + oy) + 9 * (ck + oz))]] = True
+    assert touched.sum() < len(keys)
+    assert np.all(J[~touched] == 0.0) and np.all(Mm[~touched] == 0.0)
+    assert np.allclose(Jc[touched], J[touched], rtol=1e-12, atol=1e-13 * np.abs(J).max())
+    assert np.allclose(Mc[touched], Mm[touched], rtol=1e-12, atol=1e-13 * np.abs(Mm).max())
+    # a different summation order, not a copy of the serial flush
+    assert not np.array_equal(Mc[touched], Mm[touched])
+
+
 def test_ecsim_mass_apply_single_particle_closed_form():
     """One particle in a dipole B (alpha not symmetric): the mass matrix is
     M_ij = (q h / V) W_i W_j alpha, so the mass term at node i is
