@@ -14,6 +14,11 @@ import numpy as np
 from . import _lib as L
 
 PUSH_TILES = (1024, 2048, 4096, 8192)
+# register budgets of the push: 1 = the compiler's allocation, 4 = held to
+# 128 VGPRs for four wavefronts per SIMD
+PUSH_WAVES = (1, 4)
+# nodes per wavefront of the deposit node gather
+NODE_BATCHES = (4, 8, 16)
 # (cell-pass kernel, chunks): 0 = per-lane particle loop, 1 = shared-memory
 # record batches, 2 = FP64 matrix cores, 3 = eight lanes per cell (chunks =
 # particles per cell per record batch)
@@ -48,24 +53,38 @@ def _save(path, d):
 
 
 def tune_pic(ctx, state, opts_kw, deposit_dt, signature, cache=None, reps=2):
-    """Pick the push tile and the deposit cell-pass variant for a PicState."""
+    """Pick the push tile and register budget (occupancy), the deposit
+    cell-pass variant and the node gather's batch for a PicState."""
     db = _load(cache)
-    if signature in db and "deposit_kernel" in db[signature]:
+    if signature in db and "node_batch" in db[signature]:
         best = db[signature]
         ctx.set_param("tile", best["tile"])
+        ctx.set_param("push_waves", best["push_waves"])
         set_deposit_variant(ctx, best["deposit_kernel"], best["deposit_chunks"])
+        ctx.set_param("node_batch", best["node_batch"])
         return best, True
     ctx.set_timing(True)
     res = {}
-    for tile in PUSH_TILES:
-        ctx.set_param("tile", tile)
+
+    def push_ms():
         ctx.reset_timing()
         for _ in range(reps):
             state.MoveParticles(compact=True, **opts_kw)
         ms, n = ctx.timing()
-        res[("tile", tile)] = ms[L.K_PUSH] / max(n[L.K_PUSH], 1)
+        return ms[L.K_PUSH] / max(n[L.K_PUSH], 1)
+
+    ctx.set_param("push_waves", PUSH_WAVES[0])
+    for tile in PUSH_TILES:
+        ctx.set_param("tile", tile)
+        res[("tile", tile)] = push_ms()
     best_tile = min(PUSH_TILES, key=lambda t: res[("tile", t)])
     ctx.set_param("tile", best_tile)
+    res[("waves", PUSH_WAVES[0])] = res[("tile", best_tile)]
+    for w in PUSH_WAVES[1:]:
+        ctx.set_param("push_waves", w)
+        res[("waves", w)] = push_ms()
+    best_waves = min(PUSH_WAVES, key=lambda w: res[("waves", w)])
+    ctx.set_param("push_waves", best_waves)
     for var in DEPOSIT_VARIANTS:
         set_deposit_variant(ctx, *var)
         ctx.reset_timing()
@@ -75,10 +94,22 @@ def tune_pic(ctx, state, opts_kw, deposit_dt, signature, cache=None, reps=2):
         res[("dep", var)] = (ms[L.K_DEPOSIT_CELL] + ms[L.K_DEPOSIT_NODE]) / max(n[L.K_DEPOSIT_CELL], 1)
     best_var = min(DEPOSIT_VARIANTS, key=lambda v: res[("dep", v)])
     set_deposit_variant(ctx, *best_var)
+    for nb in NODE_BATCHES:
+        ctx.set_param("node_batch", nb)
+        ctx.reset_timing()
+        for _ in range(reps):
+            state.UpdateJMassMatrix(deposit_dt, copy_out=False)
+        ms, n = ctx.timing()
+        res[("node", nb)] = ms[L.K_DEPOSIT_NODE] / max(n[L.K_DEPOSIT_NODE], 1)
+    best_nb = min(NODE_BATCHES, key=lambda b: res[("node", b)])
+    ctx.set_param("node_batch", best_nb)
     ctx.set_timing(False)
-    best = {"tile": int(best_tile), "deposit_kernel": int(best_var[0]), "deposit_chunks": int(best_var[1]),
+    best = {"tile": int(best_tile), "push_waves": int(best_waves), "deposit_kernel": int(best_var[0]),
+            "deposit_chunks": int(best_var[1]), "node_batch": int(best_nb),
             "push_ms": {str(t): round(res[("tile", t)], 4) for t in PUSH_TILES},
-            "deposit_ms": {f"k{k}c{c}": round(res[("dep", (k, c))], 4) for k, c in DEPOSIT_VARIANTS}}
+            "push_waves_ms": {str(w): round(res[("waves", w)], 4) for w in PUSH_WAVES},
+            "deposit_ms": {f"k{k}c{c}": round(res[("dep", (k, c))], 4) for k, c in DEPOSIT_VARIANTS},
+            "node_batch_ms": {str(b): round(res[("node", b)], 4) for b in NODE_BATCHES}}
     db[signature] = best
     _save(cache, db)
     return best, False

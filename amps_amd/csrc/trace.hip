@@ -964,26 +964,83 @@ __device__ __forceinline__ int8_t state_of(int term) {
   return term == 0 ? 1 : (term <= 5 ? 0 : 2);
 }
 
+// BRACKET mode phase of a location, read and published across workgroups.
+// A lane that reads it only decides whether to trace read-only inputs, so the
+// read needs no ordering (an acquire would invalidate the caches at every poll)
+__device__ __forceinline__ int load_status(const int* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void store_status(int* p, int v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// The cutoff bin of one location from its samples s[lo..hi]
+// (CutoffForDirectionUpperScan_GV read off a sample vector): the allowed side
+// of the highest forbidden -> allowed transition; -1 when the grid's top is
+// forbidden.  Returns false when [lo, hi] does not hold the transition (top
+// forbidden below the grid's top, or allowed down to a bottom above Rmin).
+__device__ __forceinline__ bool bin_from_samples(const int8_t* s, int lo, int hi, int nR, int& b) {
+  if (s[hi] != 1) {
+    b = -1;
+    return hi == nR - 1;
+  }
+  b = lo;
+  for (int k = hi - 1; k >= lo; --k)
+    if (s[k] != 1) {
+      b = k + 1;
+      break;
+    }
+  return !(b == lo && lo > 0);
+}
+
 // ---------------------------------------------------------------------
 // Persistent kernel.  Work item semantics by mode:
 //   TRACE: item i = trajectory i of an explicit (x0, dir, R) list
 //   FULL : item i = (loc i / nbin, bin lo(loc) + i % nbin)   [states only]
 //   UPPER: item i = loc i, bins walked downward from hi(loc) until the first
 //          forbidden sample (CutoffForDirectionUpperScan_GV semantics)
+//   BRACKET: items [0, nloc W) are FULL over each location's bracket of W
+//          bins (the bins below its centre first); items [nloc W, nloc nR)
+//          are the location's other bins, each traced only if the bracket
+//          misses the transition.  The whole-grid rescan (phase 2) opens as
+//          soon as the bracket's top bin comes out not allowed below the
+//          grid's top, or, when the last bracket bin is done, if every bin
+//          was allowed above Rmin; otherwise that last lane writes the
+//          cutoff bin (phase 1: the other bins are skipped).  pend counts a
+//          location's outstanding items (the rescan's are added before it
+//          opens); the lane that retires the last one reads the bins off.  A
+//          lane that draws an other-bin item of a location still in phase 0
+//          holds it until the phase is known: every bracket item was drawn
+//          before any other-bin item and is being traced, so every wait ends.
 // ---------------------------------------------------------------------
 template <class S>
 __global__ __launch_bounds__(256) void k_trace_persistent(TraceView a) {
   typename S::Lane L;
   int64_t item = -1;
   int loc = 0, bin = 0, attempt = 0;
-  bool has = false, done = false;
+  bool has = false, done = false, wait = false;
   int64_t my_steps = 0;
   const int lane = __lane_id();
-  const int64_t nloc_full = (a.mode == TRACE_MODE_FULL) ? a.nitems / a.nbin_item : 1;
+  const bool BRK = a.mode == TRACE_MODE_BRACKET;
+  const int64_t nloc_full = (a.mode == TRACE_MODE_FULL) ? a.nitems / a.nbin_item : (BRK ? a.nloc : 1);
   for (;;) {
-    const bool need = !has && !done;
-    const uint64_t mask = __ballot(need);
-    if (mask) {
+    if (BRK && wait) {
+      const int st = load_status(a.status + loc);
+      if (st == 2) {
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");  // after the rescan's count
+        wait = false;
+        S::init(a, L, a.x0 + 3 * (int64_t)loc, a.dir + 3 * (int64_t)loc, a.grid[bin], 0);
+      } else if (st == 1) {
+        wait = false;
+        has = false;
+      }
+    }
+    // draws; in BRACKET mode an item of a resolved location is skipped and
+    // its lane draws again (a few rounds per pass)
+    for (int round = 0; round < (BRK ? 4 : 1); round++) {
+      const bool need = !has && !done;
+      const uint64_t mask = __ballot(need);
+      if (!mask) break;
       const int leader = __ffsll((long long)mask) - 1;
       unsigned long long base = 0;
       if (lane == leader) base = atomicAdd(a.queue, (unsigned long long)__popcll(mask));
@@ -994,6 +1051,7 @@ __global__ __launch_bounds__(256) void k_trace_persistent(TraceView a) {
         if ((int64_t)my < a.nitems) {
           item = (int64_t)my;
           attempt = 0;
+          has = true;
           if (a.mode == TRACE_MODE_LIST) {
             loc = (int)item;
             S::init(a, L, a.x0 + 3 * item, a.dir + 3 * item, a.R[item], 0);
@@ -1004,19 +1062,42 @@ __global__ __launch_bounds__(256) void k_trace_persistent(TraceView a) {
             loc = (int)(item % nloc_full);
             bin = (a.lo ? a.lo[loc] : 0) + (int)(item / nloc_full);
             S::init(a, L, a.x0 + 3 * (int64_t)loc, a.dir + 3 * (int64_t)loc, a.grid[bin], 0);
+          } else if (BRK) {
+            loc = (int)(item % nloc_full);
+            const int kk = (int)(item / nloc_full), lo = a.lo[loc];
+            if (kk < a.nbin_item) {
+              // the bins just below the bracket's centre first: the predicted
+              // cutoff's forbidden side, where the quasi-trapped (longest)
+              // trajectories are, then the centre and the allowed side
+              const int half = a.nbin_item / 2;
+              bin = lo + (kk < half ? half - 1 - kk : kk);
+              S::init(a, L, a.x0 + 3 * (int64_t)loc, a.dir + 3 * (int64_t)loc, a.grid[bin], 0);
+            } else {
+              const int k2 = kk - a.nbin_item;
+              bin = k2 < lo ? k2 : k2 + a.nbin_item;
+              const int st = load_status(a.status + loc);
+              if (st == 2) {
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+                S::init(a, L, a.x0 + 3 * (int64_t)loc, a.dir + 3 * (int64_t)loc, a.grid[bin], 0);
+              } else if (st == 0)
+                wait = true;
+              else
+                has = false;
+            }
           } else {
             loc = (int)item;
             bin = a.hi ? a.hi[loc] : a.nR - 1;
             S::init(a, L, a.x0 + 3 * (int64_t)loc, a.dir + 3 * (int64_t)loc, a.grid[bin], 0);
           }
-          has = true;
         } else {
           done = true;
         }
       }
     }
     if (__ballot(has) == 0ull) break;
-    for (int k = 0; k < a.chunk && has; k++) {
+    // a wavefront whose lanes all wait for bracket results yields its issue slots
+    if (BRK && __ballot(has && !wait) == 0ull) __builtin_amdgcn_s_sleep(64);
+    for (int k = 0; k < a.chunk && has && !wait; k++) {
       const int term = S::step(a, L);
       if (term < 0) continue;
       if (a.P.retry && attempt == 0 && retryable(a.P.retry, term)) {
@@ -1044,6 +1125,37 @@ __global__ __launch_bounds__(256) void k_trace_persistent(TraceView a) {
       } else if (a.mode == TRACE_MODE_FULL) {
         a.states[(int64_t)loc * a.nR + bin] = state_of(term);
         has = false;
+      } else if (BRK) {
+        int8_t* s = a.states + (int64_t)loc * a.nR;
+        const int8_t sv = state_of(term);
+        s[bin] = sv;
+        has = false;
+        const int top = a.lo[loc] + a.nbin_item - 1;
+        if (bin == top && sv != 1 && top < a.nR - 1 && load_status(a.status + loc) == 0) {
+          // the bracket's top is not allowed below the grid's top: the
+          // transition lies above the bracket, whatever its other bins give,
+          // so the rescan starts now (counted before this bin is retired)
+          atomicAdd(a.pend + loc, a.nR - a.nbin_item);
+          store_status(a.status + loc, 2);
+        }
+        // the sample (and any rescan count and phase) before this item retires
+        __threadfence();
+        if (atomicSub(a.pend + loc, 1) == 1) {
+          // this location's phase is complete: every sample of it is visible
+          __threadfence();
+          const int phase = load_status(a.status + loc);
+          const int lo = phase == 0 ? a.lo[loc] : 0;
+          const int hi = phase == 0 ? lo + a.nbin_item - 1 : a.nR - 1;
+          int b;
+          if (bin_from_samples(s, lo, hi, a.nR, b)) {
+            a.cutoff[loc] = b;
+            if (phase == 0) store_status(a.status + loc, 1);
+          } else {
+            // allowed down to a bottom above Rmin: the other bins, then the whole grid
+            __hip_atomic_store(a.pend + loc, a.nR - a.nbin_item, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            store_status(a.status + loc, 2);
+          }
+        }
       } else {
         const int8_t st = state_of(term);
         a.states[(int64_t)loc * a.nR + bin] = st;
@@ -1081,10 +1193,13 @@ struct TraceState {
   int* d_lo = nullptr;
   int* d_hi = nullptr;
   int* d_cut = nullptr;
+  int* d_pend = nullptr;
+  int* d_status = nullptr;
   int8_t* d_states = nullptr;
   amps_trace_result* d_out = nullptr;
   amps_trace_exit* d_exits = nullptr;
   size_t b_x0 = 0, b_dir = 0, b_R = 0, b_lo = 0, b_hi = 0, b_cut = 0, b_states = 0, b_out = 0, b_exits = 0;
+  size_t b_pend = 0, b_status = 0;
   int grid_waves = 0;  // persistent grid size (workgroups); 0 = 8 per CU
   int chunk = 32;      // steps between queue refills
 };
@@ -1102,6 +1217,8 @@ void trace_free(amps_ctx* c) {
   (void)hipFree(t->d_lo);
   (void)hipFree(t->d_hi);
   (void)hipFree(t->d_cut);
+  (void)hipFree(t->d_pend);
+  (void)hipFree(t->d_status);
   (void)hipFree(t->d_states);
   (void)hipFree(t->d_out);
   (void)hipFree(t->d_exits);
@@ -1656,7 +1773,9 @@ extern "C" int amps_gpu_cutoff_scan(amps_ctx* c, const amps_trace_params* P, int
   if (!c) return fail(AMPS_E_ARG, "null context");
   if (int r = check_params(P)) return r;
   if (precision != AMPS_FP64 && precision != AMPS_FP32) return fail(AMPS_E_ARG, "cutoff_scan: bad precision");
-  if (mode != AMPS_SCAN_FULL && mode != AMPS_SCAN_UPPER) return fail(AMPS_E_ARG, "cutoff_scan: bad mode");
+  if (mode != AMPS_SCAN_FULL && mode != AMPS_SCAN_UPPER && mode != AMPS_SCAN_BRACKETED)
+    return fail(AMPS_E_ARG, "cutoff_scan: bad mode");
+  if (mode == AMPS_SCAN_BRACKETED && !bin_lo) return fail(AMPS_E_ARG, "cutoff_scan: BRACKETED mode needs the brackets");
   if (nloc < 0 || nloc > ((int64_t)1 << 28) || nR < 2 || nR > 4096)
     return fail(AMPS_E_RANGE, "cutoff_scan: nloc or nR out of range");
   if (nloc == 0) return AMPS_OK;
@@ -1667,10 +1786,11 @@ extern "C" int amps_gpu_cutoff_scan(amps_ctx* c, const amps_trace_params* P, int
     for (int64_t i = 0; i < nloc; i++)
       if (bin_lo[i] < 0 || bin_hi[i] >= nR || bin_lo[i] > bin_hi[i])
         return fail(AMPS_E_ARG, "cutoff_scan: bracket outside [0, nR)");
-    if (mode == AMPS_SCAN_FULL) {
+    if (mode != AMPS_SCAN_UPPER) {
       const int w = bin_hi[0] - bin_lo[0] + 1;
       for (int64_t i = 0; i < nloc; i++)
-        if (bin_hi[i] - bin_lo[i] + 1 != w) return fail(AMPS_E_ARG, "cutoff_scan: FULL mode needs equal bracket widths");
+        if (bin_hi[i] - bin_lo[i] + 1 != w)
+          return fail(AMPS_E_ARG, "cutoff_scan: FULL and BRACKETED modes need equal bracket widths");
       nbin_item = w;
     }
   }
@@ -1694,10 +1814,22 @@ extern "C" int amps_gpu_cutoff_scan(amps_ctx* c, const amps_trace_params* P, int
   HIP_TRY(hipMemsetAsync(t->d_states, -1, nloc * nR, c->stream));
   HIP_TRY(hipMemsetAsync(t->d_cut, 0, sizeof(int) * nloc, c->stream));
   HIP_TRY(hipMemsetAsync(t->d_steps, 0, 8, c->stream));
+  std::vector<int> pend;
+  if (mode == AMPS_SCAN_BRACKETED) {
+    // every location starts in its bracket phase with W outstanding bins
+    if (int r = ensure((void**)&t->d_pend, &t->b_pend, sizeof(int) * nloc)) return r;
+    if (int r = ensure((void**)&t->d_status, &t->b_status, sizeof(int) * nloc)) return r;
+    pend.assign((size_t)nloc, nbin_item);
+    HIP_TRY(hipMemcpyAsync(t->d_pend, pend.data(), sizeof(int) * nloc, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipMemsetAsync(t->d_status, 0, sizeof(int) * nloc, c->stream));
+  }
   TraceView a;
   fill_view(c, t, P, a);
-  a.mode = (mode == AMPS_SCAN_FULL) ? TRACE_MODE_FULL : TRACE_MODE_UPPER;
-  a.nitems = (mode == AMPS_SCAN_FULL) ? nloc * nbin_item : nloc;
+  a.mode = (mode == AMPS_SCAN_FULL) ? TRACE_MODE_FULL : (mode == AMPS_SCAN_UPPER ? TRACE_MODE_UPPER : TRACE_MODE_BRACKET);
+  a.nitems = (mode == AMPS_SCAN_FULL) ? nloc * nbin_item : (mode == AMPS_SCAN_UPPER ? nloc : nloc * nR);
+  a.nloc = nloc;
+  a.pend = t->d_pend;
+  a.status = t->d_status;
   a.nbin_item = nbin_item;
   a.nR = nR;
   a.x0 = t->d_x0;
@@ -1719,7 +1851,9 @@ extern "C" int amps_gpu_cutoff_scan(amps_ctx* c, const amps_trace_params* P, int
   for (int64_t i = 0; i < nloc; i++) {
     const int8_t* s = states + i * nR;
     const int lo = bin_lo ? bin_lo[i] : 0, hi = bin_hi ? bin_hi[i] : nR - 1;
-    if (mode == AMPS_SCAN_UPPER) {
+    if (mode == AMPS_SCAN_BRACKETED) {
+      cutoff_bin[i] = cut[(size_t)i];  // read off on the device, rescan included
+    } else if (mode == AMPS_SCAN_UPPER) {
       int b = cut[(size_t)i];
       // bracket diagnostics: top sample forbidden below Rmax, or allowed down to a bottom above Rmin
       if (s[hi] != 1) b = (hi == nR - 1) ? -1 : AMPS_BIN_ABOVE;

@@ -6,7 +6,7 @@
 
 namespace amps {
 
-enum { TRACE_MODE_LIST = 0, TRACE_MODE_FULL = 1, TRACE_MODE_UPPER = 2 };
+enum { TRACE_MODE_LIST = 0, TRACE_MODE_FULL = 1, TRACE_MODE_UPPER = 2, TRACE_MODE_BRACKET = 3 };
 
 struct TraceView {
   amps_trace_params P;
@@ -29,7 +29,12 @@ struct TraceView {
   amps_trace_result* out;
   amps_trace_exit* exits;  // LIST mode exit states (may be null)
   int8_t* states;     // [nloc][nR]: -1 untraced, 0 forbidden, 1 allowed, 2 failed
-  int* cutoff;        // UPPER mode raw result
+  int* cutoff;        // UPPER mode raw result; BRACKET mode cutoff bin
+  // BRACKET mode: locations, per-location outstanding items of the current
+  // phase and phase (0 bracket running, 1 resolved, 2 whole grid rescanned)
+  int64_t nloc;
+  int* pend;
+  int* status;
   // mesh field (field_model 1): the block mesh and the compact interior-cell B
   MeshView mesh;
   const double* Bm;

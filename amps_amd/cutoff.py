@@ -39,6 +39,7 @@ MOVERS = {"BORIS": 0, "HC4": 1, "RK2": 2, "RK4": 3, "RK6": 4}
 
 SCAN_FULL = 0
 SCAN_UPPER = 1
+SCAN_BRACKETED = 2  # the bracket's bins, the whole grid for pairs it misses, one launch
 BIN_ABOVE = -2
 BIN_BELOW = -3
 

@@ -120,18 +120,29 @@ def bracket_bins(pb, nR, half=3):
     return lo, (lo + w - 1).astype(np.int32)
 
 
-def bracketed_map(ctx, sur, params, x0, dirs, feats, grid, half=3, precision=L.FP32):
+def bracketed_map(ctx, sur, params, x0, dirs, feats, grid, half=3, precision=L.FP32, one_launch=True):
     """Cutoff bins with the surrogate bracket and the full-scan fallback.
 
-    Every bin of every bracket is traced as an independent trajectory (one
-    persistent launch over nloc x (2 half + 1) items); the bins come from the
-    bracket's samples as CutoffForDirectionUpperScan_GV would read them.
-    Points whose bracket shows no transition (the top sample forbidden below
-    Rmax, or every sample allowed above Rmin) are rescanned over the whole
-    grid.  Returns (bins, fallback mask, predicted bins, total steps)."""
+    Every bin of every bracket is traced as an independent trajectory; the
+    bins come from the bracket's samples as CutoffForDirectionUpperScan_GV
+    would read them.  Points whose bracket shows no transition (the top
+    sample forbidden below Rmax, or every sample allowed above Rmin) are
+    rescanned over the whole grid.  one_launch: the rescan runs inside the
+    same persistent launch, each pair's other bins starting as soon as its
+    bracket is done (AMPS_SCAN_BRACKETED); otherwise a second launch follows
+    the bracket scan.  Returns (bins, fallback mask, predicted bins, total
+    steps)."""
     y = sur.predict(feats)
     pb = predicted_bin(y, grid)
     lo, hi = bracket_bins(pb, len(grid), half)
+    if one_launch:
+        states, bins, steps = CU.cutoff_scan(ctx, params, x0, dirs, grid, mode=CU.SCAN_BRACKETED, lo=lo, hi=hi,
+                                             precision=precision)
+        outside = np.ones(states.shape, bool)
+        cols = np.arange(len(grid))[None, :]
+        outside &= (cols < lo[:, None]) | (cols > hi[:, None])
+        fb = ((states >= 0) & outside).any(axis=1)
+        return bins, fb, pb, steps
     _, bins, steps = CU.cutoff_scan(ctx, params, x0, dirs, grid, mode=CU.SCAN_FULL, lo=lo, hi=hi,
                                     precision=precision)
     fb = bins < -1

@@ -48,7 +48,8 @@ def _pmc_pass(args, tune, counter, outdir):
            "-d", outdir, "-o", "pmc", "--", sys.executable, os.path.abspath(__file__), "--pmc-child",
            "--workload", "pic", "--particles-log2", str(args.particles_log2), "--tile", str(tune["tile"]),
            "--deposit-kernel", str(tune["deposit_kernel"]), "--deposit-chunks", str(tune["deposit_chunks"]),
-           "--node-batch", str(args.node_batch), "--push-waves", str(args.push_waves),
+           "--node-batch", str(tune.get("node_batch", args.node_batch)),
+           "--push-waves", str(tune.get("push_waves", args.push_waves)),
            "--node-staged", str(args.node_staged), "--deposit-paired", str(args.deposit_paired),
            "--deterministic", str(args.deterministic)]
     env = dict(os.environ, TMPDIR="/tmp")
@@ -363,8 +364,15 @@ def cutoff_map_bench(ctx, n_cu, n_train=1 << 20, epochs=20, cpu_check_points=16,
         t0 = time.perf_counter()
         bs, fb, pb, sst = SG.bracketed_map(ctx, sur, P, x0, dirs, feats, grid, half=3, precision=prec)
         dt_s = time.perf_counter() - t0
+        # the same map with the fallback as a second launch after the bracket scan
+        t0 = time.perf_counter()
+        bs_2, _, _, sst_2 = SG.bracketed_map(ctx, sur, P, x0, dirs, feats, grid, half=3, precision=prec,
+                                             one_launch=False)
+        dt_2 = time.perf_counter() - t0
         out[f"surrogate_{tag}"] = {
             "map_s": dt_s, "steps": int(sst), "speedup_vs_plain_full": out[plain] / dt_s,
+            "two_launch_map_s": dt_2, "two_launch_steps": int(sst_2),
+            "two_launch_bins_equal": bool(np.array_equal(bs, bs_2)),
             "fallback_frac": float(np.mean(fb)),
             "bins_off_gt1_vs_fp64": float(np.mean(np.abs(bs.astype(int) - b64.astype(int)) > 1))}
     out["surrogate_pred_bin_mae"] = float(np.mean(np.abs(pb.astype(int) - np.where(b64 < 0, 47, b64))))
@@ -499,8 +507,9 @@ def main():
     if not args.no_tune and not fixed:
         sig = f"{arch}|pic|nb{mesh.nblocks}|n{n}"
         tune, cached = autotune.tune_pic(ctx, st, opts, dt, sig, cache=args.tune_cache)
-        tune = {"tile": tune["tile"], "deposit_kernel": tune["deposit_kernel"],
-                "deposit_chunks": tune["deposit_chunks"], "deposit_ms": tune["deposit_ms"], "cached": cached}
+        tune = {k: tune[k] for k in ("tile", "push_waves", "deposit_kernel", "deposit_chunks", "node_batch",
+                                     "push_ms", "push_waves_ms", "deposit_ms", "node_batch_ms")}
+        tune["cached"] = cached
     # warm-up, then the timed region (graph-captured steps)
     if args.warmup > 0:
         st.run_steps(args.warmup, dt, use_graph=True, **opts)

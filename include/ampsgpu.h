@@ -463,6 +463,11 @@ int amps_gpu_transmission(amps_ctx* ctx, const amps_trace_params* params, int32_
 
 #define AMPS_SCAN_FULL 0   /* trace every bin of the bracket */
 #define AMPS_SCAN_UPPER 1  /* downward from the bracket top to the first forbidden bin */
+/* every bin of the bracket; a (location, direction) pair whose bracket holds
+ * no transition has its other bins traced by the same launch, as soon as its
+ * bracket is done, and its cutoff read off the whole grid (no reference
+ * counterpart: the surrogate's bracket with the full-scan fallback) */
+#define AMPS_SCAN_BRACKETED 2
 #define AMPS_BIN_ABOVE (-2) /* bracket top forbidden below Rmax: cutoff above the bracket */
 #define AMPS_BIN_BELOW (-3) /* bracket allowed down to a bottom above Rmin */
 /* Cutoff map (CutoffForDirectionUpperScan_GV, CutoffRigidityGridless.cpp:
@@ -473,7 +478,8 @@ int amps_gpu_transmission(amps_ctx* ctx, const amps_trace_params* params, int32_
  * cutoff_bin[nloc]: index of the allowed side of the highest forbidden ->
  * allowed transition, 0 when all allowed down to Rmin, -1 when Rmax is
  * forbidden, or AMPS_BIN_ABOVE / AMPS_BIN_BELOW when the bracket does not
- * contain the transition. */
+ * contain the transition (never in AMPS_SCAN_BRACKETED mode, which rescans
+ * such pairs; their states outside the bracket are then traced, not -1). */
 int amps_gpu_cutoff_scan(amps_ctx* ctx, const amps_trace_params* params, int32_t precision,
                          int32_t mode, int64_t nloc, const double* x0, const double* dir,
                          int32_t nR, const double* grid, const int32_t* bin_lo,

@@ -149,6 +149,52 @@ def test_bracketed_full_scan_matches_full_scan(gpu_ctx):
     print(f"bracket results: {np.mean(bb >= 0):.3f} resolved, {np.mean(bb == bf):.3f} equal to the full scan")
 
 
+@pytest.mark.parametrize("precision", [L.FP64, L.FP32])
+def test_one_launch_bracketed_scan_matches_two_launches(gpu_ctx, precision):
+    """AMPS_SCAN_BRACKETED: the bracket's bins, and for every pair whose
+    bracket misses the transition the whole grid, traced by one persistent
+    launch.  Its bins equal the two-launch form (bracket scan, then a full
+    rescan of the missed pairs), which are the full scan's bins where the
+    bracket holds the transition; the missed pairs' samples equal the full
+    scan's on every bin, the others are traced only inside their bracket.
+    Brackets around perturbed guesses make about a third of the pairs miss."""
+    from amps_amd import surrogate as SG
+    P = CU.config2_params()
+    P.max_steps = 20000
+    pts, lat, lon = CU.config2_points(nlon=8, nlat=8)
+    zen, az = CU.config2_directions(4)
+    dirs = CU.arrival_dirs(pts, zen, az).reshape(-1, 3)
+    x0 = np.repeat(pts, len(zen), axis=0)
+    grid = CU.build_search_grid(0.1, 20.0, 48)
+    sf, bf, st_full = CU.cutoff_scan(gpu_ctx, P, x0, dirs, grid, mode=CU.SCAN_FULL, precision=precision)
+    rng = np.random.default_rng(13)
+    guess = np.clip(np.where(bf < 0, 47, bf) + rng.integers(-6, 7, len(bf)), 0, 47)
+    lo, hi = SG.bracket_bins(guess, 48)
+    print(f"one-launch bracketed scan, precision {precision}, {len(lo)} pairs", flush=True)
+    s1, b1, st1 = CU.cutoff_scan(gpu_ctx, P, x0, dirs, grid, mode=CU.SCAN_BRACKETED, lo=lo, hi=hi,
+                                 precision=precision)
+    # the two-launch form
+    _, b2, st2 = CU.cutoff_scan(gpu_ctx, P, x0, dirs, grid, mode=CU.SCAN_FULL, lo=lo, hi=hi, precision=precision)
+    miss = b2 < -1
+    b2 = b2.copy()
+    if miss.any():
+        _, b2m, st2m = CU.cutoff_scan(gpu_ctx, P, x0[miss], dirs[miss], grid, mode=CU.SCAN_FULL,
+                                      precision=precision)
+        b2[miss] = b2m
+        st2 += st2m
+    assert 0.1 < miss.mean() < 0.9
+    assert np.array_equal(b1, b2)
+    assert np.array_equal(b1[miss], bf[miss])
+    assert np.array_equal(s1[miss], sf[miss])
+    inb = np.zeros_like(sf, dtype=bool)
+    for i in range(len(lo)):
+        inb[i, lo[i]:hi[i] + 1] = True
+    ok = ~miss
+    assert np.array_equal(s1[ok][inb[ok]], sf[ok][inb[ok]]) and np.all(s1[ok][~inb[ok]] == -1)
+    assert st1 < st2 and st1 < st_full
+    print(f"missed brackets {miss.mean():.3f}; steps one launch {st1}, two launches {st2}, full {st_full}")
+
+
 def test_refine_cutoff_matches_reference_bisection(gpu_ctx):
     """Batched bisection (FP64) vs the reference's RefineCutoffTransition
     compiled from CutoffBandSearch.h, driven by the oracle tracer: the same
