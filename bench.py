@@ -32,9 +32,14 @@ COPY_PROBE_BYTES = 1 << 30
 PMC_KERNELS = (("k_push_f64", "push"), ("k_push_f32", "push"), ("k_deposit_cell", "deposit_cell"),
                ("k_deposit_cell_rec", "deposit_cell"), ("k_deposit_cell_mfma", "deposit_cell"),
                ("k_deposit_node", "deposit_node"), ("k_deposit_node_staged", "deposit_node"),
-               ("k_scatter", "scatter"), ("k_count_keys", "count_keys"), ("k_copy_probe", "copy"),
+               ("k_scatter", "scatter"), ("k_mig_scatter", "scatter"), ("k_mig_classify", "mig_classify"),
+               ("k_mig_order", "mig_order"), ("k_copy_probe", "copy"),
                ("k_ecsim_mass", "ecsim_mass"), ("k_ecsim_op", "ecsim_op"),
                ("k_deposit_cell_lanes", "deposit_cell"))
+# kernels of the particle step: counted only after the copy probe, which the
+# counter child launches between the ECSIM set-up (a deposit of its own) and
+# the step
+STEP_KEYS = ("push", "deposit_cell", "deposit_node", "scatter", "mig_classify", "mig_order")
 
 
 def _pmc_pass(args, tune, counter, outdir):
@@ -58,7 +63,7 @@ def _pmc_pass(args, tune, counter, outdir):
     if not files:
         raise RuntimeError(f"rocprofv3 {counter} pass: exit {r.returncode}, no counter file; "
                            + r.stdout.decode(errors="replace")[-400:])
-    sums = {}
+    rows = []
     for f in files:
         with open(f) as fh:
             for row in csv.DictReader(fh):
@@ -67,9 +72,15 @@ def _pmc_pass(args, tune, counter, outdir):
                 name = row.get("Kernel_Name", "")
                 for ident, key in PMC_KERNELS:
                     if re.search(r"\b" + ident + r"\b", name):
-                        tot, cnt = sums.get(key, (0.0, 0))
-                        sums[key] = (tot + float(row["Counter_Value"]), cnt + 1)
+                        rows.append((int(row["Start_Timestamp"]), key, float(row["Counter_Value"])))
                         break
+    marker = max((t for t, k, _ in rows if k == "copy"), default=0)
+    sums = {}
+    for t, key, val in rows:
+        if key in STEP_KEYS and t < marker:
+            continue
+        tot, cnt = sums.get(key, (0.0, 0))
+        sums[key] = (tot + val, cnt + 1)
     return {k: tot / cnt for k, (tot, cnt) in sums.items()}
 
 
@@ -271,6 +282,37 @@ def mlp_bench(ctx, arch, args):
             "train_step_us": 1e6 * t_train / nsteps}
 
 
+def scan_bench(ctx, arch, args):
+    """The plain full-grid cutoff map (64x32 points x 8 directions x 48
+    rigidities, every bin traced) once in FP64 and once in FP32, after one
+    small warm-up scan: the tracer kernel's own rate, for profiles."""
+    import numpy as np
+
+    from amps_amd import _lib as L
+    from amps_amd import cutoff as CU
+
+    P = CU.config2_params()
+    pts, lat, lon = CU.config2_points()
+    zen, az = CU.config2_directions(8)
+    dirs = CU.arrival_dirs(pts, zen, az).reshape(-1, 3)
+    x0 = np.repeat(pts, len(zen), axis=0)
+    grid = CU.build_search_grid(0.1, 20.0, 48)
+    CU.cutoff_scan(ctx, P, x0[::256], dirs[::256], grid, mode=CU.SCAN_FULL, precision=L.FP32)
+    out = {}
+    for prec, tag in ((L.FP64, "fp64"), (L.FP32, "fp32")):
+        ctx.sync()
+        t0 = time.perf_counter()
+        _, _, st = CU.cutoff_scan(ctx, P, x0, dirs, grid, mode=CU.SCAN_FULL, precision=prec)
+        t = time.perf_counter() - t0
+        out[tag] = {"s": t, "steps": int(st), "steps_per_s": st / t}
+    return {"metric": "cutoff-map Boris steps/s, plain full scan (FP64)", "value": out["fp64"]["steps_per_s"],
+            "unit": "steps/s", "n_gpus": 1, "steps": 1, "warmup": 1, "ms_per_step": 1e3 * out["fp64"]["s"],
+            "higher_is_better": True, "scaling": "strong", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+            "config": {"workload": "64x32 lat/lon x 8 directions x 48 rigidities, every bin traced, 1e5-step cap",
+                       "device": arch},
+            "scan": out}
+
+
 def cutoff_map_bench(ctx, n_cu, n_train=1 << 20, epochs=20, cpu_check_points=16, tune_cache=None, n_batch=16):
     """Batched backward cutoff map (64x32 points x 8 directions x 48 bins)
     and the surrogate-bracketed map.  Returns a dict of timings and errors."""
@@ -416,7 +458,8 @@ def main():
     ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--tune-cache", default=None,
                     help="JSON file of tuning results to reuse (default: tune on every run)")
-    ap.add_argument("--workload", choices=["all", "pic", "map", "mlp"], default="all")
+    ap.add_argument("--workload", choices=["all", "pic", "map", "mlp", "scan"], default="all",
+                    help="scan: the plain cutoff-map scans alone (FP64, FP32), for tracer profiles")
     ap.add_argument("--train-samples-log2", type=int, default=20)
     ap.add_argument("--epochs", type=int, default=20)
     ap.add_argument("--tile", type=int, default=0, help="push tile (skips the tuner with the two below)")
@@ -467,6 +510,10 @@ def main():
         print(json.dumps(mlp_bench(ctx, arch, args)), flush=True)
         ctx.close()
         return
+    if args.workload == "scan":
+        print(json.dumps(scan_bench(ctx, arch, args)), flush=True)
+        ctx.close()
+        return
     mesh = M.three_level_mesh()
     E, B = M.corner_fields(mesh)
     bmax = float(np.max(np.linalg.norm(B, axis=-1)))
@@ -494,13 +541,17 @@ def main():
         tune = {"tile": args.tile, "deposit_kernel": args.deposit_kernel,
                 "deposit_chunks": args.deposit_chunks, "cached": "given"}
     if args.pmc_child:
-        # one eager step and the copy kernel of known size, for the counter passes
-        st.run_steps(1, dt, use_graph=False, **opts)
+        # the ECSIM kernels on their own mesh, the copy kernel of known size
+        # (calibration, and the marker after which the step's kernels count),
+        # then one eager step, for the counter passes
         c2, st2, En2, dt2 = ecsim_mass_setup()
         st2.MassMatrixApply(En2, -4 * np.pi * dt2 * 0.5)
         st2.EcsimOperator(st2.ecsim_params(dt2, cDt=2 * float(st2.mesh.cell_size()[0][0])), En2)
+        c2.sync()
         c2.close()
         ctx.copy_probe(COPY_PROBE_BYTES, reps=1)
+        ctx.sync()
+        st.run_steps(1, dt, use_graph=False, **opts)
         ctx.sync()
         ctx.close()
         return
