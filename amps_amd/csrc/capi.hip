@@ -1146,10 +1146,12 @@ extern "C" int amps_gpu_copy_probe(amps_ctx* c, int64_t nbytes, int32_t reps, do
   HIP_TRY(hipMalloc(&a, n4 * 16));
   HIP_TRY(hipMalloc(&b, n4 * 16));
   hipLaunchKernelGGL(k_fill_int, dim3(4096), dim3(256), 0, c->stream, (int*)a, n4 * 4, 1);
-  hipLaunchKernelGGL(k_copy_probe, dim3(8192), dim3(256), 0, c->stream, a, b, n4);
+  // one 16 KB block per workgroup (at most 65,536 workgroups, then a grid-stride loop)
+  const unsigned grid = (unsigned)std::min<int64_t>((n4 + 1023) / 1024, 65536);
+  hipLaunchKernelGGL(k_copy_probe, dim3(grid), dim3(256), 0, c->stream, a, b, n4);
   HIP_TRY(hipEventRecord(c->ev0, c->stream));
   for (int i = 0; i < reps; i++)
-    hipLaunchKernelGGL(k_copy_probe, dim3(8192), dim3(256), 0, c->stream, a, b, n4);
+    hipLaunchKernelGGL(k_copy_probe, dim3(grid), dim3(256), 0, c->stream, a, b, n4);
   HIP_TRY(hipEventRecord(c->ev1, c->stream));
   HIP_TRY(hipEventSynchronize(c->ev1));
   float t = 0.f;
