@@ -1,14 +1,17 @@
 // mlp.hip — the cutoff surrogate network on BF16 MFMA (gfx950).
 //
 // Network 6 -> 128 -> 128 -> 1 with GELU, BF16 weights and activations, FP32
-// accumulation (v_mfma_f32_16x16x16_bf16).  One workgroup (4 wavefronts)
-// processes 64 samples: wave w owns rows 16w..16w+15 of every activation tile.
-// Training fuses forward, loss and backward in one kernel that writes a
-// per-workgroup gradient partial; a second kernel reduces the partials in a
-// fixed order and applies Adam to the FP32 master weights, then refreshes the
-// BF16 copies (W2 is kept in both orientations so every MFMA operand is a
-// contiguous 64-bit load).  The training step loop is captured in a hipGraph;
-// batch offsets and Adam step counts live in device memory.
+// accumulation (v_mfma_f32_16x16x16_bf16).  Inference: one workgroup (4
+// wavefronts) per 64 samples, wave w owning rows 16w..16w+15 of every
+// activation tile.  Training: one workgroup per 16 samples, wave w owning
+// hidden units 32w..32w+31, forward, loss and backward fused in one kernel
+// that writes a per-workgroup gradient partial; a second kernel reduces the
+// partials in a fixed order, applies Adam to the
+// FP32 master weights, refreshes the BF16 copies (W2 is kept in both
+// orientations so every MFMA operand is a contiguous 64-bit load) and, in its
+// last workgroup, advances the step count and batch offset.  The training
+// step loop is captured in a hipGraph; batch offsets and Adam step counts
+// live in device memory.
 //
 // Parameter layout: W1[128][6] | b1[128] | W2[128][128] | b2[128] | w3[128] | b3
 #include <hip/hip_runtime.h>
@@ -136,183 +139,244 @@ __global__ __launch_bounds__(256) void k_mlp_forward(MlpView mv, const float* __
   }
 }
 
-// Fused forward + MSE loss + backward for 64 rows of the batch starting at
-// rows [*batch_off, ...) of the data (wrapping at n).  Writes this
-// workgroup's gradient partial (NPARAM floats) and loss partial.
+// Fused forward + MSE loss + backward for TS = 16 rows of the batch starting
+// at rows [*batch_off, ...) of the data (wrapping at n), spread over the
+// columns: wavefront w owns hidden units 32w .. 32w + 31 of both layers (its
+// two 16-wide column tiles), so a workgroup's chain of dependent products is
+// a quarter of what one wavefront per 16 rows would take, and a 4,096-sample
+// batch is 256 workgroups.  Writes this workgroup's gradient partial (NPARAM
+// floats) and loss partial.  LDS rows are padded (HP, TP) off the bank
+// period; every MFMA operand stays an aligned 8-byte read.
+constexpr int TS = 16;               // samples per training workgroup
+constexpr int HP = HID + 4;          // row stride of [TS][HID] tiles
+constexpr int TP = TS + 4;           // row stride of [HID][TS] and [16][TS] tiles
+
 __global__ __launch_bounds__(256) void k_mlp_train(MlpView mv, const float* __restrict__ X,
                                                     const float* __restrict__ T, int64_t n, int batch,
                                                     const int64_t* __restrict__ batch_off,
                                                     float* __restrict__ gpart,
                                                     float* __restrict__ lpart) {
-  __shared__ unsigned short Xb[TB * 16], XTb[16 * TB], H1b[TB * HID], H1Tb[HID * TB];
-  __shared__ unsigned short D2b[TB * HID], D2Tb[HID * TB], D1Tb[HID * TB];
-  __shared__ float red[4][3][HID];
-  __shared__ float lred[4];
-  const int64_t base = (batch_off ? *batch_off : 0) + (int64_t)blockIdx.x * TB;
-  // gather rows (with wrap-around) into contiguous LDS
-  for (int i = threadIdx.x; i < TB * 16; i += blockDim.x) {
+  __shared__ __align__(16) unsigned short Xb[TS * 16], XTb[16 * TP];
+  __shared__ __align__(16) unsigned short H1b[TS * HP], H1Tb[HID * TP];
+  __shared__ __align__(16) unsigned short D2b[TS * HP], D2Tb[HID * TP], D1Tb[HID * TP];
+  __shared__ float ypart[4][TS];
+  const int64_t base = (batch_off ? *batch_off : 0) + (int64_t)blockIdx.x * TS;
+  for (int i = threadIdx.x; i < TS * 16; i += blockDim.x) {
     const int m = i >> 4, k = i & 15;
     const int64_t r = (base + m) % n;
-    const float val = (k < DIN) ? X[r * DIN + k] : 0.0f;
-    const unsigned short h = f2bf(val);
+    const unsigned short h = f2bf((k < DIN) ? X[r * DIN + k] : 0.0f);
     Xb[m * 16 + k] = h;
-    XTb[k * TB + m] = h;
+    XTb[k * TP + m] = h;
   }
   __syncthreads();
-  const int w = threadIdx.x >> 6, l = threadIdx.x & 63, rl = l & 15, kq = 4 * (l >> 4), r0 = 16 * w;
-  float z1[8][4], z2[8][4], h2[8][4];
-  forward_rows(mv, Xb, H1b, H1Tb, w, l, z1, z2, h2);
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63, rl = l & 15, kq = 4 * (l >> 4);
+  float* g = gpart + (int64_t)blockIdx.x * NPARAM;
+  // layer 1 for this wave's two column tiles; the D layout: row kq + v, column 16 nt + rl
+  float z1[2][4], p1[2][4];  // pre-activation and Phi(z) = (1 + erf(z / sqrt 2)) / 2
+  const bf16x4 ax = ld4(Xb + rl * 16 + kq);
+#pragma unroll
+  for (int t = 0; t < 2; t++) {
+    const int nt = 2 * w + t, nn = 16 * nt + rl;
+    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+    acc = mfma(ax, ld4(mv.W1b + nn * 16 + kq), acc);
+    const float b1 = mv.theta[OFF_B1 + nn];
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+      const float z = acc[v] + b1;
+      const float ph = 0.5f * (1.0f + erff(z * 0.70710678118654752f));
+      z1[t][v] = z;
+      p1[t][v] = ph;
+      const unsigned short h = f2bf(z * ph);
+      H1b[(kq + v) * HP + nn] = h;
+      H1Tb[nn * TP + kq + v] = h;
+    }
+  }
+  __syncthreads();
+  // layer 2 for the same columns, and the output layer's partial sums
+  float z2[2][4], p2[2][4], h2[2][4];
+  float ys[4] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int t = 0; t < 2; t++) {
+    const int nt = 2 * w + t, nn = 16 * nt + rl;
+    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int kt = 0; kt < 8; kt++)
+      acc = mfma(ld4(H1b + rl * HP + 16 * kt + kq), ld4(mv.W2b + nn * HID + 16 * kt + kq), acc);
+    const float b2 = mv.theta[OFF_B2 + nn], w3 = mv.theta[OFF_W3 + nn];
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+      const float z = acc[v] + b2;
+      const float ph = 0.5f * (1.0f + erff(z * 0.70710678118654752f));
+      z2[t][v] = z;
+      p2[t][v] = ph;
+      h2[t][v] = bf2f(f2bf(z * ph));
+      ys[v] += h2[t][v] * w3;
+    }
+  }
+#pragma unroll
+  for (int v = 0; v < 4; v++) {
+    const float sv = sum16(ys[v]);
+    if (rl == 0) ypart[w][kq + v] = sv;
+  }
+  __syncthreads();
+  // the loss gradient of this lane's rows (every wave computes the same)
   const float b3 = mv.theta[OFF_B3];
   float dy[4];
   float lsum = 0.f;
 #pragma unroll
   for (int v = 0; v < 4; v++) {
-    float s = 0.f;
-#pragma unroll
-    for (int nt = 0; nt < 8; nt++) s += h2[nt][v] * mv.theta[OFF_W3 + 16 * nt + rl];
-    s = sum16(s) + b3;
-    const int64_t r = (base + r0 + kq + v) % n;
-    const float res = s - T[r];
+    const int m = kq + v;
+    const float y = (((ypart[0][m] + ypart[1][m]) + ypart[2][m]) + ypart[3][m]) + b3;
+    const float res = y - T[(base + m) % n];
     dy[v] = 2.0f * res / (float)batch;
     lsum += res * res;
   }
-  // dZ2 = dy w3 gelu'(z2); bias and w3 gradients
-  float db2[8], dw3[8];
+  // dZ2 = dy w3 gelu'(z2) on this wave's columns; their b2 and w3 gradients
 #pragma unroll
-  for (int nt = 0; nt < 8; nt++) {
-    const int nn = 16 * nt + rl;
+  for (int t = 0; t < 2; t++) {
+    const int nn = 16 * (2 * w + t) + rl;
     const float w3 = mv.theta[OFF_W3 + nn];
     float sb = 0.f, sw = 0.f;
 #pragma unroll
     for (int v = 0; v < 4; v++) {
-      const float d = dy[v] * w3 * gelu_grad(z2[nt][v]);
+      const float z = z2[t][v];
+      const float gg = p2[t][v] + z * __expf(-0.5f * z * z) * 0.39894228040143268f;
+      const float d = dy[v] * w3 * gg;
       const unsigned short h = f2bf(d);
-      const int m = r0 + kq + v;
-      D2b[m * HID + nn] = h;
-      D2Tb[nn * TB + m] = h;
+      D2b[(kq + v) * HP + nn] = h;
+      D2Tb[nn * TP + kq + v] = h;
       sb += d;
-      sw += dy[v] * h2[nt][v];
+      sw += dy[v] * h2[t][v];
     }
-    db2[nt] = sum_rows(sb);
-    dw3[nt] = sum_rows(sw);
+    sb = sum_rows(sb);
+    sw = sum_rows(sw);
+    if (l < 16) {
+      g[OFF_B2 + nn] = sb;
+      g[OFF_W3 + nn] = sw;
+    }
   }
-  float dysum = dy[0] + dy[1] + dy[2] + dy[3];
-  dysum = sum_rows(dysum);
+  if (w == 0) {
+    float ds = dy[0] + dy[1] + dy[2] + dy[3];
+    float ls = lsum;
+    ds = sum_rows(ds);
+    ls = sum_rows(ls);
+    if (l == 0) {
+      g[OFF_B3] = ds;
+      lpart[blockIdx.x] = ls;
+    }
+  }
   __syncthreads();
-  // dW2[n][k] = sum_m dZ2[m][n] H1[m][k]: wave w owns n-tiles 2w, 2w+1
+  // dW2[n][k] = sum_m dZ2[m][n] H1[m][k] for this wave's rows n (one
+  // 16-sample step per tile); dH1 = dZ2 W2 for this wave's columns k
 #pragma unroll
-  for (int ii = 0; ii < 2; ii++) {
-    const int it = 2 * w + ii;
+  for (int t = 0; t < 2; t++) {
+    const int it = 2 * w + t;
+    const bf16x4 a = ld4(D2Tb + (16 * it + rl) * TP + kq);
 #pragma unroll
     for (int jt = 0; jt < 8; jt++) {
       f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+      acc = mfma(a, ld4(H1Tb + (16 * jt + rl) * TP + kq), acc);
 #pragma unroll
-      for (int ks = 0; ks < 4; ks++)
-        acc = mfma(ld4(D2Tb + (16 * it + rl) * TB + 16 * ks + kq), ld4(H1Tb + (16 * jt + rl) * TB + 16 * ks + kq), acc);
-#pragma unroll
-      for (int v = 0; v < 4; v++)
-        gpart[(int64_t)blockIdx.x * NPARAM + OFF_W2 + (16 * it + kq + v) * HID + 16 * jt + rl] = acc[v];
+      for (int v = 0; v < 4; v++) g[OFF_W2 + (16 * it + kq + v) * HID + 16 * jt + rl] = acc[v];
     }
   }
-  // dH1 = dZ2 W2 ; dZ1 = dH1 gelu'(z1)
-  float db1[8];
 #pragma unroll
-  for (int jt = 0; jt < 8; jt++) {
+  for (int t = 0; t < 2; t++) {
+    const int jt = 2 * w + t, kk = 16 * jt + rl;
     f32x4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
     for (int ks = 0; ks < 8; ks++)
-      acc = mfma(ld4(D2b + (r0 + rl) * HID + 16 * ks + kq), ld4(mv.W2Tb + (16 * jt + rl) * HID + 16 * ks + kq), acc);
+      acc = mfma(ld4(D2b + rl * HP + 16 * ks + kq), ld4(mv.W2Tb + kk * HID + 16 * ks + kq), acc);
+    // dZ1 = dH1 gelu'(z1), the same columns as this wave's layer 1
     float sb = 0.f;
 #pragma unroll
     for (int v = 0; v < 4; v++) {
-      const float d = acc[v] * gelu_grad(z1[jt][v]);
-      D1Tb[(16 * jt + rl) * TB + r0 + kq + v] = f2bf(d);
+      const float z = z1[t][v];
+      const float gg = p1[t][v] + z * __expf(-0.5f * z * z) * 0.39894228040143268f;
+      const float d = acc[v] * gg;
+      D1Tb[kk * TP + kq + v] = f2bf(d);
       sb += d;
     }
-    db1[jt] = sum_rows(sb);
+    sb = sum_rows(sb);
+    if (l < 16) g[OFF_B1 + kk] = sb;
   }
-  // per-wave vector partials, reduced in a fixed order below
-  if (l < 16) {
+  // dW1[n][j] = sum_m dZ1[m][n] X[m][j]: the wave's own rows of D1Tb (the
+  // LDS operations of one wavefront execute in issue order, so a compiler
+  // barrier is all the ordering needed)
+  asm volatile("" ::: "memory");
 #pragma unroll
-    for (int nt = 0; nt < 8; nt++) {
-      red[w][0][16 * nt + l] = db1[nt];
-      red[w][1][16 * nt + l] = db2[nt];
-      red[w][2][16 * nt + l] = dw3[nt];
-    }
-  }
-  const float ls = sum_rows(lsum);
-  if (l == 0) lred[w] = ls;
-  __syncthreads();
-  // dW1[n][j] = sum_m dZ1[m][n] X[m][j]: wave w owns n-tiles 2w, 2w+1
-#pragma unroll
-  for (int ii = 0; ii < 2; ii++) {
-    const int it = 2 * w + ii;
+  for (int t = 0; t < 2; t++) {
+    const int it = 2 * w + t;
     f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-    for (int ks = 0; ks < 4; ks++)
-      acc = mfma(ld4(D1Tb + (16 * it + rl) * TB + 16 * ks + kq), ld4(XTb + rl * TB + 16 * ks + kq), acc);
+    acc = mfma(ld4(D1Tb + (16 * it + rl) * TP + kq), ld4(XTb + rl * TP + kq), acc);
     if (rl < DIN) {
 #pragma unroll
-      for (int v = 0; v < 4; v++)
-        gpart[(int64_t)blockIdx.x * NPARAM + OFF_W1 + (16 * it + kq + v) * DIN + rl] = acc[v];
+      for (int v = 0; v < 4; v++) g[OFF_W1 + (16 * it + kq + v) * DIN + rl] = acc[v];
     }
   }
-  for (int i = threadIdx.x; i < HID; i += blockDim.x) {
-    float* g = gpart + (int64_t)blockIdx.x * NPARAM;
-    g[OFF_B1 + i] = ((red[0][0][i] + red[1][0][i]) + red[2][0][i]) + red[3][0][i];
-    g[OFF_B2 + i] = ((red[0][1][i] + red[1][1][i]) + red[2][1][i]) + red[3][1][i];
-    g[OFF_W3 + i] = ((red[0][2][i] + red[1][2][i]) + red[2][2][i]) + red[3][2][i];
+}
+
+// Sum the workgroup partials in a fixed order, Adam on the FP32 master copy,
+// BF16 refresh.  A workgroup takes 64 consecutive parameters: wavefront w of
+// its 16 sums rows [w r, (w + 1) r) of the partials (r = nwg / 16 rounded
+// up) for the 64 parameters, one coalesced 256-byte row per load, and the
+// sixteen sums are added in order.  The last workgroup to finish (a ticket counter)
+// advances the step count and the batch offset, after every workgroup has
+// read them.
+constexpr int ADAM_PARAMS = 64, ADAM_WAVES = 16;
+
+__global__ __launch_bounds__(64 * ADAM_WAVES) void k_mlp_adam(const float* __restrict__ gpart, int nwg, float* theta,
+                                                   float* m, float* v, float* grad_out, int64_t* step_dev,
+                                                   unsigned* ticket, int64_t batch, int64_t nrows, float lr,
+                                                   int apply, unsigned short* W1b, unsigned short* W2b,
+                                                   unsigned short* W2Tb) {
+  __shared__ float part[ADAM_WAVES][ADAM_PARAMS];
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  const int j = blockIdx.x * ADAM_PARAMS + l;
+  const int64_t step = step_dev[0] + 1;
+  const int rows = (nwg + ADAM_WAVES - 1) / ADAM_WAVES, k0 = w * rows, k1 = min(nwg, k0 + rows);
+  float gsum = 0.f;
+  if (j < NPARAM) {
+#pragma unroll 8
+    for (int k = k0; k < k1; k++) gsum += gpart[(int64_t)k * NPARAM + j];
   }
-  if (threadIdx.x == 0) {
-    // b3: every wave's lanes hold the wave sum of dy; take it from red via lred path
-    lpart[blockIdx.x] = ((lred[0] + lred[1]) + lred[2]) + lred[3];
-  }
-  __shared__ float dys[4];
-  if (l == 0) dys[w] = dysum;
+  part[w][l] = gsum;
   __syncthreads();
-  if (threadIdx.x == 0)
-    gpart[(int64_t)blockIdx.x * NPARAM + OFF_B3] = ((dys[0] + dys[1]) + dys[2]) + dys[3];
-}
-
-// Sum the workgroup partials in order, Adam on the FP32 master copy, BF16
-// refresh.  step_dev counts Adam steps (incremented by thread 0 after use via
-// a separate tiny kernel so every thread reads the same value).
-__global__ __launch_bounds__(256) void k_mlp_adam(const float* __restrict__ gpart, int nwg,
-                                                   const float* __restrict__ lpart, float* theta,
-                                                   float* m, float* v, float* grad_out,
-                                                   const int64_t* __restrict__ step_dev, float lr,
-                                                   int apply, unsigned short* W1b,
-                                                   unsigned short* W2b, unsigned short* W2Tb) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= NPARAM) return;
-  float g = 0.f;
-  for (int k = 0; k < nwg; k++) g += gpart[(int64_t)k * NPARAM + j];
-  if (grad_out) grad_out[j] = g;
-  if (!apply) return;
-  const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
-  const int64_t step = *step_dev + 1;
-  const float mj = b1 * m[j] + (1.f - b1) * g;
-  const float vj = b2 * v[j] + (1.f - b2) * g * g;
-  m[j] = mj;
-  v[j] = vj;
-  const float mh = mj / (1.f - powf(b1, (float)step));
-  const float vh = vj / (1.f - powf(b2, (float)step));
-  const float th = theta[j] - lr * mh / (sqrtf(vh) + eps);
-  theta[j] = th;
-  const unsigned short hb = f2bf(th);
-  if (j < OFF_B1) {
-    W1b[(j / DIN) * 16 + (j % DIN)] = hb;
-  } else if (j >= OFF_W2 && j < OFF_B2) {
-    const int q = j - OFF_W2, nn = q / HID, kk = q % HID;
-    W2b[nn * HID + kk] = hb;
-    W2Tb[kk * HID + nn] = hb;
+  if (w == 0 && j < NPARAM) {
+    float gt = part[0][l];
+#pragma unroll
+    for (int q = 1; q < ADAM_WAVES; q++) gt += part[q][l];
+    if (grad_out) grad_out[j] = gt;
+    if (apply) {
+      const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+      const float mj = b1 * m[j] + (1.f - b1) * gt;
+      const float vj = b2 * v[j] + (1.f - b2) * gt * gt;
+      m[j] = mj;
+      v[j] = vj;
+      const float mh = mj / (1.f - powf(b1, (float)step));
+      const float vh = vj / (1.f - powf(b2, (float)step));
+      const float th = theta[j] - lr * mh / (sqrtf(vh) + eps);
+      theta[j] = th;
+      const unsigned short hb = f2bf(th);
+      if (j < OFF_B1) {
+        W1b[(j / DIN) * 16 + (j % DIN)] = hb;
+      } else if (j >= OFF_W2 && j < OFF_B2) {
+        const int q = j - OFF_W2, nn = q / HID, kk = q % HID;
+        W2b[nn * HID + kk] = hb;
+        W2Tb[kk * HID + nn] = hb;
+      }
+    }
   }
-}
-
-__global__ void k_mlp_advance(int64_t* step_dev, int64_t* batch_off, int64_t batch, int64_t n) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) {
-    *step_dev += 1;
-    *batch_off = (*batch_off + batch) % n;
+  if (!apply) return;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __threadfence();
+    if (atomicAdd(ticket, 1u) == gridDim.x - 1) {
+      step_dev[0] = step;
+      step_dev[1] = (step_dev[1] + batch) % nrows;
+      *ticket = 0u;
+      __threadfence();
+    }
   }
 }
 
@@ -347,6 +411,7 @@ struct MlpState {
   float* lpart = nullptr;
   int64_t gpart_wg = 0;
   int64_t* step_dev = nullptr;   // [0] Adam step count, [1] batch offset
+  unsigned* ticket = nullptr;    // workgroups of the Adam kernel done
   float* grad = nullptr;
   float* qX = nullptr;
   float* qy = nullptr;
@@ -365,7 +430,7 @@ void mlp_free(amps_ctx* c) {
   if (s->graph) (void)hipGraphDestroy(s->graph);
   for (void* p : {(void*)s->theta, (void*)s->m, (void*)s->v, (void*)s->W1b, (void*)s->W2b,
                   (void*)s->W2Tb, (void*)s->X, (void*)s->T, (void*)s->gpart, (void*)s->lpart,
-                  (void*)s->step_dev, (void*)s->grad, (void*)s->qX, (void*)s->qy})
+                  (void*)s->step_dev, (void*)s->ticket, (void*)s->grad, (void*)s->qX, (void*)s->qy})
     (void)hipFree(p);
   delete s;
   c->mlp_state = nullptr;
@@ -374,8 +439,18 @@ void mlp_free(amps_ctx* c) {
 
 static MlpView view(MlpState* s) { return MlpView{s->theta, s->W1b, s->W2b, s->W2Tb}; }
 
+// The captured training step holds the data and partial buffers' addresses
+// and the row count: it is dropped whenever one of them changes.
+static void drop_graph(MlpState* s) {
+  if (s->gexec) (void)hipGraphExecDestroy(s->gexec);
+  if (s->graph) (void)hipGraphDestroy(s->graph);
+  s->gexec = nullptr;
+  s->graph = nullptr;
+}
+
 static int ensure_gpart(amps_ctx* c, MlpState* s, int64_t nwg) {
   if (nwg <= s->gpart_wg) return 0;
+  drop_graph(s);
   (void)hipFree(s->gpart);
   (void)hipFree(s->lpart);
   s->gpart = nullptr;
@@ -402,7 +477,9 @@ extern "C" int amps_gpu_mlp_init(amps_ctx* c, const float* theta) {
     HIP_TRY(hipMalloc(&s->W2b, 2 * HID * HID));
     HIP_TRY(hipMalloc(&s->W2Tb, 2 * HID * HID));
     HIP_TRY(hipMalloc(&s->step_dev, sizeof(int64_t) * 2));
+    HIP_TRY(hipMalloc(&s->ticket, sizeof(unsigned)));
   }
+  HIP_TRY(hipMemsetAsync(s->ticket, 0, sizeof(unsigned), c->stream));
   HIP_TRY(hipMemcpyAsync(s->theta, theta, sizeof(float) * NPARAM, hipMemcpyHostToDevice, c->stream));
   HIP_TRY(hipMemsetAsync(s->m, 0, sizeof(float) * NPARAM, c->stream));
   HIP_TRY(hipMemsetAsync(s->v, 0, sizeof(float) * NPARAM, c->stream));
@@ -429,6 +506,9 @@ extern "C" int amps_gpu_mlp_set_data(amps_ctx* c, int64_t n, const float* X, con
   MlpState* s = mstate(c);
   if (!s) return fail(AMPS_E_STATE, "mlp: call amps_gpu_mlp_init first");
   if (n <= 0 || !X || !T) return fail(AMPS_E_ARG, "mlp_set_data: empty data");
+  (void)hipSetDevice(c->device);
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  drop_graph(s);
   if (n > s->cap) {
     (void)hipFree(s->X);
     (void)hipFree(s->T);
@@ -447,15 +527,15 @@ extern "C" int amps_gpu_mlp_set_data(amps_ctx* c, int64_t n, const float* X, con
   return AMPS_OK;
 }
 
+constexpr int ADAM_BLOCKS = (NPARAM + ADAM_PARAMS - 1) / ADAM_PARAMS;
+
 static void enqueue_step(amps_ctx* c, MlpState* s, int batch, float lr) {
-  const int nwg = batch / TB;
+  const int nwg = batch / TS;
   hipLaunchKernelGGL(k_mlp_train, dim3(nwg), dim3(256), 0, c->stream, view(s), s->X, s->T, s->n,
                      batch, s->step_dev + 1, s->gpart, s->lpart);
-  hipLaunchKernelGGL(k_mlp_adam, dim3((NPARAM + 255) / 256), dim3(256), 0, c->stream, s->gpart, nwg,
-                     s->lpart, s->theta, s->m, s->v, (float*)nullptr, s->step_dev, lr, 1, s->W1b,
-                     s->W2b, s->W2Tb);
-  hipLaunchKernelGGL(k_mlp_advance, dim3(1), dim3(64), 0, c->stream, s->step_dev, s->step_dev + 1,
-                     (int64_t)batch, s->n);
+  hipLaunchKernelGGL(k_mlp_adam, dim3(ADAM_BLOCKS), dim3(64 * ADAM_WAVES), 0, c->stream, s->gpart, nwg, s->theta, s->m,
+                     s->v, (float*)nullptr, s->step_dev, s->ticket, (int64_t)batch, s->n, lr, 1, s->W1b, s->W2b,
+                     s->W2Tb);
 }
 
 // Gradient of the mean-squared error over the first n rows of the resident
@@ -464,15 +544,14 @@ extern "C" int amps_gpu_mlp_grad(amps_ctx* c, int64_t n, float* grad_out, float*
   MlpState* s = mstate(c);
   if (!s || s->n == 0) return fail(AMPS_E_STATE, "mlp_grad: init and set_data first");
   if (n <= 0 || n % TB || n > s->n || n > (1 << 24)) return fail(AMPS_E_ARG, "mlp_grad: n must be a positive multiple of 64 <= data size");
-  const int nwg = (int)(n / TB);
+  const int nwg = (int)(n / TS);
   if (int r = ensure_gpart(c, s, nwg)) return r;
   int64_t zero = 0;
   HIP_TRY(hipMemcpyAsync(s->step_dev + 1, &zero, sizeof(int64_t), hipMemcpyHostToDevice, c->stream));
   hipLaunchKernelGGL(k_mlp_train, dim3(nwg), dim3(256), 0, c->stream, view(s), s->X, s->T, s->n, (int)n,
                      s->step_dev + 1, s->gpart, s->lpart);
-  hipLaunchKernelGGL(k_mlp_adam, dim3((NPARAM + 255) / 256), dim3(256), 0, c->stream, s->gpart, nwg,
-                     s->lpart, s->theta, s->m, s->v, s->grad, s->step_dev, 0.f, 0, s->W1b, s->W2b,
-                     s->W2Tb);
+  hipLaunchKernelGGL(k_mlp_adam, dim3(ADAM_BLOCKS), dim3(64 * ADAM_WAVES), 0, c->stream, s->gpart, nwg, s->theta, s->m, s->v,
+                     s->grad, s->step_dev, s->ticket, (int64_t)0, s->n, 0.f, 0, s->W1b, s->W2b, s->W2Tb);
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipStreamSynchronize(c->stream));
   if (grad_out) HIP_TRY(hipMemcpy(grad_out, s->grad, sizeof(float) * NPARAM, hipMemcpyDeviceToHost));
@@ -495,13 +574,10 @@ extern "C" int amps_gpu_mlp_train(amps_ctx* c, int64_t nsteps, int32_t batch, do
   if (batch <= 0 || batch % TB || batch > (1 << 20)) return fail(AMPS_E_ARG, "mlp_train: batch must be a positive multiple of 64");
   if (nsteps < 0) return fail(AMPS_E_ARG, "mlp_train: nsteps < 0");
   (void)hipSetDevice(c->device);
-  const int nwg = batch / TB;
+  const int nwg = batch / TS;
   if (int r = ensure_gpart(c, s, nwg)) return r;
   if (!s->gexec || s->graph_batch != batch || s->graph_lr != (float)lr) {
-    if (s->gexec) (void)hipGraphExecDestroy(s->gexec);
-    if (s->graph) (void)hipGraphDestroy(s->graph);
-    s->gexec = nullptr;
-    s->graph = nullptr;
+    drop_graph(s);
     HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
     enqueue_step(c, s, batch, (float)lr);
     HIP_TRY(hipStreamEndCapture(c->stream, &s->graph));

@@ -74,3 +74,23 @@ def test_adam_step_and_training(gpu_ctx):
     losses = sur.train(300, batch=4096, lr=1e-3, record_loss=True)
     print("loss first/last:", losses[0], losses[-1])
     assert losses[-1] < 0.2 * losses[0]
+
+
+def test_training_step_follows_new_data(gpu_ctx):
+    """Training with one data set, then a larger one (new device buffers) at
+    the same batch and rate: the captured training step is rebuilt, so the
+    next Adam step is the one of the new data's gradient."""
+    th = MO.init_params(6).astype(np.float32)
+    sur = Surrogate(gpu_ctx, theta=th)
+    X, t = _data(1 << 12, 3)
+    sur.set_data(X, t)
+    sur.train(3, batch=4096, lr=1e-3)
+    sur = Surrogate(gpu_ctx, theta=th)
+    X2, t2 = _data(1 << 15, 4)
+    sur.set_data(X2, t2)
+    g, _ = sur.grad(4096)
+    sur.train(1, batch=4096, lr=1e-3)
+    th1, steps = sur.params()
+    assert steps == 1
+    want, _, _ = MO.adam_step(th.astype(np.float64), 0, 0, g.astype(np.float64), 1, lr=1e-3)
+    assert np.allclose(th1, want, rtol=0, atol=2e-6)
