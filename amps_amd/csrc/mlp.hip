@@ -8,10 +8,10 @@
 // that writes a per-workgroup gradient partial; a second kernel reduces the
 // partials in a fixed order, applies Adam to the
 // FP32 master weights, refreshes the BF16 copies (W2 is kept in both
-// orientations so every MFMA operand is a contiguous 64-bit load) and, in its
-// last workgroup, advances the step count and batch offset.  The training
-// step loop is captured in a hipGraph; batch offsets and Adam step counts
-// live in device memory.
+// orientations so every MFMA operand is a contiguous 64-bit load) and
+// advances the step count and batch offset, kept in two device-memory slots
+// that consecutive steps use in turn.  Two training steps are captured in one
+// hipGraph.
 //
 // Parameter layout: W1[128][6] | b1[128] | W2[128][128] | b2[128] | w3[128] | b3
 #include <hip/hip_runtime.h>
@@ -320,20 +320,21 @@ __global__ __launch_bounds__(256) void k_mlp_train(MlpView mv, const float* __re
 // BF16 refresh.  A workgroup takes 64 consecutive parameters: wavefront w of
 // its 16 sums rows [w r, (w + 1) r) of the partials (r = nwg / 16 rounded
 // up) for the 64 parameters, one coalesced 256-byte row per load, and the
-// sixteen sums are added in order.  The last workgroup to finish (a ticket counter)
-// advances the step count and the batch offset, after every workgroup has
-// read them.
+// sixteen sums are added in order.  The step count and batch offset are
+// read from ctr_in and the advanced ones written to ctr_out by the first
+// workgroup: two slots used in turn by consecutive steps, so no workgroup
+// reads what another is writing.
 constexpr int ADAM_PARAMS = 64, ADAM_WAVES = 16;
 
 __global__ __launch_bounds__(64 * ADAM_WAVES) void k_mlp_adam(const float* __restrict__ gpart, int nwg, float* theta,
-                                                   float* m, float* v, float* grad_out, int64_t* step_dev,
-                                                   unsigned* ticket, int64_t batch, int64_t nrows, float lr,
-                                                   int apply, unsigned short* W1b, unsigned short* W2b,
-                                                   unsigned short* W2Tb) {
+                                                   float* m, float* v, float* grad_out,
+                                                   const int64_t* __restrict__ ctr_in, int64_t* ctr_out,
+                                                   int64_t batch, int64_t nrows, float lr, int apply,
+                                                   unsigned short* W1b, unsigned short* W2b, unsigned short* W2Tb) {
   __shared__ float part[ADAM_WAVES][ADAM_PARAMS];
   const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
   const int j = blockIdx.x * ADAM_PARAMS + l;
-  const int64_t step = step_dev[0] + 1;
+  const int64_t step = ctr_in[0] + 1;
   const int rows = (nwg + ADAM_WAVES - 1) / ADAM_WAVES, k0 = w * rows, k1 = min(nwg, k0 + rows);
   float gsum = 0.f;
   if (j < NPARAM) {
@@ -367,16 +368,9 @@ __global__ __launch_bounds__(64 * ADAM_WAVES) void k_mlp_adam(const float* __res
       }
     }
   }
-  if (!apply) return;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    __threadfence();
-    if (atomicAdd(ticket, 1u) == gridDim.x - 1) {
-      step_dev[0] = step;
-      step_dev[1] = (step_dev[1] + batch) % nrows;
-      *ticket = 0u;
-      __threadfence();
-    }
+  if (apply && blockIdx.x == 0 && threadIdx.x == 0) {
+    ctr_out[0] = step;
+    ctr_out[1] = (ctr_in[1] + batch) % nrows;
   }
 }
 
@@ -410,8 +404,9 @@ struct MlpState {
   float* gpart = nullptr;
   float* lpart = nullptr;
   int64_t gpart_wg = 0;
-  int64_t* step_dev = nullptr;   // [0] Adam step count, [1] batch offset
-  unsigned* ticket = nullptr;    // workgroups of the Adam kernel done
+  // two slots of (Adam step count, batch offset), read and written in turn by
+  // consecutive steps; slot 0 is the current one between calls
+  int64_t* step_dev = nullptr;
   float* grad = nullptr;
   float* qX = nullptr;
   float* qy = nullptr;
@@ -430,7 +425,7 @@ void mlp_free(amps_ctx* c) {
   if (s->graph) (void)hipGraphDestroy(s->graph);
   for (void* p : {(void*)s->theta, (void*)s->m, (void*)s->v, (void*)s->W1b, (void*)s->W2b,
                   (void*)s->W2Tb, (void*)s->X, (void*)s->T, (void*)s->gpart, (void*)s->lpart,
-                  (void*)s->step_dev, (void*)s->ticket, (void*)s->grad, (void*)s->qX, (void*)s->qy})
+                  (void*)s->step_dev, (void*)s->grad, (void*)s->qX, (void*)s->qy})
     (void)hipFree(p);
   delete s;
   c->mlp_state = nullptr;
@@ -476,14 +471,12 @@ extern "C" int amps_gpu_mlp_init(amps_ctx* c, const float* theta) {
     HIP_TRY(hipMalloc(&s->W1b, 2 * HID * 16));
     HIP_TRY(hipMalloc(&s->W2b, 2 * HID * HID));
     HIP_TRY(hipMalloc(&s->W2Tb, 2 * HID * HID));
-    HIP_TRY(hipMalloc(&s->step_dev, sizeof(int64_t) * 2));
-    HIP_TRY(hipMalloc(&s->ticket, sizeof(unsigned)));
+    HIP_TRY(hipMalloc(&s->step_dev, sizeof(int64_t) * 4));
   }
-  HIP_TRY(hipMemsetAsync(s->ticket, 0, sizeof(unsigned), c->stream));
   HIP_TRY(hipMemcpyAsync(s->theta, theta, sizeof(float) * NPARAM, hipMemcpyHostToDevice, c->stream));
   HIP_TRY(hipMemsetAsync(s->m, 0, sizeof(float) * NPARAM, c->stream));
   HIP_TRY(hipMemsetAsync(s->v, 0, sizeof(float) * NPARAM, c->stream));
-  HIP_TRY(hipMemsetAsync(s->step_dev, 0, sizeof(int64_t) * 2, c->stream));
+  HIP_TRY(hipMemsetAsync(s->step_dev, 0, sizeof(int64_t) * 4, c->stream));
   hipLaunchKernelGGL(k_mlp_refresh_bf16, dim3((HID * HID + 255) / 256), dim3(256), 0, c->stream, s->theta,
                      s->W1b, s->W2b, s->W2Tb);
   HIP_TRY(hipGetLastError());
@@ -529,13 +522,15 @@ extern "C" int amps_gpu_mlp_set_data(amps_ctx* c, int64_t n, const float* X, con
 
 constexpr int ADAM_BLOCKS = (NPARAM + ADAM_PARAMS - 1) / ADAM_PARAMS;
 
-static void enqueue_step(amps_ctx* c, MlpState* s, int batch, float lr) {
+// one Adam step reading the counters of slot `from` and writing the other
+static void enqueue_step(amps_ctx* c, MlpState* s, int batch, float lr, int from) {
   const int nwg = batch / TS;
+  const int64_t* in = s->step_dev + 2 * from;
+  int64_t* out = s->step_dev + 2 * (1 - from);
   hipLaunchKernelGGL(k_mlp_train, dim3(nwg), dim3(256), 0, c->stream, view(s), s->X, s->T, s->n,
-                     batch, s->step_dev + 1, s->gpart, s->lpart);
-  hipLaunchKernelGGL(k_mlp_adam, dim3(ADAM_BLOCKS), dim3(64 * ADAM_WAVES), 0, c->stream, s->gpart, nwg, s->theta, s->m,
-                     s->v, (float*)nullptr, s->step_dev, s->ticket, (int64_t)batch, s->n, lr, 1, s->W1b, s->W2b,
-                     s->W2Tb);
+                     batch, in + 1, s->gpart, s->lpart);
+  hipLaunchKernelGGL(k_mlp_adam, dim3(ADAM_BLOCKS), dim3(64 * ADAM_WAVES), 0, c->stream, s->gpart, nwg, s->theta,
+                     s->m, s->v, (float*)nullptr, in, out, (int64_t)batch, s->n, lr, 1, s->W1b, s->W2b, s->W2Tb);
 }
 
 // Gradient of the mean-squared error over the first n rows of the resident
@@ -550,8 +545,9 @@ extern "C" int amps_gpu_mlp_grad(amps_ctx* c, int64_t n, float* grad_out, float*
   HIP_TRY(hipMemcpyAsync(s->step_dev + 1, &zero, sizeof(int64_t), hipMemcpyHostToDevice, c->stream));
   hipLaunchKernelGGL(k_mlp_train, dim3(nwg), dim3(256), 0, c->stream, view(s), s->X, s->T, s->n, (int)n,
                      s->step_dev + 1, s->gpart, s->lpart);
-  hipLaunchKernelGGL(k_mlp_adam, dim3(ADAM_BLOCKS), dim3(64 * ADAM_WAVES), 0, c->stream, s->gpart, nwg, s->theta, s->m, s->v,
-                     s->grad, s->step_dev, s->ticket, (int64_t)0, s->n, 0.f, 0, s->W1b, s->W2b, s->W2Tb);
+  hipLaunchKernelGGL(k_mlp_adam, dim3(ADAM_BLOCKS), dim3(64 * ADAM_WAVES), 0, c->stream, s->gpart, nwg, s->theta,
+                     s->m, s->v, s->grad, s->step_dev, s->step_dev + 2, (int64_t)0, s->n, 0.f, 0, s->W1b, s->W2b,
+                     s->W2Tb);
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipStreamSynchronize(c->stream));
   if (grad_out) HIP_TRY(hipMemcpy(grad_out, s->grad, sizeof(float) * NPARAM, hipMemcpyDeviceToHost));
@@ -578,8 +574,10 @@ extern "C" int amps_gpu_mlp_train(amps_ctx* c, int64_t nsteps, int32_t batch, do
   if (int r = ensure_gpart(c, s, nwg)) return r;
   if (!s->gexec || s->graph_batch != batch || s->graph_lr != (float)lr) {
     drop_graph(s);
+    // two steps, slot 0 -> 1 -> 0: a replay leaves the counters in slot 0
     HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-    enqueue_step(c, s, batch, (float)lr);
+    enqueue_step(c, s, batch, (float)lr, 0);
+    enqueue_step(c, s, batch, (float)lr, 1);
     HIP_TRY(hipStreamEndCapture(c->stream, &s->graph));
     HIP_TRY(hipGraphInstantiate(&s->gexec, s->graph, nullptr, nullptr, 0));
     s->graph_batch = batch;
@@ -587,17 +585,27 @@ extern "C" int amps_gpu_mlp_train(amps_ctx* c, int64_t nsteps, int32_t batch, do
   }
   std::vector<float> lp;
   tbegin(c);
-  for (int64_t i = 0; i < nsteps; i++) {
-    HIP_TRY(hipGraphLaunch(s->gexec, c->stream));
-    if (loss_out) {
-      lp.resize((size_t)nwg);
+  if (loss_out) {
+    // one step at a time, each batch loss read back
+    lp.resize((size_t)nwg);
+    for (int64_t i = 0; i < nsteps; i++) {
+      enqueue_step(c, s, batch, (float)lr, (int)(i & 1));
+      HIP_TRY(hipGetLastError());
       HIP_TRY(hipMemcpyAsync(lp.data(), s->lpart, sizeof(float) * nwg, hipMemcpyDeviceToHost, c->stream));
       HIP_TRY(hipStreamSynchronize(c->stream));
-      double L = 0;
-      for (float x : lp) L += x;
-      loss_out[i] = (float)(L / batch);
+      double Ls = 0;
+      for (float x : lp) Ls += x;
+      loss_out[i] = (float)(Ls / batch);
+    }
+  } else {
+    for (int64_t i = 0; i + 1 < nsteps; i += 2) HIP_TRY(hipGraphLaunch(s->gexec, c->stream));
+    if (nsteps & 1) {
+      enqueue_step(c, s, batch, (float)lr, 0);
+      HIP_TRY(hipGetLastError());
     }
   }
+  if (nsteps & 1)  // the counters back in slot 0
+    HIP_TRY(hipMemcpyAsync(s->step_dev, s->step_dev + 2, sizeof(int64_t) * 2, hipMemcpyDeviceToDevice, c->stream));
   tend(c, AMPS_K_MLP);
   HIP_TRY(hipStreamSynchronize(c->stream));
   return AMPS_OK;
