@@ -181,9 +181,20 @@ def test_user_mode_exit_list_graph_steps_and_capacity(gpu_ctx):
         gpu_ctx.set_param("exit_capacity", 1 << 20)
 
 
-def test_push_multistep_with_migration(gpu_ctx):
+@pytest.mark.parametrize("deterministic", [1, 0])
+def test_push_multistep_with_migration(gpu_ctx, deterministic):
+    """12 pushes with migration against the oracle, per particle bit for bit;
+    deterministic 0 is the scatter in the order its atomics land."""
     mesh = M.central_octant_mesh()
     E, B, parts, sp = _case(mesh, 60000, 0.2, 0.05, seed=7)
+    gpu_ctx.set_param("deterministic", deterministic)
+    try:
+        _multistep(gpu_ctx, mesh, E, B, parts, sp)
+    finally:
+        gpu_ctx.set_param("deterministic", 1)
+
+
+def _multistep(gpu_ctx, mesh, E, B, parts, sp):
     st = PicState(mesh, E, B, sp, ctx=gpu_ctx)
     st.set_particles(parts)
     om = O.Mesh(mesh)

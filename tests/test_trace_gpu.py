@@ -195,6 +195,33 @@ def test_one_launch_bracketed_scan_matches_two_launches(gpu_ctx, precision):
     print(f"missed brackets {miss.mean():.3f}; steps one launch {st1}, two launches {st2}, full {st_full}")
 
 
+@pytest.mark.parametrize("half", [0, 24])
+def test_bracketed_scan_extreme_widths(gpu_ctx, half):
+    """One-bin brackets (most directions rescan the whole grid) and brackets
+    that are the whole grid (nothing to rescan): the one-launch bracketed
+    scan still gives the full scan's bins and samples."""
+    from amps_amd import surrogate as SG
+    P = CU.config2_params()
+    P.max_steps = 20000
+    pts, lat, lon = CU.config2_points(nlon=6, nlat=4)
+    zen, az = CU.config2_directions(4)
+    dirs = CU.arrival_dirs(pts, zen, az).reshape(-1, 3)
+    x0 = np.repeat(pts, len(zen), axis=0)
+    grid = CU.build_search_grid(0.1, 20.0, 48)
+    sf, bf, _ = CU.cutoff_scan(gpu_ctx, P, x0, dirs, grid, mode=CU.SCAN_FULL)
+    guess = np.clip(np.where(bf < 0, 47, bf) + np.random.default_rng(3).integers(-2, 3, len(bf)), 0, 47)
+    lo, hi = SG.bracket_bins(guess, 48, half=half)
+    if half == 24:
+        lo, hi = np.zeros_like(lo), np.full_like(hi, 47)
+    print(f"bracketed scan with brackets of {hi[0] - lo[0] + 1} bins", flush=True)
+    sb, bb, _ = CU.cutoff_scan(gpu_ctx, P, x0, dirs, grid, mode=CU.SCAN_BRACKETED, lo=lo, hi=hi)
+    assert np.array_equal(bb, bf)
+    traced = sb >= 0
+    assert np.array_equal(sb[traced], sf[traced])
+    if half == 24:
+        assert traced.all()
+
+
 def test_refine_cutoff_matches_reference_bisection(gpu_ctx):
     """Batched bisection (FP64) vs the reference's RefineCutoffTransition
     compiled from CutoffBandSearch.h, driven by the oracle tracer: the same
