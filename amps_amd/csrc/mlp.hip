@@ -161,6 +161,27 @@ __global__ __launch_bounds__(256) void k_mlp_train(MlpView mv, const float* __re
   __shared__ __align__(16) unsigned short D2b[TS * HP], D2Tb[HID * TP], D1Tb[HID * TP];
   __shared__ float ypart[4][TS];
   const int64_t base = (batch_off ? *batch_off : 0) + (int64_t)blockIdx.x * TS;
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63, rl = l & 15, kq = 4 * (l >> 4);
+  // everything this lane reads from the weights and targets, loaded up front
+  // so that the load latency is paid once, not once per phase
+  bf16x4 bw1[2], bw2[2][8], bw2t[2][8];
+  float pb1[2], pb2[2], pw3[2], tv[4];
+#pragma unroll
+  for (int t = 0; t < 2; t++) {
+    const int nn = 16 * (2 * w + t) + rl;
+    bw1[t] = ld4(mv.W1b + nn * 16 + kq);
+#pragma unroll
+    for (int kt = 0; kt < 8; kt++) {
+      bw2[t][kt] = ld4(mv.W2b + nn * HID + 16 * kt + kq);
+      bw2t[t][kt] = ld4(mv.W2Tb + nn * HID + 16 * kt + kq);
+    }
+    pb1[t] = mv.theta[OFF_B1 + nn];
+    pb2[t] = mv.theta[OFF_B2 + nn];
+    pw3[t] = mv.theta[OFF_W3 + nn];
+  }
+  const float b3 = mv.theta[OFF_B3];
+#pragma unroll
+  for (int v = 0; v < 4; v++) tv[v] = T[(base + kq + v) % n];
   for (int i = threadIdx.x; i < TS * 16; i += blockDim.x) {
     const int m = i >> 4, k = i & 15;
     const int64_t r = (base + m) % n;
@@ -169,7 +190,6 @@ __global__ __launch_bounds__(256) void k_mlp_train(MlpView mv, const float* __re
     XTb[k * TP + m] = h;
   }
   __syncthreads();
-  const int w = threadIdx.x >> 6, l = threadIdx.x & 63, rl = l & 15, kq = 4 * (l >> 4);
   float* g = gpart + (int64_t)blockIdx.x * NPARAM;
   // layer 1 for this wave's two column tiles; the D layout: row kq + v, column 16 nt + rl
   float z1[2][4], p1[2][4];  // pre-activation and Phi(z) = (1 + erf(z / sqrt 2)) / 2
@@ -178,8 +198,8 @@ __global__ __launch_bounds__(256) void k_mlp_train(MlpView mv, const float* __re
   for (int t = 0; t < 2; t++) {
     const int nt = 2 * w + t, nn = 16 * nt + rl;
     f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-    acc = mfma(ax, ld4(mv.W1b + nn * 16 + kq), acc);
-    const float b1 = mv.theta[OFF_B1 + nn];
+    acc = mfma(ax, bw1[t], acc);
+    const float b1 = pb1[t];
 #pragma unroll
     for (int v = 0; v < 4; v++) {
       const float z = acc[v] + b1;
@@ -201,8 +221,8 @@ __global__ __launch_bounds__(256) void k_mlp_train(MlpView mv, const float* __re
     f32x4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
     for (int kt = 0; kt < 8; kt++)
-      acc = mfma(ld4(H1b + rl * HP + 16 * kt + kq), ld4(mv.W2b + nn * HID + 16 * kt + kq), acc);
-    const float b2 = mv.theta[OFF_B2 + nn], w3 = mv.theta[OFF_W3 + nn];
+      acc = mfma(ld4(H1b + rl * HP + 16 * kt + kq), bw2[t][kt], acc);
+    const float b2 = pb2[t], w3 = pw3[t];
 #pragma unroll
     for (int v = 0; v < 4; v++) {
       const float z = acc[v] + b2;
@@ -220,14 +240,13 @@ __global__ __launch_bounds__(256) void k_mlp_train(MlpView mv, const float* __re
   }
   __syncthreads();
   // the loss gradient of this lane's rows (every wave computes the same)
-  const float b3 = mv.theta[OFF_B3];
   float dy[4];
   float lsum = 0.f;
 #pragma unroll
   for (int v = 0; v < 4; v++) {
     const int m = kq + v;
     const float y = (((ypart[0][m] + ypart[1][m]) + ypart[2][m]) + ypart[3][m]) + b3;
-    const float res = y - T[(base + m) % n];
+    const float res = y - tv[v];
     dy[v] = 2.0f * res / (float)batch;
     lsum += res * res;
   }
@@ -235,7 +254,7 @@ __global__ __launch_bounds__(256) void k_mlp_train(MlpView mv, const float* __re
 #pragma unroll
   for (int t = 0; t < 2; t++) {
     const int nn = 16 * (2 * w + t) + rl;
-    const float w3 = mv.theta[OFF_W3 + nn];
+    const float w3 = pw3[t];
     float sb = 0.f, sw = 0.f;
 #pragma unroll
     for (int v = 0; v < 4; v++) {
@@ -286,7 +305,7 @@ __global__ __launch_bounds__(256) void k_mlp_train(MlpView mv, const float* __re
     f32x4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
     for (int ks = 0; ks < 8; ks++)
-      acc = mfma(ld4(D2b + rl * HP + 16 * ks + kq), ld4(mv.W2Tb + kk * HID + 16 * ks + kq), acc);
+      acc = mfma(ld4(D2b + rl * HP + 16 * ks + kq), bw2t[t][ks], acc);
     // dZ1 = dH1 gelu'(z1), the same columns as this wave's layer 1
     float sb = 0.f;
 #pragma unroll
