@@ -961,7 +961,14 @@ static int launch_deposit(amps_ctx* c, double dt) {
   const int nb_eff = (c->node_batch == 0 && !stride_grid) ? 8 : c->node_batch;
   const int64_t per_wg = 4 * (int64_t)(nb_eff > 0 ? nb_eff : 1);
   int64_t node_wgs = ((c->nnodes + per_wg - 1) / per_wg + 7) / 8 * 8;
-  if (stride_grid) node_wgs = std::min<int64_t>(node_wgs, (int64_t)c->n_cu * 8);
+  if (stride_grid) {
+    // every workgroup resident at once, so that each XCD's band moves as one
+    int per_cu = 0;
+    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(k_deposit_node_staged),
+                                                         256, 0));
+    const int64_t resident = std::max<int64_t>(8, (int64_t)c->n_cu * std::max(per_cu, 1) / 8 * 8);
+    node_wgs = std::min<int64_t>(node_wgs, resident);
+  }
   // the record-batched and matrix-core cell passes write the moment form unless bit-exact
   if (c->dep_kernel >= 1 && !c->dep_exact && c->node_staged)
     hipLaunchKernelGGL(k_deposit_node_staged, dim3((unsigned)node_wgs), dim3(256), 0, c->stream,

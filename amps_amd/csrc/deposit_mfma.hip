@@ -79,7 +79,9 @@ __global__ __launch_bounds__(64 * waves_of<PAIRED>()) void k_deposit_cell_mfma(M
   constexpr int MW = waves_of<PAIRED>();
   constexpr int NCW = PAIRED ? 8 : 16;  // cells per wavefront
   __shared__ double sB[125 * 3];
-  __shared__ double rec[MW][NFLD * FS];
+  // 8 doubles of padding: the operand loads run up to two steps (8
+  // particles) past a segment's last particle, and are not used there
+  __shared__ double rec[MW][NFLD * FS + 8];
   __shared__ double sconst[MAX_SPEC][3];  // chargeQ, QdT_over_2m, matrixConst
   __shared__ int sfirst[MW][NCW], scum[MW][NCW + 1];
   __shared__ double sstage[MW][CS];  // a finished cell's record, written out in full lines
@@ -166,28 +168,24 @@ __global__ __launch_bounds__(64 * waves_of<PAIRED>()) void k_deposit_cell_mfma(M
   // destinations of the 8 accumulator values; -1: unused.  v_mfma_f64_16x16x4f64
   // holds A[i][k] in lane i + 16k, B[k][j] in lane j + 16k and D[i][j] in
   // lane 16 (i % 4) + j, accumulator register i / 4.
+  double* S = sstage[w];
   int oidx[8];
-  bool ojc[8];
 #pragma unroll
   for (int T = 0; T < 2; T++) {
 #pragma unroll
     for (int r = 0; r < 4; r++) {
       const int i = slot + 4 * r, col = 16 * T + j;
       int o = -1;
-      bool jc = false;
       if (i < 9 && col < 27) {
         o = 24 + 9 * (i + 9 * (col / 9)) + col % 9;
       } else if (i >= 9 && i < 15 && col >= 27 && col < 31) {
         const int bx = (i - 9) / 3, d = (i - 9) % 3, by = (col - 27) & 1, bz = (col - 27) >> 1;
         o = 3 * corner_of(bx, by, bz) + d;
-        jc = true;
       }
       oidx[4 * T + r] = o;
-      ojc[4 * T + r] = jc;
     }
   }
   const double invV = 1.0 / CellVolume;
-  double* S = sstage[w];
   if (lane < CS - CELL_VALS_MOMENT) S[CELL_VALS_MOMENT + lane] = 0.0;
 
   // each lane's particles of the next two chunks, loaded ahead
@@ -195,12 +193,23 @@ __global__ __launch_bounds__(64 * waves_of<PAIRED>()) void k_deposit_cell_mfma(M
     double x[3], v[3];
     int s, c;
   };
-  auto fetch = [&](int pos, Pre& n) {
+  // lane k < NCW: where cell k starts in the concatenation, and the offset
+  // from a position in the concatenation to the particle's index
+  const int cumk = lane < NCW ? cum[lane] : 0;
+  const int dk = lane < NCW ? sfirst[w][lane] - cum[lane] : 0;
+  // the particles at positions pbase + lane (called by the whole wavefront):
+  // a position's cell is the last one that starts at or before it
+  auto fetch = [&](int pbase, Pre& n) {
+    const int pos = pbase + lane;
+    int c = __popcll(__ballot(lane >= 1 && lane < NCW && cumk <= pbase));
+    uint64_t starts = __ballot(lane >= 1 && lane < NCW && cumk > pbase && cumk <= pbase + 63);
+    while (starts) {
+      const int k = __builtin_ctzll(starts);
+      starts &= starts - 1;
+      c += (pos >= __builtin_amdgcn_readlane(cumk, k)) ? 1 : 0;
+    }
+    const int p = pos + __shfl(dk, c, 64);
     if (pos < total) {
-      int c = 0;
-#pragma unroll
-      for (int k = 1; k < NCW; k++) c += (pos >= cum[k]) ? 1 : 0;
-      const int p = sfirst[w][c] + (pos - cum[c]);
       n.c = c;
       n.x[0] = x[p], n.x[1] = x[cap + p], n.x[2] = x[2 * cap + p];
       if (vf32) {
@@ -214,21 +223,18 @@ __global__ __launch_bounds__(64 * waves_of<PAIRED>()) void k_deposit_cell_mfma(M
   };
   Pre n1 = {{0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}, 0, 0}, n2 = n1;
 
-  // two accumulator pairs (even and odd steps): four independent MFMA chains
-  dbl4 C0 = {0.0, 0.0, 0.0, 0.0}, C1 = C0, D0 = C0, D1 = C0;
+  // the two 16 x 16 halves of the cell's 16 x 32 tile
+  dbl4 C0 = {0.0, 0.0, 0.0, 0.0}, C1 = C0;
   int cur = 0;  // the cell being accumulated (uniform)
   auto flush = [&](int c) {
     const int key = cellkey(c);
 #pragma unroll
     for (int r = 0; r < 4; r++) {
-      const double e0 = C0[r] + D0[r], e1 = C1[r] + D1[r];
-      if (oidx[r] >= 0) S[oidx[r]] = ojc[r] ? e0 * invV : e0;
-      if (oidx[4 + r] >= 0) S[oidx[4 + r]] = ojc[4 + r] ? e1 * invV : e1;
+      if (oidx[r] >= 0) S[oidx[r]] = C0[r];
+      if (oidx[4 + r] >= 0) S[oidx[4 + r]] = C1[r];
     }
     C0 = dbl4{0.0, 0.0, 0.0, 0.0};
     C1 = C0;
-    D0 = C0;
-    D1 = C0;
     wave_lds_sync();
     // 2176 bytes from a 128-byte boundary: every store covers whole lines
     double* dst = cellpart + (size_t)key * CS;
@@ -239,8 +245,8 @@ __global__ __launch_bounds__(64 * waves_of<PAIRED>()) void k_deposit_cell_mfma(M
   // cells with no particles before the first particle
   while (cur < NCW && uniform(cum[cur + 1]) == 0) flush(cur++);
 
-  fetch(lane, n1);
-  fetch(lane + 64, n2);
+  fetch(0, n1);
+  fetch(64, n2);
   // the records of the chunk starting at particle base (VALU); the particles
   // two chunks ahead are loaded meanwhile
   auto build = [&](int base) {
@@ -269,51 +275,44 @@ __global__ __launch_bounds__(64 * waves_of<PAIRED>()) void k_deposit_cell_mfma(M
       if constexpr (BC) {
         center_B(m, Bc, xn, b, Bv);  // centre-based B, pic_field_solver_ecsim.cpp:2097-2129
       } else {
-        double wn[8], norm = 0.0;
+        // the corner weights sum to 1 (InitStencil's normalization changes
+        // them by rounding only)
+        double lxy[4];
+#pragma unroll
+        for (int l = 0; l < 4; l++) lxy[l] = Lf[l >> 1] * Lf[2 + (l & 1)];
 #pragma unroll
         for (int l = 0; l < 8; l++) {
-          wn[l] = Lf[l >> 2] * Lf[2 + ((l >> 1) & 1)] * Lf[4 + (l & 1)];
-          norm += wn[l];
-        }
-        const double rn = 1.0 / norm;
-#pragma unroll
-        for (int l = 0; l < 8; l++) {
-          const double wl = wn[l] * rn;
+          const double wl = lxy[l >> 1] * Lf[4 + (l & 1)];
           const int c = (iX[0] - sx + (l >> 2)) + 5 * ((iX[1] - sy + ((l >> 1) & 1)) + 5 * (iX[2] - sz + (l & 1)));
 #pragma unroll
           for (int d = 0; d < 3; d++) Bv[d] += wl * sB[3 * c + d];
         }
       }
-      const double QdT_over_2m = sconst[s][1];
-      const double QdT_over_2m_squared = QdT_over_2m * QdT_over_2m;
-      double BB[3][3], P[3], alpha[9];
+      // alpha = c0 (I + P x + BB) with P = -QdT_over_2m B and BB = P P^T,
+      // kept as its numerators a[]; c0 goes into the two factors that use it
+      // (MC alpha, and q alpha v, which also takes the 1 / V of J)
+      double qb[3];
 #pragma unroll
-      for (int ii = 0; ii < 3; ii++) {
-        P[ii] = -QdT_over_2m * Bv[ii];
-#pragma unroll
-        for (int jj = 0; jj <= ii; jj++) {
-          BB[ii][jj] = QdT_over_2m_squared * Bv[ii] * Bv[jj];
-          BB[jj][ii] = BB[ii][jj];
-        }
-      }
-      const double c0 = 1.0 / (1.0 + QdT_over_2m_squared * (Bv[0] * Bv[0] + Bv[1] * Bv[1] + Bv[2] * Bv[2]));
-      alpha[0] = c0 * (1.0 + BB[0][0]);
-      alpha[1] = c0 * (-P[2] + BB[0][1]);
-      alpha[2] = c0 * (P[1] + BB[0][2]);
-      alpha[3] = c0 * (P[2] + BB[1][0]);
-      alpha[4] = c0 * (1.0 + BB[1][1]);
-      alpha[5] = c0 * (-P[0] + BB[1][2]);
-      alpha[6] = c0 * (-P[1] + BB[2][0]);
-      alpha[7] = c0 * (P[0] + BB[2][1]);
-      alpha[8] = c0 * (1.0 + BB[2][2]);
-      const double chargeQ = sconst[s][0], MC = sconst[s][2];
+      for (int ii = 0; ii < 3; ii++) qb[ii] = sconst[s][1] * Bv[ii];
+      const double c0 = 1.0 / (1.0 + (qb[0] * qb[0] + qb[1] * qb[1] + qb[2] * qb[2]));
+      double alpha[9];
+      alpha[0] = 1.0 + qb[0] * qb[0];
+      alpha[1] = qb[2] + qb[0] * qb[1];
+      alpha[2] = -qb[1] + qb[0] * qb[2];
+      alpha[3] = -qb[2] + qb[1] * qb[0];
+      alpha[4] = 1.0 + qb[1] * qb[1];
+      alpha[5] = qb[0] + qb[1] * qb[2];
+      alpha[6] = qb[1] + qb[2] * qb[0];
+      alpha[7] = -qb[0] + qb[2] * qb[1];
+      alpha[8] = 1.0 + qb[2] * qb[2];
+      const double MC = sconst[s][2] * c0, qc = sconst[s][0] * c0 * invV;
       double qv[3];
 #pragma unroll
       for (int i = 0; i < 3; i++) {
         double vr = 0.0;
 #pragma unroll
         for (int jj = 0; jj < 3; jj++) vr += alpha[3 * i + jj] * vI[jj];
-        qv[i] = chargeQ * vr;
+        qv[i] = qc * vr;
       }
       double mo[3][3];
 #pragma unroll
@@ -344,7 +343,7 @@ __global__ __launch_bounds__(64 * waves_of<PAIRED>()) void k_deposit_cell_mfma(M
 #undef FLD
     }
     n1 = n2;
-    fetch(pos + 128, n2);  // in flight while two chunks are accumulated
+    fetch(base + 128, n2);  // in flight while two chunks are accumulated
   };
   // the matrix products of that chunk, cell by cell, and the finished cells' records
   auto accumulate = [&](int base) {
@@ -363,38 +362,40 @@ __global__ __launch_bounds__(64 * waves_of<PAIRED>()) void k_deposit_cell_mfma(M
         struct Ops {
           double a, x0, y0, x1, y1;
         };
-        auto ld = [&](const double* Rs, Ops& o) {
-          o.a = Rs[oa];
-          o.x0 = Rs[o0a];
-          o.y0 = Rs[o0b];
-          o.x1 = Rs[o1a];
-          o.y1 = Rs[o1b];
+        // one pointer per operand field, advanced by two steps per iteration
+        const double *Ra = R + oa, *Rx0 = R + o0a, *Ry0 = R + o0b, *Rx1 = R + o1a, *Ry1 = R + o1b;
+        auto ld = [&](int k, Ops& o) {
+          o.a = Ra[k];
+          o.x0 = Rx0[k];
+          o.y0 = Ry0[k];
+          o.x1 = Rx1[k];
+          o.y1 = Ry1[k];
         };
-        auto step = [&](const Ops& o, dbl4& A0, dbl4& A1) {
+        auto step = [&](const Ops& o) {
           const double b0 = o.x0 * o.y0, b1 = o.x1 * fma(o.y1, keep, one);
-          A0 = __builtin_amdgcn_mfma_f64_16x16x4f64(o.a, b0, A0, 0, 0, 0);
-          A1 = __builtin_amdgcn_mfma_f64_16x16x4f64(o.a, b1, A1, 0, 0, 0);
+          C0 = __builtin_amdgcn_mfma_f64_16x16x4f64(o.a, b0, C0, 0, 0, 0);
+          C1 = __builtin_amdgcn_mfma_f64_16x16x4f64(o.a, b1, C1, 0, 0, 0);
         };
         // two operand sets, unrolled by two so that neither is copied while
         // its loads are in flight; each is reloaded right after its products
-        // (the loads past the last step re-read the last step's records, so
-        // that every load is unconditional and the waits stay exact)
-        const int last = nfull - 1;
+        // (the loads of the two steps past the last one read records past
+        // the segment, inside rec[], and are not used)
         Ops pa, pb;
-        ld(R, pa);
-        ld(R + 4 * (last < 1 ? last : 1), pb);
+        ld(0, pa);
+        ld(4, pb);
         __builtin_amdgcn_sched_barrier(0);
         int t = 0;
         for (; t + 1 < nfull; t += 2) {
           // the scheduling barriers keep each reload where it is issued
-          step(pa, C0, C1);
-          ld(R + 4 * (t + 2 < last ? t + 2 : last), pa);
+          step(pa);
+          Ra += 8, Rx0 += 8, Ry0 += 8, Rx1 += 8, Ry1 += 8;
+          ld(0, pa);
           __builtin_amdgcn_sched_barrier(0);
-          step(pb, D0, D1);
-          ld(R + 4 * (t + 3 < last ? t + 3 : last), pb);
+          step(pb);
+          ld(4, pb);
           __builtin_amdgcn_sched_barrier(0);
         }
-        if (t < nfull) step(pa, C0, C1);
+        if (t < nfull) step(pa);
         R += 4 * nfull;
       }
       if (q + 4 * nfull < se) {

@@ -687,14 +687,25 @@ __global__ __launch_bounds__(256) void k_deposit_node_staged(int64_t nnodes, int
     reinterpret_cast<int4*>(&gtab[0][0])[threadIdx.x] = reinterpret_cast<const int4*>(&c_gtab[0][0])[threadIdx.x];
   __syncthreads();
   const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  // npw > 0: wavefront g takes nodes [g npw, (g+1) npw).  npw == 0: the grid
-  // strides over the nodes, wavefront g taking g, g + G, g + 2G, ... (G the
-  // grid's wavefronts), so the nodes in flight at any time form one
-  // contiguous band whose incident cells stay in L2.
-  const int64_t gw = (int64_t)xcd_tile() * 4 + wv;
-  const int64_t first = npw > 0 ? gw * npw : gw;
-  const int64_t last = npw > 0 ? ((first + npw < nnodes) ? first + npw : nnodes) : nnodes;
-  const int64_t stride = npw > 0 ? 1 : (int64_t)gridDim.x * 4;
+  // npw > 0: workgroup g takes nodes [4 g npw, 4 (g+1) npw), its wavefronts
+  // interleaved (wavefront w: the w-th, (w+4)-th, ... of them), so that the
+  // four nodes a workgroup has in flight are consecutive and read the cells
+  // they share (a node and the next one share four) while those are cached.
+  // npw == 0 (a resident grid): XCD x takes the x-th eighth of the nodes and
+  // its G wavefronts stride over it, wavefront i taking i, i + G, i + 2G, ...
+  int64_t first, last, stride;
+  if (npw > 0) {
+    const int64_t g0 = (int64_t)xcd_tile() * 4 * npw;
+    first = g0 + wv;
+    last = (g0 + 4 * npw < nnodes) ? g0 + 4 * npw : nnodes;
+    stride = 4;
+  } else {
+    const int64_t part = (nnodes + N_XCD - 1) / N_XCD;
+    const int64_t x0 = (int64_t)(blockIdx.x % N_XCD) * part;
+    first = x0 + (int64_t)(blockIdx.x / N_XCD) * 4 + wv;
+    last = (x0 + part < nnodes) ? x0 + part : nnodes;
+    stride = (int64_t)(gridDim.x / N_XCD) * 4;
+  }
   const int lane = threadIdx.x & 63;
   double* Gw = G[wv];
   // this lane's gathered values e = lane + 64i: incident cell e / 75, value e % 75
