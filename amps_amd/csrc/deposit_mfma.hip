@@ -44,8 +44,8 @@ typedef double dbl4 __attribute__((ext_vector_type(4)));
 // (16 fields x 2 particles per 32-lane pass) fall on distinct banks.
 constexpr int NFLD = 32;
 constexpr int FS = 66;
-constexpr int F_A = 0;      // A operand rows 0-15 (row 15 stays zero)
-constexpr int F_ZERO = 15;
+constexpr int F_A = 0;      // A operand rows 0-14
+constexpr int F_ONE = 15;   // 1: A row 15 and the factor of the wy wz columns
 constexpr int F_MA = 16;    // MC alpha[k], k = 0..8
 constexpr int F_MZ = 25;    // mz[tz], tz = 0..2
 constexpr int F_WYZ = 28;   // wy[by] wz[bz] at by + 2bz
@@ -150,21 +150,18 @@ __global__ __launch_bounds__(64 * waves_of<PAIRED>()) void k_deposit_cell_mfma(M
   const int j = lane & 15, slot = lane >> 4;
   const int f0a = F_MZ + j / 9, f0b = F_MA + j % 9;
   int f1a, f1b;
-  bool f1w = false;  // column 27-30: the record field itself
   {
+    // A row 15 and B column 31 hold ones; the results they touch are not kept
     const int col = 16 + j;
     if (col < 27) {
       f1a = F_MZ + col / 9, f1b = F_MA + col % 9;
     } else if (col < 31) {
-      f1a = F_WYZ + (col - 27), f1b = F_ZERO, f1w = true;
+      f1a = F_WYZ + (col - 27), f1b = F_ONE;
     } else {
-      f1a = F_ZERO, f1b = F_ZERO;
+      f1a = F_ONE, f1b = F_ONE;
     }
   }
   const int oa = (F_A + j) * FS, o0a = f0a * FS, o0b = f0b * FS, o1a = f1a * FS, o1b = f1b * FS;
-  // B1 factor blend: the loaded factor, or 1 for the wy wz columns (the field read
-  // there is F_ZERO, so loaded * keep + one is exact either way)
-  const double keep = f1w ? 0.0 : 1.0, one = f1w ? 1.0 : 0.0;
   // destinations of the 8 accumulator values; -1: unused.  v_mfma_f64_16x16x4f64
   // holds A[i][k] in lane i + 16k, B[k][j] in lane j + 16k and D[i][j] in
   // lane 16 (i % 4) + j, accumulator register i / 4.
@@ -247,18 +244,19 @@ __global__ __launch_bounds__(64 * waves_of<PAIRED>()) void k_deposit_cell_mfma(M
 
   fetch(0, n1);
   fetch(64, n2);
-  // the records of the chunk starting at particle base (VALU); the particles
-  // two chunks ahead are loaded meanwhile
-  auto build = [&](int base) {
+  // the records of the chunk starting at particle base (VALU) from its
+  // particles in nb; the particles two chunks ahead are then loaded into nb
+  // (chunks alternate between n1 and n2)
+  auto build = [&](int base, Pre& nb) {
     const int pos = base + lane;
     if (pos < total) {
       // the record: the reference's expressions, contracted to FMA where
       // the compiler finds them (this pass is the tolerance-bound one)
 #pragma clang fp contract(fast)
-      const double xI[3] = {n1.x[0], n1.x[1], n1.x[2]};
-      const double vI[3] = {n1.v[0], n1.v[1], n1.v[2]};
-      const int s = n1.s;
-      const int nc = n1.c;
+      const double xI[3] = {nb.x[0], nb.x[1], nb.x[2]};
+      const double vI[3] = {nb.v[0], nb.v[1], nb.v[2]};
+      const int s = nb.s;
+      const int nc = nb.c;
       const int iX[3] = {sx + (nc & 3), sy + row0 + (nc >> 2), zc};
       double Lf[6], xn[3];
 #pragma unroll
@@ -267,7 +265,7 @@ __global__ __launch_bounds__(64 * waves_of<PAIRED>()) void k_deposit_cell_mfma(M
         if (fabs(xd - g.xmax[d]) < 1e-10 * g.dxs[d]) xd = g.xmax[d] - 1e-10 * g.dxs[d];
         xn[d] = xd;
         const double t = (xd - g.xmin[d]) * inv_dxs[d] - iX[d];
-        const double f = t < 0.0 ? 0.0 : (t > 1.0 ? 1.0 : t);
+        const double f = fmin(fmax(t, 0.0), 1.0);
         Lf[2 * d] = 1.0 - f;
         Lf[2 * d + 1] = f;
       }
@@ -280,12 +278,15 @@ __global__ __launch_bounds__(64 * waves_of<PAIRED>()) void k_deposit_cell_mfma(M
         double lxy[4];
 #pragma unroll
         for (int l = 0; l < 4; l++) lxy[l] = Lf[l >> 1] * Lf[2 + (l & 1)];
+        // the cell's lower corner in the staged 5 x 5 x 5 corners; corner l
+        // at a fixed offset from it
+        const double* sBc = sB + 3 * ((nc & 3) + 5 * (row0 + (nc >> 2)) + 25 * (w & 3));
 #pragma unroll
         for (int l = 0; l < 8; l++) {
           const double wl = lxy[l >> 1] * Lf[4 + (l & 1)];
-          const int c = (iX[0] - sx + (l >> 2)) + 5 * ((iX[1] - sy + ((l >> 1) & 1)) + 5 * (iX[2] - sz + (l & 1)));
+          const int c = (l >> 2) + 5 * ((l >> 1) & 1) + 25 * (l & 1);
 #pragma unroll
-          for (int d = 0; d < 3; d++) Bv[d] += wl * sB[3 * c + d];
+          for (int d = 0; d < 3; d++) Bv[d] += wl * sBc[3 * c + d];
         }
       }
       // alpha = c0 (I + P x + BB) with P = -QdT_over_2m B and BB = P P^T,
@@ -331,7 +332,7 @@ __global__ __launch_bounds__(64 * waves_of<PAIRED>()) void k_deposit_cell_mfma(M
       for (int bx = 0; bx < 2; bx++)
 #pragma unroll
         for (int d = 0; d < 3; d++) FLD(F_A + 9 + 3 * bx + d) = Lf[bx] * qv[d];
-      FLD(F_ZERO) = 0.0;
+      FLD(F_ONE) = 1.0;
 #pragma unroll
       for (int k = 0; k < 9; k++) FLD(F_MA + k) = MC * alpha[k];
 #pragma unroll
@@ -342,8 +343,7 @@ __global__ __launch_bounds__(64 * waves_of<PAIRED>()) void k_deposit_cell_mfma(M
         for (int by = 0; by < 2; by++) FLD(F_WYZ + by + 2 * bz) = Lf[2 + by] * Lf[4 + bz];
 #undef FLD
     }
-    n1 = n2;
-    fetch(base + 128, n2);  // in flight while two chunks are accumulated
+    fetch(base + 128, nb);  // in flight while two chunks are accumulated
   };
   // the matrix products of that chunk, cell by cell, and the finished cells' records
   auto accumulate = [&](int base) {
@@ -372,7 +372,7 @@ __global__ __launch_bounds__(64 * waves_of<PAIRED>()) void k_deposit_cell_mfma(M
           o.y1 = Ry1[k];
         };
         auto step = [&](const Ops& o) {
-          const double b0 = o.x0 * o.y0, b1 = o.x1 * fma(o.y1, keep, one);
+          const double b0 = o.x0 * o.y0, b1 = o.x1 * o.y1;
           C0 = __builtin_amdgcn_mfma_f64_16x16x4f64(o.a, b0, C0, 0, 0, 0);
           C1 = __builtin_amdgcn_mfma_f64_16x16x4f64(o.a, b1, C1, 0, 0, 0);
         };
@@ -404,7 +404,7 @@ __global__ __launch_bounds__(64 * waves_of<PAIRED>()) void k_deposit_cell_mfma(M
         if (q + 4 * nfull + slot < se) {
           a = R[oa];
           b0 = R[o0a] * R[o0b];
-          b1 = R[o1a] * fma(R[o1b], keep, one);
+          b1 = R[o1a] * R[o1b];
         }
         C0 = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b0, C0, 0, 0, 0);
         C1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b1, C1, 0, 0, 0);
@@ -417,11 +417,17 @@ __global__ __launch_bounds__(64 * waves_of<PAIRED>()) void k_deposit_cell_mfma(M
     }
   };
   if constexpr (!PAIRED) {
-    for (int base = 0; base < total; base += 64) {
-      build(base);
+    for (int base = 0; base < total; base += 128) {
+      build(base, n1);
       wave_lds_sync();
       accumulate(base);
       wave_lds_sync();  // the records are read before the next chunk overwrites them
+      if (base + 64 < total) {
+        build(base + 64, n2);
+        wave_lds_sync();
+        accumulate(base + 64);
+        wave_lds_sync();
+      }
     }
   } else {
     // phase p: waves 0-3 build chunk p/2 at even p and accumulate it at odd
@@ -434,9 +440,12 @@ __global__ __launch_bounds__(64 * waves_of<PAIRED>()) void k_deposit_cell_mfma(M
     for (int ph = 0; ph < nphase; ph++) {
       const int k = ph - grp;
       if (k >= 0 && (k >> 1) < nch) {
-        if ((k & 1) == 0)
-          build(64 * (k >> 1));
-        else
+        if ((k & 1) == 0) {
+          if ((k >> 1) & 1)
+            build(64 * (k >> 1), n2);
+          else
+            build(64 * (k >> 1), n1);
+        } else
           accumulate(64 * (k >> 1));
       }
       __syncthreads();
