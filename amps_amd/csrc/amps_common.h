@@ -94,6 +94,11 @@ struct amps_ctx {
   std::vector<int32_t> h_node_of_corner;
   std::vector<int64_t> h_node_keys;
   int* d_inc8 = nullptr;  // [nnodes][8] in node_order: cell_gid*8 + corner, ascending cell order, -1 padded
+  // [nnodes][8] in node_order, slot t = the octant (t & 1, t >> 1 & 1, t >> 2)
+  // of the node that the cell lies in; an empty octant holds zero_rec*8 + the
+  // corner a cell there would have (the moment-record gathers read zeros)
+  int* d_inc8_oct = nullptr;
+  int64_t zero_rec = 0;   // a moment record index past every cell record, kept zero
   int* d_node_order = nullptr; // node processing order of the corner flush
   // fields and species
   bool has_fields = false;

@@ -123,6 +123,7 @@ static void free_mesh(amps_ctx* c) {
   c->d_Bm = nullptr;
   c->has_mode3d = false;
   (void)hipFree(c->d_inc8);
+  (void)hipFree(c->d_inc8_oct);
   (void)hipFree(c->d_node_order);
   c->d_node_order = nullptr;
   (void)hipFree(c->d_offset);
@@ -140,7 +141,7 @@ static void free_mesh(amps_ctx* c) {
   (void)hipFree(c->d_E);
   (void)hipFree(c->d_B);
   c->d_xmin = c->d_xmax = c->d_dxc = c->d_dxs = nullptr;
-  c->d_leaf = c->d_inc8 = nullptr;
+  c->d_leaf = c->d_inc8 = c->d_inc8_oct = nullptr;
   c->d_offset = c->d_cursor = c->d_tcount = nullptr;
   c->d_cellpart = c->d_nodeout = nullptr;
   c->d_E = c->d_B = nullptr;
@@ -437,6 +438,31 @@ extern "C" int amps_gpu_set_mesh(amps_ctx* c, const amps_mesh_desc* desc, const 
   }
   HIP_TRY(hipMalloc(&c->d_inc8, sizeof(int) * std::max<size_t>(8, inc8.size())));
   if (!inc8.empty()) HIP_TRY(hipMemcpy(c->d_inc8, inc8.data(), sizeof(int) * inc8.size(), hipMemcpyHostToDevice));
+  // the same rows by octant: a cell of which the node is corner q lies on the
+  // node's side 1 - (corner offset bits of q); one cell per octant at most.
+  // The zero record lies past both record layouts of the cell buffer.
+  c->zero_rec = ((int64_t)nb * NCELL * CELL_VALS + CELL_STRIDE_MOMENT - 1) / CELL_STRIDE_MOMENT;
+  if (c->zero_rec * 8 + 8 >= ((int64_t)1 << 31)) return fail(AMPS_E_RANGE, "too many cells for the node gather");
+  int q_of_oct[8];
+  for (int q = 0; q < 8; q++)
+    q_of_oct[(1 - CornerOff[q][0]) + 2 * (1 - CornerOff[q][1]) + 4 * (1 - CornerOff[q][2])] = q;
+  std::vector<int> inc8o((size_t)c->nnodes * 8);
+  for (int64_t sl = 0; sl < c->nnodes; sl++) {
+    int* row = &inc8o[(size_t)sl * 8];
+    for (int t = 0; t < 8; t++) row[t] = (int)(c->zero_rec * 8) + q_of_oct[t];
+    for (int e = 0; e < 8; e++) {
+      const int ic = inc8[(size_t)sl * 8 + e];
+      if (ic < 0) continue;
+      const int q = ic & 7;
+      const int t = (1 - CornerOff[q][0]) + 2 * (1 - CornerOff[q][1]) + 4 * (1 - CornerOff[q][2]);
+      if ((row[t] >> 3) != c->zero_rec)
+        return fail(AMPS_E_ARG, "amps_gpu_set_mesh: two cells in one octant of a corner node");
+      row[t] = ic;
+    }
+  }
+  HIP_TRY(hipMalloc(&c->d_inc8_oct, sizeof(int) * std::max<size_t>(8, inc8o.size())));
+  if (!inc8o.empty())
+    HIP_TRY(hipMemcpy(c->d_inc8_oct, inc8o.data(), sizeof(int) * inc8o.size(), hipMemcpyHostToDevice));
   // device upload
   HIP_TRY(hipMalloc(&c->d_xmin, sizeof(double) * 3 * nb));
   HIP_TRY(hipMalloc(&c->d_xmax, sizeof(double) * 3 * nb));
@@ -972,10 +998,10 @@ static int launch_deposit(amps_ctx* c, double dt) {
   // the record-batched and matrix-core cell passes write the moment form unless bit-exact
   if (c->dep_kernel >= 1 && !c->dep_exact && c->node_staged)
     hipLaunchKernelGGL(k_deposit_node_staged, dim3((unsigned)node_wgs), dim3(256), 0, c->stream,
-                       c->nnodes, nb_eff, c->d_node_order, c->d_inc8, c->d_cellpart, c->d_nodeout);
+                       c->nnodes, nb_eff, c->d_node_order, c->d_inc8_oct, c->d_cellpart, c->d_nodeout);
   else if (c->dep_kernel >= 1 && !c->dep_exact)
     hipLaunchKernelGGL(k_deposit_node<true>, dim3((unsigned)node_wgs), dim3(256), 0, c->stream,
-                       c->nnodes, nb_eff, c->d_node_order, c->d_inc8, c->d_cellpart, c->d_nodeout);
+                       c->nnodes, nb_eff, c->d_node_order, c->d_inc8_oct, c->d_cellpart, c->d_nodeout);
   else
     hipLaunchKernelGGL(k_deposit_node<false>, dim3((unsigned)node_wgs), dim3(256), 0, c->stream,
                        c->nnodes, nb_eff, c->d_node_order, c->d_inc8, c->d_cellpart, c->d_nodeout);
@@ -985,7 +1011,14 @@ static int launch_deposit(amps_ctx* c, double dt) {
 }
 
 static int ensure_deposit_buffers(amps_ctx* c) {
-  if (!c->d_cellpart) HIP_TRY(hipMalloc(&c->d_cellpart, sizeof(double) * (size_t)c->mv.nb * NCELL * CELL_VALS));
+  if (!c->d_cellpart) {
+    // the cell records (either layout), then the zero moment record
+    const size_t n = std::max<size_t>((size_t)c->mv.nb * NCELL * CELL_VALS,
+                                      (size_t)(c->zero_rec + 1) * CELL_STRIDE_MOMENT);
+    HIP_TRY(hipMalloc(&c->d_cellpart, sizeof(double) * n));
+    HIP_TRY(hipMemset(c->d_cellpart + (size_t)c->zero_rec * CELL_STRIDE_MOMENT, 0,
+                      sizeof(double) * CELL_STRIDE_MOMENT));
+  }
   if (!c->d_nodeout) HIP_TRY(hipMalloc(&c->d_nodeout, sizeof(double) * (size_t)c->nnodes * NODE_STRIDE));
   return 0;
 }

@@ -657,18 +657,23 @@ __global__ __launch_bounds__(256) void k_deposit_node(int64_t nnodes, int npw,
 template __global__ void k_deposit_node<false>(int64_t, int, const int*, const int*, const double*, double*);
 template __global__ void k_deposit_node<true>(int64_t, int, const int*, const int*, const double*, double*);
 
-// Node gather for the moment record, staged.  A node that is corner q of an
-// incident cell needs 75 of the cell's 267 values: J of corner q and the
-// moments t = b_q + b_j of its 8 neighbours j, which lie in four runs of 18
-// consecutive values.  The wavefront first copies the 8 x 75 values its node
-// needs into LDS with loads whose lanes walk those runs (a handful of cache
-// lines per load instead of one per lane), then each lane sums its outputs
-// from LDS in ascending cell order.  Same operands, same order: the result is
-// bit-identical to k_deposit_node<true>.
-// c_gtab[q][w]: offset in the cell record of gathered value w (w < 72: run
-// w / 18 of the 2x2x2 moment block above b_q, w >= 72: J of corner q).
-// c_wtab[q][val]: the gathered value that output val takes from such a cell,
-// -1 where the cell does not touch that slot (val >= 246: padding).
+// Node gather for the moment record, staged.  The incidence rows come in
+// octant order (slot t: the cell on the node's side (t & 1, t >> 1 & 1,
+// t >> 2), of which the node is a fixed corner; an empty octant points at a
+// record of zeros).  A node that is corner q of a cell needs 75 of the cell's
+// 267 values: J of corner q and the moments t = b_q + b_j of its 8 neighbours
+// j, which lie in four runs of 18 consecutive values.  The wavefront first
+// copies the 8 x 75 values its node needs into LDS with loads whose lanes
+// walk those runs (a handful of cache lines per load instead of one per
+// lane), then each lane sums its outputs from LDS in octant order.  Which
+// value a lane loads and which LDS slot each of its outputs reads depend on
+// the lane alone, so they are worked out once per wavefront.  Same operands
+// in the same order as k_deposit_node<true> over the same rows: bit-identical.
+// c_gtab[t][w]: offset in the record of octant t's cell of gathered value w
+// (w < 72: run w / 18 of the 2x2x2 moment block above the node's corner,
+// w >= 72: J of that corner).
+// c_wtab[t][val]: the gathered value that output val takes from octant t's
+// cell, -1 where that cell does not touch the slot (val >= 246: padding).
 __constant__ __align__(16) short c_gtab[8][80];
 __constant__ __align__(16) short c_wtab[8][256];
 
@@ -679,14 +684,10 @@ __global__ __launch_bounds__(256) void k_deposit_node_staged(int64_t nnodes, int
                                                              const int* __restrict__ inc8,
                                                              const double* __restrict__ cellpart,
                                                              double* __restrict__ nodeout) {
-  __shared__ __align__(16) short wtab[8][256];
-  __shared__ __align__(16) short gtab[8][80];
-  __shared__ double G[4][8 * NODE_G];
-  reinterpret_cast<int4*>(&wtab[0][0])[threadIdx.x] = reinterpret_cast<const int4*>(&c_wtab[0][0])[threadIdx.x];
-  if (threadIdx.x < 8 * 80 / 8)
-    reinterpret_cast<int4*>(&gtab[0][0])[threadIdx.x] = reinterpret_cast<const int4*>(&c_gtab[0][0])[threadIdx.x];
-  __syncthreads();
+  __shared__ double G[4][8 * NODE_G + 1];  // the gathered values, then a zero
   const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  double* Gw = G[wv];
   // npw > 0: workgroup g takes nodes [4 g npw, 4 (g+1) npw), its wavefronts
   // interleaved (wavefront w: the w-th, (w+4)-th, ... of them), so that the
   // four nodes a workgroup has in flight are consecutive and read the cells
@@ -706,32 +707,52 @@ __global__ __launch_bounds__(256) void k_deposit_node_staged(int64_t nnodes, int
     last = (x0 + part < nnodes) ? x0 + part : nnodes;
     stride = (int64_t)(gridDim.x / N_XCD) * 4;
   }
-  const int lane = threadIdx.x & 63;
-  double* Gw = G[wv];
-  // this lane's gathered values e = lane + 64i: incident cell e / 75, value e % 75
+  if (first >= last) return;
+  // this lane's gathered values e = lane + 64 i: value e % 75 of octant e / 75;
+  // step i covers octants t0(i) and t0(i) + 1, hib bit i: the second one
   constexpr int NI = (8 * NODE_G + 63) / 64;
-  // The next node's incidence row and gathered values are loaded into
-  // registers while the current node is summed (two buffers, the loop
-  // unrolled by two so that neither is copied while its loads are in flight;
-  // past the last node the last one is loaded again).
+  int goff[NI];
+  uint32_t hib = 0;
+#pragma unroll
+  for (int i = 0; i < NI; i++) {
+    const int e = lane + 64 * i;
+    const int t0 = (64 * i) / NODE_G;
+    const int t = e < 8 * NODE_G ? e / NODE_G : t0;
+    goff[i] = e < 8 * NODE_G ? c_gtab[t][e - NODE_G * t] : 0;
+    hib |= (t != t0 ? 1u : 0u) << i;
+  }
+  // the LDS slot each output val = lane + 64 r takes from octant t (the zero
+  // where the octant's cell does not touch it)
+  const double* src[8][4];
+#pragma unroll
+  for (int t = 0; t < 8; t++)
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      const int w = c_wtab[t][lane + 64 * r];
+      src[t][r] = Gw + (w < 0 ? 8 * NODE_G : NODE_G * t + w);
+    }
+  if (lane == 0) Gw[8 * NODE_G] = 0.0;
+  // The next node's gathered values are loaded into registers while the
+  // current node is summed (two buffers, the loop unrolled by two so that
+  // neither is copied while its loads are in flight; past the last node the
+  // last one is loaded again).
   struct Buf {
-    int ics[8];
     double v[NI];
     int64_t node;
   };
   auto gather = [&](int64_t slot, Buf& B) {
+    int ics[8];
 #pragma unroll
-    for (int t = 0; t < 8; t++) B.ics[t] = inc8[slot * 8 + t];
+    for (int t = 0; t < 8; t++) ics[t] = inc8[slot * 8 + t];
     B.node = node_order[slot];
 #pragma unroll
     for (int i = 0; i < NI; i++) {
-      const int e = lane + 64 * i;
-      const int t0 = (64 * i) / NODE_G;  // the lanes of step i cover cells t0 and t0 + 1
-      const bool hi = t0 + 1 < 8 && e >= NODE_G * (t0 + 1);
-      const int ic = hi ? B.ics[t0 + 1 < 8 ? t0 + 1 : 7] : B.ics[t0];
-      const int w = e - NODE_G * (hi ? t0 + 1 : t0);
-      B.v[i] = (e < 8 * NODE_G && ic >= 0) ? cellpart[(size_t)(ic >> 3) * CELL_STRIDE_MOMENT + gtab[ic & 7][w]]
-                                            : 0.0;
+      // the cell of octant t0 or t0 + 1, chosen by masking (not by indexing
+      // ics[] with a lane-dependent subscript)
+      const int t0 = (64 * i) / NODE_G;
+      const int clo = ics[t0] >> 3, chi = ics[t0 + 1 < 8 ? t0 + 1 : 7] >> 3;
+      const int cell = clo + ((chi - clo) & -(int)((hib >> i) & 1));
+      B.v[i] = cellpart[(size_t)cell * CELL_STRIDE_MOMENT + goff[i]];
     }
   };
   auto finish = [&](const Buf& B) {
@@ -743,18 +764,9 @@ __global__ __launch_bounds__(256) void k_deposit_node_staged(int64_t nnodes, int
     asm volatile("" ::: "memory");  // the DS ops of a wavefront execute in issue order
     double acc[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-    for (int t = 0; t < 8; t++) {
-      const int ic = B.ics[t];
-      if (ic >= 0) {
-        const short* wt = wtab[ic & 7];
+    for (int t = 0; t < 8; t++)
 #pragma unroll
-        for (int r = 0; r < 4; r++) {
-          const int w = wt[lane + 64 * r];
-          const double val = Gw[NODE_G * t + (w < 0 ? 0 : w)];
-          acc[r] += (w >= 0) ? val : -0.0;
-        }
-      }
-    }
+      for (int r = 0; r < 4; r++) acc[r] += *src[t][r];
     asm volatile("" ::: "memory");
     // node records are not read again by this step: non-temporal stores keep
     // them from displacing the cell records still being gathered
@@ -765,7 +777,6 @@ __global__ __launch_bounds__(256) void k_deposit_node_staged(int64_t nnodes, int
       __builtin_nontemporal_store((val < NODE_VALS) ? acc[r] : 0.0, &o[val]);
     }
   };
-  if (first >= last) return;
   const int64_t final_slot = first + (last - 1 - first) / stride * stride;
   auto clampslot = [&](int64_t s2) { return s2 < final_slot ? s2 : final_slot; };
   Buf A, Bb;
@@ -811,9 +822,13 @@ int upload_inv_index_matrix(hipStream_t s) {
       soffm[q][val] = (short)offm;
     }
   }
-  // staged gather tables for the moment record
+  // staged gather tables for the moment record, by octant t of the cell
+  // around the node (the node is corner q of it, the corner whose offset
+  // bits are 1 - the octant's bits)
   short gtab[8][80], wtab[8][256];
-  for (int q = 0; q < 8; q++) {
+  for (int t = 0; t < 8; t++) {
+    int q = 0;
+    while (CB[q][0] != 1 - (t & 1) || CB[q][1] != 1 - ((t >> 1) & 1) || CB[q][2] != 1 - (t >> 2)) q++;
     for (int w = 0; w < 80; w++) {
       int off = 0;
       if (w < 72) {
@@ -822,7 +837,7 @@ int upload_inv_index_matrix(hipStream_t s) {
       } else if (w < 75) {
         off = 3 * q + (w - 72);
       }
-      gtab[q][w] = (short)off;
+      gtab[t][w] = (short)off;
     }
     int inv[27];
     for (int s2 = 0; s2 < 27; s2++) inv[s2] = -1;
@@ -835,7 +850,7 @@ int upload_inv_index_matrix(hipStream_t s) {
         const int s2 = (val - 3) / 9, k = (val - 3) % 9, j = inv[s2];
         if (j >= 0) w = 18 * (CB[j][1] + 2 * CB[j][2]) + 9 * CB[j][0] + k;
       }
-      wtab[q][val] = (short)w;
+      wtab[t][val] = (short)w;
     }
   }
   HIP_TRY(hipMemcpyToSymbolAsync(HIP_SYMBOL(c_gtab), gtab, sizeof(gtab), 0, hipMemcpyHostToDevice, s));
