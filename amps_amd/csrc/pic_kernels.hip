@@ -581,9 +581,12 @@ INST_DEP(6, false)
 
 // ======================================================================
 // ECSIM deposit, phase 2: corner flush.  One wavefront per unique node sums
-// the contributions of its incident cells in ascending cell order — the
-// order of the reference's serial cell loop — into J (3) and the 27x9 mass
-// matrix row block (IndexMatrix, pic_field_solver_ecsim.cpp:1377-1379).
+// the contributions of its incident cells in the order of its incidence row
+// into J (3) and the 27x9 mass matrix row block (IndexMatrix,
+// pic_field_solver_ecsim.cpp:1377-1379).  The exact records (MOMENT false)
+// come with rows in ascending cell order, the order of the reference's
+// serial cell loop, so the sums reproduce it bit for bit; the moment
+// records with rows in octant order (k_deposit_node_staged's).
 // ======================================================================
 // c_soff[q][val]: where output value val (J[3], then 27 slots x 9) of a node
 // that is local corner q of an incident cell sits in that cell's record
@@ -620,8 +623,8 @@ __global__ __launch_bounds__(256) void k_deposit_node(int64_t nnodes, int npw,
 #pragma unroll
   for (int t = 0; t < 8; t++) ics[t] = inc8[slot * 8 + t];
   const int64_t node = node_order[slot];
-  // all loads first, then the sums in ascending cell order (x + -0.0 == x,
-  // so absent slots change nothing)
+  // all loads first, then the sums in row order (x + -0.0 == x, so absent
+  // slots change nothing)
   double vv[8][4];
 #pragma unroll
   for (int t = 0; t < 8; t++) {
