@@ -163,6 +163,11 @@ struct amps_ctx {
   int* d_wcnt = nullptr;        // [ceil(cap / 256) * 4 + 1]
   uint64_t* d_wmask = nullptr;  // [ceil(cap / 256) * 4]
   bool layout_valid = false;
+  // the last compaction left the particle records in the other buffer: the
+  // destination's source indices are in d_tmpsrc, and the next matrix-core
+  // cell pass moves the records (only a step, whose deposit follows its
+  // compaction, sets it)
+  bool pending_gather = false;
   // exit list of the USER_FUNCTION boundary mode (exits.hip)
   int64_t exit_cap = (int64_t)1 << 20;  // records the list holds
   int64_t exit_alloc = 0;               // records allocated

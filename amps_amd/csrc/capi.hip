@@ -35,6 +35,8 @@ __global__ void k_mig_order_big(int64_t, const int*, const int*, int*, int*, int
 __global__ void k_mig_scatter(const int*, const int*, const int8_t*, const int*, const int*, const int*,
                               const uint64_t*, const int*, int64_t, const double*, const double*, int, const uint8_t*,
                               const int64_t*, double*, double*, uint8_t*, int*, int64_t*);
+__global__ void k_mig_dest(const int*, const int*, const int8_t*, const int*, const int*, const int*,
+                           const uint64_t*, const int*, int*, int*);
 __global__ void k_copy_ints(int64_t, const int*, int*);
 __global__ void k_tile_count(int, int, const int*, int*);
 __global__ void k_tile_emit(int, int, const int*, const int*, Tile*, int*);
@@ -799,7 +801,11 @@ static void clear_counts(amps_ctx* c) {
 }
 
 // counted: the push has already counted the survivors into a cleared d_offset
-static int compact_impl(amps_ctx* c, bool counted = false) {
+// defer: leave the records where they are and write only each destination's
+// source index (into d_tmpsrc, free once the movers are ordered) and key; the
+// deposit's matrix-core cell pass then moves the records as it reads them
+// (pending_gather), so a step reads the particle state once instead of twice.
+static int compact_impl(amps_ctx* c, bool counted = false, bool defer = false) {
   const int64_t nkeys = (int64_t)c->mv.nb * NCELL;
   const int64_t cap = c->cap;
   const unsigned gridp = (unsigned)((cap + 255) / 256);
@@ -827,9 +833,16 @@ static int compact_impl(amps_ctx* c, bool counted = false) {
     // spare array of the merge sort: the other buffer's keys, written later
     hipLaunchKernelGGL(k_mig_order_big, dim3(512), dim3(256), 0, c->stream, nkeys, c->d_offset, c->d_cursor,
                        c->d_tmpsrc, c->d_key[b], c->d_dest, c->d_big);
-    hipLaunchKernelGGL(k_mig_scatter, dim3(gridp), dim3(256), 0, c->stream, c->d_n, c->d_key[a], c->d_code,
-                       c->d_offset, c->d_off_layout, c->d_wcnt, c->d_wmask, c->d_dest, cap, c->d_x[a], c->d_v[a],
-                       vf32, c->d_spec[a], c->d_id[a], c->d_x[b], c->d_v[b], c->d_spec[b], c->d_key[b], c->d_id[b]);
+    if (defer) {
+      hipLaunchKernelGGL(k_mig_dest, dim3(gridp), dim3(256), 0, c->stream, c->d_n, c->d_key[a], c->d_code,
+                         c->d_offset, c->d_off_layout, c->d_wcnt, c->d_wmask, c->d_dest, c->d_tmpsrc, c->d_key[b]);
+      c->pending_gather = true;
+    } else {
+      hipLaunchKernelGGL(k_mig_scatter, dim3(gridp), dim3(256), 0, c->stream, c->d_n, c->d_key[a], c->d_code,
+                         c->d_offset, c->d_off_layout, c->d_wcnt, c->d_wmask, c->d_dest, cap, c->d_x[a], c->d_v[a],
+                         vf32, c->d_spec[a], c->d_id[a], c->d_x[b], c->d_v[b], c->d_spec[b], c->d_key[b],
+                         c->d_id[b]);
+    }
   } else {
     hipLaunchKernelGGL(k_scatter, dim3(gridp), dim3(256), 0, c->stream, c->d_n, c->d_key[a], c->d_code, c->d_offset,
                        c->d_cursor, cap, c->d_x[a], c->d_v[a], vf32, c->d_spec[a], c->d_id[a], c->d_x[b], c->d_v[b],
@@ -1122,8 +1135,11 @@ static int one_step(amps_ctx* c, const amps_push_opts* o, double dt) {
   tend(c, AMPS_K_PUSH);
   if (o->bc_mode == AMPS_BC_USER)
     if (int r = collect_exits(c)) return r;
+  // the deposit follows at once: with the matrix-core cell pass, the
+  // migration leaves the records to it
+  const bool defer = c->deterministic && c->dep_kernel == 2 && !c->dep_exact && !c->dep_paired;
   tbegin(c);
-  if (int r = compact_impl(c, true)) return r;
+  if (int r = compact_impl(c, true, defer)) return r;
   tend(c, AMPS_K_COMPACT);
   return launch_deposit(c, dt);
 }

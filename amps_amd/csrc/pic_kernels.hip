@@ -369,6 +369,42 @@ __global__ __launch_bounds__(256) void k_mig_order_big(int64_t nkeys, const int*
   }
 }
 
+// Where particle p (key k) goes: a stayer after the stayers of its key that
+// precede it (its wave's stayer mask and the scanned per-wave counts), a
+// mover where k_mig_order put it.
+__device__ __forceinline__ int64_t mig_dest(int64_t p, int k, const int* __restrict__ offset,
+                                            const int* __restrict__ lay, const int* __restrict__ wscan,
+                                            const uint64_t* __restrict__ wmask, const int* __restrict__ dest) {
+  const int lane = (int)(p & 63);
+  const uint64_t mw = wmask[p >> 6];
+  if ((mw >> lane) & 1ull) {
+    const int q = lay[k];
+    const int sp = wscan[p >> 6] + __popcll(mw & ((1ull << lane) - 1ull));
+    const int sq = wscan[q >> 6] + __popcll(wmask[q >> 6] & ((1ull << (q & 63)) - 1ull));
+    return (int64_t)offset[k] + (sp - sq);
+  }
+  return dest[p];
+}
+
+// The same destinations without moving the records: src[d] = p and the key
+// at d, for a cell pass of the deposit that gathers the particles from the
+// previous arrays, deposits them and writes them to their places.
+__global__ __launch_bounds__(256) void k_mig_dest(const int* __restrict__ n_dev, const int* __restrict__ key,
+                                                  const int8_t* __restrict__ code, const int* __restrict__ offset,
+                                                  const int* __restrict__ lay, const int* __restrict__ wscan,
+                                                  const uint64_t* __restrict__ wmask,
+                                                  const int* __restrict__ dest, int* __restrict__ src,
+                                                  int* __restrict__ kb) {
+  const int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int n = *n_dev;
+  const bool valid = (p < n) && alive_code(code[p]);
+  if (!valid) return;
+  const int k = key[p];
+  const int64_t d = mig_dest(p, k, offset, lay, wscan, wmask, dest);
+  src[d] = (int)p;
+  kb[d] = k;
+}
+
 __global__ __launch_bounds__(256) void k_mig_scatter(const int* __restrict__ n_dev, const int* __restrict__ key,
                                                       const int8_t* __restrict__ code, const int* __restrict__ offset,
                                                       const int* __restrict__ lay, const int* __restrict__ wscan,
@@ -384,17 +420,7 @@ __global__ __launch_bounds__(256) void k_mig_scatter(const int* __restrict__ n_d
   const bool valid = (p < n) && alive_code(code[p]);
   if (!valid) return;
   const int k = key[p];
-  const int lane = __lane_id();
-  const uint64_t mw = wmask[p >> 6];
-  int64_t d;
-  if ((mw >> lane) & 1ull) {
-    const int q = lay[k];
-    const int sp = wscan[p >> 6] + __popcll(mw & ((1ull << lane) - 1ull));
-    const int sq = wscan[q >> 6] + __popcll(wmask[q >> 6] & ((1ull << (q & 63)) - 1ull));
-    d = (int64_t)offset[k] + (sp - sq);
-  } else {
-    d = dest[p];
-  }
+  const int64_t d = mig_dest(p, k, offset, lay, wscan, wmask, dest);
 #pragma unroll
   for (int c = 0; c < 3; c++) {
     xb[c * cap + d] = xa[c * cap + p];

@@ -23,6 +23,10 @@ HBM_PEAK_GBPS = 8000.0
 PUSH_BYTES = 104          # per particle-step: x,v read (48) + write (48) + ids (8)
 PUSH_BYTES_F32 = 80       # FP32 velocities: x read+write (48) + v read+write (24) + ids (8)
 DEPOSIT_BYTES_P = 48      # per particle: x, v read
+# the matrix-core cell pass of a step also makes the migration's record move:
+# per particle it reads x, v (48), species (1), id (8) and the source index (4)
+# once, and writes x, v, species and id (57)
+FUSED_BYTES_P = 61 + 57
 NODE_BYTES = 246 * 8      # per unique node: J (3) + mass matrix (243) written once
 ECSIM_NODE_BYTES = 243 * 8 + 27 * 4 + 24 + 24  # mass-term: M row, neighbour ids, own E once, result
 
@@ -32,14 +36,15 @@ COPY_PROBE_BYTES = 1 << 30
 PMC_KERNELS = (("k_push_f64", "push"), ("k_push_f32", "push"), ("k_deposit_cell", "deposit_cell"),
                ("k_deposit_cell_rec", "deposit_cell"), ("k_deposit_cell_mfma", "deposit_cell"),
                ("k_deposit_node", "deposit_node"), ("k_deposit_node_staged", "deposit_node"),
-               ("k_scatter", "scatter"), ("k_mig_scatter", "scatter"), ("k_mig_classify", "mig_classify"),
+               ("k_scatter", "scatter"), ("k_mig_scatter", "scatter"), ("k_mig_dest", "mig_dest"),
+               ("k_mig_classify", "mig_classify"),
                ("k_mig_order", "mig_order"), ("k_copy_probe", "copy"),
                ("k_ecsim_mass", "ecsim_mass"), ("k_ecsim_op", "ecsim_op"),
                ("k_deposit_cell_lanes", "deposit_cell"))
 # kernels of the particle step: counted only after the copy probe, which the
 # counter child launches between the ECSIM set-up (a deposit of its own) and
 # the step
-STEP_KEYS = ("push", "deposit_cell", "deposit_node", "scatter", "mig_classify", "mig_order")
+STEP_KEYS = ("push", "deposit_cell", "deposit_node", "scatter", "mig_dest", "mig_classify", "mig_order")
 
 
 def _pmc_pass(args, tune, counter, outdir):
@@ -581,7 +586,11 @@ def main():
                                                   ("deposit_node", L.K_DEPOSIT_NODE)]}
     nb, nnodes = mesh.nblocks, st.nnodes
     push_bytes = n * PUSH_BYTES + nb * 729 * 6 * 8
-    dep_bytes = n * DEPOSIT_BYTES_P + nnodes * NODE_BYTES + nb * 729 * 3 * 8
+    # the deposit moves the migrated records itself with the deterministic
+    # migration and the single-phase matrix-core cell pass
+    dep_kernel_used = tune["deposit_kernel"] if tune else (args.deposit_kernel if args.deposit_kernel >= 0 else 2)
+    fused = args.deterministic == 1 and dep_kernel_used == 2 and args.deposit_paired != 1
+    dep_bytes = n * (FUSED_BYTES_P if fused else DEPOSIT_BYTES_P) + nnodes * NODE_BYTES + nb * 729 * 3 * 8
     push_gbps = push_bytes / (avg["push"] * 1e-3) / 1e9
     dep_ms = avg["deposit_cell"] + avg["deposit_node"]
     dep_gbps = dep_bytes / (dep_ms * 1e-3) / 1e9
@@ -605,7 +614,9 @@ def main():
         tr = None
         if traffic and "deposit_cell" in traffic and "deposit_node" in traffic:
             tr = traffic["deposit_cell"] + traffic["deposit_node"]
-        roof = {"kernel": "ECSIM deposit (deposit cell pass + node gather, one launch each)", "bound": "hbm",
+        roof = {"kernel": ("ECSIM deposit with the migration's record move (fused cell pass + node gather, "
+                           "one launch each)" if fused else
+                           "ECSIM deposit (deposit cell pass + node gather, one launch each)"), "bound": "hbm",
                 "achieved": round(dep_gbps, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                 "frac": round(dep_gbps / HBM_PEAK_GBPS, 4), "traffic": tr,
                 "algorithmic_bytes": int(dep_bytes)}

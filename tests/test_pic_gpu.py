@@ -493,6 +493,55 @@ def test_generator_and_graph_steps_match_eager(gpu_ctx):
     assert np.array_equal(Ja, Jb) and np.array_equal(Ma, Mb)
 
 
+@pytest.mark.parametrize("precision", [L.FP64, L.FP32])
+def test_step_deposit_moves_the_migrated_records(gpu_ctx, precision):
+    """In a step the deterministic migration only works out where each
+    particle goes and the matrix-core cell pass moves the records as it
+    deposits them.  The arrays (in array order) and the resident J and M
+    after each step equal those of separate MoveParticles(compact) and
+    UpdateJMassMatrix calls, whose migration moves the records itself; J and
+    M are compared through the field equation's mass term and right-hand
+    side, which read them on the device."""
+    mesh = M.uniform_mesh(1)
+    E, B = M.corner_fields(mesh)
+    # thermal speeds of 0.3 and steps of 0.05: a particle moves ~0.015 per
+    # step, an eighth of a cell, so a good share change cells every step
+    dt = 0.05
+    sp = Species([-1.0, 1.0], [0.01, 1.0], [dt, dt])
+    rng = np.random.default_rng(3)
+    res = {}
+    for fused in (True, False):  # one after the other: the states share the device context
+        st = PicState(mesh, E, B, sp, ctx=gpu_ctx)
+        st.generate_particles(mesh.ncells * 3 + 777, [0.3, 0.3], drift=(0.01, 0.0, 0.0), seed=5,
+                              precision=precision)
+        if fused:
+            En = rng.normal(size=(st.nnodes, 3))
+            Bc = rng.normal(size=(mesh.nblocks * 512, 3))
+            prm = st.ecsim_params(dt)
+        out = []
+        for step in range(3):
+            if fused:
+                st.run_steps(1, dt, r_sphere=0.2, use_graph=False)
+            else:
+                st.MoveParticles(r_sphere=0.2, compact=True)
+                st.UpdateJMassMatrix(dt, copy_out=False)
+            y = st.MassMatrixApply(En, -4 * np.pi * dt * 0.5)
+            r = st.EcsimRhs(prm, En, Bc)
+            out.append((st.get_particles()[0], y, r))
+        res[fused] = out
+    prev = None
+    for step, ((a, ya, ra), (b, yb, rb)) in enumerate(zip(res[True], res[False])):
+        key = dict(zip(a.id.tolist(), (a.blk.astype(np.int64) * 512 + a.cell).tolist()))
+        moved = 0 if prev is None else sum(prev.get(i, -1) != k for i, k in key.items())
+        prev = key
+        print(f"step {step}: {a.n} particles, {moved} changed cells", flush=True)
+        assert a.n == b.n and a.n > 0 and (step == 0 or moved > a.n // 10)
+        assert np.array_equal(a.id, b.id)
+        assert np.array_equal(a.x, b.x) and np.array_equal(a.v, b.v) and np.array_equal(a.spec, b.spec)
+        assert np.array_equal(a.blk, b.blk) and np.array_equal(a.cell, b.cell)
+        assert np.array_equal(ya, yb) and np.array_equal(ra, rb)
+
+
 @pytest.mark.parametrize("mover", [L.MOVER_LAPENTA, L.MOVER_BORIS])
 def test_push_fp32_error_against_fp64_oracle(gpu_ctx, mover):
     """FP32 push (FP32 fields and velocity, FP64 positions): state the error of
