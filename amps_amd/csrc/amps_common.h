@@ -60,6 +60,21 @@ struct Faces {
   Face f[6];
 };
 
+// Where a push reads its particles.  src null: in place.  Otherwise the
+// particles are still where the last compaction found them: particle p of the
+// new order is read at src[p] from x, v, spec, id (the other buffer), and its
+// species and id are written to spec_out, id_out at p with its new state (the
+// push makes the migration's record move).
+struct PushSrc {
+  const int* src;
+  const double* x;
+  const void* v;
+  const uint8_t* spec;
+  const int64_t* id;
+  uint8_t* spec_out;
+  int64_t* id_out;
+};
+
 // Synthetic-population generator arguments (k_generate).
 struct GenArgs {
   int64_t ntotal, base, rem;
@@ -164,9 +179,10 @@ struct amps_ctx {
   uint64_t* d_wmask = nullptr;  // [ceil(cap / 256) * 4]
   bool layout_valid = false;
   // the last compaction left the particle records in the other buffer: the
-  // destination's source indices are in d_tmpsrc, and the next matrix-core
-  // cell pass moves the records (only a step, whose deposit follows its
-  // compaction, sets it)
+  // destination's source indices are in d_tmpsrc.  The step's matrix-core
+  // cell pass reads the particles through them, the next push moves them;
+  // amps_gpu_run_steps, the only caller that sets it, moves any left over
+  // before it returns.
   bool pending_gather = false;
   // exit list of the USER_FUNCTION boundary mode (exits.hip)
   int64_t exit_cap = (int64_t)1 << 20;  // records the list holds

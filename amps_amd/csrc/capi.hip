@@ -17,11 +17,11 @@ using namespace amps;
 template <int MOVER, bool BC, int W>
 __global__ void k_push_f64(MeshView, Faces, Species, const double*, const double*, const Tile*,
                            const int*, int, double, int64_t, double*, double*, const uint8_t*,
-                           int*, int8_t*, const double*, int*);
+                           int*, int8_t*, const double*, int*, PushSrc);
 template <int MOVER, int W>
 __global__ void k_push_f32(MeshView, Faces, Species, const double*, const double*, const Tile*,
                            const int*, int, double, int64_t, double*, float*, const uint8_t*, int*,
-                           int8_t*, int*);
+                           int8_t*, int*, PushSrc);
 __global__ void k_fill_int(int*, int64_t, int);
 __global__ void k_count_keys(const int*, const int*, const int8_t*, int*);
 __global__ void k_scatter(const int*, const int*, const int8_t*, const int*, int*, int64_t,
@@ -37,6 +37,8 @@ __global__ void k_mig_scatter(const int*, const int*, const int8_t*, const int*,
                               const int64_t*, double*, double*, uint8_t*, int*, int64_t*);
 __global__ void k_mig_dest(const int*, const int*, const int8_t*, const int*, const int*, const int*,
                            const uint64_t*, const int*, int*, int*);
+__global__ void k_mig_move(const int*, const int*, int64_t, const double*, const double*, int, const uint8_t*,
+                           const int64_t*, double*, double*, uint8_t*, int64_t*);
 __global__ void k_copy_ints(int64_t, const int*, int*);
 __global__ void k_tile_count(int, int, const int*, int*);
 __global__ void k_tile_emit(int, int, const int*, const int*, Tile*, int*);
@@ -802,9 +804,10 @@ static void clear_counts(amps_ctx* c) {
 
 // counted: the push has already counted the survivors into a cleared d_offset
 // defer: leave the records where they are and write only each destination's
-// source index (into d_tmpsrc, free once the movers are ordered) and key; the
-// deposit's matrix-core cell pass then moves the records as it reads them
-// (pending_gather), so a step reads the particle state once instead of twice.
+// source index (into d_tmpsrc, free once the movers are ordered) and key
+// (pending_gather): the step's matrix-core cell pass reads the particles
+// through them and the next push moves them as it updates them, so a step
+// writes the particle state once instead of twice.
 static int compact_impl(amps_ctx* c, bool counted = false, bool defer = false) {
   const int64_t nkeys = (int64_t)c->mv.nb * NCELL;
   const int64_t cap = c->cap;
@@ -894,10 +897,18 @@ static int check_mover(const amps_ctx* c, const amps_push_opts* o, const char* w
 static int launch_push(amps_ctx* c, const amps_push_opts* o, int* count = nullptr) {
   const int k = c->cur;
   const bool w4 = c->push_waves == 4;
+  // records the last compaction left in the other buffer: the push reads them
+  // there and writes them to their places
+  PushSrc ps = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+  if (c->pending_gather) {
+    const int r = 1 - k;
+    ps = {c->d_tmpsrc, c->d_x[r], c->d_v[r], c->d_spec[r], c->d_id[r], c->d_spec[k], c->d_id[k]};
+    c->pending_gather = false;
+  }
 #define LAUNCH_PUSH32(MV, W)                                                                              \
   hipLaunchKernelGGL((k_push_f32<MV, W>), dim3((unsigned)c->max_tiles), dim3(256), 0, c->stream, c->mv,     \
                      c->faces, c->sp, c->d_E, c->d_B, c->d_tiles, c->d_ntiles, o->bc_mode, o->r_sphere, c->cap, \
-                     c->d_x[k], (float*)c->d_v[k], c->d_spec[k], c->d_key[k], c->d_code, count)
+                     c->d_x[k], (float*)c->d_v[k], c->d_spec[k], c->d_key[k], c->d_code, count, ps)
   if (c->precision == AMPS_FP32) {
     if (o->mover == AMPS_MOVER_LAPENTA) {
       if (w4) LAUNCH_PUSH32(AMPS_MOVER_LAPENTA, 4);
@@ -913,7 +924,7 @@ static int launch_push(amps_ctx* c, const amps_push_opts* o, int* count = nullpt
 #define LAUNCH_PUSH64(MV, BCV, W)                                                                       \
   hipLaunchKernelGGL((k_push_f64<MV, BCV, W>), dim3((unsigned)c->max_tiles), dim3(256), 0, c->stream, c->mv,  \
                      c->faces, c->sp, c->d_E, c->d_B, c->d_tiles, c->d_ntiles, o->bc_mode, o->r_sphere, c->cap, \
-                     c->d_x[k], c->d_v[k], c->d_spec[k], c->d_key[k], c->d_code, c->d_Bc, count)
+                     c->d_x[k], c->d_v[k], c->d_spec[k], c->d_key[k], c->d_code, c->d_Bc, count, ps)
   const bool bc = c->b_center != 0;
   if (o->mover == AMPS_MOVER_LAPENTA) {
     if (bc) LAUNCH_PUSH64(AMPS_MOVER_LAPENTA, true, 1);
@@ -1127,6 +1138,18 @@ extern "C" int amps_gpu_generate_particles(amps_ctx* c, int64_t n_per_species, i
   return AMPS_OK;
 }
 
+// the record move a step's compaction left to the next push
+static int settle_records(amps_ctx* c) {
+  if (!c->pending_gather) return 0;
+  const int k = c->cur, r = 1 - k;
+  hipLaunchKernelGGL(k_mig_move, dim3((unsigned)((c->cap + 255) / 256)), dim3(256), 0, c->stream, c->d_n,
+                     c->d_tmpsrc, c->cap, c->d_x[r], c->d_v[r], c->precision == AMPS_FP32 ? 1 : 0, c->d_spec[r],
+                     c->d_id[r], c->d_x[k], c->d_v[k], c->d_spec[k], c->d_id[k]);
+  c->pending_gather = false;
+  HIP_TRY(hipGetLastError());
+  return 0;
+}
+
 static int one_step(amps_ctx* c, const amps_push_opts* o, double dt) {
   // the push counts the survivors per key for the migration that follows
   clear_counts(c);
@@ -1136,7 +1159,8 @@ static int one_step(amps_ctx* c, const amps_push_opts* o, double dt) {
   if (o->bc_mode == AMPS_BC_USER)
     if (int r = collect_exits(c)) return r;
   // the deposit follows at once: with the matrix-core cell pass, the
-  // migration leaves the records to it
+  // migration leaves the records where they are (the cell pass reads them
+  // there, the next push moves them)
   const bool defer = c->deterministic && c->dep_kernel == 2 && !c->dep_exact && !c->dep_paired;
   tbegin(c);
   if (int r = compact_impl(c, true, defer)) return r;
@@ -1144,9 +1168,7 @@ static int one_step(amps_ctx* c, const amps_push_opts* o, double dt) {
   return launch_deposit(c, dt);
 }
 
-extern "C" int amps_gpu_run_steps(amps_ctx* c, const amps_push_opts* o, double dt, int64_t nsteps,
-                                  int32_t use_graph) {
-  if (!c || !o) return fail(AMPS_E_ARG, "amps_gpu_run_steps: null argument");
+static int run_steps_impl(amps_ctx* c, const amps_push_opts* o, double dt, int64_t nsteps, int32_t use_graph) {
   if (!c->has_mesh || !c->has_fields || !c->has_species || c->cap == 0)
     return fail(AMPS_E_STATE, "amps_gpu_run_steps: mesh, fields, species and particles are required");
   if (int r = check_mover(c, o, "amps_gpu_run_steps")) return r;
@@ -1192,6 +1214,15 @@ extern "C" int amps_gpu_run_steps(amps_ctx* c, const amps_push_opts* o, double d
   c->offsets_valid = true;
   c->tiles_valid = true;
   return AMPS_OK;
+}
+
+extern "C" int amps_gpu_run_steps(amps_ctx* c, const amps_push_opts* o, double dt, int64_t nsteps,
+                                  int32_t use_graph) {
+  if (!c || !o) return fail(AMPS_E_ARG, "amps_gpu_run_steps: null argument");
+  const int r = run_steps_impl(c, o, dt, nsteps, use_graph);
+  // the last step's records to their places, on every path out
+  const int r2 = settle_records(c);
+  return r ? r : r2;
 }
 
 extern "C" int amps_gpu_copy_probe(amps_ctx* c, int64_t nbytes, int32_t reps, double* ms) {

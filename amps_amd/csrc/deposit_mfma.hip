@@ -65,18 +65,6 @@ __device__ __forceinline__ int corner_of(int bx, int by, int bz) { return 4 * bz
 template <bool PAIRED>
 constexpr int waves_of() { return PAIRED ? 8 : 4; }
 
-// The migration's record moves, done by the cell pass (FUSED): particle p of
-// the new order is particle src[p] of the arrays x, v, spec, ids, and is
-// written to xd, vd, sd, idd at p as its record is built.
-struct Gather {
-  const int* src;
-  const int64_t* ids;
-  double* xd;
-  void* vd;
-  uint8_t* sd;
-  int64_t* idd;
-};
-
 template <bool BC, bool PAIRED, bool FUSED>
 __global__ __launch_bounds__(64 * waves_of<PAIRED>()) void k_deposit_cell_mfma(MeshView m, Species sp, double dtTotal,
                                                                  const double* __restrict__ B,
@@ -87,7 +75,8 @@ __global__ __launch_bounds__(64 * waves_of<PAIRED>()) void k_deposit_cell_mfma(M
                                                                  int vf32,
                                                                  const uint8_t* __restrict__ spec,
                                                                  double* __restrict__ cellpart,
-                                                                 const double* __restrict__ Bc, Gather gt) {
+                                                                 const double* __restrict__ Bc,
+                                                                 const int* __restrict__ src) {
   constexpr int MW = waves_of<PAIRED>();
   constexpr int NCW = PAIRED ? 8 : 16;  // cells per wavefront
   __shared__ double sB[125 * 3];
@@ -200,8 +189,7 @@ __global__ __launch_bounds__(64 * waves_of<PAIRED>()) void k_deposit_cell_mfma(M
   // each lane's particles of the next two chunks, loaded ahead
   struct Pre {
     double x[3], v[3];
-    int s, c, p;
-    int64_t id;
+    int s, c;
   };
   // lane k < NCW: where cell k starts in the concatenation, and the offset
   // from a position in the concatenation to the particle's index
@@ -221,8 +209,9 @@ __global__ __launch_bounds__(64 * waves_of<PAIRED>()) void k_deposit_cell_mfma(M
     const int p = pos + __shfl(dk, c, 64);
     if (pos < total) {
       n.c = c;
-      n.p = p;
-      const int q = FUSED ? gt.src[p] : p;  // where the particle is read
+      // FUSED: the particles are still where the last compaction found
+      // them, particle p of the new order at src[p]
+      const int q = FUSED ? src[p] : p;
       n.x[0] = x[q], n.x[1] = x[cap + q], n.x[2] = x[2 * cap + q];
       if (vf32) {
         const float* vf = (const float*)v;
@@ -231,10 +220,9 @@ __global__ __launch_bounds__(64 * waves_of<PAIRED>()) void k_deposit_cell_mfma(M
         n.v[0] = v[q], n.v[1] = v[cap + q], n.v[2] = v[2 * cap + q];
       }
       n.s = spec[q];
-      if constexpr (FUSED) n.id = gt.ids[q];
     }
   };
-  Pre n1 = {{0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}, 0, 0, 0, 0}, n2 = n1;
+  Pre n1 = {{0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}, 0, 0}, n2 = n1;
 
   // the two 16 x 16 halves of the cell's 16 x 32 tile
   dbl4 C0 = {0.0, 0.0, 0.0, 0.0}, C1 = C0;
@@ -273,20 +261,6 @@ __global__ __launch_bounds__(64 * waves_of<PAIRED>()) void k_deposit_cell_mfma(M
       const double vI[3] = {nb.v[0], nb.v[1], nb.v[2]};
       const int s = nb.s;
       const int nc = nb.c;
-      if constexpr (FUSED) {
-        // the particle's record at its place in the new order
-        const int64_t p = nb.p;
-#pragma unroll
-        for (int d = 0; d < 3; d++) {
-          gt.xd[d * cap + p] = xI[d];
-          if (vf32)
-            ((float*)gt.vd)[d * cap + p] = (float)vI[d];
-          else
-            ((double*)gt.vd)[d * cap + p] = vI[d];
-        }
-        gt.sd[p] = (uint8_t)s;
-        gt.idd[p] = nb.id;
-      }
       const int iX[3] = {sx + (nc & 3), sy + row0 + (nc >> 2), zc};
       double Lf[6], xn[3];
 #pragma unroll
@@ -487,13 +461,12 @@ template <bool BC, bool PAIRED, bool FUSED>
 void launch(amps_ctx* c, double dt) {
   const int k = c->cur;
   // FUSED: the particles are read from the previous buffer through the
-  // migration's source indices and written to the current one
+  // migration's source indices
   const int r = FUSED ? 1 - k : k;
-  const Gather gt = {c->d_tmpsrc, c->d_id[r], c->d_x[k], c->d_v[k], c->d_spec[k], c->d_id[k]};
   hipLaunchKernelGGL((k_deposit_cell_mfma<BC, PAIRED, FUSED>), dim3((unsigned)c->mv.nb * 8),
                      dim3(64 * waves_of<PAIRED>()), 0, c->stream, c->mv, c->sp, dt, c->d_B, c->d_offset, c->cap,
                      c->d_x[r], c->d_v[r], c->precision == AMPS_FP32 ? 1 : 0, c->d_spec[r], c->d_cellpart, c->d_Bc,
-                     gt);
+                     c->d_tmpsrc);
 }
 
 }  // namespace
@@ -502,10 +475,9 @@ namespace amps {
 int launch_deposit_cell_mfma(amps_ctx* c, double dt) {
   const bool paired = c->dep_paired != 0;
   if (c->pending_gather) {
-    // the records of the last compaction are still to be moved: the fused
-    // pass (set only with the single-wavefront-phase pass)
+    // the records of the last compaction are still where it found them: the
+    // pass reads them there (set only with the single-phase pass)
     if (paired) return 1;
-    c->pending_gather = false;
     if (c->b_center)
       launch<true, false, true>(c, dt);
     else

@@ -57,7 +57,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W))) void k
                                                    double r_sphere, int64_t cap, double* x,
                                                    double* v, const uint8_t* __restrict__ spec,
                                                    int* key, int8_t* code, const double* __restrict__ Bc,
-                                                   int* __restrict__ count) {
+                                                   int* __restrict__ count, PushSrc ps) {
   __shared__ double sE[NCORNER * 3];
   __shared__ double sB[NCORNER * 3];
   const int t = blockIdx.x;
@@ -77,15 +77,27 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W))) void k
   load_block_lat(m, b, lat);
   const int p1 = tl.start + tl.count;
   // every lane runs the same number of passes, so that the wavefront's run
-  // aggregation of the migration count sees all 64 lanes
+  // aggregation of the migration count sees all 64 lanes.  Where the next
+  // pass reads (the last compaction's source index) is loaded a pass ahead.
+  int qn = (ps.src && tl.start + (int)threadIdx.x < p1) ? ps.src[tl.start + threadIdx.x] : 0;
   for (int p0 = tl.start; p0 < p1; p0 += blockDim.x) {
     const int p = p0 + threadIdx.x;
     bool alive = false;
     int knew = -1;
+    const int qs = qn;
+    if (ps.src && p + (int)blockDim.x < p1) qn = ps.src[p + blockDim.x];
     if (p < p1) {
-    double xp[3] = {x[p], x[cap + p], x[2 * cap + p]};
-    double vp[3] = {v[p], v[cap + p], v[2 * cap + p]};
-    const int s = spec[p];
+    // read in place, or where the last compaction left the particle
+    const int q = ps.src ? qs : p;
+    const double* xr = ps.src ? ps.x : x;
+    const double* vr = ps.src ? (const double*)ps.v : v;
+    double xp[3] = {xr[q], xr[cap + q], xr[2 * cap + q]};
+    double vp[3] = {vr[q], vr[cap + q], vr[2 * cap + q]};
+    const int s = ps.src ? ps.spec[q] : spec[p];
+    if (ps.src) {
+      ps.spec_out[p] = (uint8_t)s;
+      ps.id_out[p] = ps.id[q];
+    }
     int nb_out = b, cell_out = 0;
     int c;
     if constexpr (MOVER == AMPS_MOVER_RELATIVISTIC)
@@ -118,7 +130,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W))) void k
 #define INST_PUSH64(MV, BCV, W)                                                                  \
   template __global__ void k_push_f64<MV, BCV, W>(MeshView, Faces, Species, const double*, const double*, \
                                                   const Tile*, const int*, int, double, int64_t, double*, \
-                                                  double*, const uint8_t*, int*, int8_t*, const double*, int*);
+                                                  double*, const uint8_t*, int*, int8_t*, const double*, int*, \
+                                                  PushSrc);
 INST_PUSH64(AMPS_MOVER_LAPENTA, false, 1)
 INST_PUSH64(AMPS_MOVER_BORIS, false, 1)
 INST_PUSH64(AMPS_MOVER_LAPENTA, false, 4)
@@ -137,7 +150,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W))) void k
                                                    const int* __restrict__ ntiles, int bc_mode,
                                                    double r_sphere, int64_t cap, double* x,
                                                    float* v, const uint8_t* __restrict__ spec,
-                                                   int* key, int8_t* code, int* __restrict__ count) {
+                                                   int* key, int8_t* code, int* __restrict__ count, PushSrc ps) {
   __shared__ float sE[NCORNER * 3];
   __shared__ float sB[NCORNER * 3];
   const int t = blockIdx.x;
@@ -156,14 +169,24 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W))) void k
   BlockLat lat;
   load_block_lat(m, b, lat);
   const int p1 = tl.start + tl.count;
+  int qn = (ps.src && tl.start + (int)threadIdx.x < p1) ? ps.src[tl.start + threadIdx.x] : 0;
   for (int p0 = tl.start; p0 < p1; p0 += blockDim.x) {
     const int p = p0 + threadIdx.x;
     bool alive = false;
     int knew = -1;
+    const int qs = qn;
+    if (ps.src && p + (int)blockDim.x < p1) qn = ps.src[p + blockDim.x];
     if (p < p1) {
-    double xp[3] = {x[p], x[cap + p], x[2 * cap + p]};
-    float vp[3] = {v[p], v[cap + p], v[2 * cap + p]};
-    const int s = spec[p];
+    const int q = ps.src ? qs : p;
+    const double* xr = ps.src ? ps.x : x;
+    const float* vr = ps.src ? (const float*)ps.v : v;
+    double xp[3] = {xr[q], xr[cap + q], xr[2 * cap + q]};
+    float vp[3] = {vr[q], vr[cap + q], vr[2 * cap + q]};
+    const int s = ps.src ? ps.spec[q] : spec[p];
+    if (ps.src) {
+      ps.spec_out[p] = (uint8_t)s;
+      ps.id_out[p] = ps.id[q];
+    }
     int nb_out = b, cell_out = 0;
     const int c = push_particle_f32<MOVER>(m, faces, g, lat, sE, sB, (float)sp.charge[s], (float)sp.mass[s],
                                            sp.dt[s], bc_mode, r_sphere, xp, vp, &nb_out, &cell_out);
@@ -189,7 +212,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W))) void k
 #define INST_PUSH32(MV, W)                                                                        \
   template __global__ void k_push_f32<MV, W>(MeshView, Faces, Species, const double*, const double*, \
                                              const Tile*, const int*, int, double, int64_t, double*,  \
-                                             float*, const uint8_t*, int*, int8_t*, int*);
+                                             float*, const uint8_t*, int*, int8_t*, int*, PushSrc);
 INST_PUSH32(AMPS_MOVER_LAPENTA, 1)
 INST_PUSH32(AMPS_MOVER_BORIS, 1)
 INST_PUSH32(AMPS_MOVER_LAPENTA, 4)
@@ -403,6 +426,29 @@ __global__ __launch_bounds__(256) void k_mig_dest(const int* __restrict__ n_dev,
   const int64_t d = mig_dest(p, k, offset, lay, wscan, wmask, dest);
   src[d] = (int)p;
   kb[d] = k;
+}
+
+// The record move that k_mig_dest left: particle d of the new order from
+// src[d] of the previous arrays (the key is in place already).
+__global__ __launch_bounds__(256) void k_mig_move(const int* __restrict__ n_dev, const int* __restrict__ src,
+                                                  int64_t cap, const double* __restrict__ xa,
+                                                  const double* __restrict__ va, int vf32,
+                                                  const uint8_t* __restrict__ sa, const int64_t* __restrict__ ia,
+                                                  double* __restrict__ xb, double* __restrict__ vb,
+                                                  uint8_t* __restrict__ sb, int64_t* __restrict__ ib) {
+  const int64_t d = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (d >= *n_dev) return;
+  const int64_t p = src[d];
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    xb[c * cap + d] = xa[c * cap + p];
+    if (vf32)
+      ((float*)vb)[c * cap + d] = ((const float*)va)[c * cap + p];
+    else
+      vb[c * cap + d] = va[c * cap + p];
+  }
+  sb[d] = sa[p];
+  ib[d] = ia[p];
 }
 
 __global__ __launch_bounds__(256) void k_mig_scatter(const int* __restrict__ n_dev, const int* __restrict__ key,

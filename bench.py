@@ -23,10 +23,11 @@ HBM_PEAK_GBPS = 8000.0
 PUSH_BYTES = 104          # per particle-step: x,v read (48) + write (48) + ids (8)
 PUSH_BYTES_F32 = 80       # FP32 velocities: x read+write (48) + v read+write (24) + ids (8)
 DEPOSIT_BYTES_P = 48      # per particle: x, v read
-# the matrix-core cell pass of a step also makes the migration's record move:
-# per particle it reads x, v (48), species (1), id (8) and the source index (4)
-# once, and writes x, v, species and id (57)
-FUSED_BYTES_P = 61 + 57
+# a step's push also makes the previous migration's record move: per particle
+# it reads the source index (4), x, v (48), species (1) and id (8), and writes
+# x, v (48), species (1), id (8), key (4) and code (1)
+PUSH_MOVE_BYTES = 61 + 62
+PUSH_MOVE_BYTES_F32 = 49 + 50   # the same with FP32 velocities
 NODE_BYTES = 246 * 8      # per unique node: J (3) + mass matrix (243) written once
 ECSIM_NODE_BYTES = 243 * 8 + 27 * 4 + 24 + 24  # mass-term: M row, neighbour ids, own E once, result
 
@@ -37,6 +38,7 @@ PMC_KERNELS = (("k_push_f64", "push"), ("k_push_f32", "push"), ("k_deposit_cell"
                ("k_deposit_cell_rec", "deposit_cell"), ("k_deposit_cell_mfma", "deposit_cell"),
                ("k_deposit_node", "deposit_node"), ("k_deposit_node_staged", "deposit_node"),
                ("k_scatter", "scatter"), ("k_mig_scatter", "scatter"), ("k_mig_dest", "mig_dest"),
+               ("k_mig_move", "mig_move"),
                ("k_mig_classify", "mig_classify"),
                ("k_mig_order", "mig_order"), ("k_copy_probe", "copy"),
                ("k_ecsim_mass", "ecsim_mass"), ("k_ecsim_op", "ecsim_op"),
@@ -44,12 +46,14 @@ PMC_KERNELS = (("k_push_f64", "push"), ("k_push_f32", "push"), ("k_deposit_cell"
 # kernels of the particle step: counted only after the copy probe, which the
 # counter child launches between the ECSIM set-up (a deposit of its own) and
 # the step
-STEP_KEYS = ("push", "deposit_cell", "deposit_node", "scatter", "mig_dest", "mig_classify", "mig_order")
+STEP_KEYS = ("push", "deposit_cell", "deposit_node", "scatter", "mig_dest", "mig_move", "mig_classify",
+             "mig_order")
 
 
 def _pmc_pass(args, tune, counter, outdir):
-    """One rocprofv3 counter pass over one eager step of the same workload
-    (a child process; the counter in KiB per dispatch, averaged per kernel)."""
+    """One rocprofv3 counter pass over two eager steps of the same workload
+    (a child process; the counter in KiB per dispatch: the second step's
+    dispatch for the step's kernels, the average for the others)."""
     import csv
     import glob
     import subprocess
@@ -80,13 +84,19 @@ def _pmc_pass(args, tune, counter, outdir):
                         rows.append((int(row["Start_Timestamp"]), key, float(row["Counter_Value"])))
                         break
     marker = max((t for t, k, _ in rows if k == "copy"), default=0)
-    sums = {}
-    for t, key, val in rows:
-        if key in STEP_KEYS and t < marker:
+    sums, last = {}, {}
+    for t, key, val in sorted(rows):
+        if key in STEP_KEYS:
+            # the step kernels of the second step (its push moves the records
+            # the first step's migration left)
+            if t >= marker:
+                last[key] = val
             continue
         tot, cnt = sums.get(key, (0.0, 0))
         sums[key] = (tot + val, cnt + 1)
-    return {k: tot / cnt for k, (tot, cnt) in sums.items()}
+    out = {k: tot / cnt for k, (tot, cnt) in sums.items()}
+    out.update(last)
+    return out
 
 
 def ecsim_mass_setup():
@@ -548,7 +558,7 @@ def main():
     if args.pmc_child:
         # the ECSIM kernels on their own mesh, the copy kernel of known size
         # (calibration, and the marker after which the step's kernels count),
-        # then one eager step, for the counter passes
+        # then two eager steps, for the counter passes
         c2, st2, En2, dt2 = ecsim_mass_setup()
         st2.MassMatrixApply(En2, -4 * np.pi * dt2 * 0.5)
         st2.EcsimOperator(st2.ecsim_params(dt2, cDt=2 * float(st2.mesh.cell_size()[0][0])), En2)
@@ -556,7 +566,7 @@ def main():
         c2.close()
         ctx.copy_probe(COPY_PROBE_BYTES, reps=1)
         ctx.sync()
-        st.run_steps(1, dt, use_graph=False, **opts)
+        st.run_steps(2, dt, use_graph=False, **opts)
         ctx.sync()
         ctx.close()
         return
@@ -585,12 +595,13 @@ def main():
                                                   ("deposit_cell", L.K_DEPOSIT_CELL),
                                                   ("deposit_node", L.K_DEPOSIT_NODE)]}
     nb, nnodes = mesh.nblocks, st.nnodes
-    push_bytes = n * PUSH_BYTES + nb * 729 * 6 * 8
-    # the deposit moves the migrated records itself with the deterministic
-    # migration and the single-phase matrix-core cell pass
+    # with the deterministic migration and the single-phase matrix-core cell
+    # pass the push moves the migrated records (the cell pass reads them where
+    # they were): 3 of the 4 timed pushes, all of the hipGraph steps'
     dep_kernel_used = tune["deposit_kernel"] if tune else (args.deposit_kernel if args.deposit_kernel >= 0 else 2)
     fused = args.deterministic == 1 and dep_kernel_used == 2 and args.deposit_paired != 1
-    dep_bytes = n * (FUSED_BYTES_P if fused else DEPOSIT_BYTES_P) + nnodes * NODE_BYTES + nb * 729 * 3 * 8
+    push_bytes = n * (PUSH_MOVE_BYTES if fused else PUSH_BYTES) + nb * 729 * 6 * 8
+    dep_bytes = n * DEPOSIT_BYTES_P + nnodes * NODE_BYTES + nb * 729 * 3 * 8
     push_gbps = push_bytes / (avg["push"] * 1e-3) / 1e9
     dep_ms = avg["deposit_cell"] + avg["deposit_node"]
     dep_gbps = dep_bytes / (dep_ms * 1e-3) / 1e9
@@ -606,7 +617,8 @@ def main():
             tune_used = tune
         traffic = pmc_traffic(args, tune_used)
     if avg["push"] >= dep_ms:
-        roof = {"kernel": "k_push_f64<Lapenta2017>", "bound": "hbm", "achieved": round(push_gbps, 1),
+        roof = {"kernel": ("k_push_f64<Lapenta2017> with the migration's record move" if fused else
+                           "k_push_f64<Lapenta2017>"), "bound": "hbm", "achieved": round(push_gbps, 1),
                 "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": round(push_gbps / HBM_PEAK_GBPS, 4),
                 "traffic": traffic["push"] if traffic and "push" in traffic else None,
                 "algorithmic_bytes": int(push_bytes)}
@@ -614,9 +626,7 @@ def main():
         tr = None
         if traffic and "deposit_cell" in traffic and "deposit_node" in traffic:
             tr = traffic["deposit_cell"] + traffic["deposit_node"]
-        roof = {"kernel": ("ECSIM deposit with the migration's record move (fused cell pass + node gather, "
-                           "one launch each)" if fused else
-                           "ECSIM deposit (deposit cell pass + node gather, one launch each)"), "bound": "hbm",
+        roof = {"kernel": "ECSIM deposit (deposit cell pass + node gather, one launch each)", "bound": "hbm",
                 "achieved": round(dep_gbps, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                 "frac": round(dep_gbps / HBM_PEAK_GBPS, 4), "traffic": tr,
                 "algorithmic_bytes": int(dep_bytes)}
@@ -662,7 +672,7 @@ def main():
     avg32 = {k: ms32[i] / max(cnt32[i], 1) for k, i in [("push", L.K_PUSH), ("compact", L.K_COMPACT),
                                                         ("deposit_cell", L.K_DEPOSIT_CELL),
                                                         ("deposit_node", L.K_DEPOSIT_NODE)]}
-    push32_bytes = n * PUSH_BYTES_F32 + nb * 729 * 6 * 8
+    push32_bytes = n * (PUSH_MOVE_BYTES_F32 if fused else PUSH_BYTES_F32) + nb * 729 * 6 * 8
     fp32_variant = {"value": n * args.steps / el32, "unit": "particle-pushes/s", "ms_per_step": 1e3 * el32 / args.steps,
                     "kernel_ms": {k: round(v, 4) for k, v in avg32.items()},
                     "push_GBps": round(push32_bytes / (avg32["push"] * 1e-3) / 1e9, 1)}

@@ -496,12 +496,15 @@ def test_generator_and_graph_steps_match_eager(gpu_ctx):
 @pytest.mark.parametrize("precision", [L.FP64, L.FP32])
 def test_step_deposit_moves_the_migrated_records(gpu_ctx, precision):
     """In a step the deterministic migration only works out where each
-    particle goes and the matrix-core cell pass moves the records as it
-    deposits them.  The arrays (in array order) and the resident J and M
-    after each step equal those of separate MoveParticles(compact) and
-    UpdateJMassMatrix calls, whose migration moves the records itself; J and
-    M are compared through the field equation's mass term and right-hand
-    side, which read them on the device."""
+    particle goes: the matrix-core cell pass reads the particles where they
+    were, the next push moves them as it updates them, and run_steps moves
+    the last step's before it returns.  The arrays (in array order) and the
+    resident J and M after each step equal those of separate
+    MoveParticles(compact) and UpdateJMassMatrix calls, whose migration moves
+    the records itself, and so do those after three steps in one
+    graph-captured run_steps call (pushes that move the records); J and M
+    are compared through the field equation's mass term and right-hand side,
+    which read them on the device."""
     mesh = M.uniform_mesh(1)
     E, B = M.corner_fields(mesh)
     # thermal speeds of 0.3 and steps of 0.05: a particle moves ~0.015 per
@@ -510,15 +513,20 @@ def test_step_deposit_moves_the_migrated_records(gpu_ctx, precision):
     sp = Species([-1.0, 1.0], [0.01, 1.0], [dt, dt])
     rng = np.random.default_rng(3)
     res = {}
-    for fused in (True, False):  # one after the other: the states share the device context
+    for fused in (True, False, "run"):  # one after the other: the states share the device context
         st = PicState(mesh, E, B, sp, ctx=gpu_ctx)
         st.generate_particles(mesh.ncells * 3 + 777, [0.3, 0.3], drift=(0.01, 0.0, 0.0), seed=5,
                               precision=precision)
-        if fused:
+        if fused is True:
             En = rng.normal(size=(st.nnodes, 3))
             Bc = rng.normal(size=(mesh.nblocks * 512, 3))
             prm = st.ecsim_params(dt)
         out = []
+        if fused == "run":
+            st.run_steps(3, dt, r_sphere=0.2, use_graph=True)
+            res[fused] = (st.get_particles()[0], st.MassMatrixApply(En, -4 * np.pi * dt * 0.5),
+                          st.EcsimRhs(prm, En, Bc))
+            continue
         for step in range(3):
             if fused:
                 st.run_steps(1, dt, r_sphere=0.2, use_graph=False)
@@ -540,6 +548,11 @@ def test_step_deposit_moves_the_migrated_records(gpu_ctx, precision):
         assert np.array_equal(a.x, b.x) and np.array_equal(a.v, b.v) and np.array_equal(a.spec, b.spec)
         assert np.array_equal(a.blk, b.blk) and np.array_equal(a.cell, b.cell)
         assert np.array_equal(ya, yb) and np.array_equal(ra, rb)
+    (a, ya, ra), (b, yb, rb) = res["run"], res[False][-1]
+    assert np.array_equal(a.id, b.id) and np.array_equal(a.spec, b.spec)
+    assert np.array_equal(a.x, b.x) and np.array_equal(a.v, b.v)
+    assert np.array_equal(a.blk, b.blk) and np.array_equal(a.cell, b.cell)
+    assert np.array_equal(ya, yb) and np.array_equal(ra, rb)
 
 
 @pytest.mark.parametrize("mover", [L.MOVER_LAPENTA, L.MOVER_BORIS])
