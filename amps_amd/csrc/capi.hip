@@ -36,9 +36,9 @@ __global__ void k_mig_scatter(const int*, const int*, const int8_t*, const int*,
                               const uint64_t*, const int*, int64_t, const double*, const double*, int, const uint8_t*,
                               const int64_t*, double*, double*, uint8_t*, int*, int64_t*);
 __global__ void k_mig_dest(const int*, const int*, const int8_t*, const int*, const int*, const int*,
-                           const uint64_t*, const int*, int*, int*);
+                           const uint64_t*, const int*, int*);
 __global__ void k_mig_move(const int*, const int*, int64_t, const double*, const double*, int, const uint8_t*,
-                           const int64_t*, double*, double*, uint8_t*, int64_t*);
+                           const int64_t*, const int*, double*, double*, uint8_t*, int64_t*, int*);
 __global__ void k_copy_ints(int64_t, const int*, int*);
 __global__ void k_tile_count(int, int, const int*, int*);
 __global__ void k_tile_emit(int, int, const int*, const int*, Tile*, int*);
@@ -838,7 +838,7 @@ static int compact_impl(amps_ctx* c, bool counted = false, bool defer = false) {
                        c->d_tmpsrc, c->d_key[b], c->d_dest, c->d_big);
     if (defer) {
       hipLaunchKernelGGL(k_mig_dest, dim3(gridp), dim3(256), 0, c->stream, c->d_n, c->d_key[a], c->d_code,
-                         c->d_offset, c->d_off_layout, c->d_wcnt, c->d_wmask, c->d_dest, c->d_tmpsrc, c->d_key[b]);
+                         c->d_offset, c->d_off_layout, c->d_wcnt, c->d_wmask, c->d_dest, c->d_tmpsrc);
       c->pending_gather = true;
     } else {
       hipLaunchKernelGGL(k_mig_scatter, dim3(gridp), dim3(256), 0, c->stream, c->d_n, c->d_key[a], c->d_code,
@@ -1144,7 +1144,7 @@ static int settle_records(amps_ctx* c) {
   const int k = c->cur, r = 1 - k;
   hipLaunchKernelGGL(k_mig_move, dim3((unsigned)((c->cap + 255) / 256)), dim3(256), 0, c->stream, c->d_n,
                      c->d_tmpsrc, c->cap, c->d_x[r], c->d_v[r], c->precision == AMPS_FP32 ? 1 : 0, c->d_spec[r],
-                     c->d_id[r], c->d_x[k], c->d_v[k], c->d_spec[k], c->d_id[k]);
+                     c->d_id[r], c->d_key[r], c->d_x[k], c->d_v[k], c->d_spec[k], c->d_id[k], c->d_key[k]);
   c->pending_gather = false;
   HIP_TRY(hipGetLastError());
   return 0;

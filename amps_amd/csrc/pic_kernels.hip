@@ -409,33 +409,32 @@ __device__ __forceinline__ int64_t mig_dest(int64_t p, int k, const int* __restr
   return dest[p];
 }
 
-// The same destinations without moving the records: src[d] = p and the key
-// at d, for a cell pass of the deposit that gathers the particles from the
-// previous arrays, deposits them and writes them to their places.
+// The same destinations without moving the records: src[d] = p, for a
+// deposit cell pass that reads the particles where they are and a push that
+// moves them (the push writes every surviving particle's key; the keys stay
+// in the previous arrays until then, where k_mig_move finds them).
 __global__ __launch_bounds__(256) void k_mig_dest(const int* __restrict__ n_dev, const int* __restrict__ key,
                                                   const int8_t* __restrict__ code, const int* __restrict__ offset,
                                                   const int* __restrict__ lay, const int* __restrict__ wscan,
                                                   const uint64_t* __restrict__ wmask,
-                                                  const int* __restrict__ dest, int* __restrict__ src,
-                                                  int* __restrict__ kb) {
+                                                  const int* __restrict__ dest, int* __restrict__ src) {
   const int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   const int n = *n_dev;
   const bool valid = (p < n) && alive_code(code[p]);
   if (!valid) return;
   const int k = key[p];
-  const int64_t d = mig_dest(p, k, offset, lay, wscan, wmask, dest);
-  src[d] = (int)p;
-  kb[d] = k;
+  src[mig_dest(p, k, offset, lay, wscan, wmask, dest)] = (int)p;
 }
 
 // The record move that k_mig_dest left: particle d of the new order from
-// src[d] of the previous arrays (the key is in place already).
+// src[d] of the previous arrays.
 __global__ __launch_bounds__(256) void k_mig_move(const int* __restrict__ n_dev, const int* __restrict__ src,
                                                   int64_t cap, const double* __restrict__ xa,
                                                   const double* __restrict__ va, int vf32,
                                                   const uint8_t* __restrict__ sa, const int64_t* __restrict__ ia,
-                                                  double* __restrict__ xb, double* __restrict__ vb,
-                                                  uint8_t* __restrict__ sb, int64_t* __restrict__ ib) {
+                                                  const int* __restrict__ ka, double* __restrict__ xb,
+                                                  double* __restrict__ vb, uint8_t* __restrict__ sb,
+                                                  int64_t* __restrict__ ib, int* __restrict__ kb) {
   const int64_t d = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (d >= *n_dev) return;
   const int64_t p = src[d];
@@ -449,6 +448,7 @@ __global__ __launch_bounds__(256) void k_mig_move(const int* __restrict__ n_dev,
   }
   sb[d] = sa[p];
   ib[d] = ia[p];
+  kb[d] = ka[p];
 }
 
 __global__ __launch_bounds__(256) void k_mig_scatter(const int* __restrict__ n_dev, const int* __restrict__ key,
