@@ -477,7 +477,7 @@ int launch_deposit_cell_mfma(amps_ctx* c, double dt) {
   if (c->pending_gather) {
     // the records of the last compaction are still where it found them: the
     // pass reads them there (set only with the single-phase pass)
-    if (paired) return 1;
+    if (paired) return fail(AMPS_E_STATE, "deposit: records left by a step's migration need the single-phase cell pass");
     if (c->b_center)
       launch<true, false, true>(c, dt);
     else
