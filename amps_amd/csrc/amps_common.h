@@ -173,6 +173,7 @@ struct amps_ctx {
   // stayer count (scanned) and mask of every wavefront of the scatter grid
   int* d_off_layout = nullptr;  // [nkeys + 1]
   int* d_big = nullptr;         // [nkeys + 1]: keys with many movers, count last
+  int* d_kbase = nullptr;       // [nkeys]: a key's new offset less the stayers before its old run
   int* d_dest = nullptr;        // [cap]
   int* d_tmpsrc = nullptr;      // [cap]
   int* d_wcnt = nullptr;        // [ceil(cap / 256) * 4 + 1]

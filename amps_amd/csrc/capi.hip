@@ -30,13 +30,14 @@ __global__ void k_scatter(const int*, const int*, const int8_t*, const int*, int
 __global__ void k_copy_total(const int*, int*);
 __global__ void k_mig_classify(const int*, const int*, const int8_t*, const int*, int, const int*, int*, int*, int*,
                                uint64_t*);
-__global__ void k_mig_order(int64_t, int, const int*, const int*, int*, int*, int*);
+__global__ void k_mig_order(int64_t, int, const int*, const int*, int*, int*, int*, int, const int*, const int*,
+                            const uint64_t*, int*);
 __global__ void k_mig_order_big(int64_t, const int*, const int*, int*, int*, int*, const int*);
-__global__ void k_mig_scatter(const int*, const int*, const int8_t*, const int*, const int*, const int*,
-                              const uint64_t*, const int*, int64_t, const double*, const double*, int, const uint8_t*,
+__global__ void k_mig_scatter(const int*, const int*, const int8_t*, const int*, const int*, const uint64_t*,
+                              const int*, int64_t, const double*, const double*, int, const uint8_t*,
                               const int64_t*, double*, double*, uint8_t*, int*, int64_t*);
-__global__ void k_mig_dest(const int*, const int*, const int8_t*, const int*, const int*, const int*,
-                           const uint64_t*, const int*, int*);
+__global__ void k_mig_dest(const int*, const int*, const int8_t*, const int*, const int*, const uint64_t*,
+                           const int*, int*);
 __global__ void k_mig_move(const int*, const int*, int64_t, const double*, const double*, int, const uint8_t*,
                            const int64_t*, const int*, double*, double*, uint8_t*, int64_t*, int*);
 __global__ void k_copy_ints(int64_t, const int*, int*);
@@ -133,7 +134,8 @@ static void free_mesh(amps_ctx* c) {
   (void)hipFree(c->d_offset);
   (void)hipFree(c->d_off_layout);
   (void)hipFree(c->d_big);
-  c->d_off_layout = c->d_big = nullptr;
+  (void)hipFree(c->d_kbase);
+  c->d_off_layout = c->d_big = c->d_kbase = nullptr;
   c->layout_valid = false;
   (void)hipFree(c->d_cursor);
   (void)hipFree(c->d_tcount);
@@ -493,6 +495,7 @@ extern "C" int amps_gpu_set_mesh(amps_ctx* c, const amps_mesh_desc* desc, const 
   HIP_TRY(hipMalloc(&c->d_offset, sizeof(int) * (nkeys + 1)));
   HIP_TRY(hipMalloc(&c->d_off_layout, sizeof(int) * (nkeys + 1)));
   HIP_TRY(hipMalloc(&c->d_big, sizeof(int) * (nkeys + 1)));
+  HIP_TRY(hipMalloc(&c->d_kbase, sizeof(int) * std::max<int64_t>(nkeys, 1)));
   HIP_TRY(hipMalloc(&c->d_cursor, sizeof(int) * nkeys));
   c->layout_valid = false;
   HIP_TRY(hipMalloc(&c->d_tcount, sizeof(int) * (nb + 1)));
@@ -832,17 +835,18 @@ static int compact_impl(amps_ctx* c, bool counted = false, bool defer = false) {
     if (int r = scan_exclusive(c, c->d_wcnt, nw + 1, nullptr)) return r;
     hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, c->stream, c->d_big + nkeys, 1, 0);
     hipLaunchKernelGGL(k_mig_order, dim3(gridk), dim3(256), 0, c->stream, nkeys, c->mig_small, c->d_offset,
-                       c->d_cursor, c->d_tmpsrc, c->d_dest, c->d_big);
+                       c->d_cursor, c->d_tmpsrc, c->d_dest, c->d_big, c->layout_valid ? 1 : 0, c->d_off_layout,
+                       c->d_wcnt, c->d_wmask, c->d_kbase);
     // spare array of the merge sort: the other buffer's keys, written later
     hipLaunchKernelGGL(k_mig_order_big, dim3(512), dim3(256), 0, c->stream, nkeys, c->d_offset, c->d_cursor,
                        c->d_tmpsrc, c->d_key[b], c->d_dest, c->d_big);
     if (defer) {
       hipLaunchKernelGGL(k_mig_dest, dim3(gridp), dim3(256), 0, c->stream, c->d_n, c->d_key[a], c->d_code,
-                         c->d_offset, c->d_off_layout, c->d_wcnt, c->d_wmask, c->d_dest, c->d_tmpsrc);
+                         c->d_kbase, c->d_wcnt, c->d_wmask, c->d_dest, c->d_tmpsrc);
       c->pending_gather = true;
     } else {
       hipLaunchKernelGGL(k_mig_scatter, dim3(gridp), dim3(256), 0, c->stream, c->d_n, c->d_key[a], c->d_code,
-                         c->d_offset, c->d_off_layout, c->d_wcnt, c->d_wmask, c->d_dest, cap, c->d_x[a], c->d_v[a],
+                         c->d_kbase, c->d_wcnt, c->d_wmask, c->d_dest, cap, c->d_x[a], c->d_v[a],
                          vf32, c->d_spec[a], c->d_id[a], c->d_x[b], c->d_v[b], c->d_spec[b], c->d_key[b],
                          c->d_id[b]);
     }

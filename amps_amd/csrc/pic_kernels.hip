@@ -319,9 +319,18 @@ __global__ __launch_bounds__(256) void k_mig_classify(const int* __restrict__ n_
 // listed in big[] (count in big[nkeys]) for k_mig_order_big.
 __global__ __launch_bounds__(256) void k_mig_order(int64_t nkeys, int small, const int* __restrict__ offset,
                                                     const int* __restrict__ cursor, int* __restrict__ tmpsrc,
-                                                    int* __restrict__ dest, int* __restrict__ big) {
+                                                    int* __restrict__ dest, int* __restrict__ big, int has_layout,
+                                                    const int* __restrict__ lay, const int* __restrict__ wscan,
+                                                    const uint64_t* __restrict__ wmask, int* __restrict__ kbase) {
   const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (k >= nkeys) return;
+  // the destination base of key k's stayers: its new offset less the stayers
+  // (of any key) before its old run (a key with an empty old run has none)
+  if (has_layout) {
+    const int q = lay[k];
+    if (lay[k + 1] > q)
+      kbase[k] = offset[k] - (wscan[q >> 6] + __popcll(wmask[q >> 6] & ((1ull << (q & 63)) - 1ull)));
+  }
   const int m = cursor[k];
   if (m == 0) return;
   if (m > small) {
@@ -393,19 +402,15 @@ __global__ __launch_bounds__(256) void k_mig_order_big(int64_t nkeys, const int*
 }
 
 // Where particle p (key k) goes: a stayer after the stayers of its key that
-// precede it (its wave's stayer mask and the scanned per-wave counts), a
-// mover where k_mig_order put it.
-__device__ __forceinline__ int64_t mig_dest(int64_t p, int k, const int* __restrict__ offset,
-                                            const int* __restrict__ lay, const int* __restrict__ wscan,
-                                            const uint64_t* __restrict__ wmask, const int* __restrict__ dest) {
+// precede it (the stayers before it, from its wave's stayer mask and the
+// scanned per-wave counts, past its key's base from k_mig_order), a mover
+// where k_mig_order put it.
+__device__ __forceinline__ int64_t mig_dest(int64_t p, int k, const int* __restrict__ kbase,
+                                            const int* __restrict__ wscan, const uint64_t* __restrict__ wmask,
+                                            const int* __restrict__ dest) {
   const int lane = (int)(p & 63);
   const uint64_t mw = wmask[p >> 6];
-  if ((mw >> lane) & 1ull) {
-    const int q = lay[k];
-    const int sp = wscan[p >> 6] + __popcll(mw & ((1ull << lane) - 1ull));
-    const int sq = wscan[q >> 6] + __popcll(wmask[q >> 6] & ((1ull << (q & 63)) - 1ull));
-    return (int64_t)offset[k] + (sp - sq);
-  }
+  if ((mw >> lane) & 1ull) return (int64_t)kbase[k] + wscan[p >> 6] + __popcll(mw & ((1ull << lane) - 1ull));
   return dest[p];
 }
 
@@ -414,16 +419,15 @@ __device__ __forceinline__ int64_t mig_dest(int64_t p, int k, const int* __restr
 // moves them (the push writes every surviving particle's key; the keys stay
 // in the previous arrays until then, where k_mig_move finds them).
 __global__ __launch_bounds__(256) void k_mig_dest(const int* __restrict__ n_dev, const int* __restrict__ key,
-                                                  const int8_t* __restrict__ code, const int* __restrict__ offset,
-                                                  const int* __restrict__ lay, const int* __restrict__ wscan,
-                                                  const uint64_t* __restrict__ wmask,
+                                                  const int8_t* __restrict__ code, const int* __restrict__ kbase,
+                                                  const int* __restrict__ wscan, const uint64_t* __restrict__ wmask,
                                                   const int* __restrict__ dest, int* __restrict__ src) {
   const int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   const int n = *n_dev;
   const bool valid = (p < n) && alive_code(code[p]);
   if (!valid) return;
   const int k = key[p];
-  src[mig_dest(p, k, offset, lay, wscan, wmask, dest)] = (int)p;
+  src[mig_dest(p, k, kbase, wscan, wmask, dest)] = (int)p;
 }
 
 // The record move that k_mig_dest left: particle d of the new order from
@@ -452,8 +456,8 @@ __global__ __launch_bounds__(256) void k_mig_move(const int* __restrict__ n_dev,
 }
 
 __global__ __launch_bounds__(256) void k_mig_scatter(const int* __restrict__ n_dev, const int* __restrict__ key,
-                                                      const int8_t* __restrict__ code, const int* __restrict__ offset,
-                                                      const int* __restrict__ lay, const int* __restrict__ wscan,
+                                                      const int8_t* __restrict__ code, const int* __restrict__ kbase,
+                                                      const int* __restrict__ wscan,
                                                       const uint64_t* __restrict__ wmask,
                                                       const int* __restrict__ dest, int64_t cap,
                                                       const double* __restrict__ xa, const double* __restrict__ va,
@@ -466,7 +470,7 @@ __global__ __launch_bounds__(256) void k_mig_scatter(const int* __restrict__ n_d
   const bool valid = (p < n) && alive_code(code[p]);
   if (!valid) return;
   const int k = key[p];
-  const int64_t d = mig_dest(p, k, offset, lay, wscan, wmask, dest);
+  const int64_t d = mig_dest(p, k, kbase, wscan, wmask, dest);
 #pragma unroll
   for (int c = 0; c < 3; c++) {
     xb[c * cap + d] = xa[c * cap + p];
