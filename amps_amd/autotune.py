@@ -52,6 +52,24 @@ def _save(path, d):
         pass
 
 
+def weigh_steps(ctx, state, opts_kw, deposit_dt, variants, reps=2):
+    """The deposit variant whose whole steps (push, migration, both deposit
+    passes, by HIP events over reps + 1 eager steps) are fastest, and each
+    variant's time per step; leaves the winner set."""
+    ctx.set_timing(True)
+    times = {}
+    for var in variants:
+        set_deposit_variant(ctx, *var)
+        ctx.reset_timing()
+        state.run_steps(reps + 1, deposit_dt, use_graph=False, **opts_kw)
+        ms, n = ctx.timing()
+        times[var] = sum(ms[k] for k in (L.K_PUSH, L.K_COMPACT, L.K_DEPOSIT_CELL, L.K_DEPOSIT_NODE)) / \
+            max(n[L.K_PUSH], 1)
+    best = min(variants, key=lambda v: times[v])
+    set_deposit_variant(ctx, *best)
+    return best, times
+
+
 def tune_pic(ctx, state, opts_kw, deposit_dt, signature, cache=None, reps=2):
     """Pick the push tile and register budget (occupancy), the deposit
     cell-pass variant and the node gather's batch for a PicState."""
@@ -93,6 +111,13 @@ def tune_pic(ctx, state, opts_kw, deposit_dt, signature, cache=None, reps=2):
         ms, n = ctx.timing()
         res[("dep", var)] = (ms[L.K_DEPOSIT_CELL] + ms[L.K_DEPOSIT_NODE]) / max(n[L.K_DEPOSIT_CELL], 1)
     best_var = min(DEPOSIT_VARIANTS, key=lambda v: res[("dep", v)])
+    # only the matrix-core pass (kernel 2) reads a step's particles where the
+    # migration found them, so that the step skips the record scatter: when
+    # another pass wins on the deposit alone, the whole steps decide
+    mfma = next((v for v in DEPOSIT_VARIANTS if v[0] == 2), None)
+    if mfma is not None and best_var != mfma:
+        best_var, steps = weigh_steps(ctx, state, opts_kw, deposit_dt, (best_var, mfma), reps)
+        res.update({("step", v): t for v, t in steps.items()})
     set_deposit_variant(ctx, *best_var)
     for nb in NODE_BATCHES:
         ctx.set_param("node_batch", nb)
@@ -109,6 +134,7 @@ def tune_pic(ctx, state, opts_kw, deposit_dt, signature, cache=None, reps=2):
             "push_ms": {str(t): round(res[("tile", t)], 4) for t in PUSH_TILES},
             "push_waves_ms": {str(w): round(res[("waves", w)], 4) for w in PUSH_WAVES},
             "deposit_ms": {f"k{k}c{c}": round(res[("dep", (k, c))], 4) for k, c in DEPOSIT_VARIANTS},
+            "step_ms": {f"k{key[1][0]}c{key[1][1]}": round(v, 4) for key, v in res.items() if key[0] == "step"},
             "node_batch_ms": {str(b): round(res[("node", b)], 4) for b in NODE_BATCHES}}
     db[signature] = best
     _save(cache, db)

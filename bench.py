@@ -574,7 +574,8 @@ def main():
         sig = f"{arch}|pic|nb{mesh.nblocks}|n{n}"
         tune, cached = autotune.tune_pic(ctx, st, opts, dt, sig, cache=args.tune_cache)
         tune = {k: tune[k] for k in ("tile", "push_waves", "deposit_kernel", "deposit_chunks", "node_batch",
-                                     "push_ms", "push_waves_ms", "deposit_ms", "node_batch_ms")}
+                                     "push_ms", "push_waves_ms", "deposit_ms", "step_ms", "node_batch_ms")
+                if k in tune}
         tune["cached"] = cached
     # warm-up, then the timed region (graph-captured steps)
     if args.warmup > 0:

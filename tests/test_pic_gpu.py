@@ -691,3 +691,44 @@ def test_ecsim_mass_apply_single_particle_closed_form(gpu_ctx):
         want[ni] = (q * h / V) * wi * (alpha @ Ep) * c1 * c2
     assert np.allclose(out, want, rtol=1e-11, atol=1e-13 * np.abs(want).max())
     assert np.count_nonzero(np.abs(out).sum(1)) == 8
+
+
+def test_autotuner_weighs_whole_steps(gpu_ctx, monkeypatch):
+    """The tuner picks the deposit pass by the deposit's time, but a pass
+    other than the matrix-core one (the only one that reads a step's
+    particles where the migration left them) wins only if whole steps are
+    faster with it too; the chosen configuration then runs steps."""
+    from amps_amd import autotune
+
+    mesh = M.central_octant_mesh()
+    E, B = M.corner_fields(mesh)
+    sp = Species([-1.0, 1.0], [0.01, 1.0], [0.05, 0.05])
+    st = PicState(mesh, E, B, sp, ctx=gpu_ctx)
+    st.generate_particles(mesh.ncells * 8, [0.3, 0.3], seed=3)
+    monkeypatch.setattr(autotune, "DEPOSIT_VARIANTS", ((3, 4), (2, 4)))
+    opts = dict(mover=L.MOVER_LAPENTA, bc_mode=L.BC_REFLECT, r_sphere=0.2)
+    try:
+        best, cached = autotune.tune_pic(gpu_ctx, st, opts, 0.05, "test", cache=None)
+        print(f"tuned: {best}", flush=True)
+        assert not cached and best["deposit_kernel"] in (2, 3)
+        dep = best["deposit_ms"]
+        if dep["k3c4"] < dep["k2c4"]:
+            steps = best["step_ms"]
+            assert set(steps) == {"k3c4", "k2c4"}
+            assert best["deposit_kernel"] == (2 if steps["k2c4"] <= steps["k3c4"] else 3)
+        else:
+            assert best["deposit_kernel"] == 2 and not best["step_ms"]
+        n0 = st.particle_count()
+        st.run_steps(3, 0.05, **opts)
+        assert 0 < st.particle_count() <= n0
+        # the whole-step comparison itself
+        var, steps = autotune.weigh_steps(gpu_ctx, st, opts, 0.05, ((3, 4), (2, 4)))
+        print(f"steps: {steps}", flush=True)
+        assert set(steps) == {(3, 4), (2, 4)} and all(t > 0 for t in steps.values())
+        assert var == min(steps, key=steps.get)
+    finally:  # the session context's defaults
+        autotune.set_deposit_variant(gpu_ctx, 2, 12)
+        gpu_ctx.set_param("tile", 8192)
+        gpu_ctx.set_param("push_waves", 1)
+        gpu_ctx.set_param("node_batch", 8)
+        gpu_ctx.set_timing(False)
