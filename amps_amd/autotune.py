@@ -111,12 +111,14 @@ def tune_pic(ctx, state, opts_kw, deposit_dt, signature, cache=None, reps=2):
         ms, n = ctx.timing()
         res[("dep", var)] = (ms[L.K_DEPOSIT_CELL] + ms[L.K_DEPOSIT_NODE]) / max(n[L.K_DEPOSIT_CELL], 1)
     best_var = min(DEPOSIT_VARIANTS, key=lambda v: res[("dep", v)])
-    # only the matrix-core pass (kernel 2) reads a step's particles where the
-    # migration found them, so that the step skips the record scatter: when
-    # another pass wins on the deposit alone, the whole steps decide
-    mfma = next((v for v in DEPOSIT_VARIANTS if v[0] == 2), None)
-    if mfma is not None and best_var != mfma:
-        best_var, steps = weigh_steps(ctx, state, opts_kw, deposit_dt, (best_var, mfma), reps)
+    # the matrix-core and the lane passes (kernels 2 and 3) read a step's
+    # particles where the migration found them, so that the step skips the
+    # record scatter: when another pass wins on the deposit alone, the whole
+    # steps decide against the faster of those two
+    fusable = [v for v in DEPOSIT_VARIANTS if v[0] in (2, 3)]
+    if fusable and best_var not in fusable:
+        alt = min(fusable, key=lambda v: res[("dep", v)])
+        best_var, steps = weigh_steps(ctx, state, opts_kw, deposit_dt, (best_var, alt), reps)
         res.update({("step", v): t for v, t in steps.items()})
     set_deposit_variant(ctx, *best_var)
     for nb in NODE_BATCHES:

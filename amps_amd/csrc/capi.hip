@@ -1162,10 +1162,11 @@ static int one_step(amps_ctx* c, const amps_push_opts* o, double dt) {
   tend(c, AMPS_K_PUSH);
   if (o->bc_mode == AMPS_BC_USER)
     if (int r = collect_exits(c)) return r;
-  // the deposit follows at once: with the matrix-core cell pass, the
-  // migration leaves the records where they are (the cell pass reads them
-  // there, the next push moves them)
-  const bool defer = c->deterministic && c->dep_kernel == 2 && !c->dep_exact && !c->dep_paired;
+  // the deposit follows at once: with the matrix-core (single-phase) or the
+  // lane cell pass, the migration leaves the records where they are (the
+  // cell pass reads them there, the next push moves them)
+  const bool defer = c->deterministic && !c->dep_exact &&
+                     ((c->dep_kernel == 2 && !c->dep_paired) || c->dep_kernel == 3);
   tbegin(c);
   if (int r = compact_impl(c, true, defer)) return r;
   tend(c, AMPS_K_COMPACT);

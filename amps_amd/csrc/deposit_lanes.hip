@@ -58,7 +58,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) void 
                                                            const double* __restrict__ v, int vf32,
                                                            const uint8_t* __restrict__ spec,
                                                            double* __restrict__ cellpart,
-                                                           const double* __restrict__ Bc) {
+                                                           const double* __restrict__ Bc,
+                                                           const int* __restrict__ src) {
   constexpr int RS = 8 * (PB + 1);  // slots per field: cell c, particle j at c (PB + 1) + j
   static_assert(NFL * RS >= 4 * CSR, "the record buffer stages four cell records");
   __shared__ double rec[NFL * RS];
@@ -116,7 +117,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) void 
   int ps = 0;
   auto fetch = [&](int rank) {
     if (l < PB && rank < n) {
-      const int p = p0 + rank;
+      // src: the particles are still where the last compaction found them,
+      // particle p of the new order at src[p]
+      const int p = src ? src[p0 + rank] : p0 + rank;
       px[0] = x[p], px[1] = x[cap + p], px[2] = x[2 * cap + p];
       if (vf32) {
         const float* vf = (const float*)v;
@@ -293,10 +296,13 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) void 
 
 template <int PB, bool BC, int WPE>
 void launch(amps_ctx* c, double dt) {
-  const int k = c->cur;
+  // records a step's migration left: read from the previous buffer through
+  // its source indices
+  const int k = c->cur, r = c->pending_gather ? 1 - k : k;
   hipLaunchKernelGGL((k_deposit_cell_lanes<PB, BC, WPE>), dim3((unsigned)c->mv.nb * 64), dim3(64), 0, c->stream,
-                     c->mv, c->sp, dt, c->d_B, c->d_offset, c->cap, c->d_x[k], c->d_v[k],
-                     c->precision == AMPS_FP32 ? 1 : 0, c->d_spec[k], c->d_cellpart, c->d_Bc);
+                     c->mv, c->sp, dt, c->d_B, c->d_offset, c->cap, c->d_x[r], c->d_v[r],
+                     c->precision == AMPS_FP32 ? 1 : 0, c->d_spec[r], c->d_cellpart, c->d_Bc,
+                     c->pending_gather ? c->d_tmpsrc : nullptr);
 }
 
 template <int PB>

@@ -596,11 +596,12 @@ def main():
                                                   ("deposit_cell", L.K_DEPOSIT_CELL),
                                                   ("deposit_node", L.K_DEPOSIT_NODE)]}
     nb, nnodes = mesh.nblocks, st.nnodes
-    # with the deterministic migration and the single-phase matrix-core cell
-    # pass the push moves the migrated records (the cell pass reads them where
-    # they were): 3 of the 4 timed pushes, all of the hipGraph steps'
+    # with the deterministic migration and the single-phase matrix-core or
+    # the lane cell pass the push moves the migrated records (the cell pass
+    # reads them where they were): 3 of the 4 timed pushes, all of the
+    # hipGraph steps'
     dep_kernel_used = tune["deposit_kernel"] if tune else (args.deposit_kernel if args.deposit_kernel >= 0 else 2)
-    fused = args.deterministic == 1 and dep_kernel_used == 2 and args.deposit_paired != 1
+    fused = args.deterministic == 1 and (dep_kernel_used == 3 or (dep_kernel_used == 2 and args.deposit_paired != 1))
     push_bytes = n * (PUSH_MOVE_BYTES if fused else PUSH_BYTES) + nb * 729 * 6 * 8
     dep_bytes = n * DEPOSIT_BYTES_P + nnodes * NODE_BYTES + nb * 729 * 3 * 8
     push_gbps = push_bytes / (avg["push"] * 1e-3) / 1e9

@@ -493,41 +493,29 @@ def test_generator_and_graph_steps_match_eager(gpu_ctx):
     assert np.array_equal(Ja, Jb) and np.array_equal(Ma, Mb)
 
 
-@pytest.mark.parametrize("precision", [L.FP64, L.FP32])
-def test_step_deposit_moves_the_migrated_records(gpu_ctx, precision):
-    """In a step the deterministic migration only works out where each
-    particle goes: the matrix-core cell pass reads the particles where they
-    were, the next push moves them as it updates them, and run_steps moves
-    the last step's before it returns.  The arrays (in array order) and the
-    resident J and M after each step equal those of separate
-    MoveParticles(compact) and UpdateJMassMatrix calls, whose migration moves
-    the records itself, and so do those after three steps in one
-    graph-captured run_steps call (pushes that move the records); J and M
-    are compared through the field equation's mass term and right-hand side,
-    which read them on the device."""
-    mesh = M.uniform_mesh(1)
-    E, B = M.corner_fields(mesh)
-    # thermal speeds of 0.3 and steps of 0.05: a particle moves ~0.015 per
-    # step, an eighth of a cell, so a good share change cells every step
-    dt = 0.05
-    sp = Species([-1.0, 1.0], [0.01, 1.0], [dt, dt])
+def _record_move_runs(ctx, mesh, E, B, sp, dt, precision):
+    """Three steps three ways, one after the other (the states share the
+    device context): one run_steps call per step (True), separate
+    MoveParticles(compact) and UpdateJMassMatrix calls (False), one
+    graph-captured run_steps call ("run"); the arrays and the mass term and
+    right-hand side over the resident J and M after each step."""
     rng = np.random.default_rng(3)
     res = {}
-    for fused in (True, False, "run"):  # one after the other: the states share the device context
-        st = PicState(mesh, E, B, sp, ctx=gpu_ctx)
+    for fused in (True, False, "run"):
+        st = PicState(mesh, E, B, sp, ctx=ctx)
         st.generate_particles(mesh.ncells * 3 + 777, [0.3, 0.3], drift=(0.01, 0.0, 0.0), seed=5,
                               precision=precision)
         if fused is True:
             En = rng.normal(size=(st.nnodes, 3))
             Bc = rng.normal(size=(mesh.nblocks * 512, 3))
             prm = st.ecsim_params(dt)
-        out = []
         if fused == "run":
             st.run_steps(3, dt, r_sphere=0.2, use_graph=True)
             res[fused] = (st.get_particles()[0], st.MassMatrixApply(En, -4 * np.pi * dt * 0.5),
                           st.EcsimRhs(prm, En, Bc))
             continue
-        for step in range(3):
+        out = []
+        for _ in range(3):
             if fused:
                 st.run_steps(1, dt, r_sphere=0.2, use_graph=False)
             else:
@@ -537,12 +525,41 @@ def test_step_deposit_moves_the_migrated_records(gpu_ctx, precision):
             r = st.EcsimRhs(prm, En, Bc)
             out.append((st.get_particles()[0], y, r))
         res[fused] = out
+    return res
+
+
+@pytest.mark.parametrize("kernel", [2, 3])
+@pytest.mark.parametrize("precision", [L.FP64, L.FP32])
+def test_step_deposit_moves_the_migrated_records(gpu_ctx, precision, kernel):
+    """In a step the deterministic migration only works out where each
+    particle goes: the cell pass (matrix-core or lane) reads the particles
+    where they were, the next push moves them as it updates them, and
+    run_steps moves the last step's before it returns.  The arrays (in array
+    order) and the resident J and M after each step equal those of separate
+    MoveParticles(compact) and UpdateJMassMatrix calls, whose migration moves
+    the records itself, and so do those after three steps in one
+    graph-captured run_steps call (pushes that move the records); J and M
+    are compared through the field equation's mass term and right-hand side,
+    which read them on the device."""
+    from amps_amd import autotune
+
+    mesh = M.uniform_mesh(1)
+    E, B = M.corner_fields(mesh)
+    # thermal speeds of 0.3 and steps of 0.05: a particle moves ~0.015 per
+    # step, an eighth of a cell, so a good share change cells every step
+    dt = 0.05
+    sp = Species([-1.0, 1.0], [0.01, 1.0], [dt, dt])
+    autotune.set_deposit_variant(gpu_ctx, kernel, 4)
+    try:
+        res = _record_move_runs(gpu_ctx, mesh, E, B, sp, dt, precision)
+    finally:  # the session context's default
+        autotune.set_deposit_variant(gpu_ctx, 2, 12)
     prev = None
     for step, ((a, ya, ra), (b, yb, rb)) in enumerate(zip(res[True], res[False])):
         key = dict(zip(a.id.tolist(), (a.blk.astype(np.int64) * 512 + a.cell).tolist()))
         moved = 0 if prev is None else sum(prev.get(i, -1) != k for i, k in key.items())
         prev = key
-        print(f"step {step}: {a.n} particles, {moved} changed cells", flush=True)
+        print(f"kernel {kernel} step {step}: {a.n} particles, {moved} changed cells", flush=True)
         assert a.n == b.n and a.n > 0 and (step == 0 or moved > a.n // 10)
         assert np.array_equal(a.id, b.id)
         assert np.array_equal(a.x, b.x) and np.array_equal(a.v, b.v) and np.array_equal(a.spec, b.spec)
@@ -695,9 +712,9 @@ def test_ecsim_mass_apply_single_particle_closed_form(gpu_ctx):
 
 def test_autotuner_weighs_whole_steps(gpu_ctx, monkeypatch):
     """The tuner picks the deposit pass by the deposit's time, but a pass
-    other than the matrix-core one (the only one that reads a step's
-    particles where the migration left them) wins only if whole steps are
-    faster with it too; the chosen configuration then runs steps."""
+    other than the matrix-core and lane ones (which read a step's particles
+    where the migration left them) wins only if whole steps are faster with
+    it too; the chosen configuration then runs steps."""
     from amps_amd import autotune
 
     mesh = M.central_octant_mesh()
@@ -705,26 +722,26 @@ def test_autotuner_weighs_whole_steps(gpu_ctx, monkeypatch):
     sp = Species([-1.0, 1.0], [0.01, 1.0], [0.05, 0.05])
     st = PicState(mesh, E, B, sp, ctx=gpu_ctx)
     st.generate_particles(mesh.ncells * 8, [0.3, 0.3], seed=3)
-    monkeypatch.setattr(autotune, "DEPOSIT_VARIANTS", ((3, 4), (2, 4)))
+    monkeypatch.setattr(autotune, "DEPOSIT_VARIANTS", ((1, 4), (2, 4)))
     opts = dict(mover=L.MOVER_LAPENTA, bc_mode=L.BC_REFLECT, r_sphere=0.2)
     try:
         best, cached = autotune.tune_pic(gpu_ctx, st, opts, 0.05, "test", cache=None)
         print(f"tuned: {best}", flush=True)
-        assert not cached and best["deposit_kernel"] in (2, 3)
+        assert not cached and best["deposit_kernel"] in (1, 2)
         dep = best["deposit_ms"]
-        if dep["k3c4"] < dep["k2c4"]:
+        if dep["k1c4"] < dep["k2c4"]:
             steps = best["step_ms"]
-            assert set(steps) == {"k3c4", "k2c4"}
-            assert best["deposit_kernel"] == (2 if steps["k2c4"] <= steps["k3c4"] else 3)
+            assert set(steps) == {"k1c4", "k2c4"}
+            assert best["deposit_kernel"] == (2 if steps["k2c4"] <= steps["k1c4"] else 1)
         else:
             assert best["deposit_kernel"] == 2 and not best["step_ms"]
         n0 = st.particle_count()
         st.run_steps(3, 0.05, **opts)
         assert 0 < st.particle_count() <= n0
         # the whole-step comparison itself
-        var, steps = autotune.weigh_steps(gpu_ctx, st, opts, 0.05, ((3, 4), (2, 4)))
+        var, steps = autotune.weigh_steps(gpu_ctx, st, opts, 0.05, ((1, 4), (2, 4)))
         print(f"steps: {steps}", flush=True)
-        assert set(steps) == {(3, 4), (2, 4)} and all(t > 0 for t in steps.values())
+        assert set(steps) == {(1, 4), (2, 4)} and all(t > 0 for t in steps.values())
         assert var == min(steps, key=steps.get)
     finally:  # the session context's defaults
         autotune.set_deposit_variant(gpu_ctx, 2, 12)
