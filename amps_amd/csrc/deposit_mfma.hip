@@ -65,14 +65,16 @@ __device__ __forceinline__ int corner_of(int bx, int by, int bz) { return 4 * bz
 template <bool PAIRED>
 constexpr int waves_of() { return PAIRED ? 8 : 4; }
 
-template <bool BC, bool PAIRED, bool FUSED>
+// VF32: FP32 velocities (widened when the record is built, not at the load:
+// a conversion there would wait for the prefetched data)
+template <bool BC, bool PAIRED, bool FUSED, bool VF32>
 __global__ __launch_bounds__(64 * waves_of<PAIRED>()) void k_deposit_cell_mfma(MeshView m, Species sp, double dtTotal,
                                                                  const double* __restrict__ B,
                                                                  const int* __restrict__ offset,
                                                                  int64_t cap,
                                                                  const double* __restrict__ x,
                                                                  const double* __restrict__ v,
-                                                                 int vf32,
+                                                                 int /*vf32*/,
                                                                  const uint8_t* __restrict__ spec,
                                                                  double* __restrict__ cellpart,
                                                                  const double* __restrict__ Bc,
@@ -189,6 +191,7 @@ __global__ __launch_bounds__(64 * waves_of<PAIRED>()) void k_deposit_cell_mfma(M
   // each lane's particles of the next two chunks, loaded ahead
   struct Pre {
     double x[3], v[3];
+    float vf[3];
     int s, c;
   };
   // lane k < NCW: where cell k starts in the concatenation, and the offset
@@ -213,16 +216,16 @@ __global__ __launch_bounds__(64 * waves_of<PAIRED>()) void k_deposit_cell_mfma(M
       // them, particle p of the new order at src[p]
       const int q = FUSED ? src[p] : p;
       n.x[0] = x[q], n.x[1] = x[cap + q], n.x[2] = x[2 * cap + q];
-      if (vf32) {
+      if constexpr (VF32) {
         const float* vf = (const float*)v;
-        n.v[0] = vf[q], n.v[1] = vf[cap + q], n.v[2] = vf[2 * cap + q];
+        n.vf[0] = vf[q], n.vf[1] = vf[cap + q], n.vf[2] = vf[2 * cap + q];
       } else {
         n.v[0] = v[q], n.v[1] = v[cap + q], n.v[2] = v[2 * cap + q];
       }
       n.s = spec[q];
     }
   };
-  Pre n1 = {{0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}, 0, 0}, n2 = n1;
+  Pre n1 = {{0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}, {0.f, 0.f, 0.f}, 0, 0}, n2 = n1;
 
   // the two 16 x 16 halves of the cell's 16 x 32 tile
   dbl4 C0 = {0.0, 0.0, 0.0, 0.0}, C1 = C0;
@@ -258,7 +261,8 @@ __global__ __launch_bounds__(64 * waves_of<PAIRED>()) void k_deposit_cell_mfma(M
       // the compiler finds them (this pass is the tolerance-bound one)
 #pragma clang fp contract(fast)
       const double xI[3] = {nb.x[0], nb.x[1], nb.x[2]};
-      const double vI[3] = {nb.v[0], nb.v[1], nb.v[2]};
+      const double vI[3] = {VF32 ? (double)nb.vf[0] : nb.v[0], VF32 ? (double)nb.vf[1] : nb.v[1],
+                            VF32 ? (double)nb.vf[2] : nb.v[2]};
       const int s = nb.s;
       const int nc = nb.c;
       const int iX[3] = {sx + (nc & 3), sy + row0 + (nc >> 2), zc};
@@ -457,16 +461,21 @@ __global__ __launch_bounds__(64 * waves_of<PAIRED>()) void k_deposit_cell_mfma(M
   }
 }
 
-template <bool BC, bool PAIRED, bool FUSED>
+template <bool BC, bool PAIRED, bool FUSED, bool VF32>
 void launch(amps_ctx* c, double dt) {
   const int k = c->cur;
   // FUSED: the particles are read from the previous buffer through the
   // migration's source indices
   const int r = FUSED ? 1 - k : k;
-  hipLaunchKernelGGL((k_deposit_cell_mfma<BC, PAIRED, FUSED>), dim3((unsigned)c->mv.nb * 8),
+  hipLaunchKernelGGL((k_deposit_cell_mfma<BC, PAIRED, FUSED, VF32>), dim3((unsigned)c->mv.nb * 8),
                      dim3(64 * waves_of<PAIRED>()), 0, c->stream, c->mv, c->sp, dt, c->d_B, c->d_offset, c->cap,
                      c->d_x[r], c->d_v[r], c->precision == AMPS_FP32 ? 1 : 0, c->d_spec[r], c->d_cellpart, c->d_Bc,
                      c->d_tmpsrc);
+}
+
+template <bool BC, bool PAIRED, bool FUSED>
+void launch_v(amps_ctx* c, double dt, bool vf32) {
+  vf32 ? launch<BC, PAIRED, FUSED, true>(c, dt) : launch<BC, PAIRED, FUSED, false>(c, dt);
 }
 
 }  // namespace
@@ -474,20 +483,18 @@ void launch(amps_ctx* c, double dt) {
 namespace amps {
 int launch_deposit_cell_mfma(amps_ctx* c, double dt) {
   const bool paired = c->dep_paired != 0;
+  const bool vf32 = c->precision == AMPS_FP32;
   if (c->pending_gather) {
     // the records of the last compaction are still where it found them: the
     // pass reads them there (set only with the single-phase pass)
     if (paired) return fail(AMPS_E_STATE, "deposit: records left by a step's migration need the single-phase cell pass");
-    if (c->b_center)
-      launch<true, false, true>(c, dt);
-    else
-      launch<false, false, true>(c, dt);
+    c->b_center ? launch_v<true, false, true>(c, dt, vf32) : launch_v<false, false, true>(c, dt, vf32);
     return 0;
   }
   if (c->b_center)
-    paired ? launch<true, true, false>(c, dt) : launch<true, false, false>(c, dt);
+    paired ? launch_v<true, true, false>(c, dt, vf32) : launch_v<true, false, false>(c, dt, vf32);
   else
-    paired ? launch<false, true, false>(c, dt) : launch<false, false, false>(c, dt);
+    paired ? launch_v<false, true, false>(c, dt, vf32) : launch_v<false, false, false>(c, dt, vf32);
   return 0;
 }
 }  // namespace amps
