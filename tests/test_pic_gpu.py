@@ -493,16 +493,19 @@ def test_generator_and_graph_steps_match_eager(gpu_ctx):
     assert np.array_equal(Ja, Jb) and np.array_equal(Ma, Mb)
 
 
-def _record_move_runs(ctx, mesh, E, B, sp, dt, precision):
+def _record_move_runs(ctx, mesh, E, B, sp, dt, precision, B_center=None):
     """Three steps three ways, one after the other (the states share the
     device context): one run_steps call per step (True), separate
     MoveParticles(compact) and UpdateJMassMatrix calls (False), one
     graph-captured run_steps call ("run"); the arrays and the mass term and
-    right-hand side over the resident J and M after each step."""
+    right-hand side over the resident J and M after each step.  B_center,
+    when given, is set as each state's cell-centred B."""
     rng = np.random.default_rng(3)
     res = {}
     for fused in (True, False, "run"):
         st = PicState(mesh, E, B, sp, ctx=ctx)
+        if B_center is not None:
+            st.set_center_fields(B_center)
         st.generate_particles(mesh.ncells * 3 + 777, [0.3, 0.3], drift=(0.01, 0.0, 0.0), seed=5,
                               precision=precision)
         if fused is True:
@@ -554,6 +557,34 @@ def test_step_deposit_moves_the_migrated_records(gpu_ctx, precision, kernel):
         res = _record_move_runs(gpu_ctx, mesh, E, B, sp, dt, precision)
     finally:  # the session context's default
         autotune.set_deposit_variant(gpu_ctx, 2, 12)
+    _check_record_move(res, kernel)
+
+
+@pytest.mark.parametrize("kernel", [2, 3])
+def test_step_deposit_moves_the_migrated_records_center_B(gpu_ctx, kernel):
+    """The same with the cell-centred B (FP64 push and cell pass reading the
+    centre field): a step's cell pass and the next push read the particles
+    through the source indices and match the separate calls bit for bit."""
+    from amps_amd import autotune
+
+    mesh = M.uniform_mesh(1)
+    E, B = M.corner_fields(mesh)
+    Bc = M.center_fields(mesh, lambda P: M.dipole_field(P))
+    dt = 0.05
+    sp = Species([-1.0, 1.0], [0.01, 1.0], [dt, dt])
+    autotune.set_deposit_variant(gpu_ctx, kernel, 4)
+    gpu_ctx.set_param("b_center", 1)
+    try:
+        res = _record_move_runs(gpu_ctx, mesh, E, B, sp, dt, L.FP64, B_center=Bc)
+    finally:  # the session context's defaults
+        gpu_ctx.set_param("b_center", 0)
+        autotune.set_deposit_variant(gpu_ctx, 2, 12)
+    _check_record_move(res, kernel)
+
+
+def _check_record_move(res, kernel):
+    """Bit-equal arrays and J/M terms after each step; more than a tenth of
+    the particles change cells in each step after the first."""
     prev = None
     for step, ((a, ya, ra), (b, yb, rb)) in enumerate(zip(res[True], res[False])):
         key = dict(zip(a.id.tolist(), (a.blk.astype(np.int64) * 512 + a.cell).tolist()))
