@@ -30,14 +30,13 @@ __global__ void k_scatter(const int*, const int*, const int8_t*, const int*, int
 __global__ void k_copy_total(const int*, int*);
 __global__ void k_mig_classify(const int*, const int*, const int8_t*, const int*, int, const int*, int*, int*, int*,
                                uint64_t*);
-__global__ void k_mig_order(int64_t, int, const int*, const int*, int*, int*, int*, int, const int*, const int*,
-                            const uint64_t*, int*);
-__global__ void k_mig_order_big(int64_t, const int*, const int*, int*, int*, int*, const int*);
+__global__ void k_mig_order(int64_t, int, int, const int*, const int*, int*, int*, int*, int, const int*,
+                            const int*, const uint64_t*, int*);
+__global__ void k_mig_order_big(int64_t, int, const int*, const int*, int*, int*, int*, const int*);
 __global__ void k_mig_scatter(const int*, const int*, const int8_t*, const int*, const int*, const uint64_t*,
                               const int*, int64_t, const double*, const double*, int, const uint8_t*,
                               const int64_t*, double*, double*, uint8_t*, int*, int64_t*);
-__global__ void k_mig_dest(const int*, const int*, const int8_t*, const int*, const int*, const uint64_t*,
-                           const int*, int*);
+__global__ void k_mig_dest(const int*, const int*, const int*, const int*, const uint64_t*, int*);
 __global__ void k_mig_move(const int*, const int*, int64_t, const double*, const double*, int, const uint8_t*,
                            const int64_t*, const int*, double*, double*, uint8_t*, int64_t*, int*);
 __global__ void k_copy_ints(int64_t, const int*, int*);
@@ -834,15 +833,18 @@ static int compact_impl(amps_ctx* c, bool counted = false, bool defer = false) {
                        c->d_wmask);
     if (int r = scan_exclusive(c, c->d_wcnt, nw + 1, nullptr)) return r;
     hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, c->stream, c->d_big + nkeys, 1, 0);
-    hipLaunchKernelGGL(k_mig_order, dim3(gridk), dim3(256), 0, c->stream, nkeys, c->mig_small, c->d_offset,
-                       c->d_cursor, c->d_tmpsrc, c->d_dest, c->d_big, c->layout_valid ? 1 : 0, c->d_off_layout,
-                       c->d_wcnt, c->d_wmask, c->d_kbase);
+    // the movers' destinations in dest[] only for the scatter; in a deferred
+    // move their source indices are the sorted lists, in place in tmpsrc
+    const int write_dest = defer ? 0 : 1;
+    hipLaunchKernelGGL(k_mig_order, dim3(gridk), dim3(256), 0, c->stream, nkeys, c->mig_small, write_dest,
+                       c->d_offset, c->d_cursor, c->d_tmpsrc, c->d_dest, c->d_big, c->layout_valid ? 1 : 0,
+                       c->d_off_layout, c->d_wcnt, c->d_wmask, c->d_kbase);
     // spare array of the merge sort: the other buffer's keys, written later
-    hipLaunchKernelGGL(k_mig_order_big, dim3(512), dim3(256), 0, c->stream, nkeys, c->d_offset, c->d_cursor,
-                       c->d_tmpsrc, c->d_key[b], c->d_dest, c->d_big);
+    hipLaunchKernelGGL(k_mig_order_big, dim3(512), dim3(256), 0, c->stream, nkeys, write_dest, c->d_offset,
+                       c->d_cursor, c->d_tmpsrc, c->d_key[b], c->d_dest, c->d_big);
     if (defer) {
-      hipLaunchKernelGGL(k_mig_dest, dim3(gridp), dim3(256), 0, c->stream, c->d_n, c->d_key[a], c->d_code,
-                         c->d_kbase, c->d_wcnt, c->d_wmask, c->d_dest, c->d_tmpsrc);
+      hipLaunchKernelGGL(k_mig_dest, dim3(gridp), dim3(256), 0, c->stream, c->d_n, c->d_key[a], c->d_kbase,
+                         c->d_wcnt, c->d_wmask, c->d_tmpsrc);
       c->pending_gather = true;
     } else {
       hipLaunchKernelGGL(k_mig_scatter, dim3(gridp), dim3(256), 0, c->stream, c->d_n, c->d_key[a], c->d_code,

@@ -282,10 +282,13 @@ __global__ __launch_bounds__(256) void k_scatter(const int* __restrict__ n_dev,
 // then the particles that moved in (movers), in array order, whatever order
 // the atomics land in:
 //   classify: one stayer mask and count per wavefront; each mover lists its
-//             position in its new key's region of tmpsrc (atomic order);
-//   order:    per key, the movers' positions sorted, their destinations
-//             written to dest[] (one lane per key, a workgroup merge sort for
-//             keys with many movers);
+//             position at the end of its new key's region of tmpsrc (atomic
+//             order), where the key's movers go;
+//   order:    per key, the movers' positions sorted in place (one lane per
+//             key, a workgroup merge sort for keys with many movers), which
+//             makes tmpsrc over the movers' destinations the source index of
+//             each; when the records are moved by scatter, each mover's
+//             destination is also written to dest[];
 //   scatter:  a stayer's destination is its key's offset plus the stayers
 //             between its key's old run start and itself, from the scanned
 //             wavefront counts and masks; a mover's is dest[p].
@@ -310,14 +313,16 @@ __global__ __launch_bounds__(256) void k_mig_classify(const int* __restrict__ n_
     wcnt[p >> 6] = __popcll(m);
     wmask[p >> 6] = m;
   }
-  if (mover) tmpsrc[(int64_t)offset[k] + base + (__lane_id() - ru.head_lane)] = (int)p;
+  if (mover) tmpsrc[(int64_t)offset[k + 1] - 1 - (base + (__lane_id() - ru.head_lane))] = (int)p;
 }
 
-// per key: the movers' source positions (cursor[k] of them from offset[k] in
-// tmpsrc) in ascending order, destinations after the key's stayers.  Up to
-// `small` (at most 64) are sorted by the key's own lane; keys with more are
-// listed in big[] (count in big[nkeys]) for k_mig_order_big.
-__global__ __launch_bounds__(256) void k_mig_order(int64_t nkeys, int small, const int* __restrict__ offset,
+// per key: the movers' source positions (the last cursor[k] entries of the
+// key's region of tmpsrc, its movers' destinations) in ascending order, and
+// with write_dest each mover's destination in dest[].  Up to `small` (at most
+// 64) are sorted by the key's own lane; keys with more are listed in big[]
+// (count in big[nkeys]) for k_mig_order_big.
+__global__ __launch_bounds__(256) void k_mig_order(int64_t nkeys, int small, int write_dest,
+                                                    const int* __restrict__ offset,
                                                     const int* __restrict__ cursor, int* __restrict__ tmpsrc,
                                                     int* __restrict__ dest, int* __restrict__ big, int has_layout,
                                                     const int* __restrict__ lay, const int* __restrict__ wscan,
@@ -337,8 +342,8 @@ __global__ __launch_bounds__(256) void k_mig_order(int64_t nkeys, int small, con
     big[atomicAdd(&big[nkeys], 1)] = (int)k;
     return;
   }
-  int* q = tmpsrc + offset[k];
   const int d0 = offset[k + 1] - m;
+  int* q = tmpsrc + d0;
   for (int i = 1; i < m; i++) {
     const int t = q[i];
     int j = i - 1;
@@ -348,7 +353,8 @@ __global__ __launch_bounds__(256) void k_mig_order(int64_t nkeys, int small, con
     }
     q[j + 1] = t;
   }
-  for (int i = 0; i < m; i++) dest[q[i]] = d0 + i;
+  if (write_dest)
+    for (int i = 0; i < m; i++) dest[q[i]] = d0 + i;
 }
 
 // number of entries of the ascending run r[0, n) below t
@@ -367,16 +373,18 @@ __device__ __forceinline__ int count_below(const int* r, int n, int t) {
 // the keys with many movers, one workgroup each (a grid-stride loop over the
 // list): a bottom-up merge sort of the distinct source positions, every
 // element placed by a binary search in the partner run, ping-ponging between
-// tmpsrc and a spare array of the same size
-__global__ __launch_bounds__(256) void k_mig_order_big(int64_t nkeys, const int* __restrict__ offset,
+// tmpsrc and a spare array of the same size; the sorted run ends in tmpsrc
+__global__ __launch_bounds__(256) void k_mig_order_big(int64_t nkeys, int write_dest,
+                                                        const int* __restrict__ offset,
                                                         const int* __restrict__ cursor, int* tmpsrc, int* spare,
                                                         int* __restrict__ dest, const int* __restrict__ big) {
   const int nbig = big[nkeys];
   for (int e = blockIdx.x; e < nbig; e += gridDim.x) {
     const int k = big[e];
     const int m = cursor[k];
-    int* src = tmpsrc + offset[k];
-    int* dst = spare + offset[k];
+    const int d0 = offset[k + 1] - m;
+    int* src = tmpsrc + d0;
+    int* dst = spare + d0;
     for (int w = 1; w < m; w *= 2) {
       for (int i = threadIdx.x; i < m; i += blockDim.x) {
         const int r = i / w, s = (r >> 1) * 2 * w;
@@ -395,8 +403,11 @@ __global__ __launch_bounds__(256) void k_mig_order_big(int64_t nkeys, const int*
       src = dst;
       dst = tmp;
     }
-    const int d0 = offset[k + 1] - m;
-    for (int i = threadIdx.x; i < m; i += blockDim.x) dest[src[i]] = d0 + i;
+    for (int i = threadIdx.x; i < m; i += blockDim.x) {
+      const int t = src[i];
+      if (src != tmpsrc + d0) tmpsrc[d0 + i] = t;
+      if (write_dest) dest[t] = d0 + i;
+    }
     __syncthreads();
   }
 }
@@ -417,17 +428,21 @@ __device__ __forceinline__ int64_t mig_dest(int64_t p, int k, const int* __restr
 // The same destinations without moving the records: src[d] = p, for a
 // deposit cell pass that reads the particles where they are and a push that
 // moves them (the push writes every surviving particle's key; the keys stay
-// in the previous arrays until then, where k_mig_move finds them).
+// in the previous arrays until then, where k_mig_move finds them).  The
+// movers' entries of src are the sorted lists k_mig_order left in place, so
+// only the stayers are written here.
 __global__ __launch_bounds__(256) void k_mig_dest(const int* __restrict__ n_dev, const int* __restrict__ key,
-                                                  const int8_t* __restrict__ code, const int* __restrict__ kbase,
-                                                  const int* __restrict__ wscan, const uint64_t* __restrict__ wmask,
-                                                  const int* __restrict__ dest, int* __restrict__ src) {
+                                                  const int* __restrict__ kbase, const int* __restrict__ wscan,
+                                                  const uint64_t* __restrict__ wmask, int* __restrict__ src) {
   const int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   const int n = *n_dev;
-  const bool valid = (p < n) && alive_code(code[p]);
-  if (!valid) return;
+  if (p >= n) return;
+  const int lane = (int)(p & 63);
+  const uint64_t mw = wmask[p >> 6];
+  // a stayer is alive and kept its key
+  if (!((mw >> lane) & 1ull)) return;
   const int k = key[p];
-  src[mig_dest(p, k, kbase, wscan, wmask, dest)] = (int)p;
+  src[(int64_t)kbase[k] + wscan[p >> 6] + __popcll(mw & ((1ull << lane) - 1ull))] = (int)p;
 }
 
 // The record move that k_mig_dest left: particle d of the new order from
