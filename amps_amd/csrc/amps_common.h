@@ -75,6 +75,25 @@ struct PushSrc {
   int64_t* id_out;
 };
 
+// The migration's classification made by the push (a deterministic step
+// whose arrays have the layout of a previous compaction; lay null: none).
+// lay[] holds that layout's key offsets.  A survivor whose new key's old run
+// holds it (a stayer) sets its bit of the 64-aligned stayer mask word wmask
+// and counts in wcnt and in the push's count[]; a survivor that changes keys
+// (a mover) counts in cursor[] and takes its rank among its new key's movers
+// from that atomic, and is listed at slab[key * MIG_SLAB + rank], or, past
+// the slab, as (position, rank) in ovf[] (novf entries).
+constexpr int MIG_SLAB = 16;
+struct PushMig {
+  const int* lay;
+  int* cursor;
+  int* slab;
+  int2* ovf;
+  int* novf;
+  uint64_t* wmask;
+  int* wcnt;
+};
+
 // Synthetic-population generator arguments (k_generate).
 struct GenArgs {
   int64_t ntotal, base, rem;
@@ -178,6 +197,10 @@ struct amps_ctx {
   int* d_tmpsrc = nullptr;      // [cap]
   int* d_wcnt = nullptr;        // [ceil(cap / 256) * 4 + 1]
   uint64_t* d_wmask = nullptr;  // [ceil(cap / 256) * 4]
+  // the push's classification (PushMig): movers listed per key, the rest
+  // after the slab
+  int* d_mig_slab = nullptr;    // [nkeys * MIG_SLAB + 1], the overflow count last
+  int2* d_mig_ovf = nullptr;    // [cap]
   bool layout_valid = false;
   // the last compaction left the particle records in the other buffer: the
   // destination's source indices are in d_tmpsrc.  The step's matrix-core

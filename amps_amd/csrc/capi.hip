@@ -17,11 +17,11 @@ using namespace amps;
 template <int MOVER, bool BC, int W>
 __global__ void k_push_f64(MeshView, Faces, Species, const double*, const double*, const Tile*,
                            const int*, int, double, int64_t, double*, double*, const uint8_t*,
-                           int*, int8_t*, const double*, int*, PushSrc);
+                           int*, int8_t*, const double*, int*, PushSrc, PushMig);
 template <int MOVER, int W>
 __global__ void k_push_f32(MeshView, Faces, Species, const double*, const double*, const Tile*,
                            const int*, int, double, int64_t, double*, float*, const uint8_t*, int*,
-                           int8_t*, int*, PushSrc);
+                           int8_t*, int*, PushSrc, PushMig);
 __global__ void k_fill_int(int*, int64_t, int);
 __global__ void k_count_keys(const int*, const int*, const int8_t*, int*);
 __global__ void k_scatter(const int*, const int*, const int8_t*, const int*, int*, int64_t,
@@ -32,11 +32,15 @@ __global__ void k_mig_classify(const int*, const int*, const int8_t*, const int*
                                uint64_t*);
 __global__ void k_mig_order(int64_t, int, int, const int*, const int*, int*, int*, int*, int, const int*,
                             const int*, const uint64_t*, int*);
+__global__ void k_mig_order_slab(int64_t, int, int, const int*, const int*, const int*, int*, int*, int*,
+                                 const int*, const int*, const uint64_t*, int*);
+__global__ void k_mig_place(const int*, const int2*, const int*, const int*, int*);
+__global__ void k_add_ints(int64_t, int*, const int*);
 __global__ void k_mig_order_big(int64_t, int, const int*, const int*, int*, int*, int*, const int*);
 __global__ void k_mig_scatter(const int*, const int*, const int8_t*, const int*, const int*, const uint64_t*,
                               const int*, int64_t, const double*, const double*, int, const uint8_t*,
                               const int64_t*, double*, double*, uint8_t*, int*, int64_t*);
-__global__ void k_mig_dest(const int*, const int*, const int*, const int*, const uint64_t*, int*);
+__global__ void k_mig_dest(const int*, const int*, const int*, const int*, const uint64_t*, int64_t, int*);
 __global__ void k_mig_move(const int*, const int*, int64_t, const double*, const double*, int, const uint8_t*,
                            const int64_t*, const int*, double*, double*, uint8_t*, int64_t*, int*);
 __global__ void k_copy_ints(int64_t, const int*, int*);
@@ -102,8 +106,10 @@ static void free_particles(amps_ctx* c) {
   (void)hipFree(c->d_tmpsrc);
   (void)hipFree(c->d_wcnt);
   (void)hipFree(c->d_wmask);
+  (void)hipFree(c->d_mig_ovf);
   c->d_dest = c->d_tmpsrc = c->d_wcnt = nullptr;
   c->d_wmask = nullptr;
+  c->d_mig_ovf = nullptr;
   c->layout_valid = false;
   c->cap = 0;
   c->n = 0;
@@ -134,7 +140,8 @@ static void free_mesh(amps_ctx* c) {
   (void)hipFree(c->d_off_layout);
   (void)hipFree(c->d_big);
   (void)hipFree(c->d_kbase);
-  c->d_off_layout = c->d_big = c->d_kbase = nullptr;
+  (void)hipFree(c->d_mig_slab);
+  c->d_off_layout = c->d_big = c->d_kbase = c->d_mig_slab = nullptr;
   c->layout_valid = false;
   (void)hipFree(c->d_cursor);
   (void)hipFree(c->d_tcount);
@@ -496,6 +503,7 @@ extern "C" int amps_gpu_set_mesh(amps_ctx* c, const amps_mesh_desc* desc, const 
   HIP_TRY(hipMalloc(&c->d_big, sizeof(int) * (nkeys + 1)));
   HIP_TRY(hipMalloc(&c->d_kbase, sizeof(int) * std::max<int64_t>(nkeys, 1)));
   HIP_TRY(hipMalloc(&c->d_cursor, sizeof(int) * nkeys));
+  HIP_TRY(hipMalloc(&c->d_mig_slab, sizeof(int) * (nkeys * MIG_SLAB + 1)));
   c->layout_valid = false;
   HIP_TRY(hipMalloc(&c->d_tcount, sizeof(int) * (nb + 1)));
   m.blk_xmin = c->d_xmin;
@@ -652,6 +660,7 @@ int amps::alloc_particles(amps_ctx* c, int64_t n) {
     HIP_TRY(hipMalloc(&c->d_wcnt, sizeof(int) * (nw + 1)));
     HIP_TRY(hipMalloc(&c->d_wmask, sizeof(uint64_t) * nw));
   }
+  HIP_TRY(hipMalloc(&c->d_mig_ovf, sizeof(int2) * cap));
   c->cap = cap;
   c->layout_valid = false;
   return 0;
@@ -810,7 +819,10 @@ static void clear_counts(amps_ctx* c) {
 // (pending_gather): the step's matrix-core cell pass reads the particles
 // through them and the next push moves them as it updates them, so a step
 // writes the particle state once instead of twice.
-static int compact_impl(amps_ctx* c, bool counted = false, bool defer = false) {
+// classified: the push also classified the survivors (PushMig): d_offset
+// holds the stayers per key, d_cursor the movers, d_wcnt and d_wmask the
+// stayer counts and masks, the slab and overflow list the movers
+static int compact_impl(amps_ctx* c, bool counted = false, bool defer = false, bool classified = false) {
   const int64_t nkeys = (int64_t)c->mv.nb * NCELL;
   const int64_t cap = c->cap;
   const unsigned gridp = (unsigned)((cap + 255) / 256);
@@ -820,6 +832,8 @@ static int compact_impl(amps_ctx* c, bool counted = false, bool defer = false) {
     hipLaunchKernelGGL(k_count_keys, dim3(gridp), dim3(256), 0, c->stream, c->d_n, c->d_key[a],
                        c->d_code, c->d_offset);
   }
+  if (classified)
+    hipLaunchKernelGGL(k_add_ints, dim3(1024), dim3(256), 0, c->stream, nkeys, c->d_offset, c->d_cursor);
   if (int r = scan_exclusive(c, c->d_offset, nkeys + 1, nullptr)) return r;
   const int vf32 = c->precision == AMPS_FP32 ? 1 : 0;
   if (c->deterministic) {
@@ -828,23 +842,32 @@ static int compact_impl(amps_ctx* c, bool counted = false, bool defer = false) {
     // layout of a previous compaction yet, e.g. a generated population)
     const unsigned gridk = (unsigned)((nkeys + 255) / 256);
     const int64_t nw = (int64_t)gridp * 4;
-    hipLaunchKernelGGL(k_mig_classify, dim3(gridp), dim3(256), 0, c->stream, c->d_n, c->d_key[a], c->d_code,
-                       c->d_off_layout, c->layout_valid ? 1 : 0, c->d_offset, c->d_cursor, c->d_tmpsrc, c->d_wcnt,
-                       c->d_wmask);
+    if (!classified)
+      hipLaunchKernelGGL(k_mig_classify, dim3(gridp), dim3(256), 0, c->stream, c->d_n, c->d_key[a], c->d_code,
+                         c->d_off_layout, c->layout_valid ? 1 : 0, c->d_offset, c->d_cursor, c->d_tmpsrc,
+                         c->d_wcnt, c->d_wmask);
     if (int r = scan_exclusive(c, c->d_wcnt, nw + 1, nullptr)) return r;
+    if (classified)
+      hipLaunchKernelGGL(k_mig_place, dim3(1024), dim3(256), 0, c->stream, c->d_mig_slab + nkeys * MIG_SLAB,
+                         c->d_mig_ovf, c->d_key[a], c->d_offset, c->d_tmpsrc);
     hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, c->stream, c->d_big + nkeys, 1, 0);
     // the movers' destinations in dest[] only for the scatter; in a deferred
     // move their source indices are the sorted lists, in place in tmpsrc
     const int write_dest = defer ? 0 : 1;
-    hipLaunchKernelGGL(k_mig_order, dim3(gridk), dim3(256), 0, c->stream, nkeys, c->mig_small, write_dest,
-                       c->d_offset, c->d_cursor, c->d_tmpsrc, c->d_dest, c->d_big, c->layout_valid ? 1 : 0,
-                       c->d_off_layout, c->d_wcnt, c->d_wmask, c->d_kbase);
+    if (classified)
+      hipLaunchKernelGGL(k_mig_order_slab, dim3(gridk), dim3(256), 0, c->stream, nkeys, c->mig_small, write_dest,
+                         c->d_mig_slab, c->d_offset, c->d_cursor, c->d_tmpsrc, c->d_dest, c->d_big, c->d_off_layout,
+                         c->d_wcnt, c->d_wmask, c->d_kbase);
+    else
+      hipLaunchKernelGGL(k_mig_order, dim3(gridk), dim3(256), 0, c->stream, nkeys, c->mig_small, write_dest,
+                         c->d_offset, c->d_cursor, c->d_tmpsrc, c->d_dest, c->d_big, c->layout_valid ? 1 : 0,
+                         c->d_off_layout, c->d_wcnt, c->d_wmask, c->d_kbase);
     // spare array of the merge sort: the other buffer's keys, written later
     hipLaunchKernelGGL(k_mig_order_big, dim3(512), dim3(256), 0, c->stream, nkeys, write_dest, c->d_offset,
                        c->d_cursor, c->d_tmpsrc, c->d_key[b], c->d_dest, c->d_big);
     if (defer) {
-      hipLaunchKernelGGL(k_mig_dest, dim3(gridp), dim3(256), 0, c->stream, c->d_n, c->d_key[a], c->d_kbase,
-                         c->d_wcnt, c->d_wmask, c->d_tmpsrc);
+      hipLaunchKernelGGL(k_mig_dest, dim3((unsigned)((cap + 1023) / 1024)), dim3(256), 0, c->stream, c->d_n,
+                         c->d_key[a], c->d_kbase, c->d_wcnt, c->d_wmask, cap, c->d_tmpsrc);
       c->pending_gather = true;
     } else {
       hipLaunchKernelGGL(k_mig_scatter, dim3(gridp), dim3(256), 0, c->stream, c->d_n, c->d_key[a], c->d_code,
@@ -899,9 +922,17 @@ static int check_mover(const amps_ctx* c, const amps_push_opts* o, const char* w
 }
 
 // count: when non-null, the push also counts the surviving particles per key
-// (the migration's counting pass), into a zeroed array
-static int launch_push(amps_ctx* c, const amps_push_opts* o, int* count = nullptr) {
+// (the migration's counting pass), into a zeroed array; classify: it also
+// classifies them for the deterministic migration (the stayers alone in
+// count, see PushMig), into zeroed d_cursor, d_wcnt, d_wmask and overflow count
+static int launch_push(amps_ctx* c, const amps_push_opts* o, int* count = nullptr, bool classify = false) {
   const int k = c->cur;
+  PushMig pm = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+  if (classify) {
+    const int64_t nkeys = (int64_t)c->mv.nb * NCELL;
+    pm = {c->d_off_layout, c->d_cursor, c->d_mig_slab, c->d_mig_ovf, c->d_mig_slab + nkeys * MIG_SLAB,
+          c->d_wmask, c->d_wcnt};
+  }
   const bool w4 = c->push_waves == 4;
   // records the last compaction left in the other buffer: the push reads them
   // there and writes them to their places
@@ -914,7 +945,7 @@ static int launch_push(amps_ctx* c, const amps_push_opts* o, int* count = nullpt
 #define LAUNCH_PUSH32(MV, W)                                                                              \
   hipLaunchKernelGGL((k_push_f32<MV, W>), dim3((unsigned)c->max_tiles), dim3(256), 0, c->stream, c->mv,     \
                      c->faces, c->sp, c->d_E, c->d_B, c->d_tiles, c->d_ntiles, o->bc_mode, o->r_sphere, c->cap, \
-                     c->d_x[k], (float*)c->d_v[k], c->d_spec[k], c->d_key[k], c->d_code, count, ps)
+                     c->d_x[k], (float*)c->d_v[k], c->d_spec[k], c->d_key[k], c->d_code, count, ps, pm)
   if (c->precision == AMPS_FP32) {
     if (o->mover == AMPS_MOVER_LAPENTA) {
       if (w4) LAUNCH_PUSH32(AMPS_MOVER_LAPENTA, 4);
@@ -930,7 +961,7 @@ static int launch_push(amps_ctx* c, const amps_push_opts* o, int* count = nullpt
 #define LAUNCH_PUSH64(MV, BCV, W)                                                                       \
   hipLaunchKernelGGL((k_push_f64<MV, BCV, W>), dim3((unsigned)c->max_tiles), dim3(256), 0, c->stream, c->mv,  \
                      c->faces, c->sp, c->d_E, c->d_B, c->d_tiles, c->d_ntiles, o->bc_mode, o->r_sphere, c->cap, \
-                     c->d_x[k], c->d_v[k], c->d_spec[k], c->d_key[k], c->d_code, c->d_Bc, count, ps)
+                     c->d_x[k], c->d_v[k], c->d_spec[k], c->d_key[k], c->d_code, c->d_Bc, count, ps, pm)
   const bool bc = c->b_center != 0;
   if (o->mover == AMPS_MOVER_LAPENTA) {
     if (bc) LAUNCH_PUSH64(AMPS_MOVER_LAPENTA, true, 1);
@@ -1157,10 +1188,20 @@ static int settle_records(amps_ctx* c) {
 }
 
 static int one_step(amps_ctx* c, const amps_push_opts* o, double dt) {
-  // the push counts the survivors per key for the migration that follows
+  // the push counts the survivors per key for the migration that follows,
+  // and for the deterministic one (the arrays in a compaction's layout)
+  // classifies them
+  const bool cls = c->deterministic && c->layout_valid;
   clear_counts(c);
+  if (cls) {
+    const int64_t nkeys = (int64_t)c->mv.nb * NCELL;
+    const int64_t nw = (c->cap + 255) / 256 * 4;
+    hipLaunchKernelGGL(k_fill_int, dim3(1024), dim3(256), 0, c->stream, c->d_wcnt, nw + 1, 0);
+    hipLaunchKernelGGL(k_fill_int, dim3(1024), dim3(256), 0, c->stream, (int*)c->d_wmask, 2 * nw, 0);
+    hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, c->stream, c->d_mig_slab + nkeys * MIG_SLAB, 1, 0);
+  }
   tbegin(c);
-  if (int r = launch_push(c, o, c->d_offset)) return r;
+  if (int r = launch_push(c, o, c->d_offset, cls)) return r;
   tend(c, AMPS_K_PUSH);
   if (o->bc_mode == AMPS_BC_USER)
     if (int r = collect_exits(c)) return r;
@@ -1170,7 +1211,7 @@ static int one_step(amps_ctx* c, const amps_push_opts* o, double dt) {
   const bool defer = c->deterministic && !c->dep_exact &&
                      ((c->dep_kernel == 2 && !c->dep_paired) || c->dep_kernel == 3);
   tbegin(c);
-  if (int r = compact_impl(c, true, defer)) return r;
+  if (int r = compact_impl(c, true, defer, cls)) return r;
   tend(c, AMPS_K_COMPACT);
   return launch_deposit(c, dt);
 }

@@ -47,6 +47,80 @@ __device__ __forceinline__ RunInfo wave_runs(bool valid, int k) {
   return r;
 }
 
+// A pass's movers of the push's classification, listed a pass later, when
+// their rank (the cursor atomic's return, on the run's head lane) has had the
+// next pass's loads to arrive.
+struct MigPend {
+  int raw;        // the atomic's return (head lanes)
+  int head_lane;  // the lane holding the run's return
+  int off;        // lane - head_lane: the rank within the run
+  int knew;
+  int p;
+  bool mover;
+};
+
+// the movers of a pass: each at its rank in its key's slab, or past it in
+// the overflow list (made by every lane of the wavefront)
+__device__ __forceinline__ void mig_list(const PushMig& pm, const MigPend& pd) {
+  const int lane = __lane_id();
+  const int rank = __shfl(pd.raw, pd.head_lane) + pd.off;
+  const bool ov = pd.mover && rank >= MIG_SLAB;
+  if (pd.mover && !ov) pm.slab[(int64_t)pd.knew * MIG_SLAB + rank] = pd.p;
+  const uint64_t om = __ballot(ov);
+  if (om) {
+    const int lead = __ffsll((long long)om) - 1;
+    int ob = 0;
+    if (lane == lead) ob = atomicAdd(pm.novf, __popcll(om));
+    ob = __shfl(ob, lead);
+    if (ov) pm.ovf[ob + __popcll(om & ((1ull << lane) - 1ull))] = make_int2(pd.p, rank);
+  }
+}
+
+// The push's count for the migration that follows, made by every lane of
+// the wavefront (p: the lane's position, start..p1 the tile's).  Without
+// pm.lay every survivor counts in count[knew], one atomic per run of equal
+// keys.  With it (the passes 64-aligned, so that a wavefront's pass is one
+// stayer mask word) the push classifies: a stayer (its new key's old run,
+// sLay[] of the tile's block, holds it) counts in count[] and sets its mask
+// bit, stored plainly where the word lies inside the tile and by atomics at
+// the tile's ends, which neighbouring tiles share; a mover takes its rank
+// among its new key's movers from the cursor[] atomic, and the previous
+// pass's movers are listed (mig_list; the last pass's after the loop).
+__device__ __forceinline__ void push_count(int* __restrict__ count, const PushMig& pm, const int* sLay, int b,
+                                           int p, int start, int p1, bool alive, int knew, MigPend& pd) {
+  if (!pm.lay) {
+    const RunInfo r = wave_runs(alive, knew);
+    if (r.head) atomicAdd(&count[knew], r.len);
+    return;
+  }
+  const int lane = __lane_id();
+  const int cb = knew - b * NCELL;
+  const bool stay = alive && cb >= 0 && cb < NCELL && p >= sLay[cb] && p < sLay[cb + 1];
+  const bool mover = alive && !stay;
+  const RunInfo rs = wave_runs(stay, knew);
+  if (rs.head) atomicAdd(&count[knew], rs.len);
+  const uint64_t sm = __ballot(stay);
+  if (lane == 0 && sm) {
+    const int64_t w = (int64_t)p >> 6;  // lane 0 holds the word's first position
+    if ((w << 6) >= start && (w << 6) + 64 <= p1) {
+      pm.wmask[w] = sm;
+      pm.wcnt[w] = __popcll(sm);
+    } else {
+      atomicOr((unsigned long long*)(pm.wmask + w), (unsigned long long)sm);
+      atomicAdd(pm.wcnt + w, __popcll(sm));
+    }
+  }
+  mig_list(pm, pd);
+  const RunInfo rm = wave_runs(mover, knew);
+  pd.raw = 0;
+  if (rm.head) pd.raw = atomicAdd(&pm.cursor[knew], rm.len);
+  pd.head_lane = rm.head_lane;
+  pd.off = lane - rm.head_lane;
+  pd.knew = knew;
+  pd.p = p;
+  pd.mover = mover;
+}
+
 // W: the occupancy the register allocation aims at (waves per SIMD).
 template <int MOVER, bool BC, int W>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W))) void k_push_f64(MeshView m, Faces faces, Species sp,
@@ -57,9 +131,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W))) void k
                                                    double r_sphere, int64_t cap, double* x,
                                                    double* v, const uint8_t* __restrict__ spec,
                                                    int* key, int8_t* code, const double* __restrict__ Bc,
-                                                   int* __restrict__ count, PushSrc ps) {
+                                                   int* __restrict__ count, PushSrc ps, PushMig pm) {
   __shared__ double sE[NCORNER * 3];
   __shared__ double sB[NCORNER * 3];
+  __shared__ int sLay[NCELL + 1];
   const int t = blockIdx.x;
   if (t >= *ntiles) return;
   const Tile tl = tiles[t];
@@ -70,6 +145,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W))) void k
     sE[i] = Eb[i];
     sB[i] = Bb[i];
   }
+  if (pm.lay)
+    for (int i = threadIdx.x; i <= NCELL; i += blockDim.x) sLay[i] = pm.lay[(int64_t)b * NCELL + i];
   __syncthreads();
   BlockGeom g;
   load_block_geom(m, b, g);
@@ -77,16 +154,20 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W))) void k
   load_block_lat(m, b, lat);
   const int p1 = tl.start + tl.count;
   // every lane runs the same number of passes, so that the wavefront's run
-  // aggregation of the migration count sees all 64 lanes.  Where the next
-  // pass reads (the last compaction's source index) is loaded a pass ahead.
-  int qn = (ps.src && tl.start + (int)threadIdx.x < p1) ? ps.src[tl.start + threadIdx.x] : 0;
-  for (int p0 = tl.start; p0 < p1; p0 += blockDim.x) {
+  // aggregation of the migration count sees all 64 lanes; the passes start
+  // 64-aligned when the push classifies.  Where the next pass reads (the last
+  // compaction's source index) is loaded a pass ahead.
+  const int pa = pm.lay ? (tl.start & ~63) : tl.start;
+  MigPend pd = {0, 0, 0, 0, 0, false};
+  const int pf = pa + (int)threadIdx.x;
+  int qn = (ps.src && pf >= tl.start && pf < p1) ? ps.src[pf] : 0;
+  for (int p0 = pa; p0 < p1; p0 += blockDim.x) {
     const int p = p0 + threadIdx.x;
     bool alive = false;
     int knew = -1;
     const int qs = qn;
     if (ps.src && p + (int)blockDim.x < p1) qn = ps.src[p + blockDim.x];
-    if (p < p1) {
+    if (p >= tl.start && p < p1) {
     // read in place, or where the last compaction left the particle
     const int q = ps.src ? qs : p;
     const double* xr = ps.src ? ps.x : x;
@@ -118,20 +199,17 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W))) void k
     if (alive || c >= AMPS_CODE_EXIT_FACE0) key[p] = knew;
     code[p] = (int8_t)c;
     }
-    // migration count of the particles that stay: one atomic per run of
-    // equal keys (count is null when the caller counts separately)
-    if (count) {
-      const RunInfo r = wave_runs(alive, knew);
-      if (r.head) atomicAdd(&count[knew], r.len);
-    }
+    // the migration's count (null when the caller counts separately)
+    if (count) push_count(count, pm, sLay, b, p, tl.start, p1, alive, knew, pd);
   }
+  if (count && pm.lay) mig_list(pm, pd);
 }
 
 #define INST_PUSH64(MV, BCV, W)                                                                  \
   template __global__ void k_push_f64<MV, BCV, W>(MeshView, Faces, Species, const double*, const double*, \
                                                   const Tile*, const int*, int, double, int64_t, double*, \
                                                   double*, const uint8_t*, int*, int8_t*, const double*, int*, \
-                                                  PushSrc);
+                                                  PushSrc, PushMig);
 INST_PUSH64(AMPS_MOVER_LAPENTA, false, 1)
 INST_PUSH64(AMPS_MOVER_BORIS, false, 1)
 INST_PUSH64(AMPS_MOVER_LAPENTA, false, 4)
@@ -150,9 +228,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W))) void k
                                                    const int* __restrict__ ntiles, int bc_mode,
                                                    double r_sphere, int64_t cap, double* x,
                                                    float* v, const uint8_t* __restrict__ spec,
-                                                   int* key, int8_t* code, int* __restrict__ count, PushSrc ps) {
+                                                   int* key, int8_t* code, int* __restrict__ count, PushSrc ps,
+                                                   PushMig pm) {
   __shared__ float sE[NCORNER * 3];
   __shared__ float sB[NCORNER * 3];
+  __shared__ int sLay[NCELL + 1];
   const int t = blockIdx.x;
   if (t >= *ntiles) return;
   const Tile tl = tiles[t];
@@ -163,20 +243,25 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W))) void k
     sE[i] = (float)Eb[i];
     sB[i] = (float)Bb[i];
   }
+  if (pm.lay)
+    for (int i = threadIdx.x; i <= NCELL; i += blockDim.x) sLay[i] = pm.lay[(int64_t)b * NCELL + i];
   __syncthreads();
   BlockGeom g;
   load_block_geom(m, b, g);
   BlockLat lat;
   load_block_lat(m, b, lat);
   const int p1 = tl.start + tl.count;
-  int qn = (ps.src && tl.start + (int)threadIdx.x < p1) ? ps.src[tl.start + threadIdx.x] : 0;
-  for (int p0 = tl.start; p0 < p1; p0 += blockDim.x) {
+  const int pa = pm.lay ? (tl.start & ~63) : tl.start;
+  MigPend pd = {0, 0, 0, 0, 0, false};
+  const int pf = pa + (int)threadIdx.x;
+  int qn = (ps.src && pf >= tl.start && pf < p1) ? ps.src[pf] : 0;
+  for (int p0 = pa; p0 < p1; p0 += blockDim.x) {
     const int p = p0 + threadIdx.x;
     bool alive = false;
     int knew = -1;
     const int qs = qn;
     if (ps.src && p + (int)blockDim.x < p1) qn = ps.src[p + blockDim.x];
-    if (p < p1) {
+    if (p >= tl.start && p < p1) {
     const int q = ps.src ? qs : p;
     const double* xr = ps.src ? ps.x : x;
     const float* vr = ps.src ? (const float*)ps.v : v;
@@ -202,17 +287,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W))) void k
     if (alive || c >= AMPS_CODE_EXIT_FACE0) key[p] = knew;
     code[p] = (int8_t)c;
     }
-    if (count) {
-      const RunInfo r = wave_runs(alive, knew);
-      if (r.head) atomicAdd(&count[knew], r.len);
-    }
+    if (count) push_count(count, pm, sLay, b, p, tl.start, p1, alive, knew, pd);
   }
+  if (count && pm.lay) mig_list(pm, pd);
 }
 
 #define INST_PUSH32(MV, W)                                                                        \
   template __global__ void k_push_f32<MV, W>(MeshView, Faces, Species, const double*, const double*, \
                                              const Tile*, const int*, int, double, int64_t, double*,  \
-                                             float*, const uint8_t*, int*, int8_t*, int*, PushSrc);
+                                             float*, const uint8_t*, int*, int8_t*, int*, PushSrc,   \
+                                             PushMig);
 INST_PUSH32(AMPS_MOVER_LAPENTA, 1)
 INST_PUSH32(AMPS_MOVER_BORIS, 1)
 INST_PUSH32(AMPS_MOVER_LAPENTA, 4)
@@ -316,6 +400,51 @@ __global__ __launch_bounds__(256) void k_mig_classify(const int* __restrict__ n_
   if (mover) tmpsrc[(int64_t)offset[k + 1] - 1 - (base + (__lane_id() - ru.head_lane))] = (int)p;
 }
 
+// the movers the push listed past its slab: each at the end of its key's
+// region of tmpsrc by its rank
+__global__ __launch_bounds__(256) void k_mig_place(const int* __restrict__ novf, const int2* __restrict__ ovf,
+                                                    const int* __restrict__ key, const int* __restrict__ offset,
+                                                    int* __restrict__ tmpsrc) {
+  const int n = *novf;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const int2 e = ovf[i];
+    tmpsrc[(int64_t)offset[key[e.x] + 1] - 1 - e.y] = e.x;
+  }
+}
+
+// a[i] += b[i]
+__global__ __launch_bounds__(256) void k_add_ints(int64_t n, int* __restrict__ a, const int* __restrict__ b) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    a[i] += b[i];
+}
+
+// the destination base of key k's stayers: its new offset less the stayers
+// (of any key) before its old run (a key with an empty old run has none)
+__device__ __forceinline__ void key_base(int64_t k, const int* __restrict__ offset, const int* __restrict__ lay,
+                                         const int* __restrict__ wscan, const uint64_t* __restrict__ wmask,
+                                         int* __restrict__ kbase) {
+  const int q = lay[k];
+  if (lay[k + 1] > q)
+    kbase[k] = offset[k] - (wscan[q >> 6] + __popcll(wmask[q >> 6] & ((1ull << (q & 63)) - 1ull)));
+}
+
+// one lane's insertion sort of a key's m movers at q = tmpsrc + d0, and with
+// write_dest each mover's destination in dest[]
+__device__ __forceinline__ void sort_movers(int* __restrict__ q, int m, int d0, int write_dest,
+                                            int* __restrict__ dest) {
+  for (int i = 1; i < m; i++) {
+    const int t = q[i];
+    int j = i - 1;
+    while (j >= 0 && q[j] > t) {
+      q[j + 1] = q[j];
+      j--;
+    }
+    q[j + 1] = t;
+  }
+  if (write_dest)
+    for (int i = 0; i < m; i++) dest[q[i]] = d0 + i;
+}
+
 // per key: the movers' source positions (the last cursor[k] entries of the
 // key's region of tmpsrc, its movers' destinations) in ascending order, and
 // with write_dest each mover's destination in dest[].  Up to `small` (at most
@@ -329,13 +458,7 @@ __global__ __launch_bounds__(256) void k_mig_order(int64_t nkeys, int small, int
                                                     const uint64_t* __restrict__ wmask, int* __restrict__ kbase) {
   const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (k >= nkeys) return;
-  // the destination base of key k's stayers: its new offset less the stayers
-  // (of any key) before its old run (a key with an empty old run has none)
-  if (has_layout) {
-    const int q = lay[k];
-    if (lay[k + 1] > q)
-      kbase[k] = offset[k] - (wscan[q >> 6] + __popcll(wmask[q >> 6] & ((1ull << (q & 63)) - 1ull)));
-  }
+  if (has_layout) key_base(k, offset, lay, wscan, wmask, kbase);
   const int m = cursor[k];
   if (m == 0) return;
   if (m > small) {
@@ -343,18 +466,34 @@ __global__ __launch_bounds__(256) void k_mig_order(int64_t nkeys, int small, int
     return;
   }
   const int d0 = offset[k + 1] - m;
-  int* q = tmpsrc + d0;
-  for (int i = 1; i < m; i++) {
-    const int t = q[i];
-    int j = i - 1;
-    while (j >= 0 && q[j] > t) {
-      q[j + 1] = q[j];
-      j--;
-    }
-    q[j + 1] = t;
+  sort_movers(tmpsrc + d0, m, d0, write_dest, dest);
+}
+
+// The same for movers the push listed (PushMig): the key's lane places its
+// slab row beside the entries k_mig_place put past it (rank r at
+// d0 + m - 1 - r) and sorts them, or past `small` lists the key for
+// k_mig_order_big.
+__global__ __launch_bounds__(256) void k_mig_order_slab(int64_t nkeys, int small, int write_dest,
+                                                         const int* __restrict__ slab,
+                                                         const int* __restrict__ offset,
+                                                         const int* __restrict__ cursor, int* __restrict__ tmpsrc,
+                                                         int* __restrict__ dest, int* __restrict__ big,
+                                                         const int* __restrict__ lay, const int* __restrict__ wscan,
+                                                         const uint64_t* __restrict__ wmask,
+                                                         int* __restrict__ kbase) {
+  const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (k >= nkeys) return;
+  key_base(k, offset, lay, wscan, wmask, kbase);
+  const int m = cursor[k];
+  if (m == 0) return;
+  const int d0 = offset[k + 1] - m;
+  const int ms = min(m, MIG_SLAB);
+  for (int j = 0; j < ms; j++) tmpsrc[d0 + m - 1 - j] = slab[k * MIG_SLAB + j];
+  if (m > small) {
+    big[atomicAdd(&big[nkeys], 1)] = (int)k;
+    return;
   }
-  if (write_dest)
-    for (int i = 0; i < m; i++) dest[q[i]] = d0 + i;
+  sort_movers(tmpsrc + d0, m, d0, write_dest, dest);
 }
 
 // number of entries of the ascending run r[0, n) below t
@@ -433,16 +572,38 @@ __device__ __forceinline__ int64_t mig_dest(int64_t p, int k, const int* __restr
 // only the stayers are written here.
 __global__ __launch_bounds__(256) void k_mig_dest(const int* __restrict__ n_dev, const int* __restrict__ key,
                                                   const int* __restrict__ kbase, const int* __restrict__ wscan,
-                                                  const uint64_t* __restrict__ wmask, int* __restrict__ src) {
-  const int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+                                                  const uint64_t* __restrict__ wmask, int64_t cap,
+                                                  int* __restrict__ src) {
+  // four consecutive positions per thread (one mask word per 16 threads), so
+  // that each wavefront has four times the loads in flight
+  const int64_t p0 = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) * 4;
   const int n = *n_dev;
-  if (p >= n) return;
-  const int lane = (int)(p & 63);
-  const uint64_t mw = wmask[p >> 6];
-  // a stayer is alive and kept its key
-  if (!((mw >> lane) & 1ull)) return;
-  const int k = key[p];
-  src[(int64_t)kbase[k] + wscan[p >> 6] + __popcll(mw & ((1ull << lane) - 1ull))] = (int)p;
+  if (p0 >= n) return;
+  const uint64_t mw = wmask[p0 >> 6];
+  const int sh = (int)(p0 & 63);
+  const unsigned bits = (unsigned)(mw >> sh) & 15u;
+  if (!bits) return;
+  int kk[4];
+  if (p0 + 4 <= cap) {
+    const int4 k4 = *(const int4*)(key + p0);
+    kk[0] = k4.x;
+    kk[1] = k4.y;
+    kk[2] = k4.z;
+    kk[3] = k4.w;
+  } else {
+#pragma unroll
+    for (int j = 0; j < 4; j++) kk[j] = (p0 + j < cap) ? key[p0 + j] : 0;
+  }
+  // a stayer is alive and kept its key (its bit: no position at or past n has one)
+  const int before = wscan[p0 >> 6] + __popcll(mw & ((1ull << sh) - 1ull));
+  int r = 0;
+#pragma unroll
+  for (int j = 0; j < 4; j++) {
+    if ((bits >> j) & 1u) {
+      src[(int64_t)kbase[kk[j]] + before + r] = (int)(p0 + j);
+      r++;
+    }
+  }
 }
 
 // The record move that k_mig_dest left: particle d of the new order from
