@@ -81,10 +81,12 @@ struct PushSrc {
 // holds it (a stayer) sets its bit of the 64-aligned stayer mask word wmask
 // and counts in wcnt and in the push's count[]; a survivor that changes keys
 // (a mover) counts in cursor[] and takes its rank among its new key's movers
-// from that atomic, and is listed at slab[key * MIG_SLAB + rank], or, past
-// the slab, as (position, rank) in ovf[] (novf entries).
+// from that atomic, and is listed at slab[rank * nkeys + key] (rank-major:
+// the first movers of neighbouring cells share cache lines), or, past the
+// slab, as (position, rank) in ovf[] (novf entries).
 constexpr int MIG_SLAB = 16;
 struct PushMig {
+  int64_t nkeys;
   const int* lay;
   int* cursor;
   int* slab;

@@ -927,10 +927,10 @@ static int check_mover(const amps_ctx* c, const amps_push_opts* o, const char* w
 // count, see PushMig), into zeroed d_cursor, d_wcnt, d_wmask and overflow count
 static int launch_push(amps_ctx* c, const amps_push_opts* o, int* count = nullptr, bool classify = false) {
   const int k = c->cur;
-  PushMig pm = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+  PushMig pm = {0, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
   if (classify) {
     const int64_t nkeys = (int64_t)c->mv.nb * NCELL;
-    pm = {c->d_off_layout, c->d_cursor, c->d_mig_slab, c->d_mig_ovf, c->d_mig_slab + nkeys * MIG_SLAB,
+    pm = {nkeys, c->d_off_layout, c->d_cursor, c->d_mig_slab, c->d_mig_ovf, c->d_mig_slab + nkeys * MIG_SLAB,
           c->d_wmask, c->d_wcnt};
   }
   const bool w4 = c->push_waves == 4;

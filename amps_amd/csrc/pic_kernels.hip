@@ -65,7 +65,7 @@ __device__ __forceinline__ void mig_list(const PushMig& pm, const MigPend& pd) {
   const int lane = __lane_id();
   const int rank = __shfl(pd.raw, pd.head_lane) + pd.off;
   const bool ov = pd.mover && rank >= MIG_SLAB;
-  if (pd.mover && !ov) pm.slab[(int64_t)pd.knew * MIG_SLAB + rank] = pd.p;
+  if (pd.mover && !ov) pm.slab[rank * pm.nkeys + pd.knew] = pd.p;
   const uint64_t om = __ballot(ov);
   if (om) {
     const int lead = __ffsll((long long)om) - 1;
@@ -470,7 +470,7 @@ __global__ __launch_bounds__(256) void k_mig_order(int64_t nkeys, int small, int
 }
 
 // The same for movers the push listed (PushMig): the key's lane places its
-// slab row beside the entries k_mig_place put past it (rank r at
+// slab entries beside the entries k_mig_place put past it (rank r at
 // d0 + m - 1 - r) and sorts them, or past `small` lists the key for
 // k_mig_order_big.
 __global__ __launch_bounds__(256) void k_mig_order_slab(int64_t nkeys, int small, int write_dest,
@@ -488,7 +488,7 @@ __global__ __launch_bounds__(256) void k_mig_order_slab(int64_t nkeys, int small
   if (m == 0) return;
   const int d0 = offset[k + 1] - m;
   const int ms = min(m, MIG_SLAB);
-  for (int j = 0; j < ms; j++) tmpsrc[d0 + m - 1 - j] = slab[k * MIG_SLAB + j];
+  for (int j = 0; j < ms; j++) tmpsrc[d0 + m - 1 - j] = slab[j * nkeys + k];
   if (m > small) {
     big[atomicAdd(&big[nkeys], 1)] = (int)k;
     return;

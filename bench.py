@@ -40,7 +40,7 @@ PMC_KERNELS = (("k_push_f64", "push"), ("k_push_f32", "push"), ("k_deposit_cell"
                ("k_scatter", "scatter"), ("k_mig_scatter", "scatter"), ("k_mig_dest", "mig_dest"),
                ("k_mig_move", "mig_move"),
                ("k_mig_classify", "mig_classify"),
-               ("k_mig_order", "mig_order"), ("k_copy_probe", "copy"),
+               ("k_mig_order", "mig_order"), ("k_mig_order_slab", "mig_order"), ("k_copy_probe", "copy"),
                ("k_ecsim_mass", "ecsim_mass"), ("k_ecsim_op", "ecsim_op"),
                ("k_deposit_cell_lanes", "deposit_cell"))
 # kernels of the particle step: counted only after the copy probe, which the
