@@ -34,7 +34,7 @@ __global__ void k_mig_order(int64_t, int, int, const int*, const int*, int*, int
                             const int*, const uint64_t*, int*);
 __global__ void k_mig_order_slab(int64_t, int, int, const int*, const int*, const int*, int*, int*, int*,
                                  const int*, const int*, const uint64_t*, int*);
-__global__ void k_mig_place(const int*, const int2*, const int*, const int*, int*);
+__global__ void k_mig_place(const int*, const int2*, const int*, const int*, const int*, int*);
 __global__ void k_add_ints(int64_t, int*, const int*);
 __global__ void k_mig_order_big(int64_t, int, const int*, const int*, int*, int*, int*, const int*);
 __global__ void k_mig_scatter(const int*, const int*, const int8_t*, const int*, const int*, const uint64_t*,
@@ -849,7 +849,7 @@ static int compact_impl(amps_ctx* c, bool counted = false, bool defer = false, b
     if (int r = scan_exclusive(c, c->d_wcnt, nw + 1, nullptr)) return r;
     if (classified)
       hipLaunchKernelGGL(k_mig_place, dim3(1024), dim3(256), 0, c->stream, c->d_mig_slab + nkeys * MIG_SLAB,
-                         c->d_mig_ovf, c->d_key[a], c->d_offset, c->d_tmpsrc);
+                         c->d_mig_ovf, c->d_key[a], c->d_offset, c->d_cursor, c->d_tmpsrc);
     hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, c->stream, c->d_big + nkeys, 1, 0);
     // the movers' destinations in dest[] only for the scatter; in a deferred
     // move their source indices are the sorted lists, in place in tmpsrc

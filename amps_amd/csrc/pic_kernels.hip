@@ -366,8 +366,8 @@ __global__ __launch_bounds__(256) void k_scatter(const int* __restrict__ n_dev,
 // then the particles that moved in (movers), in array order, whatever order
 // the atomics land in:
 //   classify: one stayer mask and count per wavefront; each mover lists its
-//             position at the end of its new key's region of tmpsrc (atomic
-//             order), where the key's movers go;
+//             position at the end of its new key's region of tmpsrc, where
+//             the key's movers go (backwards from the end, in atomic order);
 //   order:    per key, the movers' positions sorted in place (one lane per
 //             key, a workgroup merge sort for keys with many movers), which
 //             makes tmpsrc over the movers' destinations the source index of
@@ -400,15 +400,16 @@ __global__ __launch_bounds__(256) void k_mig_classify(const int* __restrict__ n_
   if (mover) tmpsrc[(int64_t)offset[k + 1] - 1 - (base + (__lane_id() - ru.head_lane))] = (int)p;
 }
 
-// the movers the push listed past its slab: each at the end of its key's
-// region of tmpsrc by its rank
+// the movers the push listed past its slab: rank r of key k's m movers at
+// offset[k + 1] - m + r of tmpsrc (the key's movers in rank order)
 __global__ __launch_bounds__(256) void k_mig_place(const int* __restrict__ novf, const int2* __restrict__ ovf,
                                                     const int* __restrict__ key, const int* __restrict__ offset,
-                                                    int* __restrict__ tmpsrc) {
+                                                    const int* __restrict__ cursor, int* __restrict__ tmpsrc) {
   const int n = *novf;
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
     const int2 e = ovf[i];
-    tmpsrc[(int64_t)offset[key[e.x] + 1] - 1 - e.y] = e.x;
+    const int k = key[e.x];
+    tmpsrc[(int64_t)offset[k + 1] - cursor[k] + e.y] = e.x;
   }
 }
 
@@ -428,7 +429,8 @@ __device__ __forceinline__ void key_base(int64_t k, const int* __restrict__ offs
     kbase[k] = offset[k] - (wscan[q >> 6] + __popcll(wmask[q >> 6] & ((1ull << (q & 63)) - 1ull)));
 }
 
-// one lane's insertion sort of a key's m movers at q = tmpsrc + d0, and with
+// one lane's insertion sort of a key's m movers at q = tmpsrc + d0 (nearly
+// ascending, as the atomics ran, when it starts: linear time), and with
 // write_dest each mover's destination in dest[]
 __device__ __forceinline__ void sort_movers(int* __restrict__ q, int m, int d0, int write_dest,
                                             int* __restrict__ dest) {
@@ -466,13 +468,20 @@ __global__ __launch_bounds__(256) void k_mig_order(int64_t nkeys, int small, int
     return;
   }
   const int d0 = offset[k + 1] - m;
-  sort_movers(tmpsrc + d0, m, d0, write_dest, dest);
+  // the classification listed them backwards: reversed, they are nearly
+  // ascending
+  int* q = tmpsrc + d0;
+  for (int i = 0, j = m - 1; i < j; i++, j--) {
+    const int t = q[i];
+    q[i] = q[j];
+    q[j] = t;
+  }
+  sort_movers(q, m, d0, write_dest, dest);
 }
 
 // The same for movers the push listed (PushMig): the key's lane places its
-// slab entries beside the entries k_mig_place put past it (rank r at
-// d0 + m - 1 - r) and sorts them, or past `small` lists the key for
-// k_mig_order_big.
+// slab entries beside the entries k_mig_place put past them (rank r at
+// d0 + r) and sorts them, or past `small` lists the key for k_mig_order_big.
 __global__ __launch_bounds__(256) void k_mig_order_slab(int64_t nkeys, int small, int write_dest,
                                                          const int* __restrict__ slab,
                                                          const int* __restrict__ offset,
@@ -488,7 +497,7 @@ __global__ __launch_bounds__(256) void k_mig_order_slab(int64_t nkeys, int small
   if (m == 0) return;
   const int d0 = offset[k + 1] - m;
   const int ms = min(m, MIG_SLAB);
-  for (int j = 0; j < ms; j++) tmpsrc[d0 + m - 1 - j] = slab[j * nkeys + k];
+  for (int j = 0; j < ms; j++) tmpsrc[d0 + j] = slab[j * nkeys + k];
   if (m > small) {
     big[atomicAdd(&big[nkeys], 1)] = (int)k;
     return;
