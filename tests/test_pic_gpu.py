@@ -250,6 +250,47 @@ def test_migration_order_is_deterministic(gpu_ctx, small):
         gpu_ctx.set_param("migration_sort_small", 64)
 
 
+@pytest.mark.parametrize("small", [64, 1])
+def test_step_migration_order_with_crowded_arrivals(gpu_ctx, small):
+    """The same layout from the step's own migration, whose push classifies
+    the survivors and lists each cell's arrivals (16 per cell, the rest in
+    an overflow list): about 200 particles per cell and fast ones, so that
+    many cells take more than 16 arrivals and some more than 64 (the
+    workgroup merge sort; small=1 sends every cell with two or more there)."""
+    mesh = M.uniform_mesh(1)
+    E, B = M.corner_fields(mesh)
+    dt = 0.05
+    sp = Species([-1.0, 1.0], [0.01, 1.0], [dt, dt])
+    st = PicState(mesh, E, B, sp, ctx=gpu_ctx)
+    st.generate_particles(mesh.ncells * 100, [0.8, 0.8], seed=11)
+    gpu_ctx.set_param("migration_sort_small", small)
+    try:
+        st.run_steps(1, dt, r_sphere=0.2, use_graph=False)
+        over16 = over64 = 0
+        for step in range(3):
+            a, _ = st.get_particles()
+            st.run_steps(1, dt, r_sphere=0.2, use_graph=False)
+            b, _ = st.get_particles()
+            pos = np.full(int(a.id.max()) + 1, -1)
+            pos[a.id] = np.arange(a.n)
+            old = pos[b.id]
+            assert (old >= 0).all()
+            ka = a.blk.astype(np.int64) * 512 + a.cell
+            kb = b.blk.astype(np.int64) * 512 + b.cell
+            mover = kb != ka[old]
+            want = np.lexsort((old, mover, kb))
+            assert np.array_equal(want, np.arange(b.n)), step
+            per_key = np.bincount(kb[mover], minlength=mesh.ncells)
+            over16 += int((per_key > 16).sum())
+            over64 += int((per_key > 64).sum())
+            print(f"step {step}: {b.n} particles, {int(mover.sum())} arrivals, at most {per_key.max()} "
+                  f"in a cell", flush=True)
+        print(f"cells with > 16 arrivals: {over16}, > 64: {over64}", flush=True)
+        assert over16 > 100 and over64 > 0
+    finally:
+        gpu_ctx.set_param("migration_sort_small", 64)
+
+
 def test_config0_workload_steps(gpu_ctx):
     """The plain FP64 workload (2^18 of the 2^20 protons, dt from max
     omega_c dt = 0.1) for 20 steps with reflection and absorption."""
