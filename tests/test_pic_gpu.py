@@ -250,13 +250,18 @@ def test_migration_order_is_deterministic(gpu_ctx, small):
         gpu_ctx.set_param("migration_sort_small", 64)
 
 
-@pytest.mark.parametrize("small", [64, 1])
-def test_step_migration_order_with_crowded_arrivals(gpu_ctx, small):
+@pytest.mark.parametrize("small,kernel", [(64, 2), (1, 2), (64, 1)])
+def test_step_migration_order_with_crowded_arrivals(gpu_ctx, small, kernel):
     """The same layout from the step's own migration, whose push classifies
     the survivors and lists each cell's arrivals (16 per cell, the rest in
     an overflow list): about 200 particles per cell and fast ones, so that
     many cells take more than 16 arrivals and some more than 64 (the
-    workgroup merge sort; small=1 sends every cell with two or more there)."""
+    workgroup merge sort; small=1 sends every cell with two or more there).
+    With the matrix-core cell pass the step leaves the records for the next
+    push to move; with the record-batch pass (kernel 1) its migration moves
+    them through each arrival's destination."""
+    from amps_amd import autotune
+
     mesh = M.uniform_mesh(1)
     E, B = M.corner_fields(mesh)
     dt = 0.05
@@ -264,6 +269,7 @@ def test_step_migration_order_with_crowded_arrivals(gpu_ctx, small):
     st = PicState(mesh, E, B, sp, ctx=gpu_ctx)
     st.generate_particles(mesh.ncells * 100, [0.8, 0.8], seed=11)
     gpu_ctx.set_param("migration_sort_small", small)
+    autotune.set_deposit_variant(gpu_ctx, kernel, 4)
     try:
         st.run_steps(1, dt, r_sphere=0.2, use_graph=False)
         over16 = over64 = 0
@@ -287,8 +293,9 @@ def test_step_migration_order_with_crowded_arrivals(gpu_ctx, small):
                   f"in a cell", flush=True)
         print(f"cells with > 16 arrivals: {over16}, > 64: {over64}", flush=True)
         assert over16 > 100 and over64 > 0
-    finally:
+    finally:  # the session context's defaults
         gpu_ctx.set_param("migration_sort_small", 64)
+        autotune.set_deposit_variant(gpu_ctx, 2, 12)
 
 
 def test_config0_workload_steps(gpu_ctx):
