@@ -54,7 +54,6 @@ def _pmc_pass(args, tune, counter, outdir):
     """One rocprofv3 counter pass over two eager steps of the same workload
     (a child process; the counter in KiB per dispatch: the second step's
     dispatch for the step's kernels, the average for the others)."""
-    import csv
     import glob
     import subprocess
 
@@ -72,6 +71,16 @@ def _pmc_pass(args, tune, counter, outdir):
     if not files:
         raise RuntimeError(f"rocprofv3 {counter} pass: exit {r.returncode}, no counter file; "
                            + r.stdout.decode(errors="replace")[-400:])
+    return _pmc_values(files, counter)
+
+
+def _pmc_values(files, counter):
+    """The counter per report key (PMC_KERNELS) from rocprofv3 counter CSV
+    files, in KiB per dispatch: for the step's kernels (STEP_KEYS) their last
+    dispatch after the last copy-probe dispatch (the counter child's second
+    step), for the others the average over their dispatches."""
+    import csv
+
     rows = []
     for f in files:
         with open(f) as fh:
@@ -165,7 +174,6 @@ def pmc_traffic(args, tune):
     import shutil
     import tempfile
 
-    out = {}
     try:
         per = {}
         for counter in ("FETCH_SIZE", "WRITE_SIZE"):
@@ -174,17 +182,25 @@ def pmc_traffic(args, tune):
                 per[counter] = _pmc_pass(args, tune, counter, d)
             finally:
                 shutil.rmtree(d, ignore_errors=True)
-        kib = COPY_PROBE_BYTES / 1024.0
-        f_fetch = kib / per["FETCH_SIZE"]["copy"]
-        f_write = kib / per["WRITE_SIZE"]["copy"]
-        out["calibration"] = {"fetch": round(f_fetch, 4), "write": round(f_write, 4)}
-        for key in per["FETCH_SIZE"]:
-            if key == "copy" or key not in per["WRITE_SIZE"]:
-                continue
-            out[key] = int(1024 * (per["FETCH_SIZE"][key] * f_fetch + per["WRITE_SIZE"][key] * f_write))
-            out[key + "_fetch"] = int(1024 * per["FETCH_SIZE"][key] * f_fetch)
+        return _traffic_bytes(per)
     except Exception as e:  # the line stays valid without the counter passes
         return {"error": repr(e)[:300]}
+
+
+def _traffic_bytes(per):
+    """Bytes per launch from the two counter passes ({"FETCH_SIZE": {key:
+    KiB}, "WRITE_SIZE": {...}}): each counter scaled by the factor that makes
+    the copy probe (COPY_PROBE_BYTES read, the same written) read exactly
+    its bytes; total and fetch bytes per key present in both passes."""
+    kib = COPY_PROBE_BYTES / 1024.0
+    f_fetch = kib / per["FETCH_SIZE"]["copy"]
+    f_write = kib / per["WRITE_SIZE"]["copy"]
+    out = {"calibration": {"fetch": round(f_fetch, 4), "write": round(f_write, 4)}}
+    for key in per["FETCH_SIZE"]:
+        if key == "copy" or key not in per["WRITE_SIZE"]:
+            continue
+        out[key] = int(1024 * (per["FETCH_SIZE"][key] * f_fetch + per["WRITE_SIZE"][key] * f_write))
+        out[key + "_fetch"] = int(1024 * per["FETCH_SIZE"][key] * f_fetch)
     return out
 
 
