@@ -2,6 +2,9 @@
 rocprofv3's per-dispatch rows to the report's kernel keys, and how the copy
 probe calibrates them into bytes per launch."""
 import csv
+import os
+import subprocess
+import sys
 
 import bench
 
@@ -69,3 +72,12 @@ def test_traffic_bytes_calibrated_by_the_copy_probe():
     assert out["push"] == 1024 * (1000 * 2 + 500)
     assert out["push_fetch"] == 1024 * 1000 * 2
     assert set(out) == {"calibration", "push", "push_fetch"}
+
+
+def test_more_than_one_gpu_exits_with_a_message():
+    """The benchmark measures one GPU: asked for more it exits with status 2
+    and says so, before touching a device."""
+    r = subprocess.run([sys.executable, os.path.abspath(bench.__file__), "--gpus", "2"],
+                       stdout=subprocess.PIPE, stderr=subprocess.PIPE, timeout=120)
+    assert r.returncode == 2 and r.stdout == b""
+    assert b"one GPU only" in r.stderr and b"--gpus 2" in r.stderr
