@@ -243,7 +243,11 @@ int amps_gpu_generate_particles(amps_ctx* ctx, int64_t n_per_species, int32_t ns
 /* nsteps full particle steps: MoveParticles, migration (compaction) and
  * UpdateJMassMatrix at deposit_dt.  use_graph=1 runs one eager step, then
  * replays a hipGraph holding two captured steps (an odd remainder runs
- * eagerly). */
+ * eagerly).  On return the particle arrays (in array order) and the
+ * resident J and M are those of the same steps made by separate
+ * amps_gpu_move_particles (with compaction) and amps_gpu_update_j_mass_matrix
+ * calls; inside the call a step may leave its particle records for the next
+ * step's push to move. */
 int amps_gpu_run_steps(amps_ctx* ctx, const amps_push_opts* opts, double deposit_dt,
                        int64_t nsteps, int32_t use_graph);
 
