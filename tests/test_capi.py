@@ -1,5 +1,6 @@
 """The C-ABI library loads and exports every entry point include/ampsgpu.h
-declares (no compute calls: this runs without a GPU)."""
+declares, and the Python binding declares each with the header's signature
+(no compute calls: this runs without a GPU)."""
 import ctypes
 import os
 import re
@@ -111,3 +112,46 @@ def test_pointer_helpers_reject_strided_arrays():
         with pytest.raises(ValueError):
             L._ptr(b, L.C.c_double)
     assert L._ptr(np.ascontiguousarray(b), L.C.c_double) is not None
+
+
+def _prototypes(header):
+    """name -> (return type, [parameter declarations]) of every entry point."""
+    txt = re.sub(r"/\*.*?\*/", "", open(os.path.join(ROOT, "include", header)).read(), flags=re.S)
+    out = {}
+    for m in re.finditer(r"^\s*(int|int64_t|const char\s*\*)\s+(amps_\w+)\s*\(([^;]*?)\)\s*;", txt, re.M | re.S):
+        params = [p.strip() for p in m.group(3).replace("\n", " ").split(",")]
+        out[m.group(2)] = (m.group(1).replace(" ", ""), [] if params in (["void"], [""]) else params)
+    return out
+
+
+def _kind_of_decl(p):
+    if "*" in p or "[" in p:
+        return "ptr"
+    t = p.rsplit(" ", 1)[0].replace("const ", "").strip()
+    return {"double": "f64", "float": "f32", "int64_t": "i64", "uint64_t": "i64", "int32_t": "i32", "int": "i32",
+            "uint32_t": "i32"}.get(t, "other " + t)
+
+
+def _kind_of_ctype(a):
+    if a in (ctypes.c_void_p, ctypes.c_char_p) or (isinstance(a, type) and issubclass(a, ctypes._Pointer)):
+        return "ptr"
+    return {ctypes.c_double: "f64", ctypes.c_float: "f32", ctypes.c_int64: "i64", ctypes.c_uint64: "i64",
+            ctypes.c_int32: "i32", ctypes.c_uint32: "i32"}.get(a, f"other {a}")
+
+
+@pytest.mark.skipif(not os.path.exists(LIB), reason="libampsgpu.so not built")
+def test_binding_signatures_match_the_header():
+    """The Python binding declares every entry point with the header's
+    parameter list (pointer, 32- or 64-bit integer, double, in order) and
+    return type: a mismatch would pass arguments in the wrong registers."""
+    from amps_amd import _lib as L
+
+    lib = L.load()
+    protos = _prototypes("ampsgpu.h")
+    assert len(protos) == len(declared("ampsgpu.h"))
+    ret = {"int": ctypes.c_int, "int64_t": ctypes.c_int64, "constchar*": ctypes.c_char_p}
+    for name, (rtype, params) in sorted(protos.items()):
+        f = getattr(lib, name)
+        assert f.argtypes is not None, f"{name} has no binding"
+        assert [_kind_of_decl(p) for p in params] == [_kind_of_ctype(a) for a in f.argtypes], name
+        assert f.restype == ret[rtype], name
