@@ -155,3 +155,41 @@ def test_binding_signatures_match_the_header():
         assert f.argtypes is not None, f"{name} has no binding"
         assert [_kind_of_decl(p) for p in params] == [_kind_of_ctype(a) for a in f.argtypes], name
         assert f.restype == ret[rtype], name
+
+
+def test_binding_structs_match_the_header(tmp_path):
+    """Every parameter and result struct of the header has the same size and
+    field offsets in the Python binding (ctypes structures and the numpy
+    record types of the trace results and exits), as the C compiler lays
+    the header's structs out."""
+    import shutil
+    import subprocess
+
+    from amps_amd import _lib as L
+
+    if shutil.which("gcc") is None:
+        pytest.skip("gcc not available")
+    pairs = [("amps_mesh_desc", L.MeshDesc), ("amps_push_opts", L.PushOpts), ("amps_aos_layout", L.AosLayout),
+             ("amps_ecsim_params", L.EcsimParams), ("amps_trace_params", L.TraceParams),
+             ("amps_trace_result", L.TraceResult), ("amps_trace_result", L.TRACE_RESULT_DTYPE),
+             ("amps_trace_exit", L.TRACE_EXIT_DTYPE)]
+    want, lines = [], []
+    for cname, py in pairs:
+        if isinstance(py, np.dtype):
+            fields = [(f, py.fields[f][1]) for f in py.names]
+            size = py.itemsize
+        else:
+            fields = [(f, getattr(py, f).offset) for f, *_ in py._fields_]
+            size = ctypes.sizeof(py)
+        want.append(f"{cname} sizeof {size}")
+        lines.append(f'  printf("{cname} sizeof %zu\\n", sizeof({cname}));')
+        for f, off in fields:
+            want.append(f"{cname} {f} {off}")
+            lines.append(f'  printf("{cname} {f} %zu\\n", offsetof({cname}, {f}));')
+    src = tmp_path / "layout.c"
+    src.write_text("#include <stddef.h>\n#include <stdio.h>\n#include \"ampsgpu.h\"\nint main(void) {\n"
+                   + "\n".join(lines) + "\n  return 0;\n}\n")
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c99", "-I", os.path.join(ROOT, "include"), "-o", str(exe), str(src)], check=True)
+    got = subprocess.run([str(exe)], check=True, stdout=subprocess.PIPE).stdout.decode().split("\n")[:-1]
+    assert got == want
