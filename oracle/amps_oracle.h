@@ -69,15 +69,24 @@ int orc_push(const orc_mesh* m, const double* E_corner, const double* B_corner,
              const int* spec, int* blk, int* cell, int* code, int nthreads);
 
 /* Unique corner nodes: fills node_of_corner[nb*729], returns node count and,
- * if keys!=NULL, the lattice key of each node (sorted ascending). */
+ * if keys!=NULL, the lattice key of each node (sorted ascending).  Corners of
+ * blocks that share a position on the finest lattice are one node, whatever
+ * the blocks' levels; a hanging node (a fine corner on a coarse face) is a
+ * node of its own.  The reference's deposit runs on one level, so this rule
+ * at coarse/fine interfaces is decided here: parity unpinned there (the
+ * count is checked against the distinct corner positions). */
 int64_t orc_build_nodes(const orc_mesh* m, int* node_of_corner, int64_t* keys);
 
-/* ECSIM J + mass matrix.  Particles may be in any order; within a cell they
- * are processed in array order.  J[nnodes*3], M[nnodes*243] are overwritten. */
 /* Particle energy and per-species CFL of UpdateJMassMatrix (see the .c) */
 int orc_energy_cfl(const orc_mesh* m, const double* mass, const double* weight, int nspec, double dt,
                    int64_t n, const double* v, const int* spec, const int* blk, const int* cell,
                    double* energy, double* cfl, double* cell_energy, double* cell_cfl);
+
+/* ECSIM J + mass matrix.  Particles may be in any order; within a cell they
+ * are processed in array order.  J[nnodes*3], M[nnodes*243] are overwritten.
+ * Pinned by the single-particle closed forms, symmetry and linearity; parity
+ * unpinned at coarse/fine block interfaces, where the node rule of
+ * orc_build_nodes applies. */
 int orc_deposit(const orc_mesh* m, const double* B_corner, const double* charge,
                 const double* mass, const double* weight, double dt, int64_t n,
                 const double* x, const double* v, const int* spec, const int* blk,
