@@ -9,7 +9,9 @@
  *
  * Every function cites the reference file:line it follows.  Units are the
  * reference's: normalized PIC units (c = 1) for the mover/deposit, SI for the
- * gridless tracer.
+ * gridless tracer.  Where the reference has vectorised (AVX) and scalar
+ * branches of one computation, the scalar branch is restated; the AVX
+ * branches are parity unpinned.
  */
 #ifndef AMPS_ORACLE_H
 #define AMPS_ORACLE_H
